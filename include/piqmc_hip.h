@@ -1,0 +1,215 @@
+/*
+ * piqmc_hip.h -- C ABI of the MI355X (gfx950) bisection-move action library.
+ *
+ * The library evaluates the multilevel-bisection Monte Carlo hot path of
+ * pi-qmc on one GPU: spring, harmonic-oscillator and tabulated Coulomb pair
+ * action differences, the fixed-node action with SHO or free-particle
+ * density-matrix determinants, the free-particle midpoint mover, the
+ * Metropolis decision per level, and the thermodynamic-energy and g(r)
+ * estimators.  Every entry point takes plain pointers and sizes; host arrays
+ * are borrowed for the duration of the call and the library owns all device
+ * memory.  Every call returns PQ_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is returned by pq_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the pi-qmc source tree):
+ *   Action::getActionDifference            src/action/Action.h:44
+ *   Action::getBeadAction                  src/action/Action.h:53-54
+ *   DoubleAction / FixedNodeAction         src/action/DoubleAction.h:32-58,
+ *                                          src/fixednode/FixedNodeAction.cc:39-131,331-367
+ *   NodeModel::evaluate / evaluateDistance src/fixednode/NodeModel.h:93-100
+ *   Mover::makeMove                        src/advancer/mover/Mover.h:7-17
+ *   MultiLevelSampler::run / tryMove       src/advancer/MultiLevelSampler.cc:68-177
+ *   DoubleMLSampler::run / tryMove         src/advancer/DoubleMLSampler.cc:66-245
+ *   SectionChooser::run                    src/advancer/SectionChooser.cc:30-48
+ *   DoubleSectionChooser::run              src/advancer/DoubleSectionChooser.cc:33-65
+ *   ThermoEnergyEstimator::handleLink      src/estimator/ThermoEnergyEstimator.cc:32-54
+ *   PairCFEstimator<1>::handleLink         src/estimator/PairCFEstimator.h:72-93
+ */
+#ifndef PIQMC_HIP_H
+#define PIQMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PQ_OK 0
+#define PQ_ERR_ARG (-1)         /* invalid argument or unsupported configuration */
+#define PQ_ERR_HIP (-2)         /* HIP runtime failure */
+#define PQ_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define PQ_ERR_STATE (-4)       /* call made in the wrong order */
+
+#define PQ_MAX_SPECIES 4
+#define PQ_MAX_SHO 4
+#define PQ_MAX_NODES 4
+#define PQ_MAX_SAMPLES 8
+#define PQ_MAX_MOVING 4
+#define PQ_MAX_NORDER 4
+#define PQ_MAX_NODE_PART 32     /* largest fermion species handled by a node model */
+
+#define PQ_MOVER_FREE 0         /* FreeMover      (src/advancer/mover/FreeMover.cc:59-124) */
+#define PQ_MOVER_FREE_PBC 1     /* FreeMoverPBC   (src/advancer/mover/FreeMoverPBC.cc:59-162) */
+
+#define PQ_NODES_FREE 0         /* FreeParticleNodes (src/fixednode/FreeParticleNodes.cc:72-234) */
+#define PQ_NODES_SHO 1          /* SHONodes          (src/fixednode/SHONodes.cc:51-170) */
+
+#define PQ_PREC_FP64 0          /* every operation in FP64 */
+#define PQ_PREC_MIXED 1         /* FP32 spring/SHO/pair/mover/propagator arithmetic with
+                                   compensated sums; FP64 node LU, sign test, exponent */
+
+#define PQ_RNG_PHILOX 0         /* Philox4x32-10 counter stream per chain */
+#define PQ_RNG_INJECTED 1       /* uniforms supplied by the caller, consumed in order */
+
+/* One particle species (<Species> element; src/base/Species.h). */
+typedef struct pq_species_desc {
+  int count;
+  double mass;
+  double charge;
+} pq_species_desc;
+
+/* <SHOAction omega=".." species=".." x y z ndim> (src/parser/ActionParser.cc:307-324). */
+typedef struct pq_sho_desc {
+  int species;
+  double omega;
+  double center[3];
+  int ndim;            /* 0 means the full dimension */
+} pq_sho_desc;
+
+/* <CoulombAction epsilon norder rmin rmax ngridPoints> without Ewald or exchange
+   (src/parser/ActionParser.cc:118-153, src/action/CoulombAction.cc:19-91). */
+typedef struct pq_coulomb_desc {
+  int enabled;
+  double epsilon;      /* 0 means 1 */
+  int norder;
+  double rmin;         /* 0 means 0.005/(2 mu |q1 q2|) of the first interacting pair */
+  double rmax;         /* 0 means the length of the cell diagonal */
+  int ngpts;           /* 0 means 500 */
+} pq_coulomb_desc;
+
+/* <FixedNodeAction species model omega temperature noNodalAction>
+   (src/parser/ActionParser.cc:433-452,759-815). */
+typedef struct pq_node_desc {
+  int species;
+  int model;           /* PQ_NODES_FREE or PQ_NODES_SHO */
+  double omega;        /* SHONodes only */
+  double temperature;  /* 0 means the simulation temperature */
+  int nodal_action;    /* 1 unless noNodalAction="true" */
+} pq_node_desc;
+
+/* <Sample npart species noPermutation="true"> inside <ChooseSection>. */
+typedef struct pq_sample_desc {
+  int species;
+  int nmoving;
+} pq_sample_desc;
+
+/* One Markov chain (one clone of the reference).  Particle i of species s has
+   global index species_first(s)+i, species in the order given. */
+typedef struct pq_chain_desc {
+  int ndim;            /* 2 or 3 */
+  int nslice;          /* M */
+  int nlevel;          /* L of <ChooseSection nlevel>; a section holds 2^L+1 slices */
+  int nspecies;
+  double tau;
+  double cell[3];      /* SuperCell a */
+  pq_species_desc species[PQ_MAX_SPECIES];
+  int spring;          /* <SpringAction> present */
+  int nsho;
+  pq_sho_desc sho[PQ_MAX_SHO];
+  pq_coulomb_desc coulomb;
+  int nnodes;
+  pq_node_desc nodes[PQ_MAX_NODES];
+  int mover;           /* PQ_MOVER_* */
+  int nsample;         /* <Sample> elements per <ChooseSection>, run in order */
+  pq_sample_desc sample[PQ_MAX_SAMPLES];
+  int nsection;        /* <ChooseSection> repetitions per sweep */
+} pq_chain_desc;
+
+/* Per-chain counters kept on the device by the sampler. */
+typedef struct pq_chain_stats {
+  uint64_t draws;                /* uniforms consumed */
+  uint64_t bead_updates;         /* beads proposed and evaluated */
+  uint64_t accepted_bead_updates;
+  uint64_t tries[8];             /* per level: moves that reached this level */
+  uint64_t accepts[8];           /* per level: moves accepted at this level */
+  uint64_t node_errors;          /* crossings or singular matrices met in initialize */
+} pq_chain_stats;
+
+typedef struct pq_batch pq_batch;
+
+const char* pq_last_error(void);
+
+/* Builds pair tables, uploads parameters and allocates per-chain device state
+   (beads slice-major [M][N][D] in FP64).  precision is PQ_PREC_*. */
+int pq_batch_create(const pq_chain_desc* chains, int nchain, int precision,
+                    int device, pq_batch** out);
+void pq_batch_destroy(pq_batch* batch);
+int pq_batch_info(const pq_batch* batch, int chain, int* npart, int* nslice,
+                  int* ndim, int64_t* bead_offset);
+
+/* Path coordinates of one chain, slice-major [M][N][D] FP64
+   (Beads<NDIM> layout, src/base/Beads.h:215,225; identity permutation). */
+int pq_set_paths(pq_batch* batch, int chain, const double* beads);
+int pq_get_paths(pq_batch* batch, int chain, double* beads);
+/* All chains at once, concatenated in chain order. */
+int pq_set_all_paths(pq_batch* batch, const double* beads, int64_t count);
+int pq_get_all_paths(pq_batch* batch, double* beads, int64_t count);
+
+/* Uniform stream of every chain: Philox4x32-10 keyed by seed, stream id =
+   first_stream + chain, or caller-supplied uniforms (per_chain values per
+   chain, chain-major).  Replaces RandomNumGenerator (src/util/RandomNumGenerator.h:26-86). */
+int pq_seed(pq_batch* batch, uint64_t seed, uint32_t first_stream);
+int pq_set_uniforms(pq_batch* batch, const double* u, int64_t per_chain);
+
+/* Device-resident sampler: nsweep sweeps of every chain, each sweep nsection
+   x (choose section, initialize node caches, run each Sample, put beads back).
+   One fused kernel launch per call; asynchronous on the batch stream. */
+int pq_run_sweeps(pq_batch* batch, int nsweep);
+int pq_sync(pq_batch* batch);
+int pq_get_stats(pq_batch* batch, int chain, pq_chain_stats* out);
+int pq_reset_stats(pq_batch* batch);
+/* Node-model magnitude scale of each FixedNodeAction of a chain
+   (SHONodes.cc:113-122 / FreeParticleNodes.cc:142-151 state). */
+int pq_get_node_scale(pq_batch* batch, int chain, double* scale);
+
+/* Thermodynamic energy estimate of every chain's current paths:
+   E = (1/M) sum_{t,i} utau(i,t) (ThermoEnergyEstimator + CompositeAction::getBeadAction). */
+int pq_measure_energy(pq_batch* batch, double* energy);
+/* Radial pair histogram of every chain (PairCFEstimator<1> with PairRadial):
+   integer counts over all slices, species1 particle i against every particle
+   j != i of species2; counts[chain][nbin], added to what the array holds. */
+int pq_pair_histogram(pq_batch* batch, int species1, int species2, double rmin,
+                      double rmax, int nbin, uint64_t* counts);
+
+/* Reference call-level entry points.  A query is one section (S=2^L+1 slices,
+   [S][N][D]) with the moving beads ([S][nmov][D]) and their particle indices.
+   dS[q] = CompositeAction::getActionDifference (spring + SHO + Coulomb pair)
+   of query q at level[q] for chain chain[q]. */
+int pq_action_difference(pq_batch* batch, int nquery, const int* chain,
+                         const int* level, const double* section,
+                         const double* moving, const int* index, int nmov,
+                         double* dS);
+/* Per-link utau of CompositeAction::getBeadAction for a set of (chain, ipart,
+   islice) on the current device paths. */
+int pq_bead_utau(pq_batch* batch, int nquery, const int* chain,
+                 const int* ipart, const int* islice, double* utau);
+/* FixedNodeAction protocol on one chain: section_begin = initialize with both
+   sections ([S][N][D] each, sec2 at +M/2); double_action_difference =
+   CompositeDoubleAction::getActionDifference at one level (2e100 on a crossing);
+   accept_last_move = acceptLastMove. */
+int pq_section_begin(pq_batch* batch, int chain, const double* sec1,
+                     const double* sec2);
+int pq_double_action_difference(pq_batch* batch, int chain, int level,
+                                const double* moving, const int* index,
+                                int nmov, double* dS);
+int pq_accept_last_move(pq_batch* batch, int chain);
+/* NodeModel::evaluate + evaluateDistance on nmat slice pairs (r1,r2 [nmat][N][D])
+   for FixedNodeAction inode of chain, without changing the chain's scale. */
+int pq_node_evaluate(pq_batch* batch, int chain, int inode, int nmat,
+                     const double* r1, const double* r2, double* det,
+                     int* err, double* d1, double* d2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

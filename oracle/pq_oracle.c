@@ -1,0 +1,1396 @@
+/*
+ * pq_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * A plain-C FP64 restatement of pi-qmc's multilevel-bisection hot path.  It
+ * follows the reference loop order, uniform-draw order and level-dependent
+ * approximations so that, given the same uniform stream, it makes the same
+ * accept/reject decisions.  Each function names the reference lines it
+ * restates (paths relative to the pi-qmc source tree).
+ *
+ * Deliberate differences from the reference, none of which changes a result:
+ *  - SHONodes/FreeParticleNodes run the Hungarian assignment only to fill
+ *    kindex, whose sole reader (fgrad) is discarded when useHungarian is off
+ *    (SHONodes.cc:77-81,145-147; FreeParticleNodes.cc:105-112,203).  It is
+ *    skipped here.
+ *  - Node distances are stored for the node model's own species only; the
+ *    reference stores 200 for the other particles (SHONodes.cc:132), whose
+ *    nodal-action term is log((1-e^-40000)/(1-e^-40000)) = 0 exactly.
+ *  - LAPACK dgetrf/dgetri are replaced by a textbook unblocked LU with
+ *    partial pivoting (first maximal |pivot|, as idamax) and an inverse from
+ *    the LU factors; the determinant is the product of U's diagonal with the
+ *    pivot sign, as SHONodes.cc:97-101.
+ *  - The rescale loop of evaluate(...,scaleMagnitude=true) gives up after 200
+ *    passes and reports an error instead of spinning (SHONodes.cc:55-126).
+ *  - The CompositeAction sums spring, then SHO actions, then Coulomb.
+ */
+#include "pq_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXN 256
+
+/* ------------------------------------------------------------------ RNG */
+
+typedef struct {
+  int mode;               /* 0 glibc, 1 philox, 2 injected */
+  int32_t st[31];
+  int f, r;
+  uint64_t seed;
+  uint32_t stream;
+  const double* inj;
+  int64_t ninj;
+  uint64_t count;
+} oq_rng;
+
+/* glibc srandom_r/random_r TYPE_3 (degree 31, separation 3), the generator
+   behind rand() in RandomNumGenerator::getRand (src/util/RandomNumGenerator.h:42). */
+static void glibc_seed(oq_rng* g, unsigned int seed) {
+  if (seed == 0) seed = 1;
+  g->st[0] = (int32_t)seed;
+  int32_t word = (int32_t)seed;
+  for (int i = 1; i < 31; ++i) {
+    long hi = word / 127773;
+    long lo = word % 127773;
+    word = (int32_t)(16807 * lo - 2836 * hi);
+    if (word < 0) word += 2147483647;
+    g->st[i] = word;
+  }
+  g->f = 3;
+  g->r = 0;
+  for (int k = 0; k < 310; ++k) {
+    uint32_t v = (uint32_t)g->st[g->f] + (uint32_t)g->st[g->r];
+    g->st[g->f] = (int32_t)v;
+    if (++g->f >= 31) { g->f = 0; ++g->r; }
+    else if (++g->r >= 31) g->r = 0;
+  }
+}
+
+static int32_t glibc_next(oq_rng* g) {
+  uint32_t v = (uint32_t)g->st[g->f] + (uint32_t)g->st[g->r];
+  g->st[g->f] = (int32_t)v;
+  int32_t res = (int32_t)(v >> 1);
+  if (++g->f >= 31) { g->f = 0; ++g->r; }
+  else if (++g->r >= 31) g->r = 0;
+  return res;
+}
+
+void oq_glibc_sequence(unsigned int seed, int n, int32_t* out) {
+  oq_rng g;
+  glibc_seed(&g, seed);
+  for (int i = 0; i < n; ++i) out[i] = glibc_next(&g);
+}
+
+/* Philox4x32-10 (Salmon et al., SC'11).  Draw k of stream s under seed:
+   counter (k>>2, k>>34, s, 0), key (seed lo, seed hi), word k&3, u = w*2^-32. */
+static void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int rnd = 0; rnd < 10; ++rnd) {
+    if (rnd > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+
+uint32_t oq_philox_u32(uint64_t seed, uint32_t stream, uint64_t k) {
+  uint32_t c[4] = {(uint32_t)(k >> 2), (uint32_t)(k >> 34), stream, 0u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return c[k & 3];
+}
+
+double oq_philox_uniform(uint64_t seed, uint32_t stream, uint64_t k) {
+  return (double)oq_philox_u32(seed, stream, k) * 2.3283064365386963e-10;
+}
+
+static double rng_next(oq_rng* g) {
+  double u;
+  if (g->mode == 0) {
+    /* rand()*SCALE_RAND, SCALE_RAND = 1./RAND_MAX (RandomNumGenerator.cc:3) */
+    const double scale = 1. / 2147483647;
+    u = glibc_next(g) * scale;
+  } else if (g->mode == 1) {
+    u = oq_philox_uniform(g->seed, g->stream, g->count);
+  } else {
+    u = (g->count < (uint64_t)g->ninj) ? g->inj[g->count] : 0.5;
+  }
+  g->count++;
+  return u;
+}
+
+/* RandomNumGenerator::makeGaussRand (src/util/RandomNumGenerator.h:75-86) */
+static void gauss_rand(oq_rng* g, double* a, int n) {
+  for (int i = 0; i + 1 < n; i += 2) {
+    double t1 = 1 - 0.9999999999 * rng_next(g);
+    double t2 = rng_next(g);
+    a[i] = sqrt(-2.0 * log(t1)) * cos(6.283185306 * t2);
+    a[i + 1] = sqrt(-2.0 * log(t1)) * sin(6.283185306 * t2);
+  }
+  if (n % 2 == 1) {
+    double t1 = 1 - 0.9999999999 * rng_next(g);
+    double t2 = rng_next(g);
+    a[n - 1] = sqrt(-2.0 * log(t1)) * cos(6.283185306 * t2);
+  }
+}
+
+/* -------------------------------------------------- SuperCell, PG */
+
+typedef struct { double a[3], b[3], rcut2; int D; } cell_t;
+
+/* SuperCell::computeRecipricalVectors / pbc (src/util/SuperCell.cc:14-31) */
+static void cell_init(cell_t* c, const double* a, int D) {
+  c->D = D;
+  for (int d = 0; d < 3; ++d) { c->a[d] = a[d]; c->b[d] = (d < D) ? 1. / a[d] : 0; }
+  double l2 = a[0] * a[0];
+  c->rcut2 = l2;
+  for (int i = 1; i < D; ++i) { l2 = a[i] * a[i]; c->rcut2 = (c->rcut2 > l2) ? l2 : c->rcut2; }
+  c->rcut2 *= 0.25;
+}
+
+static void pbc(const cell_t* c, double* v) {
+  double r2 = 0;
+  for (int d = 0; d < c->D; ++d) r2 += v[d] * v[d];
+  if (r2 > c->rcut2) {
+    for (int d = 0; d < c->D; ++d) {
+      double i = floor(c->b[d] * v[d] + 1000.5) - 1000;
+      v[d] -= c->a[d] * i;
+    }
+  }
+}
+
+void oq_pbc(const double cell[3], int ndim, double* v) {
+  cell_t c;
+  cell_init(&c, cell, ndim);
+  pbc(&c, v);
+}
+
+/* PeriodicGaussian (src/util/PeriodicGaussian.cc:7-40) */
+typedef struct { double nome, prefactor, k; int nmax; } pg_t;
+static const double PG_PI = 3.141592653589793;
+
+int oq_pg_nterms(double alpha, double length) {
+  return 1 + (int)(2.0 * sqrt(alpha * length * length));
+}
+
+static void pg_init(pg_t* p, double alpha, double length) {
+  p->nome = exp(-PG_PI * PG_PI / (alpha * length * length));
+  p->nmax = oq_pg_nterms(alpha, length);
+  p->prefactor = sqrt(PG_PI / (alpha * length * length));
+  p->k = 2 * PG_PI / length;
+}
+
+/* value, gradient, second derivative; complex arithmetic written out */
+static double pg_eval3(const pg_t* p, double x, double* grad, double* d2) {
+  double wr = cos(p->k * x), wi = sin(p->k * x);
+  double s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0;
+  double tr = 1.0, ti = 0.0;
+  double k2 = -p->k * p->k;
+  double nomeN = 1.0;
+  for (int n = 1; n < p->nmax; ++n) {
+    double f = nomeN * nomeN * p->nome;
+    /* term *= f*w  (std::complex: (f*w) computed first, then term*(f*w)) */
+    double fwr = f * wr, fwi = f * wi;
+    double nr = tr * fwr - ti * fwi, ni = tr * fwi + ti * fwr;
+    tr = nr; ti = ni;
+    s0r += tr; s0i += ti;
+    /* term * ik * n : (term*ik) then *n */
+    double ar = -ti * p->k, ai = tr * p->k;
+    s1r += ar * (double)n; s1i += ai * (double)n;
+    s2r += tr * k2 * (double)(n * n); s2i += ti * k2 * (double)(n * n);
+    nomeN *= p->nome;
+  }
+  (void)s0i; (void)s1i; (void)s2i;
+  if (grad) *grad = p->prefactor * 2 * s1r;
+  if (d2) *d2 = p->prefactor * 2 * s2r;
+  return p->prefactor * (1 + 2 * s0r);
+}
+
+static double pg_eval(const pg_t* p, double x) { return pg_eval3(p, x, 0, 0); }
+
+void oq_pg_eval(double alpha, double length, double x, double out[3]) {
+  pg_t p;
+  pg_init(&p, alpha, length);
+  out[0] = pg_eval3(&p, x, &out[1], &out[2]);
+}
+
+/* ------------------------------------------------ Coulomb link action */
+
+/* Coulomb1DLinkAction (src/action/coulomb/Coulomb1DLinkAction.cpp:4-133) */
+typedef struct { double stau, m1, m2, m3, m4, m5; } c1d_t;
+
+static void c1d_init(c1d_t* c, double stau) {
+  c->stau = stau;
+  c->m1 = 1.0 / stau;
+  c->m2 = c->m1 * c->m1;
+  c->m3 = c->m2 * c->m1;
+  c->m4 = c->m3 * c->m1;
+  c->m5 = c->m4 * c->m1;
+}
+
+static double c1d_origin(const c1d_t* c) {
+  double s = c->stau;
+  return s * (1.772453851 + s * (-0.074137740 + s * (0.005834805 + s * (-0.000382686
+         + s * (0.000008738 + s * 0.000002138)))));
+}
+
+static double c1d_u0(const c1d_t* c, double u0, double reff) {
+  double a = 0.25300593 * c->m1 + 0.01432126;
+  double b = 0.07936898 * c->m2 - 0.01634421 / c->stau;
+  double cc = 0.07263383 * c->m3;
+  double d = 0.00940013 * c->m4;
+  double e = 0.03160181 * c->m5;
+  double f = 0.18976335 * c->m1;
+  double g = 0.00343053 * c->m2;
+  double v = u0 + reff * ((u0 * a - 1.) + reff * (f + reff * (g + reff * e * c->stau * c->stau)));
+  v /= 1. + reff * (a + reff * (b + reff * (cc + reff * (d + reff * e))));
+  return v;
+}
+
+static double c1d_u1(const c1d_t* c, double reff) {
+  double a = 0.07776906 * c->m1, b = -0.04695226 * c->m2, cc = 0.01291994 * c->m3;
+  double d = 0.60942687 * c->m1, e = -1.09588064 * c->m2, f = 1.15591565 * c->m3;
+  double g = -0.57485278 * c->m4, h = 0.15212765 * c->m5;
+  double v = reff * reff * (a + reff * (b + reff * cc));
+  v /= (1 + reff * (d + reff * (e + reff * (f + reff * (g + reff * h)))));
+  return v;
+}
+
+static double c1d_u2(const c1d_t* c, double reff) {
+  double a = 9.52130652e-04 * c->m3, b = -6.39946071e-04 * c->m4, cc = 1.14359558e-04 * c->m5;
+  double d = -7.59575215e-01 * c->m3, e = 5.53798980e-01 * c->m4, f = -7.15626226e-02 * c->m5;
+  double g = -3.49081618e-02 * c->m5 * c->m1, h = 8.40084881e-03 * c->m5 * c->m2;
+  return pow(reff, 4) * (a + reff * (b + reff * cc))
+         / (1 + pow(reff, 3) * (d + reff * (e + reff * (f + reff * (g + reff * h)))));
+}
+
+static double c1d_u3(const c1d_t* c, double reff) {
+  double a = -2.06984256e-1 * c->m5, b = -9.51423947e-3 * c->m5 * c->m1;
+  double cc = 8.97532561e-2 * c->m5 * c->m2, d = 6.10876797e+4 * c->m3;
+  double e = -1.19016292e+5 * c->m4, f = 9.29836599e+4 * c->m5;
+  double g = -3.49919130e+4 * c->m5 * c->m1, h = 6.42490539e+3 * c->m5 * c->m2;
+  double i = -6.02729064e+2 * c->m5 * c->m3, j = 6.01285550e+1 * c->m5 * c->m4;
+  return pow(reff, 6) * (a + reff * (b + reff * cc))
+         / (1 + pow(reff, 3) * (d + reff * (e + reff * (f + reff * (g + reff * (h + reff * (i + reff * j)))))));
+}
+
+static double c1d_u4(const c1d_t* c, double reff) {
+  double a = -2.84102296e-6 * c->m5, b = 6.26961672e-7 * c->m5 * c->m2;
+  double cc = -2.20166473e-1 * c->m3, d = 1.07981903e-1 * c->m5;
+  double e = -1.94671111e-2 * c->m5 * c->m2, f = 1.56217930e-3 * c->m5 * c->m4;
+  return pow(reff, 6) * (a + b * reff * reff)
+         / (1 + reff * reff * reff * (cc + reff * reff * (d + reff * reff * (e + reff * reff * f))));
+}
+
+double oq_coulomb1d(double stau, int which, double reff) {
+  c1d_t c;
+  c1d_init(&c, stau);
+  switch (which) {
+    case -1: return c1d_origin(&c);
+    case 0: return c1d_u0(&c, c1d_origin(&c), reff);
+    case 1: return c1d_u1(&c, reff);
+    case 2: return c1d_u2(&c, reff);
+    case 3: return c1d_u3(&c, reff);
+    default: return c1d_u4(&c, reff);
+  }
+}
+
+/* Coulomb3DLinkAction (src/action/coulomb/Coulomb3DLinkAction.cpp:13-90) */
+static double c3d(const c1d_t* c, int which, double reff) {
+  double stau = c->stau;
+  if (which == 0) {
+    double u0 = c1d_origin(c);
+    double u1_0 = c1d_u0(c, u0, reff);
+    double u1_1 = c1d_u1(c, reff);
+    double a = 0.5 * reff * reff / (stau * stau);
+    double b = exp(-2. * a);
+    double cc = 1.0 / (1.0 + 2 * (1.0 - b) * u1_1 / (a + 1e-200));
+    return u1_0 + log(cc);
+  } else if (which == 1) {
+    double u1_1 = c1d_u1(c, reff), u1_2 = c1d_u2(c, reff);
+    double a = 0.5 * reff * reff / (stau * stau);
+    double b = exp(-2. * a);
+    double cc = 1.0 / (1.0 + 2 * (1.0 - b) * u1_1 / (a + 1e-200));
+    return u1_1 + cc * (b * u1_1 - 4 * (1 - b) * u1_2 / (a + 1e-200));
+  } else if (which == 2) {
+    double u1_1 = c1d_u1(c, reff), u1_2 = c1d_u2(c, reff), u1_3 = c1d_u3(c, reff);
+    double a = 2. * stau * stau / (reff * reff + 1e-50);
+    double b = exp(-2. / (a + 1e-200));
+    double cc = 1. / (1 + 2 * a * (1 - b) * u1_1);
+    return u1_2 + 0.25 * cc / (a + 1e-200)
+           * (cc * (b * u1_1 * (1 + 2 * a * u1_1) + 8 * a * b * u1_2
+                    + 32 * a * a * a * (1 - b) * (1 - b) * u1_2 * u1_2)
+              - 24 * a * a * (1 - b) * u1_3);
+  } else if (which == 3) {
+    double u1_1 = c1d_u1(c, reff), u1_2 = c1d_u2(c, reff);
+    double u1_3 = c1d_u3(c, reff), u1_4 = c1d_u4(c, reff);
+    double a = 2. * stau * stau / (reff * reff + 1e-50);
+    double b = exp(-2. / (a + 1e-200));
+    double cc = 1. / (1 + 2 * a * (1 - b) * u1_1);
+    return u1_3 + cc * cc * cc / (24. * a * a + 1e-200)
+      * (128 * a * a * a * a * a * b * b * b * (4 * u1_2 * u1_2 * u1_2
+      - 9 * u1_1 * u1_2 * u1_3 + 6 * u1_1 * u1_1 * u1_4)
+      + 64 * a * a * a * (a * u1_2 * (
+      -8 * a * u1_2 * u1_2 + 9 * (1 + 2 * a * u1_1) * u1_3)
+      - 3 * (1 + 2 * a * u1_1) * (1 + 2 * a * u1_1)
+      * u1_4) + 2 * a * b * b
+      * (2 * a * u1_1 * u1_1 * u1_1
+      + 96 * a * a * u1_2 * (u1_2 - 8 * a * a * u1_2 * u1_2
+      + 3 * a * u1_3) + 12 * a * u1_1 * (u1_2 - 6 * a * u1_3
+      + 144 * a * a * a * u1_2 * u1_3
+      - 32 * a * a * u1_4) + u1_1 * u1_1
+      * (1 - 1152 * a * a * a * a * u1_4))
+      + b * (4 * a * a * u1_1 * u1_1 * u1_1
+      + 12 * a * (u1_2 - 16 * a * a * u1_2 * u1_2
+      + 128 * a * a * a * a * u1_2 * u1_2 * u1_2 + 6 * a * u1_3 - 96 * a * a * a
+      * u1_2 * u1_3 + 16 * a * a * u1_4) + 4 * u1_1 * u1_1
+      * (a + 576 * a * a * a * a * a * u1_4)
+      + u1_1 * (1 + 24 * a * a * (u1_2 + 6 * a * u1_3
+              - 144 * a * a * a * u1_2 * u1_3 + 64 * a * a * u1_4))));
+  }
+  return c1d_u4(c, reff);
+}
+
+double oq_coulomb3d(double stau, int which, double reff) {
+  c1d_t c;
+  c1d_init(&c, stau);
+  return c3d(&c, which, reff);
+}
+
+/* CoulombAction::u (src/action/CoulombAction.cc:209-248), no screening shift */
+static double coulomb_u(double r, int order, double mu, double q1q2, double tau) {
+  r = sqrt(r * r + 0.0);
+  double reff = 2.0 * mu * q1q2 * r;
+  double stau = q1q2 * sqrt(2.0 * mu * tau);
+  c1d_t c;
+  c1d_init(&c, stau);
+  return c3d(&c, order, reff);
+}
+
+/* CoulombAction::utau (src/action/CoulombAction.cc:250-259) */
+static double coulomb_utau(double r, int order, double mu, double q1q2, double tau) {
+  const double eps = 1e-7;
+  double up = coulomb_u(r, order, mu, q1q2, tau * (1.0 + eps));
+  double um = coulomb_u(r, order, mu, q1q2, tau * (1.0 - eps));
+  return (up - um) / (2.0 * eps * tau);
+}
+
+/* ----------------------------------------------------------- the chain */
+
+typedef struct {
+  int s1, s2, norder, ngpts;
+  double rgridinv, logrratioinv;
+  double* u;   /* [ngpts][norder+1] */
+  double* ut;  /* [ngpts][norder+1] */
+} pair_t;
+
+typedef struct {
+  int sp, model, ifirst, ns, nodal;
+  double mass, omega, T, coshwt, sinhwt, c, scale;
+  pg_t pg[3];
+  double *dm, *newdm, *dist, *newdist;  /* dist [S][2][ns] */
+  double* inv;                           /* [S][ns*ns] latest inverse per slice */
+  int not_my_species;
+  long nerror;
+} fn_t;
+
+struct oq_chain {
+  pq_chain_desc d;
+  int N, D, M, L, S, nsp;
+  int ifirst[PQ_MAX_SPECIES];
+  int spec[MAXN];
+  double lambda[MAXN];
+  double tau, T;
+  cell_t cell;
+  /* periodic Gaussians of spring (level, species, dim), mover */
+  int npg_levels;
+  pg_t* spg; unsigned char* spg_on;   /* [(L+2)][nsp][3] */
+  pg_t* mpg; unsigned char* mpg_on;
+  double* beads;
+  int npair;
+  pair_t pair[16];
+  int nfn;
+  fn_t fn[PQ_MAX_NODES];
+  double *sec1, *sec2, *mov;
+  int idx[PQ_MAX_MOVING];
+  oq_rng rng;
+  pq_chain_stats st;
+};
+
+static int pg_index(const oq_chain* ch, int level, int sp, int d) {
+  return (level * ch->nsp + sp) * 3 + d;
+}
+
+oq_chain* oq_create(const pq_chain_desc* desc) {
+  oq_chain* ch = (oq_chain*)calloc(1, sizeof(oq_chain));
+  ch->d = *desc;
+  ch->D = desc->ndim;
+  ch->M = desc->nslice;
+  ch->L = desc->nlevel;
+  ch->S = (1 << ch->L) + 1;
+  ch->nsp = desc->nspecies;
+  ch->tau = desc->tau;
+  ch->T = 1.0 / (desc->tau * desc->nslice);
+  int n = 0;
+  for (int s = 0; s < ch->nsp; ++s) {
+    ch->ifirst[s] = n;
+    for (int i = 0; i < desc->species[s].count; ++i) {
+      ch->spec[n] = s;
+      ch->lambda[n] = 0.5 / desc->species[s].mass;
+      ++n;
+    }
+  }
+  ch->N = n;
+  cell_init(&ch->cell, desc->cell, ch->D);
+  /* SpringAction pg(level,species,dim): alpha = m/(2 tau 2^l) (SpringAction.cc:31-49);
+     FreeMover / FreeMoverPBC: alpha = m/(tau 2^l), levels 0..L+1 (FreeMoverPBC.cc:37-51) */
+  ch->npg_levels = ch->L + 2;
+  int npg = ch->npg_levels * ch->nsp * 3;
+  ch->spg = (pg_t*)calloc(npg, sizeof(pg_t));
+  ch->mpg = (pg_t*)calloc(npg, sizeof(pg_t));
+  ch->spg_on = (unsigned char*)calloc(npg, 1);
+  ch->mpg_on = (unsigned char*)calloc(npg, 1);
+  for (int l = 0; l < ch->npg_levels; ++l)
+    for (int s = 0; s < ch->nsp; ++s)
+      for (int d = 0; d < ch->D; ++d) {
+        double m = desc->species[s].mass, len = desc->cell[d];
+        double as = m / (2 * ch->tau * (1 << l));
+        double am = m / (ch->tau * (1 << l));
+        int k = pg_index(ch, l, s, d);
+        if (l <= ch->L && oq_pg_nterms(as, len) < 16) { ch->spg_on[k] = 1; pg_init(&ch->spg[k], as, len); }
+        if (oq_pg_nterms(am, len) < 16) { ch->mpg_on[k] = 1; pg_init(&ch->mpg[k], am, len); }
+      }
+  ch->beads = (double*)calloc((size_t)ch->M * ch->N * ch->D, sizeof(double));
+  /* CoulombAction constructor (src/action/CoulombAction.cc:45-90): one
+     PairAction per interacting species pair; rmin/rmax/ngpts defaults are
+     written back and inherited by later pairs. */
+  if (desc->coulomb.enabled) {
+    double eps = desc->coulomb.epsilon == 0 ? 1 : desc->coulomb.epsilon;
+    double rmin = desc->coulomb.rmin, rmax = desc->coulomb.rmax;
+    int ngpts = desc->coulomb.ngpts;
+    for (int i = 0; i < ch->nsp; ++i)
+      for (int j = i; j < ch->nsp; ++j) {
+        const pq_species_desc *s1 = &desc->species[i], *s2 = &desc->species[j];
+        double q1q2 = s1->charge * s2->charge / eps;
+        if (q1q2 != 0 && (i != j || s1->count > 1)) {
+          double mu = 1. / (1. / s1->mass + 1. / s2->mass);
+          if (rmin == 0) rmin = 0.005 / (2 * ((mu > 500) ? 1 : mu) * fabs(q1q2));
+          if (rmax == 0) {
+            double a2 = 0;
+            for (int d = 0; d < ch->D; ++d) a2 += desc->cell[d] * desc->cell[d];
+            rmax = sqrt(a2);
+          }
+          if (ngpts == 0) ngpts = 500;
+          pair_t* p = &ch->pair[ch->npair++];
+          p->s1 = i; p->s2 = j;
+          p->norder = (mu > 500) ? 0 : desc->coulomb.norder;
+          p->ngpts = ngpts;
+          /* PairAction(EmpiricalPairAction) (src/action/PairAction.cc:161-185) */
+          p->rgridinv = 1.0 / rmin;
+          p->logrratioinv = (ngpts - 1) / log(rmax / rmin);
+          int K1 = p->norder + 1;
+          p->u = (double*)calloc((size_t)ngpts * K1, sizeof(double));
+          p->ut = (double*)calloc((size_t)ngpts * K1, sizeof(double));
+          for (int ig = 0; ig < ngpts; ++ig) {
+            double r = (1. / p->rgridinv) * exp(ig / p->logrratioinv);
+            for (int k = 0; k < K1; ++k) {
+              p->u[ig * K1 + k] = coulomb_u(r, k, mu, q1q2, ch->tau);
+              p->ut[ig * K1 + k] = coulomb_utau(r, k, mu, q1q2, ch->tau);
+            }
+          }
+        }
+      }
+  }
+  /* FixedNodeAction + node model per <FixedNodeAction> */
+  ch->nfn = desc->nnodes;
+  for (int a = 0; a < ch->nfn; ++a) {
+    const pq_node_desc* nd = &desc->nodes[a];
+    fn_t* f = &ch->fn[a];
+    f->sp = nd->species;
+    f->model = nd->model;
+    f->ifirst = ch->ifirst[nd->species];
+    f->ns = desc->species[nd->species].count;
+    f->nodal = nd->nodal_action;
+    f->mass = desc->species[nd->species].mass;
+    f->omega = nd->omega;
+    f->T = (nd->temperature == 0) ? ch->T : nd->temperature;
+    f->scale = 1.0;
+    /* SHONodes constructor (src/fixednode/SHONodes.cc:22-35) */
+    f->coshwt = cosh(f->omega * 0.5 / f->T);
+    f->sinhwt = sinh(f->omega * 0.5 / f->T);
+    f->c = f->mass * 0.5 * f->omega / f->sinhwt;
+    /* FreeParticleNodes pg[d] = PeriodicGaussian(mass*T, a_d) (FreeParticleNodes.cc:54-58) */
+    for (int d = 0; d < ch->D; ++d) pg_init(&f->pg[d], f->mass * f->T, desc->cell[d]);
+    f->dm = (double*)calloc(ch->S, sizeof(double));
+    f->newdm = (double*)calloc(ch->S, sizeof(double));
+    f->dist = (double*)calloc((size_t)ch->S * 2 * f->ns, sizeof(double));
+    f->newdist = (double*)calloc((size_t)ch->S * 2 * f->ns, sizeof(double));
+    f->inv = (double*)calloc((size_t)ch->S * f->ns * f->ns, sizeof(double));
+  }
+  size_t secsz = (size_t)ch->S * ch->N * ch->D;
+  ch->sec1 = (double*)calloc(secsz, sizeof(double));
+  ch->sec2 = (double*)calloc(secsz, sizeof(double));
+  ch->mov = (double*)calloc((size_t)ch->S * PQ_MAX_MOVING * ch->D, sizeof(double));
+  oq_rng_glibc(ch, 0);
+  return ch;
+}
+
+void oq_destroy(oq_chain* ch) {
+  if (!ch) return;
+  for (int p = 0; p < ch->npair; ++p) { free(ch->pair[p].u); free(ch->pair[p].ut); }
+  for (int a = 0; a < ch->nfn; ++a) {
+    free(ch->fn[a].dm); free(ch->fn[a].newdm); free(ch->fn[a].dist);
+    free(ch->fn[a].newdist); free(ch->fn[a].inv);
+  }
+  free(ch->spg); free(ch->mpg); free(ch->spg_on); free(ch->mpg_on);
+  free(ch->beads); free(ch->sec1); free(ch->sec2); free(ch->mov);
+  free(ch);
+}
+
+void oq_set_beads(oq_chain* ch, const double* b) {
+  memcpy(ch->beads, b, sizeof(double) * ch->M * ch->N * ch->D);
+}
+void oq_get_beads(const oq_chain* ch, double* b) {
+  memcpy(b, ch->beads, sizeof(double) * ch->M * ch->N * ch->D);
+}
+int oq_npair(const oq_chain* ch) { return ch->npair; }
+
+int oq_pair_table(const oq_chain* ch, int ip, int* meta, double* grid, double* u, double* ut) {
+  if (ip < 0 || ip >= ch->npair) return -1;
+  const pair_t* p = &ch->pair[ip];
+  meta[0] = p->s1; meta[1] = p->s2; meta[2] = p->norder; meta[3] = p->ngpts;
+  grid[0] = p->rgridinv; grid[1] = p->logrratioinv;
+  size_t n = (size_t)p->ngpts * (p->norder + 1);
+  if (u) memcpy(u, p->u, n * sizeof(double));
+  if (ut) memcpy(ut, p->ut, n * sizeof(double));
+  return 0;
+}
+
+void oq_rng_glibc(oq_chain* ch, unsigned int seed) {
+  memset(&ch->rng, 0, sizeof(ch->rng));
+  ch->rng.mode = 0;
+  glibc_seed(&ch->rng, seed);
+}
+void oq_rng_philox(oq_chain* ch, uint64_t seed, uint32_t stream) {
+  memset(&ch->rng, 0, sizeof(ch->rng));
+  ch->rng.mode = 1; ch->rng.seed = seed; ch->rng.stream = stream;
+}
+void oq_rng_inject(oq_chain* ch, const double* u, int64_t n) {
+  memset(&ch->rng, 0, sizeof(ch->rng));
+  ch->rng.mode = 2; ch->rng.inj = u; ch->rng.ninj = n;
+}
+double oq_rng_next(oq_chain* ch) { return rng_next(&ch->rng); }
+uint64_t oq_rng_count(const oq_chain* ch) { return ch->rng.count; }
+
+/* ---------------------------------------------------------- actions */
+
+#define SEC(b, t, i, d) ((b)[((size_t)(t) * ch->N + (i)) * ch->D + (d)])
+#define MOV(b, t, m, d) ((b)[((size_t)(t) * nmov + (m)) * ch->D + (d)])
+
+/* SpringAction::getActionDifference (src/action/SpringAction.cc:56-98) */
+static double spring_diff(oq_chain* ch, const double* sec, const double* mov,
+                          const int* idx, int nmov, int level) {
+  const int s = 1 << level, D = ch->D;
+  double dA = 0;
+  for (int t = s; t < ch->S; t += s) {
+    for (int m = 0; m < nmov; ++m) {
+      int i = idx[m], sp = ch->spec[i];
+      double inv2Sigma2 = 0.25 / (ch->lambda[i] * ch->tau * s);
+      double delta[3];
+      for (int d = 0; d < D; ++d) delta[d] = MOV(mov, t, m, d) - MOV(mov, t - s, m, d);
+      pbc(&ch->cell, delta);
+      double temp = 1.0;
+      for (int d = 0; d < D; ++d) {
+        int k = pg_index(ch, level, sp, d);
+        if (ch->spg_on[k]) temp *= pg_eval(&ch->spg[k], delta[d]);
+        else dA += delta[d] * delta[d] * inv2Sigma2;
+      }
+      temp = 1.0 / temp;
+      for (int d = 0; d < D; ++d) delta[d] = SEC(sec, t, i, d) - SEC(sec, t - s, i, d);
+      pbc(&ch->cell, delta);
+      for (int d = 0; d < D; ++d) {
+        int k = pg_index(ch, level, sp, d);
+        if (ch->spg_on[k]) temp *= pg_eval(&ch->spg[k], delta[d]);
+        else dA -= delta[d] * delta[d] * inv2Sigma2;
+      }
+      dA += log(temp);
+    }
+  }
+  return dA;
+}
+
+/* SHOAction::getActionDifference (src/action/SHOAction.cc:19-65) */
+static double sho_diff(oq_chain* ch, const pq_sho_desc* a, const double* sec,
+                       const double* mov, const int* idx, int nmov, int level) {
+  const int s = 1 << level, D = ch->D;
+  const int ifirst = ch->ifirst[a->species], np = ch->d.species[a->species].count;
+  const int ndim = a->ndim ? a->ndim : D;
+  const double omega = a->omega, m = ch->d.species[a->species].mass;
+  double dA = 0;
+  double wt = omega * ch->tau * s;
+  double sinhwt = sinh(wt), coshwt = cosh(wt);
+  double div1 = 1.0 / (2.0 * sinhwt), div2 = 1.0 / (2.0 * ch->tau * s);
+  for (int t = s; t < ch->S; t += s) {
+    for (int mm = 0; mm < nmov; ++mm) {
+      int i = idx[mm];
+      if (i < ifirst || i >= ifirst + np) continue;
+      for (int pass = 0; pass < 2; ++pass) {
+        double r1[3], r0[3];
+        for (int d = 0; d < D; ++d) {
+          if (pass == 0) { r1[d] = MOV(mov, t, mm, d) - a->center[d]; r0[d] = MOV(mov, t - s, mm, d) - a->center[d]; }
+          else { r1[d] = SEC(sec, t, i, d) - a->center[d]; r0[d] = SEC(sec, t - s, i, d) - a->center[d]; }
+        }
+        double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
+        for (int d = D - ndim; d < D; ++d) {
+          double dl = r1[d] - r0[d];
+          r0r0 += r0[d] * r0[d]; r0r1 += r0[d] * r1[d]; r1r1 += r1[d] * r1[d]; delta2 += dl * dl;
+        }
+        double v = m * omega * ((r1r1 + r0r0) * coshwt - 2.0 * r0r1) * div1 - m * delta2 * div2;
+        if (pass == 0) dA += v; else dA -= v;
+      }
+    }
+  }
+  return dA;
+}
+
+/* PairAction grid lookups u00 / uk0 (src/action/PairAction.cc:601-621) */
+static void grid_locate(const pair_t* p, double q, int* ig, double* x) {
+  q *= p->rgridinv;
+  double xx = log(q) * p->logrratioinv;
+  int i = (int)xx;
+  if (q < 1) { i = 0; xx = 0; }
+  else if (i > p->ngpts - 2) { i = p->ngpts - 2; xx = 1; }
+  else xx -= i;
+  *ig = i; *x = xx;
+}
+
+static double u00(const pair_t* p, double r) {
+  int i; double x;
+  grid_locate(p, r, &i, &x);
+  int K1 = p->norder + 1;
+  return (1 - x) * p->u[i * K1] + x * p->u[(i + 1) * K1];
+}
+
+static double uk0(const pair_t* p, double q, double s2) {
+  int i; double x;
+  grid_locate(p, q, &i, &x);
+  int K1 = p->norder + 1;
+  double action = 0;
+  for (int k = p->norder; k >= 0; k--) {
+    action *= s2;
+    action += (1 - x) * p->u[i * K1 + k] + x * p->u[(i + 1) * K1 + k];
+  }
+  return action;
+}
+
+/* uk0CalcDerivatives, the u and utau parts (src/action/PairAction.cc:640-657) */
+static void uk0_u_utau(const pair_t* p, double q, double s2, double* u, double* ut) {
+  int i; double x;
+  grid_locate(p, q, &i, &x);
+  int K1 = p->norder + 1;
+  double a = 0, b = 0;
+  for (int k = p->norder; k >= 0; k--) {
+    a *= s2; a += (1 - x) * p->u[i * K1 + k] + x * p->u[(i + 1) * K1 + k];
+    b *= s2; b += (1 - x) * p->ut[i * K1 + k] + x * p->ut[(i + 1) * K1 + k];
+  }
+  *u = a; *ut = b;
+}
+
+static double dot3(const double* a, const double* b, int D) {
+  double s = 0;
+  for (int d = 0; d < D; ++d) s += a[d] * b[d];
+  return s;
+}
+
+/* PairAction::getActionDifference without exchange (src/action/PairAction.cc:219-360) */
+static double pair_diff(oq_chain* ch, const pair_t* p, const double* sec,
+                        const double* mov, const int* idx, int nmov, int level) {
+  const int s = 1 << level, D = ch->D;
+  const int if1 = ch->ifirst[p->s1], n1 = ch->d.species[p->s1].count;
+  const int if2 = ch->ifirst[p->s2], n2 = ch->d.species[p->s2].count;
+  double dA = 0;
+  for (int im = 0; im < nmov; ++im) {
+    int i = idx[im], jb, je;
+    if (i >= if1 && i < if1 + n1) { jb = if2; je = if2 + n2; }
+    else if (i >= if2 && i < if2 + n2) { jb = if1; je = if1 + n1; }
+    else continue;
+    for (int j = jb; j < je; ++j) {
+      int isMoving = 0, jm = 0;
+      for (int k = 0; k < nmov; ++k) if (j == idx[k]) { isMoving = 1; jm = k; break; }
+      if (isMoving && i <= j) continue;
+      double prevDelta[3], prevMovingDelta[3];
+      for (int d = 0; d < D; ++d) prevDelta[d] = SEC(sec, 0, i, d) - SEC(sec, 0, j, d);
+      pbc(&ch->cell, prevDelta);
+      for (int d = 0; d < D; ++d) prevMovingDelta[d] = prevDelta[d];
+      double prevR = sqrt(dot3(prevDelta, prevDelta, D));
+      double prevMovingR = prevR;
+      for (int t = s; t < ch->S; t += s) {
+        double delta[3], svec[3];
+        for (int d = 0; d < D; ++d)
+          delta[d] = MOV(mov, t, im, d) - (isMoving ? MOV(mov, t, jm, d) : SEC(sec, t, j, d));
+        pbc(&ch->cell, delta);
+        double r = sqrt(dot3(delta, delta, D));
+        double q = 0.5 * (r + prevMovingR);
+        double action = 0.;
+        if (level < 3 && p->norder > 0) {
+          for (int d = 0; d < D; ++d) svec[d] = delta[d] - prevMovingDelta[d];
+          double s2 = dot3(svec, svec, D) / (q * q);
+          action += uk0(p, q, s2) * s;
+        } else {
+          action += u00(p, q) * s;
+        }
+        dA += action;
+        for (int d = 0; d < D; ++d) prevMovingDelta[d] = delta[d];
+        prevMovingR = r;
+        for (int d = 0; d < D; ++d) delta[d] = SEC(sec, t, i, d) - SEC(sec, t, j, d);
+        pbc(&ch->cell, delta);
+        r = sqrt(dot3(delta, delta, D));
+        q = 0.5 * (r + prevR);
+        action = 0.;
+        if (level < 3 && p->norder > 0) {
+          for (int d = 0; d < D; ++d) svec[d] = delta[d] - prevDelta[d];
+          double s2 = dot3(svec, svec, D) / (q * q);
+          action += uk0(p, q, s2) * s;
+        } else {
+          action += u00(p, q) * s;
+        }
+        dA -= action;
+        for (int d = 0; d < D; ++d) prevDelta[d] = delta[d];
+        prevR = r;
+      }
+    }
+  }
+  return dA;
+}
+
+/* CompositeAction::getActionDifference (src/action/CompositeAction.cc:30-37)
+   and CoulombAction::getActionDifference (src/action/CoulombAction.cc:100-105) */
+double oq_action_difference(oq_chain* ch, const double* sec, const double* mov,
+                            const int* idx, int nmov, int level, int mask) {
+  double diff = 0;
+  if ((mask & 1) && ch->d.spring) diff += spring_diff(ch, sec, mov, idx, nmov, level);
+  if (mask & 2)
+    for (int a = 0; a < ch->d.nsho; ++a) diff += sho_diff(ch, &ch->d.sho[a], sec, mov, idx, nmov, level);
+  if ((mask & 4) && ch->npair) {
+    double u = 0;
+    for (int p = 0; p < ch->npair; ++p) u += pair_diff(ch, &ch->pair[p], sec, mov, idx, nmov, level);
+    diff += u;
+  }
+  return diff;
+}
+
+/* path access with wrap, identity permutation (SerialPaths.cc:68-117) */
+static const double* bead(const oq_chain* ch, int i, int t) {
+  t %= ch->M;
+  if (t < 0) t += ch->M;
+  return &ch->beads[((size_t)t * ch->N + i) * ch->D];
+}
+
+/* getBeadAction utau of SpringAction (SpringAction.cc:112-123), SHOAction
+   (SHOAction.cc:71-105), PairAction (PairAction.cc:509-576), summed as
+   CompositeAction::getBeadAction (CompositeAction.cc:57-67). */
+double oq_bead_utau(const oq_chain* ch, int ipart, int islice, int mask) {
+  const int D = ch->D;
+  double utau = 0;
+  if ((mask & 1) && ch->d.spring) {
+    double delta[3];
+    const double *a = bead(ch, ipart, islice), *b = bead(ch, ipart, islice - 1);
+    for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
+    pbc(&ch->cell, delta);
+    double ui = D / (2 * ch->tau) - dot3(delta, delta, D) / (4.0 * ch->lambda[ipart] * ch->tau * ch->tau);
+    utau += ui;
+  }
+  if (mask & 2) {
+    for (int s = 0; s < ch->d.nsho; ++s) {
+      const pq_sho_desc* a = &ch->d.sho[s];
+      int ifirst = ch->ifirst[a->species], np = ch->d.species[a->species].count;
+      double ui = 0;
+      if (!(ipart < ifirst || ipart >= ifirst + np)) {
+        int ndim = a->ndim ? a->ndim : D;
+        double omega = a->omega, m = ch->d.species[a->species].mass;
+        double wt = omega * ch->tau, sinhwt = sinh(wt), coshwt = cosh(wt);
+        double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / ch->tau;
+        const double *p1 = bead(ch, ipart, islice), *p0 = bead(ch, ipart, islice - 1);
+        double r1[3], r0[3], delta[3];
+        for (int d = 0; d < D; ++d) { r1[d] = p1[d] - a->center[d]; r0[d] = p0[d] - a->center[d]; delta[d] = p1[d] - p0[d]; }
+        pbc(&ch->cell, delta);
+        double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
+        for (int d = D - ndim; d < D; ++d) {
+          r0r0 += r0[d] * r0[d]; r0r1 += r0[d] * r1[d]; r1r1 += r1[d] * r1[d]; delta2 += delta[d] * delta[d];
+        }
+        ui = 0.5 * ndim * omega * cothwt
+             + 0.5 * m * omega * omega * ((r1r1 + r0r0) * (1 - cothwt * cothwt) + 2.0 * r0r1 * coshwt * cschwt * cschwt)
+             + 0.5 * m * delta2 * tin * tin - 0.5 * ndim * tin;
+      }
+      utau += ui;
+    }
+  }
+  if ((mask & 4) && ch->npair) {
+    double uc = 0;
+    for (int pp = 0; pp < ch->npair; ++pp) {
+      const pair_t* p = &ch->pair[pp];
+      int if1 = ch->ifirst[p->s1], n1 = ch->d.species[p->s1].count;
+      int if2 = ch->ifirst[p->s2], n2 = ch->d.species[p->s2].count;
+      int jb, je;
+      double up = 0;
+      if (ipart >= if1 && ipart < if1 + n1) { jb = if2; je = if2 + n2; }
+      else if (ipart >= if2 && ipart < if2 + n2) { jb = if1; je = if1 + n1; }
+      else { uc += up; continue; }
+      for (int j = jb; j < je; ++j) {
+        if (ipart == j) continue;
+        double delta[3], prevDelta[3], svec[3];
+        const double *ai = bead(ch, ipart, islice), *aj = bead(ch, j, islice);
+        const double *bi = bead(ch, ipart, islice - 1), *bj = bead(ch, j, islice - 1);
+        for (int d = 0; d < D; ++d) delta[d] = ai[d] - aj[d];
+        pbc(&ch->cell, delta);
+        double r = sqrt(dot3(delta, delta, D));
+        for (int d = 0; d < D; ++d) prevDelta[d] = bi[d] - bj[d];
+        pbc(&ch->cell, prevDelta);
+        double prevR = sqrt(dot3(prevDelta, prevDelta, D));
+        double q = 0.5 * (r + prevR);
+        for (int d = 0; d < D; ++d) svec[d] = delta[d] - prevDelta[d];
+        double s2 = dot3(svec, svec, D) / (q * q);
+        double v, vtau;
+        uk0_u_utau(p, q, s2, &v, &vtau);
+        up += 0.5 * vtau;
+      }
+      uc += up;
+    }
+    utau += uc;
+  }
+  return utau;
+}
+
+/* -------------------------------------------------------- node models */
+
+/* Unblocked LU with partial pivoting (as LAPACK dgetf2) on a row-major n x n
+   matrix; returns info (0 ok, k+1 for the first zero pivot) and the pivot
+   sign product; then the inverse A^-1 = U^-1 L^-1 P written row-major. */
+static int lu_det_inverse(double* a, int n, double* det, double* inv) {
+  int ipiv[PQ_MAX_NODE_PART];
+  int info = 0;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double amax = fabs(a[k * n + k]);
+    for (int r = k + 1; r < n; ++r)
+      if (fabs(a[r * n + k]) > amax) { amax = fabs(a[r * n + k]); p = r; }
+    ipiv[k] = p;
+    if (a[p * n + k] != 0.0) {
+      if (p != k)
+        for (int c = 0; c < n; ++c) { double t = a[k * n + c]; a[k * n + c] = a[p * n + c]; a[p * n + c] = t; }
+      for (int r = k + 1; r < n; ++r) a[r * n + k] /= a[k * n + k];
+    } else if (info == 0) {
+      info = k + 1;
+    }
+    for (int r = k + 1; r < n; ++r)
+      for (int c = k + 1; c < n; ++c) a[r * n + c] -= a[r * n + k] * a[k * n + c];
+  }
+  double d = 1;
+  for (int i = 0; i < n; ++i) { d *= a[i * n + i]; d *= (i == ipiv[i]) ? 1 : -1; }
+  *det = d;
+  if (info) return info;
+  for (int j = 0; j < n; ++j) {
+    double x[PQ_MAX_NODE_PART];
+    for (int r = 0; r < n; ++r) x[r] = (r == j) ? 1.0 : 0.0;
+    for (int k = 0; k < n; ++k) if (ipiv[k] != k) { double t = x[k]; x[k] = x[ipiv[k]]; x[ipiv[k]] = t; }
+    for (int r = 0; r < n; ++r) for (int c = 0; c < r; ++c) x[r] -= a[r * n + c] * x[c];
+    for (int r = n - 1; r >= 0; --r) {
+      for (int c = r + 1; c < n; ++c) x[r] -= a[r * n + c] * x[c];
+      x[r] /= a[r * n + r];
+    }
+    for (int r = 0; r < n; ++r) inv[r * n + j] = x[r];
+  }
+  return 0;
+}
+
+/* density-matrix element rho(i,j): row i from r2 (section +beta/2), column j
+   from r1 (SHONodes.cc:58-67, FreeParticleNodes.cc:79-90) */
+static double node_elem(const oq_chain* ch, const fn_t* f, const double* r1,
+                        const double* r2, int i, int j) {
+  const int D = ch->D;
+  const double* rj = &r1[(size_t)(f->ifirst + j) * D];
+  const double* ri = &r2[(size_t)(f->ifirst + i) * D];
+  if (f->model == PQ_NODES_SHO) {
+    double rj2 = dot3(rj, rj, D), ri2 = dot3(ri, ri, D), rirj = dot3(ri, rj, D);
+    return f->scale * exp(-f->c * ((ri2 + rj2) * f->coshwt - 2.0 * rirj));
+  }
+  double delta[3];
+  for (int d = 0; d < D; ++d) delta[d] = rj[d] - ri[d];
+  pbc(&ch->cell, delta);
+  double ear2 = f->scale;
+  for (int d = 0; d < D; ++d) ear2 *= pg_eval(&f->pg[d], delta[d]);
+  return ear2;
+}
+
+/* NodeModel::evaluate for SHONodes (SHONodes.cc:51-128) and FreeParticleNodes
+   (FreeParticleNodes.cc:72-157): returns err, det in *det; the inverse of the
+   slice is kept for evaluateDistance. */
+static int node_evaluate(oq_chain* ch, fn_t* f, const double* r1, const double* r2,
+                         int islice, int scaleMag, double* detOut) {
+  const int n = f->ns;
+  double a[PQ_MAX_NODE_PART * PQ_MAX_NODE_PART];
+  int err = 0, pass = 0;
+  double det = 0;
+  do {
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) a[i * n + j] = node_elem(ch, f, r1, r2, i, j);
+    int info = lu_det_inverse(a, n, &det, &f->inv[(size_t)islice * n * n]);
+    err = info != 0;
+    if (err) f->nerror++;
+    if (scaleMag && (!err && (fabs(det) < 1e-50 || fabs(det) > 1e50))) {
+      if (fabs(det) < 1e-250) f->scale *= 2;
+      else f->scale *= pow(fabs(det), -1. / n);
+    }
+    if (++pass >= 200) break;
+  } while (scaleMag && (err || (fabs(det) < 1e-50 || fabs(det) > 1e50)));
+  *detOut = det;
+  return err;
+}
+
+/* evaluateDistance: SHONodes.cc:130-170, FreeParticleNodes.cc:184-234 */
+static void node_distance(oq_chain* ch, fn_t* f, const double* r1, const double* r2,
+                          int islice, double* d1, double* d2) {
+  const int n = f->ns, D = ch->D;
+  const double* inv = &f->inv[(size_t)islice * n * n];
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int a = 0; a < n; ++a) {
+      double lg[3] = {0, 0, 0};
+      for (int b = 0; b < n; ++b) {
+        int jp = pass == 0 ? a : b, ip = pass == 0 ? b : a;
+        const double* rj = &r1[(size_t)(f->ifirst + jp) * D];
+        const double* ri = &r2[(size_t)(f->ifirst + ip) * D];
+        double grad[3];
+        double w = inv[jp * n + ip];
+        if (f->model == PQ_NODES_SHO) {
+          double ri2 = dot3(ri, ri, D), rj2 = dot3(rj, rj, D), rirj = dot3(ri, rj, D);
+          double e = exp(-f->c * ((ri2 + rj2) * f->coshwt - 2 * rirj));
+          for (int d = 0; d < D; ++d) {
+            double g = pass == 0 ? 2 * f->c * (ri[d] - f->coshwt * rj[d]) * e
+                                 : 2 * f->c * (rj[d] - f->coshwt * ri[d]) * e;
+            lg[d] += w * f->scale * g;
+          }
+        } else {
+          double delta[3], value[3];
+          for (int d = 0; d < D; ++d) delta[d] = pass == 0 ? rj[d] - ri[d] : ri[d] - rj[d];
+          pbc(&ch->cell, delta);
+          for (int d = 0; d < D; ++d) {
+            double gr;
+            value[d] = pg_eval3(&f->pg[d], delta[d], &gr, 0);
+            grad[d] = gr / (value[d] + 1e-300);
+          }
+          for (int k = 0; k < D; ++k) for (int d = 0; d < D; ++d) grad[d] *= value[k];
+          for (int d = 0; d < D; ++d) lg[d] += w * grad[d] * f->scale;
+        }
+      }
+      double dist = sqrt(2 * f->mass / ((dot3(lg, lg, D) + 1e-15) * ch->tau));
+      if (pass == 0) d1[a] = dist; else d2[a] = dist;
+    }
+  }
+}
+
+int oq_node_evaluate(oq_chain* ch, int inode, const double* r1, const double* r2,
+                     int islice, int scaleMag, double* det) {
+  return node_evaluate(ch, &ch->fn[inode], r1, r2, islice, scaleMag, det);
+}
+void oq_node_distance(oq_chain* ch, int inode, const double* r1, const double* r2,
+                      int islice, double* d1, double* d2) {
+  node_distance(ch, &ch->fn[inode], r1, r2, islice, d1, d2);
+}
+double oq_node_scale(const oq_chain* ch, int inode) { return ch->fn[inode].scale; }
+
+/* FixedNodeAction::initialize (src/fixednode/FixedNodeAction.cc:331-357) */
+void oq_fn_initialize(oq_chain* ch, const double* sec1, const double* sec2) {
+  const size_t slice = (size_t)ch->N * ch->D;
+  if (sec1 != ch->sec1) memcpy(ch->sec1, sec1, sizeof(double) * ch->S * slice);
+  if (sec2 != ch->sec2) memcpy(ch->sec2, sec2, sizeof(double) * ch->S * slice);
+  for (int a = 0; a < ch->nfn; ++a) {
+    fn_t* f = &ch->fn[a];
+    const int ns = f->ns;
+    for (int t = 0; t < ch->S; ++t) {
+      const double* r1 = &ch->sec1[t * slice];
+      const double* r2 = &ch->sec2[t * slice];
+      double det;
+      int err = node_evaluate(ch, f, r1, r2, t, 1, &det);
+      f->dm[t] = det;
+      if (err || f->dm[t] * f->dm[0] <= 1e-200) { f->nerror++; ch->st.node_errors++; }
+      if (f->nodal) node_distance(ch, f, r1, r2, t, &f->dist[(t * 2 + 0) * ns], &f->dist[(t * 2 + 1) * ns]);
+    }
+    f->newdm[0] = f->dm[0];
+    for (int k = 0; k < 2 * ns; ++k) f->newdist[k] = f->dist[k];
+  }
+}
+
+/* FixedNodeAction::getActionDifference (src/fixednode/FixedNodeAction.cc:39-131),
+   summed over FixedNodeActions as CompositeDoubleAction. */
+double oq_fn_action_difference(oq_chain* ch, const double* mov, const int* idx,
+                               int nmov, int level) {
+  const int D = ch->D, N = ch->N;
+  const size_t slice = (size_t)N * D;
+  double total = 0;
+  double r1[MAXN * 3];
+  for (int a = 0; a < ch->nfn; ++a) {
+    fn_t* f = &ch->fn[a];
+    const int ns = f->ns;
+    double dA = 0;
+    int mine = 0;
+    for (int m = 0; m < nmov; ++m)
+      if (idx[m] >= f->ifirst && idx[m] < f->ifirst + ns) { mine = 1; break; }
+    if (!mine) { f->not_my_species = 1; total += 0; continue; }
+    f->not_my_species = 0;
+    const int nStride = 1 << (level + 1);
+    int crossed = 0;
+    for (int t = nStride / 2; t < ch->S; t += nStride) {
+      memcpy(r1, &ch->sec1[t * slice], sizeof(double) * slice);
+      for (int m = 0; m < nmov; ++m)
+        for (int d = 0; d < D; ++d) r1[idx[m] * D + d] = mov[((size_t)t * nmov + m) * D + d];
+      double det;
+      int err = node_evaluate(ch, f, r1, &ch->sec2[t * slice], t, 0, &det);
+      if (err) { crossed = 1; break; }
+      f->newdm[t] = det;
+      if (f->newdm[t] * f->dm[0] <= 1e-200) { crossed = 1; break; }
+    }
+    if (crossed) { dA = 2e100; total += dA; continue; }
+    if (f->nodal && level == 0) {
+      for (int t = 1; t < ch->S; ++t) {
+        memcpy(r1, &ch->sec1[t * slice], sizeof(double) * slice);
+        for (int m = 0; m < nmov; ++m)
+          for (int d = 0; d < D; ++d) r1[idx[m] * D + d] = mov[((size_t)t * nmov + m) * D + d];
+        node_distance(ch, f, r1, &ch->sec2[t * slice], t, &f->newdist[(t * 2 + 0) * ns],
+                      &f->newdist[(t * 2 + 1) * ns]);
+        for (int i = 0; i < ns; ++i) {
+          const double* od = f->dist; const double* nd = f->newdist;
+          dA += log((1 - exp(-od[(t * 2 + 0) * ns + i] * od[((t - 1) * 2 + 0) * ns + i]))
+                    / (1 - exp(-nd[(t * 2 + 0) * ns + i] * nd[((t - 1) * 2 + 0) * ns + i])));
+          dA += log((1 - exp(-od[(t * 2 + 1) * ns + i] * od[((t - 1) * 2 + 1) * ns + i]))
+                    / (1 - exp(-nd[(t * 2 + 1) * ns + i] * nd[((t - 1) * 2 + 1) * ns + i])));
+        }
+      }
+    }
+    total += dA;
+  }
+  return total;
+}
+
+/* FixedNodeAction::acceptLastMove (src/fixednode/FixedNodeAction.cc:359-367) */
+void oq_fn_accept(oq_chain* ch) {
+  for (int a = 0; a < ch->nfn; ++a) {
+    fn_t* f = &ch->fn[a];
+    if (f->not_my_species) continue;
+    memcpy(f->dm, f->newdm, sizeof(double) * ch->S);
+    memcpy(f->dist, f->newdist, sizeof(double) * ch->S * 2 * f->ns);
+  }
+}
+
+/* ------------------------------------------------------------- movers */
+
+/* FreeMover::makeMove (src/advancer/mover/FreeMover.cc:59-124) */
+static double free_mover(oq_chain* ch, const double* sec, double* mov, const int* idx,
+                         int nmov, int level) {
+  const int s = 1 << level, D = ch->D;
+  double g[PQ_MAX_MOVING * 3];
+  double lnT = 0;
+  for (int t = s; t < ch->S - s; t += 2 * s) {
+    gauss_rand(&ch->rng, g, nmov * D);
+    for (int m = 0; m < nmov; ++m) {
+      int i = idx[m], sp = ch->spec[i];
+      double sigma = 1.0 * sqrt(ch->lambda[i] * ch->tau * s);
+      double inv2Sigma2 = 0.5 / (sigma * sigma);
+      double mid[3], delta[3];
+      for (int d = 0; d < D; ++d) mid[d] = MOV(mov, t + s, m, d) - MOV(mov, t - s, m, d);
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += MOV(mov, t - s, m, d); }
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) delta[d] = g[m * D + d] * sigma;
+      pbc(&ch->cell, delta);
+      double nb[3];
+      for (int d = 0; d < D; ++d) nb[d] = mid[d] + delta[d];
+      pbc(&ch->cell, nb);
+      for (int d = 0; d < D; ++d) MOV(mov, t, m, d) = nb[d];
+      double temp = 1.0;
+      for (int d = 0; d < D; ++d) {
+        int k = pg_index(ch, level, sp, d);
+        if (ch->mpg_on[k]) temp *= pg_eval(&ch->mpg[k], delta[d]);
+        else lnT += delta[d] * delta[d] * inv2Sigma2;
+      }
+      temp = 1.0 / temp;
+      for (int d = 0; d < D; ++d) mid[d] = SEC(sec, t + s, i, d) - SEC(sec, t - s, i, d);
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += SEC(sec, t - s, i, d); }
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) delta[d] = SEC(sec, t, i, d) - mid[d];
+      pbc(&ch->cell, delta);
+      for (int d = 0; d < D; ++d) {
+        int k = pg_index(ch, level, sp, d);
+        if (ch->mpg_on[k]) temp *= pg_eval(&ch->mpg[k], delta[d]);
+        else lnT -= delta[d] * delta[d] * inv2Sigma2;
+      }
+      lnT += log(temp);
+    }
+  }
+  return lnT;
+}
+
+/* FreeMoverPBC::makeMove (src/advancer/mover/FreeMoverPBC.cc:59-162) */
+static double free_mover_pbc(oq_chain* ch, const double* sec, double* mov, const int* idx,
+                             int nmov, int level) {
+  const int s = 1 << level, D = ch->D;
+  double g[PQ_MAX_MOVING * 3];
+  double lnT = 0.;
+  for (int t = s; t < ch->S - s; t += 2 * s) {
+    gauss_rand(&ch->rng, g, nmov * D);
+    for (int m = 0; m < nmov; ++m) {
+      int i = idx[m], sp = ch->spec[i];
+      double sigma = 1.0 * sqrt(ch->lambda[i] * ch->tau * s);
+      double inv2Sigma2 = 0.5 / (sigma * sigma);
+      double mid[3], mid2[3];
+      for (int d = 0; d < D; ++d) mid[d] = MOV(mov, t + s, m, d) - MOV(mov, t - s, m, d);
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) mid[d] *= 0.5;
+      for (int d = 0; d < D; ++d) mid2[d] = mid[d] + 0.5 * ch->cell.a[d];
+      pbc(&ch->cell, mid2);
+      for (int d = 0; d < D; ++d) {
+        int k = pg_index(ch, level, sp, d);
+        double w1, w2;
+        if (ch->mpg_on[k]) { w1 = pg_eval(&ch->mpg[k], mid[d]); w2 = pg_eval(&ch->mpg[k], mid2[d]); }
+        else { w1 = exp(-inv2Sigma2 * mid[d] * mid[d]); w2 = exp(-inv2Sigma2 * mid2[d] * mid2[d]); }
+        if (rng_next(&ch->rng) > w1 / (w1 + w2)) mid[d] = mid2[d];
+      }
+      for (int d = 0; d < D; ++d) mid[d] += MOV(mov, t - s, m, d);
+      double nb[3];
+      for (int d = 0; d < D; ++d) nb[d] = mid[d] + g[m * D + d] * sigma;
+      pbc(&ch->cell, nb);
+      for (int d = 0; d < D; ++d) MOV(mov, t, m, d) = nb[d];
+      double dpn[3], dnn[3], dnew[3], dpo[3], dno[3], dold[3];
+      for (int d = 0; d < D; ++d) {
+        dpn[d] = MOV(mov, t, m, d) - MOV(mov, t - s, m, d);
+        dnn[d] = MOV(mov, t, m, d) - MOV(mov, t + s, m, d);
+        dnew[d] = MOV(mov, t + s, m, d) - MOV(mov, t - s, m, d);
+        dpo[d] = SEC(sec, t, i, d) - SEC(sec, t - s, i, d);
+        dno[d] = SEC(sec, t, i, d) - SEC(sec, t + s, i, d);
+        dold[d] = SEC(sec, t + s, i, d) - SEC(sec, t - s, i, d);
+      }
+      pbc(&ch->cell, dpn); pbc(&ch->cell, dnn); pbc(&ch->cell, dnew);
+      pbc(&ch->cell, dpo); pbc(&ch->cell, dno); pbc(&ch->cell, dold);
+      double temp = 1.;
+      for (int d = 0; d < D; ++d) {
+        int k1 = pg_index(ch, level + 1, sp, d), k2 = pg_index(ch, level + 2, sp, d);
+        if (level + 1 < ch->npg_levels && ch->mpg_on[k1]) {
+          const pg_t* p = &ch->mpg[k1];
+          temp *= pg_eval(p, dpo[d]); temp *= pg_eval(p, dno[d]);
+          temp /= pg_eval(p, dpn[d]); temp /= pg_eval(p, dnn[d]);
+        } else {
+          lnT -= dpo[d] * dpo[d] * inv2Sigma2 * 0.5;
+          lnT -= dno[d] * dno[d] * inv2Sigma2 * 0.5;
+          lnT += dpn[d] * dpn[d] * inv2Sigma2 * 0.5;
+          lnT += dnn[d] * dnn[d] * inv2Sigma2 * 0.5;
+        }
+        if (level + 2 < ch->npg_levels && ch->mpg_on[k2]) {
+          const pg_t* p = &ch->mpg[k2];
+          temp /= pg_eval(p, dold[d]); temp *= pg_eval(p, dnew[d]);
+        } else {
+          lnT += dold[d] * dold[d] * inv2Sigma2 * 0.25;
+          lnT -= dnew[d] * dnew[d] * inv2Sigma2 * 0.25;
+        }
+      }
+      lnT += log(temp);
+    }
+  }
+  return lnT;
+}
+
+double oq_mover(oq_chain* ch, const double* sec, double* mov, const int* idx,
+                int nmov, int level) {
+  if (ch->d.mover == PQ_MOVER_FREE) return free_mover(ch, sec, mov, idx, nmov, level);
+  return free_mover_pbc(ch, sec, mov, idx, nmov, level);
+}
+
+/* ------------------------------------------------------------ sampler */
+
+/* SerialPaths::getBeads / putBeads with the identity permutation
+   (src/base/SerialPaths.cc:119-185) */
+static void get_beads(oq_chain* ch, int t0, double* sec) {
+  const size_t slice = (size_t)ch->N * ch->D;
+  for (int k = 0; k < ch->S; ++k) {
+    int t = k + t0;
+    if (t < 0) t += ch->M;
+    else if (t >= ch->M) t -= ch->M;
+    memcpy(&sec[k * slice], &ch->beads[(size_t)t * slice], sizeof(double) * slice);
+  }
+}
+
+static void put_beads(oq_chain* ch, int t0, const double* sec) {
+  const size_t slice = (size_t)ch->N * ch->D;
+  for (int k = 0; k < ch->S; ++k) {
+    int t = k + t0;
+    if (t < 0) continue;
+    if (t >= ch->M) t %= ch->M;
+    memcpy(&ch->beads[(size_t)t * slice], &sec[k * slice], sizeof(double) * slice);
+  }
+}
+
+/* SpeciesParticleChooser::chooseParticles (src/advancer/SpeciesParticleChooser.cc:19-31) */
+static void choose_particles(oq_chain* ch, int sp, int nmov, int* idx) {
+  int npart = ch->d.species[sp].count, ifirst = ch->ifirst[sp];
+  for (int im = 0; im < nmov;) {
+    int i = (int)(npart * rng_next(&ch->rng) + ifirst);
+    if (i == npart + ifirst) i = npart + ifirst - 1;
+    for (int jm = 0; jm <= im; ++jm) {
+      if (jm == im) { idx[im++] = i; break; }
+      if (idx[jm] == i) break;
+    }
+  }
+}
+
+/* MultiLevelSampler::run/tryMove (src/advancer/MultiLevelSampler.cc:68-177) and
+   DoubleMLSampler::run/tryMove (src/advancer/DoubleMLSampler.cc:66-245) for a
+   PermutationChooser (noPermutation), nrepeat 1, no delayed rejection. */
+static int sample(oq_chain* ch, int sp, int nmov) {
+  const int D = ch->D, S = ch->S, L = ch->L;
+  int* idx = ch->idx;
+  choose_particles(ch, sp, nmov, idx);
+  double* mov = ch->mov;
+  for (int m = 0; m < nmov; ++m)
+    for (int d = 0; d < D; ++d) {
+      MOV(mov, 0, m, d) = SEC(ch->sec1, 0, idx[m], d);
+      MOV(mov, S - 1, m, d) = SEC(ch->sec1, S - 1, idx[m], d);
+    }
+  double old = 0.0;
+  for (int l = L; l >= 0; --l) {
+    ch->st.tries[l]++;
+    if (l < L) ch->st.bead_updates += (uint64_t)nmov << (L - 1 - l);
+    double lnT = oq_mover(ch, ch->sec1, mov, idx, nmov, l);
+    double dA = oq_action_difference(ch, ch->sec1, mov, idx, nmov, l, 7);
+    if (ch->nfn) dA += oq_fn_action_difference(ch, mov, idx, nmov, l);
+    double piRatio = -dA + old;
+    double acceptProb = exp(lnT + piRatio);
+    if (rng_next(&ch->rng) > acceptProb) return 0;
+    old = dA;
+    ch->st.accepts[l]++;
+  }
+  if (ch->nfn) oq_fn_accept(ch);
+  for (int t = 0; t < S; ++t)
+    for (int m = 0; m < nmov; ++m)
+      for (int d = 0; d < D; ++d) SEC(ch->sec1, t, idx[m], d) = MOV(mov, t, m, d);
+  ch->st.accepted_bead_updates += (uint64_t)nmov * ((1u << L) - 1);
+  return 1;
+}
+
+/* SectionChooser::run (src/advancer/SectionChooser.cc:30-48) and
+   DoubleSectionChooser::run (src/advancer/DoubleSectionChooser.cc:33-65) */
+static void run_section(oq_chain* ch) {
+  const int M = ch->M;
+  int t0;
+  if (ch->nfn) {
+    double x = rng_next(&ch->rng) * (1 - 1e-8);
+    int ilo = -1, ihi = M / 2 - 1;
+    t0 = ilo + (int)((ihi + 1 - ilo) * x);
+    if (t0 > ihi) t0 = ihi;
+    if (rng_next(&ch->rng) >= 0.5) t0 = (t0 + M / 2 + 1) % M - 1;
+    int t0b = (t0 + M / 2 + 1) % M - 1;
+    get_beads(ch, t0, ch->sec1);
+    get_beads(ch, t0b, ch->sec2);
+    oq_fn_initialize(ch, ch->sec1, ch->sec2);
+  } else {
+    double x = rng_next(&ch->rng) * (1 - 1e-8);
+    int ilo = -1, ihi = M - 1;
+    t0 = ilo + (int)((ihi + 1 - ilo) * x);
+    if (t0 > ihi) t0 = ihi;
+    get_beads(ch, t0, ch->sec1);
+  }
+  for (int s = 0; s < ch->d.nsample; ++s) sample(ch, ch->d.sample[s].species, ch->d.sample[s].nmoving);
+  put_beads(ch, t0, ch->sec1);
+}
+
+void oq_run_sweeps(oq_chain* ch, int nsweep) {
+  for (int k = 0; k < nsweep; ++k)
+    for (int s = 0; s < ch->d.nsection; ++s) run_section(ch);
+  ch->st.draws = ch->rng.count;
+}
+
+void oq_get_stats(const oq_chain* ch, pq_chain_stats* st) {
+  *st = ch->st;
+  st->draws = ch->rng.count;
+}
+
+/* ThermoEnergyEstimator over SerialPaths::sumOverLinks
+   (src/estimator/ThermoEnergyEstimator.cc:32-54, src/base/SerialPaths.cc:48-62,
+   src/stats/SimpleScalarAccumulator.cpp:25-38).  FixedNodeAction::getBeadAction
+   adds utau = 0 when useDistDerivative is off (FixedNodeAction.cc:306-313). */
+double oq_thermo_energy(const oq_chain* ch) {
+  double value = 0;
+  for (int t = 0; t < ch->M; ++t)
+    for (int i = 0; i < ch->N; ++i) value += oq_bead_utau(ch, i, t, 7);
+  return value / ch->M;
+}
+
+/* PairCFEstimator<1>::handleLink with PairRadial, two species
+   (src/estimator/PairCFEstimator.h:72-93, src/util/PairDistance.h:22-39) */
+void oq_pair_histogram(const oq_chain* ch, int s1, int s2, double rmin,
+                       double rmax, int nbin, uint64_t* counts) {
+  const int D = ch->D;
+  double deltaInv = nbin / (rmax - rmin);
+  int if1 = ch->ifirst[s1], n1 = ch->d.species[s1].count;
+  int if2 = ch->ifirst[s2], n2 = ch->d.species[s2].count;
+  for (int t = 0; t < ch->M; ++t)
+    for (int i = if1; i < if1 + n1; ++i)
+      for (int j = if2; j < if2 + n2; ++j) {
+        if (i == j) continue;
+        const double *a = bead(ch, i, t), *b = bead(ch, j, t);
+        double delta[3];
+        for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
+        pbc(&ch->cell, delta);
+        double r2 = 0;
+        for (int d = 0; d < D; ++d) r2 += delta[d] * delta[d] * 1.0;
+        double dd = sqrt(r2);
+        int ibin = (int)((dd - rmin) * deltaInv);
+        if (dd < rmin || ibin > nbin - 1) continue;
+        counts[ibin]++;
+      }
+}
+
+uint64_t oq_run_batch(const pq_chain_desc* descs, int nchain, double* beads,
+                      int nsweep, uint64_t seed, uint32_t first_stream,
+                      int nthreads, double* energy) {
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (nchain + 1));
+  off[0] = 0;
+  for (int c = 0; c < nchain; ++c) {
+    int n = 0;
+    for (int s = 0; s < descs[c].nspecies; ++s) n += descs[c].species[s].count;
+    off[c + 1] = off[c] + (int64_t)descs[c].nslice * n * descs[c].ndim;
+  }
+  uint64_t total = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+#endif
+  for (int c = 0; c < nchain; ++c) {
+    oq_chain* ch = oq_create(&descs[c]);
+    oq_set_beads(ch, beads + off[c]);
+    oq_rng_philox(ch, seed, first_stream + (uint32_t)c);
+    oq_run_sweeps(ch, nsweep);
+    total += ch->st.bead_updates;
+    if (energy) energy[c] = oq_thermo_energy(ch);
+    oq_get_beads(ch, beads + off[c]);
+    oq_destroy(ch);
+  }
+  free(off);
+  return total;
+}
+
+/* Loop/Measure/Collect: nblock blocks of nmeasure (sweep, Measure) pairs; the
+   block value is the mean of its measurements (SimpleScalarAccumulator). */
+void oq_energy_blocks(oq_chain* ch, int nblock, int nmeasure, double* block) {
+  for (int b = 0; b < nblock; ++b) {
+    double sum = 0;
+    for (int k = 0; k < nmeasure; ++k) {
+      oq_run_sweeps(ch, 1);
+      sum += oq_thermo_energy(ch);
+    }
+    block[b] = sum / nmeasure;
+  }
+}

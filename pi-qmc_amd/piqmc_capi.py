@@ -1,0 +1,323 @@
+"""ctypes view of the C ABI in include/piqmc_hip.h and the workload builders.
+
+The structures mirror the header field for field.  ``load_library`` opens the
+in-tree HIP library (pi-qmc_amd/libpiqmc_hip.so) and raises if it is missing:
+there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpiqmc_hip.so")
+
+PQ_MAX_SPECIES = 4
+PQ_MAX_SHO = 4
+PQ_MAX_NODES = 4
+PQ_MAX_SAMPLES = 8
+PQ_MAX_MOVING = 4
+PQ_MAX_NODE_PART = 32
+PQ_MOVER_FREE = 0
+PQ_MOVER_FREE_PBC = 1
+PQ_NODES_FREE = 0
+PQ_NODES_SHO = 1
+PQ_PREC_FP64 = 0
+PQ_PREC_MIXED = 1
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [("count", C.c_int), ("mass", C.c_double), ("charge", C.c_double)]
+
+
+class ShoDesc(C.Structure):
+    _fields_ = [("species", C.c_int), ("omega", C.c_double),
+                ("center", C.c_double * 3), ("ndim", C.c_int)]
+
+
+class CoulombDesc(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("epsilon", C.c_double), ("norder", C.c_int),
+                ("rmin", C.c_double), ("rmax", C.c_double), ("ngpts", C.c_int)]
+
+
+class NodeDesc(C.Structure):
+    _fields_ = [("species", C.c_int), ("model", C.c_int), ("omega", C.c_double),
+                ("temperature", C.c_double), ("nodal_action", C.c_int)]
+
+
+class SampleDesc(C.Structure):
+    _fields_ = [("species", C.c_int), ("nmoving", C.c_int)]
+
+
+class ChainDesc(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("nslice", C.c_int), ("nlevel", C.c_int),
+                ("nspecies", C.c_int), ("tau", C.c_double), ("cell", C.c_double * 3),
+                ("species", SpeciesDesc * PQ_MAX_SPECIES), ("spring", C.c_int),
+                ("nsho", C.c_int), ("sho", ShoDesc * PQ_MAX_SHO),
+                ("coulomb", CoulombDesc), ("nnodes", C.c_int),
+                ("nodes", NodeDesc * PQ_MAX_NODES), ("mover", C.c_int),
+                ("nsample", C.c_int), ("sample", SampleDesc * PQ_MAX_SAMPLES),
+                ("nsection", C.c_int)]
+
+    @property
+    def npart(self):
+        return sum(self.species[s].count for s in range(self.nspecies))
+
+    def nbeads(self):
+        return self.nslice * self.npart * self.ndim
+
+
+class ChainStats(C.Structure):
+    _fields_ = [("draws", C.c_uint64), ("bead_updates", C.c_uint64),
+                ("accepted_bead_updates", C.c_uint64), ("tries", C.c_uint64 * 8),
+                ("accepts", C.c_uint64 * 8), ("node_errors", C.c_uint64)]
+
+
+def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
+              sho=(), coulomb=None, nodes=(), mover=PQ_MOVER_FREE, samples=None,
+              nsection=None):
+    """species: [(count, mass, charge)]; sho: [(species, omega)];
+    coulomb: dict(epsilon, norder, rmin, rmax, ngpts) or None;
+    nodes: [(species, model, omega)]; samples default one npart=1 per species;
+    nsection default M / 2^L (one sweep touches every slice once on average)."""
+    d = ChainDesc()
+    d.ndim, d.nslice, d.nlevel, d.nspecies, d.tau = ndim, nslice, nlevel, len(species), tau
+    cells = cell if isinstance(cell, (list, tuple)) else [cell] * 3
+    for k in range(3):
+        d.cell[k] = cells[k]
+    for s, (n, m, q) in enumerate(species):
+        d.species[s].count, d.species[s].mass, d.species[s].charge = n, m, q
+    d.spring = 1 if spring else 0
+    d.nsho = len(sho)
+    for k, (s, w) in enumerate(sho):
+        d.sho[k].species, d.sho[k].omega, d.sho[k].ndim = s, w, 0
+    if coulomb:
+        d.coulomb.enabled = 1
+        d.coulomb.epsilon = coulomb.get("epsilon", 1.0)
+        d.coulomb.norder = coulomb.get("norder", 3)
+        d.coulomb.rmin = coulomb.get("rmin", 0.0)
+        d.coulomb.rmax = coulomb.get("rmax", 0.0)
+        d.coulomb.ngpts = coulomb.get("ngpts", 0)
+    d.nnodes = len(nodes)
+    for k, (s, model, w) in enumerate(nodes):
+        d.nodes[k].species, d.nodes[k].model, d.nodes[k].omega = s, model, w
+        d.nodes[k].temperature, d.nodes[k].nodal_action = 0.0, 1
+    d.mover = mover
+    if samples is None:
+        samples = [(s, 1) for s in range(len(species))]
+    d.nsample = len(samples)
+    for k, (s, nm) in enumerate(samples):
+        d.sample[k].species, d.sample[k].nmoving = s, nm
+    d.nsection = nsection if nsection is not None else max(1, nslice >> nlevel)
+    return d
+
+
+# ----------------------------------------------------------- workloads
+
+def cfg1_desc():
+    """2 e- (up, dn) in a 3D harmonic dot, omega=1, Coulomb 1/r, beta=10, M=200, L=3."""
+    return make_desc(3, 200, 3, 0.05, [(1, 1.0, -1.0), (1, 1.0, -1.0)],
+                     sho=[(0, 1.0), (1, 1.0)],
+                     coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024))
+
+
+def cfg2_desc():
+    """3+3 e- in a 2D dot, omega=1, Coulomb eps=1, SHONodes, beta=8, M=400, L=4."""
+    return make_desc(2, 400, 4, 0.02, [(3, 1.0, -1.0), (3, 1.0, -1.0)],
+                     sho=[(0, 1.0), (1, 1.0)],
+                     coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024),
+                     nodes=[(0, PQ_NODES_SHO, 1.0), (1, PQ_NODES_SHO, 1.0)])
+
+
+def cfg3_descs(nchain=1024, seed=0x5EED):
+    """Uneven batch of 2D dots: N~U{2..20}, M in {128..1024}, omega~U[0.5,2],
+    SHONodes on even chains, FreeParticleNodes (box 20) on odd chains."""
+    out = []
+    for c in range(nchain):
+        rng = np.random.default_rng(seed + c)
+        n = int(rng.integers(2, 21))
+        up, dn = (n + 1) // 2, n // 2
+        m = int(rng.choice([128, 256, 512, 1024]))
+        w = float(rng.uniform(0.5, 2.0))
+        sp = [(up, 1.0, -1.0)] + ([(dn, 1.0, -1.0)] if dn else [])
+        model = PQ_NODES_SHO if c % 2 == 0 else PQ_NODES_FREE
+        nodes = [(s, model, w) for s in range(len(sp))]
+        out.append(make_desc(2, m, 4, 0.02, sp, cell=20.0 if model == PQ_NODES_FREE else 1000.0,
+                             sho=[(s, w) for s in range(len(sp))],
+                             coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024),
+                             nodes=nodes))
+    return out
+
+
+def cfg4_desc():
+    """32 e (m=1) + 32 h (m=2), spin-polarised, cubic box L=10, min-image Coulomb,
+    FreeParticleNodes 32x32 per species, beta=20, M=1024, L=5, FreePBC mover."""
+    return make_desc(3, 1024, 5, 20.0 / 1024, [(32, 1.0, -1.0), (32, 2.0, 1.0)], cell=10.0,
+                     coulomb=dict(epsilon=1.0, norder=3, rmin=0.0, rmax=float(np.sqrt(3) * 10),
+                                  ngpts=1024),
+                     nodes=[(0, PQ_NODES_FREE, 0.0), (1, PQ_NODES_FREE, 0.0)],
+                     mover=PQ_MOVER_FREE_PBC)
+
+
+def cfg5_descs(nchain=4096, seed=0xC0FFEE):
+    """Large uneven batch of 3D dots: N~U{2..64} in two species, M=16*U{16..128}, SHONodes."""
+    out = []
+    for c in range(nchain):
+        rng = np.random.default_rng(seed + c)
+        n = int(rng.integers(2, 65))
+        up, dn = (n + 1) // 2, n // 2
+        m = 16 * int(rng.integers(16, 129))
+        sp = [(up, 1.0, -1.0), (dn, 1.0, -1.0)]
+        out.append(make_desc(3, m, 4, 0.05, sp, sho=[(0, 1.0), (1, 1.0)],
+                             coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024),
+                             nodes=[(0, PQ_NODES_SHO, 1.0), (1, PQ_NODES_SHO, 1.0)]))
+    return out
+
+
+def initial_beads(desc, seed, replicate=None):
+    """Seeded N(0,1) beads; replicated across slices when the chain has node
+    actions (R(t) = R(t+beta/2) makes every determinant positive)."""
+    rng = np.random.default_rng(seed)
+    n, d, m = desc.npart, desc.ndim, desc.nslice
+    if replicate is None:
+        replicate = desc.nnodes > 0
+    if replicate:
+        if desc.mover == PQ_MOVER_FREE_PBC and desc.cell[0] < 100:
+            r = rng.uniform(-0.5, 0.5, size=(n, d)) * np.array(desc.cell[:d])
+        else:
+            r = rng.standard_normal((n, d))
+        b = np.broadcast_to(r, (m, n, d)).copy()
+    else:
+        b = rng.standard_normal((m, n, d))
+    return np.ascontiguousarray(b, dtype=np.float64)
+
+
+def descs_array(descs):
+    arr = (ChainDesc * len(descs))()
+    for k, d in enumerate(descs):
+        arr[k] = d
+    return arr
+
+
+# ----------------------------------------------------------- library
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Open the HIP library; raises OSError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    dp, ip = P(C.c_double), P(C.c_int)
+    sig = {
+        "pq_last_error": (C.c_char_p, []),
+        "pq_batch_create": (C.c_int, [P(ChainDesc), C.c_int, C.c_int, C.c_int, P(C.c_void_p)]),
+        "pq_batch_destroy": (None, [C.c_void_p]),
+        "pq_batch_info": (C.c_int, [C.c_void_p, C.c_int, ip, ip, ip, P(C.c_int64)]),
+        "pq_set_paths": (C.c_int, [C.c_void_p, C.c_int, dp]),
+        "pq_get_paths": (C.c_int, [C.c_void_p, C.c_int, dp]),
+        "pq_set_all_paths": (C.c_int, [C.c_void_p, dp, C.c_int64]),
+        "pq_get_all_paths": (C.c_int, [C.c_void_p, dp, C.c_int64]),
+        "pq_seed": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32]),
+        "pq_set_uniforms": (C.c_int, [C.c_void_p, dp, C.c_int64]),
+        "pq_run_sweeps": (C.c_int, [C.c_void_p, C.c_int]),
+        "pq_sync": (C.c_int, [C.c_void_p]),
+        "pq_get_stats": (C.c_int, [C.c_void_p, C.c_int, P(ChainStats)]),
+        "pq_reset_stats": (C.c_int, [C.c_void_p]),
+        "pq_get_node_scale": (C.c_int, [C.c_void_p, C.c_int, dp]),
+        "pq_measure_energy": (C.c_int, [C.c_void_p, dp]),
+        "pq_pair_histogram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, P(C.c_uint64)]),
+        "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
+        "pq_bead_utau": (C.c_int, [C.c_void_p, C.c_int, ip, ip, ip, dp]),
+        "pq_section_begin": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
+        "pq_double_action_difference": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, ip, C.c_int, dp]),
+        "pq_accept_last_move": (C.c_int, [C.c_void_p, C.c_int]),
+        "pq_node_evaluate": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, dp, dp, dp, ip, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class PQError(RuntimeError):
+    pass
+
+
+def check(rc, lib=None):
+    if rc != 0:
+        lib = lib or load_library()
+        msg = lib.pq_last_error()
+        raise PQError(f"pq call failed ({rc}): {msg.decode() if msg else ''}")
+
+
+class Batch:
+    """Owning handle of a pq_batch (device state of many chains)."""
+
+    def __init__(self, descs, precision=PQ_PREC_FP64, device=0):
+        self.lib = load_library()
+        self.descs = list(descs)
+        self._arr = descs_array(self.descs)
+        h = C.c_void_p()
+        check(self.lib.pq_batch_create(self._arr, len(self.descs), precision, device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.pq_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_paths(self, c, beads):
+        b = np.ascontiguousarray(beads, dtype=np.float64)
+        check(self.lib.pq_set_paths(self.h, c, dptr(b)))
+
+    def get_paths(self, c):
+        d = self.descs[c]
+        b = np.empty((d.nslice, d.npart, d.ndim))
+        check(self.lib.pq_get_paths(self.h, c, dptr(b)))
+        return b
+
+    def seed(self, seed, first_stream=0):
+        check(self.lib.pq_seed(self.h, seed, first_stream))
+
+    def set_uniforms(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        check(self.lib.pq_set_uniforms(self.h, dptr(u), u.shape[-1]))
+
+    def run_sweeps(self, n):
+        check(self.lib.pq_run_sweeps(self.h, n))
+
+    def sync(self):
+        check(self.lib.pq_sync(self.h))
+
+    def stats(self, c):
+        st = ChainStats()
+        check(self.lib.pq_get_stats(self.h, c, C.byref(st)))
+        return st
+
+    def energy(self):
+        e = np.empty(len(self.descs))
+        check(self.lib.pq_measure_energy(self.h, dptr(e)))
+        return e
