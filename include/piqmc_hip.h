@@ -165,8 +165,12 @@ int pq_set_uniforms(pq_batch* batch, const double* u, int64_t per_chain);
    x (choose section, initialize node caches, run each Sample, put beads back).
    One fused kernel launch per call; asynchronous on the batch stream. */
 int pq_run_sweeps(pq_batch* batch, int nsweep);
+/* Same launch bracketed by HIP events on the batch stream; waits for it and
+   returns the kernel's elapsed time in milliseconds. */
+int pq_run_sweeps_timed(pq_batch* batch, int nsweep, float* ms);
 int pq_sync(pq_batch* batch);
 int pq_get_stats(pq_batch* batch, int chain, pq_chain_stats* out);
+int pq_get_all_stats(pq_batch* batch, pq_chain_stats* out);
 int pq_reset_stats(pq_batch* batch);
 /* Node-model magnitude scale of each FixedNodeAction of a chain
    (SHONodes.cc:113-122 / FreeParticleNodes.cc:142-151 state). */
@@ -208,6 +212,13 @@ int pq_accept_last_move(pq_batch* batch, int chain);
 int pq_node_evaluate(pq_batch* batch, int chain, int inode, int nmat,
                      const double* r1, const double* r2, double* det,
                      int* err, double* d1, double* d2);
+
+/* Tabulated pair action ipair of a chain (CoulombAction -> PairAction grid,
+   src/action/CoulombAction.cc:45-90, src/action/PairAction.cc:161-185):
+   meta = {species1, species2, norder, ngpts}, grid = {1/rmin, (ngpts-1)/ln(rmax/rmin)},
+   u and ut [ngpts][norder+1]. */
+int pq_get_pair_table(pq_batch* batch, int chain, int ipair, int* meta,
+                      double* grid, double* u, double* ut);
 
 #ifdef __cplusplus
 }

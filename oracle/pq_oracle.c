@@ -99,6 +99,11 @@ static void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
   }
 }
 
+void oq_philox_raw(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  for (int i = 0; i < 4; ++i) out[i] = ctr[i];
+  philox4x32_10(out, key[0], key[1]);
+}
+
 uint32_t oq_philox_u32(uint64_t seed, uint32_t stream, uint64_t k) {
   uint32_t c[4] = {(uint32_t)(k >> 2), (uint32_t)(k >> 34), stream, 0u};
   philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));

@@ -19,6 +19,7 @@ typedef struct oq_chain oq_chain;
 
 /* uniform streams */
 void oq_glibc_sequence(unsigned int seed, int n, int32_t* out);
+void oq_philox_raw(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 uint32_t oq_philox_u32(uint64_t seed, uint32_t stream, uint64_t k);
 double oq_philox_uniform(uint64_t seed, uint32_t stream, uint64_t k);
 

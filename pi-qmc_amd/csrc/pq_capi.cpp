@@ -1,0 +1,892 @@
+// pq_capi.cpp -- the C ABI of include/piqmc_hip.h: chain setup, Coulomb pair
+// tables, device memory, and the launches of the gfx950 kernels.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/piqmc_hip.h"
+#include "pq_launch.h"
+#include "pq_params.h"
+
+using namespace pq;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(PQ_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                          \
+  } while (0)
+
+extern "C" const char* pq_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------- Coulomb tables
+// Coulomb1DLinkAction (src/action/coulomb/Coulomb1DLinkAction.cpp:4-133) and
+// Coulomb3DLinkAction (src/action/coulomb/Coulomb3DLinkAction.cpp:13-90):
+// the published rational fits of the 1D Coulomb link action and the 3D
+// orders built from them.  The coefficients are data and must match the
+// reference exactly.
+namespace {
+struct Fit1D {
+  double s, m1, m2, m3, m4, m5;
+  explicit Fit1D(double stau) : s(stau) {
+    m1 = 1.0 / stau; m2 = m1 * m1; m3 = m2 * m1; m4 = m3 * m1; m5 = m4 * m1;
+  }
+  double origin() const {
+    return s * (1.772453851 + s * (-0.074137740 + s * (0.005834805 + s * (-0.000382686
+           + s * (0.000008738 + s * 0.000002138)))));
+  }
+  double u0(double uo, double r) const {
+    const double a = 0.25300593 * m1 + 0.01432126, b = 0.07936898 * m2 - 0.01634421 / s;
+    const double c = 0.07263383 * m3, d = 0.00940013 * m4, e = 0.03160181 * m5;
+    const double f = 0.18976335 * m1, g = 0.00343053 * m2;
+    double v = uo + r * ((uo * a - 1.) + r * (f + r * (g + r * e * s * s)));
+    return v / (1. + r * (a + r * (b + r * (c + r * (d + r * e)))));
+  }
+  double u1(double r) const {
+    const double a = 0.07776906 * m1, b = -0.04695226 * m2, c = 0.01291994 * m3;
+    const double d = 0.60942687 * m1, e = -1.09588064 * m2, f = 1.15591565 * m3;
+    const double g = -0.57485278 * m4, h = 0.15212765 * m5;
+    return r * r * (a + r * (b + r * c)) / (1 + r * (d + r * (e + r * (f + r * (g + r * h)))));
+  }
+  double u2(double r) const {
+    const double a = 9.52130652e-04 * m3, b = -6.39946071e-04 * m4, c = 1.14359558e-04 * m5;
+    const double d = -7.59575215e-01 * m3, e = 5.53798980e-01 * m4, f = -7.15626226e-02 * m5;
+    const double g = -3.49081618e-02 * m5 * m1, h = 8.40084881e-03 * m5 * m2;
+    return std::pow(r, 4) * (a + r * (b + r * c))
+           / (1 + std::pow(r, 3) * (d + r * (e + r * (f + r * (g + r * h)))));
+  }
+  double u3(double r) const {
+    const double a = -2.06984256e-1 * m5, b = -9.51423947e-3 * m5 * m1, c = 8.97532561e-2 * m5 * m2;
+    const double d = 6.10876797e+4 * m3, e = -1.19016292e+5 * m4, f = 9.29836599e+4 * m5;
+    const double g = -3.49919130e+4 * m5 * m1, h = 6.42490539e+3 * m5 * m2;
+    const double i = -6.02729064e+2 * m5 * m3, j = 6.01285550e+1 * m5 * m4;
+    return std::pow(r, 6) * (a + r * (b + r * c))
+           / (1 + std::pow(r, 3) * (d + r * (e + r * (f + r * (g + r * (h + r * (i + r * j)))))));
+  }
+  double u4(double r) const {
+    const double a = -2.84102296e-6 * m5, b = 6.26961672e-7 * m5 * m2, c = -2.20166473e-1 * m3;
+    const double d = 1.07981903e-1 * m5, e = -1.94671111e-2 * m5 * m2, f = 1.56217930e-3 * m5 * m4;
+    return std::pow(r, 6) * (a + b * r * r)
+           / (1 + r * r * r * (c + r * r * (d + r * r * (e + r * r * f))));
+  }
+};
+
+double coulomb3d(const Fit1D& F, int order, double r) {
+  const double s = F.s;
+  switch (order) {
+    case 0: {
+      const double a = 0.5 * r * r / (s * s), b = std::exp(-2. * a);
+      const double c = 1.0 / (1.0 + 2 * (1.0 - b) * F.u1(r) / (a + 1e-200));
+      return F.u0(F.origin(), r) + std::log(c);
+    }
+    case 1: {
+      const double w1 = F.u1(r), w2 = F.u2(r);
+      const double a = 0.5 * r * r / (s * s), b = std::exp(-2. * a);
+      const double c = 1.0 / (1.0 + 2 * (1.0 - b) * w1 / (a + 1e-200));
+      return w1 + c * (b * w1 - 4 * (1 - b) * w2 / (a + 1e-200));
+    }
+    case 2: {
+      const double w1 = F.u1(r), w2 = F.u2(r), w3 = F.u3(r);
+      const double a = 2. * s * s / (r * r + 1e-50), b = std::exp(-2. / (a + 1e-200));
+      const double c = 1. / (1 + 2 * a * (1 - b) * w1);
+      return w2 + 0.25 * c / (a + 1e-200)
+             * (c * (b * w1 * (1 + 2 * a * w1) + 8 * a * b * w2 + 32 * a * a * a * (1 - b) * (1 - b) * w2 * w2)
+                - 24 * a * a * (1 - b) * w3);
+    }
+    case 3: {
+      const double u1 = F.u1(r), u2 = F.u2(r), u3 = F.u3(r), u4 = F.u4(r);
+      const double a = 2. * s * s / (r * r + 1e-50), b = std::exp(-2. / (a + 1e-200));
+      const double c = 1. / (1 + 2 * a * (1 - b) * u1);
+      const double A = 1 + 2 * a * u1;
+      return u3 + c * c * c / (24. * a * a + 1e-200)
+        * (128 * a * a * a * a * a * b * b * b * (4 * u2 * u2 * u2 - 9 * u1 * u2 * u3 + 6 * u1 * u1 * u4)
+           + 64 * a * a * a * (a * u2 * (-8 * a * u2 * u2 + 9 * A * u3) - 3 * A * A * u4)
+           + 2 * a * b * b * (2 * a * u1 * u1 * u1
+                              + 96 * a * a * u2 * (u2 - 8 * a * a * u2 * u2 + 3 * a * u3)
+                              + 12 * a * u1 * (u2 - 6 * a * u3 + 144 * a * a * a * u2 * u3 - 32 * a * a * u4)
+                              + u1 * u1 * (1 - 1152 * a * a * a * a * u4))
+           + b * (4 * a * a * u1 * u1 * u1
+                  + 12 * a * (u2 - 16 * a * a * u2 * u2 + 128 * a * a * a * a * u2 * u2 * u2 + 6 * a * u3
+                              - 96 * a * a * a * u2 * u3 + 16 * a * a * u4)
+                  + 4 * u1 * u1 * (a + 576 * a * a * a * a * a * u4)
+                  + u1 * (1 + 24 * a * a * (u2 + 6 * a * u3 - 144 * a * a * a * u2 * u3 + 64 * a * a * u4))));
+    }
+    default:
+      return F.u4(r);
+  }
+}
+
+// CoulombAction::u and ::utau (src/action/CoulombAction.cc:209-259)
+double coulomb_u(double r, int order, double mu, double q1q2, double tau) {
+  r = std::sqrt(r * r + 0.0);
+  const double reff = 2.0 * mu * q1q2 * r;
+  Fit1D F(q1q2 * std::sqrt(2.0 * mu * tau));
+  return coulomb3d(F, order, reff);
+}
+
+double coulomb_utau(double r, int order, double mu, double q1q2, double tau) {
+  const double eps = 1e-7;
+  const double up = coulomb_u(r, order, mu, q1q2, tau * (1.0 + eps));
+  const double um = coulomb_u(r, order, mu, q1q2, tau * (1.0 - eps));
+  return (up - um) / (2.0 * eps * tau);
+}
+
+int pg_nterms(double alpha, double length) { return 1 + (int)(2.0 * std::sqrt(alpha * length * length)); }
+}  // namespace
+
+// ------------------------------------------------------------- the batch
+
+struct pq_batch {
+  int device = 0;
+  int precision = 0;
+  int nchain = 0;
+  int D = 3;
+  int G = 4;
+  hipStream_t stream = nullptr;
+  std::vector<pq_chain_desc> descs;
+  std::vector<ChainP> hc;
+  std::vector<TableP> ht;
+  std::vector<double> hu, hut;
+  std::vector<int> order;
+  int64_t nbeads = 0;
+  SweepLayoutH lay{};
+  // device
+  ChainP* dchains = nullptr;
+  TableP* dtables = nullptr;
+  double *du = nullptr, *dut = nullptr;
+  float* duf = nullptr;
+  double* dbeads = nullptr;
+  double* dinv = nullptr;
+  double* dfn = nullptr;
+  double* dscale = nullptr;
+  uint64_t* dcounter = nullptr;
+  pq_chain_stats* dstats = nullptr;
+  double* dinj = nullptr;
+  int64_t inj_per_chain = 0;
+  int* dorder = nullptr;
+  int rng_mode = PQ_RNG_PHILOX;
+  uint64_t seed = 0;
+  uint32_t first_stream = 0;
+  // call-level scratch
+  double* dscr = nullptr;
+  size_t scr_bytes = 0;
+  double* dfnscr = nullptr;   // fixed-node protocol buffers, kept between calls
+  size_t fnscr_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static DevCtx make_ctx(const pq_batch* b) {
+  DevCtx x;
+  x.chains = b->dchains;
+  x.tables = b->dtables;
+  x.tab_u = b->du;
+  x.tab_ut = b->dut;
+  x.tab_uf = b->duf;
+  x.beads = b->dbeads;
+  x.inv = b->dinv;
+  x.fnstate = b->dfn;
+  x.scale = b->dscale;
+  x.counter = b->dcounter;
+  x.stats = b->dstats;
+  x.inj = b->dinj;
+  x.inj_per_chain = b->inj_per_chain;
+  x.rng_mode = b->rng_mode;
+  x.seed = b->seed;
+  x.first_stream = b->first_stream;
+  x.order = b->dorder;
+  return x;
+}
+
+static int scratch(pq_batch* b, size_t bytes, void** p) {
+  if (bytes > b->scr_bytes) {
+    if (b->dscr) hipFree(b->dscr);
+    b->dscr = nullptr;
+    b->scr_bytes = 0;
+    HIPCHK(hipMalloc(&b->dscr, bytes));
+    b->scr_bytes = bytes;
+  }
+  *p = b->dscr;
+  return PQ_OK;
+}
+
+static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
+                       std::map<std::tuple<double, double, double, double, double, int, int>, int>& tabs) {
+  std::memset(&c, 0, sizeof c);
+  if (d.ndim != 2 && d.ndim != 3) return fail(PQ_ERR_ARG, "ndim must be 2 or 3 (got %d)", d.ndim);
+  if (d.nspecies < 1 || d.nspecies > PQ_MAX_SPECIES) return fail(PQ_ERR_ARG, "nspecies out of range");
+  if (d.nlevel < 0 || d.nlevel > 6) return fail(PQ_ERR_ARG, "nlevel must be in 0..6");
+  if (d.tau <= 0) return fail(PQ_ERR_ARG, "tau must be positive");
+  c.D = d.ndim; c.M = d.nslice; c.L = d.nlevel; c.S = (1 << d.nlevel) + 1; c.nsp = d.nspecies;
+  c.tau = d.tau;
+  if (c.M < c.S) return fail(PQ_ERR_ARG, "nslice %d shorter than a section", c.M);
+  int n = 0;
+  for (int s = 0; s < c.nsp; ++s) {
+    if (d.species[s].count < 1 || d.species[s].mass <= 0) return fail(PQ_ERR_ARG, "bad species %d", s);
+    c.ifirst[s] = n;
+    c.count[s] = d.species[s].count;
+    c.mass[s] = d.species[s].mass;
+    c.lambda[s] = 0.5 / d.species[s].mass;
+    n += d.species[s].count;
+  }
+  c.N = n;
+  // SuperCell (src/util/SuperCell.cc:14-23)
+  double l2 = d.cell[0] * d.cell[0];
+  c.rcut2 = l2;
+  for (int k = 0; k < 3; ++k) { c.a[k] = d.cell[k]; c.b[k] = (k < c.D) ? 1. / d.cell[k] : 0; }
+  for (int k = 1; k < c.D; ++k) { l2 = d.cell[k] * d.cell[k]; c.rcut2 = (c.rcut2 > l2) ? l2 : c.rcut2; }
+  c.rcut2 *= 0.25;
+  // periodic Gaussians of SpringAction (SpringAction.cc:31-49) and the movers
+  // (FreeMover.cc:37-51, FreeMoverPBC.cc:37-51), used when fewer than 16 terms
+  for (int l = 0; l < PQ_PG_LEVELS; ++l)
+    for (int s = 0; s < c.nsp; ++s)
+      for (int k = 0; k < c.D; ++k) {
+        c.spg_on[l][s][k] = (l <= c.L && pg_nterms(c.mass[s] / (2 * c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
+        c.mpg_on[l][s][k] = (l <= c.L + 1 && pg_nterms(c.mass[s] / (c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
+      }
+  c.spring = d.spring;
+  if (d.nsho < 0 || d.nsho > PQ_MAX_SHO) return fail(PQ_ERR_ARG, "nsho out of range");
+  c.nsho = d.nsho;
+  for (int a = 0; a < d.nsho; ++a) {
+    if (d.sho[a].species < 0 || d.sho[a].species >= c.nsp) return fail(PQ_ERR_ARG, "SHO species");
+    c.sho_sp[a] = d.sho[a].species;
+    c.sho_ndim[a] = d.sho[a].ndim;
+    c.sho_omega[a] = d.sho[a].omega == 0 ? 1.0 : d.sho[a].omega;
+    for (int k = 0; k < 3; ++k) c.sho_center[a][k] = d.sho[a].center[k];
+  }
+  // CoulombAction constructor (src/action/CoulombAction.cc:45-90): rmin, rmax
+  // and ngpts defaults set by the first interacting pair stick for the rest.
+  if (d.coulomb.enabled) {
+    if (d.coulomb.norder < 0 || d.coulomb.norder > PQ_MAX_NORDER) return fail(PQ_ERR_ARG, "norder out of range");
+    const double eps = d.coulomb.epsilon == 0 ? 1 : d.coulomb.epsilon;
+    double rmin = d.coulomb.rmin, rmax = d.coulomb.rmax;
+    int ngpts = d.coulomb.ngpts;
+    for (int i = 0; i < c.nsp; ++i)
+      for (int j = i; j < c.nsp; ++j) {
+        const double q1q2 = d.species[i].charge * d.species[j].charge / eps;
+        if (!(q1q2 != 0 && (i != j || d.species[i].count > 1))) continue;
+        const double mu = 1. / (1. / d.species[i].mass + 1. / d.species[j].mass);
+        if (rmin == 0) rmin = 0.005 / (2 * ((mu > 500) ? 1 : mu) * std::fabs(q1q2));
+        if (rmax == 0) {
+          double a2 = 0;
+          for (int k = 0; k < c.D; ++k) a2 += d.cell[k] * d.cell[k];
+          rmax = std::sqrt(a2);
+        }
+        if (ngpts == 0) ngpts = 500;
+        if (ngpts < 2) return fail(PQ_ERR_ARG, "ngridPoints must be at least 2");
+        const int norder = (mu > 500) ? 0 : d.coulomb.norder;
+        if (c.npair >= PQ_MAX_PAIRS) return fail(PQ_ERR_ARG, "too many pair actions");
+        auto key = std::make_tuple(mu, q1q2, c.tau, rmin, rmax, ngpts, norder);
+        auto it = tabs.find(key);
+        int tid;
+        if (it == tabs.end()) {
+          // PairAction(EmpiricalPairAction) grid (src/action/PairAction.cc:161-185)
+          TableP t;
+          t.ngpts = ngpts;
+          t.K1 = norder + 1;
+          t.rgridinv = 1.0 / rmin;
+          t.logrratioinv = (ngpts - 1) / std::log(rmax / rmin);
+          t.off = (int64_t)b->hu.size();
+          b->hu.resize(b->hu.size() + (size_t)ngpts * t.K1);
+          b->hut.resize(b->hu.size());
+          for (int ig = 0; ig < ngpts; ++ig) {
+            const double r = (1. / t.rgridinv) * std::exp(ig / t.logrratioinv);
+            for (int k = 0; k < t.K1; ++k) {
+              b->hu[t.off + (int64_t)ig * t.K1 + k] = coulomb_u(r, k, mu, q1q2, c.tau);
+              b->hut[t.off + (int64_t)ig * t.K1 + k] = coulomb_utau(r, k, mu, q1q2, c.tau);
+            }
+          }
+          tid = (int)b->ht.size();
+          b->ht.push_back(t);
+          tabs[key] = tid;
+        } else {
+          tid = it->second;
+        }
+        c.pair[c.npair].s1 = i;
+        c.pair[c.npair].s2 = j;
+        c.pair[c.npair].table = tid;
+        c.npair++;
+      }
+  }
+  if (d.nnodes < 0 || d.nnodes > PQ_MAX_NODES) return fail(PQ_ERR_ARG, "nnodes out of range");
+  c.nfn = d.nnodes;
+  const double T = 1.0 / (c.tau * c.M);
+  for (int a = 0; a < d.nnodes; ++a) {
+    const pq_node_desc& nd = d.nodes[a];
+    FnP& f = c.fn[a];
+    if (nd.species < 0 || nd.species >= c.nsp) return fail(PQ_ERR_ARG, "node species");
+    f.sp = nd.species;
+    f.model = nd.model;
+    f.ifirst = c.ifirst[nd.species];
+    f.ns = c.count[nd.species];
+    if (f.ns > PQ_MAX_NODE_PART) return fail(PQ_ERR_ARG, "node species of %d particles exceeds %d", f.ns, PQ_MAX_NODE_PART);
+    f.nodal = nd.nodal_action;
+    f.mass = c.mass[nd.species];
+    const double Tn = nd.temperature == 0 ? T : nd.temperature;
+    // SHONodes constructor (src/fixednode/SHONodes.cc:22-35)
+    f.coshwt = std::cosh(nd.omega * 0.5 / Tn);
+    const double sinhwt = std::sinh(nd.omega * 0.5 / Tn);
+    f.c = f.mass * 0.5 * nd.omega / sinhwt;
+    // FreeParticleNodes periodic Gaussians, alpha = m T (FreeParticleNodes.cc:54-58)
+    const double PI = 3.141592653589793;
+    for (int k = 0; k < c.D; ++k) {
+      const double alpha = f.mass * Tn, len = d.cell[k];
+      f.pg_nome[k] = std::exp(-PI * PI / (alpha * len * len));
+      f.pg_nmax[k] = pg_nterms(alpha, len);
+      f.pg_pref[k] = std::sqrt(PI / (alpha * len * len));
+      f.pg_k[k] = 2 * PI / len;
+    }
+    if (nd.model != PQ_NODES_SHO && nd.model != PQ_NODES_FREE) return fail(PQ_ERR_ARG, "unknown node model");
+    c.ns_max = std::max(c.ns_max, f.ns);
+  }
+  if (c.nfn && 2 * ((1 << c.L) + 1) > c.M)
+    return fail(PQ_ERR_ARG, "the two sections beta/2 apart overlap: need 2(2^L+1) <= M");
+  if (d.mover != PQ_MOVER_FREE && d.mover != PQ_MOVER_FREE_PBC) return fail(PQ_ERR_ARG, "unknown mover");
+  c.mover = d.mover;
+  if (d.nsample < 1 || d.nsample > PQ_MAX_SAMPLES) return fail(PQ_ERR_ARG, "nsample out of range");
+  c.nsample = d.nsample;
+  for (int k = 0; k < d.nsample; ++k) {
+    if (d.sample[k].nmoving != 1) return fail(PQ_ERR_ARG, "the device sampler moves one particle per Sample");
+    if (d.sample[k].species < 0 || d.sample[k].species >= c.nsp) return fail(PQ_ERR_ARG, "sample species");
+    c.sample_sp[k] = d.sample[k].species;
+  }
+  c.nsection = d.nsection;
+  return PQ_OK;
+}
+
+static int64_t align8(int64_t v) { return (v + 7) & ~(int64_t)7; }
+
+extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int precision, int device,
+                               pq_batch** out) {
+  if (!chains || nchain < 1 || !out) return fail(PQ_ERR_ARG, "pq_batch_create: bad arguments");
+  if (precision != PQ_PREC_FP64 && precision != PQ_PREC_MIXED) return fail(PQ_ERR_ARG, "unknown precision");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(PQ_ERR_ARG, "device %d not present (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  pq_batch* b = new pq_batch;
+  b->device = device;
+  b->precision = precision;
+  b->nchain = nchain;
+  b->descs.assign(chains, chains + nchain);
+  b->hc.resize(nchain);
+  b->D = chains[0].ndim;
+  std::map<std::tuple<double, double, double, double, double, int, int>, int> tabs;
+  int64_t beads = 0, inv = 0, fnst = 0;
+  int ns_max = 1;
+  for (int c = 0; c < nchain; ++c) {
+    if (chains[c].ndim != b->D) { delete b; return fail(PQ_ERR_ARG, "all chains of a batch share ndim"); }
+    int rc = build_chain(b, chains[c], b->hc[c], tabs);
+    if (rc) { delete b; return rc; }
+    ns_max = std::max(ns_max, b->hc[c].ns_max);
+  }
+  b->G = ns_max <= 4 ? 4 : (ns_max <= 16 ? 16 : 32);
+  int lds = 0;
+  for (int c = 0; c < nchain; ++c) {
+    ChainP& ch = b->hc[c];
+    ch.bead_off = beads;
+    beads += (int64_t)ch.M * ch.N * ch.D;
+    ch.inv_off = inv;
+    if (ch.nfn) inv += (int64_t)ch.S * b->G * b->G;
+    ch.fn_off = fnst;
+    fnst += fn_state_doubles(ch);
+    // sampler LDS: chain block, sections, moving beads, node state, scratch
+    int64_t o = align8(sizeof(ChainP));
+    int64_t sec = (int64_t)ch.S * ch.N * ch.D * 8;
+    int64_t need = o + sec + (ch.nfn ? sec : 0) + align8(ch.S * ch.D * 8) + fn_state_doubles(ch) * 8 +
+                   PQ_WAVE * 8 + PQ_WAVE * 4 + align8(ch.S * 8) + PQ_MAX_NODES * 8 + PQ_MAX_NODES * 4 + 64;
+    lds = std::max<int>(lds, (int)need);
+  }
+  // one layout for the batch, sized by its largest chain
+  {
+    int64_t mS = 0, mSND = 0, mSD = 0, mF = 0;
+    for (auto& ch : b->hc) {
+      mS = std::max<int64_t>(mS, ch.S);
+      mSND = std::max<int64_t>(mSND, (int64_t)ch.S * ch.N * ch.D);
+      mSD = std::max<int64_t>(mSD, (int64_t)ch.S * ch.D);
+      mF = std::max<int64_t>(mF, fn_state_doubles(ch));
+    }
+    int64_t o = align8(sizeof(ChainP));
+    b->lay.off_sec1 = (int)o; o += mSND * 8;
+    b->lay.off_sec2 = (int)o; o += mSND * 8;
+    b->lay.off_mov = (int)o; o += align8(mSD * 8);
+    b->lay.off_fst = (int)o; o += mF * 8;
+    b->lay.off_wdet = (int)o; o += PQ_WAVE * 8;
+    b->lay.off_wscale = (int)o; o += align8(mS * 8);
+    b->lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
+    b->lay.off_werr = (int)o; o += PQ_WAVE * 4;
+    b->lay.off_notmine = (int)o; o += PQ_MAX_NODES * 4;
+    b->lay.bytes = (int)align8(o);
+  }
+  (void)lds;
+  if (b->lay.bytes > 160 * 1024) {
+    int n = b->lay.bytes;
+    delete b;
+    return fail(PQ_ERR_ARG, "chain too large for one workgroup's LDS (%d bytes)", n);
+  }
+  b->nbeads = beads;
+  // heaviest chains first so the longest workgroups start earliest
+  b->order.resize(nchain);
+  for (int c = 0; c < nchain; ++c) b->order[c] = c;
+  std::vector<double> cost(nchain);
+  for (int c = 0; c < nchain; ++c) {
+    const ChainP& ch = b->hc[c];
+    double ns2 = 0;
+    for (int a = 0; a < ch.nfn; ++a) ns2 += (double)ch.fn[a].ns * ch.fn[a].ns * ch.fn[a].ns;
+    cost[c] = (double)ch.nsection * ch.nsample * (1 << ch.L) * (ch.N + ns2 / std::max(1, ch.nsample));
+  }
+  std::stable_sort(b->order.begin(), b->order.end(), [&](int p, int q) { return cost[p] > cost[q]; });
+
+  auto cleanup_fail = [&](int rc) { pq_batch_destroy(b); return rc; };
+  if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup_fail(fail(PQ_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return cleanup_fail(fail(PQ_ERR_HIP, "hipEventCreate failed"));
+  auto dalloc = [&](void** p, size_t bytes) -> bool {
+    if (bytes == 0) bytes = 8;
+    return hipMalloc(p, bytes) == hipSuccess;
+  };
+  bool ok = dalloc((void**)&b->dchains, sizeof(ChainP) * nchain) &&
+            dalloc((void**)&b->dtables, sizeof(TableP) * std::max<size_t>(1, b->ht.size())) &&
+            dalloc((void**)&b->du, sizeof(double) * b->hu.size()) &&
+            dalloc((void**)&b->dut, sizeof(double) * b->hut.size()) &&
+            dalloc((void**)&b->duf, sizeof(float) * b->hu.size()) &&
+            dalloc((void**)&b->dbeads, sizeof(double) * beads) &&
+            dalloc((void**)&b->dinv, sizeof(double) * inv) &&
+            dalloc((void**)&b->dfn, sizeof(double) * fnst) &&
+            dalloc((void**)&b->dscale, sizeof(double) * nchain * PQ_MAX_NODES) &&
+            dalloc((void**)&b->dcounter, sizeof(uint64_t) * nchain) &&
+            dalloc((void**)&b->dstats, sizeof(pq_chain_stats) * nchain) &&
+            dalloc((void**)&b->dorder, sizeof(int) * nchain);
+  if (!ok) return cleanup_fail(fail(PQ_ERR_NOMEM, "device allocation failed"));
+  std::vector<float> uf(b->hu.begin(), b->hu.end());
+  std::vector<double> ones((size_t)nchain * PQ_MAX_NODES, 1.0);
+  if (hipMemcpy(b->dchains, b->hc.data(), sizeof(ChainP) * nchain, hipMemcpyHostToDevice) != hipSuccess ||
+      (b->ht.size() && hipMemcpy(b->dtables, b->ht.data(), sizeof(TableP) * b->ht.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      (b->hu.size() && hipMemcpy(b->du, b->hu.data(), sizeof(double) * b->hu.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      (b->hut.size() && hipMemcpy(b->dut, b->hut.data(), sizeof(double) * b->hut.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      (uf.size() && hipMemcpy(b->duf, uf.data(), sizeof(float) * uf.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(b->dscale, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dorder, b->order.data(), sizeof(int) * nchain, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(b->dcounter, 0, sizeof(uint64_t) * nchain) != hipSuccess ||
+      hipMemset(b->dstats, 0, sizeof(pq_chain_stats) * nchain) != hipSuccess ||
+      hipMemset(b->dbeads, 0, sizeof(double) * std::max<int64_t>(1, beads)) != hipSuccess ||
+      hipMemset(b->dfn, 0, sizeof(double) * std::max<int64_t>(1, fnst)) != hipSuccess)
+    return cleanup_fail(fail(PQ_ERR_HIP, "device upload failed"));
+  *out = b;
+  return PQ_OK;
+}
+
+extern "C" void pq_batch_destroy(pq_batch* b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn,
+                  b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+static int check_chain(const pq_batch* b, int c) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (c < 0 || c >= b->nchain) return fail(PQ_ERR_ARG, "chain %d out of range", c);
+  return PQ_OK;
+}
+
+extern "C" int pq_batch_info(const pq_batch* b, int c, int* npart, int* nslice, int* ndim, int64_t* off) {
+  if (int rc = check_chain(b, c)) return rc;
+  const ChainP& ch = b->hc[c];
+  if (npart) *npart = ch.N;
+  if (nslice) *nslice = ch.M;
+  if (ndim) *ndim = ch.D;
+  if (off) *off = ch.bead_off;
+  return PQ_OK;
+}
+
+extern "C" int pq_set_paths(pq_batch* b, int c, const double* beads) {
+  if (int rc = check_chain(b, c)) return rc;
+  if (!beads) return fail(PQ_ERR_ARG, "null beads");
+  const ChainP& ch = b->hc[c];
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(b->dbeads + ch.bead_off, beads, sizeof(double) * ch.M * ch.N * ch.D, hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+extern "C" int pq_get_paths(pq_batch* b, int c, double* beads) {
+  if (int rc = check_chain(b, c)) return rc;
+  if (!beads) return fail(PQ_ERR_ARG, "null beads");
+  const ChainP& ch = b->hc[c];
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(beads, b->dbeads + ch.bead_off, sizeof(double) * ch.M * ch.N * ch.D, hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+extern "C" int pq_set_all_paths(pq_batch* b, const double* beads, int64_t count) {
+  if (!b || !beads || count != b->nbeads) return fail(PQ_ERR_ARG, "pq_set_all_paths: expected %lld values", (long long)(b ? b->nbeads : 0));
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(b->dbeads, beads, sizeof(double) * count, hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+extern "C" int pq_get_all_paths(pq_batch* b, double* beads, int64_t count) {
+  if (!b || !beads || count != b->nbeads) return fail(PQ_ERR_ARG, "pq_get_all_paths: expected %lld values", (long long)(b ? b->nbeads : 0));
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(beads, b->dbeads, sizeof(double) * count, hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+extern "C" int pq_seed(pq_batch* b, uint64_t seed, uint32_t first_stream) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->rng_mode = PQ_RNG_PHILOX;
+  b->seed = seed;
+  b->first_stream = first_stream;
+  HIPCHK(hipMemset(b->dcounter, 0, sizeof(uint64_t) * b->nchain));
+  return PQ_OK;
+}
+
+extern "C" int pq_set_uniforms(pq_batch* b, const double* u, int64_t per_chain) {
+  if (!b || !u || per_chain < 1) return fail(PQ_ERR_ARG, "pq_set_uniforms: bad arguments");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->dinj) hipFree(b->dinj);
+  b->dinj = nullptr;
+  HIPCHK(hipMalloc(&b->dinj, sizeof(double) * per_chain * b->nchain));
+  HIPCHK(hipMemcpy(b->dinj, u, sizeof(double) * per_chain * b->nchain, hipMemcpyHostToDevice));
+  b->inj_per_chain = per_chain;
+  b->rng_mode = PQ_RNG_INJECTED;
+  HIPCHK(hipMemset(b->dcounter, 0, sizeof(uint64_t) * b->nchain));
+  return PQ_OK;
+}
+
+extern "C" int pq_run_sweeps(pq_batch* b, int nsweep) {
+  if (!b || nsweep < 0) return fail(PQ_ERR_ARG, "pq_run_sweeps: bad arguments");
+  if (nsweep == 0) return PQ_OK;
+  HIPCHK(hipSetDevice(b->device));
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_sweep(b->D, b->G, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_run_sweeps_timed(pq_batch* b, int nsweep, float* ms) {
+  if (!b || nsweep < 1 || !ms) return fail(PQ_ERR_ARG, "pq_run_sweeps_timed: bad arguments");
+  HIPCHK(hipSetDevice(b->device));
+  DevCtx x = make_ctx(b);
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  HIPCHK(launch_sweep(b->D, b->G, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+  return PQ_OK;
+}
+
+extern "C" int pq_sync(pq_batch* b) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_get_stats(pq_batch* b, int c, pq_chain_stats* out) {
+  if (int rc = check_chain(b, c)) return rc;
+  if (!out) return fail(PQ_ERR_ARG, "null stats");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(out, b->dstats + c, sizeof(pq_chain_stats), hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+extern "C" int pq_get_all_stats(pq_batch* b, pq_chain_stats* out) {
+  if (!b || !out) return fail(PQ_ERR_ARG, "pq_get_all_stats: bad arguments");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(out, b->dstats, sizeof(pq_chain_stats) * b->nchain, hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+extern "C" int pq_reset_stats(pq_batch* b) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  std::vector<uint64_t> cnt(b->nchain);
+  HIPCHK(hipMemcpy(cnt.data(), b->dcounter, sizeof(uint64_t) * b->nchain, hipMemcpyDeviceToHost));
+  std::vector<pq_chain_stats> st(b->nchain);
+  std::memset(st.data(), 0, sizeof(pq_chain_stats) * b->nchain);
+  for (int c = 0; c < b->nchain; ++c) st[c].draws = cnt[c];
+  HIPCHK(hipMemcpy(b->dstats, st.data(), sizeof(pq_chain_stats) * b->nchain, hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+extern "C" int pq_get_node_scale(pq_batch* b, int c, double* scale) {
+  if (int rc = check_chain(b, c)) return rc;
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(scale, b->dscale + (int64_t)c * PQ_MAX_NODES, sizeof(double) * b->hc[c].nfn, hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+extern "C" int pq_measure_energy(pq_batch* b, double* energy) {
+  if (!b || !energy) return fail(PQ_ERR_ARG, "pq_measure_energy: bad arguments");
+  HIPCHK(hipSetDevice(b->device));
+  void* p;
+  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_energy(b->D, x, b->nchain, (double*)p, b->stream));
+  HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_pair_histogram(pq_batch* b, int s1, int s2, double rmin, double rmax, int nbin,
+                                 uint64_t* counts) {
+  if (!b || !counts || nbin < 1 || nbin > 16384 || !(rmax > rmin))
+    return fail(PQ_ERR_ARG, "pq_pair_histogram: bad arguments");
+  if (s1 < 0 || s2 < 0 || s1 >= PQ_MAX_SPECIES || s2 >= PQ_MAX_SPECIES) return fail(PQ_ERR_ARG, "species out of range");
+  HIPCHK(hipSetDevice(b->device));
+  void* p;
+  const size_t bytes = sizeof(uint64_t) * (size_t)nbin * b->nchain;
+  if (int rc = scratch(b, bytes, &p)) return rc;
+  HIPCHK(hipMemcpyAsync(p, counts, bytes, hipMemcpyHostToDevice, b->stream));
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_histogram(b->D, x, b->nchain, s1, s2, rmin, rmax, nbin, (uint64_t*)p, b->stream));
+  HIPCHK(hipMemcpyAsync(counts, p, bytes, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_action_difference(pq_batch* b, int nq, const int* chain, const int* level,
+                                    const double* section, const double* moving, const int* index,
+                                    int nmov, double* dS) {
+  if (!b || nq < 1 || !chain || !level || !section || !moving || !index || !dS)
+    return fail(PQ_ERR_ARG, "pq_action_difference: bad arguments");
+  if (nmov != 1) return fail(PQ_ERR_ARG, "pq_action_difference: one moving particle per query");
+  int64_t sec_stride = 0, mov_stride = 0;
+  for (int q = 0; q < nq; ++q) {
+    if (int rc = check_chain(b, chain[q])) return rc;
+    const ChainP& ch = b->hc[chain[q]];
+    if (level[q] < 0 || level[q] > ch.L) return fail(PQ_ERR_ARG, "query %d: level out of range", q);
+    if (index[q] < 0 || index[q] >= ch.N) return fail(PQ_ERR_ARG, "query %d: index out of range", q);
+    sec_stride = std::max<int64_t>(sec_stride, (int64_t)ch.S * ch.N * ch.D);
+    mov_stride = std::max<int64_t>(mov_stride, (int64_t)ch.S * ch.D);
+  }
+  // queries arrive packed by their own chain's sizes; restage with a fixed stride
+  std::vector<double> hs((size_t)nq * sec_stride, 0.0), hm((size_t)nq * mov_stride, 0.0);
+  int64_t os = 0, om = 0;
+  for (int q = 0; q < nq; ++q) {
+    const ChainP& ch = b->hc[chain[q]];
+    const int64_t ns = (int64_t)ch.S * ch.N * ch.D, nm = (int64_t)ch.S * ch.D;
+    std::memcpy(&hs[q * sec_stride], section + os, sizeof(double) * ns);
+    std::memcpy(&hm[q * mov_stride], moving + om, sizeof(double) * nm);
+    os += ns;
+    om += nm;
+  }
+  HIPCHK(hipSetDevice(b->device));
+  const size_t bytes = sizeof(double) * (hs.size() + hm.size() + nq) + sizeof(int) * 3 * nq + 64;
+  void* p;
+  if (int rc = scratch(b, bytes, &p)) return rc;
+  double* dsec = (double*)p;
+  double* dmov = dsec + hs.size();
+  double* dout = dmov + hm.size();
+  int* dq = (int*)(dout + nq);
+  HIPCHK(hipMemcpyAsync(dsec, hs.data(), sizeof(double) * hs.size(), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dmov, hm.data(), sizeof(double) * hm.size(), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq, chain, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq + nq, level, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq + 2 * nq, index, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  CallBufsH cb{dq, dq + nq, dsec, dmov, dq + 2 * nq, sec_stride, mov_stride, dout};
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_action_diff(b->D, b->G, b->precision == PQ_PREC_MIXED, x, nq, cb, b->stream));
+  HIPCHK(hipMemcpyAsync(dS, dout, sizeof(double) * nq, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_bead_utau(pq_batch* b, int nq, const int* chain, const int* ipart, const int* islice,
+                            double* utau) {
+  if (!b || nq < 1 || !chain || !ipart || !islice || !utau) return fail(PQ_ERR_ARG, "pq_bead_utau: bad arguments");
+  for (int q = 0; q < nq; ++q) {
+    if (int rc = check_chain(b, chain[q])) return rc;
+    const ChainP& ch = b->hc[chain[q]];
+    if (ipart[q] < 0 || ipart[q] >= ch.N) return fail(PQ_ERR_ARG, "query %d: particle out of range", q);
+  }
+  HIPCHK(hipSetDevice(b->device));
+  void* p;
+  if (int rc = scratch(b, sizeof(double) * nq + sizeof(int) * 3 * nq + 64, &p)) return rc;
+  double* dout = (double*)p;
+  int* dq = (int*)(dout + nq);
+  HIPCHK(hipMemcpyAsync(dq, chain, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq + nq, ipart, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq + 2 * nq, islice, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_bead_utau(b->D, x, nq, dq, dq + nq, dq + 2 * nq, dout, b->stream));
+  HIPCHK(hipMemcpyAsync(utau, dout, sizeof(double) * nq, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+// fixed-node protocol on one chain; section buffers live in the scratch block
+struct FnScr {
+  double *sec1, *sec2, *mov, *wdet, *wscale, *out;
+  int *werr, *notmine;
+};
+
+static int fn_scratch(pq_batch* b, int c, FnScr* s) {
+  const ChainP& ch = b->hc[c];
+  const int64_t sec = (int64_t)ch.S * ch.N * ch.D;
+  const size_t bytes = sizeof(double) * (2 * sec + ch.S * ch.D + PQ_WAVE + ch.S + 8) + sizeof(int) * (PQ_WAVE + 8) + 256;
+  if (bytes > b->fnscr_bytes) {
+    // sized for the largest chain so the buffers survive between calls
+    size_t big = 0;
+    for (const ChainP& h : b->hc) {
+      const int64_t hs = (int64_t)h.S * h.N * h.D;
+      big = std::max(big, sizeof(double) * (2 * hs + h.S * h.D + PQ_WAVE + h.S + 8) + sizeof(int) * (PQ_WAVE + 8) + 256);
+    }
+    if (b->dfnscr) hipFree(b->dfnscr);
+    b->dfnscr = nullptr;
+    b->fnscr_bytes = 0;
+    HIPCHK(hipMalloc(&b->dfnscr, big));
+    HIPCHK(hipMemset(b->dfnscr, 0, big));
+    b->fnscr_bytes = big;
+  }
+  double* d = b->dfnscr;
+  s->sec1 = d; d += sec;
+  s->sec2 = d; d += sec;
+  s->mov = d; d += ch.S * ch.D;
+  s->wdet = d; d += PQ_WAVE;
+  s->wscale = d; d += ch.S;
+  s->out = d; d += 8;
+  s->werr = (int*)d;
+  s->notmine = s->werr + PQ_WAVE;
+  return PQ_OK;
+}
+
+extern "C" int pq_section_begin(pq_batch* b, int c, const double* sec1, const double* sec2) {
+  if (int rc = check_chain(b, c)) return rc;
+  if (!sec1 || !sec2) return fail(PQ_ERR_ARG, "null section");
+  const ChainP& ch = b->hc[c];
+  if (!ch.nfn) return fail(PQ_ERR_STATE, "chain %d has no FixedNodeAction", c);
+  HIPCHK(hipSetDevice(b->device));
+  FnScr s;
+  if (int rc = fn_scratch(b, c, &s)) return rc;
+  const size_t sec = sizeof(double) * ch.S * ch.N * ch.D;
+  HIPCHK(hipMemcpyAsync(s.sec1, sec1, sec, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(s.sec2, sec2, sec, hipMemcpyHostToDevice, b->stream));
+  FnBufsH fb{s.sec1, s.sec2, s.mov, s.wdet, s.werr, s.wscale, s.notmine, 0, 0, 0, s.out};
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_fn(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_double_action_difference(pq_batch* b, int c, int level, const double* moving,
+                                           const int* index, int nmov, double* dS) {
+  if (int rc = check_chain(b, c)) return rc;
+  if (!moving || !index || !dS) return fail(PQ_ERR_ARG, "pq_double_action_difference: bad arguments");
+  if (nmov != 1) return fail(PQ_ERR_ARG, "pq_double_action_difference: one moving particle");
+  const ChainP& ch = b->hc[c];
+  if (!ch.nfn) { *dS = 0; return PQ_OK; }
+  if (level < 0 || level > ch.L || index[0] < 0 || index[0] >= ch.N) return fail(PQ_ERR_ARG, "level or index out of range");
+  HIPCHK(hipSetDevice(b->device));
+  FnScr s;
+  if (int rc = fn_scratch(b, c, &s)) return rc;
+  HIPCHK(hipMemcpyAsync(s.mov, moving, sizeof(double) * ch.S * ch.D, hipMemcpyHostToDevice, b->stream));
+  FnBufsH fb{s.sec1, s.sec2, s.mov, s.wdet, s.werr, s.wscale, s.notmine, 1, level, index[0], s.out};
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_fn(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
+  HIPCHK(hipMemcpyAsync(dS, s.out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_accept_last_move(pq_batch* b, int c) {
+  if (int rc = check_chain(b, c)) return rc;
+  const ChainP& ch = b->hc[c];
+  if (!ch.nfn) return PQ_OK;
+  HIPCHK(hipSetDevice(b->device));
+  FnScr s;
+  if (int rc = fn_scratch(b, c, &s)) return rc;
+  FnBufsH fb{s.sec1, s.sec2, s.mov, s.wdet, s.werr, s.wscale, s.notmine, 2, 0, 0, s.out};
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_fn(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_node_evaluate(pq_batch* b, int c, int inode, int nmat, const double* r1, const double* r2,
+                                double* det, int* err, double* d1, double* d2) {
+  if (int rc = check_chain(b, c)) return rc;
+  const ChainP& ch = b->hc[c];
+  if (inode < 0 || inode >= ch.nfn) return fail(PQ_ERR_ARG, "node action %d out of range", inode);
+  if (nmat < 1 || !r1 || !r2 || !det || !err) return fail(PQ_ERR_ARG, "pq_node_evaluate: bad arguments");
+  const int ns = ch.fn[inode].ns;
+  const int64_t sl = (int64_t)ch.N * ch.D;
+  HIPCHK(hipSetDevice(b->device));
+  const size_t nd = (size_t)nmat * sl;
+  const size_t bytes = sizeof(double) * (2 * nd + (size_t)nmat * b->G * b->G + nmat + (size_t)nmat * 2 * ns) +
+                       sizeof(int) * nmat + 256;
+  void* p;
+  if (int rc = scratch(b, bytes, &p)) return rc;
+  double* dr1 = (double*)p;
+  double* dr2 = dr1 + nd;
+  double* dinv = dr2 + nd;
+  double* ddet = dinv + (size_t)nmat * b->G * b->G;
+  double* ddist = ddet + nmat;
+  int* derr = (int*)(ddist + (size_t)nmat * 2 * ns);
+  HIPCHK(hipMemcpyAsync(dr1, r1, sizeof(double) * nd, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dr2, r2, sizeof(double) * nd, hipMemcpyHostToDevice, b->stream));
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_node_eval(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, inode, nmat, dr1, dr2, dinv,
+                          ddet, derr, ddist, b->stream));
+  std::vector<double> hd((size_t)nmat * 2 * ns);
+  HIPCHK(hipMemcpyAsync(det, ddet, sizeof(double) * nmat, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipMemcpyAsync(err, derr, sizeof(int) * nmat, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipMemcpyAsync(hd.data(), ddist, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int m = 0; m < nmat; ++m)
+    for (int k = 0; k < ns; ++k) {
+      if (d1) d1[(size_t)m * ns + k] = hd[((size_t)m * 2 + 0) * ns + k];
+      if (d2) d2[(size_t)m * ns + k] = hd[((size_t)m * 2 + 1) * ns + k];
+    }
+  return PQ_OK;
+}
+
+extern "C" int pq_get_pair_table(pq_batch* b, int c, int ipair, int* meta, double* grid, double* u, double* ut) {
+  if (int rc = check_chain(b, c)) return rc;
+  const ChainP& ch = b->hc[c];
+  if (ipair < 0 || ipair >= ch.npair) return fail(PQ_ERR_ARG, "pair %d out of range", ipair);
+  const TableP& t = b->ht[ch.pair[ipair].table];
+  if (meta) { meta[0] = ch.pair[ipair].s1; meta[1] = ch.pair[ipair].s2; meta[2] = t.K1 - 1; meta[3] = t.ngpts; }
+  if (grid) { grid[0] = t.rgridinv; grid[1] = t.logrratioinv; }
+  const size_t n = (size_t)t.ngpts * t.K1;
+  HIPCHK(hipSetDevice(b->device));
+  if (u) HIPCHK(hipMemcpy(u, b->du + t.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (ut) HIPCHK(hipMemcpy(ut, b->dut + t.off, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return PQ_OK;
+}

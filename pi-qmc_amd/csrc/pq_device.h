@@ -1,0 +1,204 @@
+// pq_device.h -- device building blocks of the bisection-move hot path.
+//
+// Execution model: one 64-lane wavefront owns one Markov chain.  Every routine
+// here is called by all 64 lanes of a single-wave workgroup with wave-uniform
+// arguments and returns wave-uniform results; lanes split the independent work
+// (moved beads, links, partner x link pairs, node matrices, matrix rows) and
+// combine it with cross-lane reductions.  Section beads live in LDS for the
+// fused sampler and in global memory for the call-level kernels; the routines
+// only see plain pointers.
+//
+// Arithmetic type R is double (FP64 mode) or float (mixed mode).  In mixed
+// mode the spring, SHO and pair-table arithmetic and the node-matrix fill run
+// in FP32 with compensated per-lane sums; coordinate differences and minimum
+// image, the mover, the node LU/inverse/determinant, the sign test and the
+// Metropolis exponent stay in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pq_params.h"
+#include "../../include/piqmc_hip.h"
+
+namespace pq {
+
+struct DevCtx {
+  const ChainP* chains;
+  const TableP* tables;
+  const double* tab_u;
+  const double* tab_ut;
+  const float* tab_uf;
+  double* beads;
+  double* inv;            // per-chain [S][ns_max][ns_max] inverse cache
+  double* fnstate;        // per-chain fixed-node state (call-level API)
+  double* scale;          // [nchain][PQ_MAX_NODES]
+  uint64_t* counter;      // uniforms consumed per chain
+  pq_chain_stats* stats;
+  const double* inj;
+  int64_t inj_per_chain;
+  int rng_mode;
+  uint64_t seed;
+  uint32_t first_stream;
+  const int* order;       // workgroup -> chain
+};
+
+// ------------------------------------------------------------ wave ops
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
+// compensated accumulator: plain sum in FP64, Neumaier sum in FP32
+template <class R> struct Acc {
+  R s = 0;
+  __device__ __forceinline__ void add(R v) { s += v; }
+  __device__ __forceinline__ double value() const { return (double)s; }
+};
+template <> struct Acc<float> {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float v) {
+    float t = s + v;
+    if (fabsf(s) >= fabsf(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
+// --------------------------------------------------------------- RNG
+
+// Philox4x32-10; draw k of stream s: counter (k>>2, k>>34, s, 0), key (seed),
+// word k&3, u = w * 2^-32.
+__device__ __forceinline__ uint32_t philox_word(uint64_t seed, uint32_t stream, uint64_t k) {
+  uint32_t c0 = (uint32_t)(k >> 2), c1 = (uint32_t)(k >> 34), c2 = stream, c3 = 0u;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+  }
+  uint32_t w = c0;
+  int sel = (int)(k & 3);
+  w = (sel == 1) ? c1 : w;
+  w = (sel == 2) ? c2 : w;
+  w = (sel == 3) ? c3 : w;
+  return w;
+}
+
+__device__ __forceinline__ double uniform(const DevCtx& x, int chain, uint64_t k) {
+  if (x.rng_mode == PQ_RNG_PHILOX)
+    return (double)philox_word(x.seed, x.first_stream + (uint32_t)chain, k) * 2.3283064365386963e-10;
+  return (k < (uint64_t)x.inj_per_chain) ? x.inj[(int64_t)chain * x.inj_per_chain + (int64_t)k] : 0.5;
+}
+
+// ------------------------------------------------- SuperCell and PG
+
+// SuperCell::pbc (src/util/SuperCell.cc:25-31)
+template <int D>
+__device__ __forceinline__ void pbc(const ChainP& c, double* v) {
+  double r2 = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) r2 += v[d] * v[d];
+  if (r2 > c.rcut2) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double i = floor(c.b[d] * v[d] + 1000.5) - 1000;
+      v[d] -= c.a[d] * i;
+    }
+  }
+}
+
+struct PG {
+  double nome, pref, k;
+  int nmax;
+};
+
+__device__ __forceinline__ PG pg_make(double alpha, double length) {
+  const double PI = 3.141592653589793;
+  PG p;
+  p.nome = exp(-PI * PI / (alpha * length * length));
+  p.nmax = 1 + (int)(2.0 * sqrt(alpha * length * length));
+  p.pref = sqrt(PI / (alpha * length * length));
+  p.k = 2 * PI / length;
+  return p;
+}
+
+// PeriodicGaussian::evaluate (src/util/PeriodicGaussian.cc:20-40): value and gradient
+__device__ __forceinline__ double pg_eval(double nome, double pref, double k, int nmax,
+                                          double x, double* grad) {
+  double sx, cx;
+  sincos(k * x, &sx, &cx);
+  double s0r = 0, s1r = 0, tr = 1.0, ti = 0.0, nomeN = 1.0;
+  for (int n = 1; n < nmax; ++n) {
+    double f = nomeN * nomeN * nome;
+    double fwr = f * cx, fwi = f * sx;
+    double nr = tr * fwr - ti * fwi, ni = tr * fwi + ti * fwr;
+    tr = nr; ti = ni;
+    s0r += tr;
+    s1r += (-ti * k) * (double)n;
+    nomeN *= nome;
+  }
+  if (grad) *grad = pref * 2 * s1r;
+  return pref * (1 + 2 * s0r);
+}
+
+__device__ __forceinline__ double pg_eval(const PG& p, double x) {
+  return pg_eval(p.nome, p.pref, p.k, p.nmax, x, nullptr);
+}
+
+__device__ __forceinline__ int species_of(const ChainP& c, int i) {
+  int s = 0;
+  for (int k = 1; k < c.nsp; ++k) s = (i >= c.ifirst[k]) ? k : s;
+  return s;
+}
+
+// fast transcendental overloads for the arithmetic type
+__device__ __forceinline__ double r_log(double v) { return log(v); }
+__device__ __forceinline__ float r_log(float v) { return __logf(v); }
+__device__ __forceinline__ double r_sqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float r_sqrt(float v) { return __fsqrt_rn(v); }
+__device__ __forceinline__ double r_exp(double v) { return exp(v); }
+__device__ __forceinline__ float r_exp(float v) { return __expf(v); }
+
+template <class R> __device__ __forceinline__ const R* table_ptr(const DevCtx& x, const TableP& t);
+template <> __device__ __forceinline__ const double* table_ptr<double>(const DevCtx& x, const TableP& t) {
+  return x.tab_u + t.off;
+}
+template <> __device__ __forceinline__ const float* table_ptr<float>(const DevCtx& x, const TableP& t) {
+  return x.tab_uf + t.off;
+}
+
+// PairAction::u00 / uk0 (src/action/PairAction.cc:601-621); K1=1 or the
+// diagonal-only branch give u00.
+template <class R>
+__device__ __forceinline__ R pair_u(const TableP& t, const R* u, R q, R s2, bool offdiag) {
+  q *= (R)t.rgridinv;
+  R xx = r_log(q) * (R)t.logrratioinv;
+  int i = (int)xx;
+  if (q < (R)1) { i = 0; xx = 0; }
+  else if (i > t.ngpts - 2) { i = t.ngpts - 2; xx = 1; }
+  else xx -= (R)i;
+  const R* u0 = u + (int64_t)i * t.K1;
+  const R* u1 = u0 + t.K1;
+  if (!offdiag) return ((R)1 - xx) * u0[0] + xx * u1[0];
+  R a = 0;
+  for (int k = t.K1 - 1; k >= 0; --k) {
+    a *= s2;
+    a += ((R)1 - xx) * u0[k] + xx * u1[k];
+  }
+  return a;
+}
+
+}  // namespace pq

@@ -1,0 +1,53 @@
+// pq_launch.h -- host-side launch entry points of the kernels (pq_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pq_device.h"
+
+namespace pq {
+
+struct SweepLayoutH {
+  int off_sec1, off_sec2, off_mov, off_fst, off_wdet, off_werr, off_wscale, off_scale, off_notmine;
+  int bytes;
+};
+
+struct CallBufsH {
+  const int* qchain;
+  const int* qlevel;
+  const double* qsec;
+  const double* qmov;
+  const int* qidx;
+  int64_t sec_stride;
+  int64_t mov_stride;
+  double* out;
+};
+
+struct FnBufsH {
+  double* sec1;
+  double* sec2;
+  double* mov;
+  double* wdet;
+  int* werr;
+  double* wscale;
+  int* notmine;
+  int op;
+  int level;
+  int idx;
+  double* out;
+};
+
+hipError_t launch_sweep(int D, int G, bool mixed, const DevCtx& x, int nchain, int nsweep,
+                        const SweepLayoutH& L, hipStream_t st);
+hipError_t launch_action_diff(int D, int G, bool mixed, const DevCtx& x, int nq, const CallBufsH& b,
+                              hipStream_t st);
+hipError_t launch_fn(int D, int G, bool mixed, const DevCtx& x, int chain, const FnBufsH& b,
+                     hipStream_t st);
+hipError_t launch_node_eval(int D, int G, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
+                            double* r1, double* r2, double* inv, double* det, int* err, double* dist,
+                            hipStream_t st);
+hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st);
+hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const int* qi, const int* qt,
+                            double* out, hipStream_t st);
+hipError_t launch_histogram(int D, const DevCtx& x, int nchain, int s1, int s2, double rmin, double rmax,
+                            int nbin, uint64_t* counts, hipStream_t st);
+
+}  // namespace pq

@@ -1,0 +1,65 @@
+// pq_params.h -- device-side parameter blocks shared by the host setup code
+// and the HIP kernels.  One ChainP per Markov chain lives in device memory;
+// pair tables are deduplicated across chains and stored once.
+#pragma once
+#include <stdint.h>
+
+#define PQ_WAVE 64
+#define PQ_MAX_PAIRS 10
+#define PQ_PG_LEVELS 8
+
+struct TableP {          // one tabulated pair action u_k(r), utau_k(r)
+  int ngpts;
+  int K1;                // norder + 1
+  double rgridinv;
+  double logrratioinv;
+  int64_t off;           // element offset of u[ngpts][K1] (and ut) in the table pool
+};
+
+struct PairP {
+  int s1, s2;
+  int table;
+};
+
+struct FnP {             // one FixedNodeAction with its node model
+  int sp, model, ifirst, ns, nodal;
+  double mass;
+  double coshwt, c;      // SHONodes
+  double pg_nome[3], pg_pref[3], pg_k[3];
+  int pg_nmax[3];        // FreeParticleNodes periodic Gaussians, alpha = m T
+};
+
+struct ChainP {
+  int N, D, M, L, S, nsp;
+  double tau;
+  double a[3], b[3], rcut2;
+  int ifirst[4], count[4];
+  double mass[4], lambda[4];
+  int spring;
+  int nsho;
+  int sho_sp[4], sho_ndim[4];
+  double sho_omega[4];
+  double sho_center[4][3];
+  int npair;
+  PairP pair[PQ_MAX_PAIRS];
+  int nfn;
+  FnP fn[4];
+  int mover;
+  int nsample;
+  int sample_sp[8];
+  int nsection;
+  unsigned char spg_on[PQ_PG_LEVELS][4][3];   // SpringAction periodic Gaussian used
+  unsigned char mpg_on[PQ_PG_LEVELS][4][3];   // mover periodic Gaussian used
+  int64_t bead_off;      // element offset of this chain's [M][N][D] beads
+  int64_t inv_off;       // element offset of this chain's [S][ns][ns] inverse cache
+  int64_t fn_off;        // element offset of this chain's fixed-node state (global copy)
+  int ns_max;            // largest fermion species with a node model
+};
+
+// Sizes of the per-chain fixed-node state block (dm, newdm, dist, newdist per
+// FixedNodeAction, [S] and [S][2][ns]).
+static inline int64_t fn_state_doubles(const ChainP& c) {
+  int64_t n = 0;
+  for (int a = 0; a < c.nfn; ++a) n += 2 * c.S + 4 * (int64_t)c.S * c.fn[a].ns;
+  return n;
+}

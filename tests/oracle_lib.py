@@ -29,6 +29,7 @@ def load():
     dp, ip, vp = P(C.c_double), P(C.c_int), C.c_void_p
     sig = {
         "oq_glibc_sequence": (None, [C.c_uint, C.c_int, P(C.c_int32)]),
+        "oq_philox_raw": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
         "oq_philox_u32": (C.c_uint32, [C.c_uint64, C.c_uint32, C.c_uint64]),
         "oq_philox_uniform": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint64]),
         "oq_coulomb1d": (C.c_double, [C.c_double, C.c_int, C.c_double]),

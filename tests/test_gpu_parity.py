@@ -1,0 +1,256 @@
+"""GPU parity: the HIP library, called through its C ABI, against the oracle
+on the same seeded inputs.  FP64 action and node values agree to 1e-12
+relative (relative to the larger of the value and the summed term scale);
+Metropolis decisions are identical given the same uniforms; integer
+histograms are bit-exact."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import piqmc_capi as capi
+from gpu_helpers import glibc_uniforms, random_section, rel_close, small_cfg2
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _descs():
+    c1 = capi.cfg1_desc()
+    c2 = small_cfg2()
+    pbc = capi.make_desc(3, 64, 3, 0.05, [(2, 1.0, -1.0), (2, 2.0, 1.0)], cell=4.0,
+                         coulomb=dict(epsilon=1.0, norder=3, rmin=0.0, rmax=0.0, ngpts=512),
+                         mover=capi.PQ_MOVER_FREE_PBC)
+    return {"cfg1": c1, "cfg2": c2, "pbc": pbc}
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cfg2", "pbc"])
+def test_action_difference(name):
+    print(f"[test] action_difference {name}", flush=True)
+    desc = _descs()[name]
+    b = capi.Batch([desc])
+    oc = ol.Chain(desc)
+    rng = np.random.default_rng(11)
+    secs, movs, idxs, lev, want = [], [], [], [], []
+    for q in range(48):
+        sec, mov, idx = random_section(desc, rng)
+        level = q % (desc.nlevel + 1)
+        secs.append(sec); movs.append(mov); idxs.append(idx); lev.append(level)
+        want.append(oc.action_difference(sec, mov, [idx], level))
+    nq = len(secs)
+    got = np.zeros(nq)
+    lib = capi.load_library()
+    sec_all = np.ascontiguousarray(np.concatenate([s.ravel() for s in secs]))
+    mov_all = np.ascontiguousarray(np.concatenate([m.ravel() for m in movs]))
+    ch = np.zeros(nq, dtype=np.int32)
+    lv = np.array(lev, dtype=np.int32)
+    ix = np.array(idxs, dtype=np.int32)
+    capi.check(lib.pq_action_difference(b.h, nq, capi.iptr(ch), capi.iptr(lv), capi.dptr(sec_all),
+                                        capi.dptr(mov_all), capi.iptr(ix), 1, capi.dptr(got)))
+    want = np.array(want)
+    assert rel_close(got, want, RTOL, scale=1.0), np.max(np.abs(got - want))
+
+
+def test_pair_table_matches_oracle():
+    import ctypes as C
+    desc = _descs()["pbc"]
+    b = capi.Batch([desc])
+    oc = ol.Chain(desc)
+    lib = capi.load_library()
+    lib.pq_get_pair_table.restype = C.c_int
+    for ip in range(ol.load().oq_npair(oc.h)):
+        meta_o, grid_o, u_o, ut_o = oc.pair_table(ip)
+        meta = np.zeros(4, dtype=np.int32)
+        grid = np.zeros(2)
+        u = np.zeros_like(u_o)
+        ut = np.zeros_like(ut_o)
+        capi.check(lib.pq_get_pair_table(b.h, 0, ip, capi.iptr(meta), capi.dptr(grid),
+                                         capi.dptr(u), capi.dptr(ut)))
+        assert list(meta) == list(meta_o)
+        assert np.array_equal(grid, grid_o)
+        assert np.allclose(u, u_o, rtol=1e-14, atol=0)
+        assert np.allclose(ut, ut_o, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("model", [capi.PQ_NODES_SHO, capi.PQ_NODES_FREE])
+def test_node_evaluate(model):
+    print(f"[test] node_evaluate model={model}", flush=True)
+    desc = capi.make_desc(2, 64, 4, 0.02, [(5, 1.0, -1.0), (3, 1.0, -1.0)], cell=6.0,
+                          sho=[(0, 1.0), (1, 1.0)], nodes=[(0, model, 1.0), (1, model, 1.0)])
+    b = capi.Batch([desc])
+    oc = ol.Chain(desc)
+    rng = np.random.default_rng(5)
+    nmat = 9
+    r1 = rng.uniform(-1.5, 1.5, size=(nmat, desc.npart, 2))
+    r2 = r1 + 0.2 * rng.standard_normal(r1.shape)
+    lib = capi.load_library()
+    for inode, ns in ((0, 5), (1, 3)):
+        det = np.zeros(nmat)
+        err = np.zeros(nmat, dtype=np.int32)
+        d1 = np.zeros(nmat * ns)
+        d2 = np.zeros(nmat * ns)
+        capi.check(lib.pq_node_evaluate(b.h, 0, inode, nmat, capi.dptr(np.ascontiguousarray(r1)),
+                                        capi.dptr(np.ascontiguousarray(r2)), capi.dptr(det),
+                                        capi.iptr(err), capi.dptr(d1), capi.dptr(d2)))
+        for m in range(nmat):
+            dw, ew = oc.node_evaluate(inode, r1[m], r2[m], 0)
+            e1, e2 = oc.node_distance(inode, r1[m], r2[m], 0, ns)
+            assert err[m] == ew
+            assert rel_close(det[m], dw, RTOL, scale=0.0), (m, det[m], dw)
+            assert rel_close(d1[m * ns:(m + 1) * ns], e1, 1e-11)
+            assert rel_close(d2[m * ns:(m + 1) * ns], e2, 1e-11)
+
+
+def test_fixed_node_protocol():
+    """initialize, per-level action difference (with crossings) and accept on one chain."""
+    print("[test] fixed_node_protocol", flush=True)
+    desc = small_cfg2(64)
+    b = capi.Batch([desc])
+    oc = ol.Chain(desc)
+    lib = capi.load_library()
+    rng = np.random.default_rng(3)
+    S = (1 << desc.nlevel) + 1
+    base = rng.standard_normal((1, desc.npart, 2))
+    sec1 = np.ascontiguousarray(base + 0.15 * rng.standard_normal((S, desc.npart, 2)))
+    sec2 = np.ascontiguousarray(base + 0.15 * rng.standard_normal((S, desc.npart, 2)))
+    capi.check(lib.pq_section_begin(b.h, 0, capi.dptr(sec1), capi.dptr(sec2)))
+    oc.fn_initialize(sec1, sec2)
+    nbig = 0
+    for trial in range(16):
+        idx = int(rng.integers(0, desc.npart))
+        mov = np.ascontiguousarray(sec1[:, idx:idx + 1, :].copy())
+        if trial % 4 == 3:
+            # walk onto a same-species partner: the determinant passes through zero
+            other = (idx // 3) * 3 + (idx + 1) % 3
+            mov[1:-1, 0] = sec1[1:-1, other] + 1e-3 * rng.standard_normal((S - 2, 2))
+        else:
+            mov[1:-1] += (0.05 if trial % 4 else 0.6) * rng.standard_normal((S - 2, 1, 2))
+        for level in range(desc.nlevel, -1, -1):
+            got = np.zeros(1)
+            ix = np.array([idx], dtype=np.int32)
+            capi.check(lib.pq_double_action_difference(b.h, 0, level, capi.dptr(mov), capi.iptr(ix), 1,
+                                                       capi.dptr(got)))
+            want = oc.fn_action_difference(mov, [idx], level)
+            if want >= 1e100:
+                nbig += 1
+                assert got[0] == want
+                break
+            assert rel_close(got[0], want, 1e-11, scale=1.0), (trial, level, got[0], want)
+        else:
+            capi.check(lib.pq_accept_last_move(b.h, 0))
+            oc.fn_accept()
+            sec1[:, idx:idx + 1, :] = mov
+    assert nbig >= 1
+
+
+def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False):
+    b = capi.Batch([desc], precision=precision)
+    oc = ol.Chain(desc)
+    beads = capi.initial_beads(desc, seed)
+    b.set_paths(0, beads)
+    oc.set_beads(beads)
+    if inject:
+        u = glibc_uniforms(seed, 400000)
+        b.set_uniforms(u[None, :])
+        oc.rng_inject(u)
+    else:
+        b.seed(1234, 0)
+        oc.rng_philox(1234, 0)
+    b.run_sweeps(nsweep)
+    b.sync()
+    oc.run_sweeps(nsweep)
+    return b, oc
+
+
+@pytest.mark.parametrize("name,inject", [("cfg1", True), ("cfg1", False), ("cfg2", True), ("pbc", False)])
+def test_sampler_decisions(name, inject):
+    """Same uniforms in, same accept/reject decisions and paths out."""
+    print(f"[test] sampler_decisions {name} inject={inject}", flush=True)
+    desc = _descs()[name]
+    b, oc = _paired(desc, 6, inject=inject)
+    sg, so = b.stats(0), oc.stats()
+    assert sg.draws == so.draws
+    assert list(sg.tries) == list(so.tries)
+    assert list(sg.accepts) == list(so.accepts)
+    assert sg.bead_updates == so.bead_updates
+    assert sg.node_errors == so.node_errors
+    pg, po = b.get_paths(0), oc.get_beads()
+    assert np.max(np.abs(pg - po)) < 1e-9
+
+
+def test_energy_estimator():
+    print("[test] energy_estimator", flush=True)
+    descs = [capi.cfg1_desc(), capi.cfg1_desc()]
+    b = capi.Batch(descs)
+    for c, d in enumerate(descs):
+        beads = capi.initial_beads(d, 100 + c) * 0.7
+        b.set_paths(c, beads)
+        oc = ol.Chain(d)
+        oc.set_beads(beads)
+        want = oc.energy()
+        got = b.energy()[c]
+        assert rel_close(got, want, RTOL, scale=10.0), (got, want)
+
+
+def test_histogram_bit_exact():
+    print("[test] histogram", flush=True)
+    desc = small_cfg2(128)
+    b = capi.Batch([desc, desc])
+    lib = capi.load_library()
+    import ctypes as C
+    for s1, s2 in ((0, 1), (0, 0)):
+        counts = np.zeros(2 * 64, dtype=np.uint64)
+        for c in range(2):
+            beads = capi.initial_beads(desc, 40 + c, replicate=False) * 1.3
+            b.set_paths(c, beads)
+        capi.check(lib.pq_pair_histogram(b.h, s1, s2, 0.0, 4.0, 64,
+                                         counts.ctypes.data_as(C.POINTER(C.c_uint64))))
+        for c in range(2):
+            oc = ol.Chain(desc)
+            oc.set_beads(b.get_paths(c))
+            want = oc.pair_histogram(s1, s2, 0.0, 4.0, 64)
+            assert np.array_equal(counts[c * 64:(c + 1) * 64], want)
+
+
+def test_batched_uneven_decisions():
+    """A slice of the uneven batch (SHO and free-particle nodes, N 2..20, M up to 256)."""
+    print("[test] batched_uneven", flush=True)
+    descs = [d for d in capi.cfg3_descs(24) if d.nslice <= 256][:8]
+    b = capi.Batch(descs)
+    ocs = []
+    for c, d in enumerate(descs):
+        beads = capi.initial_beads(d, 500 + c)
+        b.set_paths(c, beads)
+        oc = ol.Chain(d)
+        oc.set_beads(beads)
+        oc.rng_philox(99, c)
+        ocs.append(oc)
+    b.seed(99, 0)
+    b.run_sweeps(2)
+    b.sync()
+    for c, oc in enumerate(ocs):
+        oc.run_sweeps(2)
+        sg, so = b.stats(c), oc.stats()
+        assert sg.draws == so.draws, c
+        assert list(sg.accepts) == list(so.accepts), c
+        assert np.max(np.abs(b.get_paths(c) - oc.get_beads())) < 1e-9
+
+
+def test_mixed_precision_energy():
+    """Mixed precision tracks the FP64 chain: same uniforms, energy close."""
+    print("[test] mixed_precision", flush=True)
+    desc = small_cfg2(64)
+    b64 = capi.Batch([desc])
+    b32 = capi.Batch([desc], precision=capi.PQ_PREC_MIXED)
+    beads = capi.initial_beads(desc, 9)
+    for b in (b64, b32):
+        b.set_paths(0, beads)
+        b.seed(77, 0)
+        b.run_sweeps(20)
+        b.sync()
+    e64, e32 = b64.energy()[0], b32.energy()[0]
+    s64, s32 = b64.stats(0), b32.stats(0)
+    agree = sum(a == c for a, c in zip(s64.accepts, s32.accepts))
+    print("energies", e64, e32, list(s64.accepts), list(s32.accepts))
+    assert np.isfinite(e32)
+    assert agree >= 1
