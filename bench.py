@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Bead-updates/s of the bisection-move hot path on one MI355X.
+
+A step is one sweep of every chain of the batch: one fused sweep_kernel
+launch (choose section, node caches, every Sample's levels L..0, write back).
+Default workload: 1024 independent 2D quantum-dot chains, N ~ U{2..20} split
+between two spin species, M in {128,256,512,1024}, omega ~ U[0.5,2], SHO nodes
+on even chains and free-particle nodes on odd chains, 4-level bisection, run
+in mixed precision.  Beside it, on the same uniforms: the FP64 run of the same
+batch (energy error and FP64 batched rate) and the FP64 single-chain rate of
+the 3+3-electron SHONodes dot (M=400).  The CPU baseline is the oracle, the
+FP64 C restatement, on a bounded sample of the same chains on all host cores.
+Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pi-qmc_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import piqmc_capi as capi  # noqa: E402
+
+METRIC = "bead-updates/sec on 1 MI355X, mixed-batched vs FP64 at fixed energy error"
+HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_TFLOPS = 78.6
+
+
+def workload(name, nchain):
+    if name == "cfg3":
+        return capi.cfg3_descs(nchain), "1024 uneven 2D dots (N 2-20, M 128-1024, SHO/free nodes, L=4)"
+    if name == "cfg5":
+        return capi.cfg5_descs(nchain), "4096 uneven 3D dots (N 2-64, M 256-2048, SHO nodes, L=4)"
+    if name == "cfg2":
+        return [capi.cfg2_desc()] * nchain, "3+3 e- 2D dot, SHONodes, M=400, L=4"
+    if name == "cfg1":
+        return [capi.cfg1_desc()] * nchain, "2 e- 3D dot, Coulomb, M=200, L=3"
+    if name == "cfg4":
+        return [capi.cfg4_desc()] * nchain, "32 e + 32 h, box L=10, FreeParticleNodes 32x32, M=1024, L=5"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def bytes_per_update(d, w=8):
+    """Compulsory bytes per bead-update: section gather + moved-bead scatter,
+    w*D*[N(2^L+1) + (2^L-1)] / (2^L-1) (one moving particle)."""
+    L = d.nlevel
+    return w * d.ndim * (d.npart * ((1 << L) + 1) + ((1 << L) - 1)) / ((1 << L) - 1)
+
+
+def flops_per_update(d):
+    """FP64-equivalent flops per bead-update (sqrt/log/exp/div count 1):
+    2 lam_L [P(8D+6K+12) + (6D+2) + S(11D+8)] + (20D+10) + node terms."""
+    L, D = d.nlevel, d.ndim
+    lam = ((1 << (L + 1)) - 1) / ((1 << L) - 1)
+    P = d.npart - 1
+    K = d.coulomb.norder if d.coulomb.enabled else 0
+    S = 1 if d.nsho else 0
+    f = 2 * lam * (P * (8 * D + 6 * K + 12) + (6 * D + 2) + S * (11 * D + 8)) + (20 * D + 10)
+    if d.nnodes:
+        ns = max(d.species[d.nodes[a].species].count for a in range(d.nnodes))
+        f += ns * ns * (3 * D + 7) + 2 * ns ** 3 + ((1 << L) + 1) / ((1 << L) - 1) * 4 * ns * ns * (3 * D + 8)
+    return f
+
+
+def make_batch(descs, precision, seed):
+    b = capi.Batch(descs, precision=precision)
+    for c, d in enumerate(descs):
+        b.set_paths(c, capi.initial_beads(d, 1000 + c))
+    b.seed(seed, 0)
+    return b
+
+
+def all_stats(b):
+    st = (capi.ChainStats * len(b.descs))()
+    b.lib.pq_get_all_stats.restype = C.c_int
+    b.lib.pq_get_all_stats.argtypes = [C.c_void_p, C.POINTER(capi.ChainStats)]
+    capi.check(b.lib.pq_get_all_stats(b.h, st))
+    return st
+
+
+def timed_sweeps(b, steps):
+    """steps launches bracketed by HIP events on the batch stream; returns the
+    per-launch kernel times (ms) and the wall time of the whole timed region."""
+    lib = b.lib
+    lib.pq_run_sweeps_timed.restype = C.c_int
+    lib.pq_run_sweeps_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+    ms = []
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        v = C.c_float()
+        capi.check(lib.pq_run_sweeps_timed(b.h, 1, C.byref(v)))
+        ms.append(v.value)
+    b.sync()
+    return ms, time.perf_counter() - t0
+
+
+def run_gpu(descs, precision, steps, warmup, seed):
+    b = make_batch(descs, precision, seed)
+    b.run_sweeps(warmup)
+    b.sync()
+    capi.check(b.lib.pq_reset_stats(b.h))
+    ms, wall = timed_sweeps(b, steps)
+    st = all_stats(b)
+    bu = np.array([s.bead_updates for s in st], dtype=np.float64)
+    abu = np.array([s.accepted_bead_updates for s in st], dtype=np.float64)
+    energy = b.energy()
+    return b, dict(ms=ms, wall=wall, bu=bu, abu=abu, energy=energy)
+
+
+def cpu_baseline(descs, budget_s=12.0):
+    """The oracle (FP64 C restatement) on host cores: a bounded sample of the
+    workload's chains, one chain per thread (OpenMP)."""
+    import oracle_lib as ol
+    lib = ol.load()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    nch = min(len(descs), max(cores, 2 * cores))
+    sample = descs[:nch]
+    arr = capi.descs_array(sample)
+    beads = np.concatenate([capi.initial_beads(d, 1000 + c).ravel() for c, d in enumerate(sample)])
+    # calibrate one sweep, then scale the sweep count to the time budget
+    t0 = time.perf_counter()
+    bu = lib.oq_run_batch(arr, nch, capi.dptr(beads), 1, 7, 0, cores, None)
+    t1 = time.perf_counter() - t0
+    nsweep = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    bu = lib.oq_run_batch(arr, nch, capi.dptr(beads), nsweep, 7, 0, cores, None)
+    dt = time.perf_counter() - t0
+    return dict(value=bu / dt, unit="bead-updates/s", cores=cores, kind="port",
+                sample=f"{nch} chains of the workload x {nsweep} sweeps, oracle (FP64 C, "
+                       f"-O3 -march=x86-64-v3, OpenMP {cores} threads), {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cfg3")
+    ap.add_argument("--nchain", type=int, default=None)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-pair", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261203)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
+                                   "in DESIGN.md and not part of the benchmark", "n_gpus": args.gpus}))
+        sys.exit(2)
+    nchain = args.nchain or {"cfg3": 1024, "cfg5": 4096}.get(args.workload, 1)
+    descs, wdesc = workload(args.workload, nchain)
+    prec = capi.PQ_PREC_MIXED if args.precision == "mixed" else capi.PQ_PREC_FP64
+
+    _, main_run = run_gpu(descs, prec, args.steps, args.warmup, args.seed)
+    bu_total = float(main_run["bu"].sum())
+    value = bu_total / main_run["wall"]
+    kernel_ms = float(np.mean(main_run["ms"]))
+    w = 8
+    bytes_launch = float(sum(bu_c * bytes_per_update(d, w) for bu_c, d in zip(main_run["bu"], descs))) / args.steps
+    flops_launch = float(sum(bu_c * flops_per_update(d) for bu_c, d in zip(main_run["bu"], descs))) / args.steps
+    achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "bead-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * main_run["wall"] / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64" if prec == capi.PQ_PREC_MIXED else "fp64",
+        "data": "synthetic (seeded Gaussian beads, replicated across slices for node chains)",
+        "config": {"workload": f"{args.workload}: {wdesc}", "chains": len(descs),
+                   "precision": args.precision, "step": "one sweep of every chain (one kernel launch)"},
+        "accepted_bead_updates_per_s": float(main_run["abu"].sum()) / main_run["wall"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "sweep_kernel", "kernel_ms": kernel_ms,
+                     "fp64_equiv_tflops": flops_launch / (kernel_ms * 1e-3) / 1e12,
+                     "valu_frac_of_fp64_peak": flops_launch / (kernel_ms * 1e-3) / 1e12 / FP64_VECTOR_TFLOPS},
+    }
+    if not args.no_fp64_pair and prec == capi.PQ_PREC_MIXED:
+        _, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed)
+        de = main_run["energy"] - ref["energy"]
+        out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
+                               "kernel_ms": float(np.mean(ref["ms"]))}
+        out["energy_vs_fp64"] = {"mean_abs_diff_per_chain": float(np.mean(np.abs(de))),
+                                 "max_abs_diff": float(np.max(np.abs(de))),
+                                 "note": "paired runs, same seeds and uniforms, energy of the final paths"}
+        single = [capi.cfg2_desc()]
+        _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed)
+        r1 = float(one["bu"].sum()) / one["wall"]
+        out["fp64_single_chain"] = {"workload": "cfg2: 3+3 e- 2D dot, SHONodes, M=400", "value": r1}
+        out["ratio_vs_fp64_single_chain"] = value / r1
+    if not args.no_cpu:
+        cb = cpu_baseline(descs)
+        out["cpu_baseline"] = cb
+        out["ratio_vs_cpu"] = value / cb["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
