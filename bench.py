@@ -151,6 +151,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-pair", action="store_true")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="MiB copied by a plain device copy kernel first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
@@ -159,6 +161,13 @@ def main():
     nchain = args.nchain or {"cfg3": 1024, "cfg5": 4096}.get(args.workload, 1)
     descs, wdesc = workload(args.workload, nchain)
     prec = capi.PQ_PREC_MIXED if args.precision == "mixed" else capi.PQ_PREC_FP64
+    if args.calib_copy:
+        # a plain copy of known size: its FETCH_SIZE/WRITE_SIZE scale the counters
+        import torch
+        src = torch.ones(args.calib_copy * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = src.clone()
+        torch.cuda.synchronize()
+        del src, dst
 
     _, main_run = run_gpu(descs, prec, args.steps, args.warmup, args.seed)
     bu_total = float(main_run["bu"].sum())

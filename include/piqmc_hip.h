@@ -169,6 +169,11 @@ int pq_run_sweeps(pq_batch* batch, int nsweep);
    returns the kernel's elapsed time in milliseconds. */
 int pq_run_sweeps_timed(pq_batch* batch, int nsweep, float* ms);
 int pq_sync(pq_batch* batch);
+/* Schedule of the sampler kernel: team = threads per chain (64 = one wave,
+   256 = four waves, -1 automatic), inv_lds = node inverse cache in LDS (1),
+   global memory (0) or automatic (-1).  Results do not depend on it. */
+int pq_set_schedule(pq_batch* batch, int team, int inv_lds);
+int pq_get_schedule(const pq_batch* batch, int* team, int* inv_lds, int* lds_bytes);
 int pq_get_stats(pq_batch* batch, int chain, pq_chain_stats* out);
 int pq_get_all_stats(pq_batch* batch, pq_chain_stats* out);
 int pq_reset_stats(pq_batch* batch);

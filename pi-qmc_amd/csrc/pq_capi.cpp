@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -177,6 +178,7 @@ struct pq_batch {
   float* duf = nullptr;
   double* dbeads = nullptr;
   double* dinv = nullptr;
+  double* degr = nullptr;
   double* dfn = nullptr;
   double* dscale = nullptr;
   uint64_t* dcounter = nullptr;
@@ -204,6 +206,7 @@ static DevCtx make_ctx(const pq_batch* b) {
   x.tab_uf = b->duf;
   x.beads = b->dbeads;
   x.inv = b->dinv;
+  x.egr = b->degr;
   x.fnstate = b->dfn;
   x.scale = b->dscale;
   x.counter = b->dcounter;
@@ -263,6 +266,11 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
         c.spg_on[l][s][k] = (l <= c.L && pg_nterms(c.mass[s] / (2 * c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
         c.mpg_on[l][s][k] = (l <= c.L + 1 && pg_nterms(c.mass[s] / (c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
       }
+  // per-level constants of SpringAction (SpringAction.cc:71) and SHOAction (SHOAction.cc:28-33)
+  for (int l = 0; l < PQ_PG_LEVELS; ++l) {
+    c.div2[l] = 1.0 / (2.0 * c.tau * (1 << l));
+    for (int s = 0; s < c.nsp; ++s) c.spring_c[l][s] = 0.25 / (c.lambda[s] * c.tau * (1 << l));
+  }
   c.spring = d.spring;
   if (d.nsho < 0 || d.nsho > PQ_MAX_SHO) return fail(PQ_ERR_ARG, "nsho out of range");
   c.nsho = d.nsho;
@@ -272,6 +280,11 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     c.sho_ndim[a] = d.sho[a].ndim;
     c.sho_omega[a] = d.sho[a].omega == 0 ? 1.0 : d.sho[a].omega;
     for (int k = 0; k < 3; ++k) c.sho_center[a][k] = d.sho[a].center[k];
+    for (int l = 0; l < PQ_PG_LEVELS; ++l) {
+      const double wt = c.sho_omega[a] * c.tau * (1 << l);
+      c.sho_cosh[a][l] = std::cosh(wt);
+      c.sho_div1[a][l] = 1.0 / (2.0 * std::sinh(wt));
+    }
   }
   // CoulombAction constructor (src/action/CoulombAction.cc:45-90): rmin, rmax
   // and ngpts defaults set by the first interacting pair stick for the rest.
@@ -354,6 +367,11 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
       f.pg_nmax[k] = pg_nterms(alpha, len);
       f.pg_pref[k] = std::sqrt(PI / (alpha * len * len));
       f.pg_k[k] = 2 * PI / len;
+      f.pg_alpha[k] = alpha;
+      f.pg_len[k] = len;
+      // with alpha L^2 >= 20 the images beyond the nearest three weigh less than
+      // e^-40 of the leading one, so their sum equals the theta series to rounding
+      f.pg_img[k] = (alpha * len * len >= 20.0) ? 1 : 0;
     }
     if (nd.model != PQ_NODES_SHO && nd.model != PQ_NODES_FREE) return fail(PQ_ERR_ARG, "unknown node model");
     c.ns_max = std::max(c.ns_max, f.ns);
@@ -375,6 +393,50 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
 
 static int64_t align8(int64_t v) { return (v + 7) & ~(int64_t)7; }
 
+// LDS carve-up of the sampler workgroup, sized by the batch's largest chain.
+// team: threads per chain (64 or 256, -1 automatic); inv_lds: inverse cache in
+// LDS (1), in global memory (0) or automatic (-1).
+static void make_layout(pq_batch* b, int team_req, int inv_req) {
+  int64_t mS = 0, mSND = 0, mSD = 0, mF = 0, mInv = 0;
+  bool nodes = false;
+  for (auto& ch : b->hc) {
+    mS = std::max<int64_t>(mS, ch.S);
+    mSND = std::max<int64_t>(mSND, (int64_t)ch.S * ch.N * ch.D);
+    mSD = std::max<int64_t>(mSD, (int64_t)ch.S * ch.D);
+    mF = std::max<int64_t>(mF, fn_state_doubles(ch));
+    if (ch.nfn) { nodes = true; mInv = std::max<int64_t>(mInv, (int64_t)ch.S * ch.ns_max * ch.ns_max); }
+  }
+  int64_t o = align8(sizeof(ChainP));
+  b->lay.off_sec1 = (int)o; o += mSND * 8;
+  b->lay.off_sec2 = (int)o; o += mSND * 8;
+  b->lay.off_mov = (int)o; o += align8(mSD * 8);
+  b->lay.off_fst = (int)o; o += mF * 8;
+  b->lay.off_wdet = (int)o; o += PQ_WAVE * 8;
+  b->lay.off_wscale = (int)o; o += align8(mS * 8);
+  b->lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
+  b->lay.off_red = (int)o; o += 8 * 8;
+  b->lay.off_cnt = (int)o; o += 24 * 8;
+  b->lay.off_werr = (int)o; o += PQ_WAVE * 4;
+  b->lay.off_notmine = (int)o; o += PQ_MAX_NODES * 4;
+  o = align8(o);
+  b->lay.off_gj = (int)o; o += 2 * 8 * 256;
+  // automatic: keep the inverse cache in LDS while four chains still fit on one CU
+  bool inv_lds = nodes && o + mInv * 8 <= 40 * 1024;
+  if (inv_req == 0) inv_lds = false;
+  if (inv_req == 1) inv_lds = nodes && o + mInv * 8 <= 160 * 1024;
+  if (inv_lds) {
+    b->lay.off_inv = (int)o;
+    o += mInv * 8;
+  } else {
+    b->lay.off_inv = -1;
+  }
+  b->lay.bytes = (int)align8(o);
+  // automatic team: four waves per chain when node matrices dominate, else one
+  int team = nodes ? 256 : PQ_WAVE;
+  if (team_req == 64 || team_req == 256) team = team_req;
+  b->lay.team = team;
+}
+
 extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int precision, int device,
                                pq_batch** out) {
   if (!chains || nchain < 1 || !out) return fail(PQ_ERR_ARG, "pq_batch_create: bad arguments");
@@ -391,7 +453,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   b->hc.resize(nchain);
   b->D = chains[0].ndim;
   std::map<std::tuple<double, double, double, double, double, int, int>, int> tabs;
-  int64_t beads = 0, inv = 0, fnst = 0;
+  int64_t beads = 0, inv = 0, fnst = 0, egr = 0;
   int ns_max = 1;
   for (int c = 0; c < nchain; ++c) {
     if (chains[c].ndim != b->D) { delete b; return fail(PQ_ERR_ARG, "all chains of a batch share ndim"); }
@@ -400,44 +462,18 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     ns_max = std::max(ns_max, b->hc[c].ns_max);
   }
   b->G = ns_max <= 4 ? 4 : (ns_max <= 16 ? 16 : 32);
-  int lds = 0;
   for (int c = 0; c < nchain; ++c) {
     ChainP& ch = b->hc[c];
     ch.bead_off = beads;
     beads += (int64_t)ch.M * ch.N * ch.D;
     ch.inv_off = inv;
-    if (ch.nfn) inv += (int64_t)ch.S * b->G * b->G;
+    if (ch.nfn) inv += (int64_t)ch.S * ch.ns_max * ch.ns_max;
+    ch.egr_off = egr;
+    if (ch.nfn) egr += (int64_t)ch.S * ch.ns_max * ch.ns_max * ch.D;
     ch.fn_off = fnst;
     fnst += fn_state_doubles(ch);
-    // sampler LDS: chain block, sections, moving beads, node state, scratch
-    int64_t o = align8(sizeof(ChainP));
-    int64_t sec = (int64_t)ch.S * ch.N * ch.D * 8;
-    int64_t need = o + sec + (ch.nfn ? sec : 0) + align8(ch.S * ch.D * 8) + fn_state_doubles(ch) * 8 +
-                   PQ_WAVE * 8 + PQ_WAVE * 4 + align8(ch.S * 8) + PQ_MAX_NODES * 8 + PQ_MAX_NODES * 4 + 64;
-    lds = std::max<int>(lds, (int)need);
   }
-  // one layout for the batch, sized by its largest chain
-  {
-    int64_t mS = 0, mSND = 0, mSD = 0, mF = 0;
-    for (auto& ch : b->hc) {
-      mS = std::max<int64_t>(mS, ch.S);
-      mSND = std::max<int64_t>(mSND, (int64_t)ch.S * ch.N * ch.D);
-      mSD = std::max<int64_t>(mSD, (int64_t)ch.S * ch.D);
-      mF = std::max<int64_t>(mF, fn_state_doubles(ch));
-    }
-    int64_t o = align8(sizeof(ChainP));
-    b->lay.off_sec1 = (int)o; o += mSND * 8;
-    b->lay.off_sec2 = (int)o; o += mSND * 8;
-    b->lay.off_mov = (int)o; o += align8(mSD * 8);
-    b->lay.off_fst = (int)o; o += mF * 8;
-    b->lay.off_wdet = (int)o; o += PQ_WAVE * 8;
-    b->lay.off_wscale = (int)o; o += align8(mS * 8);
-    b->lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
-    b->lay.off_werr = (int)o; o += PQ_WAVE * 4;
-    b->lay.off_notmine = (int)o; o += PQ_MAX_NODES * 4;
-    b->lay.bytes = (int)align8(o);
-  }
-  (void)lds;
+  make_layout(b, -1, -1);
   if (b->lay.bytes > 160 * 1024) {
     int n = b->lay.bytes;
     delete b;
@@ -471,6 +507,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
             dalloc((void**)&b->duf, sizeof(float) * b->hu.size()) &&
             dalloc((void**)&b->dbeads, sizeof(double) * beads) &&
             dalloc((void**)&b->dinv, sizeof(double) * inv) &&
+            dalloc((void**)&b->degr, sizeof(double) * egr) &&
             dalloc((void**)&b->dfn, sizeof(double) * fnst) &&
             dalloc((void**)&b->dscale, sizeof(double) * nchain * PQ_MAX_NODES) &&
             dalloc((void**)&b->dcounter, sizeof(uint64_t) * nchain) &&
@@ -500,7 +537,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn,
-                  b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr};
+                  b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (b->ev0) hipEventDestroy(b->ev0);
@@ -583,6 +620,25 @@ extern "C" int pq_set_uniforms(pq_batch* b, const double* u, int64_t per_chain) 
   b->inj_per_chain = per_chain;
   b->rng_mode = PQ_RNG_INJECTED;
   HIPCHK(hipMemset(b->dcounter, 0, sizeof(uint64_t) * b->nchain));
+  return PQ_OK;
+}
+
+extern "C" int pq_set_schedule(pq_batch* b, int team, int inv_lds) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (team != -1 && team != 64 && team != 256) return fail(PQ_ERR_ARG, "team must be 64, 256 or -1");
+  if (inv_lds < -1 || inv_lds > 1) return fail(PQ_ERR_ARG, "inv_lds must be -1, 0 or 1");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  make_layout(b, team, inv_lds);
+  if (b->lay.bytes > 160 * 1024) return fail(PQ_ERR_ARG, "schedule needs %d bytes of LDS", b->lay.bytes);
+  return PQ_OK;
+}
+
+extern "C" int pq_get_schedule(const pq_batch* b, int* team, int* inv_lds, int* lds_bytes) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (team) *team = b->lay.team;
+  if (inv_lds) *inv_lds = b->lay.off_inv >= 0;
+  if (lds_bytes) *lds_bytes = b->lay.bytes;
   return PQ_OK;
 }
 
@@ -849,21 +905,23 @@ extern "C" int pq_node_evaluate(pq_batch* b, int c, int inode, int nmat, const d
   const int64_t sl = (int64_t)ch.N * ch.D;
   HIPCHK(hipSetDevice(b->device));
   const size_t nd = (size_t)nmat * sl;
-  const size_t bytes = sizeof(double) * (2 * nd + (size_t)nmat * b->G * b->G + nmat + (size_t)nmat * 2 * ns) +
+  const size_t nsm = (size_t)ch.ns_max;
+  const size_t bytes = sizeof(double) * (2 * nd + (size_t)nmat * nsm * nsm * (1 + ch.D) + nmat + (size_t)nmat * 2 * ns) +
                        sizeof(int) * nmat + 256;
   void* p;
   if (int rc = scratch(b, bytes, &p)) return rc;
   double* dr1 = (double*)p;
   double* dr2 = dr1 + nd;
   double* dinv = dr2 + nd;
-  double* ddet = dinv + (size_t)nmat * b->G * b->G;
+  double* degr = dinv + (size_t)nmat * nsm * nsm;
+  double* ddet = degr + (size_t)nmat * nsm * nsm * ch.D;
   double* ddist = ddet + nmat;
   int* derr = (int*)(ddist + (size_t)nmat * 2 * ns);
   HIPCHK(hipMemcpyAsync(dr1, r1, sizeof(double) * nd, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipMemcpyAsync(dr2, r2, sizeof(double) * nd, hipMemcpyHostToDevice, b->stream));
   DevCtx x = make_ctx(b);
   HIPCHK(launch_node_eval(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, inode, nmat, dr1, dr2, dinv,
-                          ddet, derr, ddist, b->stream));
+                          degr, ddet, derr, ddist, b->stream));
   std::vector<double> hd((size_t)nmat * 2 * ns);
   HIPCHK(hipMemcpyAsync(det, ddet, sizeof(double) * nmat, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipMemcpyAsync(err, derr, sizeof(int) * nmat, hipMemcpyDeviceToHost, b->stream));

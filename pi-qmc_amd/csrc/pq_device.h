@@ -1,10 +1,10 @@
 // pq_device.h -- device building blocks of the bisection-move hot path.
 //
-// Execution model: one 64-lane wavefront owns one Markov chain.  Every routine
-// here is called by all 64 lanes of a single-wave workgroup with wave-uniform
-// arguments and returns wave-uniform results; lanes split the independent work
-// (moved beads, links, partner x link pairs, node matrices, matrix rows) and
-// combine it with cross-lane reductions.  Section beads live in LDS for the
+// Execution model: one workgroup (a team of 64 or 256 threads) owns one
+// Markov chain.  Every routine is called by all threads of the team with
+// team-uniform arguments and returns team-uniform results; threads split the
+// independent work (moved beads, links, partner x link pairs, node matrices,
+// matrix rows) and combine it with cross-lane and LDS reductions.  Section beads live in LDS for the
 // fused sampler and in global memory for the call-level kernels; the routines
 // only see plain pointers.
 //
@@ -29,6 +29,7 @@ struct DevCtx {
   const float* tab_uf;
   double* beads;
   double* inv;            // per-chain [S][ns_max][ns_max] inverse cache
+  double* egr;            // per-chain [S][ns_max][ns_max][D] density-matrix gradient factors
   double* fnstate;        // per-chain fixed-node state (call-level API)
   double* scale;          // [nchain][PQ_MAX_NODES]
   uint64_t* counter;      // uniforms consumed per chain
@@ -158,12 +159,6 @@ __device__ __forceinline__ double pg_eval(const PG& p, double x) {
   return pg_eval(p.nome, p.pref, p.k, p.nmax, x, nullptr);
 }
 
-__device__ __forceinline__ int species_of(const ChainP& c, int i) {
-  int s = 0;
-  for (int k = 1; k < c.nsp; ++k) s = (i >= c.ifirst[k]) ? k : s;
-  return s;
-}
-
 // fast transcendental overloads for the arithmetic type
 __device__ __forceinline__ double r_log(double v) { return log(v); }
 __device__ __forceinline__ float r_log(float v) { return __logf(v); }
@@ -171,6 +166,37 @@ __device__ __forceinline__ double r_sqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float r_sqrt(float v) { return __fsqrt_rn(v); }
 __device__ __forceinline__ double r_exp(double v) { return exp(v); }
 __device__ __forceinline__ float r_exp(float v) { return __expf(v); }
+
+// exp(-alpha x^2) summed over the three nearest images x, x-L, x+L, with its
+// derivative; equal to the periodic Gaussian's theta series (Poisson summation)
+// to rounding when alpha L^2 >= 20 and |x| <= L/2.
+template <class R>
+__device__ __forceinline__ R pg_images(R alpha, R len, R x, R* grad) {
+  R v = 0, g = 0;
+#pragma unroll
+  for (int m = -1; m <= 1; ++m) {
+    const R y = x - (R)m * len;
+    const R e = r_exp(-alpha * y * y);
+    v += e;
+    g += (R)-2 * alpha * y * e;
+  }
+  if (grad) *grad = g;
+  return v;
+}
+
+// g / (v + tiny) of FreeParticleNodes.cc:201 (tiny = 1e-300); FP32 uses its
+// smallest normal number, the nearest representable guard
+template <class R> __device__ __forceinline__ R ratio(R g, R v);
+template <> __device__ __forceinline__ double ratio<double>(double g, double v) { return g / (v + 1e-300); }
+template <> __device__ __forceinline__ float ratio<float>(float g, float v) {
+  return g * __frcp_rn(v + 1.17549435e-38f);
+}
+
+__device__ __forceinline__ int species_of(const ChainP& c, int i) {
+  int s = 0;
+  for (int k = 1; k < c.nsp; ++k) s = (i >= c.ifirst[k]) ? k : s;
+  return s;
+}
 
 template <class R> __device__ __forceinline__ const R* table_ptr(const DevCtx& x, const TableP& t);
 template <> __device__ __forceinline__ const double* table_ptr<double>(const DevCtx& x, const TableP& t) {

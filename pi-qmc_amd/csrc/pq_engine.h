@@ -1,6 +1,7 @@
-// pq_engine.h -- one wavefront evaluates one chain's bisection move.
+// pq_engine.h -- a team of NT threads (1 or 4 wavefronts) evaluates one
+// chain's bisection move.
 //
-// Eng<D,G,R> bundles the views of one chain (section beads, moving beads,
+// Eng<D,G,R,NT> bundles the views of one chain (section beads, moving beads,
 // fixed-node state, inverse cache) with the routines of one level of
 // MultiLevelSampler/DoubleMLSampler::tryMove:
 //   mover()            FreeMover / FreeMoverPBC::makeMove
@@ -9,7 +10,9 @@
 //   fn_diff()          CompositeDoubleAction::getActionDifference
 //   fn_accept()        FixedNodeAction::acceptLastMove
 // G is the lane-group width of the node-matrix Gauss-Jordan inversion: one
-// matrix row per lane, 64/G matrices per pass, G >= largest fermion species.
+// matrix row per lane, NT/G matrices per pass, G >= largest fermion species.
+// All routines are called by every thread of the team with team-uniform
+// arguments and return team-uniform results.
 #pragma once
 #include "pq_device.h"
 
@@ -17,75 +20,104 @@ namespace pq {
 
 // Row-per-lane Gauss-Jordan inversion with partial pivoting of one G x G
 // matrix per G-lane group.  Lane r holds row r of A (a) and of the identity
-// (b); on exit b holds row r of A^-1 and det = prod(pivots) * (-1)^swaps,
-// which equals the LU determinant with the same (first maximal) pivots.
-// Rows and columns >= n are an identity block.
+// (b).  Rows are not moved: each lane tracks the position its row would have
+// in the row-swapping elimination (mypos), the pivot of step k is the largest
+// |a[k]| among positions >= k with the lowest position on ties (LAPACK's
+// idamax), and every virtual swap flips the determinant's sign.  The pivot
+// row reaches the group through LDS (row: 2G doubles per group).  On exit the
+// lane with mypos = k holds row k of A^-1 in b, and det = prod(pivots) *
+// (-1)^swaps, the LU determinant with the same pivots.  Rows and columns >= n
+// are an identity block.
 template <int G>
 __device__ __forceinline__ void gj_invert(double (&a)[G], double (&b)[G], int r, int n,
-                                          double& det, int& err) {
+                                          double* row, double& det, int& err, int& mypos) {
   det = 1.0;
   err = 0;
+  mypos = r;
 #pragma unroll
   for (int k = 0; k < G; ++k) {
     if (k < n) {
-      double v = (r >= k) ? fabs(a[k]) : -1.0;
-      int p = r;
+      double v = (mypos >= k) ? fabs(a[k]) : -1.0;
+      int key = (mypos << 8) | r;
 #pragma unroll
       for (int o = G / 2; o > 0; o >>= 1) {
-        double v2 = __shfl_xor(v, o, G);
-        int p2 = __shfl_xor(p, o, G);
-        if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
+        const double v2 = __shfl_xor(v, o, G);
+        const int k2 = __shfl_xor(key, o, G);
+        if (v2 > v || (v2 == v && k2 < key)) { v = v2; key = k2; }
       }
+      const int p = key & 0xff;
+      const int ppos = key >> 8;
       if (v == 0.0) err = 1;
-      if (__any(p != k)) {
-        int src = (r == k) ? p : ((r == p) ? k : r);
+      if (ppos != k) det = -det;
+      if (r == p) {
 #pragma unroll
-        for (int cc = 0; cc < G; ++cc) {
-          a[cc] = __shfl(a[cc], src, G);
-          b[cc] = __shfl(b[cc], src, G);
-        }
+        for (int cc = 0; cc < G; ++cc) { row[cc] = a[cc]; row[G + cc] = b[cc]; }
       }
-      if (p != k) det = -det;
-      double piv = __shfl(a[k], k, G);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const double piv = row[k];
       det *= piv;
-      double f = (r == k) ? 0.0 : a[k] / piv;
-      double ipiv = 1.0 / piv;
+      const bool ok = v != 0.0;
+      // one division per step: the pivot row is scaled by the reciprocal
+      const double ipiv = ok ? 1.0 / piv : 0.0;
+      const double f = a[k] * ipiv;
 #pragma unroll
       for (int cc = 0; cc < G; ++cc) {
-        double pa = __shfl(a[cc], k, G);
-        double pb = __shfl(b[cc], k, G);
-        if (r == k) {
-          a[cc] = (v == 0.0) ? a[cc] : a[cc] / piv;
-          b[cc] = (v == 0.0) ? b[cc] : b[cc] / piv;
+        const double pa = row[cc], pb = row[G + cc];
+        if (r == p) {
+          a[cc] = ok ? a[cc] * ipiv : a[cc];
+          b[cc] = ok ? b[cc] * ipiv : b[cc];
         } else {
-          a[cc] = (v == 0.0) ? a[cc] : a[cc] - f * pa;
-          b[cc] = (v == 0.0) ? b[cc] : b[cc] - f * pb;
+          a[cc] = ok ? a[cc] - f * pa : a[cc];
+          b[cc] = ok ? b[cc] - f * pb : b[cc];
         }
       }
-      (void)ipiv;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      // the row at position k moves to the pivot's old position
+      if (r == p) mypos = k;
+      else if (mypos == k) mypos = ppos;
     }
   }
 }
 
-template <int D, int G, class R>
+template <int D, int G, class R, int NT>
 struct Eng {
   const DevCtx& x;
   const ChainP& c;
   int chain;
+  int tid;
   int lane;
+  int wave;
+  int ins;          // row stride of the inverse cache (largest node species)
+  double* red;      // [NT/64] block-reduction scratch
   double* sec1;     // [S][N][D]
   double* sec2;     // [S][N][D]
   double* mov;      // [S][D]
   double* fst;      // fixed-node state block
-  double* inv;      // [S][G][G]
+  double* inv;      // [S][ins][ins] latest inverse per section slice
+  double* egr;      // [S][ins][ins][D] gradient factors of the latest matrix elements
   double* scale;    // [nfn]
   int* notmine;     // [nfn]
-  double* wdet;     // [64] scratch: determinants of a pass
-  int* werr;        // [64] scratch: error flags of a pass
+  double* wdet;     // [NT/G] scratch: determinants of a pass
+  int* werr;        // [NT/G] scratch: error flags of a pass
   double* wscale;   // [S] scale used per slice in initialize
+  double* gjrow;    // [NT/G][2G] pivot rows of the Gauss-Jordan groups
 
   __device__ Eng(const DevCtx& x_, const ChainP& c_, int chain_)
-      : x(x_), c(c_), chain(chain_), lane(lane_id()) {}
+      : x(x_), c(c_), chain(chain_), tid(threadIdx.x), lane(threadIdx.x & 63), wave(threadIdx.x >> 6),
+        ins(c_.ns_max > 0 ? c_.ns_max : 1), red(nullptr) {}
+
+  // sum over the NT threads of the team; every thread gets the result
+  __device__ __forceinline__ double bsum(double v) {
+    v = wave_sum(v);
+    if constexpr (NT == PQ_WAVE) return v;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < NT / PQ_WAVE; ++w) s += red[w];
+    __syncthreads();
+    return s;
+  }
 
   __device__ __forceinline__ double U(uint64_t k) const { return uniform(x, chain, k); }
 
@@ -106,7 +138,7 @@ struct Eng {
     *ndraws = nmoved * pd;
     const int sp = species_of(c, i);
     double lnT = 0;
-    for (int kk = lane; kk < nmoved; kk += PQ_WAVE) {
+    for (int kk = tid; kk < nmoved; kk += NT) {
       const int t = s * (2 * kk + 1);
       uint64_t kb = k0 + (uint64_t)kk * pd;
       double g[4];
@@ -233,7 +265,7 @@ struct Eng {
         lnT += log(temp);
       }
     }
-    return wave_sum(lnT);
+    return bsum(lnT);
   }
 
   // ---------------------------------------------------- action layer
@@ -245,14 +277,14 @@ struct Eng {
     const int nl = (c.S - 1) >> level;
     const int sp = species_of(c, i);
     Acc<R> acc;
-    for (int k = lane; k < nl; k += PQ_WAVE) {
+    for (int k = tid; k < nl; k += NT) {
       const int t = s * (k + 1);
       const double* m1 = mov + t * D;
       const double* m0 = mov + (t - s) * D;
       const double* o1 = sb(sec1, t, i);
       const double* o0 = sb(sec1, t - s, i);
       if (c.spring) {
-        const R inv2Sigma2 = (R)(0.25 / (c.lambda[sp] * c.tau * s));
+        const R inv2Sigma2 = (R)c.spring_c[level][sp];
         double dn[D], dd[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) { dn[d] = m1[d] - m0[d]; dd[d] = o1[d] - o0[d]; }
@@ -276,9 +308,8 @@ struct Eng {
         const int ssp = c.sho_sp[a];
         if (i < c.ifirst[ssp] || i >= c.ifirst[ssp] + c.count[ssp]) continue;
         const R m = (R)c.mass[ssp], omega = (R)c.sho_omega[a];
-        const double wt = c.sho_omega[a] * c.tau * s;
-        const R coshwt = (R)cosh(wt), div1 = (R)(1.0 / (2.0 * sinh(wt)));
-        const R div2 = (R)(1.0 / (2.0 * c.tau * s));
+        const R coshwt = (R)c.sho_cosh[a][level], div1 = (R)c.sho_div1[a][level];
+        const R div2 = (R)c.div2[level];
         const int d0 = D - (c.sho_ndim[a] ? c.sho_ndim[a] : D);
         R v[2];
 #pragma unroll
@@ -310,7 +341,7 @@ struct Eng {
       const bool offdiag = level < 3 && T.K1 > 1;
       const int jb = c.ifirst[js], nj = c.count[js];
       const int items = nj * nl;
-      for (int it = lane; it < items; it += PQ_WAVE) {
+      for (int it = tid; it < items; it += NT) {
         const int j = jb + it / nl;
         if (j == i) continue;
         const int t = s * (it % nl + 1);
@@ -342,7 +373,7 @@ struct Eng {
         acc.add(-vals[1]);
       }
     }
-    return wave_sum(acc.value());
+    return bsum(acc.value());
   }
 
   // -------------------------------------------------------- node model
@@ -370,39 +401,59 @@ struct Eng {
   __device__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, int i, double scl) {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
-    const int g = lane / G, r = lane % G;
-    constexpr int NG = PQ_WAVE / G;
+    const int g = tid / G, r = tid % G;
+    constexpr int NG = NT / G;
+    constexpr int GPW = PQ_WAVE / G;   // groups per wave
+    double* row = gjrow + (int64_t)g * 2 * G;
     for (int base = 0; base < nt; base += NG) {
-      const int m = base + g;
-      const bool live = m < nt;
-      const int t = t0 + (live ? m : 0) * step;
-      double a[G], b[G];
+      // a wave whose groups all lie past the last matrix has nothing to do
+      if (base + wave * GPW < nt) {
+        const int m = base + g;
+        const bool live = m < nt;
+        const int t = t0 + (live ? m : 0) * step;
+        double a[G], b[G];
 #pragma unroll
-      for (int cc = 0; cc < G; ++cc) { a[cc] = (cc == r) ? 1.0 : 0.0; b[cc] = a[cc]; }
-      if (r < n) {
-        const double* ri = sb(sec2, t, f.ifirst + r);
+        for (int cc = 0; cc < G; ++cc) { a[cc] = (cc == r) ? 1.0 : 0.0; b[cc] = a[cc]; }
+        if (r < n) {
+          const double* ri = sb(sec2, t, f.ifirst + r);
+          double* eg = egr + ((int64_t)t * ins + r) * ins * D;
 #pragma unroll
-        for (int cc = 0; cc < G; ++cc) {
-          if (cc < n) {
-            const double* rj = r1pos(t, f.ifirst + cc, use_mov, i);
-            a[cc] = elem(f, rj, ri, scl);
+          for (int cc = 0; cc < G; ++cc) {
+            if (cc < n) {
+              const double* rj = r1pos(t, f.ifirst + cc, use_mov, i);
+              double gf[D];
+              a[cc] = elem(f, rj, ri, scl, gf);
+              if (live) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) eg[cc * D + d] = gf[d];
+              }
+            }
           }
         }
-      }
-      double det;
-      int err;
-      gj_invert<G>(a, b, r, n, det, err);
-      if (live) {
-        double* dst = inv + ((int64_t)t * G + r) * G;
+        double det;
+        int err, pos;
+        gj_invert<G>(a, b, r, n, row, det, err, pos);
+        if (live) {
+          // a singular matrix can leave a padding row in a leading position
+          if (r < n && pos < n) {
+            double* dst = inv + ((int64_t)t * ins + pos) * ins;
 #pragma unroll
-        for (int cc = 0; cc < G; ++cc) dst[cc] = b[cc];
-        if (r == 0) { wdet[m] = det; werr[m] = err; }
+            for (int cc = 0; cc < G; ++cc)
+              if (cc < n) dst[cc] = b[cc];
+          }
+          if (r == 0) { wdet[m] = det; werr[m] = err; }
+        }
       }
     }
     __syncthreads();
   }
 
-  __device__ __forceinline__ double elem(const FnP& f, const double* rj, const double* ri, double scl) const {
+  // density-matrix element rho(r2_i, r1_j) times the scale, and the factors the
+  // nodal distance needs: SHONodes the unscaled element (gf[0]);
+  // FreeParticleNodes the gradient (PG'(x_d)/(PG(x_d)+1e-300)) prod_k PG(x_k)
+  // in the order of FreeParticleNodes.cc:196-207.
+  __device__ __forceinline__ double elem(const FnP& f, const double* rj, const double* ri, double scl,
+                                         double* gf) const {
     if (f.model == PQ_NODES_SHO) {
       R rj2 = 0, ri2 = 0, rirj = 0;
 #pragma unroll
@@ -410,17 +461,35 @@ struct Eng {
         R aj = (R)rj[d], ai = (R)ri[d];
         rj2 += aj * aj; ri2 += ai * ai; rirj += ai * aj;
       }
-      R e = r_exp(-(R)f.c * ((ri2 + rj2) * (R)f.coshwt - (R)2.0 * rirj));
+      const R e = r_exp(-(R)f.c * ((ri2 + rj2) * (R)f.coshwt - (R)2.0 * rirj));
+      gf[0] = (double)e;
+#pragma unroll
+      for (int d = 1; d < D; ++d) gf[d] = 0.0;
       return scl * (double)e;
     }
-    double delta[D];
+    double delta[D], value[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) delta[d] = rj[d] - ri[d];
     pbc<D>(c, delta);
     double ear2 = scl;
 #pragma unroll
-    for (int d = 0; d < D; ++d)
-      ear2 *= pg_eval(f.pg_nome[d], f.pg_pref[d], f.pg_k[d], f.pg_nmax[d], delta[d], nullptr);
+    for (int d = 0; d < D; ++d) {
+      if (f.pg_img[d]) {
+        R g2;
+        const R v2 = pg_images<R>((R)f.pg_alpha[d], (R)f.pg_len[d], (R)delta[d], &g2);
+        value[d] = (double)v2;
+        gf[d] = (double)ratio<R>(g2, v2);
+      } else {
+        double gr;
+        value[d] = pg_eval(f.pg_nome[d], f.pg_pref[d], f.pg_k[d], f.pg_nmax[d], delta[d], &gr);
+        gf[d] = gr / (value[d] + 1e-300);
+      }
+      ear2 *= value[d];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int d = 0; d < D; ++d) gf[d] *= value[k];
     return ear2;
   }
 
@@ -432,25 +501,23 @@ struct Eng {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
     const int items = nt * 2 * n;
-    for (int it = lane; it < items; it += PQ_WAVE) {
+    for (int it = tid; it < items; it += NT) {
       const int t = t0 + it / (2 * n);
       const int pass = (it / n) & 1;
       const int aa = it % n;
       const double sc = scl ? scl[t] : scl0;
-      const double* iv = inv + (int64_t)t * G * G;
+      const double* iv = inv + (int64_t)t * ins * ins;
       double lg[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) lg[d] = 0;
       for (int bb = 0; bb < n; ++bb) {
         const int jp = pass == 0 ? aa : bb, ip = pass == 0 ? bb : aa;
-        const double* rj = r1pos(t, f.ifirst + jp, use_mov, i);
-        const double* ri = sb(sec2, t, f.ifirst + ip);
-        const double w = iv[jp * G + ip];
+        const double w = iv[jp * ins + ip];
+        const double* gf = egr + (((int64_t)t * ins + ip) * ins + jp) * D;
         if (f.model == PQ_NODES_SHO) {
-          double ri2 = 0, rj2 = 0, rirj = 0;
-#pragma unroll
-          for (int d = 0; d < D; ++d) { ri2 += ri[d] * ri[d]; rj2 += rj[d] * rj[d]; rirj += ri[d] * rj[d]; }
-          double e = exp(-f.c * ((ri2 + rj2) * f.coshwt - 2 * rirj));
+          const double* rj = r1pos(t, f.ifirst + jp, use_mov, i);
+          const double* ri = sb(sec2, t, f.ifirst + ip);
+          const double e = gf[0];
 #pragma unroll
           for (int d = 0; d < D; ++d) {
             double gg = pass == 0 ? 2 * f.c * (ri[d] - f.coshwt * rj[d]) * e
@@ -458,22 +525,9 @@ struct Eng {
             lg[d] += w * sc * gg;
           }
         } else {
-          double delta[D], value[D], grad[D];
+          // the reversed separation negates the odd gradient factor exactly
 #pragma unroll
-          for (int d = 0; d < D; ++d) delta[d] = pass == 0 ? rj[d] - ri[d] : ri[d] - rj[d];
-          pbc<D>(c, delta);
-#pragma unroll
-          for (int d = 0; d < D; ++d) {
-            double gr;
-            value[d] = pg_eval(f.pg_nome[d], f.pg_pref[d], f.pg_k[d], f.pg_nmax[d], delta[d], &gr);
-            grad[d] = gr / (value[d] + 1e-300);
-          }
-#pragma unroll
-          for (int k = 0; k < D; ++k)
-#pragma unroll
-            for (int d = 0; d < D; ++d) grad[d] *= value[k];
-#pragma unroll
-          for (int d = 0; d < D; ++d) lg[d] += w * grad[d] * sc;
+          for (int d = 0; d < D; ++d) lg[d] += w * (pass == 0 ? gf[d] : -gf[d]) * sc;
         }
       }
       double g2 = 0;
@@ -492,7 +546,7 @@ struct Eng {
   // the number of crossed or singular slices.
   __device__ int fn_initialize() {
     int nerr = 0;
-    constexpr int NG = PQ_WAVE / G;
+    constexpr int NG = NT / G;
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
       double scl = scale[fa];
@@ -510,7 +564,7 @@ struct Eng {
         }
         const int ncommit = (bad < 0) ? nt : bad;
         for (int m = 0; m < ncommit; ++m) {
-          if (lane == 0) { dm[t + m] = wdet[m]; wscale[t + m] = scl; }
+          if (tid == 0) { dm[t + m] = wdet[m]; wscale[t + m] = scl; }
           if (t + m == 0) dm0 = wdet[m];
           if (werr[m] || wdet[m] * dm0 <= 1e-200) errs++;
         }
@@ -527,7 +581,7 @@ struct Eng {
           if (er || passes >= 200) {
             // the reference repeats an erring evaluation unchanged; the
             // oracle stops after 200 passes: commit the slice as it stands
-            if (lane == 0) { dm[t] = dt; wscale[t] = scl; }
+            if (tid == 0) { dm[t] = dt; wscale[t] = scl; }
             if (t == 0) dm0 = dt;
             if (er || dt * dm0 <= 1e-200) errs++;
             ++t;
@@ -537,15 +591,15 @@ struct Eng {
         __syncthreads();
       }
       nerr += errs;
-      if (lane == 0) scale[fa] = scl;
+      if (tid == 0) scale[fa] = scl;
       __syncthreads();
       if (f.nodal) node_distances(fa, 0, c.S, false, -1, wscale, 0.0, fn_dist(fa));
       // newDMValue(0)=dmValue(0); newDist(0)=dist(0)
       const int n = f.ns;
       double* nd = fn_newdist(fa);
       const double* od = fn_dist(fa);
-      for (int k = lane; k < 2 * n; k += PQ_WAVE) nd[k] = od[k];
-      if (lane == 0) fn_newdm(fa)[0] = dm[0];
+      for (int k = tid; k < 2 * n; k += NT) nd[k] = od[k];
+      if (tid == 0) fn_newdm(fa)[0] = dm[0];
       __syncthreads();
     }
     return nerr;
@@ -558,7 +612,7 @@ struct Eng {
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
       const bool mine = i >= f.ifirst && i < f.ifirst + f.ns;
-      if (lane == 0) notmine[fa] = mine ? 0 : 1;
+      if (tid == 0) notmine[fa] = mine ? 0 : 1;
       if (!mine) continue;
       const int nStride = 1 << (level + 1);
       const int t0 = nStride / 2;
@@ -569,7 +623,7 @@ struct Eng {
       double* ndm = fn_newdm(fa);
       for (int m = 0; m < nt; ++m) {
         if (werr[m] || wdet[m] * dm0 <= 1e-200) { crossed = 1; break; }
-        if (lane == 0) ndm[t0 + m * nStride] = wdet[m];
+        if (tid == 0) ndm[t0 + m * nStride] = wdet[m];
       }
       __syncthreads();
       if (crossed) { total += 2e100; continue; }
@@ -580,7 +634,7 @@ struct Eng {
         const double* od = fn_dist(fa);
         const int n = f.ns;
         double part = 0;
-        for (int it = lane; it < (c.S - 1) * n; it += PQ_WAVE) {
+        for (int it = tid; it < (c.S - 1) * n; it += NT) {
           const int t = 1 + it / n, k = it % n;
 #pragma unroll
           for (int pass = 0; pass < 2; ++pass) {
@@ -589,7 +643,7 @@ struct Eng {
             part += log((1 - exp(-od[a1] * od[a0])) / (1 - exp(-nd[a1] * nd[a0])));
           }
         }
-        dA = wave_sum(part);
+        dA = bsum(part);
       }
       total += dA;
     }
@@ -605,8 +659,8 @@ struct Eng {
       const double* ndm = fn_newdm(fa);
       double* dd = fn_dist(fa);
       const double* nd = fn_newdist(fa);
-      for (int k = lane; k < c.S; k += PQ_WAVE) dm[k] = ndm[k];
-      for (int k = lane; k < 2 * c.S * n; k += PQ_WAVE) dd[k] = nd[k];
+      for (int k = tid; k < c.S; k += NT) dm[k] = ndm[k];
+      for (int k = tid; k < 2 * c.S * n; k += NT) dd[k] = nd[k];
     }
     __syncthreads();
   }

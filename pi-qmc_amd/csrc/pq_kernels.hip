@@ -18,24 +18,26 @@ namespace pq {
 // LDS carve-up of the sampler workgroup
 struct SweepLayout {
   int off_sec1, off_sec2, off_mov, off_fst, off_wdet, off_werr, off_wscale, off_scale, off_notmine;
+  int off_red, off_cnt, off_inv;   // off_inv < 0: inverse cache in global memory
+  int off_gj;
   int bytes;
 };
 
 __device__ __forceinline__ void copy_chain(ChainP* dst, const ChainP* src) {
   const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
   uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-  for (int k = lane_id(); k < (int)(sizeof(ChainP) / 4); k += PQ_WAVE) d[k] = s[k];
+  for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
   __syncthreads();
 }
 
-template <int D, int G, class R>
-__global__ __launch_bounds__(PQ_WAVE) void sweep_kernel(DevCtx x, int nsweep, SweepLayout lay) {
+template <int D, int G, class R, int NT>
+__global__ __launch_bounds__(NT) void sweep_kernel(DevCtx x, int nsweep, SweepLayout lay) {
   extern __shared__ double smem[];
   const int chain = x.order[blockIdx.x];
   ChainP& c = *reinterpret_cast<ChainP*>(smem);
   copy_chain(&c, &x.chains[chain]);
   char* base = reinterpret_cast<char*>(smem);
-  Eng<D, G, R> e(x, c, chain);
+  Eng<D, G, R, NT> e(x, c, chain);
   e.sec1 = reinterpret_cast<double*>(base + lay.off_sec1);
   e.sec2 = reinterpret_cast<double*>(base + lay.off_sec2);
   e.mov = reinterpret_cast<double*>(base + lay.off_mov);
@@ -45,17 +47,21 @@ __global__ __launch_bounds__(PQ_WAVE) void sweep_kernel(DevCtx x, int nsweep, Sw
   e.wscale = reinterpret_cast<double*>(base + lay.off_wscale);
   e.scale = reinterpret_cast<double*>(base + lay.off_scale);
   e.notmine = reinterpret_cast<int*>(base + lay.off_notmine);
-  e.inv = x.inv + c.inv_off;
-  const int lane = e.lane;
-  if (lane < c.nfn) e.scale[lane] = x.scale[(int64_t)chain * PQ_MAX_NODES + lane];
+  e.red = reinterpret_cast<double*>(base + lay.off_red);
+  e.gjrow = reinterpret_cast<double*>(base + lay.off_gj);
+  e.inv = (lay.off_inv >= 0) ? reinterpret_cast<double*>(base + lay.off_inv) : x.inv + c.inv_off;
+  e.egr = x.egr + c.egr_off;
+  // per-level try/accept counters, bead updates, accepted, node errors
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + lay.off_cnt);
+  const int tid = e.tid;
+  if (tid < 19) cnt[tid] = 0;
+  if (tid < c.nfn) e.scale[tid] = x.scale[(int64_t)chain * PQ_MAX_NODES + tid];
   __syncthreads();
 
   const int S = c.S, N = c.N, M = c.M, L = c.L;
   const int64_t slice = (int64_t)N * D;
   double* beads = x.beads + c.bead_off;
   uint64_t k = x.counter[chain];
-  uint64_t tries[8] = {0, 0, 0, 0, 0, 0, 0, 0}, accs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint64_t bu = 0, abu = 0, nerr = 0;
 
   for (int sw = 0; sw < nsweep; ++sw) {
     for (int isec = 0; isec < c.nsection; ++isec) {
@@ -75,7 +81,7 @@ __global__ __launch_bounds__(PQ_WAVE) void sweep_kernel(DevCtx x, int nsweep, Sw
         if (t0 > ihi) t0 = ihi;
       }
       // SerialPaths::getBeads (SerialPaths.cc:119-132), identity permutation
-      for (int64_t q = lane; q < S * slice; q += PQ_WAVE) {
+      for (int64_t q = tid; q < S * slice; q += NT) {
         int kk = (int)(q / slice);
         int64_t rem = q - kk * slice;
         int t = kk + t0;
@@ -88,24 +94,29 @@ __global__ __launch_bounds__(PQ_WAVE) void sweep_kernel(DevCtx x, int nsweep, Sw
         }
       }
       __syncthreads();
-      if (c.nfn) nerr += e.fn_initialize();
+      if (c.nfn) {
+        const int ne = e.fn_initialize();
+        if (tid == 0) cnt[18] += ne;
+      }
 
       for (int smp = 0; smp < c.nsample; ++smp) {
         const int sp = c.sample_sp[smp];
         // SpeciesParticleChooser::chooseParticles (SpeciesParticleChooser.cc:19-31), npart=1
         int i = (int)(c.count[sp] * e.U(k++) + c.ifirst[sp]);
         if (i == c.count[sp] + c.ifirst[sp]) i = c.count[sp] + c.ifirst[sp] - 1;
-        if (lane < D) {
-          e.mov[lane] = e.sb(e.sec1, 0, i)[lane];
-          e.mov[(S - 1) * D + lane] = e.sb(e.sec1, S - 1, i)[lane];
+        if (tid < D) {
+          e.mov[tid] = e.sb(e.sec1, 0, i)[tid];
+          e.mov[(S - 1) * D + tid] = e.sb(e.sec1, S - 1, i)[tid];
         }
         __syncthreads();
         // MultiLevelSampler::tryMove (MultiLevelSampler.cc:100-177)
         double old = 0.0;
         bool accepted = true;
         for (int l = L; l >= 0; --l) {
-          tries[l]++;
-          if (l < L) bu += (uint64_t)1 << (L - 1 - l);
+          if (tid == 0) {
+            cnt[l]++;
+            if (l < L) cnt[16] += 1ull << (L - 1 - l);
+          }
           int nd;
           const double lnT = e.mover(l, i, k, &nd);
           k += nd;
@@ -116,21 +127,21 @@ __global__ __launch_bounds__(PQ_WAVE) void sweep_kernel(DevCtx x, int nsweep, Sw
           const double acceptProb = exp(lnT + piRatio);
           if (e.U(k++) > acceptProb) { accepted = false; break; }
           old = dA;
-          accs[l]++;
+          if (tid == 0) cnt[8 + l]++;
         }
         if (accepted) {
           if (c.nfn) e.fn_accept();
-          for (int q = lane; q < S * D; q += PQ_WAVE) {
+          for (int q = tid; q < S * D; q += NT) {
             const int t = q / D, d = q - t * D;
             e.sb(e.sec1, t, i)[d] = e.mov[q];
           }
-          abu += ((uint64_t)1 << L) - 1;
+          if (tid == 0) cnt[17] += (1ull << L) - 1;
           __syncthreads();
         }
       }
       // SerialPaths::putBeads (SerialPaths.cc:144-165): interior slices only,
       // the end slices are never moved
-      for (int64_t q = slice + lane; q < (S - 1) * slice; q += PQ_WAVE) {
+      for (int64_t q = slice + tid; q < (S - 1) * slice; q += NT) {
         int kk = (int)(q / slice);
         int64_t rem = q - kk * slice;
         int t = kk + t0;
@@ -140,16 +151,17 @@ __global__ __launch_bounds__(PQ_WAVE) void sweep_kernel(DevCtx x, int nsweep, Sw
       __syncthreads();
     }
   }
-  if (lane == 0) {
+  __syncthreads();
+  if (tid == 0) {
     x.counter[chain] = k;
     pq_chain_stats& st = x.stats[chain];
     st.draws = k;
-    st.bead_updates += bu;
-    st.accepted_bead_updates += abu;
-    st.node_errors += nerr;
-    for (int l = 0; l < 8; ++l) { st.tries[l] += tries[l]; st.accepts[l] += accs[l]; }
+    st.bead_updates += cnt[16];
+    st.accepted_bead_updates += cnt[17];
+    st.node_errors += cnt[18];
+    for (int l = 0; l < 8; ++l) { st.tries[l] += cnt[l]; st.accepts[l] += cnt[8 + l]; }
   }
-  if (lane < c.nfn) x.scale[(int64_t)chain * PQ_MAX_NODES + lane] = e.scale[lane];
+  if (tid < c.nfn) x.scale[(int64_t)chain * PQ_MAX_NODES + tid] = e.scale[tid];
 }
 
 // ---------------------------------------------------- call-level kernels
@@ -171,7 +183,7 @@ __global__ __launch_bounds__(PQ_WAVE) void action_diff_kernel(DevCtx x, CallBufs
   const int q = blockIdx.x;
   const int chain = b.qchain[q];
   copy_chain(&c, &x.chains[chain]);
-  Eng<D, G, R> e(x, c, chain);
+  Eng<D, G, R, PQ_WAVE> e(x, c, chain);
   e.sec1 = const_cast<double*>(b.qsec + q * b.sec_stride);
   e.mov = const_cast<double*>(b.qmov + q * b.mov_stride);
   double v = e.action_diff(b.qidx[q], b.qlevel[q]);
@@ -198,11 +210,14 @@ template <int D, int G, class R>
 __global__ __launch_bounds__(PQ_WAVE) void fn_kernel(DevCtx x, int chain, FnBufs b) {
   __shared__ ChainP c;
   __shared__ double sscale[PQ_MAX_NODES];
+  __shared__ double gj[2 * PQ_WAVE];
   copy_chain(&c, &x.chains[chain]);
-  Eng<D, G, R> e(x, c, chain);
+  Eng<D, G, R, PQ_WAVE> e(x, c, chain);
+  e.gjrow = gj;
   e.sec1 = b.sec1; e.sec2 = b.sec2; e.mov = b.mov;
   e.fst = x.fnstate + c.fn_off;
   e.inv = x.inv + c.inv_off;
+  e.egr = x.egr + c.egr_off;
   e.wdet = b.wdet; e.werr = b.werr; e.wscale = b.wscale; e.notmine = b.notmine;
   e.scale = sscale;
   if (e.lane < c.nfn) sscale[e.lane] = x.scale[(int64_t)chain * PQ_MAX_NODES + e.lane];
@@ -225,14 +240,18 @@ __global__ __launch_bounds__(PQ_WAVE) void fn_kernel(DevCtx x, int chain, FnBufs
 template <int D, int G, class R>
 __global__ __launch_bounds__(PQ_WAVE) void node_eval_kernel(DevCtx x, int chain, int fa, int nmat,
                                                              double* r1, double* r2, double* inv,
-                                                             double* det, int* err, double* dist) {
+                                                             double* egr, double* det, int* err,
+                                                             double* dist) {
   __shared__ ChainP c;
   __shared__ double wd[PQ_WAVE];
   __shared__ int we[PQ_WAVE];
+  __shared__ double gj[2 * PQ_WAVE];
   copy_chain(&c, &x.chains[chain]);
-  Eng<D, G, R> e(x, c, chain);
+  Eng<D, G, R, PQ_WAVE> e(x, c, chain);
+  e.gjrow = gj;
   e.sec1 = r1; e.sec2 = r2;
   e.inv = inv;
+  e.egr = egr;
   e.wdet = wd; e.werr = we;
   const double scl = x.scale[(int64_t)chain * PQ_MAX_NODES + fa];
   constexpr int NG = PQ_WAVE / G;
@@ -426,19 +445,28 @@ __global__ __launch_bounds__(256) void histogram_kernel(DevCtx x, int s1, int s2
 
 // ------------------------------------------------------------ launchers
 
-template <int D, int G, class R>
-static hipError_t launch_sweep_t(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
-                                 hipStream_t st) {
+template <int D, int G, class R, int NT>
+static hipError_t launch_sweep_nt(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
+                                  hipStream_t st) {
   SweepLayout lay;
   lay.off_sec1 = L.off_sec1; lay.off_sec2 = L.off_sec2; lay.off_mov = L.off_mov;
   lay.off_fst = L.off_fst; lay.off_wdet = L.off_wdet; lay.off_werr = L.off_werr;
   lay.off_wscale = L.off_wscale; lay.off_scale = L.off_scale; lay.off_notmine = L.off_notmine;
+  lay.off_red = L.off_red; lay.off_cnt = L.off_cnt; lay.off_inv = L.off_inv;
+  lay.off_gj = L.off_gj;
   lay.bytes = L.bytes;
-  auto k = sweep_kernel<D, G, R>;
+  auto k = sweep_kernel<D, G, R, NT>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(nchain), dim3(PQ_WAVE), L.bytes, st, x, nsweep, lay);
+  hipLaunchKernelGGL(k, dim3(nchain), dim3(NT), L.bytes, st, x, nsweep, lay);
   return hipGetLastError();
+}
+
+template <int D, int G, class R>
+static hipError_t launch_sweep_t(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
+                                 hipStream_t st) {
+  if (L.team == 256) return launch_sweep_nt<D, G, R, 256>(x, nchain, nsweep, L, st);
+  return launch_sweep_nt<D, G, R, PQ_WAVE>(x, nchain, nsweep, L, st);
 }
 
 #define PQ_DISPATCH(FN, ...)                                                          \
@@ -490,17 +518,17 @@ hipError_t launch_fn(int D, int G, bool mixed, const DevCtx& x, int chain, const
 
 template <int D, int G, class R>
 static hipError_t launch_node_eval_t(const DevCtx& x, int chain, int fa, int nmat, double* r1,
-                                     double* r2, double* inv, double* det, int* err, double* dist,
-                                     hipStream_t st) {
+                                     double* r2, double* inv, double* egr, double* det, int* err,
+                                     double* dist, hipStream_t st) {
   hipLaunchKernelGGL((node_eval_kernel<D, G, R>), dim3(1), dim3(PQ_WAVE), 0, st, x, chain, fa, nmat,
-                     r1, r2, inv, det, err, dist);
+                     r1, r2, inv, egr, det, err, dist);
   return hipGetLastError();
 }
 
 hipError_t launch_node_eval(int D, int G, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
-                            double* r1, double* r2, double* inv, double* det, int* err, double* dist,
-                            hipStream_t st) {
-  PQ_DISPATCH(launch_node_eval_t, x, chain, fa, nmat, r1, r2, inv, det, err, dist, st);
+                            double* r1, double* r2, double* inv, double* egr, double* det, int* err,
+                            double* dist, hipStream_t st) {
+  PQ_DISPATCH(launch_node_eval_t, x, chain, fa, nmat, r1, r2, inv, egr, det, err, dist, st);
 }
 
 hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st) {

@@ -7,7 +7,10 @@ namespace pq {
 
 struct SweepLayoutH {
   int off_sec1, off_sec2, off_mov, off_fst, off_wdet, off_werr, off_wscale, off_scale, off_notmine;
+  int off_red, off_cnt, off_inv;   // off_inv < 0: inverse cache in global memory
+  int off_gj;                      // Gauss-Jordan pivot rows, 2 doubles per thread
   int bytes;
+  int team;                        // threads per chain: 64 or 256
 };
 
 struct CallBufsH {
@@ -42,8 +45,8 @@ hipError_t launch_action_diff(int D, int G, bool mixed, const DevCtx& x, int nq,
 hipError_t launch_fn(int D, int G, bool mixed, const DevCtx& x, int chain, const FnBufsH& b,
                      hipStream_t st);
 hipError_t launch_node_eval(int D, int G, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
-                            double* r1, double* r2, double* inv, double* det, int* err, double* dist,
-                            hipStream_t st);
+                            double* r1, double* r2, double* inv, double* egr, double* det, int* err,
+                            double* dist, hipStream_t st);
 hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st);
 hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const int* qi, const int* qt,
                             double* out, hipStream_t st);

@@ -27,6 +27,8 @@ struct FnP {             // one FixedNodeAction with its node model
   double coshwt, c;      // SHONodes
   double pg_nome[3], pg_pref[3], pg_k[3];
   int pg_nmax[3];        // FreeParticleNodes periodic Gaussians, alpha = m T
+  double pg_alpha[3], pg_len[3];
+  int pg_img[3];         // evaluate as the sum of the three nearest images
 };
 
 struct ChainP {
@@ -48,11 +50,16 @@ struct ChainP {
   int nsample;
   int sample_sp[8];
   int nsection;
+  double spring_c[PQ_PG_LEVELS][4];          // 1/(4 lambda tau 2^l) per level, species
+  double sho_cosh[4][PQ_PG_LEVELS];          // cosh(omega tau 2^l) per SHO action
+  double sho_div1[4][PQ_PG_LEVELS];          // 1/(2 sinh(omega tau 2^l))
+  double div2[PQ_PG_LEVELS];                 // 1/(2 tau 2^l)
   unsigned char spg_on[PQ_PG_LEVELS][4][3];   // SpringAction periodic Gaussian used
   unsigned char mpg_on[PQ_PG_LEVELS][4][3];   // mover periodic Gaussian used
   int64_t bead_off;      // element offset of this chain's [M][N][D] beads
-  int64_t inv_off;       // element offset of this chain's [S][ns][ns] inverse cache
+  int64_t inv_off;       // element offset of this chain's [S][ns_max][ns_max] inverse cache
   int64_t fn_off;        // element offset of this chain's fixed-node state (global copy)
+  int64_t egr_off;       // element offset of this chain's [S][ns_max][ns_max][D] element-gradient cache
   int ns_max;            // largest fermion species with a node model
 };
 

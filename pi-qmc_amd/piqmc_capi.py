@@ -227,6 +227,11 @@ def load_library(path=LIB_PATH):
         "pq_set_uniforms": (C.c_int, [C.c_void_p, dp, C.c_int64]),
         "pq_run_sweeps": (C.c_int, [C.c_void_p, C.c_int]),
         "pq_sync": (C.c_int, [C.c_void_p]),
+        "pq_set_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "pq_get_schedule": (C.c_int, [C.c_void_p, ip, ip, ip]),
+        "pq_run_sweeps_timed": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float)]),
+        "pq_get_all_stats": (C.c_int, [C.c_void_p, P(ChainStats)]),
+        "pq_get_pair_table": (C.c_int, [C.c_void_p, C.c_int, C.c_int, ip, dp, dp, dp]),
         "pq_get_stats": (C.c_int, [C.c_void_p, C.c_int, P(ChainStats)]),
         "pq_reset_stats": (C.c_int, [C.c_void_p]),
         "pq_get_node_scale": (C.c_int, [C.c_void_p, C.c_int, dp]),
@@ -311,6 +316,14 @@ class Batch:
 
     def sync(self):
         check(self.lib.pq_sync(self.h))
+
+    def set_schedule(self, team=-1, inv_lds=-1):
+        check(self.lib.pq_set_schedule(self.h, team, inv_lds))
+
+    def schedule(self):
+        t, i, b = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.pq_get_schedule(self.h, C.byref(t), C.byref(i), C.byref(b)))
+        return t.value, i.value, b.value
 
     def stats(self, c):
         st = ChainStats()

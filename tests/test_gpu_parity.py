@@ -143,8 +143,9 @@ def test_fixed_node_protocol():
     assert nbig >= 1
 
 
-def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False):
+def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False, team=-1, inv=-1):
     b = capi.Batch([desc], precision=precision)
+    b.set_schedule(team, inv)
     oc = ol.Chain(desc)
     beads = capi.initial_beads(desc, seed)
     b.set_paths(0, beads)
@@ -162,12 +163,14 @@ def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False):
     return b, oc
 
 
-@pytest.mark.parametrize("name,inject", [("cfg1", True), ("cfg1", False), ("cfg2", True), ("pbc", False)])
-def test_sampler_decisions(name, inject):
-    """Same uniforms in, same accept/reject decisions and paths out."""
-    print(f"[test] sampler_decisions {name} inject={inject}", flush=True)
+@pytest.mark.parametrize("name,inject,team,inv", [("cfg1", True, -1, -1), ("cfg1", False, 256, -1),
+                                                   ("cfg2", True, 64, 0), ("cfg2", True, 256, 1),
+                                                   ("cfg2", False, 64, 1), ("pbc", False, -1, -1)])
+def test_sampler_decisions(name, inject, team, inv):
+    """Same uniforms in, same accept/reject decisions and paths out, for every schedule."""
+    print(f"[test] sampler_decisions {name} inject={inject} team={team} inv={inv}", flush=True)
     desc = _descs()[name]
-    b, oc = _paired(desc, 6, inject=inject)
+    b, oc = _paired(desc, 6, inject=inject, team=team, inv=inv)
     sg, so = b.stats(0), oc.stats()
     assert sg.draws == so.draws
     assert list(sg.tries) == list(so.tries)
@@ -217,6 +220,7 @@ def test_batched_uneven_decisions():
     print("[test] batched_uneven", flush=True)
     descs = [d for d in capi.cfg3_descs(24) if d.nslice <= 256][:8]
     b = capi.Batch(descs)
+    assert b.schedule()[0] == 256
     ocs = []
     for c, d in enumerate(descs):
         beads = capi.initial_beads(d, 500 + c)
