@@ -170,7 +170,7 @@ int pq_run_sweeps(pq_batch* batch, int nsweep);
 int pq_run_sweeps_timed(pq_batch* batch, int nsweep, float* ms);
 int pq_sync(pq_batch* batch);
 /* Schedule of the sampler kernel: team = threads per chain (64 = one wave,
-   256 = four waves, -1 automatic), inv_lds = node inverse cache in LDS (1),
+   128 = two waves, -1 automatic), inv_lds = node inverse cache in LDS (1),
    global memory (0) or automatic (-1).  Results do not depend on it. */
 int pq_set_schedule(pq_batch* batch, int team, int inv_lds);
 int pq_get_schedule(const pq_batch* batch, int* team, int* inv_lds, int* lds_bytes);

@@ -162,7 +162,7 @@ struct pq_batch {
   int precision = 0;
   int nchain = 0;
   int D = 3;
-  int G = 4;
+  int NC = 4;     // node-matrix width class of the batch: 4, 8, 12, 16 or 32
   hipStream_t stream = nullptr;
   std::vector<pq_chain_desc> descs;
   std::vector<ChainP> hc;
@@ -222,7 +222,7 @@ static DevCtx make_ctx(const pq_batch* b) {
 
 static int scratch(pq_batch* b, size_t bytes, void** p) {
   if (bytes > b->scr_bytes) {
-    if (b->dscr) hipFree(b->dscr);
+    if (b->dscr) (void)hipFree(b->dscr);
     b->dscr = nullptr;
     b->scr_bytes = 0;
     HIPCHK(hipMalloc(&b->dscr, bytes));
@@ -394,7 +394,7 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
 static int64_t align8(int64_t v) { return (v + 7) & ~(int64_t)7; }
 
 // LDS carve-up of the sampler workgroup, sized by the batch's largest chain.
-// team: threads per chain (64 or 256, -1 automatic); inv_lds: inverse cache in
+// team: threads per chain (64 or 128, -1 automatic); inv_lds: inverse cache in
 // LDS (1), in global memory (0) or automatic (-1).
 static void make_layout(pq_batch* b, int team_req, int inv_req) {
   int64_t mS = 0, mSND = 0, mSD = 0, mF = 0, mInv = 0;
@@ -419,7 +419,7 @@ static void make_layout(pq_batch* b, int team_req, int inv_req) {
   b->lay.off_werr = (int)o; o += PQ_WAVE * 4;
   b->lay.off_notmine = (int)o; o += PQ_MAX_NODES * 4;
   o = align8(o);
-  b->lay.off_gj = (int)o; o += 2 * 8 * 256;
+  b->lay.off_gj = (int)o; o += 2 * 8 * 128;
   // automatic: keep the inverse cache in LDS while four chains still fit on one CU
   bool inv_lds = nodes && o + mInv * 8 <= 40 * 1024;
   if (inv_req == 0) inv_lds = false;
@@ -431,9 +431,10 @@ static void make_layout(pq_batch* b, int team_req, int inv_req) {
     b->lay.off_inv = -1;
   }
   b->lay.bytes = (int)align8(o);
-  // automatic team: four waves per chain when node matrices dominate, else one
-  int team = nodes ? 256 : PQ_WAVE;
-  if (team_req == 64 || team_req == 256) team = team_req;
+  // automatic team: two waves per chain when node matrices dominate, else one;
+  // either way two waves fit per SIMD, so four chains share a CU
+  int team = nodes ? 128 : PQ_WAVE;
+  if (team_req == 64 || team_req == 128) team = team_req;
   b->lay.team = team;
 }
 
@@ -461,7 +462,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     if (rc) { delete b; return rc; }
     ns_max = std::max(ns_max, b->hc[c].ns_max);
   }
-  b->G = ns_max <= 4 ? 4 : (ns_max <= 16 ? 16 : 32);
+  b->NC = ns_max <= 4 ? 4 : ns_max <= 8 ? 8 : ns_max <= 12 ? 12 : ns_max <= 16 ? 16 : 32;
   for (int c = 0; c < nchain; ++c) {
     ChainP& ch = b->hc[c];
     ch.bead_off = beads;
@@ -534,15 +535,15 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
 
 extern "C" void pq_batch_destroy(pq_batch* b) {
   if (!b) return;
-  hipSetDevice(b->device);
-  if (b->stream) hipStreamSynchronize(b->stream);
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (b->ev0) hipEventDestroy(b->ev0);
-  if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->stream) hipStreamDestroy(b->stream);
+    if (p) (void)hipFree(p);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
@@ -613,7 +614,7 @@ extern "C" int pq_set_uniforms(pq_batch* b, const double* u, int64_t per_chain) 
   if (!b || !u || per_chain < 1) return fail(PQ_ERR_ARG, "pq_set_uniforms: bad arguments");
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
-  if (b->dinj) hipFree(b->dinj);
+  if (b->dinj) (void)hipFree(b->dinj);
   b->dinj = nullptr;
   HIPCHK(hipMalloc(&b->dinj, sizeof(double) * per_chain * b->nchain));
   HIPCHK(hipMemcpy(b->dinj, u, sizeof(double) * per_chain * b->nchain, hipMemcpyHostToDevice));
@@ -625,7 +626,7 @@ extern "C" int pq_set_uniforms(pq_batch* b, const double* u, int64_t per_chain) 
 
 extern "C" int pq_set_schedule(pq_batch* b, int team, int inv_lds) {
   if (!b) return fail(PQ_ERR_ARG, "null batch");
-  if (team != -1 && team != 64 && team != 256) return fail(PQ_ERR_ARG, "team must be 64, 256 or -1");
+  if (team != -1 && team != 64 && team != 128) return fail(PQ_ERR_ARG, "team must be 64, 128 or -1");
   if (inv_lds < -1 || inv_lds > 1) return fail(PQ_ERR_ARG, "inv_lds must be -1, 0 or 1");
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -647,7 +648,7 @@ extern "C" int pq_run_sweeps(pq_batch* b, int nsweep) {
   if (nsweep == 0) return PQ_OK;
   HIPCHK(hipSetDevice(b->device));
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_sweep(b->D, b->G, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+  HIPCHK(launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
   return PQ_OK;
 }
 
@@ -656,7 +657,7 @@ extern "C" int pq_run_sweeps_timed(pq_batch* b, int nsweep, float* ms) {
   HIPCHK(hipSetDevice(b->device));
   DevCtx x = make_ctx(b);
   HIPCHK(hipEventRecord(b->ev0, b->stream));
-  HIPCHK(launch_sweep(b->D, b->G, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+  HIPCHK(launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
   HIPCHK(hipEventRecord(b->ev1, b->stream));
   HIPCHK(hipEventSynchronize(b->ev1));
   HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
@@ -778,7 +779,7 @@ extern "C" int pq_action_difference(pq_batch* b, int nq, const int* chain, const
   HIPCHK(hipMemcpyAsync(dq + 2 * nq, index, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
   CallBufsH cb{dq, dq + nq, dsec, dmov, dq + 2 * nq, sec_stride, mov_stride, dout};
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_action_diff(b->D, b->G, b->precision == PQ_PREC_MIXED, x, nq, cb, b->stream));
+  HIPCHK(launch_action_diff(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, nq, cb, b->stream));
   HIPCHK(hipMemcpyAsync(dS, dout, sizeof(double) * nq, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
@@ -824,7 +825,7 @@ static int fn_scratch(pq_batch* b, int c, FnScr* s) {
       const int64_t hs = (int64_t)h.S * h.N * h.D;
       big = std::max(big, sizeof(double) * (2 * hs + h.S * h.D + PQ_WAVE + h.S + 8) + sizeof(int) * (PQ_WAVE + 8) + 256);
     }
-    if (b->dfnscr) hipFree(b->dfnscr);
+    if (b->dfnscr) (void)hipFree(b->dfnscr);
     b->dfnscr = nullptr;
     b->fnscr_bytes = 0;
     HIPCHK(hipMalloc(&b->dfnscr, big));
@@ -856,7 +857,7 @@ extern "C" int pq_section_begin(pq_batch* b, int c, const double* sec1, const do
   HIPCHK(hipMemcpyAsync(s.sec2, sec2, sec, hipMemcpyHostToDevice, b->stream));
   FnBufsH fb{s.sec1, s.sec2, s.mov, s.wdet, s.werr, s.wscale, s.notmine, 0, 0, 0, s.out};
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_fn(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
+  HIPCHK(launch_fn(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
 }
@@ -875,7 +876,7 @@ extern "C" int pq_double_action_difference(pq_batch* b, int c, int level, const 
   HIPCHK(hipMemcpyAsync(s.mov, moving, sizeof(double) * ch.S * ch.D, hipMemcpyHostToDevice, b->stream));
   FnBufsH fb{s.sec1, s.sec2, s.mov, s.wdet, s.werr, s.wscale, s.notmine, 1, level, index[0], s.out};
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_fn(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
+  HIPCHK(launch_fn(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
   HIPCHK(hipMemcpyAsync(dS, s.out, sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
@@ -890,7 +891,7 @@ extern "C" int pq_accept_last_move(pq_batch* b, int c) {
   if (int rc = fn_scratch(b, c, &s)) return rc;
   FnBufsH fb{s.sec1, s.sec2, s.mov, s.wdet, s.werr, s.wscale, s.notmine, 2, 0, 0, s.out};
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_fn(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
+  HIPCHK(launch_fn(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, c, fb, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
 }
@@ -920,7 +921,7 @@ extern "C" int pq_node_evaluate(pq_batch* b, int c, int inode, int nmat, const d
   HIPCHK(hipMemcpyAsync(dr1, r1, sizeof(double) * nd, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipMemcpyAsync(dr2, r2, sizeof(double) * nd, hipMemcpyHostToDevice, b->stream));
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_node_eval(b->D, b->G, b->precision == PQ_PREC_MIXED, x, c, inode, nmat, dr1, dr2, dinv,
+  HIPCHK(launch_node_eval(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, c, inode, nmat, dr1, dr2, dinv,
                           degr, ddet, derr, ddist, b->stream));
   std::vector<double> hd((size_t)nmat * 2 * ns);
   HIPCHK(hipMemcpyAsync(det, ddet, sizeof(double) * nmat, hipMemcpyDeviceToHost, b->stream));

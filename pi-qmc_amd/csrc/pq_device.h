@@ -192,6 +192,14 @@ template <> __device__ __forceinline__ float ratio<float>(float g, float v) {
   return g * __frcp_rn(v + 1.17549435e-38f);
 }
 
+// copy a chain's parameter block into LDS
+__device__ __forceinline__ void copy_chain(ChainP* dst, const ChainP* src) {
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
+  __syncthreads();
+}
+
 __device__ __forceinline__ int species_of(const ChainP& c, int i) {
   int s = 0;
   for (int k = 1; k < c.nsp; ++k) s = (i >= c.ifirst[k]) ? k : s;

@@ -9,8 +9,9 @@
 //   fn_initialize()    FixedNodeAction::initialize for every node action
 //   fn_diff()          CompositeDoubleAction::getActionDifference
 //   fn_accept()        FixedNodeAction::acceptLastMove
-// G is the lane-group width of the node-matrix Gauss-Jordan inversion: one
-// matrix row per lane, NT/G matrices per pass, G >= largest fermion species.
+// NC bounds the largest fermion species; the node-matrix Gauss-Jordan
+// inversion keeps one matrix row per lane in groups of G = gj_group(NC) lanes,
+// NT/G matrices per pass.
 // All routines are called by every thread of the team with team-uniform
 // arguments and return team-uniform results.
 #pragma once
@@ -18,69 +19,103 @@
 
 namespace pq {
 
-// Row-per-lane Gauss-Jordan inversion with partial pivoting of one G x G
-// matrix per G-lane group.  Lane r holds row r of A (a) and of the identity
-// (b).  Rows are not moved: each lane tracks the position its row would have
-// in the row-swapping elimination (mypos), the pivot of step k is the largest
-// |a[k]| among positions >= k with the lowest position on ties (LAPACK's
-// idamax), and every virtual swap flips the determinant's sign.  The pivot
-// row reaches the group through LDS (row: 2G doubles per group).  On exit the
-// lane with mypos = k holds row k of A^-1 in b, and det = prod(pivots) *
-// (-1)^swaps, the LU determinant with the same pivots.  Rows and columns >= n
-// are an identity block.
+// 64-bit DPP lane move inside 16-lane rows (quad_perm / mirror patterns)
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+  return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// maximum of v over each aligned group of G lanes (G = 4, 8, 16, 32), every
+// lane of the group receives it: quad swaps, half-row and row mirrors, and
+// for G = 32 a swizzle across the two rows of a 32-lane half.
 template <int G>
-__device__ __forceinline__ void gj_invert(double (&a)[G], double (&b)[G], int r, int n,
+__device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
+  uint64_t o = dpp_u64<0xB1>(v);            // quad_perm [1,0,3,2]
+  v = o > v ? o : v;
+  o = dpp_u64<0x4E>(v);                     // quad_perm [2,3,0,1]
+  v = o > v ? o : v;
+  if constexpr (G >= 8) {
+    o = dpp_u64<0x141>(v);                  // row_half_mirror
+    v = o > v ? o : v;
+  }
+  if constexpr (G >= 16) {
+    o = dpp_u64<0x140>(v);                  // row_mirror
+    v = o > v ? o : v;
+  }
+  if constexpr (G >= 32) {
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)v, 0x401F);   // lane ^ 16
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(v >> 32), 0x401F);
+    o = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Row-per-lane Gauss-Jordan inversion with partial pivoting of one n x n
+// matrix (n <= NC <= 32) per G-lane group; every lane of the wave calls it.
+// Lane r < n holds row r of A (a) and of the identity (b); lanes r >= n take
+// no part.  Rows are not moved: each lane tracks the position its row would
+// have in the row-swapping elimination (mypos), the pivot of step k is the
+// largest |a[k]| among positions >= k with the lowest position on ties
+// (LAPACK's idamax), and every virtual swap flips the determinant's sign.  The
+// search is one group maximum of a 64-bit key, |a[k]|'s bits with the low six
+// replaced by 63 - mypos (non-negative doubles order as their bit patterns; a
+// difference in the last six mantissa bits counts as a tie).  The pivot row
+// reaches the group through LDS (row: 2 NC doubles per group).  Columns < k
+// of A are finished at step k and are no longer updated.  On exit the lane
+// with mypos = k holds row k of A^-1 in b, and det = prod(pivots) *
+// (-1)^swaps, the LU determinant with the same pivots.
+template <int G, int NC>
+__device__ __forceinline__ void gj_invert(double (&a)[NC], double (&b)[NC], int r, int n,
                                           double* row, double& det, int& err, int& mypos) {
   det = 1.0;
   err = 0;
   mypos = r;
+  const bool act = r < n;
 #pragma unroll
-  for (int k = 0; k < G; ++k) {
+  for (int k = 0; k < NC; ++k) {
     if (k < n) {
-      double v = (mypos >= k) ? fabs(a[k]) : -1.0;
-      int key = (mypos << 8) | r;
-#pragma unroll
-      for (int o = G / 2; o > 0; o >>= 1) {
-        const double v2 = __shfl_xor(v, o, G);
-        const int k2 = __shfl_xor(key, o, G);
-        if (v2 > v || (v2 == v && k2 < key)) { v = v2; key = k2; }
-      }
-      const int p = key & 0xff;
-      const int ppos = key >> 8;
-      if (v == 0.0) err = 1;
+      const uint64_t key = (act && mypos >= k)
+          ? ((__double_as_longlong(fabs(a[k])) & ~0x3Full) | (uint64_t)(63 - mypos)) : 0ull;
+      const uint64_t kmax = group_max_u64<G>(key);
+      const int ppos = 63 - (int)(kmax & 0x3F);
+      const bool isp = act && key == kmax;
       if (ppos != k) det = -det;
-      if (r == p) {
+      if (isp) {
 #pragma unroll
-        for (int cc = 0; cc < G; ++cc) { row[cc] = a[cc]; row[G + cc] = b[cc]; }
+        for (int cc = k; cc < NC; ++cc) row[cc] = a[cc];
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) row[NC + cc] = b[cc];
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       const double piv = row[k];
       det *= piv;
-      const bool ok = v != 0.0;
-      // one division per step: the pivot row is scaled by the reciprocal
-      const double ipiv = ok ? 1.0 / piv : 0.0;
-      const double f = a[k] * ipiv;
+      // one division per step; a singular column leaves every row unchanged
+      const bool zero = piv == 0.0;
+      if (zero) err = 1;
+      const double ipiv = zero ? 0.0 : 1.0 / piv;
+      const double f = (act && !isp) ? a[k] * ipiv : 0.0;
+      const double sp = (isp && !zero) ? ipiv : 1.0;
+      // pivot row: scaled by 1/piv (f = 0); other rows: minus f * pivot row
 #pragma unroll
-      for (int cc = 0; cc < G; ++cc) {
-        const double pa = row[cc], pb = row[G + cc];
-        if (r == p) {
-          a[cc] = ok ? a[cc] * ipiv : a[cc];
-          b[cc] = ok ? b[cc] * ipiv : b[cc];
-        } else {
-          a[cc] = ok ? a[cc] - f * pa : a[cc];
-          b[cc] = ok ? b[cc] - f * pb : b[cc];
-        }
-      }
+      for (int cc = k + 1; cc < NC; ++cc) a[cc] = fma(-f, row[cc], a[cc]) * sp;
+#pragma unroll
+      for (int cc = 0; cc < NC; ++cc) b[cc] = fma(-f, row[NC + cc], b[cc]) * sp;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       // the row at position k moves to the pivot's old position
-      if (r == p) mypos = k;
+      if (isp) mypos = k;
       else if (mypos == k) mypos = ppos;
     }
   }
 }
 
-template <int D, int G, class R, int NT>
+constexpr int gj_group(int nc) { return nc <= 4 ? 4 : (nc <= 8 ? 8 : (nc <= 16 ? 16 : 32)); }
+
+template <int D, int NC, class R, int NT>
 struct Eng {
+  static constexpr int G = gj_group(NC);
   const DevCtx& x;
   const ChainP& c;
   int chain;
@@ -100,7 +135,7 @@ struct Eng {
   double* wdet;     // [NT/G] scratch: determinants of a pass
   int* werr;        // [NT/G] scratch: error flags of a pass
   double* wscale;   // [S] scale used per slice in initialize
-  double* gjrow;    // [NT/G][2G] pivot rows of the Gauss-Jordan groups
+  double* gjrow;    // [NT/G][2 NC] pivot rows of the Gauss-Jordan groups
 
   __device__ Eng(const DevCtx& x_, const ChainP& c_, int chain_)
       : x(x_), c(c_), chain(chain_), tid(threadIdx.x), lane(threadIdx.x & 63), wave(threadIdx.x >> 6),
@@ -398,47 +433,52 @@ struct Eng {
   // NodeModel::evaluate (SHONodes.cc:51-128, FreeParticleNodes.cc:72-157) on
   // slices t0, t0+step, ... (nt of them) with the given scale; inverses go to
   // the cache, determinants and error flags to wdet/werr[0..nt).
-  __device__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, int i, double scl) {
+  __device__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, int i,
+                                             double scl) {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
+    // fill: every thread takes matrix elements; they are staged in the inverse
+    // slot of their slice, the gradient factors go to their own cache
+    const int per = n * n;
+    for (int q = tid; q < nt * per; q += NT) {
+      const int m = q / per, rem = q - m * per;
+      const int ii = rem / n, jj = rem - ii * n;
+      const int t = t0 + m * step;
+      const double* ri = sb(sec2, t, f.ifirst + ii);
+      const double* rj = r1pos(t, f.ifirst + jj, use_mov, i);
+      double gf[D];
+      const int64_t e = ((int64_t)t * ins + ii) * ins + jj;
+      inv[e] = elem(f, rj, ri, scl, gf);
+#pragma unroll
+      for (int d = 0; d < D; ++d) egr[e * D + d] = gf[d];
+    }
+    __syncthreads();
     const int g = tid / G, r = tid % G;
     constexpr int NG = NT / G;
     constexpr int GPW = PQ_WAVE / G;   // groups per wave
-    double* row = gjrow + (int64_t)g * 2 * G;
+    double* row = gjrow + (int64_t)g * 2 * NC;
     for (int base = 0; base < nt; base += NG) {
       // a wave whose groups all lie past the last matrix has nothing to do
       if (base + wave * GPW < nt) {
         const int m = base + g;
         const bool live = m < nt;
-        const int t = t0 + (live ? m : 0) * step;
-        double a[G], b[G];
+        const int t = t0 + m * step;
+        double a[NC], b[NC];
+        const double* src = inv + ((int64_t)t * ins + r) * ins;
 #pragma unroll
-        for (int cc = 0; cc < G; ++cc) { a[cc] = (cc == r) ? 1.0 : 0.0; b[cc] = a[cc]; }
-        if (r < n) {
-          const double* ri = sb(sec2, t, f.ifirst + r);
-          double* eg = egr + ((int64_t)t * ins + r) * ins * D;
-#pragma unroll
-          for (int cc = 0; cc < G; ++cc) {
-            if (cc < n) {
-              const double* rj = r1pos(t, f.ifirst + cc, use_mov, i);
-              double gf[D];
-              a[cc] = elem(f, rj, ri, scl, gf);
-              if (live) {
-#pragma unroll
-                for (int d = 0; d < D; ++d) eg[cc * D + d] = gf[d];
-              }
-            }
-          }
+        for (int cc = 0; cc < NC; ++cc) {
+          a[cc] = (live && r < n && cc < n) ? src[cc] : 0.0;
+          b[cc] = (cc == r) ? 1.0 : 0.0;
         }
         double det;
         int err, pos;
-        gj_invert<G>(a, b, r, n, row, det, err, pos);
+        gj_invert<G, NC>(a, b, r, live ? n : 0, row, det, err, pos);
         if (live) {
-          // a singular matrix can leave a padding row in a leading position
+          // a singular matrix can leave a row without a leading position
           if (r < n && pos < n) {
             double* dst = inv + ((int64_t)t * ins + pos) * ins;
 #pragma unroll
-            for (int cc = 0; cc < G; ++cc)
+            for (int cc = 0; cc < NC; ++cc)
               if (cc < n) dst[cc] = b[cc];
           }
           if (r == 0) { wdet[m] = det; werr[m] = err; }

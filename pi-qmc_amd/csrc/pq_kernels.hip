@@ -1,6 +1,6 @@
 // pq_kernels.hip -- gfx950 kernels of the bisection-move hot path.
 //
-// sweep_kernel: the device-resident sampler.  One single-wave workgroup per
+// sweep_kernel (pq_sweep.h): the device-resident sampler.  One workgroup per
 // chain runs nsweep sweeps; a sweep is nsection x (choose the section, copy it
 // from HBM to LDS, initialize the node caches, run every Sample's tryMove over
 // levels L..0, write the moved slices back).  Chains never communicate, so the
@@ -15,155 +15,6 @@
 
 namespace pq {
 
-// LDS carve-up of the sampler workgroup
-struct SweepLayout {
-  int off_sec1, off_sec2, off_mov, off_fst, off_wdet, off_werr, off_wscale, off_scale, off_notmine;
-  int off_red, off_cnt, off_inv;   // off_inv < 0: inverse cache in global memory
-  int off_gj;
-  int bytes;
-};
-
-__device__ __forceinline__ void copy_chain(ChainP* dst, const ChainP* src) {
-  const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
-  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-  for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
-  __syncthreads();
-}
-
-template <int D, int G, class R, int NT>
-__global__ __launch_bounds__(NT) void sweep_kernel(DevCtx x, int nsweep, SweepLayout lay) {
-  extern __shared__ double smem[];
-  const int chain = x.order[blockIdx.x];
-  ChainP& c = *reinterpret_cast<ChainP*>(smem);
-  copy_chain(&c, &x.chains[chain]);
-  char* base = reinterpret_cast<char*>(smem);
-  Eng<D, G, R, NT> e(x, c, chain);
-  e.sec1 = reinterpret_cast<double*>(base + lay.off_sec1);
-  e.sec2 = reinterpret_cast<double*>(base + lay.off_sec2);
-  e.mov = reinterpret_cast<double*>(base + lay.off_mov);
-  e.fst = reinterpret_cast<double*>(base + lay.off_fst);
-  e.wdet = reinterpret_cast<double*>(base + lay.off_wdet);
-  e.werr = reinterpret_cast<int*>(base + lay.off_werr);
-  e.wscale = reinterpret_cast<double*>(base + lay.off_wscale);
-  e.scale = reinterpret_cast<double*>(base + lay.off_scale);
-  e.notmine = reinterpret_cast<int*>(base + lay.off_notmine);
-  e.red = reinterpret_cast<double*>(base + lay.off_red);
-  e.gjrow = reinterpret_cast<double*>(base + lay.off_gj);
-  e.inv = (lay.off_inv >= 0) ? reinterpret_cast<double*>(base + lay.off_inv) : x.inv + c.inv_off;
-  e.egr = x.egr + c.egr_off;
-  // per-level try/accept counters, bead updates, accepted, node errors
-  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + lay.off_cnt);
-  const int tid = e.tid;
-  if (tid < 19) cnt[tid] = 0;
-  if (tid < c.nfn) e.scale[tid] = x.scale[(int64_t)chain * PQ_MAX_NODES + tid];
-  __syncthreads();
-
-  const int S = c.S, N = c.N, M = c.M, L = c.L;
-  const int64_t slice = (int64_t)N * D;
-  double* beads = x.beads + c.bead_off;
-  uint64_t k = x.counter[chain];
-
-  for (int sw = 0; sw < nsweep; ++sw) {
-    for (int isec = 0; isec < c.nsection; ++isec) {
-      // SectionChooser::run (SectionChooser.cc:30-48) / DoubleSectionChooser::run (:33-65)
-      int t0, t0b = 0;
-      if (c.nfn) {
-        double xx = e.U(k++) * (1 - 1e-8);
-        const int ilo = -1, ihi = M / 2 - 1;
-        t0 = ilo + (int)((ihi + 1 - ilo) * xx);
-        if (t0 > ihi) t0 = ihi;
-        if (e.U(k++) >= 0.5) t0 = (t0 + M / 2 + 1) % M - 1;
-        t0b = (t0 + M / 2 + 1) % M - 1;
-      } else {
-        double xx = e.U(k++) * (1 - 1e-8);
-        const int ilo = -1, ihi = M - 1;
-        t0 = ilo + (int)((ihi + 1 - ilo) * xx);
-        if (t0 > ihi) t0 = ihi;
-      }
-      // SerialPaths::getBeads (SerialPaths.cc:119-132), identity permutation
-      for (int64_t q = tid; q < S * slice; q += NT) {
-        int kk = (int)(q / slice);
-        int64_t rem = q - kk * slice;
-        int t = kk + t0;
-        t = (t < 0) ? t + M : ((t >= M) ? t - M : t);
-        e.sec1[q] = beads[t * slice + rem];
-        if (c.nfn) {
-          int tb = kk + t0b;
-          tb = (tb < 0) ? tb + M : ((tb >= M) ? tb - M : tb);
-          e.sec2[q] = beads[tb * slice + rem];
-        }
-      }
-      __syncthreads();
-      if (c.nfn) {
-        const int ne = e.fn_initialize();
-        if (tid == 0) cnt[18] += ne;
-      }
-
-      for (int smp = 0; smp < c.nsample; ++smp) {
-        const int sp = c.sample_sp[smp];
-        // SpeciesParticleChooser::chooseParticles (SpeciesParticleChooser.cc:19-31), npart=1
-        int i = (int)(c.count[sp] * e.U(k++) + c.ifirst[sp]);
-        if (i == c.count[sp] + c.ifirst[sp]) i = c.count[sp] + c.ifirst[sp] - 1;
-        if (tid < D) {
-          e.mov[tid] = e.sb(e.sec1, 0, i)[tid];
-          e.mov[(S - 1) * D + tid] = e.sb(e.sec1, S - 1, i)[tid];
-        }
-        __syncthreads();
-        // MultiLevelSampler::tryMove (MultiLevelSampler.cc:100-177)
-        double old = 0.0;
-        bool accepted = true;
-        for (int l = L; l >= 0; --l) {
-          if (tid == 0) {
-            cnt[l]++;
-            if (l < L) cnt[16] += 1ull << (L - 1 - l);
-          }
-          int nd;
-          const double lnT = e.mover(l, i, k, &nd);
-          k += nd;
-          __syncthreads();
-          double dA = e.action_diff(i, l);
-          if (c.nfn) dA += e.fn_diff(i, l);
-          const double piRatio = -dA + old;
-          const double acceptProb = exp(lnT + piRatio);
-          if (e.U(k++) > acceptProb) { accepted = false; break; }
-          old = dA;
-          if (tid == 0) cnt[8 + l]++;
-        }
-        if (accepted) {
-          if (c.nfn) e.fn_accept();
-          for (int q = tid; q < S * D; q += NT) {
-            const int t = q / D, d = q - t * D;
-            e.sb(e.sec1, t, i)[d] = e.mov[q];
-          }
-          if (tid == 0) cnt[17] += (1ull << L) - 1;
-          __syncthreads();
-        }
-      }
-      // SerialPaths::putBeads (SerialPaths.cc:144-165): interior slices only,
-      // the end slices are never moved
-      for (int64_t q = slice + tid; q < (S - 1) * slice; q += NT) {
-        int kk = (int)(q / slice);
-        int64_t rem = q - kk * slice;
-        int t = kk + t0;
-        t = (t >= M) ? t - M : t;
-        beads[t * slice + rem] = e.sec1[q];
-      }
-      __syncthreads();
-    }
-  }
-  __syncthreads();
-  if (tid == 0) {
-    x.counter[chain] = k;
-    pq_chain_stats& st = x.stats[chain];
-    st.draws = k;
-    st.bead_updates += cnt[16];
-    st.accepted_bead_updates += cnt[17];
-    st.node_errors += cnt[18];
-    for (int l = 0; l < 8; ++l) { st.tries[l] += cnt[l]; st.accepts[l] += cnt[8 + l]; }
-  }
-  if (tid < c.nfn) x.scale[(int64_t)chain * PQ_MAX_NODES + tid] = e.scale[tid];
-}
-
 // ---------------------------------------------------- call-level kernels
 
 struct CallBufs {
@@ -177,13 +28,13 @@ struct CallBufs {
   double* out;
 };
 
-template <int D, int G, class R>
+template <int D, int NC, class R>
 __global__ __launch_bounds__(PQ_WAVE) void action_diff_kernel(DevCtx x, CallBufs b) {
   __shared__ ChainP c;
   const int q = blockIdx.x;
   const int chain = b.qchain[q];
   copy_chain(&c, &x.chains[chain]);
-  Eng<D, G, R, PQ_WAVE> e(x, c, chain);
+  Eng<D, NC, R, PQ_WAVE> e(x, c, chain);
   e.sec1 = const_cast<double*>(b.qsec + q * b.sec_stride);
   e.mov = const_cast<double*>(b.qmov + q * b.mov_stride);
   double v = e.action_diff(b.qidx[q], b.qlevel[q]);
@@ -206,13 +57,13 @@ struct FnBufs {
   double* out;
 };
 
-template <int D, int G, class R>
+template <int D, int NC, class R>
 __global__ __launch_bounds__(PQ_WAVE) void fn_kernel(DevCtx x, int chain, FnBufs b) {
   __shared__ ChainP c;
   __shared__ double sscale[PQ_MAX_NODES];
   __shared__ double gj[2 * PQ_WAVE];
   copy_chain(&c, &x.chains[chain]);
-  Eng<D, G, R, PQ_WAVE> e(x, c, chain);
+  Eng<D, NC, R, PQ_WAVE> e(x, c, chain);
   e.gjrow = gj;
   e.sec1 = b.sec1; e.sec2 = b.sec2; e.mov = b.mov;
   e.fst = x.fnstate + c.fn_off;
@@ -237,7 +88,7 @@ __global__ __launch_bounds__(PQ_WAVE) void fn_kernel(DevCtx x, int chain, FnBufs
 
 // NodeModel::evaluate + evaluateDistance on nmat (r1, r2) slice pairs, staged
 // as a pseudo-section of nmat slices.
-template <int D, int G, class R>
+template <int D, int NC, class R>
 __global__ __launch_bounds__(PQ_WAVE) void node_eval_kernel(DevCtx x, int chain, int fa, int nmat,
                                                              double* r1, double* r2, double* inv,
                                                              double* egr, double* det, int* err,
@@ -247,14 +98,14 @@ __global__ __launch_bounds__(PQ_WAVE) void node_eval_kernel(DevCtx x, int chain,
   __shared__ int we[PQ_WAVE];
   __shared__ double gj[2 * PQ_WAVE];
   copy_chain(&c, &x.chains[chain]);
-  Eng<D, G, R, PQ_WAVE> e(x, c, chain);
+  Eng<D, NC, R, PQ_WAVE> e(x, c, chain);
   e.gjrow = gj;
   e.sec1 = r1; e.sec2 = r2;
   e.inv = inv;
   e.egr = egr;
   e.wdet = wd; e.werr = we;
   const double scl = x.scale[(int64_t)chain * PQ_MAX_NODES + fa];
-  constexpr int NG = PQ_WAVE / G;
+  constexpr int NG = PQ_WAVE / gj_group(NC);
   for (int t = 0; t < nmat; t += NG) {
     const int nt = min(NG, nmat - t);
     e.node_matrices(fa, t, 1, nt, false, -1, scl);
@@ -445,87 +296,76 @@ __global__ __launch_bounds__(256) void histogram_kernel(DevCtx x, int s1, int s2
 
 // ------------------------------------------------------------ launchers
 
-template <int D, int G, class R, int NT>
-static hipError_t launch_sweep_nt(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
-                                  hipStream_t st) {
-  SweepLayout lay;
-  lay.off_sec1 = L.off_sec1; lay.off_sec2 = L.off_sec2; lay.off_mov = L.off_mov;
-  lay.off_fst = L.off_fst; lay.off_wdet = L.off_wdet; lay.off_werr = L.off_werr;
-  lay.off_wscale = L.off_wscale; lay.off_scale = L.off_scale; lay.off_notmine = L.off_notmine;
-  lay.off_red = L.off_red; lay.off_cnt = L.off_cnt; lay.off_inv = L.off_inv;
-  lay.off_gj = L.off_gj;
-  lay.bytes = L.bytes;
-  auto k = sweep_kernel<D, G, R, NT>;
-  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(nchain), dim3(NT), L.bytes, st, x, nsweep, lay);
-  return hipGetLastError();
-}
-
-template <int D, int G, class R>
-static hipError_t launch_sweep_t(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
-                                 hipStream_t st) {
-  if (L.team == 256) return launch_sweep_nt<D, G, R, 256>(x, nchain, nsweep, L, st);
-  return launch_sweep_nt<D, G, R, PQ_WAVE>(x, nchain, nsweep, L, st);
-}
+#define PQ_DISPATCH_R(FN, D_, R_, ...)                                              \
+  do {                                                                                \
+    switch (NC) {                                                                     \
+      case 4: return FN<D_, 4, R_>(__VA_ARGS__);                                      \
+      case 8: return FN<D_, 8, R_>(__VA_ARGS__);                                      \
+      case 12: return FN<D_, 12, R_>(__VA_ARGS__);                                    \
+      case 16: return FN<D_, 16, R_>(__VA_ARGS__);                                    \
+      default: return FN<D_, 32, R_>(__VA_ARGS__);                                    \
+    }                                                                                 \
+  } while (0)
 
 #define PQ_DISPATCH(FN, ...)                                                          \
   do {                                                                                \
     if (mixed) {                                                                      \
-      if (D == 2) { if (G <= 4) return FN<2, 4, float>(__VA_ARGS__);                  \
-                    if (G <= 16) return FN<2, 16, float>(__VA_ARGS__);                \
-                    return FN<2, 32, float>(__VA_ARGS__); }                           \
-      if (G <= 4) return FN<3, 4, float>(__VA_ARGS__);                                \
-      if (G <= 16) return FN<3, 16, float>(__VA_ARGS__);                              \
-      return FN<3, 32, float>(__VA_ARGS__);                                           \
+      if (D == 2) PQ_DISPATCH_R(FN, 2, float, __VA_ARGS__);                           \
+      PQ_DISPATCH_R(FN, 3, float, __VA_ARGS__);                                       \
     }                                                                                 \
-    if (D == 2) { if (G <= 4) return FN<2, 4, double>(__VA_ARGS__);                   \
-                  if (G <= 16) return FN<2, 16, double>(__VA_ARGS__);                 \
-                  return FN<2, 32, double>(__VA_ARGS__); }                            \
-    if (G <= 4) return FN<3, 4, double>(__VA_ARGS__);                                 \
-    if (G <= 16) return FN<3, 16, double>(__VA_ARGS__);                               \
-    return FN<3, 32, double>(__VA_ARGS__);                                            \
+    if (D == 2) PQ_DISPATCH_R(FN, 2, double, __VA_ARGS__);                            \
+    PQ_DISPATCH_R(FN, 3, double, __VA_ARGS__);                                        \
   } while (0)
 
-hipError_t launch_sweep(int D, int G, bool mixed, const DevCtx& x, int nchain, int nsweep,
+hipError_t launch_sweep_2f(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+hipError_t launch_sweep_2d(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+hipError_t launch_sweep_3f(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+hipError_t launch_sweep_3d(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+
+hipError_t launch_sweep(int D, int NC, bool mixed, const DevCtx& x, int nchain, int nsweep,
                         const SweepLayoutH& L, hipStream_t st) {
-  PQ_DISPATCH(launch_sweep_t, x, nchain, nsweep, L, st);
+  if (D == 2) return mixed ? launch_sweep_2f(NC, x, nchain, nsweep, L, st) : launch_sweep_2d(NC, x, nchain, nsweep, L, st);
+  return mixed ? launch_sweep_3f(NC, x, nchain, nsweep, L, st) : launch_sweep_3d(NC, x, nchain, nsweep, L, st);
 }
 
-template <int D, int G, class R>
+template <int D, int NC, class R>
 static hipError_t launch_action_diff_t(const DevCtx& x, int nq, const CallBufsH& b, hipStream_t st) {
   CallBufs cb{b.qchain, b.qlevel, b.qsec, b.qmov, b.qidx, b.sec_stride, b.mov_stride, b.out};
-  hipLaunchKernelGGL((action_diff_kernel<D, G, R>), dim3(nq), dim3(PQ_WAVE), 0, st, x, cb);
+  hipLaunchKernelGGL((action_diff_kernel<D, NC, R>), dim3(nq), dim3(PQ_WAVE), 0, st, x, cb);
   return hipGetLastError();
 }
 
-hipError_t launch_action_diff(int D, int G, bool mixed, const DevCtx& x, int nq, const CallBufsH& b,
+hipError_t launch_action_diff(int D, int NC, bool mixed, const DevCtx& x, int nq, const CallBufsH& b,
                               hipStream_t st) {
-  PQ_DISPATCH(launch_action_diff_t, x, nq, b, st);
+  (void)NC;
+  if (mixed) { if (D == 2) return launch_action_diff_t<2, 4, float>(x, nq, b, st);
+               return launch_action_diff_t<3, 4, float>(x, nq, b, st); }
+  if (D == 2) return launch_action_diff_t<2, 4, double>(x, nq, b, st);
+  return launch_action_diff_t<3, 4, double>(x, nq, b, st);
 }
 
-template <int D, int G, class R>
+template <int D, int NC, class R>
 static hipError_t launch_fn_t(const DevCtx& x, int chain, const FnBufsH& b, hipStream_t st) {
   FnBufs fb{b.sec1, b.sec2, b.mov, b.wdet, b.werr, b.wscale, b.notmine, b.op, b.level, b.idx, b.out};
-  hipLaunchKernelGGL((fn_kernel<D, G, R>), dim3(1), dim3(PQ_WAVE), 0, st, x, chain, fb);
+  hipLaunchKernelGGL((fn_kernel<D, NC, R>), dim3(1), dim3(PQ_WAVE), 0, st, x, chain, fb);
   return hipGetLastError();
 }
 
-hipError_t launch_fn(int D, int G, bool mixed, const DevCtx& x, int chain, const FnBufsH& b,
+hipError_t launch_fn(int D, int NC, bool mixed, const DevCtx& x, int chain, const FnBufsH& b,
                      hipStream_t st) {
   PQ_DISPATCH(launch_fn_t, x, chain, b, st);
 }
 
-template <int D, int G, class R>
+template <int D, int NC, class R>
 static hipError_t launch_node_eval_t(const DevCtx& x, int chain, int fa, int nmat, double* r1,
                                      double* r2, double* inv, double* egr, double* det, int* err,
                                      double* dist, hipStream_t st) {
-  hipLaunchKernelGGL((node_eval_kernel<D, G, R>), dim3(1), dim3(PQ_WAVE), 0, st, x, chain, fa, nmat,
+  hipLaunchKernelGGL((node_eval_kernel<D, NC, R>), dim3(1), dim3(PQ_WAVE), 0, st, x, chain, fa, nmat,
                      r1, r2, inv, egr, det, err, dist);
   return hipGetLastError();
 }
 
-hipError_t launch_node_eval(int D, int G, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
+hipError_t launch_node_eval(int D, int NC, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
                             double* r1, double* r2, double* inv, double* egr, double* det, int* err,
                             double* dist, hipStream_t st) {
   PQ_DISPATCH(launch_node_eval_t, x, chain, fa, nmat, r1, r2, inv, egr, det, err, dist, st);

@@ -38,13 +38,13 @@ struct FnBufsH {
   double* out;
 };
 
-hipError_t launch_sweep(int D, int G, bool mixed, const DevCtx& x, int nchain, int nsweep,
+hipError_t launch_sweep(int D, int NC, bool mixed, const DevCtx& x, int nchain, int nsweep,
                         const SweepLayoutH& L, hipStream_t st);
-hipError_t launch_action_diff(int D, int G, bool mixed, const DevCtx& x, int nq, const CallBufsH& b,
+hipError_t launch_action_diff(int D, int NC, bool mixed, const DevCtx& x, int nq, const CallBufsH& b,
                               hipStream_t st);
-hipError_t launch_fn(int D, int G, bool mixed, const DevCtx& x, int chain, const FnBufsH& b,
+hipError_t launch_fn(int D, int NC, bool mixed, const DevCtx& x, int chain, const FnBufsH& b,
                      hipStream_t st);
-hipError_t launch_node_eval(int D, int G, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
+hipError_t launch_node_eval(int D, int NC, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
                             double* r1, double* r2, double* inv, double* egr, double* det, int* err,
                             double* dist, hipStream_t st);
 hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st);

@@ -1,0 +1,9 @@
+// pq_sweep_2d.hip -- sweep_kernel instantiations for D=2, double arithmetic.
+#include "pq_sweep.h"
+
+namespace pq {
+hipError_t launch_sweep_2d(int NC, const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
+                           hipStream_t st) {
+  return launch_sweep_dr<2, double>(NC, x, nchain, nsweep, L, st);
+}
+}  // namespace pq

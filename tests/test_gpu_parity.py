@@ -163,8 +163,8 @@ def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False, tea
     return b, oc
 
 
-@pytest.mark.parametrize("name,inject,team,inv", [("cfg1", True, -1, -1), ("cfg1", False, 256, -1),
-                                                   ("cfg2", True, 64, 0), ("cfg2", True, 256, 1),
+@pytest.mark.parametrize("name,inject,team,inv", [("cfg1", True, -1, -1), ("cfg1", False, 128, -1),
+                                                   ("cfg2", True, 64, 0), ("cfg2", True, 128, 1),
                                                    ("cfg2", False, 64, 1), ("pbc", False, -1, -1)])
 def test_sampler_decisions(name, inject, team, inv):
     """Same uniforms in, same accept/reject decisions and paths out, for every schedule."""
@@ -220,7 +220,7 @@ def test_batched_uneven_decisions():
     print("[test] batched_uneven", flush=True)
     descs = [d for d in capi.cfg3_descs(24) if d.nslice <= 256][:8]
     b = capi.Batch(descs)
-    assert b.schedule()[0] == 256
+    assert b.schedule()[0] == 128
     ocs = []
     for c, d in enumerate(descs):
         beads = capi.initial_beads(d, 500 + c)
