@@ -32,6 +32,20 @@ HBM_PEAK_GBS = 8000.0
 FP64_VECTOR_TFLOPS = 78.6
 
 
+def measured_traffic(workload_name, precision):
+    """HBM bytes per sweep_kernel launch from the committed rocprofv3 FETCH_SIZE
+    and WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"sweep_kernel_{workload_name}_{precision}.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def workload(name, nchain):
     if name == "cfg3":
         return capi.cfg3_descs(nchain), "1024 uneven 2D dots (N 2-20, M 128-1024, SHO/free nodes, L=4)"
@@ -114,6 +128,36 @@ def run_gpu(descs, precision, steps, warmup, seed):
     return b, dict(ms=ms, wall=wall, bu=bu, abu=abu, energy=energy)
 
 
+def all_paths(b):
+    arr = np.empty(int(sum(d.nbeads() for d in b.descs)), dtype=np.float64)
+    b.lib.pq_get_all_paths.restype = C.c_int
+    b.lib.pq_get_all_paths.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64]
+    capi.check(b.lib.pq_get_all_paths(b.h, capi.dptr(arr), arr.size))
+    return arr
+
+
+def paired_energy(bm, br, em, er):
+    """Mixed vs FP64 on the same seeds and sweeps.  Both precisions move beads
+    in FP64, so a chain whose accept/reject decisions never differed ends on
+    bitwise the same path; its energy difference is the estimator's own.  The
+    paired difference over all chains (diverged chains included) gives the
+    energy error with its standard error."""
+    pm, pr = all_paths(bm), all_paths(br)
+    same, off = [], 0
+    for d in bm.descs:
+        n = d.nbeads()
+        same.append(bool(np.array_equal(pm[off:off + n], pr[off:off + n])))
+        off += n
+    same = np.array(same)
+    de = np.asarray(em) - np.asarray(er)
+    n = len(de)
+    return {"chains": n, "identical_decision_chains": int(same.sum()),
+            "max_abs_dE_identical": float(np.max(np.abs(de[same]))) if same.any() else None,
+            "mean_dE": float(de.mean()), "stderr_dE": float(de.std(ddof=1) / np.sqrt(n)) if n > 1 else None,
+            "mean_E_fp64": float(np.mean(er)),
+            "note": "per-chain thermodynamic energy of the final paths, same Philox streams and sweeps"}
+
+
 def cpu_baseline(descs, budget_s=12.0):
     """The oracle (FP64 C restatement) on host cores: a bounded sample of the
     workload's chains, one chain per thread (OpenMP)."""
@@ -131,7 +175,7 @@ def cpu_baseline(descs, budget_s=12.0):
     t0 = time.perf_counter()
     bu = lib.oq_run_batch(arr, nch, capi.dptr(beads), 1, 7, 0, cores, None)
     t1 = time.perf_counter() - t0
-    nsweep = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    nsweep = max(1, min(1000, int(budget_s / max(t1, 1e-3))))
     t0 = time.perf_counter()
     bu = lib.oq_run_batch(arr, nch, capi.dptr(beads), nsweep, 7, 0, cores, None)
     dt = time.perf_counter() - t0
@@ -169,7 +213,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    _, main_run = run_gpu(descs, prec, args.steps, args.warmup, args.seed)
+    b_main, main_run = run_gpu(descs, prec, args.steps, args.warmup, args.seed)
     bu_total = float(main_run["bu"].sum())
     value = bu_total / main_run["wall"]
     kernel_ms = float(np.mean(main_run["ms"]))
@@ -177,6 +221,7 @@ def main():
     bytes_launch = float(sum(bu_c * bytes_per_update(d, w) for bu_c, d in zip(main_run["bu"], descs))) / args.steps
     flops_launch = float(sum(bu_c * flops_per_update(d) for bu_c, d in zip(main_run["bu"], descs))) / args.steps
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args.workload, args.precision)
     out = {
         "metric": METRIC, "value": value, "unit": "bead-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -188,19 +233,17 @@ def main():
                    "precision": args.precision, "step": "one sweep of every chain (one kernel launch)"},
         "accepted_bead_updates_per_s": float(main_run["abu"].sum()) / main_run["wall"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src, "algorithmic_bytes": bytes_launch,
                      "kernel": "sweep_kernel", "kernel_ms": kernel_ms,
                      "fp64_equiv_tflops": flops_launch / (kernel_ms * 1e-3) / 1e12,
                      "valu_frac_of_fp64_peak": flops_launch / (kernel_ms * 1e-3) / 1e12 / FP64_VECTOR_TFLOPS},
     }
     if not args.no_fp64_pair and prec == capi.PQ_PREC_MIXED:
-        _, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed)
-        de = main_run["energy"] - ref["energy"]
+        b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed)
         out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
                                "kernel_ms": float(np.mean(ref["ms"]))}
-        out["energy_vs_fp64"] = {"mean_abs_diff_per_chain": float(np.mean(np.abs(de))),
-                                 "max_abs_diff": float(np.max(np.abs(de))),
-                                 "note": "paired runs, same seeds and uniforms, energy of the final paths"}
+        out["energy_vs_fp64"] = paired_energy(b_main, b64, main_run["energy"], ref["energy"])
         single = [capi.cfg2_desc()]
         _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed)
         r1 = float(one["bu"].sum()) / one["wall"]

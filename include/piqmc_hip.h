@@ -55,8 +55,8 @@ extern "C" {
 #define PQ_NODES_SHO 1          /* SHONodes          (src/fixednode/SHONodes.cc:51-170) */
 
 #define PQ_PREC_FP64 0          /* every operation in FP64 */
-#define PQ_PREC_MIXED 1         /* FP32 spring/SHO/pair/mover/propagator arithmetic with
-                                   compensated sums; FP64 node LU, sign test, exponent */
+#define PQ_PREC_MIXED 1         /* FP32 spring/SHO/pair link arithmetic with compensated
+                                   sums; FP64 mover, node matrices, sign test, exponent */
 
 #define PQ_RNG_PHILOX 0         /* Philox4x32-10 counter stream per chain */
 #define PQ_RNG_INJECTED 1       /* uniforms supplied by the caller, consumed in order */

@@ -492,20 +492,22 @@ struct Eng {
   // nodal distance needs: SHONodes the unscaled element (gf[0]);
   // FreeParticleNodes the gradient (PG'(x_d)/(PG(x_d)+1e-300)) prod_k PG(x_k)
   // in the order of FreeParticleNodes.cc:196-207.
+  // Always FP64: at low temperature the SHO matrix is close to rank one and
+  // its determinant cancels, so element rounding decides the sign test.
   __device__ __forceinline__ double elem(const FnP& f, const double* rj, const double* ri, double scl,
                                          double* gf) const {
     if (f.model == PQ_NODES_SHO) {
-      R rj2 = 0, ri2 = 0, rirj = 0;
+      double rj2 = 0, ri2 = 0, rirj = 0;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        R aj = (R)rj[d], ai = (R)ri[d];
+        const double aj = rj[d], ai = ri[d];
         rj2 += aj * aj; ri2 += ai * ai; rirj += ai * aj;
       }
-      const R e = r_exp(-(R)f.c * ((ri2 + rj2) * (R)f.coshwt - (R)2.0 * rirj));
-      gf[0] = (double)e;
+      const double e = exp(-f.c * ((ri2 + rj2) * f.coshwt - 2.0 * rirj));
+      gf[0] = e;
 #pragma unroll
       for (int d = 1; d < D; ++d) gf[d] = 0.0;
-      return scl * (double)e;
+      return scl * e;
     }
     double delta[D], value[D];
 #pragma unroll
@@ -515,10 +517,10 @@ struct Eng {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       if (f.pg_img[d]) {
-        R g2;
-        const R v2 = pg_images<R>((R)f.pg_alpha[d], (R)f.pg_len[d], (R)delta[d], &g2);
-        value[d] = (double)v2;
-        gf[d] = (double)ratio<R>(g2, v2);
+        double g2;
+        const double v2 = pg_images<double>(f.pg_alpha[d], f.pg_len[d], delta[d], &g2);
+        value[d] = v2;
+        gf[d] = ratio<double>(g2, v2);
       } else {
         double gr;
         value[d] = pg_eval(f.pg_nome[d], f.pg_pref[d], f.pg_k[d], f.pg_nmax[d], delta[d], &gr);
