@@ -180,6 +180,8 @@ struct pq_batch {
   double* dinv = nullptr;
   double* degr = nullptr;
   double* dfn = nullptr;
+  double* dfncache = nullptr;   // slice-pair node records of the sampler
+  int64_t nfncache = 0;
   double* dscale = nullptr;
   uint64_t* dcounter = nullptr;
   pq_chain_stats* dstats = nullptr;
@@ -208,6 +210,7 @@ static DevCtx make_ctx(const pq_batch* b) {
   x.inv = b->dinv;
   x.egr = b->degr;
   x.fnstate = b->dfn;
+  x.fncache = b->dfncache;
   x.scale = b->dscale;
   x.counter = b->dcounter;
   x.stats = b->dstats;
@@ -417,7 +420,7 @@ static void make_layout(pq_batch* b, int team_req, int inv_req) {
   b->lay.off_red = (int)o; o += 8 * 8;
   b->lay.off_cnt = (int)o; o += 24 * 8;
   b->lay.off_werr = (int)o; o += PQ_WAVE * 4;
-  b->lay.off_notmine = (int)o; o += PQ_MAX_NODES * 4;
+  b->lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
   o = align8(o);
   b->lay.off_gj = (int)o; o += 2 * 8 * 128;
   // automatic: keep the inverse cache in LDS while four chains still fit on one CU
@@ -454,7 +457,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   b->hc.resize(nchain);
   b->D = chains[0].ndim;
   std::map<std::tuple<double, double, double, double, double, int, int>, int> tabs;
-  int64_t beads = 0, inv = 0, fnst = 0, egr = 0;
+  int64_t beads = 0, inv = 0, fnst = 0, egr = 0, fnc = 0;
   int ns_max = 1;
   for (int c = 0; c < nchain; ++c) {
     if (chains[c].ndim != b->D) { delete b; return fail(PQ_ERR_ARG, "all chains of a batch share ndim"); }
@@ -473,6 +476,9 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     if (ch.nfn) egr += (int64_t)ch.S * ch.ns_max * ch.ns_max * ch.D;
     ch.fn_off = fnst;
     fnst += fn_state_doubles(ch);
+    const int64_t nrec = fn_cache_doubles(ch);
+    ch.fnc_off = nrec ? fnc : -1;
+    fnc += nrec;
   }
   make_layout(b, -1, -1);
   if (b->lay.bytes > 160 * 1024) {
@@ -510,11 +516,13 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
             dalloc((void**)&b->dinv, sizeof(double) * inv) &&
             dalloc((void**)&b->degr, sizeof(double) * egr) &&
             dalloc((void**)&b->dfn, sizeof(double) * fnst) &&
+            dalloc((void**)&b->dfncache, sizeof(double) * fnc) &&
             dalloc((void**)&b->dscale, sizeof(double) * nchain * PQ_MAX_NODES) &&
             dalloc((void**)&b->dcounter, sizeof(uint64_t) * nchain) &&
             dalloc((void**)&b->dstats, sizeof(pq_chain_stats) * nchain) &&
             dalloc((void**)&b->dorder, sizeof(int) * nchain);
   if (!ok) return cleanup_fail(fail(PQ_ERR_NOMEM, "device allocation failed"));
+  b->nfncache = fnc;
   std::vector<float> uf(b->hu.begin(), b->hu.end());
   std::vector<double> ones((size_t)nchain * PQ_MAX_NODES, 1.0);
   if (hipMemcpy(b->dchains, b->hc.data(), sizeof(ChainP) * nchain, hipMemcpyHostToDevice) != hipSuccess ||
@@ -527,7 +535,8 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
       hipMemset(b->dcounter, 0, sizeof(uint64_t) * nchain) != hipSuccess ||
       hipMemset(b->dstats, 0, sizeof(pq_chain_stats) * nchain) != hipSuccess ||
       hipMemset(b->dbeads, 0, sizeof(double) * std::max<int64_t>(1, beads)) != hipSuccess ||
-      hipMemset(b->dfn, 0, sizeof(double) * std::max<int64_t>(1, fnst)) != hipSuccess)
+      hipMemset(b->dfn, 0, sizeof(double) * std::max<int64_t>(1, fnst)) != hipSuccess ||
+      hipMemset(b->dfncache, 0, sizeof(double) * std::max<int64_t>(1, fnc)) != hipSuccess)
     return cleanup_fail(fail(PQ_ERR_HIP, "device upload failed"));
   *out = b;
   return PQ_OK;
@@ -537,7 +546,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn,
+  void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -570,6 +579,8 @@ extern "C" int pq_set_paths(pq_batch* b, int c, const double* beads) {
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   HIPCHK(hipMemcpy(b->dbeads + ch.bead_off, beads, sizeof(double) * ch.M * ch.N * ch.D, hipMemcpyHostToDevice));
+  // new paths: the chain's node records no longer describe them
+  if (ch.fnc_off >= 0) HIPCHK(hipMemset(b->dfncache + ch.fnc_off, 0, sizeof(double) * fn_cache_doubles(ch)));
   return PQ_OK;
 }
 
@@ -588,6 +599,7 @@ extern "C" int pq_set_all_paths(pq_batch* b, const double* beads, int64_t count)
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   HIPCHK(hipMemcpy(b->dbeads, beads, sizeof(double) * count, hipMemcpyHostToDevice));
+  if (b->nfncache) HIPCHK(hipMemset(b->dfncache, 0, sizeof(double) * b->nfncache));
   return PQ_OK;
 }
 

@@ -136,6 +136,11 @@ struct Eng {
   int* werr;        // [NT/G] scratch: error flags of a pass
   double* wscale;   // [S] scale used per slice in initialize
   double* gjrow;    // [NT/G][2 NC] pivot rows of the Gauss-Jordan groups
+  // sampler only: node records of every slice pair, reused across sections
+  double* fcache = nullptr;
+  int sec_t0 = 0;          // first slice of section 1
+  bool top_known = false;  // the section's last slice is never moved
+  int* lasterr = nullptr;  // [nfn] error flag of the last slice at initialize
 
   __device__ Eng(const DevCtx& x_, const ChainP& c_, int chain_)
       : x(x_), c(c_), chain(chain_), tid(threadIdx.x), lane(threadIdx.x & 63), wave(threadIdx.x >> 6),
@@ -580,62 +585,169 @@ struct Eng {
     __syncthreads();
   }
 
+  // Node record of section slice j for node action fa.  Section 1 slice j is
+  // absolute slice a and section 2 slice j is a + M/2 (mod M), so slices
+  // t and t + M/2 share one record, kept with the lower slice as r1.  Both
+  // models' density matrices are symmetric, rho(r, r') = rho(r', r): seen
+  // from the upper slice the matrix is the transpose, with the same
+  // determinant and the two distance sides exchanged.  lower tells whether
+  // section 1 holds the lower slice.
+  __device__ __forceinline__ double* slot_rec(int fa, int j, bool& lower) const {
+    int a = sec_t0 + j;
+    a = a < 0 ? a + c.M : (a >= c.M ? a - c.M : a);
+    const int half = c.M >> 1;
+    lower = a < half;
+    int64_t off = 0;
+    for (int k = 0; k < fa; ++k) off += (int64_t)half * fn_record_doubles(c.fn[k].ns);
+    return fcache + off + (int64_t)(lower ? a : a - half) * fn_record_doubles(c.fn[fa].ns);
+  }
+
+  // initialize from the records when every slice of the section has one:
+  // the rescale loop runs on the recorded determinants, det(s) = det(s_r)
+  // (s/s_r)^n, in slice order as the evaluating loop does.  Returns false
+  // (nothing changed) when a record is missing.
+  __device__ bool fn_init_cached(int fa, int& nerr) {
+    const FnP& f = c.fn[fa];
+    const int n = f.ns, S = c.S;
+    double* dm = fn_dm(fa);
+    double* flag = fn_newdm(fa);
+    for (int j = tid; j < S; j += NT) {
+      bool lower;
+      const double* r = slot_rec(fa, j, lower);
+      dm[j] = r[0];
+      wscale[j] = r[1];
+      flag[j] = r[2];
+    }
+    __syncthreads();
+    bool ok = true;
+    for (int j = 0; j < S; ++j) ok = ok && flag[j] == 1.0;
+    if (!ok) return false;
+    if (tid == 0) {
+      double scl = scale[fa], dm0 = 0;
+      int errs = 0, passes = 0;
+      for (int j = 0; j < S;) {
+        const double lr = (double)n * log(scl / wscale[j]);
+        const double dt = dm[j] * exp(lr);
+        const double ad = fabs(dt);
+        if (ad < 1e-50 || ad > 1e50) {
+          ++passes;
+          const double ld = log(fabs(dm[j])) + lr;   // log|det| at the current scale
+          if (ld < -575.6463) scl *= 2;               // |det| < 1e-250
+          else scl *= exp(-ld / n);
+          if (passes < 200) continue;
+        }
+        dm[j] = dt;
+        wscale[j] = scl;
+        if (j == 0) dm0 = dt;
+        if (dt * dm0 <= 1e-200) errs++;
+        passes = 0;
+        ++j;
+      }
+      scale[fa] = scl;
+      werr[0] = errs;
+      lasterr[fa] = 0;
+    }
+    __syncthreads();
+    nerr += werr[0];
+    if (f.nodal) {
+      double* dd = fn_dist(fa);
+      for (int q = tid; q < S * 2 * n; q += NT) {
+        const int j = q / (2 * n), pass = (q / n) & 1, k = q % n;
+        bool lower;
+        const double* r = slot_rec(fa, j, lower);
+        dd[q] = r[3 + (lower ? pass : 1 - pass) * n + k];
+      }
+    }
+    __syncthreads();
+    return true;
+  }
+
+  // write the records of section slices j0..j1-1 from dm/scale/dist; a slice
+  // whose evaluation erred (flag) gets no record
+  __device__ void fn_store_records(int fa, int j0, int j1, const double* det, const double* scl,
+                                   double scl0, const double* errflag, const double* dist) {
+    const int n = c.fn[fa].ns, rl = fn_record_doubles(n);
+    for (int q = tid; q < (j1 - j0) * rl; q += NT) {
+      const int j = j0 + q / rl, e = q % rl;
+      bool lower;
+      double* r = slot_rec(fa, j, lower);
+      double v;
+      if (e == 0) v = det[j];
+      else if (e == 1) v = scl ? scl[j] : scl0;
+      else if (e == 2) v = (errflag && errflag[j] != 0.0) ? 0.0 : 1.0;
+      else {
+        const int side = (e - 3) / n, k = (e - 3) % n;
+        v = c.fn[fa].nodal ? dist[((int64_t)j * 2 + (lower ? side : 1 - side)) * n + k] : 0.0;
+      }
+      r[e] = v;
+    }
+  }
+
   // FixedNodeAction::initialize (FixedNodeAction.cc:331-357) with the
   // evaluate(...,scaleMagnitude=true) rescale loop (SHONodes.cc:113-126):
   // slices are evaluated in passes with the current scale; on the first slice
   // whose |det| leaves [1e-50,1e50] the scale is updated and evaluation
-  // resumes from that slice, exactly as the sequential loop would.  Returns
-  // the number of crossed or singular slices.
+  // resumes from that slice, exactly as the sequential loop would.  In the
+  // sampler the slice-pair records replace the evaluation when all are
+  // present.  Returns the number of crossed or singular slices.
   __device__ int fn_initialize() {
     int nerr = 0;
     constexpr int NG = NT / G;
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
-      double scl = scale[fa];
       double* dm = fn_dm(fa);
-      int t = 0, passes = 0;
-      int errs = 0;
-      double dm0 = 0;
-      while (t < c.S) {
-        const int nt = min(NG, c.S - t);
-        node_matrices(fa, t, 1, nt, false, -1, scl);
-        int bad = -1;
-        for (int m = 0; m < nt; ++m) {
-          double dt = wdet[m];
-          if (werr[m] || fabs(dt) < 1e-50 || fabs(dt) > 1e50) { bad = m; break; }
-        }
-        const int ncommit = (bad < 0) ? nt : bad;
-        for (int m = 0; m < ncommit; ++m) {
-          if (tid == 0) { dm[t + m] = wdet[m]; wscale[t + m] = scl; }
-          if (t + m == 0) dm0 = wdet[m];
-          if (werr[m] || wdet[m] * dm0 <= 1e-200) errs++;
-        }
-        if (ncommit) passes = 0;
-        t += ncommit;
-        if (bad >= 0) {
-          ++passes;
-          const double dt = wdet[bad];
-          const int er = werr[bad];
-          if (!er) {
-            if (fabs(dt) < 1e-250) scl *= 2;
-            else scl *= pow(fabs(dt), -1. / f.ns);
+      double* eflag = fn_newdm(fa);   // per-slice error flags until the end
+      if (!(fcache && fn_init_cached(fa, nerr))) {
+        double scl = scale[fa];
+        int t = 0, passes = 0;
+        int errs = 0;
+        double dm0 = 0;
+        while (t < c.S) {
+          const int nt = min(NG, c.S - t);
+          node_matrices(fa, t, 1, nt, false, -1, scl);
+          int bad = -1;
+          for (int m = 0; m < nt; ++m) {
+            double dt = wdet[m];
+            if (werr[m] || fabs(dt) < 1e-50 || fabs(dt) > 1e50) { bad = m; break; }
           }
-          if (er || passes >= 200) {
-            // the reference repeats an erring evaluation unchanged; the
-            // oracle stops after 200 passes: commit the slice as it stands
-            if (tid == 0) { dm[t] = dt; wscale[t] = scl; }
-            if (t == 0) dm0 = dt;
-            if (er || dt * dm0 <= 1e-200) errs++;
-            ++t;
-            passes = 0;
+          const int ncommit = (bad < 0) ? nt : bad;
+          for (int m = 0; m < ncommit; ++m) {
+            if (tid == 0) { dm[t + m] = wdet[m]; wscale[t + m] = scl; eflag[t + m] = werr[m] ? 1.0 : 0.0; }
+            if (t + m == 0) dm0 = wdet[m];
+            if (werr[m] || wdet[m] * dm0 <= 1e-200) errs++;
           }
+          if (ncommit) passes = 0;
+          t += ncommit;
+          if (bad >= 0) {
+            ++passes;
+            const double dt = wdet[bad];
+            const int er = werr[bad];
+            if (!er) {
+              if (fabs(dt) < 1e-250) scl *= 2;
+              else scl *= pow(fabs(dt), -1. / f.ns);
+            }
+            if (er || passes >= 200) {
+              // the reference repeats an erring evaluation unchanged; the
+              // oracle stops after 200 passes: commit the slice as it stands
+              if (tid == 0) { dm[t] = dt; wscale[t] = scl; eflag[t] = er ? 1.0 : 0.0; }
+              if (t == 0) dm0 = dt;
+              if (er || dt * dm0 <= 1e-200) errs++;
+              ++t;
+              passes = 0;
+            }
+          }
+          __syncthreads();
+        }
+        nerr += errs;
+        if (tid == 0) {
+          scale[fa] = scl;
+          if (lasterr) lasterr[fa] = eflag[c.S - 1] != 0.0;
         }
         __syncthreads();
+        if (f.nodal) node_distances(fa, 0, c.S, false, -1, wscale, 0.0, fn_dist(fa));
+        if (fcache) fn_store_records(fa, 0, c.S, dm, wscale, 0.0, eflag, fn_dist(fa));
+        __syncthreads();
       }
-      nerr += errs;
-      if (tid == 0) scale[fa] = scl;
-      __syncthreads();
-      if (f.nodal) node_distances(fa, 0, c.S, false, -1, wscale, 0.0, fn_dist(fa));
       // newDMValue(0)=dmValue(0); newDist(0)=dist(0)
       const int n = f.ns;
       double* nd = fn_newdist(fa);
@@ -659,22 +771,36 @@ struct Eng {
       const int nStride = 1 << (level + 1);
       const int t0 = nStride / 2;
       const int nt = (c.S - 1 - t0) / nStride + 1;
-      node_matrices(fa, t0, nStride, nt, true, i, scale[fa]);
       const double dm0 = fn_dm(fa)[0];
       int crossed = 0;
       double* ndm = fn_newdm(fa);
-      for (int m = 0; m < nt; ++m) {
-        if (werr[m] || wdet[m] * dm0 <= 1e-200) { crossed = 1; break; }
-        if (tid == 0) ndm[t0 + m * nStride] = wdet[m];
+      if (top_known && level == c.L) {
+        // the top level evaluates only the last slice, which no level moves:
+        // its determinant is the one initialize committed at the current scale
+        const double dl = fn_dm(fa)[c.S - 1];
+        crossed = lasterr[fa] || dl * dm0 <= 1e-200;
+        if (tid == 0) ndm[c.S - 1] = dl;
+      } else {
+        node_matrices(fa, t0, nStride, nt, true, i, scale[fa]);
+        for (int m = 0; m < nt; ++m) {
+          if (werr[m] || wdet[m] * dm0 <= 1e-200) { crossed = 1; break; }
+          if (tid == 0) ndm[t0 + m * nStride] = wdet[m];
+        }
       }
       __syncthreads();
       if (crossed) { total += 2e100; continue; }
       double dA = 0;
       if (f.nodal && level == 0) {
         double* nd = fn_newdist(fa);
-        node_distances(fa, 1, c.S - 1, true, i, nullptr, scale[fa], nd);
-        const double* od = fn_dist(fa);
         const int n = f.ns;
+        const double* od = fn_dist(fa);
+        if (top_known) {
+          // the last slice keeps its distances
+          for (int k = tid; k < 2 * n; k += NT) nd[(int64_t)(c.S - 1) * 2 * n + k] = od[(int64_t)(c.S - 1) * 2 * n + k];
+          node_distances(fa, 1, c.S - 2, true, i, nullptr, scale[fa], nd);
+        } else {
+          node_distances(fa, 1, c.S - 1, true, i, nullptr, scale[fa], nd);
+        }
         double part = 0;
         for (int it = tid; it < (c.S - 1) * n; it += NT) {
           const int t = 1 + it / n, k = it % n;
@@ -692,7 +818,8 @@ struct Eng {
     return total;
   }
 
-  // FixedNodeAction::acceptLastMove (FixedNodeAction.cc:359-367)
+  // FixedNodeAction::acceptLastMove (FixedNodeAction.cc:359-367); in the
+  // sampler the moved slices' records take the accepted values
   __device__ void fn_accept() {
     for (int fa = 0; fa < c.nfn; ++fa) {
       if (notmine[fa]) continue;
@@ -703,6 +830,7 @@ struct Eng {
       const double* nd = fn_newdist(fa);
       for (int k = tid; k < c.S; k += NT) dm[k] = ndm[k];
       for (int k = tid; k < 2 * c.S * n; k += NT) dd[k] = nd[k];
+      if (fcache) fn_store_records(fa, 1, c.S - 1, ndm, nullptr, scale[fa], nullptr, nd);
     }
     __syncthreads();
   }

@@ -2,6 +2,7 @@
 // and the HIP kernels.  One ChainP per Markov chain lives in device memory;
 // pair tables are deduplicated across chains and stored once.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define PQ_WAVE 64
@@ -60,8 +61,23 @@ struct ChainP {
   int64_t inv_off;       // element offset of this chain's [S][ns_max][ns_max] inverse cache
   int64_t fn_off;        // element offset of this chain's fixed-node state (global copy)
   int64_t egr_off;       // element offset of this chain's [S][ns_max][ns_max][D] element-gradient cache
+  int64_t fnc_off;       // element offset of this chain's slice-pair node records, -1: none
   int ns_max;            // largest fermion species with a node model
 };
+
+// Doubles of one slice-pair node record of a species of ns particles:
+// determinant, the scale it was evaluated at, a valid flag, and the nodal
+// distances of the lower slice's and the upper slice's particles.
+__host__ __device__ static inline int fn_record_doubles(int ns) { return 3 + 2 * ns; }
+
+// Slice-pair node records of a chain: every FixedNodeAction keeps M/2 of
+// them (slices t and t+M/2 form one pair); 0 when M is odd.
+static inline int64_t fn_cache_doubles(const ChainP& c) {
+  if (c.M & 1) return 0;
+  int64_t n = 0;
+  for (int a = 0; a < c.nfn; ++a) n += (int64_t)(c.M / 2) * fn_record_doubles(c.fn[a].ns);
+  return n;
+}
 
 // Sizes of the per-chain fixed-node state block (dm, newdm, dist, newdist per
 // FixedNodeAction, [S] and [S][2][ns]).

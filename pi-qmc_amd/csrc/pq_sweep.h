@@ -36,6 +36,9 @@ __global__ __launch_bounds__(NT, 2) void sweep_kernel(DevCtx x, int nsweep, Swee
   e.gjrow = reinterpret_cast<double*>(base + lay.off_gj);
   e.inv = (lay.off_inv >= 0) ? reinterpret_cast<double*>(base + lay.off_inv) : x.inv + c.inv_off;
   e.egr = x.egr + c.egr_off;
+  if (c.fnc_off >= 0) e.fcache = x.fncache + c.fnc_off;
+  e.top_known = true;
+  e.lasterr = e.notmine + PQ_MAX_NODES;
   // per-level try/accept counters, bead updates, accepted, node errors
   unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + lay.off_cnt);
   const int tid = e.tid;
@@ -80,6 +83,7 @@ __global__ __launch_bounds__(NT, 2) void sweep_kernel(DevCtx x, int nsweep, Swee
       }
       __syncthreads();
       if (c.nfn) {
+        e.sec_t0 = t0;
         const int ne = e.fn_initialize();
         if (tid == 0) cnt[18] += ne;
       }

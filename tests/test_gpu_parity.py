@@ -240,6 +240,30 @@ def test_batched_uneven_decisions():
         assert np.max(np.abs(b.get_paths(c) - oc.get_beads())) < 1e-9
 
 
+def test_node_records_follow_new_paths():
+    """The sampler reuses node determinants and distances of slice pairs
+    across sections; replacing the paths must drop them.  Sweeps, new paths,
+    more sweeps: decisions and paths stay the oracle's."""
+    print("[test] node_records_follow_new_paths", flush=True)
+    desc = small_cfg2(64)
+    b = capi.Batch([desc])
+    oc = ol.Chain(desc)
+    for k, seed in enumerate((21, 22)):
+        beads = capi.initial_beads(desc, seed)
+        b.set_paths(0, beads)
+        oc.set_beads(beads)
+        if k == 0:
+            b.seed(555, 0)
+            oc.rng_philox(555, 0)
+        b.run_sweeps(3)
+        b.sync()
+        oc.run_sweeps(3)
+        sg, so = b.stats(0), oc.stats()
+        assert sg.draws == so.draws
+        assert list(sg.accepts) == list(so.accepts)
+        assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9
+
+
 def test_mixed_precision_energy():
     """Mixed precision on the same Philox streams as FP64.  Both precisions
     move beads in FP64, so a chain whose decisions never differed ends on the
