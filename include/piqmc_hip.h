@@ -54,6 +54,17 @@ extern "C" {
 #define PQ_NODES_FREE 0         /* FreeParticleNodes (src/fixednode/FreeParticleNodes.cc:72-234) */
 #define PQ_NODES_SHO 1          /* SHONodes          (src/fixednode/SHONodes.cc:51-170) */
 
+/* Particle and permutation choosers of a <Sample> with one moving particle
+   (src/parser/PIMCParser.cc:364-453). */
+#define PQ_CHOOSER_SPECIES 0    /* species given, noPermutation="true": PermutationChooser +
+                                   SpeciesParticleChooser (SpeciesParticleChooser.cc:19-31) */
+#define PQ_CHOOSER_WALKING 1    /* species given, noPermutation absent or false: WalkingChooser
+                                   (WalkingChooser.cc:39-87), a 1-cycle for one particle */
+#define PQ_CHOOSER_ALL 2        /* species "" or "all", noPermutation="true": PermutationChooser +
+                                   SimpleParticleChooser (SimpleParticleChooser.cc:15-27) */
+#define PQ_CHOOSER_ALL_RANDOM 3 /* species "" or "all", noPermutation false: RandomPermutationChooser
+                                   (RandomPermutationChooser.cc:12-24) + SimpleParticleChooser */
+
 #define PQ_PREC_FP64 0          /* every operation in FP64 */
 #define PQ_PREC_MIXED 1         /* FP32 spring/SHO/pair link arithmetic with compensated
                                    sums; FP64 mover, node matrices, sign test, exponent */
@@ -97,10 +108,11 @@ typedef struct pq_node_desc {
   int nodal_action;    /* 1 unless noNodalAction="true" */
 } pq_node_desc;
 
-/* <Sample npart species noPermutation="true"> inside <ChooseSection>. */
+/* <Sample npart species noPermutation> inside <ChooseSection>. */
 typedef struct pq_sample_desc {
-  int species;
-  int nmoving;
+  int species;         /* ignored by the PQ_CHOOSER_ALL* choosers */
+  int nmoving;         /* 1 */
+  int chooser;         /* PQ_CHOOSER_* */
 } pq_sample_desc;
 
 /* One Markov chain (one clone of the reference).  Particle i of species s has

@@ -432,6 +432,11 @@ static int pg_index(const oq_chain* ch, int level, int sp, int d) {
 }
 
 oq_chain* oq_create(const pq_chain_desc* desc) {
+  /* the samplers here move one particle per Sample */
+  for (int k = 0; k < desc->nsample; ++k)
+    if (desc->sample[k].nmoving != 1 || desc->sample[k].chooser < PQ_CHOOSER_SPECIES ||
+        desc->sample[k].chooser > PQ_CHOOSER_ALL_RANDOM)
+      return NULL;
   oq_chain* ch = (oq_chain*)calloc(1, sizeof(oq_chain));
   ch->d = *desc;
   ch->D = desc->ndim;
@@ -1251,13 +1256,65 @@ static void choose_particles(oq_chain* ch, int sp, int nmov, int* idx) {
   }
 }
 
+/* Permutation and particle choice of a Sample with one moving particle, in
+   the order of MultiLevelSampler::run (src/advancer/MultiLevelSampler.cc:68-80)
+   and DoubleMLSampler::run (src/advancer/DoubleMLSampler.cc:76-81):
+   choosePermutation, then chooseParticles.  Returns 0 when the permutation
+   chooser turns the move down (isNewPerm false).
+   - SPECIES: PermutationChooser::choosePermutation draws nothing
+     (src/advancer/PermutationChooser.cc:13-15), then SpeciesParticleChooser.
+   - WALKING: WalkingChooser::choosePermutation for nsize = 1
+     (src/advancer/WalkingChooser.cc:39-87): i0 = (int)(npart u), the cycle
+     closes on i0 itself, so tranProb and revTranProb are the same number
+     h/t(i0,i0) and accept = 1 (NaN when t(i0,i0) = 0); the move goes ahead iff
+     a second uniform is < accept.  t(i0,i0) is the product of periodic
+     Gaussians of alpha = m/(2 tau 2^L) over the separation of particle i0's
+     first and last section beads (WalkingChooser.cc:20-25,89-103); its
+     chooseParticles draws nothing (:36-37).
+   - ALL / ALL_RANDOM: PermutationChooser, or RandomPermutationChooser for
+     nsize = 1 (one draw, src/advancer/RandomPermutationChooser.cc:12-24),
+     then SimpleParticleChooser over all particles, which draws again while
+     (int)(N u) == N (src/advancer/SimpleParticleChooser.cc:15-27). */
+static int choose(oq_chain* ch, const pq_sample_desc* s, int* idx) {
+  const int kind = s->chooser;
+  if (kind == PQ_CHOOSER_WALKING) {
+    const int sp = s->species, npart = ch->d.species[sp].count;
+    int i0 = (int)(npart * rng_next(&ch->rng));
+    if (i0 == npart) i0 = npart - 1;
+    i0 += ch->ifirst[sp];
+    double delta[3];
+    for (int d = 0; d < ch->D; ++d) delta[d] = SEC(ch->sec1, 0, i0, d) - SEC(ch->sec1, ch->S - 1, i0, d);
+    pbc(&ch->cell, delta);
+    double tii = 1;
+    for (int d = 0; d < ch->D; ++d) {
+      pg_t p;
+      pg_init(&p, ch->d.species[sp].mass / (2 * ch->d.tau * (1 << ch->L)), ch->d.cell[d]);
+      tii *= pg_eval(&p, delta[d]);
+    }
+    const double accept = (tii > 0) ? 1.0 : NAN;
+    idx[0] = i0;
+    return rng_next(&ch->rng) < accept;
+  }
+  if (kind == PQ_CHOOSER_ALL || kind == PQ_CHOOSER_ALL_RANDOM) {
+    if (kind == PQ_CHOOSER_ALL_RANDOM) (void)rng_next(&ch->rng);
+    const int N = ch->N;
+    for (;;) {
+      const int i = (int)(N * rng_next(&ch->rng));
+      if (i != N) { idx[0] = i; return 1; }
+    }
+  }
+  choose_particles(ch, s->species, 1, idx);
+  return 1;
+}
+
 /* MultiLevelSampler::run/tryMove (src/advancer/MultiLevelSampler.cc:68-177) and
-   DoubleMLSampler::run/tryMove (src/advancer/DoubleMLSampler.cc:66-245) for a
-   PermutationChooser (noPermutation), nrepeat 1, no delayed rejection. */
-static int sample(oq_chain* ch, int sp, int nmov) {
+   DoubleMLSampler::run/tryMove (src/advancer/DoubleMLSampler.cc:66-245) for one
+   moving particle, nrepeat 1, no delayed rejection. */
+static int sample(oq_chain* ch, const pq_sample_desc* smp) {
   const int D = ch->D, S = ch->S, L = ch->L;
+  const int nmov = 1;
   int* idx = ch->idx;
-  choose_particles(ch, sp, nmov, idx);
+  if (!choose(ch, smp, idx)) return 0;
   double* mov = ch->mov;
   for (int m = 0; m < nmov; ++m)
     for (int d = 0; d < D; ++d) {
@@ -1307,7 +1364,7 @@ static void run_section(oq_chain* ch) {
     if (t0 > ihi) t0 = ihi;
     get_beads(ch, t0, ch->sec1);
   }
-  for (int s = 0; s < ch->d.nsample; ++s) sample(ch, ch->d.sample[s].species, ch->d.sample[s].nmoving);
+  for (int s = 0; s < ch->d.nsample; ++s) sample(ch, &ch->d.sample[s]);
   put_beads(ch, t0, ch->sec1);
 }
 

@@ -387,8 +387,12 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
   c.nsample = d.nsample;
   for (int k = 0; k < d.nsample; ++k) {
     if (d.sample[k].nmoving != 1) return fail(PQ_ERR_ARG, "the device sampler moves one particle per Sample");
-    if (d.sample[k].species < 0 || d.sample[k].species >= c.nsp) return fail(PQ_ERR_ARG, "sample species");
-    c.sample_sp[k] = d.sample[k].species;
+    const int ch = d.sample[k].chooser;
+    if (ch < PQ_CHOOSER_SPECIES || ch > PQ_CHOOSER_ALL_RANDOM) return fail(PQ_ERR_ARG, "unknown chooser");
+    const bool all = ch == PQ_CHOOSER_ALL || ch == PQ_CHOOSER_ALL_RANDOM;
+    if (!all && (d.sample[k].species < 0 || d.sample[k].species >= c.nsp)) return fail(PQ_ERR_ARG, "sample species");
+    c.sample_sp[k] = all ? 0 : d.sample[k].species;
+    c.sample_ch[k] = ch;
   }
   c.nsection = d.nsection;
   return PQ_OK;

@@ -53,61 +53,63 @@ __device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
   return v;
 }
 
-// Row-per-lane Gauss-Jordan inversion with partial pivoting of one n x n
-// matrix (n <= NC <= 32) per G-lane group; every lane of the wave calls it.
-// Lane r < n holds row r of A (a) and of the identity (b); lanes r >= n take
-// no part.  Rows are not moved: each lane tracks the position its row would
-// have in the row-swapping elimination (mypos), the pivot of step k is the
-// largest |a[k]| among positions >= k with the lowest position on ties
-// (LAPACK's idamax), and every virtual swap flips the determinant's sign.  The
-// search is one group maximum of a 64-bit key, |a[k]|'s bits with the low six
-// replaced by 63 - mypos (non-negative doubles order as their bit patterns; a
-// difference in the last six mantissa bits counts as a tie).  The pivot row
-// reaches the group through LDS (row: 2 NC doubles per group).  Columns < k
-// of A are finished at step k and are no longer updated.  On exit the lane
-// with mypos = k holds row k of A^-1 in b, and det = prod(pivots) *
-// (-1)^swaps, the LU determinant with the same pivots.
+// Row-per-lane in-place Gauss-Jordan inversion with partial pivoting of one
+// n x n matrix (n <= NC <= 32) per G-lane group; every lane of the wave calls
+// it with the same n.  Lane r < NC holds row r of diag(A, I): rows n..NC-1
+// are identity rows, which no step k < n touches (their column k is zero) and
+// which make the steps k >= n exact no-ops (pivot 1, no swap), so the
+// elimination exits only every fourth step and the unrolled steps keep their
+// registers.  Lanes r >= NC take no part.  Rows are not moved: each lane
+// tracks the position its row would have in the row-swapping elimination
+// (mypos), the pivot of step k is the largest |a[k]| among positions >= k
+// with the lowest position on ties (LAPACK's idamax), and every virtual swap
+// flips the determinant's sign.  The search is one group maximum of a 64-bit
+// key, |a[k]|'s bits with the low six replaced by 63 - mypos (non-negative
+// doubles order as their bit patterns; a difference in the last six mantissa
+// bits counts as a tie).  The pivot row reaches the group through LDS (row:
+// NC doubles per group).  Step k replaces column k by the inverse's column
+// as it goes (pivot row: a[j]/piv, a[k] = 1/piv; other rows: a[j] - f a_p[j],
+// a[k] = -f, f = a[k]/piv).  On exit the lane with mypos = k < n holds row k
+// of A^-1 with its columns in pivot order: a[j] is column P(j), P(j) the
+// row that was pivot at step j.  det = prod(pivots) * (-1)^swaps, the LU
+// determinant with the same pivots.
 template <int G, int NC>
-__device__ __forceinline__ void gj_invert(double (&a)[NC], double (&b)[NC], int r, int n,
-                                          double* row, double& det, int& err, int& mypos) {
+__device__ __forceinline__ void gj_invert(double (&a)[NC], int r, int n, double* row, double& det,
+                                          int& err, int& mypos) {
   det = 1.0;
   err = 0;
   mypos = r;
-  const bool act = r < n;
+  const bool act = r < NC;
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    if (k < n) {
-      const uint64_t key = (act && mypos >= k)
-          ? ((__double_as_longlong(fabs(a[k])) & ~0x3Full) | (uint64_t)(63 - mypos)) : 0ull;
-      const uint64_t kmax = group_max_u64<G>(key);
-      const int ppos = 63 - (int)(kmax & 0x3F);
-      const bool isp = act && key == kmax;
-      if (ppos != k) det = -det;
-      if (isp) {
+    if ((k & 3) == 0 && k >= n) break;
+    const uint64_t key = (act && mypos >= k)
+        ? ((__double_as_longlong(fabs(a[k])) & ~0x3Full) | (uint64_t)(63 - mypos)) : 0ull;
+    const uint64_t kmax = group_max_u64<G>(key);
+    const int ppos = 63 - (int)(kmax & 0x3F);
+    const bool isp = act && key == kmax;
+    if (ppos != k) det = -det;
+    if (isp) {
 #pragma unroll
-        for (int cc = k; cc < NC; ++cc) row[cc] = a[cc];
-#pragma unroll
-        for (int cc = 0; cc < NC; ++cc) row[NC + cc] = b[cc];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      const double piv = row[k];
-      det *= piv;
-      // one division per step; a singular column leaves every row unchanged
-      const bool zero = piv == 0.0;
-      if (zero) err = 1;
-      const double ipiv = zero ? 0.0 : 1.0 / piv;
-      const double f = (act && !isp) ? a[k] * ipiv : 0.0;
-      const double sp = (isp && !zero) ? ipiv : 1.0;
-      // pivot row: scaled by 1/piv (f = 0); other rows: minus f * pivot row
-#pragma unroll
-      for (int cc = k + 1; cc < NC; ++cc) a[cc] = fma(-f, row[cc], a[cc]) * sp;
-#pragma unroll
-      for (int cc = 0; cc < NC; ++cc) b[cc] = fma(-f, row[NC + cc], b[cc]) * sp;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      // the row at position k moves to the pivot's old position
-      if (isp) mypos = k;
-      else if (mypos == k) mypos = ppos;
+      for (int cc = 0; cc < NC; ++cc) row[cc] = a[cc];
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const double piv = row[k];
+    det *= piv;
+    // one division per step; a singular column leaves every row unchanged
+    const bool zero = piv == 0.0;
+    if (zero) err = 1;
+    const double ipiv = zero ? 0.0 : 1.0 / piv;
+    const double f = (act && !isp) ? a[k] * ipiv : 0.0;
+    const double sp = (isp && !zero) ? ipiv : 1.0;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc)
+      if (cc != k) a[cc] = fma(-f, row[cc], a[cc]) * sp;
+    if (!zero && act) a[k] = isp ? ipiv : -f;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // the row at position k moves to the pivot's old position
+    if (isp) mypos = k;
+    else if (mypos == k) mypos = ppos;
   }
 }
 
@@ -163,6 +165,57 @@ struct Eng {
 
   __device__ __forceinline__ double* sb(double* s, int t, int i) const {
     return s + ((int64_t)t * c.N + i) * D;
+  }
+
+  // ------------------------------------------------------------ choosers
+  // choosePermutation then chooseParticles of Sample smp with one moving
+  // particle (MultiLevelSampler.cc:68-80, DoubleMLSampler.cc:76-81); false
+  // when the permutation chooser turns the move down.
+  //   SPECIES     PermutationChooser (no draw) + SpeciesParticleChooser
+  //               (SpeciesParticleChooser.cc:19-31)
+  //   WALKING     WalkingChooser for a 1-cycle (WalkingChooser.cc:39-87):
+  //               i0 = (int)(npart u); forward and reverse probabilities are
+  //               the same number h/t(i0,i0), so the move goes ahead iff a
+  //               second uniform is < 1, and never when t(i0,i0) = 0 (the
+  //               ratio is NaN); t(i0,i0) = prod_d PG_{m/(2 tau 2^L)} of the
+  //               separation of i0's first and last section beads (:89-103)
+  //   ALL         PermutationChooser + SimpleParticleChooser over every
+  //               particle, redrawing (int)(N u) == N (SimpleParticleChooser.cc:15-27)
+  //   ALL_RANDOM  RandomPermutationChooser of one element (one draw,
+  //               RandomPermutationChooser.cc:12-24) + SimpleParticleChooser
+  __device__ bool choose(int smp, uint64_t& k, int& i) {
+    const int kind = c.sample_ch[smp];
+    const int sp = c.sample_sp[smp];
+    if (kind == PQ_CHOOSER_WALKING) {
+      int i0 = (int)(c.count[sp] * U(k++));
+      if (i0 == c.count[sp]) i0 = c.count[sp] - 1;
+      i = i0 + c.ifirst[sp];
+      double delta[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = sb(sec1, 0, i)[d] - sb(sec1, c.S - 1, i)[d];
+      pbc<D>(c, delta);
+      const double alpha = c.mass[sp] / (2 * c.tau * (1 << c.L));
+      double tii = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const double len = c.a[d];
+        tii *= (alpha * len * len >= 20.0) ? pg_images<double>(alpha, len, delta[d], nullptr)
+                                           : pg_eval(pg_make(alpha, len), delta[d]);
+      }
+      const double accept = tii > 0 ? 1.0 : __builtin_nan("");
+      return U(k++) < accept;
+    }
+    if (kind == PQ_CHOOSER_ALL || kind == PQ_CHOOSER_ALL_RANDOM) {
+      if (kind == PQ_CHOOSER_ALL_RANDOM) ++k;
+      for (;;) {
+        const int j = (int)(c.N * U(k++));
+        if (j != c.N) { i = j; return true; }
+      }
+    }
+    int j = (int)(c.count[sp] * U(k++) + c.ifirst[sp]);
+    if (j == c.count[sp] + c.ifirst[sp]) j = c.count[sp] + c.ifirst[sp] - 1;
+    i = j;
+    return true;
   }
 
   // ------------------------------------------------------------ mover
@@ -468,26 +521,29 @@ struct Eng {
         const int m = base + g;
         const bool live = m < nt;
         const int t = t0 + m * step;
-        double a[NC], b[NC];
+        double a[NC];
         const double* src = inv + ((int64_t)t * ins + r) * ins;
 #pragma unroll
-        for (int cc = 0; cc < NC; ++cc) {
-          a[cc] = (live && r < n && cc < n) ? src[cc] : 0.0;
-          b[cc] = (cc == r) ? 1.0 : 0.0;
-        }
+        for (int cc = 0; cc < NC; ++cc)
+          a[cc] = (live && r < n && cc < n) ? src[cc] : (cc == r ? 1.0 : 0.0);
         double det;
         int err, pos;
-        gj_invert<G, NC>(a, b, r, live ? n : 0, row, det, err, pos);
+        gj_invert<G, NC>(a, r, n, row, det, err, pos);
+        // column j of the result is inverse column P(j): P(pos) = r
+        int* perm = reinterpret_cast<int*>(row);
+        if (live && r < n) perm[pos] = r;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         if (live) {
-          // a singular matrix can leave a row without a leading position
-          if (r < n && pos < n) {
+          if (r < n) {
             double* dst = inv + ((int64_t)t * ins + pos) * ins;
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc)
-              if (cc < n) dst[cc] = b[cc];
+              if (cc < n) dst[perm[cc]] = a[cc];
           }
           if (r == 0) { wdet[m] = det; werr[m] = err; }
         }
+        // the next pass rewrites the row buffer that perm aliases
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
     }
     __syncthreads();

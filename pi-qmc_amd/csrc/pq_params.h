@@ -50,6 +50,7 @@ struct ChainP {
   int mover;
   int nsample;
   int sample_sp[8];
+  int sample_ch[8];      // PQ_CHOOSER_* of each Sample
   int nsection;
   double spring_c[PQ_PG_LEVELS][4];          // 1/(4 lambda tau 2^l) per level, species
   double sho_cosh[4][PQ_PG_LEVELS];          // cosh(omega tau 2^l) per SHO action

@@ -89,10 +89,8 @@ __global__ __launch_bounds__(NT, 2) void sweep_kernel(DevCtx x, int nsweep, Swee
       }
 
       for (int smp = 0; smp < c.nsample; ++smp) {
-        const int sp = c.sample_sp[smp];
-        // SpeciesParticleChooser::chooseParticles (SpeciesParticleChooser.cc:19-31), npart=1
-        int i = (int)(c.count[sp] * e.U(k++) + c.ifirst[sp]);
-        if (i == c.count[sp] + c.ifirst[sp]) i = c.count[sp] + c.ifirst[sp] - 1;
+        int i;
+        if (!e.choose(smp, k, i)) continue;
         if (tid < D) {
           e.mov[tid] = e.sb(e.sec1, 0, i)[tid];
           e.mov[(S - 1) * D + tid] = e.sb(e.sec1, S - 1, i)[tid];

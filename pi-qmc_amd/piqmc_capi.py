@@ -22,6 +22,10 @@ PQ_MOVER_FREE = 0
 PQ_MOVER_FREE_PBC = 1
 PQ_NODES_FREE = 0
 PQ_NODES_SHO = 1
+PQ_CHOOSER_SPECIES = 0
+PQ_CHOOSER_WALKING = 1
+PQ_CHOOSER_ALL = 2
+PQ_CHOOSER_ALL_RANDOM = 3
 PQ_PREC_FP64 = 0
 PQ_PREC_MIXED = 1
 
@@ -46,7 +50,7 @@ class NodeDesc(C.Structure):
 
 
 class SampleDesc(C.Structure):
-    _fields_ = [("species", C.c_int), ("nmoving", C.c_int)]
+    _fields_ = [("species", C.c_int), ("nmoving", C.c_int), ("chooser", C.c_int)]
 
 
 class ChainDesc(C.Structure):
@@ -78,7 +82,8 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
               nsection=None):
     """species: [(count, mass, charge)]; sho: [(species, omega)];
     coulomb: dict(epsilon, norder, rmin, rmax, ngpts) or None;
-    nodes: [(species, model, omega)]; samples default one npart=1 per species;
+    nodes: [(species, model, omega)]; samples [(species, npart[, chooser])],
+    default one npart=1 SpeciesParticleChooser per species;
     nsection default M / 2^L (one sweep touches every slice once on average)."""
     d = ChainDesc()
     d.ndim, d.nslice, d.nlevel, d.nspecies, d.tau = ndim, nslice, nlevel, len(species), tau
@@ -106,8 +111,9 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
     if samples is None:
         samples = [(s, 1) for s in range(len(species))]
     d.nsample = len(samples)
-    for k, (s, nm) in enumerate(samples):
-        d.sample[k].species, d.sample[k].nmoving = s, nm
+    for k, smp in enumerate(samples):
+        d.sample[k].species, d.sample[k].nmoving = smp[0], smp[1]
+        d.sample[k].chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
     d.nsection = nsection if nsection is not None else max(1, nslice >> nlevel)
     return d
 

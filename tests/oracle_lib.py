@@ -103,6 +103,8 @@ class Chain:
         self._d = capi.ChainDesc()
         C.pointer(self._d)[0] = desc
         self.h = self.lib.oq_create(C.byref(self._d))
+        if not self.h:
+            raise ValueError("configuration not supported by the oracle")
         self.N, self.D, self.M = desc.npart, desc.ndim, desc.nslice
         self.S = (1 << desc.nlevel) + 1
         self._keep = None

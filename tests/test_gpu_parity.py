@@ -21,7 +21,17 @@ def _descs():
     pbc = capi.make_desc(3, 64, 3, 0.05, [(2, 1.0, -1.0), (2, 2.0, 1.0)], cell=4.0,
                          coulomb=dict(epsilon=1.0, norder=3, rmin=0.0, rmax=0.0, ngpts=512),
                          mover=capi.PQ_MOVER_FREE_PBC)
-    return {"cfg1": c1, "cfg2": c2, "pbc": pbc}
+    # the reference's other choosers of one moving particle
+    walk1 = capi.cfg1_desc()
+    for k in range(walk1.nsample):
+        walk1.sample[k].chooser = capi.PQ_CHOOSER_WALKING
+    all1 = capi.cfg1_desc()
+    all1.sample[0].chooser = capi.PQ_CHOOSER_ALL
+    all1.sample[1].chooser = capi.PQ_CHOOSER_ALL_RANDOM
+    walk2 = small_cfg2()
+    for k in range(walk2.nsample):
+        walk2.sample[k].chooser = capi.PQ_CHOOSER_WALKING
+    return {"cfg1": c1, "cfg2": c2, "pbc": pbc, "cfg1_walk": walk1, "cfg1_all": all1, "cfg2_walk": walk2}
 
 
 @pytest.mark.parametrize("name", ["cfg1", "cfg2", "pbc"])
@@ -165,7 +175,9 @@ def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False, tea
 
 @pytest.mark.parametrize("name,inject,team,inv", [("cfg1", True, -1, -1), ("cfg1", False, 128, -1),
                                                    ("cfg2", True, 64, 0), ("cfg2", True, 128, 1),
-                                                   ("cfg2", False, 64, 1), ("pbc", False, -1, -1)])
+                                                   ("cfg2", False, 64, 1), ("pbc", False, -1, -1),
+                                                   ("cfg1_walk", True, -1, -1), ("cfg1_all", True, -1, -1),
+                                                   ("cfg2_walk", False, -1, -1)])
 def test_sampler_decisions(name, inject, team, inv):
     """Same uniforms in, same accept/reject decisions and paths out, for every schedule."""
     print(f"[test] sampler_decisions {name} inject={inject} team={team} inv={inv}", flush=True)
@@ -238,6 +250,30 @@ def test_batched_uneven_decisions():
         assert sg.draws == so.draws, c
         assert list(sg.accepts) == list(so.accepts), c
         assert np.max(np.abs(b.get_paths(c) - oc.get_beads())) < 1e-9
+
+
+def test_walking_chooser_veto_on_device():
+    """An injected uniform of exactly 1.0 in WalkingChooser's veto slot turns
+    every move down on the device as in the oracle; just below 1 the move runs."""
+    print("[test] walking_veto", flush=True)
+    d = capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0), (2, 1.0, 0.0)],
+                       samples=[(1, 1, capi.PQ_CHOOSER_WALKING)], nsection=1)
+    beads = capi.initial_beads(d, 3)
+    for veto, want_draws in ((1.0, 3), (0.999999, 12)):
+        u = np.array([0.3, 0.75, veto] + [1e-6] * 64)
+        b = capi.Batch([d])
+        b.set_paths(0, beads)
+        b.set_uniforms(u[None, :])
+        b.run_sweeps(1)
+        b.sync()
+        oc = ol.Chain(d)
+        oc.set_beads(beads)
+        oc.rng_inject(u)
+        oc.run_sweeps(1)
+        sg, so = b.stats(0), oc.stats()
+        assert sg.draws == so.draws == want_draws
+        assert list(sg.tries) == list(so.tries)
+        assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-12
 
 
 def test_node_records_follow_new_paths():

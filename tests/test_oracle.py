@@ -393,3 +393,58 @@ def test_decisions_repeatable_and_counted():
     assert sa.tries[3] == 3 * d.nsection * 2
     assert sa.bead_updates == sum(sa.tries[l] << (2 - l) for l in range(3))
     assert sa.draws >= 3 * d.nsection * 3 + tries
+
+
+# ------------------------------------------------------------- choosers
+# One section (one draw: no node action), one Sample on species 1 (particles
+# 3 and 4 of 5), free particles, L = 2 in 2D.  After the chooser's draws every
+# uniform is 1e-6: Box-Muller displacements stay small and every level
+# accepts, so the draws after the chooser are 9 (levels 2, 1, 0: 0 + 2 + 4
+# Gaussian uniforms and 3 accept draws).
+
+def _chooser_run(chooser, prefix):
+    d = capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0), (2, 1.0, 0.0)],
+                       samples=[(1, 1, chooser)], nsection=1)
+    ch = ol.Chain(d)
+    beads = capi.initial_beads(d, 3)
+    ch.set_beads(beads)
+    ch.rng_inject(np.array([0.3] + list(prefix) + [1e-6] * 64))
+    ch.run_sweeps(1)
+    moved = [p for p in range(d.npart) if not np.array_equal(ch.get_beads()[:, p], beads[:, p])]
+    return ch.stats(), moved
+
+
+def test_walking_chooser_draws_particle_then_veto():
+    # i0 = (int)(2 * 0.75) = 1 -> particle 3 + 1; the veto uniform is below 1
+    st, moved = _chooser_run(capi.PQ_CHOOSER_WALKING, [0.75, 0.999999])
+    assert moved == [4]
+    assert st.draws == 1 + 2 + 9 and st.tries[2] == 1
+
+
+def test_walking_chooser_vetoes_only_at_one():
+    # a veto uniform of exactly 1.0 (rand() == RAND_MAX) turns the move down
+    st, moved = _chooser_run(capi.PQ_CHOOSER_WALKING, [0.75, 1.0])
+    assert moved == [] and sum(st.tries) == 0 and st.draws == 3
+
+
+def test_species_chooser_offsets_species():
+    # (int)(2 * 0.25 + 3) = 3: the first particle of species 1
+    st, moved = _chooser_run(capi.PQ_CHOOSER_SPECIES, [0.25])
+    assert moved == [3] and st.draws == 1 + 1 + 9
+
+
+def test_all_chooser_redraws_at_one():
+    # (int)(5 * 1.0) == 5 is drawn again; (int)(5 * 0.5) = 2
+    st, moved = _chooser_run(capi.PQ_CHOOSER_ALL, [1.0, 0.5])
+    assert moved == [2] and st.draws == 1 + 2 + 9
+
+
+def test_all_random_chooser_spends_one_permutation_draw():
+    st, moved = _chooser_run(capi.PQ_CHOOSER_ALL_RANDOM, [0.7, 0.1])
+    assert moved == [0] and st.draws == 1 + 2 + 9
+
+
+def test_oracle_rejects_several_moving_particles():
+    d = capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0)], samples=[(0, 2)])
+    with pytest.raises(ValueError):
+        ol.Chain(d)
