@@ -162,7 +162,7 @@ struct pq_batch {
   int precision = 0;
   int nchain = 0;
   int D = 3;
-  int NC = 4;     // node-matrix width class of the batch: 4, 8, 12, 16 or 32
+  int NC = 4;     // node-matrix width class of the batch: 4, 8, 10, 12, 16 or 32
   hipStream_t stream = nullptr;
   std::vector<pq_chain_desc> descs;
   std::vector<ChainP> hc;
@@ -469,7 +469,8 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     if (rc) { delete b; return rc; }
     ns_max = std::max(ns_max, b->hc[c].ns_max);
   }
-  b->NC = ns_max <= 4 ? 4 : ns_max <= 8 ? 8 : ns_max <= 12 ? 12 : ns_max <= 16 ? 16 : 32;
+  // node-matrix row width: the instantiated widths, the smallest that fits
+  b->NC = ns_max <= 4 ? 4 : ns_max <= 8 ? 8 : ns_max <= 10 ? 10 : ns_max <= 12 ? 12 : ns_max <= 16 ? 16 : 32;
   for (int c = 0; c < nchain; ++c) {
     ChainP& ch = b->hc[c];
     ch.bead_off = beads;

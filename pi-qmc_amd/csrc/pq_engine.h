@@ -66,10 +66,11 @@ __device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
 // flips the determinant's sign.  The search is one group maximum of a 64-bit
 // key, |a[k]|'s bits with the low six replaced by 63 - mypos (non-negative
 // doubles order as their bit patterns; a difference in the last six mantissa
-// bits counts as a tie).  The pivot row reaches the group through LDS (row:
-// NC doubles per group).  Step k replaces column k by the inverse's column
-// as it goes (pivot row: a[j]/piv, a[k] = 1/piv; other rows: a[j] - f a_p[j],
-// a[k] = -f, f = a[k]/piv).  On exit the lane with mypos = k < n holds row k
+// bits counts as a tie).  The pivot row and its reciprocal pivot reach the
+// group through LDS (row: NC + 1 doubles per group).  Step k replaces column
+// k by the inverse's column as it goes (pivot row: a[j]/piv, a[k] = 1/piv;
+// other rows: a[j] - f a_p[j], a[k] = -f, f = a[k]/piv).  On exit the lane
+// with mypos = k < n holds row k
 // of A^-1 with its columns in pivot order: a[j] is column P(j), P(j) the
 // row that was pivot at step j.  det = prod(pivots) * (-1)^swaps, the LU
 // determinant with the same pivots.
@@ -85,6 +86,9 @@ __device__ __forceinline__ void gj_invert(double (&a)[NC], int r, int n, double*
     if ((k & 3) == 0 && k >= n) break;
     const uint64_t key = (act && mypos >= k)
         ? ((__double_as_longlong(fabs(a[k])) & ~0x3Full) | (uint64_t)(63 - mypos)) : 0ull;
+    // every lane inverts its own candidate while the pivot search runs; the
+    // pivot lane's reciprocal travels with its row
+    const double myinv = 1.0 / a[k];
     const uint64_t kmax = group_max_u64<G>(key);
     const int ppos = 63 - (int)(kmax & 0x3F);
     const bool isp = act && key == kmax;
@@ -92,14 +96,15 @@ __device__ __forceinline__ void gj_invert(double (&a)[NC], int r, int n, double*
     if (isp) {
 #pragma unroll
       for (int cc = 0; cc < NC; ++cc) row[cc] = a[cc];
+      row[NC] = myinv;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const double piv = row[k];
     det *= piv;
-    // one division per step; a singular column leaves every row unchanged
+    // a singular column leaves every row unchanged
     const bool zero = piv == 0.0;
     if (zero) err = 1;
-    const double ipiv = zero ? 0.0 : 1.0 / piv;
+    const double ipiv = zero ? 0.0 : row[NC];
     const double f = (act && !isp) ? a[k] * ipiv : 0.0;
     const double sp = (isp && !zero) ? ipiv : 1.0;
 #pragma unroll

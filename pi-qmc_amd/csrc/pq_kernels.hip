@@ -301,6 +301,7 @@ __global__ __launch_bounds__(256) void histogram_kernel(DevCtx x, int s1, int s2
     switch (NC) {                                                                     \
       case 4: return FN<D_, 4, R_>(__VA_ARGS__);                                      \
       case 8: return FN<D_, 8, R_>(__VA_ARGS__);                                      \
+      case 10: return FN<D_, 10, R_>(__VA_ARGS__);                                    \
       case 12: return FN<D_, 12, R_>(__VA_ARGS__);                                    \
       case 16: return FN<D_, 16, R_>(__VA_ARGS__);                                    \
       default: return FN<D_, 32, R_>(__VA_ARGS__);                                    \

@@ -182,6 +182,7 @@ static hipError_t launch_sweep_dr(int NC, const DevCtx& x, int nchain, int nswee
   switch (NC) {
     case 4: return launch_sweep_t<D, 4, R>(x, nchain, nsweep, L, st);
     case 8: return launch_sweep_t<D, 8, R>(x, nchain, nsweep, L, st);
+    case 10: return launch_sweep_t<D, 10, R>(x, nchain, nsweep, L, st);
     case 12: return launch_sweep_t<D, 12, R>(x, nchain, nsweep, L, st);
     case 16: return launch_sweep_t<D, 16, R>(x, nchain, nsweep, L, st);
     default: return launch_sweep_t<D, 32, R>(x, nchain, nsweep, L, st);
