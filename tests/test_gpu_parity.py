@@ -111,6 +111,43 @@ def test_node_evaluate(model):
             assert rel_close(d2[m * ns:(m + 1) * ns], e2, 1e-11)
 
 
+@pytest.mark.parametrize("ns,model", [(14, capi.PQ_NODES_SHO), (24, capi.PQ_NODES_FREE)])
+def test_large_species_nodes(ns, model):
+    """Node matrices of 14 and 24 particles (the 16- and 32-wide instances):
+    determinants and distances against the oracle, then a few sampler sweeps
+    with identical decisions."""
+    print(f"[test] large_species_nodes ns={ns} model={model}", flush=True)
+    desc = capi.make_desc(3, 64, 3, 0.05, [(ns, 1.0, -1.0), (2, 1.0, -1.0)], cell=12.0,
+                          sho=[(0, 1.0), (1, 1.0)], nodes=[(0, model, 1.0)], nsection=2)
+    b = capi.Batch([desc])
+    oc = ol.Chain(desc)
+    rng = np.random.default_rng(ns)
+    nmat = 5
+    r1 = rng.uniform(-2.0, 2.0, size=(nmat, desc.npart, 3))
+    r2 = r1 + 0.3 * rng.standard_normal(r1.shape)
+    lib = capi.load_library()
+    det = np.zeros(nmat)
+    err = np.zeros(nmat, dtype=np.int32)
+    d1 = np.zeros(nmat * ns)
+    d2 = np.zeros(nmat * ns)
+    capi.check(lib.pq_node_evaluate(b.h, 0, 0, nmat, capi.dptr(np.ascontiguousarray(r1)),
+                                    capi.dptr(np.ascontiguousarray(r2)), capi.dptr(det),
+                                    capi.iptr(err), capi.dptr(d1), capi.dptr(d2)))
+    for m in range(nmat):
+        dw, ew = oc.node_evaluate(0, r1[m], r2[m], 0)
+        e1, e2 = oc.node_distance(0, r1[m], r2[m], 0, ns)
+        assert err[m] == ew
+        # larger matrices: the condition number scales the rounding of both LUs
+        assert rel_close(det[m], dw, 1e-10, scale=0.0), (m, det[m], dw)
+        assert rel_close(d1[m * ns:(m + 1) * ns], e1, 1e-9)
+        assert rel_close(d2[m * ns:(m + 1) * ns], e2, 1e-9)
+    bs, os_ = _paired(desc, 3, seed=11)
+    sg, so = bs.stats(0), os_.stats()
+    assert sg.draws == so.draws
+    assert list(sg.accepts) == list(so.accepts)
+    assert np.max(np.abs(bs.get_paths(0) - os_.get_beads())) < 1e-9
+
+
 def test_fixed_node_protocol():
     """initialize, per-level action difference (with crossings) and accept on one chain."""
     print("[test] fixed_node_protocol", flush=True)
