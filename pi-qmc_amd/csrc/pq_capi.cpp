@@ -181,6 +181,7 @@ struct pq_batch {
   double* degr = nullptr;
   double* dfn = nullptr;
   double* dfncache = nullptr;   // slice-pair node records of the sampler
+  double* dfnsweep = nullptr;   // sampler fixed-node state when it does not fit LDS
   int64_t nfncache = 0;
   double* dscale = nullptr;
   uint64_t* dcounter = nullptr;
@@ -211,6 +212,7 @@ static DevCtx make_ctx(const pq_batch* b) {
   x.egr = b->degr;
   x.fnstate = b->dfn;
   x.fncache = b->dfncache;
+  x.fnsweep = b->dfnsweep;
   x.scale = b->dscale;
   x.counter = b->dcounter;
   x.stats = b->dstats;
@@ -417,7 +419,6 @@ static void make_layout(pq_batch* b, int team_req, int inv_req) {
   b->lay.off_sec1 = (int)o; o += mSND * 8;
   b->lay.off_sec2 = (int)o; o += mSND * 8;
   b->lay.off_mov = (int)o; o += align8(mSD * 8);
-  b->lay.off_fst = (int)o; o += mF * 8;
   b->lay.off_wdet = (int)o; o += PQ_WAVE * 8;
   b->lay.off_wscale = (int)o; o += align8(mS * 8);
   b->lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
@@ -427,6 +428,14 @@ static void make_layout(pq_batch* b, int team_req, int inv_req) {
   b->lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
   o = align8(o);
   b->lay.off_gj = (int)o; o += 2 * 8 * 128;
+  // fixed-node state: LDS while the workgroup stays within the 40 KB that
+  // keeps four chains per CU, else global memory
+  if (o + mF * 8 <= 40 * 1024) {
+    b->lay.off_fst = (int)o;
+    o += mF * 8;
+  } else {
+    b->lay.off_fst = -1;
+  }
   // automatic: keep the inverse cache in LDS while four chains still fit on one CU
   bool inv_lds = nodes && o + mInv * 8 <= 40 * 1024;
   if (inv_req == 0) inv_lds = false;
@@ -522,6 +531,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
             dalloc((void**)&b->degr, sizeof(double) * egr) &&
             dalloc((void**)&b->dfn, sizeof(double) * fnst) &&
             dalloc((void**)&b->dfncache, sizeof(double) * fnc) &&
+            dalloc((void**)&b->dfnsweep, sizeof(double) * fnst) &&
             dalloc((void**)&b->dscale, sizeof(double) * nchain * PQ_MAX_NODES) &&
             dalloc((void**)&b->dcounter, sizeof(uint64_t) * nchain) &&
             dalloc((void**)&b->dstats, sizeof(pq_chain_stats) * nchain) &&
@@ -551,7 +561,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache,
+  void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache, b->dfnsweep,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -32,6 +32,7 @@ struct DevCtx {
   double* egr;            // per-chain [S][ns_max][ns_max][D] density-matrix gradient factors
   double* fnstate;        // per-chain fixed-node state (call-level API)
   double* fncache;        // per-chain node records of every slice pair (sampler)
+  double* fnsweep;        // per-chain fixed-node state of the sampler when not in LDS
   double* scale;          // [nchain][PQ_MAX_NODES]
   uint64_t* counter;      // uniforms consumed per chain
   pq_chain_stats* stats;

@@ -26,7 +26,7 @@ __global__ __launch_bounds__(NT, 2) void sweep_kernel(DevCtx x, int nsweep, Swee
   e.sec1 = reinterpret_cast<double*>(base + lay.off_sec1);
   e.sec2 = reinterpret_cast<double*>(base + lay.off_sec2);
   e.mov = reinterpret_cast<double*>(base + lay.off_mov);
-  e.fst = reinterpret_cast<double*>(base + lay.off_fst);
+  e.fst = (lay.off_fst >= 0) ? reinterpret_cast<double*>(base + lay.off_fst) : x.fnsweep + c.fn_off;
   e.wdet = reinterpret_cast<double*>(base + lay.off_wdet);
   e.werr = reinterpret_cast<int*>(base + lay.off_werr);
   e.wscale = reinterpret_cast<double*>(base + lay.off_wscale);

@@ -111,13 +111,16 @@ def test_node_evaluate(model):
             assert rel_close(d2[m * ns:(m + 1) * ns], e2, 1e-11)
 
 
-@pytest.mark.parametrize("ns,model", [(14, capi.PQ_NODES_SHO), (24, capi.PQ_NODES_FREE)])
-def test_large_species_nodes(ns, model):
-    """Node matrices of 14 and 24 particles (the 16- and 32-wide instances):
-    determinants and distances against the oracle, then a few sampler sweeps
-    with identical decisions."""
-    print(f"[test] large_species_nodes ns={ns} model={model}", flush=True)
-    desc = capi.make_desc(3, 64, 3, 0.05, [(ns, 1.0, -1.0), (2, 1.0, -1.0)], cell=12.0,
+@pytest.mark.parametrize("ns,model,L", [(14, capi.PQ_NODES_SHO, 3), (24, capi.PQ_NODES_FREE, 3),
+                                        (32, capi.PQ_NODES_SHO, 4)])
+def test_large_species_nodes(ns, model, L):
+    """Node matrices of 14, 24 and 32 particles (the 16- and 32-wide
+    instances; at 32 particles and L = 4 the sampler's fixed-node state no
+    longer fits the LDS budget and lives in global memory): determinants and
+    distances against the oracle, then a few sampler sweeps with identical
+    decisions."""
+    print(f"[test] large_species_nodes ns={ns} model={model} L={L}", flush=True)
+    desc = capi.make_desc(3, 64, L, 0.05, [(ns, 1.0, -1.0), (2, 1.0, -1.0)], cell=12.0,
                           sho=[(0, 1.0), (1, 1.0)], nodes=[(0, model, 1.0)], nsection=2)
     b = capi.Batch([desc])
     oc = ol.Chain(desc)
