@@ -115,17 +115,29 @@ def timed_sweeps(b, steps):
     return ms, time.perf_counter() - t0
 
 
-def run_gpu(descs, precision, steps, warmup, seed):
+def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=None):
+    """warmup sweeps (the autotuner's timing sweeps count among them, and
+    total_warmup, when given, pads the warmup to that many sweeps), then steps
+    timed sweeps."""
     b = make_batch(descs, precision, seed)
-    b.run_sweeps(warmup)
+    tuned, done = None, 0
+    if autotune:
+        tuned, done = b.autotune(1)
+    extra = warmup if total_warmup is None else max(0, total_warmup - done)
+    b.run_sweeps(extra)
     b.sync()
+    done += extra
     capi.check(b.lib.pq_reset_stats(b.h))
     ms, wall = timed_sweeps(b, steps)
     st = all_stats(b)
     bu = np.array([s.bead_updates for s in st], dtype=np.float64)
     abu = np.array([s.accepted_bead_updates for s in st], dtype=np.float64)
     energy = b.energy()
-    return b, dict(ms=ms, wall=wall, bu=bu, abu=abu, energy=energy)
+    team, inv, lds = b.schedule()
+    tab, fst = b.staging()
+    sched = {"team": team, "inverse_lds": inv, "tables_lds": tab, "node_state_lds": fst,
+             "lds_bytes": lds, "autotune_ms": tuned}
+    return b, dict(ms=ms, wall=wall, bu=bu, abu=abu, energy=energy, schedule=sched, warmup_done=done)
 
 
 def all_paths(b):
@@ -230,7 +242,8 @@ def main():
         "dtype": "fp32+fp64" if prec == capi.PQ_PREC_MIXED else "fp64",
         "data": "synthetic (seeded Gaussian beads, replicated across slices for node chains)",
         "config": {"workload": f"{args.workload}: {wdesc}", "chains": len(descs),
-                   "precision": args.precision, "step": "one sweep of every chain (one kernel launch)"},
+                   "precision": args.precision, "step": "one sweep of every chain (one kernel launch)",
+                   "schedule": main_run["schedule"]},
         "accepted_bead_updates_per_s": float(main_run["abu"].sum()) / main_run["wall"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -240,12 +253,15 @@ def main():
                      "valu_frac_of_fp64_peak": flops_launch / (kernel_ms * 1e-3) / 1e12 / FP64_VECTOR_TFLOPS},
     }
     if not args.no_fp64_pair and prec == capi.PQ_PREC_MIXED:
-        b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed)
+        # the same number of sweeps as the mixed run (automatic schedule; the
+        # mixed run's autotuning sweeps are matched by plain warmup sweeps)
+        b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed,
+                           autotune=False, total_warmup=main_run["warmup_done"])
         out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
                                "kernel_ms": float(np.mean(ref["ms"]))}
         out["energy_vs_fp64"] = paired_energy(b_main, b64, main_run["energy"], ref["energy"])
         single = [capi.cfg2_desc()]
-        _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed)
+        _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed, autotune=False)
         r1 = float(one["bu"].sum()) / one["wall"]
         out["fp64_single_chain"] = {"workload": "cfg2: 3+3 e- 2D dot, SHONodes, M=400", "value": r1}
         out["ratio_vs_fp64_single_chain"] = value / r1

@@ -186,6 +186,21 @@ int pq_sync(pq_batch* batch);
    global memory (0) or automatic (-1).  Results do not depend on it. */
 int pq_set_schedule(pq_batch* batch, int team, int inv_lds);
 int pq_get_schedule(const pq_batch* batch, int* team, int* inv_lds, int* lds_bytes);
+/* Placement of the sampler's pair tables (u, the arithmetic type's copy) and
+   fixed-node state: LDS (1), global memory (0) or automatic (-1).  Automatic
+   placement fills a 40 KB LDS budget per chain (four chains per CU) with the
+   tables first, then the node state, then the inverse cache. */
+int pq_set_staging(pq_batch* batch, int tables_lds, int state_lds);
+int pq_get_staging(const pq_batch* batch, int* tables_lds, int* state_lds);
+/* Times every distinct schedule (team 64/128 x tables, node state and
+   inverse cache in LDS or global memory) over nsweep sampler sweeps of the
+   batch and keeps the fastest; best_ms receives its time per sweep and
+   ntimed the number of schedules timed (ntimed * nsweep sweeps were run).
+   The tuning sweeps advance the chains like pq_run_sweeps: schedules change
+   only where data lives and how many threads share a chain, so results agree
+   to rounding.  Replaces nothing in the reference (it has no device
+   schedule). */
+int pq_autotune(pq_batch* batch, int nsweep, float* best_ms, int* ntimed);
 int pq_get_stats(pq_batch* batch, int chain, pq_chain_stats* out);
 int pq_get_all_stats(pq_batch* batch, pq_chain_stats* out);
 int pq_reset_stats(pq_batch* batch);

@@ -157,6 +157,11 @@ int pg_nterms(double alpha, double length) { return 1 + (int)(2.0 * std::sqrt(al
 
 // ------------------------------------------------------------- the batch
 
+// requested sampler schedule: -1 automatic, else team size or 0/1 placement
+struct Sched {
+  int team = -1, inv = -1, tab = -1, fst = -1;
+};
+
 struct pq_batch {
   int device = 0;
   int precision = 0;
@@ -171,6 +176,7 @@ struct pq_batch {
   std::vector<int> order;
   int64_t nbeads = 0;
   SweepLayoutH lay{};
+  Sched sched{};   // requested schedule, all automatic
   // device
   ChainP* dchains = nullptr;
   TableP* dtables = nullptr;
@@ -403,17 +409,28 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
 static int64_t align8(int64_t v) { return (v + 7) & ~(int64_t)7; }
 
 // LDS carve-up of the sampler workgroup, sized by the batch's largest chain.
-// team: threads per chain (64 or 128, -1 automatic); inv_lds: inverse cache in
-// LDS (1), in global memory (0) or automatic (-1).
-static void make_layout(pq_batch* b, int team_req, int inv_req) {
-  int64_t mS = 0, mSND = 0, mSD = 0, mF = 0, mInv = 0;
+// Each request is -1 (automatic), 0 or 1: team = threads per chain (64 or
+// 128), tab = pair tables staged in LDS, fst = fixed-node state in LDS,
+// inv = node inverse cache in LDS.  Automatic placement fills a 40 KB budget
+// (four chains per CU) in the order tables, fixed-node state, inverses;
+// requested placements may use the whole 160 KB.
+static void make_layout(pq_batch* b, const Sched& req) {
+  int64_t mS = 0, mSND = 0, mSD = 0, mF = 0, mInv = 0, mTab = 0;
   bool nodes = false;
+  const int64_t w = b->precision == PQ_PREC_MIXED ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
   for (auto& ch : b->hc) {
     mS = std::max<int64_t>(mS, ch.S);
     mSND = std::max<int64_t>(mSND, (int64_t)ch.S * ch.N * ch.D);
     mSD = std::max<int64_t>(mSD, (int64_t)ch.S * ch.D);
     mF = std::max<int64_t>(mF, fn_state_doubles(ch));
     if (ch.nfn) { nodes = true; mInv = std::max<int64_t>(mInv, (int64_t)ch.S * ch.ns_max * ch.ns_max); }
+    int64_t tabs = 0;
+    for (int pp = 0; pp < ch.npair; ++pp) {
+      int q = 0;
+      while (q < pp && ch.pair[q].table != ch.pair[pp].table) ++q;
+      if (q == pp) tabs += (int64_t)b->ht[ch.pair[pp].table].ngpts * b->ht[ch.pair[pp].table].K1;
+    }
+    mTab = std::max<int64_t>(mTab, tabs);
   }
   int64_t o = align8(sizeof(ChainP));
   b->lay.off_sec1 = (int)o; o += mSND * 8;
@@ -426,31 +443,25 @@ static void make_layout(pq_batch* b, int team_req, int inv_req) {
   b->lay.off_cnt = (int)o; o += 24 * 8;
   b->lay.off_werr = (int)o; o += PQ_WAVE * 4;
   b->lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
+  b->lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
   o = align8(o);
   b->lay.off_gj = (int)o; o += 2 * 8 * 128;
-  // fixed-node state: LDS while the workgroup stays within the 40 KB that
-  // keeps four chains per CU, else global memory
-  if (o + mF * 8 <= 40 * 1024) {
-    b->lay.off_fst = (int)o;
-    o += mF * 8;
-  } else {
-    b->lay.off_fst = -1;
-  }
-  // automatic: keep the inverse cache in LDS while four chains still fit on one CU
-  bool inv_lds = nodes && o + mInv * 8 <= 40 * 1024;
-  if (inv_req == 0) inv_lds = false;
-  if (inv_req == 1) inv_lds = nodes && o + mInv * 8 <= 160 * 1024;
-  if (inv_lds) {
-    b->lay.off_inv = (int)o;
-    o += mInv * 8;
-  } else {
-    b->lay.off_inv = -1;
-  }
+  const int64_t budget = 40 * 1024, whole = 160 * 1024;
+  auto place = [&](int r, int64_t bytes, bool wanted) -> int {
+    if (!wanted || bytes <= 0 || r == 0) return -1;
+    if (o + bytes > (r == 1 ? whole : budget)) return -1;
+    const int at = (int)o;
+    o = align8(o + bytes);
+    return at;
+  };
+  b->lay.off_tab = place(req.tab, mTab * w, mTab > 0);
+  b->lay.off_fst = place(req.fst, mF * 8, nodes);
+  b->lay.off_inv = place(req.inv, mInv * 8, nodes);
   b->lay.bytes = (int)align8(o);
   // automatic team: two waves per chain when node matrices dominate, else one;
   // either way two waves fit per SIMD, so four chains share a CU
   int team = nodes ? 128 : PQ_WAVE;
-  if (team_req == 64 || team_req == 128) team = team_req;
+  if (req.team == 64 || req.team == 128) team = req.team;
   b->lay.team = team;
 }
 
@@ -494,7 +505,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     ch.fnc_off = nrec ? fnc : -1;
     fnc += nrec;
   }
-  make_layout(b, -1, -1);
+  make_layout(b, b->sched);
   if (b->lay.bytes > 160 * 1024) {
     int n = b->lay.bytes;
     delete b;
@@ -651,15 +662,30 @@ extern "C" int pq_set_uniforms(pq_batch* b, const double* u, int64_t per_chain) 
   return PQ_OK;
 }
 
+static int apply_schedule(pq_batch* b, const Sched& s) {
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const Sched keep = b->sched;
+  const SweepLayoutH keep_lay = b->lay;
+  b->sched = s;
+  make_layout(b, s);
+  if (b->lay.bytes > 160 * 1024) {
+    const int n = b->lay.bytes;
+    b->sched = keep;
+    b->lay = keep_lay;
+    return fail(PQ_ERR_ARG, "schedule needs %d bytes of LDS", n);
+  }
+  return PQ_OK;
+}
+
 extern "C" int pq_set_schedule(pq_batch* b, int team, int inv_lds) {
   if (!b) return fail(PQ_ERR_ARG, "null batch");
   if (team != -1 && team != 64 && team != 128) return fail(PQ_ERR_ARG, "team must be 64, 128 or -1");
   if (inv_lds < -1 || inv_lds > 1) return fail(PQ_ERR_ARG, "inv_lds must be -1, 0 or 1");
-  HIPCHK(hipSetDevice(b->device));
-  HIPCHK(hipStreamSynchronize(b->stream));
-  make_layout(b, team, inv_lds);
-  if (b->lay.bytes > 160 * 1024) return fail(PQ_ERR_ARG, "schedule needs %d bytes of LDS", b->lay.bytes);
-  return PQ_OK;
+  Sched s = b->sched;
+  s.team = team;
+  s.inv = inv_lds;
+  return apply_schedule(b, s);
 }
 
 extern "C" int pq_get_schedule(const pq_batch* b, int* team, int* inv_lds, int* lds_bytes) {
@@ -667,6 +693,61 @@ extern "C" int pq_get_schedule(const pq_batch* b, int* team, int* inv_lds, int* 
   if (team) *team = b->lay.team;
   if (inv_lds) *inv_lds = b->lay.off_inv >= 0;
   if (lds_bytes) *lds_bytes = b->lay.bytes;
+  return PQ_OK;
+}
+
+extern "C" int pq_set_staging(pq_batch* b, int tables_lds, int state_lds) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (tables_lds < -1 || tables_lds > 1 || state_lds < -1 || state_lds > 1)
+    return fail(PQ_ERR_ARG, "staging requests must be -1, 0 or 1");
+  Sched s = b->sched;
+  s.tab = tables_lds;
+  s.fst = state_lds;
+  return apply_schedule(b, s);
+}
+
+extern "C" int pq_get_staging(const pq_batch* b, int* tables_lds, int* state_lds) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (tables_lds) *tables_lds = b->lay.off_tab >= 0;
+  if (state_lds) *state_lds = b->lay.off_fst >= 0;
+  return PQ_OK;
+}
+
+extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed) {
+  if (!b || nsweep < 1) return fail(PQ_ERR_ARG, "pq_autotune: bad arguments");
+  HIPCHK(hipSetDevice(b->device));
+  // candidate schedules; layouts that come out identical are timed once
+  std::vector<Sched> cands;
+  for (int team : {64, 128})
+    for (int tab : {0, 1})
+      for (int fst : {0, 1})
+        for (int inv : {0, 1}) cands.push_back(Sched{team, inv, tab, fst});
+  std::vector<SweepLayoutH> seen;
+  float best = -1;
+  Sched best_s = b->sched;
+  for (const Sched& s : cands) {
+    b->sched = s;
+    make_layout(b, s);
+    if (b->lay.bytes > 160 * 1024) continue;
+    bool dup = false;
+    for (const SweepLayoutH& l : seen)
+      if (std::memcmp(&l, &b->lay, sizeof(SweepLayoutH)) == 0) dup = true;
+    if (dup) continue;
+    seen.push_back(b->lay);
+    float ms = 0;
+    DevCtx x = make_ctx(b);
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    HIPCHK(launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    ms /= nsweep;
+    if (best < 0 || ms < best) { best = ms; best_s = s; }
+  }
+  b->sched = best_s;
+  make_layout(b, best_s);
+  if (best_ms) *best_ms = best;
+  if (ntimed) *ntimed = (int)seen.size();
   return PQ_OK;
 }
 

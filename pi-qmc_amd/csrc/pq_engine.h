@@ -1,7 +1,7 @@
-// pq_engine.h -- a team of NT threads (1 or 4 wavefronts) evaluates one
+// pq_engine.h -- a team of NT threads (one or two wavefronts) evaluates one
 // chain's bisection move.
 //
-// Eng<D,G,R,NT> bundles the views of one chain (section beads, moving beads,
+// Eng<D,NC,R,NT> bundles the views of one chain (section beads, moving beads,
 // fixed-node state, inverse cache) with the routines of one level of
 // MultiLevelSampler/DoubleMLSampler::tryMove:
 //   mover()            FreeMover / FreeMoverPBC::makeMove
@@ -70,10 +70,9 @@ __device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
 // group through LDS (row: NC + 1 doubles per group).  Step k replaces column
 // k by the inverse's column as it goes (pivot row: a[j]/piv, a[k] = 1/piv;
 // other rows: a[j] - f a_p[j], a[k] = -f, f = a[k]/piv).  On exit the lane
-// with mypos = k < n holds row k
-// of A^-1 with its columns in pivot order: a[j] is column P(j), P(j) the
-// row that was pivot at step j.  det = prod(pivots) * (-1)^swaps, the LU
-// determinant with the same pivots.
+// with mypos = k < n holds row k of A^-1 with its columns in pivot order:
+// a[j] is column P(j), P(j) the row that was pivot at step j.  det =
+// prod(pivots) * (-1)^swaps, the LU determinant with the same pivots.
 template <int G, int NC>
 __device__ __forceinline__ void gj_invert(double (&a)[NC], int r, int n, double* row, double& det,
                                           int& err, int& mypos) {
@@ -143,6 +142,9 @@ struct Eng {
   int* werr;        // [NT/G] scratch: error flags of a pass
   double* wscale;   // [S] scale used per slice in initialize
   double* gjrow;    // [NT/G][2 NC] pivot rows of the Gauss-Jordan groups
+  // sampler only: pair tables staged in LDS (tabofs[pair] offsets into tab)
+  const R* tab = nullptr;
+  const int* tabofs = nullptr;
   // sampler only: node records of every slice pair, reused across sections
   double* fcache = nullptr;
   int sec_t0 = 0;          // first slice of section 1
@@ -435,7 +437,7 @@ struct Eng {
       else if (sp == P.s2) js = P.s1;
       else continue;
       const TableP T = x.tables[P.table];
-      const R* u = table_ptr<R>(x, T);
+      const R* u = tab ? tab + tabofs[pp] : table_ptr<R>(x, T);
       const bool offdiag = level < 3 && T.K1 > 1;
       const int jb = c.ifirst[js], nj = c.count[js];
       const int items = nj * nl;

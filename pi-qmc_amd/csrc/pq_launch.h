@@ -9,6 +9,8 @@ struct SweepLayoutH {
   int off_sec1, off_sec2, off_mov, off_fst, off_wdet, off_werr, off_wscale, off_scale, off_notmine;
   int off_red, off_cnt, off_inv;   // off_inv < 0: inverse cache in global memory
   int off_gj;                      // Gauss-Jordan pivot rows, 2 doubles per thread
+  int off_tab;                     // pair tables (u) staged in LDS, < 0: read from global memory
+  int off_tabofs;                  // [PQ_MAX_PAIRS] LDS offset of each pair's table
   int bytes;
   int team;                        // threads per chain: 64 or 256
 };

@@ -12,6 +12,7 @@ struct SweepLayout {
   int off_sec1, off_sec2, off_mov, off_fst, off_wdet, off_werr, off_wscale, off_scale, off_notmine;
   int off_red, off_cnt, off_inv;   // off_inv < 0: inverse cache in global memory
   int off_gj;
+  int off_tab, off_tabofs;         // off_tab < 0: pair tables read from global memory
   int bytes;
 };
 
@@ -44,6 +45,33 @@ __global__ __launch_bounds__(NT, 2) void sweep_kernel(DevCtx x, int nsweep, Swee
   const int tid = e.tid;
   if (tid < 19) cnt[tid] = 0;
   if (tid < c.nfn) e.scale[tid] = x.scale[(int64_t)chain * PQ_MAX_NODES + tid];
+  if (lay.off_tab >= 0) {
+    // this chain's distinct pair tables (u only) staged in LDS once
+    int* tabofs = reinterpret_cast<int*>(base + lay.off_tabofs);
+    R* tab = reinterpret_cast<R*>(base + lay.off_tab);
+    if (tid == 0) {
+      int used = 0;
+      for (int pp = 0; pp < c.npair; ++pp) {
+        int q = 0;
+        while (q < pp && c.pair[q].table != c.pair[pp].table) ++q;
+        if (q < pp) { tabofs[pp] = tabofs[q]; continue; }
+        tabofs[pp] = used;
+        const TableP& T = x.tables[c.pair[pp].table];
+        used += T.ngpts * T.K1;
+      }
+    }
+    __syncthreads();
+    for (int pp = 0; pp < c.npair; ++pp) {
+      int q = 0;
+      while (q < pp && c.pair[q].table != c.pair[pp].table) ++q;
+      if (q < pp) continue;
+      const TableP& T = x.tables[c.pair[pp].table];
+      const R* src = table_ptr<R>(x, T);
+      for (int k = tid; k < T.ngpts * T.K1; k += NT) tab[tabofs[pp] + k] = src[k];
+    }
+    e.tab = tab;
+    e.tabofs = tabofs;
+  }
   __syncthreads();
 
   const int S = c.S, N = c.N, M = c.M, L = c.L;
@@ -160,6 +188,8 @@ static hipError_t launch_sweep_nt(const DevCtx& x, int nchain, int nsweep, const
   lay.off_wscale = L.off_wscale; lay.off_scale = L.off_scale; lay.off_notmine = L.off_notmine;
   lay.off_red = L.off_red; lay.off_cnt = L.off_cnt; lay.off_inv = L.off_inv;
   lay.off_gj = L.off_gj;
+  lay.off_tab = L.off_tab;
+  lay.off_tabofs = L.off_tabofs;
   lay.bytes = L.bytes;
   auto k = sweep_kernel<D, NC, R, NT>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes);

@@ -235,6 +235,9 @@ def load_library(path=LIB_PATH):
         "pq_sync": (C.c_int, [C.c_void_p]),
         "pq_set_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "pq_get_schedule": (C.c_int, [C.c_void_p, ip, ip, ip]),
+        "pq_set_staging": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "pq_get_staging": (C.c_int, [C.c_void_p, ip, ip]),
+        "pq_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float), ip]),
         "pq_run_sweeps_timed": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float)]),
         "pq_get_all_stats": (C.c_int, [C.c_void_p, P(ChainStats)]),
         "pq_get_pair_table": (C.c_int, [C.c_void_p, C.c_int, C.c_int, ip, dp, dp, dp]),
@@ -330,6 +333,22 @@ class Batch:
         t, i, b = C.c_int(), C.c_int(), C.c_int()
         check(self.lib.pq_get_schedule(self.h, C.byref(t), C.byref(i), C.byref(b)))
         return t.value, i.value, b.value
+
+    def set_staging(self, tables_lds=-1, state_lds=-1):
+        check(self.lib.pq_set_staging(self.h, tables_lds, state_lds))
+
+    def staging(self):
+        t, f = C.c_int(), C.c_int()
+        check(self.lib.pq_get_staging(self.h, C.byref(t), C.byref(f)))
+        return t.value, f.value
+
+    def autotune(self, nsweep=1):
+        """Time every distinct schedule over nsweep sweeps and keep the fastest;
+        returns (ms per sweep of the fastest, sweeps run).  The sweeps advance
+        the chains."""
+        ms, n = C.c_float(), C.c_int()
+        check(self.lib.pq_autotune(self.h, nsweep, C.byref(ms), C.byref(n)))
+        return ms.value, n.value * nsweep
 
     def stats(self, c):
         st = ChainStats()

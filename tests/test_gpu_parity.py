@@ -193,9 +193,11 @@ def test_fixed_node_protocol():
     assert nbig >= 1
 
 
-def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False, team=-1, inv=-1):
+def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False, team=-1, inv=-1,
+            staging=(-1, -1)):
     b = capi.Batch([desc], precision=precision)
     b.set_schedule(team, inv)
+    b.set_staging(*staging)
     oc = ol.Chain(desc)
     beads = capi.initial_beads(desc, seed)
     b.set_paths(0, beads)
@@ -231,6 +233,48 @@ def test_sampler_decisions(name, inject, team, inv):
     assert sg.node_errors == so.node_errors
     pg, po = b.get_paths(0), oc.get_beads()
     assert np.max(np.abs(pg - po)) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["cfg1", "cfg2"])
+@pytest.mark.parametrize("staging", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_staging_keeps_decisions(name, staging):
+    """Pair tables and node state in LDS or global memory: same decisions."""
+    print(f"[test] staging {name} {staging}", flush=True)
+    desc = _descs()[name]
+    b, oc = _paired(desc, 4, inject=True, staging=staging)
+    got = b.staging()
+    assert got[0] == (staging[0] == 1) or desc.coulomb.enabled == 0
+    sg, so = b.stats(0), oc.stats()
+    assert sg.draws == so.draws
+    assert list(sg.accepts) == list(so.accepts)
+    assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9
+
+
+def test_autotune_keeps_decisions():
+    """The autotuner's timing sweeps are ordinary sweeps: after it, the chain
+    equals the oracle's after the same number of sweeps."""
+    print("[test] autotune", flush=True)
+    desc = small_cfg2(64)
+    b = capi.Batch([desc])
+    oc = ol.Chain(desc)
+    beads = capi.initial_beads(desc, 8)
+    b.set_paths(0, beads)
+    oc.set_beads(beads)
+    b.seed(4242, 0)
+    oc.rng_philox(4242, 0)
+    ms, nsw = b.autotune(1)
+    assert ms > 0 and nsw >= 2
+    b.run_sweeps(2)
+    b.sync()
+    draws = b.stats(0).draws
+    for _ in range(64):
+        if oc.stats().draws >= draws:
+            break
+        oc.run_sweeps(1)
+    sg, so = b.stats(0), oc.stats()
+    assert sg.draws == so.draws
+    assert list(sg.accepts) == list(so.accepts)
+    assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9
 
 
 def test_energy_estimator():
