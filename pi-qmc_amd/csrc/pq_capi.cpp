@@ -445,7 +445,10 @@ static void make_layout(pq_batch* b, const Sched& req) {
   b->lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
   b->lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
   o = align8(o);
-  b->lay.off_gj = (int)o; o += 2 * 8 * 128;
+  // Gauss-Jordan scratch: pivot rows (2 doubles per thread), or for 32-wide
+  // node matrices the per-wave MFMA scratch of the two waves of a team
+  b->lay.off_gj = (int)o;
+  o += b->NC == 32 ? 2 * PQ_GJ32_SCRATCH_DOUBLES * 8 : 2 * 8 * 128;
   const int64_t budget = 40 * 1024, whole = 160 * 1024;
   auto place = [&](int r, int64_t bytes, bool wanted) -> int {
     if (!wanted || bytes <= 0 || r == 0) return -1;

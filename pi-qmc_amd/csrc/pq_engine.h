@@ -117,6 +117,174 @@ __device__ __forceinline__ void gj_invert(double (&a)[NC], int r, int n, double*
   }
 }
 
+// ----------------------------------------------------------------------------
+// Node matrices of up to 32 particles: one wave per matrix, blocked in-place
+// Gauss-Jordan with FP64 MFMA trailing updates.
+//
+// The 32 x 32 matrix diag(A, I) lives in the accumulator layout of
+// v_mfma_f64_16x16x4f64, four 16 x 16 tiles: lane l, tile (rt, ct), register r
+// holds element (16 rt + 4 r + l / 16, 16 ct + l % 16).  The elimination runs
+// in panels of four columns.  A panel is eliminated row-per-lane (lanes 0..31
+// are rows) with the pivot rules of gj_invert; its four steps act on any other
+// column v as v <- v - W s, W the 32 x 4 multipliers (w_j[p_j] = 1 - 1/piv_j on
+// the pivot row, a_rk/piv_j elsewhere) and s the pivot rows' values run
+// through the steps before them (s_j = v_pj - sum_{i<j} w_i[p_j] s_i).  That
+// rank-4 update of the 28 other columns is four MFMAs; the panel's own
+// columns take the row-per-lane result.  Operand layouts: A (16 x 4) lane l
+// holds (l % 16, l / 16), B (4 x 16) lane l holds (l / 16, l % 16).
+struct GjMfmaScratch {
+  double P[32 * 4];    // panel [row][4]
+  double W[32 * 4];    // multipliers [row][4]
+  double RP[4 * 32];   // pivot rows before the panel [4][column]
+  double S[4 * 32];    // s [4][column]
+  double prow[8];      // pivot row of the current step and its reciprocal
+  int pv[4];           // physical pivot rows of the panel's steps
+  int pos[32];         // position of each row on exit
+  int perm[32];        // P(j): the row that was pivot at step j
+};
+static_assert(sizeof(GjMfmaScratch) <= PQ_GJ32_SCRATCH_DOUBLES * sizeof(double), "32-wide scratch");
+
+typedef double pq_d4 __attribute__((ext_vector_type(4)));
+
+// element (16 rt + 4 r + lane/16, column tile ct) of the accumulator tiles for a
+// wave-uniform rt, r
+__device__ __forceinline__ double acc_row(const pq_d4 (&acc)[2][2], int rt, int ct, int r) {
+  switch (rt * 4 + r) {
+    case 0: return acc[0][ct][0];
+    case 1: return acc[0][ct][1];
+    case 2: return acc[0][ct][2];
+    case 3: return acc[0][ct][3];
+    case 4: return acc[1][ct][0];
+    case 5: return acc[1][ct][1];
+    case 6: return acc[1][ct][2];
+    default: return acc[1][ct][3];
+  }
+}
+
+// In-place inversion of the n x n matrix held in acc (rows and columns n..31
+// identity) by one whole wave.  On exit acc holds the inverse with columns in
+// pivot order (as gj_invert), sc->pos[i] the position of row i and
+// sc->perm[j] the row that was pivot at step j; det and err as gj_invert.
+__device__ __forceinline__ void gj_invert_mfma32(pq_d4 (&acc)[2][2], int n, GjMfmaScratch* sc,
+                                                 double& det, int& err) {
+  const int lane = __lane_id();
+  const int row = lane & 31;
+  const bool act = lane < 32;
+  int mypos = row;
+  det = 1.0;
+  err = 0;
+#pragma unroll
+  for (int kb = 0; kb < 8; ++kb) {
+    if (4 * kb >= n) break;
+    const int c0 = 4 * kb;
+    const int ct = kb >> 2;          // column tile of the panel
+    const int cc0 = c0 & 15;
+    const int jj0 = (lane & 15) - cc0;
+    const bool holds = jj0 >= 0 && jj0 < 4;
+    // (1) panel columns to LDS, [row][4]
+    if (holds) {
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sc->P[(16 * rt + 4 * r + (lane >> 4)) * 4 + jj0] = acc[rt][ct][r];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // (2) the panel's four steps, row per lane
+    double p[4], w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p[j] = act ? sc->P[row * 4 + j] : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = c0 + j;
+      const uint64_t key = (act && mypos >= k)
+          ? ((__double_as_longlong(fabs(p[j])) & ~0x3Full) | (uint64_t)(63 - mypos)) : 0ull;
+      const double myinv = 1.0 / p[j];
+      const uint64_t kmax = group_max_u64<32>(key);
+      const int ppos = 63 - (int)(kmax & 0x3F);
+      const bool isp = act && key == kmax;
+      if (ppos != k) det = -det;
+      if (isp) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sc->prow[q] = p[q];
+        sc->prow[4] = myinv;
+        sc->pv[j] = row;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const double piv = sc->prow[j];
+      det *= piv;
+      const bool zero = piv == 0.0;
+      if (zero) err = 1;
+      const double ipiv = zero ? 0.0 : sc->prow[4];
+      const double f = (act && !isp) ? p[j] * ipiv : 0.0;
+      const double sp = (isp && !zero) ? ipiv : 1.0;
+      w[j] = (zero || !act) ? 0.0 : (isp ? 1.0 - ipiv : f);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q != j) p[q] = fma(-f, sc->prow[q], p[q]) * sp;
+      if (!zero && act) p[j] = isp ? ipiv : -f;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (isp) mypos = k;
+      else if (mypos == k) mypos = ppos;
+    }
+    // (3) final panel and multipliers to LDS
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { sc->P[row * 4 + q] = p[q]; sc->W[row * 4 + q] = w[q]; }
+    }
+    // (4) the pivot rows as they were before the panel, [4][column]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int pr = sc->pv[j];
+      if ((lane >> 4) == (pr & 3)) {
+#pragma unroll
+        for (int t2 = 0; t2 < 2; ++t2)
+          sc->RP[j * 32 + 16 * t2 + (lane & 15)] = acc_row(acc, pr >> 4, t2, (pr & 15) >> 2);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // (5) s for every column outside the panel (lanes 0..31 are columns)
+    if (act) {
+      const int col = lane;
+      const bool inpanel = col >= c0 && col < c0 + 4;
+      const int p1 = sc->pv[1], p2 = sc->pv[2], p3 = sc->pv[3];
+      const double s0 = sc->RP[col];
+      const double s1 = sc->RP[32 + col] - sc->W[p1 * 4 + 0] * s0;
+      const double s2 = sc->RP[64 + col] - sc->W[p2 * 4 + 0] * s0 - sc->W[p2 * 4 + 1] * s1;
+      const double s3 = sc->RP[96 + col] - sc->W[p3 * 4 + 0] * s0 - sc->W[p3 * 4 + 1] * s1 -
+                        sc->W[p3 * 4 + 2] * s2;
+      sc->S[col] = inpanel ? 0.0 : s0;
+      sc->S[32 + col] = inpanel ? 0.0 : s1;
+      sc->S[64 + col] = inpanel ? 0.0 : s2;
+      sc->S[96 + col] = inpanel ? 0.0 : s3;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // (6) rank-4 update of every tile: acc -= W s
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      const double a = -sc->W[(16 * rt + (lane & 15)) * 4 + (lane >> 4)];
+#pragma unroll
+      for (int t2 = 0; t2 < 2; ++t2) {
+        const double bb = sc->S[(lane >> 4) * 32 + 16 * t2 + (lane & 15)];
+        acc[rt][t2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[rt][t2], 0, 0, 0);
+      }
+    }
+    // (7) the panel's columns take the row-per-lane result
+    if (holds) {
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[rt][ct][r] = sc->P[(16 * rt + 4 * r + (lane >> 4)) * 4 + jj0];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (act) {
+    sc->pos[row] = mypos;
+    sc->perm[mypos] = row;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 constexpr int gj_group(int nc) { return nc <= 4 ? 4 : (nc <= 8 ? 8 : (nc <= 16 ? 16 : 32)); }
 
 template <int D, int NC, class R, int NT>
@@ -518,6 +686,45 @@ struct Eng {
       for (int d = 0; d < D; ++d) egr[e * D + d] = gf[d];
     }
     __syncthreads();
+    if constexpr (NC == 32) {
+      // one wave per matrix, MFMA trailing updates
+      constexpr int NW = NT / PQ_WAVE;
+      GjMfmaScratch* sc = reinterpret_cast<GjMfmaScratch*>(gjrow + (int64_t)wave * PQ_GJ32_SCRATCH_DOUBLES);
+      for (int base = 0; base < nt; base += NW) {
+        const int m = base + wave;
+        if (m < nt) {
+          const int t = t0 + m * step;
+          double* slot = inv + (int64_t)t * ins * ins;
+          pq_d4 acc[2][2];
+#pragma unroll
+          for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+              for (int r4 = 0; r4 < 4; ++r4) {
+                const int i = 16 * rt + 4 * r4 + (lane >> 4), j = 16 * ct + (lane & 15);
+                acc[rt][ct][r4] = (i < n && j < n) ? slot[i * ins + j] : (i == j ? 1.0 : 0.0);
+              }
+          double det;
+          int err;
+          gj_invert_mfma32(acc, n, sc, det, err);
+#pragma unroll
+          for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+              for (int r4 = 0; r4 < 4; ++r4) {
+                const int i = 16 * rt + 4 * r4 + (lane >> 4), j = 16 * ct + (lane & 15);
+                if (i < n && j < n) slot[sc->pos[i] * ins + sc->perm[j]] = acc[rt][ct][r4];
+              }
+          if (lane == 0) { wdet[m] = det; werr[m] = err; }
+          // the next matrix of this wave reuses the scratch
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+      }
+      __syncthreads();
+      return;
+    }
     const int g = tid / G, r = tid % G;
     constexpr int NG = NT / G;
     constexpr int GPW = PQ_WAVE / G;   // groups per wave

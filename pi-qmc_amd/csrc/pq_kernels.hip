@@ -61,7 +61,7 @@ template <int D, int NC, class R>
 __global__ __launch_bounds__(PQ_WAVE) void fn_kernel(DevCtx x, int chain, FnBufs b) {
   __shared__ ChainP c;
   __shared__ double sscale[PQ_MAX_NODES];
-  __shared__ double gj[2 * PQ_WAVE];
+  __shared__ double gj[NC == 32 ? PQ_GJ32_SCRATCH_DOUBLES : 2 * PQ_WAVE];
   copy_chain(&c, &x.chains[chain]);
   Eng<D, NC, R, PQ_WAVE> e(x, c, chain);
   e.gjrow = gj;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(PQ_WAVE) void node_eval_kernel(DevCtx x, int chain,
   __shared__ ChainP c;
   __shared__ double wd[PQ_WAVE];
   __shared__ int we[PQ_WAVE];
-  __shared__ double gj[2 * PQ_WAVE];
+  __shared__ double gj[NC == 32 ? PQ_GJ32_SCRATCH_DOUBLES : 2 * PQ_WAVE];
   copy_chain(&c, &x.chains[chain]);
   Eng<D, NC, R, PQ_WAVE> e(x, c, chain);
   e.gjrow = gj;

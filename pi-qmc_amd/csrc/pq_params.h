@@ -8,6 +8,7 @@
 #define PQ_WAVE 64
 #define PQ_MAX_PAIRS 10
 #define PQ_PG_LEVELS 8
+#define PQ_GJ32_SCRATCH_DOUBLES 560   // LDS scratch per wave of the 32-wide node-matrix inversion
 
 struct TableP {          // one tabulated pair action u_k(r), utau_k(r)
   int ngpts;
