@@ -167,6 +167,23 @@ int pq_get_paths(pq_batch* batch, int chain, double* beads);
 int pq_set_all_paths(pq_batch* batch, const double* beads, int64_t count);
 int pq_get_all_paths(pq_batch* batch, double* beads, int64_t count);
 
+/* Path files of <WritePaths file> / <ReadPaths file bfactor>
+   (src/algorithm/WritePaths.cc:36-112, src/algorithm/PathReader.cc:17-132):
+   a "#Permutations" line, a "#Path coordinates: N particles in D-d" line,
+   then one line per slice with the N*D coordinates at 15 significant digits.
+   Reading accepts the reference's files with or without the permutation
+   line (without it, a trailing wrap slice), fills slice s + k nslice/bfactor
+   from file slice s, and refuses non-identity permutations.  beads are
+   [nslice][npart][ndim].  These two touch no device. */
+int pq_write_paths_file(const char* filename, const double* beads, int nslice,
+                        int npart, int ndim);
+int pq_read_paths_file(const char* filename, double* beads, int nslice, int npart,
+                       int ndim, int bfactor);
+/* Every chain of a batch; chain c uses filename + c when the batch has more
+   than one chain, as the reference appends the clone ID when nclone > 1. */
+int pq_write_all_paths(pq_batch* batch, const char* filename);
+int pq_read_all_paths(pq_batch* batch, const char* filename, int bfactor);
+
 /* Uniform stream of every chain: Philox4x32-10 keyed by seed, stream id =
    first_stream + chain, or caller-supplied uniforms (per_chain values per
    chain, chain-major).  Replaces RandomNumGenerator (src/util/RandomNumGenerator.h:26-86). */

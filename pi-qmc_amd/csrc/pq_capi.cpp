@@ -640,6 +640,114 @@ extern "C" int pq_get_all_paths(pq_batch* b, double* beads, int64_t count) {
   return PQ_OK;
 }
 
+// ------------------------------------------------------------- path files
+
+// WritePaths::run (src/algorithm/WritePaths.cc:36-112) for one clone on one
+// worker: "#Permutations " and the permutation, each entry followed by a
+// space; "#Path coordinates: N particles in D-d"; then every slice on one
+// line, each coordinate followed by a space, at the stream's precision 15
+// (printf's %.15g).  The sampler keeps the identity permutation.
+extern "C" int pq_write_paths_file(const char* filename, const double* beads, int nslice, int npart,
+                                   int ndim) {
+  if (!filename || !beads || nslice < 1 || npart < 1 || ndim < 1) return fail(PQ_ERR_ARG, "pq_write_paths_file: bad arguments");
+  FILE* f = std::fopen(filename, "w");
+  if (!f) return fail(PQ_ERR_ARG, "cannot open %s for writing", filename);
+  std::fprintf(f, "#Permutations ");
+  for (int i = 0; i < npart; ++i) std::fprintf(f, "%d ", i);
+  std::fprintf(f, "\n#Path coordinates: %d particles in %d-d\n", npart, ndim);
+  for (int t = 0; t < nslice; ++t) {
+    for (int i = 0; i < npart; ++i)
+      for (int d = 0; d < ndim; ++d) std::fprintf(f, "%.15g ", beads[((size_t)t * npart + i) * ndim + d]);
+    std::fprintf(f, "\n");
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? PQ_OK : fail(PQ_ERR_ARG, "writing %s failed", filename);
+}
+
+// PathReader::run (src/algorithm/PathReader.cc:17-132) for one clone on one
+// worker: an optional "#Permutations" line (npart integers), the header line,
+// then nslice/bfactor slices of npart*ndim numbers; slice s of the file fills
+// slices s + k nslice/bfactor, k < bfactor.  A file without the permutation
+// line carries one more slice, whose match against the first slice
+// (|difference| summed over dimensions < 1e-5) gives the permutation.  The
+// sampler here works on the identity permutation only; any other is refused.
+extern "C" int pq_read_paths_file(const char* filename, double* beads, int nslice, int npart, int ndim,
+                                  int bfactor) {
+  if (!filename || !beads || nslice < 1 || npart < 1 || ndim < 1) return fail(PQ_ERR_ARG, "pq_read_paths_file: bad arguments");
+  if (bfactor <= 0) bfactor = 1;
+  if (nslice % bfactor) return fail(PQ_ERR_ARG, "bfactor %d does not divide %d slices", bfactor, nslice);
+  FILE* f = std::fopen(filename, "r");
+  if (!f) return fail(PQ_ERR_ARG, "cannot open %s", filename);
+  auto bad = [&](const char* what) { std::fclose(f); return fail(PQ_ERR_ARG, "%s: %s", filename, what); };
+  char tok[256];
+  if (std::fscanf(f, "%255s", tok) != 1) return bad("empty file");
+  std::vector<int> perm(npart);
+  bool have_perm = false;
+  if (std::strcmp(tok, "#Permutations") == 0) {
+    have_perm = true;
+    for (int i = 0; i < npart; ++i)
+      if (std::fscanf(f, "%d", &perm[i]) != 1) return bad("short permutation line");
+    int ch;
+    while ((ch = std::fgetc(f)) != EOF && ch != '\n') {}
+  }
+  // the rest of the header line, model state included (the token read first
+  // opens it when there is no permutation line)
+  {
+    if (have_perm && std::fscanf(f, "%255s", tok) != 1) return bad("no header line");
+    int ch;
+    while ((ch = std::fgetc(f)) != EOF && ch != '\n') {}
+  }
+  const int nf = nslice / bfactor;
+  std::vector<double> fs((size_t)nf * npart * ndim);
+  for (size_t q = 0; q < fs.size(); ++q)
+    if (std::fscanf(f, "%lf", &fs[q]) != 1) return bad("fewer coordinates than slices x particles x dimensions");
+  if (!have_perm) {
+    std::vector<double> wrap((size_t)npart * ndim);
+    for (size_t q = 0; q < wrap.size(); ++q)
+      if (std::fscanf(f, "%lf", &wrap[q]) != 1) return bad("no wrap slice and no #Permutations line");
+    for (int i = 0; i < npart; ++i) {
+      perm[i] = i;
+      for (int j = 0; j < npart; ++j) {
+        double sum = 0;
+        for (int d = 0; d < ndim; ++d) sum += std::fabs(wrap[(size_t)i * ndim + d] - fs[(size_t)j * ndim + d]);
+        if (sum < 10e-6) perm[i] = j;
+      }
+    }
+  }
+  std::fclose(f);
+  for (int i = 0; i < npart; ++i)
+    if (perm[i] != i) return fail(PQ_ERR_ARG, "%s: permuted paths are not supported (particle %d -> %d)", filename, i, perm[i]);
+  for (int k = 0; k < bfactor; ++k)
+    std::memcpy(beads + (size_t)k * nf * npart * ndim, fs.data(), sizeof(double) * fs.size());
+  return PQ_OK;
+}
+
+static std::string clone_file(const char* base, int c, int nchain) {
+  return nchain > 1 ? std::string(base) + std::to_string(c) : std::string(base);
+}
+
+extern "C" int pq_write_all_paths(pq_batch* b, const char* filename) {
+  if (!b || !filename) return fail(PQ_ERR_ARG, "pq_write_all_paths: bad arguments");
+  for (int c = 0; c < b->nchain; ++c) {
+    const ChainP& ch = b->hc[c];
+    std::vector<double> host((size_t)ch.M * ch.N * ch.D);
+    if (int rc = pq_get_paths(b, c, host.data())) return rc;
+    if (int rc = pq_write_paths_file(clone_file(filename, c, b->nchain).c_str(), host.data(), ch.M, ch.N, ch.D)) return rc;
+  }
+  return PQ_OK;
+}
+
+extern "C" int pq_read_all_paths(pq_batch* b, const char* filename, int bfactor) {
+  if (!b || !filename) return fail(PQ_ERR_ARG, "pq_read_all_paths: bad arguments");
+  for (int c = 0; c < b->nchain; ++c) {
+    const ChainP& ch = b->hc[c];
+    std::vector<double> host((size_t)ch.M * ch.N * ch.D);
+    if (int rc = pq_read_paths_file(clone_file(filename, c, b->nchain).c_str(), host.data(), ch.M, ch.N, ch.D, bfactor)) return rc;
+    if (int rc = pq_set_paths(b, c, host.data())) return rc;
+  }
+  return PQ_OK;
+}
+
 extern "C" int pq_seed(pq_batch* b, uint64_t seed, uint32_t first_stream) {
   if (!b) return fail(PQ_ERR_ARG, "null batch");
   HIPCHK(hipSetDevice(b->device));

@@ -22,6 +22,11 @@ PQ_MOVER_FREE = 0
 PQ_MOVER_FREE_PBC = 1
 PQ_NODES_FREE = 0
 PQ_NODES_SHO = 1
+PQ_OK = 0
+PQ_ERR_ARG = -1
+PQ_ERR_HIP = -2
+PQ_ERR_NOMEM = -3
+PQ_ERR_STATE = -4
 PQ_CHOOSER_SPECIES = 0
 PQ_CHOOSER_WALKING = 1
 PQ_CHOOSER_ALL = 2
@@ -238,6 +243,10 @@ def load_library(path=LIB_PATH):
         "pq_set_staging": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "pq_get_staging": (C.c_int, [C.c_void_p, ip, ip]),
         "pq_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float), ip]),
+        "pq_write_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int]),
+        "pq_read_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "pq_write_all_paths": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "pq_read_all_paths": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
         "pq_run_sweeps_timed": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float)]),
         "pq_get_all_stats": (C.c_int, [C.c_void_p, P(ChainStats)]),
         "pq_get_pair_table": (C.c_int, [C.c_void_p, C.c_int, C.c_int, ip, dp, dp, dp]),

@@ -3,6 +3,8 @@ on the same seeded inputs.  FP64 action and node values agree to 1e-12
 relative (relative to the larger of the value and the summed term scale);
 Metropolis decisions are identical given the same uniforms; integer
 histograms are bit-exact."""
+import os
+
 import numpy as np
 import pytest
 
@@ -496,3 +498,25 @@ def test_sampled_fermion_energy_matches_exact_nodes():
     z = (means.mean() - exact) / err
     print("E", means.mean(), "+-", err, "exact", exact, "z", z)
     assert abs(z) < 3
+
+
+def test_batch_path_files_round_trip(tmp_path):
+    """pq_write_all_paths / pq_read_all_paths: one file per clone (name + clone
+    index when the batch has several), sampled paths back to 15 digits."""
+    print("[test] batch_path_files", flush=True)
+    d = capi.cfg1_desc()
+    b = capi.Batch([d, d])
+    for c in range(2):
+        b.set_paths(c, capi.initial_beads(d, 60 + c))
+    b.seed(5, 0)
+    b.run_sweeps(2)
+    b.sync()
+    before = [b.get_paths(c) for c in range(2)]
+    base = str(tmp_path / "paths.out")
+    lib = capi.load_library()
+    capi.check(lib.pq_write_all_paths(b.h, base.encode()))
+    assert os.path.exists(base + "0") and os.path.exists(base + "1")
+    b2 = capi.Batch([d, d])
+    capi.check(lib.pq_read_all_paths(b2.h, base.encode(), 1))
+    for c in range(2):
+        assert np.allclose(b2.get_paths(c), before[c], rtol=1e-14, atol=1e-300)
