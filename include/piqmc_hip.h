@@ -225,6 +225,13 @@ int pq_reset_stats(pq_batch* batch);
    (SHONodes.cc:113-122 / FreeParticleNodes.cc:142-151 state). */
 int pq_get_node_scale(pq_batch* batch, int chain, double* scale);
 
+/* nblock x (nsweep sweeps of every chain, then the thermodynamic energy of
+   every chain) captured once into a HIP graph and replayed on later calls
+   with the same arguments and batch settings: energies[block][chain].  The
+   same results as nblock x (pq_run_sweeps(nsweep), pq_measure_energy), without a
+   host round trip per block (the <Loop>/<Measure> pattern). */
+int pq_run_measured(pq_batch* batch, int nblock, int nsweep, double* energies);
+
 /* Thermodynamic energy estimate of every chain's current paths:
    E = (1/M) sum_{t,i} utau(i,t) (ThermoEnergyEstimator + CompositeAction::getBeadAction). */
 int pq_measure_energy(pq_batch* batch, double* energy);

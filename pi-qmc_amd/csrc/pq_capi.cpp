@@ -204,6 +204,12 @@ struct pq_batch {
   double* dfnscr = nullptr;   // fixed-node protocol buffers, kept between calls
   size_t fnscr_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // measured-sweep graph: nblock x (sweeps, energy) captured once, replayed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> gkey;        // what the graph was captured with
+  double* dmeas = nullptr;       // [nblock][nchain] energies written by the graph
+  size_t nmeas = 0;
 };
 
 static DevCtx make_ctx(const pq_batch* b) {
@@ -579,6 +585,9 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->dmeas) (void)hipFree(b->dmeas);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -936,6 +945,67 @@ extern "C" int pq_measure_energy(pq_batch* b, double* energy) {
   DevCtx x = make_ctx(b);
   HIPCHK(launch_energy(b->D, x, b->nchain, (double*)p, b->stream));
   HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+static void drop_graph(pq_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+  b->gkey.clear();
+}
+
+// nblock x (nsweep sweeps, then ThermoEnergyEstimator on every chain) as one
+// HIP graph on the batch stream: the Loop/Measure pattern of a run without a
+// host round trip per block.  The graph is captured on the first call and
+// replayed while the batch's kernel arguments (schedule, seed, uniform
+// source) and (nblock, nsweep) stay the same.
+extern "C" int pq_run_measured(pq_batch* b, int nblock, int nsweep, double* energies) {
+  if (!b || nblock < 1 || nsweep < 1 || !energies) return fail(PQ_ERR_ARG, "pq_run_measured: bad arguments");
+  HIPCHK(hipSetDevice(b->device));
+  const size_t need = (size_t)nblock * b->nchain;
+  if (need > b->nmeas) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    drop_graph(b);
+    if (b->dmeas) (void)hipFree(b->dmeas);
+    b->dmeas = nullptr;
+    b->nmeas = 0;
+    HIPCHK(hipMalloc(&b->dmeas, sizeof(double) * need));
+    b->nmeas = need;
+  }
+  DevCtx x = make_ctx(b);
+  // the kernel arguments the graph bakes in (field by field: no padding bytes)
+  const uint64_t fields[] = {
+      (uint64_t)(uintptr_t)x.beads, (uint64_t)(uintptr_t)x.inj, (uint64_t)x.inj_per_chain,
+      (uint64_t)x.rng_mode, x.seed, (uint64_t)x.first_stream, (uint64_t)(uintptr_t)x.inv,
+      (uint64_t)(uintptr_t)x.fncache, (uint64_t)(uintptr_t)x.fnsweep, (uint64_t)nblock, (uint64_t)nsweep};
+  std::vector<char> key(sizeof(fields) + sizeof(SweepLayoutH));
+  std::memcpy(key.data(), fields, sizeof(fields));
+  std::memcpy(key.data() + sizeof(fields), &b->lay, sizeof(SweepLayoutH));
+  if (!b->gexec || key != b->gkey) {
+    drop_graph(b);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int blk = 0; blk < nblock && e == hipSuccess; ++blk) {
+      e = launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream);
+      if (e == hipSuccess) e = launch_energy(b->D, x, b->nchain, b->dmeas + (size_t)blk * b->nchain, b->stream);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(PQ_ERR_HIP, "capturing the measured sweeps failed: %s",
+                  hipGetErrorString(e != hipSuccess ? e : ee));
+    }
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->gkey = key;
+  }
+  HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+  HIPCHK(hipMemcpyAsync(energies, b->dmeas, sizeof(double) * need, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
 }

@@ -244,6 +244,7 @@ def load_library(path=LIB_PATH):
         "pq_get_staging": (C.c_int, [C.c_void_p, ip, ip]),
         "pq_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float), ip]),
         "pq_write_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int]),
+        "pq_run_measured": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp]),
         "pq_read_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "pq_write_all_paths": (C.c_int, [C.c_void_p, C.c_char_p]),
         "pq_read_all_paths": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),

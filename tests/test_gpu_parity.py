@@ -520,3 +520,34 @@ def test_batch_path_files_round_trip(tmp_path):
     capi.check(lib.pq_read_all_paths(b2.h, base.encode(), 1))
     for c in range(2):
         assert np.allclose(b2.get_paths(c), before[c], rtol=1e-14, atol=1e-300)
+
+
+def test_measured_graph_matches_eager():
+    """pq_run_measured's captured graph, replayed several times with eager
+    sweeps in between, gives the energies (and paths) of the same sweeps run
+    one launch at a time on a twin batch."""
+    print("[test] measured_graph", flush=True)
+    descs = [d for d in capi.cfg3_descs(16) if d.nslice <= 256][:6]
+    twins = []
+    for _ in range(2):
+        b = capi.Batch(descs, precision=capi.PQ_PREC_MIXED)
+        for c, d in enumerate(descs):
+            b.set_paths(c, capi.initial_beads(d, 300 + c))
+        b.seed(17, 0)
+        twins.append(b)
+    g, e = twins
+    lib = capi.load_library()
+    nblock, nsweep = 3, 2
+    for rep in range(3):
+        out = np.zeros(nblock * len(descs))
+        capi.check(lib.pq_run_measured(g.h, nblock, nsweep, capi.dptr(out)))
+        want = []
+        for _ in range(nblock):
+            e.run_sweeps(nsweep)
+            want.append(e.energy())
+        assert np.array_equal(out, np.concatenate(want)), rep
+        # eager work between replays
+        g.run_sweeps(1)
+        e.run_sweeps(1)
+    for c in range(len(descs)):
+        assert np.array_equal(g.get_paths(c), e.get_paths(c))
