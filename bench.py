@@ -115,13 +115,17 @@ def timed_sweeps(b, steps):
     return ms, time.perf_counter() - t0
 
 
-def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=None):
+def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=None, schedule=None):
     """warmup sweeps (the autotuner's timing sweeps count among them, and
     total_warmup, when given, pads the warmup to that many sweeps), then steps
-    timed sweeps."""
+    timed sweeps.  schedule = (team, inverse, tables, node state) fixes the
+    schedule instead of autotuning."""
     b = make_batch(descs, precision, seed)
     tuned, done = None, 0
-    if autotune:
+    if schedule is not None:
+        b.set_schedule(schedule[0], schedule[1])
+        b.set_staging(schedule[2], schedule[3])
+    elif autotune:
         tuned, done = b.autotune(1)
     extra = warmup if total_warmup is None else max(0, total_warmup - done)
     b.run_sweeps(extra)
@@ -207,6 +211,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-pair", action="store_true")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--schedule", default=None,
+                    help="team,inverse_lds,tables_lds,node_state_lds (e.g. 128,0,1,1): fixed schedule, no autotuning")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied by a plain device copy kernel first (HBM counter calibration)")
     args = ap.parse_args()
@@ -225,7 +231,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    b_main, main_run = run_gpu(descs, prec, args.steps, args.warmup, args.seed)
+    sched = tuple(int(v) for v in args.schedule.split(",")) if args.schedule else None
+    b_main, main_run = run_gpu(descs, prec, args.steps, args.warmup, args.seed, schedule=sched)
     bu_total = float(main_run["bu"].sum())
     value = bu_total / main_run["wall"]
     kernel_ms = float(np.mean(main_run["ms"]))
