@@ -199,8 +199,9 @@ int pq_run_sweeps(pq_batch* batch, int nsweep);
 int pq_run_sweeps_timed(pq_batch* batch, int nsweep, float* ms);
 int pq_sync(pq_batch* batch);
 /* Schedule of the sampler kernel: team = threads per chain (64 = one wave,
-   128 = two waves, -1 automatic), inv_lds = node inverse cache in LDS (1),
-   global memory (0) or automatic (-1).  Results do not depend on it. */
+   128 = two waves, 256 or 512 = four or eight waves for batches of few
+   chains, -1 automatic), inv_lds = node inverse cache in LDS (1), global memory (0) or
+   automatic (-1).  Results agree to rounding whatever the schedule. */
 int pq_set_schedule(pq_batch* batch, int team, int inv_lds);
 int pq_get_schedule(const pq_batch* batch, int* team, int* inv_lds, int* lds_bytes);
 /* Placement of the sampler's pair tables (u, the arithmetic type's copy) and
@@ -209,7 +210,7 @@ int pq_get_schedule(const pq_batch* batch, int* team, int* inv_lds, int* lds_byt
    tables first, then the node state, then the inverse cache. */
 int pq_set_staging(pq_batch* batch, int tables_lds, int state_lds);
 int pq_get_staging(const pq_batch* batch, int* tables_lds, int* state_lds);
-/* Times every distinct schedule (team 64/128 x tables, node state and
+/* Times every distinct schedule (team 64/128/256/512 x tables, node state and
    inverse cache in LDS or global memory) over nsweep sampler sweeps of the
    batch and keeps the fastest; best_ms receives its time per sweep and
    ntimed the number of schedules timed (ntimed * nsweep sweeps were run).

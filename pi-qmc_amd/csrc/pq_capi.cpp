@@ -438,6 +438,11 @@ static void make_layout(pq_batch* b, const Sched& req) {
     }
     mTab = std::max<int64_t>(mTab, tabs);
   }
+  // team: two waves per chain when node matrices dominate, else one; either
+  // way two waves fit per SIMD.  Four-wave teams serve batches of few chains.
+  int team = nodes ? 128 : PQ_WAVE;
+  if (req.team == 64 || req.team == 128 || req.team == 256 || req.team == 512) team = req.team;
+  b->lay.team = team;
   int64_t o = align8(sizeof(ChainP));
   b->lay.off_sec1 = (int)o; o += mSND * 8;
   b->lay.off_sec2 = (int)o; o += mSND * 8;
@@ -452,9 +457,9 @@ static void make_layout(pq_batch* b, const Sched& req) {
   b->lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
   o = align8(o);
   // Gauss-Jordan scratch: pivot rows (2 doubles per thread), or for 32-wide
-  // node matrices the per-wave MFMA scratch of the two waves of a team
+  // node matrices the MFMA scratch of every wave of the team
   b->lay.off_gj = (int)o;
-  o += b->NC == 32 ? 2 * PQ_GJ32_SCRATCH_DOUBLES * 8 : 2 * 8 * 128;
+  o += b->NC == 32 ? (int64_t)(team / PQ_WAVE) * PQ_GJ32_SCRATCH_DOUBLES * 8 : (int64_t)2 * 8 * team;
   const int64_t budget = 40 * 1024, whole = 160 * 1024;
   auto place = [&](int r, int64_t bytes, bool wanted) -> int {
     if (!wanted || bytes <= 0 || r == 0) return -1;
@@ -467,11 +472,6 @@ static void make_layout(pq_batch* b, const Sched& req) {
   b->lay.off_fst = place(req.fst, mF * 8, nodes);
   b->lay.off_inv = place(req.inv, mInv * 8, nodes);
   b->lay.bytes = (int)align8(o);
-  // automatic team: two waves per chain when node matrices dominate, else one;
-  // either way two waves fit per SIMD, so four chains share a CU
-  int team = nodes ? 128 : PQ_WAVE;
-  if (req.team == 64 || req.team == 128) team = req.team;
-  b->lay.team = team;
 }
 
 extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int precision, int device,
@@ -800,7 +800,8 @@ static int apply_schedule(pq_batch* b, const Sched& s) {
 
 extern "C" int pq_set_schedule(pq_batch* b, int team, int inv_lds) {
   if (!b) return fail(PQ_ERR_ARG, "null batch");
-  if (team != -1 && team != 64 && team != 128) return fail(PQ_ERR_ARG, "team must be 64, 128 or -1");
+  if (team != -1 && team != 64 && team != 128 && team != 256 && team != 512)
+    return fail(PQ_ERR_ARG, "team must be 64, 128, 256, 512 or -1");
   if (inv_lds < -1 || inv_lds > 1) return fail(PQ_ERR_ARG, "inv_lds must be -1, 0 or 1");
   Sched s = b->sched;
   s.team = team;
@@ -838,7 +839,7 @@ extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed)
   HIPCHK(hipSetDevice(b->device));
   // candidate schedules; layouts that come out identical are timed once
   std::vector<Sched> cands;
-  for (int team : {64, 128})
+  for (int team : {64, 128, 256, 512})
     for (int tab : {0, 1})
       for (int fst : {0, 1})
         for (int inv : {0, 1}) cands.push_back(Sched{team, inv, tab, fst});

@@ -16,8 +16,10 @@ struct SweepLayout {
   int bytes;
 };
 
+// two waves per SIMD whatever the team: up to 256 threads two workgroups share
+// a CU, a 512-thread team has one to itself
 template <int D, int NC, class R, int NT>
-__global__ __launch_bounds__(NT, 2) void sweep_kernel(DevCtx x, int nsweep, SweepLayout lay) {
+__global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, int nsweep, SweepLayout lay) {
   extern __shared__ double smem[];
   const int chain = x.order[blockIdx.x];
   ChainP& c = *reinterpret_cast<ChainP*>(smem);
@@ -201,6 +203,8 @@ static hipError_t launch_sweep_nt(const DevCtx& x, int nchain, int nsweep, const
 template <int D, int NC, class R>
 static hipError_t launch_sweep_t(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
                                  hipStream_t st) {
+  if (L.team == 512) return launch_sweep_nt<D, NC, R, 512>(x, nchain, nsweep, L, st);
+  if (L.team == 256) return launch_sweep_nt<D, NC, R, 256>(x, nchain, nsweep, L, st);
   if (L.team == 128) return launch_sweep_nt<D, NC, R, 128>(x, nchain, nsweep, L, st);
   return launch_sweep_nt<D, NC, R, PQ_WAVE>(x, nchain, nsweep, L, st);
 }

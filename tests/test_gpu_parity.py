@@ -146,7 +146,7 @@ def test_large_species_nodes(ns, model, L):
         assert rel_close(det[m], dw, 1e-10, scale=0.0), (m, det[m], dw)
         assert rel_close(d1[m * ns:(m + 1) * ns], e1, 1e-9)
         assert rel_close(d2[m * ns:(m + 1) * ns], e2, 1e-9)
-    bs, os_ = _paired(desc, 3, seed=11)
+    bs, os_ = _paired(desc, 3, seed=11, team={32: 256, 24: 512}.get(ns, -1))
     sg, so = bs.stats(0), os_.stats()
     assert sg.draws == so.draws
     assert list(sg.accepts) == list(so.accepts)
@@ -221,7 +221,8 @@ def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False, tea
                                                    ("cfg2", True, 64, 0), ("cfg2", True, 128, 1),
                                                    ("cfg2", False, 64, 1), ("pbc", False, -1, -1),
                                                    ("cfg1_walk", True, -1, -1), ("cfg1_all", True, -1, -1),
-                                                   ("cfg2_walk", False, -1, -1)])
+                                                   ("cfg2_walk", False, -1, -1), ("cfg2", True, 256, 0),
+                                                   ("pbc", True, 256, -1), ("cfg2", False, 512, 1)])
 def test_sampler_decisions(name, inject, team, inv):
     """Same uniforms in, same accept/reject decisions and paths out, for every schedule."""
     print(f"[test] sampler_decisions {name} inject={inject} team={team} inv={inv}", flush=True)
