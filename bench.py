@@ -118,13 +118,15 @@ def timed_sweeps(b, steps):
 def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=None, schedule=None):
     """warmup sweeps (the autotuner's timing sweeps count among them, and
     total_warmup, when given, pads the warmup to that many sweeps), then steps
-    timed sweeps.  schedule = (team, inverse, tables, node state) fixes the
-    schedule instead of autotuning."""
+    timed sweeps.  schedule = (team, inverse, tables, node state[, heavy
+    chains, heavy team]) fixes the schedule instead of autotuning."""
     b = make_batch(descs, precision, seed)
     tuned, done = None, 0
     if schedule is not None:
         b.set_schedule(schedule[0], schedule[1])
         b.set_staging(schedule[2], schedule[3])
+        if len(schedule) > 4:
+            b.set_split(schedule[4], schedule[5])
     elif autotune:
         tuned, done = b.autotune(1)
     extra = warmup if total_warmup is None else max(0, total_warmup - done)
@@ -139,8 +141,9 @@ def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=N
     energy = b.energy()
     team, inv, lds = b.schedule()
     tab, fst = b.staging()
+    heavy, heavy_team = b.split()
     sched = {"team": team, "inverse_lds": inv, "tables_lds": tab, "node_state_lds": fst,
-             "lds_bytes": lds, "autotune_ms": tuned}
+             "lds_bytes": lds, "heavy_chains": heavy, "heavy_team": heavy_team, "autotune_ms": tuned}
     return b, dict(ms=ms, wall=wall, bu=bu, abu=abu, energy=energy, schedule=sched, warmup_done=done)
 
 
@@ -212,7 +215,8 @@ def main():
     ap.add_argument("--no-fp64-pair", action="store_true")
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--schedule", default=None,
-                    help="team,inverse_lds,tables_lds,node_state_lds (e.g. 128,0,1,1): fixed schedule, no autotuning")
+                    help="team,inverse_lds,tables_lds,node_state_lds[,heavy_chains,heavy_team] "
+                         "(e.g. 128,0,1,1,64,512): fixed schedule, no autotuning")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied by a plain device copy kernel first (HBM counter calibration)")
     args = ap.parse_args()
@@ -260,10 +264,13 @@ def main():
                      "valu_frac_of_fp64_peak": flops_launch / (kernel_ms * 1e-3) / 1e12 / FP64_VECTOR_TFLOPS},
     }
     if not args.no_fp64_pair and prec == capi.PQ_PREC_MIXED:
-        # the same number of sweeps as the mixed run (automatic schedule; the
-        # mixed run's autotuning sweeps are matched by plain warmup sweeps)
+        # the same number of sweeps and the schedule the mixed run settled on
+        # (its autotuning sweeps are matched by plain warmup sweeps)
+        ms_ = main_run["schedule"]
+        same = (ms_["team"], ms_["inverse_lds"], ms_["tables_lds"], ms_["node_state_lds"],
+                ms_["heavy_chains"], ms_["heavy_team"])
         b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed,
-                           autotune=False, total_warmup=main_run["warmup_done"])
+                           total_warmup=main_run["warmup_done"], schedule=same)
         out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
                                "kernel_ms": float(np.mean(ref["ms"]))}
         out["energy_vs_fp64"] = paired_energy(b_main, b64, main_run["energy"], ref["energy"])

@@ -210,9 +210,17 @@ int pq_get_schedule(const pq_batch* batch, int* team, int* inv_lds, int* lds_byt
    tables first, then the node state, then the inverse cache. */
 int pq_set_staging(pq_batch* batch, int tables_lds, int state_lds);
 int pq_get_staging(const pq_batch* batch, int* tables_lds, int* state_lds);
+/* The heaviest heavy_chains chains (the batch launches chains in decreasing
+   order of estimated cost) run in a launch of their own with heavy_team
+   threads per chain (128, 256 or 512), concurrently with the rest; 0 turns
+   the split off.  The sweep's length is the heaviest chain's, so giving it
+   more waves shortens every launch of an uneven batch. */
+int pq_set_split(pq_batch* batch, int heavy_chains, int heavy_team);
+int pq_get_split(const pq_batch* batch, int* heavy_chains, int* heavy_team);
 /* Times every distinct schedule (team 64/128/256/512 x tables, node state and
-   inverse cache in LDS or global memory) over nsweep sampler sweeps of the
-   batch and keeps the fastest; best_ms receives its time per sweep and
+   inverse cache in LDS or global memory, then splits of the heaviest chains
+   on the best of those) over nsweep sampler sweeps of the batch and keeps
+   the fastest; best_ms receives its time per sweep and
    ntimed the number of schedules timed (ntimed * nsweep sweeps were run).
    The tuning sweeps advance the chains like pq_run_sweeps: schedules change
    only where data lives and how many threads share a chain, so results agree

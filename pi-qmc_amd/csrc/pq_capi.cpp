@@ -160,6 +160,8 @@ int pg_nterms(double alpha, double length) { return 1 + (int)(2.0 * std::sqrt(al
 // requested sampler schedule: -1 automatic, else team size or 0/1 placement
 struct Sched {
   int team = -1, inv = -1, tab = -1, fst = -1;
+  int heavy = 0;          // the heaviest chains launched apart with a larger team
+  int heavy_team = 256;
 };
 
 struct pq_batch {
@@ -169,6 +171,8 @@ struct pq_batch {
   int D = 3;
   int NC = 4;     // node-matrix width class of the batch: 4, 8, 10, 12, 16 or 32
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // the heavy chains' launch, forked from and joined to stream
+  hipEvent_t evf = nullptr, evj = nullptr;
   std::vector<pq_chain_desc> descs;
   std::vector<ChainP> hc;
   std::vector<TableP> ht;
@@ -176,6 +180,7 @@ struct pq_batch {
   std::vector<int> order;
   int64_t nbeads = 0;
   SweepLayoutH lay{};
+  SweepLayoutH lay_heavy{};
   Sched sched{};   // requested schedule, all automatic
   // device
   ChainP* dchains = nullptr;
@@ -420,7 +425,8 @@ static int64_t align8(int64_t v) { return (v + 7) & ~(int64_t)7; }
 // inv = node inverse cache in LDS.  Automatic placement fills a 40 KB budget
 // (four chains per CU) in the order tables, fixed-node state, inverses;
 // requested placements may use the whole 160 KB.
-static void make_layout(pq_batch* b, const Sched& req) {
+static SweepLayoutH layout_for(const pq_batch* b, const Sched& req) {
+  SweepLayoutH lay{};
   int64_t mS = 0, mSND = 0, mSD = 0, mF = 0, mInv = 0, mTab = 0;
   bool nodes = false;
   const int64_t w = b->precision == PQ_PREC_MIXED ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
@@ -442,23 +448,23 @@ static void make_layout(pq_batch* b, const Sched& req) {
   // way two waves fit per SIMD.  Four-wave teams serve batches of few chains.
   int team = nodes ? 128 : PQ_WAVE;
   if (req.team == 64 || req.team == 128 || req.team == 256 || req.team == 512) team = req.team;
-  b->lay.team = team;
+  lay.team = team;
   int64_t o = align8(sizeof(ChainP));
-  b->lay.off_sec1 = (int)o; o += mSND * 8;
-  b->lay.off_sec2 = (int)o; o += mSND * 8;
-  b->lay.off_mov = (int)o; o += align8(mSD * 8);
-  b->lay.off_wdet = (int)o; o += PQ_WAVE * 8;
-  b->lay.off_wscale = (int)o; o += align8(mS * 8);
-  b->lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
-  b->lay.off_red = (int)o; o += 8 * 8;
-  b->lay.off_cnt = (int)o; o += 24 * 8;
-  b->lay.off_werr = (int)o; o += PQ_WAVE * 4;
-  b->lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
-  b->lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
+  lay.off_sec1 = (int)o; o += mSND * 8;
+  lay.off_sec2 = (int)o; o += mSND * 8;
+  lay.off_mov = (int)o; o += align8(mSD * 8);
+  lay.off_wdet = (int)o; o += PQ_WAVE * 8;
+  lay.off_wscale = (int)o; o += align8(mS * 8);
+  lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
+  lay.off_red = (int)o; o += 8 * 8;
+  lay.off_cnt = (int)o; o += 24 * 8;
+  lay.off_werr = (int)o; o += PQ_WAVE * 4;
+  lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
+  lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
   o = align8(o);
   // Gauss-Jordan scratch: pivot rows (2 doubles per thread), or for 32-wide
   // node matrices the MFMA scratch of every wave of the team
-  b->lay.off_gj = (int)o;
+  lay.off_gj = (int)o;
   o += b->NC == 32 ? (int64_t)(team / PQ_WAVE) * PQ_GJ32_SCRATCH_DOUBLES * 8 : (int64_t)2 * 8 * team;
   const int64_t budget = 40 * 1024, whole = 160 * 1024;
   auto place = [&](int r, int64_t bytes, bool wanted) -> int {
@@ -468,10 +474,41 @@ static void make_layout(pq_batch* b, const Sched& req) {
     o = align8(o + bytes);
     return at;
   };
-  b->lay.off_tab = place(req.tab, mTab * w, mTab > 0);
-  b->lay.off_fst = place(req.fst, mF * 8, nodes);
-  b->lay.off_inv = place(req.inv, mInv * 8, nodes);
-  b->lay.bytes = (int)align8(o);
+  lay.off_tab = place(req.tab, mTab * w, mTab > 0);
+  lay.off_fst = place(req.fst, mF * 8, nodes);
+  lay.off_inv = place(req.inv, mInv * 8, nodes);
+  lay.bytes = (int)align8(o);
+  return lay;
+}
+
+static void make_layout(pq_batch* b, const Sched& req) {
+  b->lay = layout_for(b, req);
+  b->lay_heavy = b->lay;
+  if (req.heavy > 0) {
+    Sched h = req;
+    h.team = req.heavy_team;
+    b->lay_heavy = layout_for(b, h);
+  }
+}
+
+static int layout_bytes(const pq_batch* b) { return std::max(b->lay.bytes, b->lay_heavy.bytes); }
+
+// one sampler launch over the batch: the heaviest chains (first in the cost
+// order) on the second stream with their own team size, the rest on the
+// batch stream; the batch stream waits for both
+static hipError_t run_launch(pq_batch* b, const DevCtx& x, int nsweep) {
+  const bool mixed = b->precision == PQ_PREC_MIXED;
+  const int h = b->sched.heavy;
+  if (h <= 0 || h >= b->nchain) return launch_sweep(b->D, b->NC, mixed, x, b->nchain, nsweep, b->lay, b->stream);
+  hipError_t e = hipEventRecord(b->evf, b->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->stream2, b->evf, 0);
+  if (e == hipSuccess) e = launch_sweep(b->D, b->NC, mixed, x, h, nsweep, b->lay_heavy, b->stream2);
+  DevCtx xl = x;
+  xl.order = x.order + h;
+  if (e == hipSuccess) e = launch_sweep(b->D, b->NC, mixed, xl, b->nchain - h, nsweep, b->lay, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(b->evj, b->stream2);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->evj, 0);
+  return e;
 }
 
 extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int precision, int device,
@@ -515,8 +552,8 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     fnc += nrec;
   }
   make_layout(b, b->sched);
-  if (b->lay.bytes > 160 * 1024) {
-    int n = b->lay.bytes;
+  if (layout_bytes(b) > 160 * 1024) {
+    int n = layout_bytes(b);
     delete b;
     return fail(PQ_ERR_ARG, "chain too large for one workgroup's LDS (%d bytes)", n);
   }
@@ -534,8 +571,11 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   std::stable_sort(b->order.begin(), b->order.end(), [&](int p, int q) { return cost[p] > cost[q]; });
 
   auto cleanup_fail = [&](int rc) { pq_batch_destroy(b); return rc; };
-  if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup_fail(fail(PQ_ERR_HIP, "hipStreamCreate failed"));
-  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->stream2) != hipSuccess)
+    return cleanup_fail(fail(PQ_ERR_HIP, "hipStreamCreate failed"));
+  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&b->evf, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->evj, hipEventDisableTiming) != hipSuccess)
     return cleanup_fail(fail(PQ_ERR_HIP, "hipEventCreate failed"));
   auto dalloc = [&](void** p, size_t bytes) -> bool {
     if (bytes == 0) bytes = 8;
@@ -588,6 +628,10 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->dmeas) (void)hipFree(b->dmeas);
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  if (b->evf) (void)hipEventDestroy(b->evf);
+  if (b->evj) (void)hipEventDestroy(b->evj);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -786,13 +830,14 @@ static int apply_schedule(pq_batch* b, const Sched& s) {
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   const Sched keep = b->sched;
-  const SweepLayoutH keep_lay = b->lay;
+  const SweepLayoutH keep_lay = b->lay, keep_heavy = b->lay_heavy;
   b->sched = s;
   make_layout(b, s);
-  if (b->lay.bytes > 160 * 1024) {
-    const int n = b->lay.bytes;
+  if (layout_bytes(b) > 160 * 1024) {
+    const int n = layout_bytes(b);
     b->sched = keep;
     b->lay = keep_lay;
+    b->lay_heavy = keep_heavy;
     return fail(PQ_ERR_ARG, "schedule needs %d bytes of LDS", n);
   }
   return PQ_OK;
@@ -834,41 +879,77 @@ extern "C" int pq_get_staging(const pq_batch* b, int* tables_lds, int* state_lds
   return PQ_OK;
 }
 
+extern "C" int pq_set_split(pq_batch* b, int heavy_chains, int heavy_team) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (heavy_chains < 0 || heavy_chains >= b->nchain) return fail(PQ_ERR_ARG, "heavy_chains must be in [0, nchain)");
+  if (heavy_team != 128 && heavy_team != 256 && heavy_team != 512) return fail(PQ_ERR_ARG, "heavy_team must be 128, 256 or 512");
+  Sched s = b->sched;
+  s.heavy = heavy_chains;
+  s.heavy_team = heavy_team;
+  return apply_schedule(b, s);
+}
+
+extern "C" int pq_get_split(const pq_batch* b, int* heavy_chains, int* heavy_team) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (heavy_chains) *heavy_chains = b->sched.heavy;
+  if (heavy_team) *heavy_team = b->sched.heavy_team;
+  return PQ_OK;
+}
+
 extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed) {
   if (!b || nsweep < 1) return fail(PQ_ERR_ARG, "pq_autotune: bad arguments");
   HIPCHK(hipSetDevice(b->device));
-  // candidate schedules; layouts that come out identical are timed once
-  std::vector<Sched> cands;
-  for (int team : {64, 128, 256, 512})
-    for (int tab : {0, 1})
-      for (int fst : {0, 1})
-        for (int inv : {0, 1}) cands.push_back(Sched{team, inv, tab, fst});
-  std::vector<SweepLayoutH> seen;
+  std::vector<std::vector<char>> seen;
   float best = -1;
   Sched best_s = b->sched;
-  for (const Sched& s : cands) {
+  int timed = 0;
+  // time one candidate unless an identical launch configuration was timed
+  auto trial = [&](const Sched& s) -> int {
     b->sched = s;
     make_layout(b, s);
-    if (b->lay.bytes > 160 * 1024) continue;
-    bool dup = false;
-    for (const SweepLayoutH& l : seen)
-      if (std::memcmp(&l, &b->lay, sizeof(SweepLayoutH)) == 0) dup = true;
-    if (dup) continue;
-    seen.push_back(b->lay);
+    if (layout_bytes(b) > 160 * 1024) return PQ_OK;
+    std::vector<char> key(2 * sizeof(SweepLayoutH) + sizeof(int));
+    std::memcpy(key.data(), &b->lay, sizeof(SweepLayoutH));
+    std::memcpy(key.data() + sizeof(SweepLayoutH), &b->lay_heavy, sizeof(SweepLayoutH));
+    std::memcpy(key.data() + 2 * sizeof(SweepLayoutH), &s.heavy, sizeof(int));
+    for (const auto& k : seen)
+      if (k == key) return PQ_OK;
+    seen.push_back(key);
     float ms = 0;
     DevCtx x = make_ctx(b);
     HIPCHK(hipEventRecord(b->ev0, b->stream));
-    HIPCHK(launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+    HIPCHK(run_launch(b, x, nsweep));
     HIPCHK(hipEventRecord(b->ev1, b->stream));
     HIPCHK(hipEventSynchronize(b->ev1));
     HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     ms /= nsweep;
+    ++timed;
     if (best < 0 || ms < best) { best = ms; best_s = s; }
-  }
+    return PQ_OK;
+  };
+  // every team size and placement in one launch
+  for (int team : {64, 128, 256, 512})
+    for (int tab : {0, 1})
+      for (int fst : {0, 1})
+        for (int inv : {0, 1}) {
+          Sched s{};
+          s.team = team; s.inv = inv; s.tab = tab; s.fst = fst;
+          if (int rc = trial(s)) return rc;
+        }
+  // then the heaviest chains apart with a larger team, on the best placement
+  const Sched base = best_s;
+  for (int frac : {16, 8, 4})
+    for (int ht : {256, 512}) {
+      Sched s = base;
+      s.heavy = b->nchain / frac;
+      s.heavy_team = ht;
+      if (s.heavy < 1 || s.heavy >= b->nchain) continue;
+      if (int rc = trial(s)) return rc;
+    }
   b->sched = best_s;
   make_layout(b, best_s);
   if (best_ms) *best_ms = best;
-  if (ntimed) *ntimed = (int)seen.size();
+  if (ntimed) *ntimed = timed;
   return PQ_OK;
 }
 
@@ -877,7 +958,7 @@ extern "C" int pq_run_sweeps(pq_batch* b, int nsweep) {
   if (nsweep == 0) return PQ_OK;
   HIPCHK(hipSetDevice(b->device));
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+  HIPCHK(run_launch(b, x, nsweep));
   return PQ_OK;
 }
 
@@ -886,7 +967,7 @@ extern "C" int pq_run_sweeps_timed(pq_batch* b, int nsweep, float* ms) {
   HIPCHK(hipSetDevice(b->device));
   DevCtx x = make_ctx(b);
   HIPCHK(hipEventRecord(b->ev0, b->stream));
-  HIPCHK(launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream));
+  HIPCHK(run_launch(b, x, nsweep));
   HIPCHK(hipEventRecord(b->ev1, b->stream));
   HIPCHK(hipEventSynchronize(b->ev1));
   HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
@@ -981,17 +1062,19 @@ extern "C" int pq_run_measured(pq_batch* b, int nblock, int nsweep, double* ener
   const uint64_t fields[] = {
       (uint64_t)(uintptr_t)x.beads, (uint64_t)(uintptr_t)x.inj, (uint64_t)x.inj_per_chain,
       (uint64_t)x.rng_mode, x.seed, (uint64_t)x.first_stream, (uint64_t)(uintptr_t)x.inv,
-      (uint64_t)(uintptr_t)x.fncache, (uint64_t)(uintptr_t)x.fnsweep, (uint64_t)nblock, (uint64_t)nsweep};
-  std::vector<char> key(sizeof(fields) + sizeof(SweepLayoutH));
+      (uint64_t)(uintptr_t)x.fncache, (uint64_t)(uintptr_t)x.fnsweep, (uint64_t)nblock, (uint64_t)nsweep,
+      (uint64_t)b->sched.heavy};
+  std::vector<char> key(sizeof(fields) + 2 * sizeof(SweepLayoutH));
   std::memcpy(key.data(), fields, sizeof(fields));
   std::memcpy(key.data() + sizeof(fields), &b->lay, sizeof(SweepLayoutH));
+  std::memcpy(key.data() + sizeof(fields) + sizeof(SweepLayoutH), &b->lay_heavy, sizeof(SweepLayoutH));
   if (!b->gexec || key != b->gkey) {
     drop_graph(b);
     HIPCHK(hipStreamSynchronize(b->stream));
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int blk = 0; blk < nblock && e == hipSuccess; ++blk) {
-      e = launch_sweep(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, b->nchain, nsweep, b->lay, b->stream);
+      e = run_launch(b, x, nsweep);
       if (e == hipSuccess) e = launch_energy(b->D, x, b->nchain, b->dmeas + (size_t)blk * b->nchain, b->stream);
     }
     hipGraph_t g = nullptr;

@@ -243,6 +243,8 @@ def load_library(path=LIB_PATH):
         "pq_set_staging": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "pq_get_staging": (C.c_int, [C.c_void_p, ip, ip]),
         "pq_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float), ip]),
+        "pq_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "pq_get_split": (C.c_int, [C.c_void_p, ip, ip]),
         "pq_write_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int]),
         "pq_run_measured": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp]),
         "pq_read_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -351,6 +353,14 @@ class Batch:
         t, f = C.c_int(), C.c_int()
         check(self.lib.pq_get_staging(self.h, C.byref(t), C.byref(f)))
         return t.value, f.value
+
+    def set_split(self, heavy_chains=0, heavy_team=256):
+        check(self.lib.pq_set_split(self.h, heavy_chains, heavy_team))
+
+    def split(self):
+        n, t = C.c_int(), C.c_int()
+        check(self.lib.pq_get_split(self.h, C.byref(n), C.byref(t)))
+        return n.value, t.value
 
     def autotune(self, nsweep=1):
         """Time every distinct schedule over nsweep sweeps and keep the fastest;

@@ -314,12 +314,16 @@ def test_histogram_bit_exact():
             assert np.array_equal(counts[c * 64:(c + 1) * 64], want)
 
 
-def test_batched_uneven_decisions():
-    """A slice of the uneven batch (SHO and free-particle nodes, N 2..20, M up to 256)."""
-    print("[test] batched_uneven", flush=True)
+@pytest.mark.parametrize("split", [(0, 256), (3, 256), (2, 512)])
+def test_batched_uneven_decisions(split):
+    """A slice of the uneven batch (SHO and free-particle nodes, N 2..20, M up
+    to 256), whole or with its heaviest chains in a launch of their own."""
+    print(f"[test] batched_uneven split={split}", flush=True)
     descs = [d for d in capi.cfg3_descs(24) if d.nslice <= 256][:8]
     b = capi.Batch(descs)
     assert b.schedule()[0] == 128
+    b.set_split(*split)
+    assert b.split() == split
     ocs = []
     for c, d in enumerate(descs):
         beads = capi.initial_beads(d, 500 + c)
@@ -524,9 +528,10 @@ def test_batch_path_files_round_trip(tmp_path):
 
 
 def test_measured_graph_matches_eager():
-    """pq_run_measured's captured graph, replayed several times with eager
-    sweeps in between, gives the energies (and paths) of the same sweeps run
-    one launch at a time on a twin batch."""
+    """pq_run_measured's captured graph (heaviest chains forked onto the
+    second stream), replayed several times with eager sweeps in between,
+    gives the energies (and paths) of the same sweeps run one launch at a
+    time on a twin batch."""
     print("[test] measured_graph", flush=True)
     descs = [d for d in capi.cfg3_descs(16) if d.nslice <= 256][:6]
     twins = []
@@ -537,6 +542,7 @@ def test_measured_graph_matches_eager():
         b.seed(17, 0)
         twins.append(b)
     g, e = twins
+    g.set_split(2, 256)
     lib = capi.load_library()
     nblock, nsweep = 3, 2
     for rep in range(3):
