@@ -33,13 +33,14 @@ FP64_VECTOR_TFLOPS = 78.6
 
 
 def measured_traffic(workload_name, precision):
-    """HBM bytes per sweep_kernel launch from the committed rocprofv3 FETCH_SIZE
-    and WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+    """HBM bytes per step (every sweep_kernel launch of one sweep) from the
+    committed rocprofv3 FETCH_SIZE and WRITE_SIZE summary of this workload
+    (copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", f"sweep_kernel_{workload_name}_{precision}.md")
     try:
         with open(path) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith("traffic_bytes_per_step:"):
                     return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
     except OSError:
         pass

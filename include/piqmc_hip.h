@@ -219,9 +219,10 @@ int pq_set_split(pq_batch* batch, int heavy_chains, int heavy_team);
 int pq_get_split(const pq_batch* batch, int* heavy_chains, int* heavy_team);
 /* Times every distinct schedule (team 64/128/256/512 x tables, node state and
    inverse cache in LDS or global memory, then splits of the heaviest chains
-   on the best of those) over nsweep sampler sweeps of the batch and keeps
-   the fastest; best_ms receives its time per sweep and
-   ntimed the number of schedules timed (ntimed * nsweep sweeps were run).
+   on the two best of those) over nsweep sampler sweeps of the batch, after
+   one untimed launch of nsweep sweeps, and keeps the fastest; best_ms
+   receives its time per sweep and ntimed the number of launches run, the
+   untimed one included (ntimed * nsweep sweeps were run).
    The tuning sweeps advance the chains like pq_run_sweeps: schedules change
    only where data lives and how many threads share a chain, so results agree
    to rounding.  Replaces nothing in the reference (it has no device
