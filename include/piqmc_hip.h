@@ -44,7 +44,7 @@ extern "C" {
 #define PQ_MAX_SHO 4
 #define PQ_MAX_NODES 4
 #define PQ_MAX_SAMPLES 8
-#define PQ_MAX_MOVING 4
+#define PQ_MAX_MOVING 8
 #define PQ_MAX_NORDER 4
 #define PQ_MAX_NODE_PART 32     /* largest fermion species handled by a node model */
 
@@ -54,12 +54,12 @@ extern "C" {
 #define PQ_NODES_FREE 0         /* FreeParticleNodes (src/fixednode/FreeParticleNodes.cc:72-234) */
 #define PQ_NODES_SHO 1          /* SHONodes          (src/fixednode/SHONodes.cc:51-170) */
 
-/* Particle and permutation choosers of a <Sample> with one moving particle
+/* Particle and permutation choosers of a <Sample npart=nmoving>
    (src/parser/PIMCParser.cc:364-453). */
 #define PQ_CHOOSER_SPECIES 0    /* species given, noPermutation="true": PermutationChooser +
                                    SpeciesParticleChooser (SpeciesParticleChooser.cc:19-31) */
 #define PQ_CHOOSER_WALKING 1    /* species given, noPermutation absent or false: WalkingChooser
-                                   (WalkingChooser.cc:39-87), a 1-cycle for one particle */
+                                   (WalkingChooser.cc:39-142), an nmoving-cycle */
 #define PQ_CHOOSER_ALL 2        /* species "" or "all", noPermutation="true": PermutationChooser +
                                    SimpleParticleChooser (SimpleParticleChooser.cc:15-27) */
 #define PQ_CHOOSER_ALL_RANDOM 3 /* species "" or "all", noPermutation false: RandomPermutationChooser
@@ -111,7 +111,7 @@ typedef struct pq_node_desc {
 /* <Sample npart species noPermutation> inside <ChooseSection>. */
 typedef struct pq_sample_desc {
   int species;         /* ignored by the PQ_CHOOSER_ALL* choosers */
-  int nmoving;         /* 1 */
+  int nmoving;         /* npart: 1..PQ_MAX_MOVING, at most the particles it chooses from */
   int chooser;         /* PQ_CHOOSER_* */
 } pq_sample_desc;
 
@@ -165,6 +165,12 @@ int pq_set_paths(pq_batch* batch, int chain, const double* beads);
 int pq_get_paths(pq_batch* batch, int chain, double* beads);
 /* All chains at once, concatenated in chain order. */
 int pq_set_all_paths(pq_batch* batch, const double* beads, int64_t count);
+/* Path permutation of a chain: the bead after (i, M-1) is (perm[i], 0)
+   (SerialPaths permutation / inversePermutation, src/base/SerialPaths.cc:68-117;
+   ReadPaths sets it, src/algorithm/PathReader.cc:38-50,113-117).  Setting
+   paths resets it to the identity; permutation sampling changes it. */
+int pq_set_permutation(pq_batch* batch, int chain, const int* perm);
+int pq_get_permutation(pq_batch* batch, int chain, int* perm);
 int pq_get_all_paths(pq_batch* batch, double* beads, int64_t count);
 
 /* Path files of <WritePaths file> / <ReadPaths file bfactor>
@@ -172,15 +178,19 @@ int pq_get_all_paths(pq_batch* batch, double* beads, int64_t count);
    a "#Permutations" line, a "#Path coordinates: N particles in D-d" line,
    then one line per slice with the N*D coordinates at 15 significant digits.
    Reading accepts the reference's files with or without the permutation
-   line (without it, a trailing wrap slice), fills slice s + k nslice/bfactor
-   from file slice s, and refuses non-identity permutations.  beads are
+   line (without it, a trailing wrap slice) and fills copy k of the file's
+   slices, k < bfactor, relabelled by the k-th power of the file's
+   permutation p, as ReadPaths' putBeads of each copy's last slice leaves
+   them; perm receives p^bfactor (null: a non-identity p is refused).  Writing
+   takes the permutation (null: the identity).  beads are
    [nslice][npart][ndim].  These two touch no device. */
-int pq_write_paths_file(const char* filename, const double* beads, int nslice,
-                        int npart, int ndim);
-int pq_read_paths_file(const char* filename, double* beads, int nslice, int npart,
-                       int ndim, int bfactor);
-/* Every chain of a batch; chain c uses filename + c when the batch has more
-   than one chain, as the reference appends the clone ID when nclone > 1. */
+int pq_write_paths_file(const char* filename, const double* beads, const int* perm,
+                        int nslice, int npart, int ndim);
+int pq_read_paths_file(const char* filename, double* beads, int* perm, int nslice,
+                       int npart, int ndim, int bfactor);
+/* Every chain of a batch, paths and permutation; chain c uses filename + c
+   when the batch has more than one chain, as the reference appends the clone
+   ID when nclone > 1. */
 int pq_write_all_paths(pq_batch* batch, const char* filename);
 int pq_read_all_paths(pq_batch* batch, const char* filename, int bfactor);
 
@@ -220,9 +230,10 @@ int pq_get_split(const pq_batch* batch, int* heavy_chains, int* heavy_team);
 /* Times every distinct schedule (team 64/128/256/512 x tables, node state and
    inverse cache in LDS or global memory, then splits of the heaviest chains
    on the two best of those) over nsweep sampler sweeps of the batch, after
-   one untimed launch of nsweep sweeps, and keeps the fastest; best_ms
-   receives its time per sweep and ntimed the number of launches run, the
-   untimed one included (ntimed * nsweep sweeps were run).
+   one untimed launch of nsweep sweeps; the four fastest are timed three
+   more times each, in turn, and the lowest mean wins.  best_ms receives that
+   mean per sweep and ntimed the number of launches run, the untimed one
+   included (ntimed * nsweep sweeps were run).
    The tuning sweeps advance the chains like pq_run_sweeps: schedules change
    only where data lives and how many threads share a chain, so results agree
    to rounding.  Replaces nothing in the reference (it has no device

@@ -423,6 +423,16 @@ struct oq_chain {
   fn_t fn[PQ_MAX_NODES];
   double *sec1, *sec2, *mov;
   int idx[PQ_MAX_MOVING];
+  int cyc[PQ_MAX_MOVING];          /* permutation of the moving particles */
+  /* SerialPaths permutation / inversePermutation: bead (i, t + M) is bead
+     (perm[i], t) (src/base/SerialPaths.cc:68-117) */
+  int perm[MAXN], invperm[MAXN];
+  int secperm[MAXN];               /* section permutation of section 1 */
+  /* WalkingChooser tables t(i,j) and cump(i,j) of the active Sample, rows
+     filled on first use (src/advancer/WalkingChooser.cc:89-118) */
+  double *wt, *wcump;
+  unsigned char* wrow;
+  pg_t wpg[PQ_MAX_SPECIES][3];
   oq_rng rng;
   pq_chain_stats st;
 };
@@ -432,11 +442,21 @@ static int pg_index(const oq_chain* ch, int level, int sp, int d) {
 }
 
 oq_chain* oq_create(const pq_chain_desc* desc) {
-  /* the samplers here move one particle per Sample */
-  for (int k = 0; k < desc->nsample; ++k)
-    if (desc->sample[k].nmoving != 1 || desc->sample[k].chooser < PQ_CHOOSER_SPECIES ||
-        desc->sample[k].chooser > PQ_CHOOSER_ALL_RANDOM)
-      return NULL;
+  /* a Sample moves 1..PQ_MAX_MOVING distinct particles; more than its
+     particle pool holds would make the reference's choosers draw forever */
+  int ntot = 0, maxcount = 1;
+  for (int s = 0; s < desc->nspecies; ++s) {
+    ntot += desc->species[s].count;
+    if (desc->species[s].count > maxcount) maxcount = desc->species[s].count;
+  }
+  for (int k = 0; k < desc->nsample; ++k) {
+    const pq_sample_desc* sm = &desc->sample[k];
+    if (sm->chooser < PQ_CHOOSER_SPECIES || sm->chooser > PQ_CHOOSER_ALL_RANDOM) return NULL;
+    if (sm->nmoving < 1 || sm->nmoving > PQ_MAX_MOVING) return NULL;
+    const int pool = (sm->chooser == PQ_CHOOSER_ALL || sm->chooser == PQ_CHOOSER_ALL_RANDOM)
+                         ? ntot : desc->species[sm->species].count;
+    if (sm->nmoving > pool) return NULL;
+  }
   oq_chain* ch = (oq_chain*)calloc(1, sizeof(oq_chain));
   ch->d = *desc;
   ch->D = desc->ndim;
@@ -546,6 +566,14 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
   ch->sec1 = (double*)calloc(secsz, sizeof(double));
   ch->sec2 = (double*)calloc(secsz, sizeof(double));
   ch->mov = (double*)calloc((size_t)ch->S * PQ_MAX_MOVING * ch->D, sizeof(double));
+  for (int i = 0; i < ch->N; ++i) ch->perm[i] = ch->invperm[i] = i;
+  ch->wt = (double*)calloc((size_t)maxcount * maxcount, sizeof(double));
+  ch->wcump = (double*)calloc((size_t)maxcount * maxcount, sizeof(double));
+  ch->wrow = (unsigned char*)calloc(maxcount, 1);
+  /* WalkingChooser pg(d) = PeriodicGaussian(m/(2 tau 2^L), a_d) (WalkingChooser.cc:20-25) */
+  for (int s = 0; s < ch->nsp; ++s)
+    for (int d = 0; d < ch->D; ++d)
+      pg_init(&ch->wpg[s][d], desc->species[s].mass / (2 * ch->tau * (1 << ch->L)), desc->cell[d]);
   oq_rng_glibc(ch, 0);
   return ch;
 }
@@ -559,11 +587,28 @@ void oq_destroy(oq_chain* ch) {
   }
   free(ch->spg); free(ch->mpg); free(ch->spg_on); free(ch->mpg_on);
   free(ch->beads); free(ch->sec1); free(ch->sec2); free(ch->mov);
+  free(ch->wt); free(ch->wcump); free(ch->wrow);
   free(ch);
 }
 
 void oq_set_beads(oq_chain* ch, const double* b) {
   memcpy(ch->beads, b, sizeof(double) * ch->M * ch->N * ch->D);
+}
+
+/* Paths permutation as ReadPaths sets it (src/algorithm/PathReader.cc:60-75);
+   returns -1 unless p is a permutation of 0..N-1 */
+int oq_set_permutation(oq_chain* ch, const int* p) {
+  unsigned char seen[MAXN] = {0};
+  for (int i = 0; i < ch->N; ++i) {
+    if (p[i] < 0 || p[i] >= ch->N || seen[p[i]]) return -1;
+    seen[p[i]] = 1;
+  }
+  for (int i = 0; i < ch->N; ++i) ch->perm[i] = p[i];
+  for (int i = 0; i < ch->N; ++i) ch->invperm[p[i]] = i;
+  return 0;
+}
+void oq_get_permutation(const oq_chain* ch, int* p) {
+  for (int i = 0; i < ch->N; ++i) p[i] = ch->perm[i];
 }
 void oq_get_beads(const oq_chain* ch, double* b) {
   memcpy(b, ch->beads, sizeof(double) * ch->M * ch->N * ch->D);
@@ -794,10 +839,12 @@ double oq_action_difference(oq_chain* ch, const double* sec, const double* mov,
   return diff;
 }
 
-/* path access with wrap, identity permutation (SerialPaths.cc:68-117) */
+/* path access for -M <= t < 2M: slices before 0 belong to
+   inversePermutation[i], slices from M on to permutation[i]
+   (SerialPaths::operator(), src/base/SerialPaths.cc:68-104) */
 static const double* bead(const oq_chain* ch, int i, int t) {
-  t %= ch->M;
-  if (t < 0) t += ch->M;
+  if (t < 0) { i = ch->invperm[i]; t += ch->M; }
+  else if (t >= ch->M) { i = ch->perm[i]; t -= ch->M; }
   return &ch->beads[((size_t)t * ch->N + i) * ch->D];
 }
 
@@ -1221,25 +1268,110 @@ double oq_mover(oq_chain* ch, const double* sec, double* mov, const int* idx,
 
 /* ------------------------------------------------------------ sampler */
 
-/* SerialPaths::getBeads / putBeads with the identity permutation
-   (src/base/SerialPaths.cc:119-185) */
+/* SerialPaths::getBeads (src/base/SerialPaths.cc:119-132): slices before 0
+   are read through inversePermutation, slices from M on through permutation */
 static void get_beads(oq_chain* ch, int t0, double* sec) {
-  const size_t slice = (size_t)ch->N * ch->D;
+  const int N = ch->N, D = ch->D, M = ch->M;
   for (int k = 0; k < ch->S; ++k) {
     int t = k + t0;
-    if (t < 0) t += ch->M;
-    else if (t >= ch->M) t -= ch->M;
-    memcpy(&sec[k * slice], &ch->beads[(size_t)t * slice], sizeof(double) * slice);
+    const int* map = NULL;
+    if (t < 0) { t += M; map = ch->invperm; }
+    else if (t >= M) { t -= M; map = ch->perm; }
+    for (int i = 0; i < N; ++i) {
+      const int src = map ? map[i] : i;
+      memcpy(&sec[((size_t)k * N + i) * D], &ch->beads[((size_t)t * N + src) * D], sizeof(double) * D);
+    }
   }
 }
 
-static void put_beads(oq_chain* ch, int t0, const double* sec) {
-  const size_t slice = (size_t)ch->N * ch->D;
-  for (int k = 0; k < ch->S; ++k) {
+static int is_identity(const int* p, int n) {
+  for (int i = 0; i < n; ++i)
+    if (p[i] != i) return 0;
+  return 1;
+}
+
+/* Permutation::prepend / append (src/util/Permutation.cc:23-43):
+   prepend(q): p(i) <- p(q(i));  append(q): p(i) <- q(p(i)) */
+static void perm_prepend(int* p, const int* q, int n) {
+  int tmp[MAXN];
+  for (int i = 0; i < n; ++i) tmp[i] = p[q[i]];
+  memcpy(p, tmp, sizeof(int) * n);
+}
+static void perm_append(int* p, const int* q, int n) {
+  int tmp[MAXN];
+  for (int i = 0; i < n; ++i) tmp[i] = q[p[i]];
+  memcpy(p, tmp, sizeof(int) * n);
+}
+
+void oq_perm_compose(int* p, const int* q, int n, int append) {
+  if (append) perm_append(p, q, n);
+  else perm_prepend(p, q, n);
+}
+
+/* SerialPaths::putBeads (src/base/SerialPaths.cc:144-185): the section's
+   slices go back (before 0: skipped, from M on: to particle permutation[i]);
+   then, for a non-identity section permutation sp, the slices after the
+   section are relabelled, bead(i, t) <- bead(temp[i], t), temp = sp, or
+   permutation o sp o inversePermutation when the section ran past M, and the
+   path permutation becomes permutation o temp. */
+static void put_beads(oq_chain* ch, int t0, const double* sec, const int* sp) {
+  const int N = ch->N, D = ch->D, M = ch->M, S = ch->S;
+  for (int k = 0; k < S; ++k) {
     int t = k + t0;
     if (t < 0) continue;
-    if (t >= ch->M) t %= ch->M;
-    memcpy(&ch->beads[(size_t)t * slice], &sec[k * slice], sizeof(double) * slice);
+    if (t < M) {
+      memcpy(&ch->beads[(size_t)t * N * D], &sec[(size_t)k * N * D], sizeof(double) * N * D);
+    } else {
+      t %= M;
+      for (int i = 0; i < N; ++i)
+        memcpy(&ch->beads[((size_t)t * N + ch->perm[i]) * D], &sec[((size_t)k * N + i) * D], sizeof(double) * D);
+    }
+  }
+  if (is_identity(sp, N)) return;
+  int ifollowing = t0 + S;
+  int temp[MAXN];
+  memcpy(temp, sp, sizeof(int) * N);
+  if (ifollowing > M) {
+    ifollowing -= M;
+    perm_prepend(temp, ch->invperm, N);
+    perm_append(temp, ch->perm, N);
+  }
+  double* buf = (double*)malloc(sizeof(double) * N * D);
+  for (int t = ifollowing; t < M; ++t) {
+    double* sl = &ch->beads[(size_t)t * N * D];
+    for (int i = 0; i < N; ++i) memcpy(&buf[i * D], &sl[temp[i] * D], sizeof(double) * D);
+    memcpy(sl, buf, sizeof(double) * N * D);
+  }
+  free(buf);
+  perm_prepend(ch->perm, temp, N);
+  for (int i = 0; i < N; ++i) ch->invperm[ch->perm[i]] = i;
+}
+
+/* SerialPaths::putDoubleBeads (src/base/SerialPaths.cc:187-206) with an
+   unsampled section 2 (DoubleMLSampler without "both": its permutation p2
+   stays the identity, so p1 p2 p1^-1 is the identity and section 2's beads
+   are relabelled by p1 when section 1 goes first). */
+static void put_double_beads(oq_chain* ch, int t0, int t0b) {
+  const int N = ch->N, D = ch->D, S = ch->S;
+  int ident[MAXN];
+  for (int i = 0; i < N; ++i) ident[i] = i;
+  if (t0 < ch->M / 2 - 1) {
+    put_beads(ch, t0, ch->sec1, ch->secperm);
+    if (!is_identity(ch->secperm, N)) {
+      /* Beads::permute (src/base/Beads.h:254-261): coord(i,t) <- coord(p[i],t) */
+      double* buf = (double*)malloc(sizeof(double) * N * D);
+      for (int k = 0; k < S; ++k) {
+        double* sl = &ch->sec2[(size_t)k * N * D];
+        for (int i = 0; i < N; ++i) memcpy(&buf[i * D], &sl[ch->secperm[i] * D], sizeof(double) * D);
+        memcpy(sl, buf, sizeof(double) * N * D);
+      }
+      free(buf);
+      put_beads(ch, t0b, ch->sec2, ident);
+    }
+  } else {
+    /* section 2 is unchanged and its permutation the identity: putting it
+       back first rewrites the same values */
+    put_beads(ch, t0, ch->sec1, ch->secperm);
   }
 }
 
@@ -1256,72 +1388,159 @@ static void choose_particles(oq_chain* ch, int sp, int nmov, int* idx) {
   }
 }
 
-/* Permutation and particle choice of a Sample with one moving particle, in
-   the order of MultiLevelSampler::run (src/advancer/MultiLevelSampler.cc:68-80)
-   and DoubleMLSampler::run (src/advancer/DoubleMLSampler.cc:76-81):
-   choosePermutation, then chooseParticles.  Returns 0 when the permutation
-   chooser turns the move down (isNewPerm false).
-   - SPECIES: PermutationChooser::choosePermutation draws nothing
-     (src/advancer/PermutationChooser.cc:13-15), then SpeciesParticleChooser.
-   - WALKING: WalkingChooser::choosePermutation for nsize = 1
-     (src/advancer/WalkingChooser.cc:39-87): i0 = (int)(npart u), the cycle
-     closes on i0 itself, so tranProb and revTranProb are the same number
-     h/t(i0,i0) and accept = 1 (NaN when t(i0,i0) = 0); the move goes ahead iff
-     a second uniform is < accept.  t(i0,i0) is the product of periodic
-     Gaussians of alpha = m/(2 tau 2^L) over the separation of particle i0's
-     first and last section beads (WalkingChooser.cc:20-25,89-103); its
-     chooseParticles draws nothing (:36-37).
-   - ALL / ALL_RANDOM: PermutationChooser, or RandomPermutationChooser for
-     nsize = 1 (one draw, src/advancer/RandomPermutationChooser.cc:12-24),
-     then SimpleParticleChooser over all particles, which draws again while
-     (int)(N u) == N (src/advancer/SimpleParticleChooser.cc:15-27). */
-static int choose(oq_chain* ch, const pq_sample_desc* s, int* idx) {
-  const int kind = s->chooser;
-  if (kind == PQ_CHOOSER_WALKING) {
-    const int sp = s->species, npart = ch->d.species[sp].count;
-    int i0 = (int)(npart * rng_next(&ch->rng));
-    if (i0 == npart) i0 = npart - 1;
-    i0 += ch->ifirst[sp];
-    double delta[3];
-    for (int d = 0; d < ch->D; ++d) delta[d] = SEC(ch->sec1, 0, i0, d) - SEC(ch->sec1, ch->S - 1, i0, d);
-    pbc(&ch->cell, delta);
-    double tii = 1;
-    for (int d = 0; d < ch->D; ++d) {
-      pg_t p;
-      pg_init(&p, ch->d.species[sp].mass / (2 * ch->d.tau * (1 << ch->L)), ch->d.cell[d]);
-      tii *= pg_eval(&p, delta[d]);
+/* SimpleParticleChooser::chooseParticles (src/advancer/SimpleParticleChooser.cc:15-27) */
+static void choose_any(oq_chain* ch, int nmov, int* idx) {
+  const int N = ch->N;
+  for (int im = 0; im < nmov;) {
+    int i = (int)(N * rng_next(&ch->rng));
+    if (i == N) continue;
+    for (int jm = 0; jm <= im; ++jm) {
+      if (jm == im) { idx[im++] = i; break; }
+      if (idx[jm] == i) break;
     }
-    const double accept = (tii > 0) ? 1.0 : NAN;
-    idx[0] = i0;
-    return rng_next(&ch->rng) < accept;
+  }
+}
+
+/* RandomPermutationChooser::choosePermutation (src/advancer/RandomPermutationChooser.cc:12-24) */
+static void random_permutation(oq_chain* ch, int nsize, int* p) {
+  for (int ifrom = 0; ifrom < nsize;) {
+    int ito = (int)(nsize * rng_next(&ch->rng));
+    if (ito == nsize) ito = nsize - 1;
+    for (int jfrom = 0; jfrom <= ifrom; ++jfrom) {
+      if (jfrom == ifrom) { p[ifrom++] = ito; break; }
+      if (p[jfrom] == ito) break;
+    }
+  }
+}
+
+/* row i of WalkingChooser::init's tables (src/advancer/WalkingChooser.cc:89-118):
+   t(i,j) = prod_d PG(pbc(r_i(first) - r_j(last))_d), cump(i,j) = sum_{k<=j} t(i,k) */
+static void walk_row(oq_chain* ch, int sp, int i) {
+  const int npart = ch->d.species[sp].count, ifirst = ch->ifirst[sp], D = ch->D;
+  if (ch->wrow[i]) return;
+  double* t = &ch->wt[(size_t)i * npart];
+  double* cump = &ch->wcump[(size_t)i * npart];
+  for (int j = 0; j < npart; ++j) {
+    double delta[3];
+    for (int d = 0; d < D; ++d)
+      delta[d] = SEC(ch->sec1, 0, i + ifirst, d) - SEC(ch->sec1, ch->S - 1, j + ifirst, d);
+    pbc(&ch->cell, delta);
+    t[j] = 1;
+    for (int d = 0; d < D; ++d) t[j] *= pg_eval(&ch->wpg[sp][d], delta[d]);
+  }
+  cump[0] = t[0];
+  for (int j = 1; j < npart; ++j) cump[j] = cump[j - 1] + t[j];
+  ch->wrow[i] = 1;
+}
+
+/* WalkingChooser::iSearch (src/advancer/WalkingChooser.cc:120-142) */
+static int walk_search(const double* cump, int npart, double x) {
+  if (x < cump[0]) return 0;
+  if (cump[npart - 2] < x) return npart - 1;
+  int ilo = 0, ihi = npart - 2, mid = 0;
+  for (int k = 0; k < npart - 1; k++) {
+    if (ihi == ilo + 1) return ihi;
+    mid = (ilo + ihi) / 2;
+    if (cump[mid] > x) ihi = mid;
+    else ilo = mid;
+  }
+  return mid;
+}
+
+/* WalkingChooser::choosePermutation (src/advancer/WalkingChooser.cc:39-87):
+   a walk of nsize distinct particles, each next one drawn with probability
+   t(prev, j)/h(prev); the moving particles form the cycle m -> m+1 (mod
+   nsize) set in the constructor (:26-28).  lnTranProb = log prod_m
+   t(i_m,i_m)/t(i_m,i_m+1); the move goes ahead iff a last uniform is below
+   the ratio of reverse to forward selection probabilities. */
+static int walk_choose(oq_chain* ch, int sp, int nsize, int* idx, double* lnTP) {
+  const int npart = ch->d.species[sp].count, ifirst = ch->ifirst[sp];
+  memset(ch->wrow, 0, npart);   /* init(): a new table per Sample */
+  double tranProb = 0, revTranProb = 0;
+  for (int ip = 0; ip < nsize; ++ip) {
+    if (ip == 0) {
+      idx[0] = (int)(npart * rng_next(&ch->rng));
+      if (idx[0] == npart) idx[0] = npart - 1;
+    } else {
+      const int prev = idx[ip - 1];
+      walk_row(ch, sp, prev);
+      const double* t = &ch->wt[(size_t)prev * npart];
+      const double h = ch->wcump[(size_t)prev * npart + npart - 1];
+      const double x = h * rng_next(&ch->rng);
+      idx[ip] = walk_search(&ch->wcump[(size_t)prev * npart], npart, x);
+      for (int jp = 0; jp < ip; ++jp)
+        if (idx[ip] == idx[jp]) return 0;
+      revTranProb += h / t[prev];
+      tranProb += h / t[idx[ip]];
+    }
+  }
+  const int last = idx[nsize - 1];
+  walk_row(ch, sp, last);
+  const double h = ch->wcump[(size_t)last * npart + npart - 1];
+  revTranProb += h / ch->wt[(size_t)last * npart + last];
+  tranProb += h / ch->wt[(size_t)last * npart + idx[0]];
+  const double accept = revTranProb / tranProb;
+  double prob = 1;
+  for (int m = 0; m < nsize; ++m) {
+    walk_row(ch, sp, idx[m]);
+    const double* t = &ch->wt[(size_t)idx[m] * npart];
+    prob *= t[idx[m]] / t[idx[(m + 1) % nsize]];
+  }
+  *lnTP = log(prob);
+  for (int m = 0; m < nsize; ++m) idx[m] += ifirst;
+  return rng_next(&ch->rng) < accept;
+}
+
+/* Permutation and particle choice of a Sample, in the order of
+   MultiLevelSampler::run (src/advancer/MultiLevelSampler.cc:68-80) and
+   DoubleMLSampler::run (src/advancer/DoubleMLSampler.cc:76-81):
+   choosePermutation, then chooseParticles.  Fills idx (the moving particles),
+   cyc (the permutation: moving particle m ends on idx[cyc[m]]'s last bead)
+   and the log transition probability; returns 0 when the permutation chooser
+   turns the move down (isNewPerm false).
+   - SPECIES: PermutationChooser (identity, no draw,
+     src/advancer/PermutationChooser.cc:13-15), then SpeciesParticleChooser.
+   - WALKING: WalkingChooser (walk_choose); its chooseParticles draws nothing
+     (src/advancer/WalkingChooser.cc:36-37).
+   - ALL / ALL_RANDOM: PermutationChooser, or RandomPermutationChooser of
+     nmoving elements, then SimpleParticleChooser over all particles. */
+static int choose(oq_chain* ch, const pq_sample_desc* s, int* idx, int* cyc, double* lnTP) {
+  const int kind = s->chooser, nmov = s->nmoving;
+  *lnTP = 0;
+  for (int m = 0; m < nmov; ++m) cyc[m] = m;
+  if (kind == PQ_CHOOSER_WALKING) {
+    for (int m = 0; m < nmov; ++m) cyc[m] = (m + 1) % nmov;
+    return walk_choose(ch, s->species, nmov, idx, lnTP);
   }
   if (kind == PQ_CHOOSER_ALL || kind == PQ_CHOOSER_ALL_RANDOM) {
-    if (kind == PQ_CHOOSER_ALL_RANDOM) (void)rng_next(&ch->rng);
-    const int N = ch->N;
-    for (;;) {
-      const int i = (int)(N * rng_next(&ch->rng));
-      if (i != N) { idx[0] = i; return 1; }
-    }
+    if (kind == PQ_CHOOSER_ALL_RANDOM) random_permutation(ch, nmov, cyc);
+    choose_any(ch, nmov, idx);
+    return 1;
   }
-  choose_particles(ch, s->species, 1, idx);
+  choose_particles(ch, s->species, nmov, idx);
   return 1;
 }
 
 /* MultiLevelSampler::run/tryMove (src/advancer/MultiLevelSampler.cc:68-177) and
-   DoubleMLSampler::run/tryMove (src/advancer/DoubleMLSampler.cc:66-245) for one
-   moving particle, nrepeat 1, no delayed rejection. */
+   DoubleMLSampler::run/tryMove (src/advancer/DoubleMLSampler.cc:66-245), nrepeat
+   1, no delayed rejection, section 2 not sampled.  The moving beads start
+   from the chosen particles' first beads and end on the permuted particles'
+   last beads; an accepted move writes them back (last slice included) and
+   appends the permutation to the section permutation. */
 static int sample(oq_chain* ch, const pq_sample_desc* smp) {
   const int D = ch->D, S = ch->S, L = ch->L;
-  const int nmov = 1;
+  const int nmov = smp->nmoving;
   int* idx = ch->idx;
-  if (!choose(ch, smp, idx)) return 0;
+  int* cyc = ch->cyc;
+  double lnTP;
+  if (!choose(ch, smp, idx, cyc, &lnTP)) return 0;
   double* mov = ch->mov;
   for (int m = 0; m < nmov; ++m)
     for (int d = 0; d < D; ++d) {
       MOV(mov, 0, m, d) = SEC(ch->sec1, 0, idx[m], d);
-      MOV(mov, S - 1, m, d) = SEC(ch->sec1, S - 1, idx[m], d);
+      MOV(mov, S - 1, m, d) = SEC(ch->sec1, S - 1, idx[cyc[m]], d);
     }
-  double old = 0.0;
+  double old = lnTP;
   for (int l = L; l >= 0; --l) {
     ch->st.tries[l]++;
     if (l < L) ch->st.bead_updates += (uint64_t)nmov << (L - 1 - l);
@@ -1338,6 +1557,9 @@ static int sample(oq_chain* ch, const pq_sample_desc* smp) {
   for (int t = 0; t < S; ++t)
     for (int m = 0; m < nmov; ++m)
       for (int d = 0; d < D; ++d) SEC(ch->sec1, t, idx[m], d) = MOV(mov, t, m, d);
+  int temp[PQ_MAX_MOVING];
+  for (int m = 0; m < nmov; ++m) temp[m] = ch->secperm[idx[m]];
+  for (int m = 0; m < nmov; ++m) ch->secperm[idx[m]] = temp[cyc[m]];
   ch->st.accepted_bead_updates += (uint64_t)nmov * ((1u << L) - 1);
   return 1;
 }
@@ -1346,14 +1568,15 @@ static int sample(oq_chain* ch, const pq_sample_desc* smp) {
    DoubleSectionChooser::run (src/advancer/DoubleSectionChooser.cc:33-65) */
 static void run_section(oq_chain* ch) {
   const int M = ch->M;
-  int t0;
+  int t0, t0b = 0;
+  for (int i = 0; i < ch->N; ++i) ch->secperm[i] = i;
   if (ch->nfn) {
     double x = rng_next(&ch->rng) * (1 - 1e-8);
     int ilo = -1, ihi = M / 2 - 1;
     t0 = ilo + (int)((ihi + 1 - ilo) * x);
     if (t0 > ihi) t0 = ihi;
     if (rng_next(&ch->rng) >= 0.5) t0 = (t0 + M / 2 + 1) % M - 1;
-    int t0b = (t0 + M / 2 + 1) % M - 1;
+    t0b = (t0 + M / 2 + 1) % M - 1;
     get_beads(ch, t0, ch->sec1);
     get_beads(ch, t0b, ch->sec2);
     oq_fn_initialize(ch, ch->sec1, ch->sec2);
@@ -1365,7 +1588,8 @@ static void run_section(oq_chain* ch) {
     get_beads(ch, t0, ch->sec1);
   }
   for (int s = 0; s < ch->d.nsample; ++s) sample(ch, &ch->d.sample[s]);
-  put_beads(ch, t0, ch->sec1);
+  if (ch->nfn) put_double_beads(ch, t0, t0b);
+  else put_beads(ch, t0, ch->sec1, ch->secperm);
 }
 
 void oq_run_sweeps(oq_chain* ch, int nsweep) {

@@ -35,6 +35,12 @@ oq_chain* oq_create(const pq_chain_desc* desc);
 void oq_destroy(oq_chain* ch);
 void oq_set_beads(oq_chain* ch, const double* beads);
 void oq_get_beads(const oq_chain* ch, double* beads);
+/* path permutation: bead (i, t + M) is bead (perm[i], t); set returns -1
+   unless perm is a permutation of 0..N-1 */
+int oq_set_permutation(oq_chain* ch, const int* perm);
+/* Permutation::prepend (append = 0) or append (append = 1) of q onto p */
+void oq_perm_compose(int* p, const int* q, int n, int append);
+void oq_get_permutation(const oq_chain* ch, int* perm);
 int oq_npair(const oq_chain* ch);
 int oq_pair_table(const oq_chain* ch, int ipair, int* meta /*s1,s2,norder,ngpts*/,
                   double* grid /*rgridinv,logrratioinv*/, double* u, double* ut);

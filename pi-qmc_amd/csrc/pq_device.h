@@ -42,6 +42,7 @@ struct DevCtx {
   uint64_t seed;
   uint32_t first_stream;
   const int* order;       // workgroup -> chain
+  int* perm;              // per-chain path permutation then its inverse, [2][N]
 };
 
 // ------------------------------------------------------------ wave ops

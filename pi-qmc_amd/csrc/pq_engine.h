@@ -287,9 +287,20 @@ __device__ __forceinline__ void gj_invert_mfma32(pq_d4 (&acc)[2][2], int n, GjMf
 
 constexpr int gj_group(int nc) { return nc <= 4 ? 4 : (nc <= 8 ? 8 : (nc <= 16 ? 16 : 32)); }
 
-template <int D, int NC, class R, int NT>
+// MM bounds the moving particles of a move: the sampler's one-particle
+// instances (MM = 1) fold the moving-particle loops away and never permute.
+template <int D, int NC, class R, int NT, int MM = PQ_MAX_MOVING>
 struct Eng {
   static constexpr int G = gj_group(NC);
+  // chooser results shared by the team through LDS
+  struct ChooseBuf {
+    int idx[PQ_MAX_MOVING];   // moving particles
+    int cyc[PQ_MAX_MOVING];   // moving particle m ends on idx[cyc[m]]'s last bead
+    unsigned long long k;     // draw counter after the chooser
+    double lnTP;              // log transition probability of the permutation
+    int ok;
+    int pad;
+  };
   const DevCtx& x;
   const ChainP& c;
   int chain;
@@ -300,7 +311,9 @@ struct Eng {
   double* red;      // [NT/64] block-reduction scratch
   double* sec1;     // [S][N][D]
   double* sec2;     // [S][N][D]
-  double* mov;      // [S][D]
+  double* mov;      // [S][nmov][D]
+  int nmov_ = 1;    // moving particles of the current move, nmov()
+  const int* midx = nullptr;   // [nmov] their indices
   double* fst;      // fixed-node state block
   double* inv;      // [S][ins][ins] latest inverse per section slice
   double* egr;      // [S][ins][ins][D] gradient factors of the latest matrix elements
@@ -310,11 +323,18 @@ struct Eng {
   int* werr;        // [NT/G] scratch: error flags of a pass
   double* wscale;   // [S] scale used per slice in initialize
   double* gjrow;    // [NT/G][2 NC] pivot rows of the Gauss-Jordan groups
+  // sampler only: chooser scratch and WalkingChooser rows (t, cump) in LDS
+  ChooseBuf* cbuf = nullptr;
+  double* wrows = nullptr;
   // sampler only: pair tables staged in LDS (tabofs[pair] offsets into tab)
   const R* tab = nullptr;
   const int* tabofs = nullptr;
   // sampler only: node records of every slice pair, reused across sections
   double* fcache = nullptr;
+  // the records hold global particle labels: a section that wraps past slice
+  // M under a non-identity path permutation sees other labels, so the
+  // sampler switches them off for it
+  bool rec_on = true;
   int sec_t0 = 0;          // first slice of section 1
   bool top_known = false;  // the section's last slice is never moved
   int* lasterr = nullptr;  // [nfn] error flag of the last slice at initialize
@@ -337,98 +357,240 @@ struct Eng {
   }
 
   __device__ __forceinline__ double U(uint64_t k) const { return uniform(x, chain, k); }
+  __device__ __forceinline__ int nmov() const { return MM == 1 ? 1 : nmov_; }
 
   __device__ __forceinline__ double* sb(double* s, int t, int i) const {
     return s + ((int64_t)t * c.N + i) * D;
   }
 
   // ------------------------------------------------------------ choosers
-  // choosePermutation then chooseParticles of Sample smp with one moving
-  // particle (MultiLevelSampler.cc:68-80, DoubleMLSampler.cc:76-81); false
-  // when the permutation chooser turns the move down.
-  //   SPECIES     PermutationChooser (no draw) + SpeciesParticleChooser
+  // choosePermutation then chooseParticles of Sample smp (MultiLevelSampler.cc:
+  // 68-80, DoubleMLSampler.cc:76-81).  Sets nmov, midx (in cbuf) and the
+  // permutation cbuf->cyc; lnTP is the permutation chooser's log transition
+  // probability; false when the permutation chooser turns the move down.
+  //   SPECIES     PermutationChooser (identity, no draw) + SpeciesParticleChooser
   //               (SpeciesParticleChooser.cc:19-31)
-  //   WALKING     WalkingChooser for a 1-cycle (WalkingChooser.cc:39-87):
-  //               i0 = (int)(npart u); forward and reverse probabilities are
-  //               the same number h/t(i0,i0), so the move goes ahead iff a
-  //               second uniform is < 1, and never when t(i0,i0) = 0 (the
-  //               ratio is NaN); t(i0,i0) = prod_d PG_{m/(2 tau 2^L)} of the
-  //               separation of i0's first and last section beads (:89-103)
+  //   WALKING     WalkingChooser (WalkingChooser.cc:39-142), walk_choose
   //   ALL         PermutationChooser + SimpleParticleChooser over every
   //               particle, redrawing (int)(N u) == N (SimpleParticleChooser.cc:15-27)
-  //   ALL_RANDOM  RandomPermutationChooser of one element (one draw,
-  //               RandomPermutationChooser.cc:12-24) + SimpleParticleChooser
-  __device__ bool choose(int smp, uint64_t& k, int& i) {
+  //   ALL_RANDOM  RandomPermutationChooser of nmov elements
+  //               (RandomPermutationChooser.cc:12-24) + SimpleParticleChooser
+  // The draws are sequential and cheap: one thread makes them and the team
+  // reads the result back from LDS.
+  __device__ bool choose(int smp, uint64_t& k, double& lnTP) {
     const int kind = c.sample_ch[smp];
     const int sp = c.sample_sp[smp];
-    if (kind == PQ_CHOOSER_WALKING) {
-      int i0 = (int)(c.count[sp] * U(k++));
-      if (i0 == c.count[sp]) i0 = c.count[sp] - 1;
-      i = i0 + c.ifirst[sp];
-      double delta[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) delta[d] = sb(sec1, 0, i)[d] - sb(sec1, c.S - 1, i)[d];
-      pbc<D>(c, delta);
-      const double alpha = c.mass[sp] / (2 * c.tau * (1 << c.L));
-      double tii = 1;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const double len = c.a[d];
-        tii *= (alpha * len * len >= 20.0) ? pg_images<double>(alpha, len, delta[d], nullptr)
-                                           : pg_eval(pg_make(alpha, len), delta[d]);
+    const int nm = MM == 1 ? 1 : c.sample_nm[smp];
+    ChooseBuf& cb = *cbuf;
+    nmov_ = nm;
+    midx = cb.idx;
+    if (kind == PQ_CHOOSER_WALKING) return walk_choose(sp, nm, k, lnTP);
+    if (tid == 0) {
+      uint64_t kk = k;
+      for (int m = 0; m < nm; ++m) cb.cyc[m] = m;
+      if (kind == PQ_CHOOSER_ALL_RANDOM) {
+        for (int ifrom = 0; ifrom < nm;) {
+          int ito = (int)(nm * U(kk++));
+          if (ito == nm) ito = nm - 1;
+          for (int jf = 0; jf <= ifrom; ++jf) {
+            if (jf == ifrom) { cb.cyc[ifrom++] = ito; break; }
+            if (cb.cyc[jf] == ito) break;
+          }
+        }
       }
-      const double accept = tii > 0 ? 1.0 : __builtin_nan("");
-      return U(k++) < accept;
-    }
-    if (kind == PQ_CHOOSER_ALL || kind == PQ_CHOOSER_ALL_RANDOM) {
-      if (kind == PQ_CHOOSER_ALL_RANDOM) ++k;
-      for (;;) {
-        const int j = (int)(c.N * U(k++));
-        if (j != c.N) { i = j; return true; }
+      const bool all = kind == PQ_CHOOSER_ALL || kind == PQ_CHOOSER_ALL_RANDOM;
+      const int pool = all ? c.N : c.count[sp], first = all ? 0 : c.ifirst[sp];
+      for (int im = 0; im < nm;) {
+        int j;
+        if (all) {
+          j = (int)(pool * U(kk++));
+          if (j == pool) continue;
+        } else {
+          j = (int)(pool * U(kk++) + first);
+          if (j == pool + first) j = pool + first - 1;
+        }
+        for (int jm = 0; jm <= im; ++jm) {
+          if (jm == im) { cb.idx[im++] = j; break; }
+          if (cb.idx[jm] == j) break;
+        }
       }
+      cb.k = kk;
     }
-    int j = (int)(c.count[sp] * U(k++) + c.ifirst[sp]);
-    if (j == c.count[sp] + c.ifirst[sp]) j = c.count[sp] + c.ifirst[sp] - 1;
-    i = j;
+    __syncthreads();
+    k = cb.k;
+    lnTP = 0;
+    // the next Sample's chooser rewrites cbuf
+    __syncthreads();
     return true;
+  }
+
+  // free-particle propagator of the WalkingChooser between particle i's first
+  // and particle j's last section bead, prod_d PG_{m/(2 tau 2^L)}(pbc(delta)_d)
+  // (WalkingChooser.cc:89-103)
+  __device__ double walk_t(int sp, int i, int j) const {
+    double delta[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) delta[d] = sb(sec1, 0, i)[d] - sb(sec1, c.S - 1, j)[d];
+    pbc<D>(c, delta);
+    const double alpha = c.mass[sp] / (2 * c.tau * (1 << c.L));
+    double t = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double len = c.a[d];
+      t *= (alpha * len * len >= 20.0) ? pg_images<double>(alpha, len, delta[d], nullptr)
+                                       : pg_eval(pg_make(alpha, len), delta[d]);
+    }
+    return t;
+  }
+
+  // WalkingChooser::choosePermutation (WalkingChooser.cc:39-87) with init's
+  // tables (:89-118) built one row per walk step: row p holds t(idx[p], j) and
+  // its running sum cump, the team fills the row and one thread walks.  The
+  // next particle is iSearch(prev, h u) (:120-142) with h = cump(prev, npart-1);
+  // a repeated particle ends the move; the cycle m -> m+1 (mod nsize) is
+  // accepted with probability revTranProb/tranProb and lnTP = log prod_m
+  // t(i_m,i_m)/t(i_m,i_m+1).
+  __device__ bool walk_choose(int sp, int nsize, uint64_t& k, double& lnTP) {
+    ChooseBuf& cb = *cbuf;
+    const int npart = c.count[sp], first = c.ifirst[sp];
+    double tran = 0, rev = 0;   // thread 0's running sums
+    if (tid == 0) {
+      uint64_t kk = k;
+      int i0 = (int)(npart * U(kk++));
+      if (i0 == npart) i0 = npart - 1;
+      cb.idx[0] = i0;
+      for (int m = 0; m < nsize; ++m) cb.cyc[m] = (m + 1) % nsize;
+      cb.k = kk;
+      cb.ok = 1;
+    }
+    __syncthreads();
+    for (int ip = 0; ip < nsize; ++ip) {
+      const int r = cb.idx[ip];
+      double* trow = wrows + (int64_t)ip * 2 * npart;
+      for (int j = tid; j < npart; j += NT) trow[j] = walk_t(sp, first + r, first + j);
+      __syncthreads();
+      if (tid == 0) {
+        double* cump = trow + npart;
+        cump[0] = trow[0];
+        for (int j = 1; j < npart; ++j) cump[j] = cump[j - 1] + trow[j];
+        if (ip + 1 < nsize) {
+          uint64_t kk = cb.k;
+          const double h = cump[npart - 1];
+          const double xx = h * U(kk++);
+          int nx;
+          if (xx < cump[0]) nx = 0;
+          else if (cump[npart - 2] < xx) nx = npart - 1;
+          else {
+            int ilo = 0, ihi = npart - 2, mid = 0;
+            nx = -1;
+            for (int q = 0; q < npart - 1; q++) {
+              if (ihi == ilo + 1) { nx = ihi; break; }
+              mid = (ilo + ihi) / 2;
+              if (cump[mid] > xx) ihi = mid;
+              else ilo = mid;
+            }
+            if (nx < 0) nx = mid;
+          }
+          cb.k = kk;
+          bool dup = false;
+          for (int jp = 0; jp <= ip; ++jp) dup |= cb.idx[jp] == nx;
+          if (dup) {
+            cb.ok = 0;
+          } else {
+            cb.idx[ip + 1] = nx;
+            rev += h / trow[r];
+            tran += h / trow[nx];
+          }
+        }
+      }
+      __syncthreads();
+      if (!cb.ok) break;
+    }
+    if (cb.ok && tid == 0) {
+      const int last = cb.idx[nsize - 1];
+      const double* tl = wrows + (int64_t)(nsize - 1) * 2 * npart;
+      const double h = tl[2 * npart - 1];
+      rev += h / tl[last];
+      tran += h / tl[cb.idx[0]];
+      const double accept = rev / tran;
+      double prob = 1;
+      for (int m = 0; m < nsize; ++m) {
+        const double* tm = wrows + (int64_t)m * 2 * npart;
+        prob *= tm[cb.idx[m]] / tm[cb.idx[(m + 1) % nsize]];
+      }
+      cb.lnTP = log(prob);
+      cb.ok = U(cb.k) < accept;
+      cb.k += 1;
+      for (int m = 0; m < nsize; ++m) cb.idx[m] += first;
+    }
+    __syncthreads();
+    k = cb.k;
+    lnTP = cb.lnTP;
+    const bool ok = cb.ok;
+    __syncthreads();
+    return ok;
   }
 
   // ------------------------------------------------------------ mover
   // FreeMover::makeMove (FreeMover.cc:59-124) / FreeMoverPBC::makeMove
-  // (FreeMoverPBC.cc:59-162) for one moving particle; lane k draws and places
-  // the bead of slice s(2k+1).  Returns ln(T_old/T_new).
-  __device__ double mover(int level, int i, uint64_t k0, int* ndraws) {
+  // (FreeMoverPBC.cc:59-162); lane k places the bead of moving particle
+  // k % nmov at slice s(2(k / nmov) + 1).  Per slice the reference draws
+  // nmov D Gaussians by Box-Muller pairs over the flattened array
+  // (RandomNumGenerator.h:75-86), then, for FreeMoverPBC, D image uniforms per
+  // moving particle; each lane computes its own from the draw offsets.
+  // Returns ln(T_old/T_new).
+  __device__ double mover(int level, uint64_t k0, int* ndraws) {
     const int s = 1 << level;
     const int nmoved = (level < c.L) ? (1 << (c.L - 1 - level)) : 0;
     const bool pbcmv = c.mover == PQ_MOVER_FREE_PBC;
-    const int gd = 2 * ((D + 1) / 2);
-    const int pd = gd + (pbcmv ? D : 0);
+    const int ng = nmov() * D;
+    const int gd = 2 * ((ng + 1) / 2);
+    const int pd = gd + (pbcmv ? ng : 0);
     *ndraws = nmoved * pd;
-    const int sp = species_of(c, i);
     double lnT = 0;
-    for (int kk = tid; kk < nmoved; kk += NT) {
+    for (int it = tid; it < nmoved * nmov(); it += NT) {
+      int kk = it, m = 0;
+      if (nmov() > 1) { kk = it / nmov(); m = it - kk * nmov(); }
+      const int i = midx[m];
+      const int sp = species_of(c, i);
       const int t = s * (2 * kk + 1);
-      uint64_t kb = k0 + (uint64_t)kk * pd;
-      double g[4];
+      const uint64_t kb = k0 + (uint64_t)kk * pd;
+      double g[D];
+      if (nmov() == 1) {
+        double gg[4];
 #pragma unroll
-      for (int p = 0; p < (D + 1) / 2; ++p) {
-        double t1 = 1 - 0.9999999999 * U(kb + 2 * p);
-        double t2 = U(kb + 2 * p + 1);
-        double rr = sqrt(-2.0 * log(t1));
-        double sn, cs;
-        sincos(6.283185306 * t2, &sn, &cs);
-        g[2 * p] = rr * cs;
-        g[2 * p + 1] = rr * sn;
+        for (int p = 0; p < (D + 1) / 2; ++p) {
+          double t1 = 1 - 0.9999999999 * U(kb + 2 * p);
+          double t2 = U(kb + 2 * p + 1);
+          double rr = sqrt(-2.0 * log(t1));
+          double sn, cs;
+          sincos(6.283185306 * t2, &sn, &cs);
+          gg[2 * p] = rr * cs;
+          gg[2 * p + 1] = rr * sn;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = gg[d];
+      } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const int e = m * D + d, p = e >> 1;
+          double t1 = 1 - 0.9999999999 * U(kb + 2 * p);
+          double t2 = U(kb + 2 * p + 1);
+          double rr = sqrt(-2.0 * log(t1));
+          double sn, cs;
+          sincos(6.283185306 * t2, &sn, &cs);
+          g[d] = (e & 1) ? rr * sn : rr * cs;
+        }
       }
       const double lam = c.lambda[sp];
       const double sigma = 1.0 * sqrt(lam * c.tau * s);
       const double inv2Sigma2 = 0.5 / (sigma * sigma);
-      const double* mp = mov + (t - s) * D;
-      const double* mn = mov + (t + s) * D;
+      const double* mp = mov + ((t - s) * nmov() + m) * D;
+      const double* mn = mov + ((t + s) * nmov() + m) * D;
       const double* op = sb(sec1, t - s, i);
       const double* on = sb(sec1, t + s, i);
       const double* oc = sb(sec1, t, i);
-      double* mc = mov + t * D;
+      double* mc = mov + (t * nmov() + m) * D;
       if (!pbcmv) {
         double mid[D], delta[D];
 #pragma unroll
@@ -489,7 +651,7 @@ struct Eng {
             w1 = exp(-inv2Sigma2 * mid[d] * mid[d]);
             w2 = exp(-inv2Sigma2 * mid2[d] * mid2[d]);
           }
-          if (U(kb + gd + d) > w1 / (w1 + w2)) mid[d] = mid2[d];
+          if (U(kb + gd + m * D + d) > w1 / (w1 + w2)) mid[d] = mid2[d];
         }
         double nb[D];
 #pragma unroll
@@ -539,16 +701,21 @@ struct Eng {
   // ---------------------------------------------------- action layer
   // Spring (SpringAction.cc:56-98), SHO (SHOAction.cc:19-65) and Coulomb pair
   // (PairAction.cc:219-360, CoulombAction.cc:100-105) action differences of
-  // moving particle i at a level.  Lanes split the links and partner x link pairs.
-  __device__ double action_diff(int i, int level) {
+  // the moving particles at a level.  Lanes split the (link, moving particle)
+  // and (moving particle, partner, link) items; a pair of two moving particles
+  // counts once, from the higher index, with both new paths.
+  __device__ double action_diff(int level) {
     const int s = 1 << level;
     const int nl = (c.S - 1) >> level;
-    const int sp = species_of(c, i);
     Acc<R> acc;
-    for (int k = tid; k < nl; k += NT) {
+    for (int it = tid; it < nl * nmov(); it += NT) {
+      int k = it, m = 0;
+      if (nmov() > 1) { k = it / nmov(); m = it - k * nmov(); }
+      const int i = midx[m];
+      const int sp = species_of(c, i);
       const int t = s * (k + 1);
-      const double* m1 = mov + t * D;
-      const double* m0 = mov + (t - s) * D;
+      const double* m1 = mov + (t * nmov() + m) * D;
+      const double* m0 = mov + ((t - s) * nmov() + m) * D;
       const double* o1 = sb(sec1, t, i);
       const double* o0 = sb(sec1, t - s, i);
       if (c.spring) {
@@ -600,26 +767,40 @@ struct Eng {
     }
     for (int pp = 0; pp < c.npair; ++pp) {
       const PairP& P = c.pair[pp];
+      const TableP T = x.tables[P.table];
+      const R* u = tab ? tab + tabofs[pp] : table_ptr<R>(x, T);
+      const bool offdiag = level < 3 && T.K1 > 1;
+      for (int m = 0; m < nmov(); ++m) {
+      const int i = midx[m];
+      const int sp = species_of(c, i);
       int js;
       if (sp == P.s1) js = P.s2;
       else if (sp == P.s2) js = P.s1;
       else continue;
-      const TableP T = x.tables[P.table];
-      const R* u = tab ? tab + tabofs[pp] : table_ptr<R>(x, T);
-      const bool offdiag = level < 3 && T.K1 > 1;
       const int jb = c.ifirst[js], nj = c.count[js];
       const int items = nj * nl;
       for (int it = tid; it < items; it += NT) {
         const int j = jb + it / nl;
         if (j == i) continue;
+        int jm = -1;
+        for (int q = 0; q < nmov(); ++q)
+          if (midx[q] == j) jm = q;
+        if (jm >= 0 && i <= j) continue;
         const int t = s * (it % nl + 1);
         const double* pj1 = sb(sec1, t, j);
         const double* pj0 = sb(sec1, t - s, j);
         R vals[2];
 #pragma unroll
         for (int pass = 0; pass < 2; ++pass) {
-          const double* pi1 = pass ? sb(sec1, t, i) : mov + t * D;
-          const double* pi0 = pass ? sb(sec1, t - s, i) : mov + (t - s) * D;
+          const double* pi1 = pass ? sb(sec1, t, i) : mov + (t * nmov() + m) * D;
+          const double* pi0 = pass ? sb(sec1, t - s, i) : mov + ((t - s) * nmov() + m) * D;
+          if (pass == 0 && jm >= 0) {
+            pj1 = mov + (t * nmov() + jm) * D;
+            pj0 = mov + ((t - s) * nmov() + jm) * D;
+          } else if (pass == 1) {
+            pj1 = sb(sec1, t, j);
+            pj0 = sb(sec1, t - s, j);
+          }
           double d1[D], d0[D];
 #pragma unroll
           for (int d = 0; d < D; ++d) { d1[d] = pi1[d] - pj1[d]; d0[d] = pi0[d] - pj0[d]; }
@@ -640,6 +821,7 @@ struct Eng {
         acc.add(vals[0]);
         acc.add(-vals[1]);
       }
+      }
     }
     return bsum(acc.value());
   }
@@ -657,17 +839,19 @@ struct Eng {
     return b;
   }
 
-  // position of particle p in section 1 at slice t, with the moving particle
-  // replaced by the moving bead when use_mov
-  __device__ __forceinline__ const double* r1pos(int t, int p, bool use_mov, int i) const {
-    return (use_mov && p == i) ? mov + t * D : sb(sec1, t, p);
+  // position of particle p in section 1 at slice t, with the moving particles
+  // replaced by their moving beads when use_mov
+  __device__ __forceinline__ const double* r1pos(int t, int p, bool use_mov) const {
+    if (use_mov)
+      for (int m = 0; m < nmov(); ++m)
+        if (midx[m] == p) return mov + (t * nmov() + m) * D;
+    return sb(sec1, t, p);
   }
 
   // NodeModel::evaluate (SHONodes.cc:51-128, FreeParticleNodes.cc:72-157) on
   // slices t0, t0+step, ... (nt of them) with the given scale; inverses go to
   // the cache, determinants and error flags to wdet/werr[0..nt).
-  __device__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, int i,
-                                             double scl) {
+  __device__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, double scl) {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
     // fill: every thread takes matrix elements; they are staged in the inverse
@@ -678,7 +862,7 @@ struct Eng {
       const int ii = rem / n, jj = rem - ii * n;
       const int t = t0 + m * step;
       const double* ri = sb(sec2, t, f.ifirst + ii);
-      const double* rj = r1pos(t, f.ifirst + jj, use_mov, i);
+      const double* rj = r1pos(t, f.ifirst + jj, use_mov);
       double gf[D];
       const int64_t e = ((int64_t)t * ins + ii) * ins + jj;
       inv[e] = elem(f, rj, ri, scl, gf);
@@ -813,7 +997,7 @@ struct Eng {
   // evaluateDistance (SHONodes.cc:130-170, FreeParticleNodes.cc:184-234) for
   // slices t0..t0+nt-1 into dist[t][2][ns]; slice t uses scale scl[t] when
   // scl is given, else scl0.
-  __device__ void node_distances(int fa, int t0, int nt, bool use_mov, int i, const double* scl,
+  __device__ void node_distances(int fa, int t0, int nt, bool use_mov, const double* scl,
                                  double scl0, double* dist) {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
@@ -832,7 +1016,7 @@ struct Eng {
         const double w = iv[jp * ins + ip];
         const double* gf = egr + (((int64_t)t * ins + ip) * ins + jp) * D;
         if (f.model == PQ_NODES_SHO) {
-          const double* rj = r1pos(t, f.ifirst + jp, use_mov, i);
+          const double* rj = r1pos(t, f.ifirst + jp, use_mov);
           const double* ri = sb(sec2, t, f.ifirst + ip);
           const double e = gf[0];
 #pragma unroll
@@ -967,14 +1151,14 @@ struct Eng {
       const FnP& f = c.fn[fa];
       double* dm = fn_dm(fa);
       double* eflag = fn_newdm(fa);   // per-slice error flags until the end
-      if (!(fcache && fn_init_cached(fa, nerr))) {
+      if (!(fcache && rec_on && fn_init_cached(fa, nerr))) {
         double scl = scale[fa];
         int t = 0, passes = 0;
         int errs = 0;
         double dm0 = 0;
         while (t < c.S) {
           const int nt = min(NG, c.S - t);
-          node_matrices(fa, t, 1, nt, false, -1, scl);
+          node_matrices(fa, t, 1, nt, false, scl);
           int bad = -1;
           for (int m = 0; m < nt; ++m) {
             double dt = wdet[m];
@@ -1014,8 +1198,8 @@ struct Eng {
           if (lasterr) lasterr[fa] = eflag[c.S - 1] != 0.0;
         }
         __syncthreads();
-        if (f.nodal) node_distances(fa, 0, c.S, false, -1, wscale, 0.0, fn_dist(fa));
-        if (fcache) fn_store_records(fa, 0, c.S, dm, wscale, 0.0, eflag, fn_dist(fa));
+        if (f.nodal) node_distances(fa, 0, c.S, false, wscale, 0.0, fn_dist(fa));
+        if (fcache && rec_on) fn_store_records(fa, 0, c.S, dm, wscale, 0.0, eflag, fn_dist(fa));
         __syncthreads();
       }
       // newDMValue(0)=dmValue(0); newDist(0)=dist(0)
@@ -1030,12 +1214,13 @@ struct Eng {
   }
 
   // CompositeDoubleAction::getActionDifference over FixedNodeAction::
-  // getActionDifference (FixedNodeAction.cc:39-131) for moving particle i.
-  __device__ double fn_diff(int i, int level) {
+  // getActionDifference (FixedNodeAction.cc:39-131) for the moving particles.
+  __device__ double fn_diff(int level) {
     double total = 0;
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
-      const bool mine = i >= f.ifirst && i < f.ifirst + f.ns;
+      bool mine = false;
+      for (int m = 0; m < nmov(); ++m) mine |= midx[m] >= f.ifirst && midx[m] < f.ifirst + f.ns;
       if (tid == 0) notmine[fa] = mine ? 0 : 1;
       if (!mine) continue;
       const int nStride = 1 << (level + 1);
@@ -1051,7 +1236,7 @@ struct Eng {
         crossed = lasterr[fa] || dl * dm0 <= 1e-200;
         if (tid == 0) ndm[c.S - 1] = dl;
       } else {
-        node_matrices(fa, t0, nStride, nt, true, i, scale[fa]);
+        node_matrices(fa, t0, nStride, nt, true, scale[fa]);
         for (int m = 0; m < nt; ++m) {
           if (werr[m] || wdet[m] * dm0 <= 1e-200) { crossed = 1; break; }
           if (tid == 0) ndm[t0 + m * nStride] = wdet[m];
@@ -1067,9 +1252,9 @@ struct Eng {
         if (top_known) {
           // the last slice keeps its distances
           for (int k = tid; k < 2 * n; k += NT) nd[(int64_t)(c.S - 1) * 2 * n + k] = od[(int64_t)(c.S - 1) * 2 * n + k];
-          node_distances(fa, 1, c.S - 2, true, i, nullptr, scale[fa], nd);
+          node_distances(fa, 1, c.S - 2, true, nullptr, scale[fa], nd);
         } else {
-          node_distances(fa, 1, c.S - 1, true, i, nullptr, scale[fa], nd);
+          node_distances(fa, 1, c.S - 1, true, nullptr, scale[fa], nd);
         }
         double part = 0;
         for (int it = tid; it < (c.S - 1) * n; it += NT) {
@@ -1089,10 +1274,13 @@ struct Eng {
   }
 
   // FixedNodeAction::acceptLastMove (FixedNodeAction.cc:359-367); in the
-  // sampler the moved slices' records take the accepted values
-  __device__ void fn_accept() {
+  // sampler the moved slices' records take the accepted values.  A move that
+  // permuted the last slice evaluated it without error, so the error flag
+  // initialize gave that slice no longer applies.
+  __device__ void fn_accept(bool permuted = false) {
     for (int fa = 0; fa < c.nfn; ++fa) {
       if (notmine[fa]) continue;
+      if (permuted && lasterr && tid == 0) lasterr[fa] = 0;
       const int n = c.fn[fa].ns;
       double* dm = fn_dm(fa);
       const double* ndm = fn_newdm(fa);
@@ -1100,7 +1288,7 @@ struct Eng {
       const double* nd = fn_newdist(fa);
       for (int k = tid; k < c.S; k += NT) dm[k] = ndm[k];
       for (int k = tid; k < 2 * c.S * n; k += NT) dd[k] = nd[k];
-      if (fcache) fn_store_records(fa, 1, c.S - 1, ndm, nullptr, scale[fa], nullptr, nd);
+      if (fcache && rec_on) fn_store_records(fa, 1, c.S - 1, ndm, nullptr, scale[fa], nullptr, nd);
     }
     __syncthreads();
   }

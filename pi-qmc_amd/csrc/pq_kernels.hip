@@ -21,8 +21,9 @@ struct CallBufs {
   const int* qchain;
   const int* qlevel;
   const double* qsec;      // [nq][S][N][D] (padded to the largest S*N*D)
-  const double* qmov;      // [nq][S][D]
-  const int* qidx;
+  const double* qmov;      // [nq][S][nmov][D]
+  const int* qidx;         // [nq][nmov]
+  int nmov;
   int64_t sec_stride;
   int64_t mov_stride;
   double* out;
@@ -37,7 +38,9 @@ __global__ __launch_bounds__(PQ_WAVE) void action_diff_kernel(DevCtx x, CallBufs
   Eng<D, NC, R, PQ_WAVE> e(x, c, chain);
   e.sec1 = const_cast<double*>(b.qsec + q * b.sec_stride);
   e.mov = const_cast<double*>(b.qmov + q * b.mov_stride);
-  double v = e.action_diff(b.qidx[q], b.qlevel[q]);
+  e.nmov_ = b.nmov;
+  e.midx = b.qidx + (int64_t)q * b.nmov;
+  double v = e.action_diff(b.qlevel[q]);
   if (e.lane == 0) b.out[q] = v;
 }
 
@@ -53,7 +56,8 @@ struct FnBufs {
   int* notmine;
   int op;
   int level;
-  int idx;
+  const int* idx;          // [nmov] moving particles
+  int nmov;
   double* out;
 };
 
@@ -66,6 +70,7 @@ __global__ __launch_bounds__(PQ_WAVE) void fn_kernel(DevCtx x, int chain, FnBufs
   Eng<D, NC, R, PQ_WAVE> e(x, c, chain);
   e.gjrow = gj;
   e.sec1 = b.sec1; e.sec2 = b.sec2; e.mov = b.mov;
+  e.nmov_ = b.nmov; e.midx = b.idx;
   e.fst = x.fnstate + c.fn_off;
   e.inv = x.inv + c.inv_off;
   e.egr = x.egr + c.egr_off;
@@ -77,7 +82,7 @@ __global__ __launch_bounds__(PQ_WAVE) void fn_kernel(DevCtx x, int chain, FnBufs
     int nerr = e.fn_initialize();
     if (e.lane == 0) b.out[0] = (double)nerr;
   } else if (b.op == 1) {
-    double v = e.fn_diff(b.idx, b.level);
+    double v = e.fn_diff(b.level);
     if (e.lane == 0) b.out[0] = v;
   } else {
     e.fn_accept();
@@ -108,22 +113,25 @@ __global__ __launch_bounds__(PQ_WAVE) void node_eval_kernel(DevCtx x, int chain,
   constexpr int NG = PQ_WAVE / gj_group(NC);
   for (int t = 0; t < nmat; t += NG) {
     const int nt = min(NG, nmat - t);
-    e.node_matrices(fa, t, 1, nt, false, -1, scl);
+    e.node_matrices(fa, t, 1, nt, false, scl);
     if (e.lane < nt) { det[t + e.lane] = wd[e.lane]; err[t + e.lane] = we[e.lane]; }
     __syncthreads();
   }
-  e.node_distances(fa, 0, nmat, false, -1, nullptr, scl, dist);
+  e.node_distances(fa, 0, nmat, false, nullptr, scl, dist);
 }
 
 // ------------------------------------------------------------ estimators
 
 // getBeadAction utau of spring, SHO and pair actions for bead (i,t) of a
 // chain, summed as CompositeAction::getBeadAction (CompositeAction.cc:57-67).
+// Slice -1 is slice M-1 of particle inversePermutation[i] (SerialPaths.cc:68-77).
 template <int D>
 __device__ double bead_utau(const DevCtx& x, const ChainP& c, const double* beads, int i, int t) {
+  const int* perm = x.perm + c.perm_off;
+  const int* invperm = perm + c.N;
   auto P = [&](int p, int tt) -> const double* {
-    tt %= c.M;
-    if (tt < 0) tt += c.M;
+    if (tt < 0) { tt += c.M; p = invperm[p]; }
+    else if (tt >= c.M) { tt -= c.M; p = perm[p]; }
     return beads + ((int64_t)tt * c.N + p) * D;
   };
   const int sp = species_of(c, i);
@@ -318,20 +326,21 @@ __global__ __launch_bounds__(256) void histogram_kernel(DevCtx x, int s1, int s2
     PQ_DISPATCH_R(FN, 3, double, __VA_ARGS__);                                        \
   } while (0)
 
-hipError_t launch_sweep_2f(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
-hipError_t launch_sweep_2d(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
-hipError_t launch_sweep_3f(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
-hipError_t launch_sweep_3d(int, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+hipError_t launch_sweep_2f(int, bool, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+hipError_t launch_sweep_2d(int, bool, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+hipError_t launch_sweep_3f(int, bool, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
+hipError_t launch_sweep_3d(int, bool, const DevCtx&, int, int, const SweepLayoutH&, hipStream_t);
 
-hipError_t launch_sweep(int D, int NC, bool mixed, const DevCtx& x, int nchain, int nsweep,
+hipError_t launch_sweep(int D, int NC, bool mixed, bool multi, const DevCtx& x, int nchain, int nsweep,
                         const SweepLayoutH& L, hipStream_t st) {
-  if (D == 2) return mixed ? launch_sweep_2f(NC, x, nchain, nsweep, L, st) : launch_sweep_2d(NC, x, nchain, nsweep, L, st);
-  return mixed ? launch_sweep_3f(NC, x, nchain, nsweep, L, st) : launch_sweep_3d(NC, x, nchain, nsweep, L, st);
+  if (D == 2)
+    return mixed ? launch_sweep_2f(NC, multi, x, nchain, nsweep, L, st) : launch_sweep_2d(NC, multi, x, nchain, nsweep, L, st);
+  return mixed ? launch_sweep_3f(NC, multi, x, nchain, nsweep, L, st) : launch_sweep_3d(NC, multi, x, nchain, nsweep, L, st);
 }
 
 template <int D, int NC, class R>
 static hipError_t launch_action_diff_t(const DevCtx& x, int nq, const CallBufsH& b, hipStream_t st) {
-  CallBufs cb{b.qchain, b.qlevel, b.qsec, b.qmov, b.qidx, b.sec_stride, b.mov_stride, b.out};
+  CallBufs cb{b.qchain, b.qlevel, b.qsec, b.qmov, b.qidx, b.nmov, b.sec_stride, b.mov_stride, b.out};
   hipLaunchKernelGGL((action_diff_kernel<D, NC, R>), dim3(nq), dim3(PQ_WAVE), 0, st, x, cb);
   return hipGetLastError();
 }
@@ -347,7 +356,7 @@ hipError_t launch_action_diff(int D, int NC, bool mixed, const DevCtx& x, int nq
 
 template <int D, int NC, class R>
 static hipError_t launch_fn_t(const DevCtx& x, int chain, const FnBufsH& b, hipStream_t st) {
-  FnBufs fb{b.sec1, b.sec2, b.mov, b.wdet, b.werr, b.wscale, b.notmine, b.op, b.level, b.idx, b.out};
+  FnBufs fb{b.sec1, b.sec2, b.mov, b.wdet, b.werr, b.wscale, b.notmine, b.op, b.level, b.idx, b.nmov, b.out};
   hipLaunchKernelGGL((fn_kernel<D, NC, R>), dim3(1), dim3(PQ_WAVE), 0, st, x, chain, fb);
   return hipGetLastError();
 }

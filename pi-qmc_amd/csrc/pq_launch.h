@@ -11,8 +11,11 @@ struct SweepLayoutH {
   int off_gj;                      // Gauss-Jordan pivot rows, 2 doubles per thread
   int off_tab;                     // pair tables (u) staged in LDS, < 0: read from global memory
   int off_tabofs;                  // [PQ_MAX_PAIRS] LDS offset of each pair's table
+  int off_choose;                  // chooser results (Eng::ChooseBuf)
+  int off_perm;                    // path permutation, its inverse, section permutation, scratch: [5][N] ints
+  int off_walk;                    // WalkingChooser rows, < 0: no walking Sample
   int bytes;
-  int team;                        // threads per chain: 64 or 256
+  int team;                        // threads per chain: 64 to 512
 };
 
 struct CallBufsH {
@@ -21,6 +24,7 @@ struct CallBufsH {
   const double* qsec;
   const double* qmov;
   const int* qidx;
+  int nmov;
   int64_t sec_stride;
   int64_t mov_stride;
   double* out;
@@ -36,11 +40,12 @@ struct FnBufsH {
   int* notmine;
   int op;
   int level;
-  int idx;
+  const int* idx;
+  int nmov;
   double* out;
 };
 
-hipError_t launch_sweep(int D, int NC, bool mixed, const DevCtx& x, int nchain, int nsweep,
+hipError_t launch_sweep(int D, int NC, bool mixed, bool multi, const DevCtx& x, int nchain, int nsweep,
                         const SweepLayoutH& L, hipStream_t st);
 hipError_t launch_action_diff(int D, int NC, bool mixed, const DevCtx& x, int nq, const CallBufsH& b,
                               hipStream_t st);

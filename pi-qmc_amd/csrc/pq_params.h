@@ -52,6 +52,7 @@ struct ChainP {
   int nsample;
   int sample_sp[8];
   int sample_ch[8];      // PQ_CHOOSER_* of each Sample
+  int sample_nm[8];      // moving particles of each Sample
   int nsection;
   double spring_c[PQ_PG_LEVELS][4];          // 1/(4 lambda tau 2^l) per level, species
   double sho_cosh[4][PQ_PG_LEVELS];          // cosh(omega tau 2^l) per SHO action
@@ -64,6 +65,7 @@ struct ChainP {
   int64_t fn_off;        // element offset of this chain's fixed-node state (global copy)
   int64_t egr_off;       // element offset of this chain's [S][ns_max][ns_max][D] element-gradient cache
   int64_t fnc_off;       // element offset of this chain's slice-pair node records, -1: none
+  int64_t perm_off;      // element offset of this chain's path permutation and its inverse ([2][N])
   int ns_max;            // largest fermion species with a node model
 };
 

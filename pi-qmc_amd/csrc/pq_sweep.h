@@ -7,25 +7,76 @@
 
 namespace pq {
 
-// LDS carve-up of the sampler workgroup
-struct SweepLayout {
-  int off_sec1, off_sec2, off_mov, off_fst, off_wdet, off_werr, off_wscale, off_scale, off_notmine;
-  int off_red, off_cnt, off_inv;   // off_inv < 0: inverse cache in global memory
-  int off_gj;
-  int off_tab, off_tabofs;         // off_tab < 0: pair tables read from global memory
-  int bytes;
-};
+// the LDS carve-up of the sampler workgroup is SweepLayoutH (pq_launch.h),
+// passed to the kernel as it is
+
+// SerialPaths::putBeads (SerialPaths.cc:144-185) of section slices k0..k1-1:
+// slices before 0 are skipped, slices from M on go to particle perm[i].
+template <int D, int NT>
+__device__ void put_slices(double* beads, const double* sec, const ChainP& c, int t0, int k0, int k1,
+                           const int* perm) {
+  const int slice = c.N * D;
+  for (int q = k0 * slice + (int)threadIdx.x; q < k1 * slice; q += NT) {
+    const int kk = q / slice;
+    const int rem = q - kk * slice;
+    const int t = kk + t0;
+    if (t < 0) continue;
+    if (t < c.M) {
+      beads[(int64_t)t * slice + rem] = sec[q];
+    } else {
+      const int i = rem / D, d = rem - i * D;
+      beads[(int64_t)(t - c.M) * slice + perm[i] * D + d] = sec[q];
+    }
+  }
+}
+
+// the rest of putBeads for a non-identity section permutation sp: slices
+// ifollowing..M-1 are relabelled, bead(i,t) <- bead(temp[i],t), with temp = sp,
+// or perm o sp o perm^-1 when the section ran past M (Permutation::prepend /
+// append, Permutation.cc:23-43), staged through buf (S slices at a time); then
+// perm <- perm o temp and its inverse.  tmp and tmp2 are N-int scratch.
+template <int D, int NT>
+__device__ void relabel_following(double* beads, double* buf, const ChainP& c, int t0, const int* sp,
+                                  int* perm, int* invperm, int* tmp, int* tmp2) {
+  const int N = c.N, M = c.M, S = c.S;
+  const int slice = N * D;
+  int ifol = t0 + S;
+  const bool wrapped = ifol > M;
+  if (wrapped) ifol -= M;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < N; ++i) tmp[i] = wrapped ? perm[sp[invperm[i]]] : sp[i];
+  }
+  __syncthreads();
+  for (int tc = ifol; tc < M; tc += S) {
+    const int nsl = min(S, M - tc);
+    for (int q = threadIdx.x; q < nsl * slice; q += NT) {
+      const int tt = q / slice;
+      const int rem = q - tt * slice;
+      const int i = rem / D, d = rem - i * D;
+      buf[q] = beads[(int64_t)(tc + tt) * slice + tmp[i] * D + d];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nsl * slice; q += NT) beads[(int64_t)tc * slice + q] = buf[q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < N; ++i) tmp2[i] = perm[tmp[i]];
+    for (int i = 0; i < N; ++i) { perm[i] = tmp2[i]; invperm[tmp2[i]] = i; }
+  }
+  __syncthreads();
+}
 
 // two waves per SIMD whatever the team: up to 256 threads two workgroups share
-// a CU, a 512-thread team has one to itself
-template <int D, int NC, class R, int NT>
-__global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, int nsweep, SweepLayout lay) {
+// a CU, a 512-thread team has one to itself.  MM bounds the moving particles
+// per Sample: MM = 1 never permutes and carries no relabelling code.
+template <int D, int NC, class R, int NT, int MM>
+__global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, int nsweep, SweepLayoutH lay) {
   extern __shared__ double smem[];
   const int chain = x.order[blockIdx.x];
   ChainP& c = *reinterpret_cast<ChainP*>(smem);
   copy_chain(&c, &x.chains[chain]);
   char* base = reinterpret_cast<char*>(smem);
-  Eng<D, NC, R, NT> e(x, c, chain);
+  Eng<D, NC, R, NT, MM> e(x, c, chain);
   e.sec1 = reinterpret_cast<double*>(base + lay.off_sec1);
   e.sec2 = reinterpret_cast<double*>(base + lay.off_sec2);
   e.mov = reinterpret_cast<double*>(base + lay.off_mov);
@@ -42,6 +93,16 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
   if (c.fnc_off >= 0) e.fcache = x.fncache + c.fnc_off;
   e.top_known = true;
   e.lasterr = e.notmine + PQ_MAX_NODES;
+  e.cbuf = reinterpret_cast<typename Eng<D, NC, R, NT, MM>::ChooseBuf*>(base + lay.off_choose);
+  if (lay.off_walk >= 0) e.wrows = reinterpret_cast<double*>(base + lay.off_walk);
+  // path permutation and inverse, section permutation, scratch
+  int* perm = reinterpret_cast<int*>(base + lay.off_perm);
+  int* invperm = perm + c.N;
+  int* secperm = invperm + c.N;
+  int* ptmp = secperm + c.N;
+  int* gperm = x.perm + c.perm_off;
+  for (int q = threadIdx.x; q < 2 * c.N; q += NT) perm[q] = gperm[q];
+  bool perm_changed = false;
   // per-level try/accept counters, bead updates, accepted, node errors
   unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base + lay.off_cnt);
   const int tid = e.tid;
@@ -98,48 +159,67 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
         t0 = ilo + (int)((ihi + 1 - ilo) * xx);
         if (t0 > ihi) t0 = ihi;
       }
-      // SerialPaths::getBeads (SerialPaths.cc:119-132), identity permutation
-      for (int64_t q = tid; q < S * slice; q += NT) {
-        int kk = (int)(q / slice);
-        int64_t rem = q - kk * slice;
-        int t = kk + t0;
-        t = (t < 0) ? t + M : ((t >= M) ? t - M : t);
-        e.sec1[q] = beads[t * slice + rem];
+      // SerialPaths::getBeads (SerialPaths.cc:119-132): slices before 0 through
+      // the inverse permutation, slices from M on through the permutation
+      for (int q = tid; q < S * (int)slice; q += NT) {
+        const int kk = q / (int)slice;
+        const int rem = q - kk * (int)slice;
+        const int i = rem / D, d = rem - i * D;
+        int t = kk + t0, src = i;
+        if (t < 0) { t += M; src = invperm[i]; }
+        else if (t >= M) { t -= M; src = perm[i]; }
+        e.sec1[q] = beads[t * slice + src * D + d];
         if (c.nfn) {
-          int tb = kk + t0b;
-          tb = (tb < 0) ? tb + M : ((tb >= M) ? tb - M : tb);
-          e.sec2[q] = beads[tb * slice + rem];
+          int tb = kk + t0b, srcb = i;
+          if (tb < 0) { tb += M; srcb = invperm[i]; }
+          else if (tb >= M) { tb -= M; srcb = perm[i]; }
+          e.sec2[q] = beads[tb * slice + srcb * D + d];
         }
       }
+      for (int q = tid; q < N; q += NT) secperm[q] = q;
       __syncthreads();
+      bool sec_permuted = false;
       if (c.nfn) {
         e.sec_t0 = t0;
+        bool pid = true;
+        for (int q = 0; q < N; ++q) pid &= perm[q] == q;
+        e.rec_on = pid || (t0 >= 0 && t0 + S <= M && t0b >= 0 && t0b + S <= M);
         const int ne = e.fn_initialize();
         if (tid == 0) cnt[18] += ne;
       }
 
       for (int smp = 0; smp < c.nsample; ++smp) {
-        int i;
-        if (!e.choose(smp, k, i)) continue;
-        if (tid < D) {
-          e.mov[tid] = e.sb(e.sec1, 0, i)[tid];
-          e.mov[(S - 1) * D + tid] = e.sb(e.sec1, S - 1, i)[tid];
+        double lnTP;
+        if (!e.choose(smp, k, lnTP)) continue;
+        const int nmov = e.nmov();
+        const int* midx = e.midx;
+        const int* mcyc = e.cbuf->cyc;
+        // the moving beads start on the chosen particles' first beads and end
+        // on the permuted particles' last beads (MultiLevelSampler.cc:74-87)
+        bool permuted = false;
+        if constexpr (MM > 1)
+          for (int m = 0; m < nmov; ++m) permuted |= mcyc[m] != m;
+        e.top_known = !permuted;
+        for (int q = tid; q < nmov * D; q += NT) {
+          const int m = q / D, d = q - m * D;
+          e.mov[m * D + d] = e.sb(e.sec1, 0, midx[m])[d];
+          e.mov[((S - 1) * nmov + m) * D + d] = e.sb(e.sec1, S - 1, midx[mcyc[m]])[d];
         }
         __syncthreads();
         // MultiLevelSampler::tryMove (MultiLevelSampler.cc:100-177)
-        double old = 0.0;
+        double old = lnTP;
         bool accepted = true;
         for (int l = L; l >= 0; --l) {
           if (tid == 0) {
             cnt[l]++;
-            if (l < L) cnt[16] += 1ull << (L - 1 - l);
+            if (l < L) cnt[16] += (unsigned long long)nmov << (L - 1 - l);
           }
           int nd;
-          const double lnT = e.mover(l, i, k, &nd);
+          const double lnT = e.mover(l, k, &nd);
           k += nd;
           __syncthreads();
-          double dA = e.action_diff(i, l);
-          if (c.nfn) dA += e.fn_diff(i, l);
+          double dA = e.action_diff(l);
+          if (c.nfn) dA += e.fn_diff(l);
           const double piRatio = -dA + old;
           const double acceptProb = exp(lnT + piRatio);
           if (e.U(k++) > acceptProb) { accepted = false; break; }
@@ -147,28 +227,65 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
           if (tid == 0) cnt[8 + l]++;
         }
         if (accepted) {
-          if (c.nfn) e.fn_accept();
-          for (int q = tid; q < S * D; q += NT) {
-            const int t = q / D, d = q - t * D;
-            e.sb(e.sec1, t, i)[d] = e.mov[q];
+          if (c.nfn) e.fn_accept(permuted);
+          for (int q = tid; q < S * nmov * D; q += NT) {
+            const int t = q / (nmov * D), rem = q - t * nmov * D;
+            const int m = rem / D, d = rem - m * D;
+            e.sb(e.sec1, t, midx[m])[d] = e.mov[q];
           }
-          if (tid == 0) cnt[17] += (1ull << L) - 1;
+          if (tid == 0) {
+            cnt[17] += (unsigned long long)nmov * ((1ull << L) - 1);
+            // append the permutation to the section permutation (MultiLevelSampler.cc:168-176)
+            if (MM > 1 && permuted) {
+              for (int m = 0; m < nmov; ++m) ptmp[m] = secperm[midx[m]];
+              for (int m = 0; m < nmov; ++m) secperm[midx[m]] = ptmp[mcyc[m]];
+            }
+          }
+          sec_permuted |= permuted;
           __syncthreads();
         }
       }
-      // SerialPaths::putBeads (SerialPaths.cc:144-165): interior slices only,
-      // the end slices are never moved
-      for (int64_t q = slice + tid; q < (S - 1) * slice; q += NT) {
-        int kk = (int)(q / slice);
-        int64_t rem = q - kk * slice;
-        int t = kk + t0;
-        t = (t >= M) ? t - M : t;
-        beads[t * slice + rem] = e.sec1[q];
-      }
+      // SerialPaths::putBeads (SerialPaths.cc:144-185) / putDoubleBeads (:187-206).
+      // Slice 0 is never moved; the last slice only under a permutation.
+      bool ident = true;
+      if (MM > 1 && sec_permuted)
+        for (int q = 0; q < N; ++q) ident &= secperm[q] == q;
+      put_slices<D, NT>(beads, e.sec1, c, t0, 1, ident ? S - 1 : S, perm);
       __syncthreads();
+      if constexpr (MM > 1) if (!ident) {
+        relabel_following<D, NT>(beads, e.sec1, c, t0, secperm, perm, invperm, ptmp, ptmp + N);
+        if (c.nfn && t0 < M / 2 - 1) {
+          // section 2 comes later in the path: its beads take the section
+          // permutation (Beads::permute, Beads.h:254-261) and go back with
+          // the identity permutation (p1 p2 p1^-1 with p2 the identity)
+          for (int q = tid; q < S * (int)slice; q += NT) {
+            const int kk = q / (int)slice;
+            const int rem = q - kk * (int)slice;
+            const int i = rem / D, d = rem - i * D;
+            e.sec1[q] = e.sec2[kk * (int)slice + secperm[i] * D + d];
+          }
+          __syncthreads();
+          put_slices<D, NT>(beads, e.sec1, c, t0b, 0, S, perm);
+          __syncthreads();
+        }
+        // the slices after the section now carry other labels: every node
+        // record of the chain is stale
+        if (e.fcache) {
+          double* rec = e.fcache;
+          for (int fa = 0; fa < c.nfn; ++fa) {
+            const int rd = fn_record_doubles(c.fn[fa].ns);
+            for (int q = tid; q < M / 2; q += NT) rec[(int64_t)q * rd + 2] = 0.0;
+            rec += (int64_t)(M / 2) * rd;
+          }
+        }
+        perm_changed = true;
+        __syncthreads();
+      }
     }
   }
   __syncthreads();
+  if (MM > 1 && perm_changed)
+    for (int q = threadIdx.x; q < 2 * c.N; q += NT) gperm[q] = perm[q];
   if (tid == 0) {
     x.counter[chain] = k;
     pq_chain_stats& st = x.stats[chain];
@@ -181,45 +298,44 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
   if (tid < c.nfn) x.scale[(int64_t)chain * PQ_MAX_NODES + tid] = e.scale[tid];
 }
 
-template <int D, int NC, class R, int NT>
+template <int D, int NC, class R, int NT, int MM>
 static hipError_t launch_sweep_nt(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
                                   hipStream_t st) {
-  SweepLayout lay;
-  lay.off_sec1 = L.off_sec1; lay.off_sec2 = L.off_sec2; lay.off_mov = L.off_mov;
-  lay.off_fst = L.off_fst; lay.off_wdet = L.off_wdet; lay.off_werr = L.off_werr;
-  lay.off_wscale = L.off_wscale; lay.off_scale = L.off_scale; lay.off_notmine = L.off_notmine;
-  lay.off_red = L.off_red; lay.off_cnt = L.off_cnt; lay.off_inv = L.off_inv;
-  lay.off_gj = L.off_gj;
-  lay.off_tab = L.off_tab;
-  lay.off_tabofs = L.off_tabofs;
-  lay.bytes = L.bytes;
-  auto k = sweep_kernel<D, NC, R, NT>;
+  auto k = sweep_kernel<D, NC, R, NT, MM>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(nchain), dim3(NT), L.bytes, st, x, nsweep, lay);
+  hipLaunchKernelGGL(k, dim3(nchain), dim3(NT), L.bytes, st, x, nsweep, L);
   return hipGetLastError();
 }
 
+// batches whose Samples all move one particle take the MM = 1 instances at
+// every team size; batches with cycles of several particles take the general
+// instances, built for one- and two-wave teams
 template <int D, int NC, class R>
-static hipError_t launch_sweep_t(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
+static hipError_t launch_sweep_t(bool multi, const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
                                  hipStream_t st) {
-  if (L.team == 512) return launch_sweep_nt<D, NC, R, 512>(x, nchain, nsweep, L, st);
-  if (L.team == 256) return launch_sweep_nt<D, NC, R, 256>(x, nchain, nsweep, L, st);
-  if (L.team == 128) return launch_sweep_nt<D, NC, R, 128>(x, nchain, nsweep, L, st);
-  return launch_sweep_nt<D, NC, R, PQ_WAVE>(x, nchain, nsweep, L, st);
+  if (multi) {
+    if (L.team == 128) return launch_sweep_nt<D, NC, R, 128, PQ_MAX_MOVING>(x, nchain, nsweep, L, st);
+    if (L.team == PQ_WAVE) return launch_sweep_nt<D, NC, R, PQ_WAVE, PQ_MAX_MOVING>(x, nchain, nsweep, L, st);
+    return hipErrorInvalidValue;
+  }
+  if (L.team == 512) return launch_sweep_nt<D, NC, R, 512, 1>(x, nchain, nsweep, L, st);
+  if (L.team == 256) return launch_sweep_nt<D, NC, R, 256, 1>(x, nchain, nsweep, L, st);
+  if (L.team == 128) return launch_sweep_nt<D, NC, R, 128, 1>(x, nchain, nsweep, L, st);
+  return launch_sweep_nt<D, NC, R, PQ_WAVE, 1>(x, nchain, nsweep, L, st);
 }
 
 // one translation unit per (D, R) instantiates every node-matrix width
 template <int D, class R>
-static hipError_t launch_sweep_dr(int NC, const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
-                                  hipStream_t st) {
+static hipError_t launch_sweep_dr(int NC, bool multi, const DevCtx& x, int nchain, int nsweep,
+                                  const SweepLayoutH& L, hipStream_t st) {
   switch (NC) {
-    case 4: return launch_sweep_t<D, 4, R>(x, nchain, nsweep, L, st);
-    case 8: return launch_sweep_t<D, 8, R>(x, nchain, nsweep, L, st);
-    case 10: return launch_sweep_t<D, 10, R>(x, nchain, nsweep, L, st);
-    case 12: return launch_sweep_t<D, 12, R>(x, nchain, nsweep, L, st);
-    case 16: return launch_sweep_t<D, 16, R>(x, nchain, nsweep, L, st);
-    default: return launch_sweep_t<D, 32, R>(x, nchain, nsweep, L, st);
+    case 4: return launch_sweep_t<D, 4, R>(multi, x, nchain, nsweep, L, st);
+    case 8: return launch_sweep_t<D, 8, R>(multi, x, nchain, nsweep, L, st);
+    case 10: return launch_sweep_t<D, 10, R>(multi, x, nchain, nsweep, L, st);
+    case 12: return launch_sweep_t<D, 12, R>(multi, x, nchain, nsweep, L, st);
+    case 16: return launch_sweep_t<D, 16, R>(multi, x, nchain, nsweep, L, st);
+    default: return launch_sweep_t<D, 32, R>(multi, x, nchain, nsweep, L, st);
   }
 }
 

@@ -2,8 +2,8 @@
 #include "pq_sweep.h"
 
 namespace pq {
-hipError_t launch_sweep_2d(int NC, const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
-                           hipStream_t st) {
-  return launch_sweep_dr<2, double>(NC, x, nchain, nsweep, L, st);
+hipError_t launch_sweep_2d(int NC, bool multi, const DevCtx& x, int nchain, int nsweep,
+                           const SweepLayoutH& L, hipStream_t st) {
+  return launch_sweep_dr<2, double>(NC, multi, x, nchain, nsweep, L, st);
 }
 }  // namespace pq

@@ -245,9 +245,11 @@ def load_library(path=LIB_PATH):
         "pq_autotune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float), ip]),
         "pq_set_split": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "pq_get_split": (C.c_int, [C.c_void_p, ip, ip]),
-        "pq_write_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int]),
+        "pq_write_paths_file": (C.c_int, [C.c_char_p, dp, ip, C.c_int, C.c_int, C.c_int]),
         "pq_run_measured": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp]),
-        "pq_read_paths_file": (C.c_int, [C.c_char_p, dp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "pq_read_paths_file": (C.c_int, [C.c_char_p, dp, ip, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "pq_set_permutation": (C.c_int, [C.c_void_p, C.c_int, ip]),
+        "pq_get_permutation": (C.c_int, [C.c_void_p, C.c_int, ip]),
         "pq_write_all_paths": (C.c_int, [C.c_void_p, C.c_char_p]),
         "pq_read_all_paths": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
         "pq_run_sweeps_timed": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float)]),
@@ -324,6 +326,15 @@ class Batch:
         b = np.empty((d.nslice, d.npart, d.ndim))
         check(self.lib.pq_get_paths(self.h, c, dptr(b)))
         return b
+
+    def set_permutation(self, c, perm):
+        p = np.ascontiguousarray(perm, dtype=np.int32)
+        check(self.lib.pq_set_permutation(self.h, c, iptr(p)))
+
+    def get_permutation(self, c):
+        p = np.zeros(self.descs[c].npart, dtype=np.int32)
+        check(self.lib.pq_get_permutation(self.h, c, iptr(p)))
+        return p
 
     def seed(self, seed, first_stream=0):
         check(self.lib.pq_seed(self.h, seed, first_stream))

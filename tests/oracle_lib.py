@@ -41,6 +41,9 @@ def load():
         "oq_destroy": (None, [vp]),
         "oq_set_beads": (None, [vp, dp]),
         "oq_get_beads": (None, [vp, dp]),
+        "oq_set_permutation": (C.c_int, [vp, ip]),
+        "oq_get_permutation": (None, [vp, ip]),
+        "oq_perm_compose": (None, [ip, ip, C.c_int, C.c_int]),
         "oq_npair": (C.c_int, [vp]),
         "oq_pair_table": (C.c_int, [vp, C.c_int, ip, dp, dp, dp]),
         "oq_rng_glibc": (None, [vp, C.c_uint]),
@@ -123,6 +126,16 @@ class Chain:
         b = np.empty((self.M, self.N, self.D))
         self.lib.oq_get_beads(self.h, dptr(b))
         return b
+
+    def set_permutation(self, p):
+        p = np.ascontiguousarray(p, dtype=np.int32)
+        if self.lib.oq_set_permutation(self.h, iptr(p)) != 0:
+            raise ValueError("not a permutation")
+
+    def get_permutation(self):
+        p = np.zeros(self.N, dtype=np.int32)
+        self.lib.oq_get_permutation(self.h, iptr(p))
+        return p
 
     def rng_glibc(self, seed):
         self.lib.oq_rng_glibc(self.h, seed)

@@ -444,7 +444,12 @@ def test_all_random_chooser_spends_one_permutation_draw():
     assert moved == [0] and st.draws == 1 + 2 + 9
 
 
-def test_oracle_rejects_several_moving_particles():
-    d = capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0)], samples=[(0, 2)])
+def test_oracle_rejects_more_moving_particles_than_the_pool():
+    # the reference's choosers would redraw forever; the oracle refuses
+    d = capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0)], samples=[(0, 4)])
     with pytest.raises(ValueError):
         ol.Chain(d)
+    d = capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0)], samples=[(0, 9)])
+    with pytest.raises(ValueError):
+        ol.Chain(d)
+    ol.Chain(capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0)], samples=[(0, 3)]))
