@@ -116,11 +116,16 @@ def timed_sweeps(b, steps):
     return ms, time.perf_counter() - t0
 
 
+TUNE_AFTER = 20   # sweeps from the replicated start before the autotuner times anything
+
+
 def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=None, schedule=None):
     """warmup sweeps (the autotuner's timing sweeps count among them, and
     total_warmup, when given, pads the warmup to that many sweeps), then steps
-    timed sweeps.  schedule = (team, inverse, tables, node state[, heavy
-    chains, heavy team]) fixes the schedule instead of autotuning."""
+    timed sweeps.  The autotuner runs after TUNE_AFTER sweeps, when the chains
+    have left the replicated start and the node records are warm.
+    schedule = (team, inverse, tables, node state[, heavy chains, heavy
+    team]) fixes the schedule instead of autotuning."""
     b = make_batch(descs, precision, seed)
     tuned, done = None, 0
     if schedule is not None:
@@ -129,7 +134,9 @@ def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=N
         if len(schedule) > 4:
             b.set_split(schedule[4], schedule[5])
     elif autotune:
-        tuned, done = b.autotune(1)
+        b.run_sweeps(TUNE_AFTER)
+        tuned, done = b.autotune(2)
+        done += TUNE_AFTER
     extra = warmup if total_warmup is None else max(0, total_warmup - done)
     b.run_sweeps(extra)
     b.sync()
@@ -248,7 +255,7 @@ def main():
     traffic, traffic_src = measured_traffic(args.workload, args.precision)
     out = {
         "metric": METRIC, "value": value, "unit": "bead-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
+        "steps": args.steps, "warmup": args.warmup, "warmup_sweeps_done": main_run["warmup_done"],
         "ms_per_step": 1e3 * main_run["wall"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64" if prec == capi.PQ_PREC_MIXED else "fp64",

@@ -1053,11 +1053,11 @@ extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed)
           s.team = team; s.inv = inv; s.tab = tab; s.fst = fst;
           if (int rc = trial(s)) return rc;
         }
-  // then the heaviest chains apart with a larger team, on the two best
+  // then the heaviest chains apart with a larger team, on the three best
   // placements (their one-sweep times differ by little more than the noise)
   std::stable_sort(bases.begin(), bases.end(),
                    [](const std::pair<float, Sched>& u, const std::pair<float, Sched>& v) { return u.first < v.first; });
-  for (size_t q = 0; q < bases.size() && q < 2 && !b->multi; ++q)
+  for (size_t q = 0; q < bases.size() && q < 3 && !b->multi; ++q)
     for (int frac : {16, 8, 4})
       for (int ht : {256, 512}) {
         Sched s = bases[q].second;
@@ -1066,21 +1066,27 @@ extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed)
         if (s.heavy < 1 || s.heavy >= b->nchain) continue;
         if (int rc = trial(s)) return rc;
       }
-  // a chain's sweep time drifts with its state by more than the schedules
-  // differ, so the four fastest are timed again, three rounds in turn, and
-  // the lowest total wins
+  // a chain's sweep time drifts with its state by about as much as the
+  // schedules differ, so the two fastest single launches and the three
+  // fastest splits are timed again, three rounds in turn, and the lowest
+  // total wins
   std::stable_sort(all.begin(), all.end(),
                    [](const std::pair<float, Sched>& u, const std::pair<float, Sched>& v) { return u.first < v.first; });
-  const size_t nfin = std::min<size_t>(all.size(), 4);
-  std::vector<float> tot(nfin, 0.f);
+  std::vector<Sched> fin;
+  int nsingle = 0, nsplit = 0;
+  for (const auto& t : all) {
+    if (t.second.heavy == 0 && nsingle < 2) { fin.push_back(t.second); ++nsingle; }
+    else if (t.second.heavy > 0 && nsplit < 3) { fin.push_back(t.second); ++nsplit; }
+  }
+  std::vector<float> tot(fin.size(), 0.f);
   for (int round = 0; round < 3; ++round)
-    for (size_t q = 0; q < nfin; ++q) {
+    for (size_t q = 0; q < fin.size(); ++q) {
       float ms = 0;
-      if (int rc = time_one(all[q].second, ms)) return rc;
+      if (int rc = time_one(fin[q], ms)) return rc;
       tot[q] += ms;
     }
-  for (size_t q = 0; q < nfin; ++q)
-    if (best < 0 || tot[q] / 3 < best) { best = tot[q] / 3; best_s = all[q].second; }
+  for (size_t q = 0; q < fin.size(); ++q)
+    if (best < 0 || tot[q] / 3 < best) { best = tot[q] / 3; best_s = fin[q]; }
   b->sched = best_s;
   make_layout(b, best_s);
   if (best_ms) *best_ms = best;

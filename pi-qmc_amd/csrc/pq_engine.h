@@ -314,6 +314,7 @@ struct Eng {
   double* mov;      // [S][nmov][D]
   int nmov_ = 1;    // moving particles of the current move, nmov()
   const int* midx = nullptr;   // [nmov] their indices
+  int mi0 = 0;      // the moving particle when MM = 1 (team-uniform register copy)
   double* fst;      // fixed-node state block
   double* inv;      // [S][ins][ins] latest inverse per section slice
   double* egr;      // [S][ins][ins][D] gradient factors of the latest matrix elements
@@ -358,6 +359,7 @@ struct Eng {
 
   __device__ __forceinline__ double U(uint64_t k) const { return uniform(x, chain, k); }
   __device__ __forceinline__ int nmov() const { return MM == 1 ? 1 : nmov_; }
+  __device__ __forceinline__ int mv(int m) const { return MM == 1 ? mi0 : midx[m]; }
 
   __device__ __forceinline__ double* sb(double* s, int t, int i) const {
     return s + ((int64_t)t * c.N + i) * D;
@@ -385,6 +387,21 @@ struct Eng {
     nmov_ = nm;
     midx = cb.idx;
     if (kind == PQ_CHOOSER_WALKING) return walk_choose(sp, nm, k, lnTP);
+    if constexpr (MM == 1) {
+      // one particle, no permutation: every thread makes the same draws
+      lnTP = 0;
+      if (kind == PQ_CHOOSER_ALL || kind == PQ_CHOOSER_ALL_RANDOM) {
+        if (kind == PQ_CHOOSER_ALL_RANDOM) ++k;
+        for (;;) {
+          const int j = (int)(c.N * U(k++));
+          if (j != c.N) { mi0 = j; return true; }
+        }
+      }
+      int j = (int)(c.count[sp] * U(k++) + c.ifirst[sp]);
+      if (j == c.count[sp] + c.ifirst[sp]) j = c.count[sp] + c.ifirst[sp] - 1;
+      mi0 = j;
+      return true;
+    }
     if (tid == 0) {
       uint64_t kk = k;
       for (int m = 0; m < nm; ++m) cb.cyc[m] = m;
@@ -419,6 +436,7 @@ struct Eng {
     __syncthreads();
     k = cb.k;
     lnTP = 0;
+    mi0 = cb.idx[0];
     // the next Sample's chooser rewrites cbuf
     __syncthreads();
     return true;
@@ -526,6 +544,7 @@ struct Eng {
     __syncthreads();
     k = cb.k;
     lnTP = cb.lnTP;
+    mi0 = cb.idx[0];
     const bool ok = cb.ok;
     __syncthreads();
     return ok;
@@ -551,7 +570,7 @@ struct Eng {
     for (int it = tid; it < nmoved * nmov(); it += NT) {
       int kk = it, m = 0;
       if (nmov() > 1) { kk = it / nmov(); m = it - kk * nmov(); }
-      const int i = midx[m];
+      const int i = mv(m);
       const int sp = species_of(c, i);
       const int t = s * (2 * kk + 1);
       const uint64_t kb = k0 + (uint64_t)kk * pd;
@@ -711,7 +730,7 @@ struct Eng {
     for (int it = tid; it < nl * nmov(); it += NT) {
       int k = it, m = 0;
       if (nmov() > 1) { k = it / nmov(); m = it - k * nmov(); }
-      const int i = midx[m];
+      const int i = mv(m);
       const int sp = species_of(c, i);
       const int t = s * (k + 1);
       const double* m1 = mov + (t * nmov() + m) * D;
@@ -771,7 +790,7 @@ struct Eng {
       const R* u = tab ? tab + tabofs[pp] : table_ptr<R>(x, T);
       const bool offdiag = level < 3 && T.K1 > 1;
       for (int m = 0; m < nmov(); ++m) {
-      const int i = midx[m];
+      const int i = mv(m);
       const int sp = species_of(c, i);
       int js;
       if (sp == P.s1) js = P.s2;
@@ -783,8 +802,9 @@ struct Eng {
         const int j = jb + it / nl;
         if (j == i) continue;
         int jm = -1;
-        for (int q = 0; q < nmov(); ++q)
-          if (midx[q] == j) jm = q;
+        if constexpr (MM > 1)
+          for (int q = 0; q < nmov(); ++q)
+            if (midx[q] == j) jm = q;
         if (jm >= 0 && i <= j) continue;
         const int t = s * (it % nl + 1);
         const double* pj1 = sb(sec1, t, j);
@@ -844,7 +864,7 @@ struct Eng {
   __device__ __forceinline__ const double* r1pos(int t, int p, bool use_mov) const {
     if (use_mov)
       for (int m = 0; m < nmov(); ++m)
-        if (midx[m] == p) return mov + (t * nmov() + m) * D;
+        if (mv(m) == p) return mov + (t * nmov() + m) * D;
     return sb(sec1, t, p);
   }
 
@@ -1220,7 +1240,7 @@ struct Eng {
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
       bool mine = false;
-      for (int m = 0; m < nmov(); ++m) mine |= midx[m] >= f.ifirst && midx[m] < f.ifirst + f.ns;
+      for (int m = 0; m < nmov(); ++m) mine |= mv(m) >= f.ifirst && mv(m) < f.ifirst + f.ns;
       if (tid == 0) notmine[fa] = mine ? 0 : 1;
       if (!mine) continue;
       const int nStride = 1 << (level + 1);

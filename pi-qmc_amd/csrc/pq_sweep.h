@@ -202,8 +202,8 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
         e.top_known = !permuted;
         for (int q = tid; q < nmov * D; q += NT) {
           const int m = q / D, d = q - m * D;
-          e.mov[m * D + d] = e.sb(e.sec1, 0, midx[m])[d];
-          e.mov[((S - 1) * nmov + m) * D + d] = e.sb(e.sec1, S - 1, midx[mcyc[m]])[d];
+          e.mov[m * D + d] = e.sb(e.sec1, 0, e.mv(m))[d];
+          e.mov[((S - 1) * nmov + m) * D + d] = e.sb(e.sec1, S - 1, e.mv(MM > 1 ? mcyc[m] : m))[d];
         }
         __syncthreads();
         // MultiLevelSampler::tryMove (MultiLevelSampler.cc:100-177)
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
           for (int q = tid; q < S * nmov * D; q += NT) {
             const int t = q / (nmov * D), rem = q - t * nmov * D;
             const int m = rem / D, d = rem - m * D;
-            e.sb(e.sec1, t, midx[m])[d] = e.mov[q];
+            e.sb(e.sec1, t, e.mv(m))[d] = e.mov[q];
           }
           if (tid == 0) {
             cnt[17] += (unsigned long long)nmov * ((1ull << L) - 1);
