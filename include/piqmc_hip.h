@@ -229,7 +229,7 @@ int pq_set_split(pq_batch* batch, int heavy_chains, int heavy_team);
 int pq_get_split(const pq_batch* batch, int* heavy_chains, int* heavy_team);
 /* Times every distinct schedule (team 64/128/256/512 x tables, node state and
    inverse cache in LDS or global memory, then splits of the heaviest chains
-   on the three best of those) over nsweep sampler sweeps of the batch, after
+   on the eight best of those) over nsweep sampler sweeps of the batch, after
    one untimed launch of nsweep sweeps; the two fastest single launches and
    the three fastest splits are timed three more times each, in turn, and the
    lowest mean wins.  best_ms receives that

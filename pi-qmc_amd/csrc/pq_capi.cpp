@@ -1053,11 +1053,12 @@ extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed)
           s.team = team; s.inv = inv; s.tab = tab; s.fst = fst;
           if (int rc = trial(s)) return rc;
         }
-  // then the heaviest chains apart with a larger team, on the three best
-  // placements (their one-sweep times differ by little more than the noise)
+  // then the heaviest chains apart with a larger team, on the eight best
+  // placements: with the heavy chains gone the bulk's best placement is often
+  // another one (tables in global memory rather than LDS on the uneven batch)
   std::stable_sort(bases.begin(), bases.end(),
                    [](const std::pair<float, Sched>& u, const std::pair<float, Sched>& v) { return u.first < v.first; });
-  for (size_t q = 0; q < bases.size() && q < 3 && !b->multi; ++q)
+  for (size_t q = 0; q < bases.size() && q < 8 && !b->multi; ++q)
     for (int frac : {16, 8, 4})
       for (int ht : {256, 512}) {
         Sched s = bases[q].second;
