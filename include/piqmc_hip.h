@@ -188,6 +188,15 @@ int pq_write_paths_file(const char* filename, const double* beads, const int* pe
                         int nslice, int npart, int ndim);
 int pq_read_paths_file(const char* filename, double* beads, int* perm, int nslice,
                        int npart, int ndim, int bfactor);
+/* pimc.dat of an ASCII report (src/stats/ascii/AsciiReportBuilder.cc:28-43,
+   AsciiScalarReportWriter.cpp:12-35): a header line "#" + "\"name (unit)\" "
+   per scalar estimator, then one line per block of (value + shift) * scale
+   at printf %g, each followed by a space.  values are [nblock][nscalar], the
+   block means already averaged over clones; units, shift and scale may be
+   null (no unit, 0, 1).  Touches no device. */
+int pq_write_ascii_report(const char* filename, int nscalar, const char* const* names,
+                          const char* const* units, const double* shift, const double* scale,
+                          int nblock, const double* values);
 /* Every chain of a batch, paths and permutation; chain c uses filename + c
    when the batch has more than one chain, as the reference appends the clone
    ID when nclone > 1. */
@@ -257,6 +266,11 @@ int pq_run_measured(pq_batch* batch, int nblock, int nsweep, double* energies);
 /* Thermodynamic energy estimate of every chain's current paths:
    E = (1/M) sum_{t,i} utau(i,t) (ThermoEnergyEstimator + CompositeAction::getBeadAction). */
 int pq_measure_energy(pq_batch* batch, double* energy);
+/* Coulomb potential energy of every chain's current paths
+   (<CoulombEnergyEstimator epsilon>, src/estimator/CoulombEnergyEstimator.cc:30-40,
+   without Ewald): (1/M) sum_t sum_{j<i} q_i q_j / (epsilon |pbc(r_i - r_j)|);
+   epsilon 0 means 1, as the parser's default. */
+int pq_measure_coulomb_energy(pq_batch* batch, double epsilon, double* energy);
 /* Radial pair histogram of every chain (PairCFEstimator<1> with PairRadial):
    integer counts over all slices, species1 particle i against every particle
    j != i of species2; counts[chain][nbin], added to what the array holds. */

@@ -1614,6 +1614,30 @@ double oq_thermo_energy(const oq_chain* ch) {
   return value / ch->M;
 }
 
+/* CoulombEnergyEstimator::handleLink (src/estimator/CoulombEnergyEstimator.cc:30-40)
+   over SerialPaths::sumOverLinks (slice 0 first, particles in order), then
+   SimpleScalarAccumulator::storeValue's division by M */
+double oq_coulomb_energy(const oq_chain* ch, double epsilon) {
+  const int D = ch->D;
+  if (epsilon == 0) epsilon = 1;
+  double value = 0;
+  for (int t = 0; t < ch->M; ++t)
+    for (int i = 0; i < ch->N; ++i) {
+      double energy = 0.0;
+      const double* end = bead(ch, i, t);
+      for (int j = 0; j < i; ++j) {
+        double delta[3];
+        const double* rj = bead(ch, j, t);
+        for (int d = 0; d < D; ++d) delta[d] = end[d] - rj[d];
+        pbc(&ch->cell, delta);
+        energy += ch->d.species[ch->spec[i]].charge * ch->d.species[ch->spec[j]].charge /
+                  (epsilon * sqrt(dot3(delta, delta, D)));
+      }
+      value += energy;
+    }
+  return value / ch->M;
+}
+
 /* PairCFEstimator<1>::handleLink with PairRadial, two species
    (src/estimator/PairCFEstimator.h:72-93, src/util/PairDistance.h:22-39) */
 void oq_pair_histogram(const oq_chain* ch, int s1, int s2, double rmin,

@@ -274,6 +274,7 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     c.ifirst[s] = n;
     c.count[s] = d.species[s].count;
     c.mass[s] = d.species[s].mass;
+    c.charge[s] = d.species[s].charge;
     c.lambda[s] = 0.5 / d.species[s].mass;
     n += d.species[s].count;
   }
@@ -865,6 +866,39 @@ extern "C" int pq_read_paths_file(const char* filename, double* beads, int* perm
   return PQ_OK;
 }
 
+// AsciiReportBuilder (src/stats/ascii/AsciiReportBuilder.cc:28-43) with
+// AsciiScalarReportWriter (AsciiScalarReportWriter.cpp:12-35): "#", then for
+// each scalar estimator a quoted name with " (unit)" when it has a unit, each
+// followed by a space; then one line per block, each value (value + shift) *
+// scale followed by a space at the stream's default precision (%g).  Array and
+// acceptance estimators write nothing to this file (their writers are null).
+extern "C" int pq_write_ascii_report(const char* filename, int nscalar, const char* const* names,
+                                     const char* const* units, const double* shift, const double* scale,
+                                     int nblock, const double* values) {
+  if (!filename || nscalar < 1 || !names || nblock < 0 || (nblock > 0 && !values))
+    return fail(PQ_ERR_ARG, "pq_write_ascii_report: bad arguments");
+  for (int k = 0; k < nscalar; ++k)
+    if (!names[k]) return fail(PQ_ERR_ARG, "pq_write_ascii_report: estimator %d has no name", k);
+  FILE* f = std::fopen(filename, "w");
+  if (!f) return fail(PQ_ERR_ARG, "cannot open %s for writing", filename);
+  std::fprintf(f, "#");
+  for (int k = 0; k < nscalar; ++k) {
+    std::fprintf(f, "\"%s", names[k]);
+    if (units && units[k] && units[k][0]) std::fprintf(f, " (%s)", units[k]);
+    std::fprintf(f, "\" ");
+  }
+  std::fprintf(f, "\n");
+  for (int blk = 0; blk < nblock; ++blk) {
+    for (int k = 0; k < nscalar; ++k) {
+      const double v = (values[(size_t)blk * nscalar + k] + (shift ? shift[k] : 0.0)) * (scale ? scale[k] : 1.0);
+      std::fprintf(f, "%g ", v);
+    }
+    std::fprintf(f, "\n");
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? PQ_OK : fail(PQ_ERR_ARG, "writing %s failed", filename);
+}
+
 static std::string clone_file(const char* base, int c, int nchain) {
   return nchain > 1 ? std::string(base) + std::to_string(c) : std::string(base);
 }
@@ -1168,6 +1202,19 @@ extern "C" int pq_measure_energy(pq_batch* b, double* energy) {
   if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
   HIPCHK(launch_energy(b->D, x, b->nchain, (double*)p, b->stream));
+  HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_measure_coulomb_energy(pq_batch* b, double epsilon, double* energy) {
+  if (!b || !energy) return fail(PQ_ERR_ARG, "pq_measure_coulomb_energy: bad arguments");
+  if (epsilon == 0) epsilon = 1;
+  HIPCHK(hipSetDevice(b->device));
+  void* p;
+  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_coulomb_energy(b->D, x, b->nchain, epsilon, (double*)p, b->stream));
   HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;

@@ -247,6 +247,52 @@ __global__ __launch_bounds__(256) void energy_kernel(DevCtx x, double* energy) {
   }
 }
 
+// CoulombEnergyEstimator::handleLink (CoulombEnergyEstimator.cc:30-40) summed
+// over every link by SerialPaths::sumOverLinks and divided by M
+// (SimpleScalarAccumulator::storeValue): sum over slices t and pairs j < i of
+// q_i q_j / (epsilon |pbc(r_i(t) - r_j(t))|).  One workgroup per chain, lanes
+// over (slice, particle).
+template <int D>
+__global__ __launch_bounds__(256) void coulomb_energy_kernel(DevCtx x, double eps, double* energy) {
+  __shared__ ChainP c;
+  __shared__ double part[4];
+  const int chain = blockIdx.x;
+  {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(&x.chains[chain]);
+    uint32_t* d = reinterpret_cast<uint32_t*>(&c);
+    for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
+    __syncthreads();
+  }
+  const double* beads = x.beads + c.bead_off;
+  double acc = 0;
+  for (int q = threadIdx.x; q < c.M * c.N; q += blockDim.x) {
+    const int t = q / c.N, i = q - t * c.N;
+    const double* ri = beads + ((int64_t)t * c.N + i) * D;
+    const double qi = c.charge[species_of(c, i)];
+    double e = 0;
+    for (int j = 0; j < i; ++j) {
+      const double* rj = beads + ((int64_t)t * c.N + j) * D;
+      double delta[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = ri[d] - rj[d];
+      pbc<D>(c, delta);
+      double dd = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) dd += delta[d] * delta[d];
+      e += qi * c.charge[species_of(c, j)] / (eps * sqrt(dd));
+    }
+    acc += e;
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0) part[threadIdx.x / PQ_WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0;
+    for (int w = 0; w < (int)(blockDim.x / PQ_WAVE); ++w) v += part[w];
+    energy[chain] = v / c.M;
+  }
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void bead_utau_kernel(DevCtx x, int nq, const int* qc, const int* qi,
                                                          const int* qt, double* out) {
@@ -384,6 +430,12 @@ hipError_t launch_node_eval(int D, int NC, bool mixed, const DevCtx& x, int chai
 hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st) {
   if (D == 2) hipLaunchKernelGGL(energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, energy);
   else hipLaunchKernelGGL(energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, energy);
+  return hipGetLastError();
+}
+
+hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps, double* energy, hipStream_t st) {
+  if (D == 2) hipLaunchKernelGGL(coulomb_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
+  else hipLaunchKernelGGL(coulomb_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
   return hipGetLastError();
 }
 

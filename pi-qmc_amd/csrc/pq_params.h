@@ -39,6 +39,7 @@ struct ChainP {
   double a[3], b[3], rcut2;
   int ifirst[4], count[4];
   double mass[4], lambda[4];
+  double charge[4];
   int spring;
   int nsho;
   int sho_sp[4], sho_ndim[4];

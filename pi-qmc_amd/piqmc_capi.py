@@ -249,6 +249,8 @@ def load_library(path=LIB_PATH):
         "pq_run_measured": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp]),
         "pq_read_paths_file": (C.c_int, [C.c_char_p, dp, ip, C.c_int, C.c_int, C.c_int, C.c_int]),
         "pq_set_permutation": (C.c_int, [C.c_void_p, C.c_int, ip]),
+        "pq_write_ascii_report": (C.c_int, [C.c_char_p, C.c_int, P(C.c_char_p), P(C.c_char_p), dp, dp,
+                                            C.c_int, dp]),
         "pq_get_permutation": (C.c_int, [C.c_void_p, C.c_int, ip]),
         "pq_write_all_paths": (C.c_int, [C.c_void_p, C.c_char_p]),
         "pq_read_all_paths": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
@@ -259,6 +261,7 @@ def load_library(path=LIB_PATH):
         "pq_reset_stats": (C.c_int, [C.c_void_p]),
         "pq_get_node_scale": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "pq_measure_energy": (C.c_int, [C.c_void_p, dp]),
+        "pq_measure_coulomb_energy": (C.c_int, [C.c_void_p, C.c_double, dp]),
         "pq_pair_histogram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_int, P(C.c_uint64)]),
         "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
@@ -274,6 +277,19 @@ def load_library(path=LIB_PATH):
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def write_ascii_report(filename, names, values, units=None, shift=None, scale=None):
+    """pimc.dat of the reference's ASCII report: values [nblock][nscalar]."""
+    lib = load_library()
+    n = len(names)
+    v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1, n)
+    cn = (C.c_char_p * n)(*[x.encode() for x in names])
+    cu = (C.c_char_p * n)(*[(u or "").encode() for u in units]) if units else None
+    sh = np.ascontiguousarray(shift, dtype=np.float64) if shift is not None else None
+    sc = np.ascontiguousarray(scale, dtype=np.float64) if scale is not None else None
+    check(lib.pq_write_ascii_report(filename.encode(), n, cn, cu, dptr(sh) if sh is not None else None,
+                                    dptr(sc) if sc is not None else None, v.shape[0], dptr(v)))
 
 
 def dptr(a):
@@ -389,4 +405,9 @@ class Batch:
     def energy(self):
         e = np.empty(len(self.descs))
         check(self.lib.pq_measure_energy(self.h, dptr(e)))
+        return e
+
+    def coulomb_energy(self, epsilon=1.0):
+        e = np.empty(len(self.descs))
+        check(self.lib.pq_measure_coulomb_energy(self.h, epsilon, dptr(e)))
         return e

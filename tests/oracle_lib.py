@@ -63,6 +63,7 @@ def load():
         "oq_run_sweeps": (None, [vp, C.c_int]),
         "oq_get_stats": (None, [vp, P(capi.ChainStats)]),
         "oq_thermo_energy": (C.c_double, [vp]),
+        "oq_coulomb_energy": (C.c_double, [vp, C.c_double]),
         "oq_energy_blocks": (None, [vp, C.c_int, C.c_int, dp]),
         "oq_pair_histogram": (None, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                      P(C.c_uint64)]),
@@ -167,6 +168,9 @@ class Chain:
 
     def energy(self):
         return self.lib.oq_thermo_energy(self.h)
+
+    def coulomb_energy(self, epsilon=1.0):
+        return self.lib.oq_coulomb_energy(self.h, epsilon)
 
     def energy_blocks(self, nblock, nmeasure):
         out = np.zeros(nblock)

@@ -558,3 +558,15 @@ def test_measured_graph_matches_eager():
         e.run_sweeps(1)
     for c in range(len(descs)):
         assert np.array_equal(g.get_paths(c), e.get_paths(c))
+
+
+def test_coulomb_energy_estimator():
+    """CoulombEnergyEstimator on the device against the oracle (1e-12), open
+    and periodic cells, after sweeps."""
+    print("[test] coulomb energy estimator", flush=True)
+    descs = [_descs()["cfg1"], _descs()["pbc"], small_cfg2()]
+    for desc in descs:
+        b, oc = _paired(desc, 3)
+        for eps in (1.0, 12.5):
+            got = b.coulomb_energy(eps)[0]
+            assert rel_close(got, oc.coulomb_energy(eps), 1e-12, scale=1e-300), (got, oc.coulomb_energy(eps))

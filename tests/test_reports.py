@@ -1,0 +1,81 @@
+"""Scalar estimator outputs on the host side: the pimc.dat layout of the
+reference's ASCII report (src/stats/ascii/AsciiReportBuilder.cc:28-43,
+AsciiScalarReportWriter.cpp:12-35) written by the C ABI, and the oracle's
+CoulombEnergyEstimator (src/estimator/CoulombEnergyEstimator.cc:30-40)
+against closed forms and a brute-force sum with the SuperCell image rule."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import piqmc_capi as capi
+
+lib_built = pytest.mark.skipif(not os.path.exists(capi.LIB_PATH), reason="library not built")
+
+
+@lib_built
+def test_ascii_report_layout(tmp_path):
+    """Header: '#' and each estimator quoted with its unit in parentheses,
+    each followed by a space; rows: (value + shift) * scale at the stream's
+    default precision (6 significant digits), each followed by a space."""
+    fn = str(tmp_path / "pimc.dat")
+    values = np.array([[1.5, -0.25, 2.0], [3.14159265358979, 1e-05, 123456789.0]])
+    capi.write_ascii_report(fn, ["thermo_energy", "coulomb_energy", "virial_energy"], values,
+                            units=["Ha", "", "meV"], shift=[0.0, 0.0, 1.0], scale=[1.0, 1.0, 0.5])
+    want = ('#"thermo_energy (Ha)" "coulomb_energy" "virial_energy (meV)" \n'
+            "1.5 -0.25 1.5 \n"
+            "3.14159 1e-05 6.17284e+07 \n")
+    assert open(fn).read() == want
+
+
+@lib_built
+def test_ascii_report_refuses_missing_names(tmp_path):
+    lib = capi.load_library()
+    import ctypes as C
+    names = (C.c_char_p * 1)(None)
+    assert lib.pq_write_ascii_report(str(tmp_path / "x").encode(), 1, names, None, None, None, 0, None) \
+        == capi.PQ_ERR_ARG
+
+
+def test_coulomb_energy_two_charges():
+    """Two opposite charges a fixed distance apart in every slice: q1 q2/(eps r)."""
+    M = 12
+    d = capi.make_desc(3, M, 2, 0.1, [(1, 1.0, -1.0), (1, 2.0, 2.0)], cell=1000.0)
+    ch = ol.Chain(d)
+    b = np.zeros((M, 2, 3))
+    b[:, 1] = [0.3, -0.4, 1.2]
+    ch.set_beads(b)
+    r = math.sqrt(0.09 + 0.16 + 1.44)
+    assert ch.coulomb_energy(1.0) == pytest.approx(-2.0 / r, rel=1e-15)
+    assert ch.coulomb_energy(12.0) == pytest.approx(-2.0 / (12.0 * r), rel=1e-15)
+    assert ch.coulomb_energy(0.0) == ch.coulomb_energy(1.0)
+
+
+def _pbc(v, a):
+    """SuperCell::pbc (src/util/SuperCell.cc:25-42): minimum image only beyond
+    half the shortest side."""
+    rcut2 = (0.5 * min(a)) ** 2
+    if np.dot(v, v) > rcut2:
+        v = v - a * (np.floor(v / a + 1000.5) - 1000)
+    return v
+
+
+def test_coulomb_energy_brute_force_with_images():
+    rng = np.random.default_rng(4)
+    M, D = 8, 3
+    cell = np.array([3.0, 3.5, 4.0])
+    species = [(3, 1.0, -1.0), (2, 2.0, 1.0)]
+    d = capi.make_desc(D, M, 2, 0.05, species, cell=list(cell))
+    ch = ol.Chain(d)
+    b = rng.uniform(-1.6, 1.6, size=(M, 5, D))
+    ch.set_beads(b)
+    q = np.array([-1.0] * 3 + [1.0] * 2)
+    want = 0.0
+    for t in range(M):
+        for i in range(5):
+            for j in range(i):
+                dv = _pbc(b[t, i] - b[t, j], cell)
+                want += q[i] * q[j] / (2.0 * math.sqrt(np.dot(dv, dv)))
+    assert ch.coulomb_energy(2.0) == pytest.approx(want / M, rel=1e-13)
