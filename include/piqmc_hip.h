@@ -271,6 +271,12 @@ int pq_measure_energy(pq_batch* batch, double* energy);
    without Ewald): (1/M) sum_t sum_{j<i} q_i q_j / (epsilon |pbc(r_i - r_j)|);
    epsilon 0 means 1, as the parser's default. */
 int pq_measure_coulomb_energy(pq_batch* batch, double epsilon, double* energy);
+/* Virial energy of every chain's current paths (<VirialEnergyEstimator nwindow>,
+   src/estimator/VirialEnergyEstimator.cc:27-78): utau plus the centroid-virial
+   force terms of getBeadAction over windows of nwindow links (0 means 1, the
+   parser's default; nwindow = 1 is the thermodynamic estimator).  The
+   FixedNodeActions contribute zero force and utau, as in the reference. */
+int pq_measure_virial_energy(pq_batch* batch, int nwindow, double* energy);
 /* Radial pair histogram of every chain (PairCFEstimator<1> with PairRadial):
    integer counts over all slices, species1 particle i against every particle
    j != i of species2; counts[chain][nbin], added to what the array holds. */

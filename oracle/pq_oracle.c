@@ -1614,6 +1614,158 @@ double oq_thermo_energy(const oq_chain* ch) {
   return value / ch->M;
 }
 
+/* PairAction::uk0CalcDerivatives without z (src/action/PairAction.cc:640-657):
+   the link action v, its tau derivative, dv/dq and dv/ds2 */
+static void uk0_derivs(const pair_t* p, double q, double s2, double* u, double* ut, double* uq, double* us2) {
+  q *= p->rgridinv;
+  double x = log(q) * p->logrratioinv;
+  int i = (int)x;
+  if (q < 1) { i = 0; x = 0; }
+  else if (i > p->ngpts - 2) { i = p->ngpts - 2; x = 1; }
+  else x -= i;
+  const int K1 = p->norder + 1;
+  double a = 0, b = 0, c = 0, e = 0;
+  for (int k = p->norder; k >= 0; k--) {
+    a *= s2; a += (1 - x) * p->u[i * K1 + k] + x * p->u[(i + 1) * K1 + k];
+    b *= s2; b += (1 - x) * p->ut[i * K1 + k] + x * p->ut[(i + 1) * K1 + k];
+    c *= s2; c += p->u[(i + 1) * K1 + k] - p->u[i * K1 + k];
+  }
+  c *= p->logrratioinv * p->rgridinv / q;
+  for (int k = p->norder; k > 0; k--) {
+    e *= s2; e += k * ((1 - x) * p->u[i * K1 + k] + x * p->u[(i + 1) * K1 + k]);
+  }
+  *u = a; *ut = b; *uq = c; *us2 = e;
+}
+
+/* CompositeAction::getBeadAction (CompositeAction.cc:57-67) over
+   SpringAction (SpringAction.cc:112-123: utau and forces, no u), SHOAction
+   (SHOAction.cc:71-105) and PairAction (PairAction.cc:509-599, no exchange):
+   u, utau, and the forces fm (from the link into slice t) and fp (from the
+   link out of it) on bead (ipart, islice). */
+void oq_bead_action(const oq_chain* ch, int ipart, int islice, double* u_out, double* utau_out,
+                    double* fm, double* fp) {
+  const int D = ch->D;
+  double u = 0, utau = 0;
+  for (int d = 0; d < D; ++d) fm[d] = fp[d] = 0;
+  const double* b1 = bead(ch, ipart, islice);
+  const double* b0 = bead(ch, ipart, islice - 1);
+  const double* b2 = bead(ch, ipart, islice + 1);
+  double dm[3], dp[3];
+  for (int d = 0; d < D; ++d) { dm[d] = b1[d] - b0[d]; dp[d] = b1[d] - b2[d]; }
+  pbc(&ch->cell, dm);
+  pbc(&ch->cell, dp);
+  if (ch->d.spring) {
+    const double lam = ch->lambda[ipart], tau = ch->tau;
+    for (int d = 0; d < D; ++d) { fm[d] -= dm[d] / (2 * lam * tau); fp[d] -= dp[d] / (2 * lam * tau); }
+    utau += D / (2 * tau) - dot3(dm, dm, D) / (4.0 * lam * tau * tau);
+  }
+  for (int s = 0; s < ch->d.nsho; ++s) {
+    const pq_sho_desc* a = &ch->d.sho[s];
+    const int ifirst = ch->ifirst[a->species], np = ch->d.species[a->species].count;
+    if (ipart < ifirst || ipart >= ifirst + np) continue;
+    const int ndim = a->ndim ? a->ndim : D;
+    const double omega = a->omega, m = ch->d.species[a->species].mass, tau = ch->tau;
+    const double wt = omega * tau, sinhwt = sinh(wt), coshwt = cosh(wt);
+    const double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / tau;
+    double r1[3], r0[3], r2[3];
+    for (int d = 0; d < D; ++d) { r1[d] = b1[d] - a->center[d]; r0[d] = b0[d] - a->center[d]; r2[d] = b2[d] - a->center[d]; }
+    double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
+    for (int d = D - ndim; d < D; ++d) {
+      r0r0 += r0[d] * r0[d]; r0r1 += r0[d] * r1[d]; r1r1 += r1[d] * r1[d]; delta2 += dm[d] * dm[d];
+    }
+    for (int d = D - ndim; d < D; ++d) {
+      fm[d] -= m * omega * (r1[d] * coshwt - r0[d]) * cschwt - m * dm[d] * tin;
+      fp[d] -= m * omega * (r1[d] * coshwt - r2[d]) * cschwt - m * dp[d] * tin;
+    }
+    utau += 0.5 * ndim * omega * cothwt
+            + 0.5 * m * omega * omega * ((r1r1 + r0r0) * (1 - cothwt * cothwt) + 2.0 * r0r1 * coshwt * cschwt * cschwt)
+            + 0.5 * m * delta2 * tin * tin - 0.5 * ndim * tin;
+    u += 0.5 * ndim * log(2 * 3.14159265358979 * sinhwt / (m * omega)) +
+         m * omega * ((r1r1 + r0r0) * coshwt - 2.0 * r0r1) * cschwt * 0.5 -
+         0.5 * log(2.0 * 3.14159265358979 * tau / m) - 0.5 * m * delta2 / tau;
+  }
+  for (int pp = 0; pp < ch->npair; ++pp) {
+    const pair_t* p = &ch->pair[pp];
+    const int if1 = ch->ifirst[p->s1], n1 = ch->d.species[p->s1].count;
+    const int if2 = ch->ifirst[p->s2], n2 = ch->d.species[p->s2].count;
+    int jb, je;
+    if (ipart >= if1 && ipart < if1 + n1) { jb = if2; je = if2 + n2; }
+    else if (ipart >= if2 && ipart < if2 + n2) { jb = if1; je = if1 + n1; }
+    else continue;
+    for (int j = jb; j < je; ++j) {
+      if (ipart == j) continue;
+      double delta[3], prev[3], next[3], svec[3];
+      const double* aj = bead(ch, j, islice);
+      const double* bj = bead(ch, j, islice - 1);
+      const double* cj = bead(ch, j, islice + 1);
+      for (int d = 0; d < D; ++d) { delta[d] = b1[d] - aj[d]; prev[d] = b0[d] - bj[d]; next[d] = b2[d] - cj[d]; }
+      pbc(&ch->cell, delta);
+      pbc(&ch->cell, prev);
+      const double r = sqrt(dot3(delta, delta, D));
+      const double prevR = sqrt(dot3(prev, prev, D));
+      double q = 0.5 * (r + prevR);
+      for (int d = 0; d < D; ++d) svec[d] = delta[d] - prev[d];
+      double s2 = dot3(svec, svec, D) / (q * q);
+      double v, vtau, vq, vs2;
+      uk0_derivs(p, q, s2, &v, &vtau, &vq, &vs2);
+      u += 0.5 * v;
+      utau += 0.5 * vtau;
+      for (int d = 0; d < D; ++d)
+        fm[d] -= vq * delta[d] / (2 * r) + vs2 * (2 * svec[d] / (q * q) - s2 * delta[d] / (q * r));
+      pbc(&ch->cell, next);
+      const double nextR = sqrt(dot3(next, next, D));
+      q = 0.5 * (r + nextR);
+      for (int d = 0; d < D; ++d) svec[d] = delta[d] - next[d];
+      s2 = dot3(svec, svec, D) / (q * q);
+      uk0_derivs(p, q, s2, &v, &vtau, &vq, &vs2);
+      for (int d = 0; d < D; ++d)
+        fp[d] -= vq * delta[d] / (2 * r) + vs2 * (2 * svec[d] / (q * q) - s2 * delta[d] / (q * r));
+    }
+  }
+  *u_out = u;
+  *utau_out = utau;
+}
+
+/* VirialEnergyEstimator (src/estimator/VirialEnergyEstimator.cc:27-78) over
+   SerialPaths::sumOverLinks with firstSlice 0 and no static particles; the
+   FixedNodeActions' getBeadAction gives zero forces and, without
+   useDistDerivative, zero utau (FixedNodeAction.cc:254-329), so they add
+   nothing.  Returns the energy of the current paths (energy / nslice). */
+double oq_virial_energy(const oq_chain* ch, int nwindow) {
+  const int D = ch->D, M = ch->M, N = ch->N;
+  const double tau = ch->tau;
+  if (nwindow < 1) nwindow = 1;
+  double* r0 = (double*)calloc((size_t)N * 3, sizeof(double));
+  double* rav = (double*)calloc((size_t)N * 3, sizeof(double));
+  double* fav = (double*)calloc((size_t)N * 3, sizeof(double));
+  double energy = 0;
+  for (int t = 0; t < M; ++t)
+    for (int i = 0; i < N; ++i) {
+      double* R0 = &r0[i * 3]; double* RA = &rav[i * 3]; double* FA = &fav[i * 3];
+      if (t % nwindow == 0) {
+        energy += (1 / (2 * tau)) * dot3(FA, R0, D);
+        for (int d = 0; d < 3; ++d) R0[d] = RA[d] = FA[d] = 0;
+      }
+      const int W = (t / nwindow < (M - 1) / nwindow) ? nwindow : (M - 1) % nwindow + 1;
+      const double *e = bead(ch, i, t), *s = bead(ch, i, t - 1);
+      double r[3];
+      for (int d = 0; d < D; ++d) r[d] = e[d] - s[d];
+      pbc(&ch->cell, r);
+      for (int d = 0; d < D; ++d) {
+        RA[d] += r[d];
+        R0[d] += (W - t % nwindow) * r[d] / W;
+      }
+      double u, utau, fm[3], fp[3], f[3];
+      oq_bead_action(ch, i, t, &u, &utau, fm, fp);
+      for (int d = 0; d < D; ++d) { f[d] = fm[d] + fp[d]; FA[d] += f[d]; }
+      energy += utau;
+      energy += -(1. - 1. / W) * (D / (2 * tau)) - (1 / (2 * tau)) * dot3(f, RA, D);
+    }
+  for (int i = 0; i < N; ++i) energy += (1 / (2 * tau)) * dot3(&fav[i * 3], &r0[i * 3], D);
+  free(r0); free(rav); free(fav);
+  return energy / M;
+}
+
 /* CoulombEnergyEstimator::handleLink (src/estimator/CoulombEnergyEstimator.cc:30-40)
    over SerialPaths::sumOverLinks (slice 0 first, particles in order), then
    SimpleScalarAccumulator::storeValue's division by M */

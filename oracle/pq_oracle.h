@@ -77,6 +77,10 @@ void oq_run_sweeps(oq_chain* ch, int nsweep);
 void oq_get_stats(const oq_chain* ch, pq_chain_stats* st);
 double oq_thermo_energy(const oq_chain* ch);
 double oq_coulomb_energy(const oq_chain* ch, double epsilon);
+/* getBeadAction of the composite action: u, utau, forces fm/fp [D] */
+void oq_bead_action(const oq_chain* ch, int ipart, int islice, double* u, double* utau,
+                    double* fm, double* fp);
+double oq_virial_energy(const oq_chain* ch, int nwindow);
 void oq_energy_blocks(oq_chain* ch, int nblock, int nmeasure, double* block);
 void oq_pair_histogram(const oq_chain* ch, int s1, int s2, double rmin,
                        double rmax, int nbin, uint64_t* counts);

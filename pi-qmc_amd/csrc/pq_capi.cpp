@@ -1220,6 +1220,19 @@ extern "C" int pq_measure_coulomb_energy(pq_batch* b, double epsilon, double* en
   return PQ_OK;
 }
 
+extern "C" int pq_measure_virial_energy(pq_batch* b, int nwindow, double* energy) {
+  if (!b || !energy) return fail(PQ_ERR_ARG, "pq_measure_virial_energy: bad arguments");
+  if (nwindow <= 0) nwindow = 1;
+  HIPCHK(hipSetDevice(b->device));
+  void* p;
+  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_virial_energy(b->D, x, b->nchain, nwindow, (double*)p, b->stream));
+  HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
 static void drop_graph(pq_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);

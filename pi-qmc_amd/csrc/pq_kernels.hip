@@ -293,6 +293,171 @@ __global__ __launch_bounds__(256) void coulomb_energy_kernel(DevCtx x, double ep
   }
 }
 
+// getBeadAction's utau and force fm + fp on bead (i,t) of the composite
+// action: SpringAction (SpringAction.cc:112-123), SHOAction (SHOAction.cc:
+// 71-105), PairAction (PairAction.cc:509-599) with uk0CalcDerivatives
+// (:640-657).  Slices -1 and M wrap through the permutation.
+template <int D>
+__device__ double bead_force(const DevCtx& x, const ChainP& c, const double* beads, int i, int t, double* f) {
+  const int* perm = x.perm + c.perm_off;
+  const int* invperm = perm + c.N;
+  auto P = [&](int p, int tt) -> const double* {
+    if (tt < 0) { tt += c.M; p = invperm[p]; }
+    else if (tt >= c.M) { tt -= c.M; p = perm[p]; }
+    return beads + ((int64_t)tt * c.N + p) * D;
+  };
+  const int sp = species_of(c, i);
+  const double* b1 = P(i, t);
+  const double* b0 = P(i, t - 1);
+  const double* b2 = P(i, t + 1);
+  double dm[D], dp[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) { dm[d] = b1[d] - b0[d]; dp[d] = b1[d] - b2[d]; f[d] = 0; }
+  pbc<D>(c, dm);
+  pbc<D>(c, dp);
+  double utau = 0;
+  if (c.spring) {
+    const double lam = c.lambda[sp], tau = c.tau;
+    double dd = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) { f[d] -= dm[d] / (2 * lam * tau) + dp[d] / (2 * lam * tau); dd += dm[d] * dm[d]; }
+    utau += D / (2 * tau) - dd / (4.0 * lam * tau * tau);
+  }
+  for (int a = 0; a < c.nsho; ++a) {
+    const int ssp = c.sho_sp[a];
+    if (i < c.ifirst[ssp] || i >= c.ifirst[ssp] + c.count[ssp]) continue;
+    const int ndim = c.sho_ndim[a] ? c.sho_ndim[a] : D;
+    const double omega = c.sho_omega[a], m = c.mass[ssp];
+    const double wt = omega * c.tau, sinhwt = sinh(wt), coshwt = cosh(wt);
+    const double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / c.tau;
+    double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
+    for (int d = D - ndim; d < D; ++d) {
+      const double r1 = b1[d] - c.sho_center[a][d], r0 = b0[d] - c.sho_center[a][d], r2 = b2[d] - c.sho_center[a][d];
+      r0r0 += r0 * r0; r0r1 += r0 * r1; r1r1 += r1 * r1; delta2 += dm[d] * dm[d];
+      f[d] -= m * omega * (r1 * coshwt - r0) * cschwt - m * dm[d] * tin;
+      f[d] -= m * omega * (r1 * coshwt - r2) * cschwt - m * dp[d] * tin;
+    }
+    utau += 0.5 * ndim * omega * cothwt
+            + 0.5 * m * omega * omega * ((r1r1 + r0r0) * (1 - cothwt * cothwt) + 2.0 * r0r1 * coshwt * cschwt * cschwt)
+            + 0.5 * m * delta2 * tin * tin - 0.5 * ndim * tin;
+  }
+  for (int pp = 0; pp < c.npair; ++pp) {
+    const PairP& PP = c.pair[pp];
+    int js;
+    if (sp == PP.s1) js = PP.s2;
+    else if (sp == PP.s2) js = PP.s1;
+    else continue;
+    const TableP T = x.tables[PP.table];
+    const double* u = x.tab_u + T.off;
+    const double* ut = x.tab_ut + T.off;
+    for (int j = c.ifirst[js]; j < c.ifirst[js] + c.count[js]; ++j) {
+      if (j == i) continue;
+      const double* a1 = P(j, t);
+      const double* a0 = P(j, t - 1);
+      const double* a2 = P(j, t + 1);
+      double dl[D], pd[D], nd[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) { dl[d] = b1[d] - a1[d]; pd[d] = b0[d] - a0[d]; nd[d] = b2[d] - a2[d]; }
+      pbc<D>(c, dl);
+      pbc<D>(c, pd);
+      pbc<D>(c, nd);
+      double r = 0, pr = 0, nr = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) { r += dl[d] * dl[d]; pr += pd[d] * pd[d]; nr += nd[d] * nd[d]; }
+      r = sqrt(r); pr = sqrt(pr); nr = sqrt(nr);
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const double* od = side ? nd : pd;
+        const double q = 0.5 * (r + (side ? nr : pr));
+        double sv[D], s2 = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) { sv[d] = dl[d] - od[d]; s2 += sv[d] * sv[d]; }
+        s2 /= q * q;
+        // uk0CalcDerivatives: v, vtau, dv/dq, dv/ds2
+        const double qq = q * T.rgridinv;
+        double xx = log(qq) * T.logrratioinv;
+        int ig = (int)xx;
+        if (qq < 1) { ig = 0; xx = 0; }
+        else if (ig > T.ngpts - 2) { ig = T.ngpts - 2; xx = 1; }
+        else xx -= ig;
+        double vt = 0, vq = 0, vs2 = 0;
+        for (int k = T.K1 - 1; k >= 0; --k) {
+          vt *= s2; vt += (1 - xx) * ut[ig * T.K1 + k] + xx * ut[(ig + 1) * T.K1 + k];
+          vq *= s2; vq += u[(ig + 1) * T.K1 + k] - u[ig * T.K1 + k];
+        }
+        vq *= T.logrratioinv * T.rgridinv / qq;
+        for (int k = T.K1 - 1; k > 0; --k) {
+          vs2 *= s2; vs2 += k * ((1 - xx) * u[ig * T.K1 + k] + xx * u[(ig + 1) * T.K1 + k]);
+        }
+        if (side == 0) utau += 0.5 * vt;
+#pragma unroll
+        for (int d = 0; d < D; ++d) f[d] -= vq * dl[d] / (2 * r) + vs2 * (2 * sv[d] / (q * q) - s2 * dl[d] / (q * r));
+      }
+    }
+  }
+  return utau;
+}
+
+// VirialEnergyEstimator (VirialEnergyEstimator.cc:27-78) over sumOverLinks
+// with firstSlice 0.  Its running sums restart at every window of nwindow
+// links of a particle, so lanes take (particle, window) items; a window
+// closes with F_av . r0 / 2tau.  FixedNodeAction adds zero force and utau.
+template <int D>
+__global__ __launch_bounds__(256) void virial_energy_kernel(DevCtx x, int nwindow, double* energy) {
+  __shared__ ChainP c;
+  __shared__ double part[4];
+  const int chain = blockIdx.x;
+  {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(&x.chains[chain]);
+    uint32_t* d = reinterpret_cast<uint32_t*>(&c);
+    for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
+    __syncthreads();
+  }
+  const double* beads = x.beads + c.bead_off;
+  const int* invperm = x.perm + c.perm_off + c.N;
+  const int M = c.M, nwin = (M + nwindow - 1) / nwindow;
+  const double tau = c.tau;
+  double acc = 0;
+  for (int it = threadIdx.x; it < c.N * nwin; it += blockDim.x) {
+    const int i = it / nwin, w = it - i * nwin;
+    const int t0 = w * nwindow, t1 = min(M, t0 + nwindow);
+    const int W = (t0 / nwindow < (M - 1) / nwindow) ? nwindow : (M - 1) % nwindow + 1;
+    double r0[D], rav[D], fav[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) { r0[d] = 0; rav[d] = 0; fav[d] = 0; }
+    double e = 0;
+    for (int t = t0; t < t1; ++t) {
+      const double* end = beads + ((int64_t)t * c.N + i) * D;
+      const double* start = (t > 0) ? beads + ((int64_t)(t - 1) * c.N + i) * D
+                                    : beads + ((int64_t)(M - 1) * c.N + invperm[i]) * D;
+      double r[D], f[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) r[d] = end[d] - start[d];
+      pbc<D>(c, r);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { rav[d] += r[d]; r0[d] += (W - t % nwindow) * r[d] / W; }
+      const double utau = bead_force<D>(x, c, beads, i, t, f);
+      double fr = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) { fav[d] += f[d]; fr += f[d] * rav[d]; }
+      e += utau;
+      e += -(1. - 1. / W) * (D / (2 * tau)) - (1 / (2 * tau)) * fr;
+    }
+    double fr0 = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) fr0 += fav[d] * r0[d];
+    acc += e + (1 / (2 * tau)) * fr0;
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0) part[threadIdx.x / PQ_WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0;
+    for (int w = 0; w < (int)(blockDim.x / PQ_WAVE); ++w) v += part[w];
+    energy[chain] = v / M;
+  }
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void bead_utau_kernel(DevCtx x, int nq, const int* qc, const int* qi,
                                                          const int* qt, double* out) {
@@ -436,6 +601,12 @@ hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hip
 hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps, double* energy, hipStream_t st) {
   if (D == 2) hipLaunchKernelGGL(coulomb_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
   else hipLaunchKernelGGL(coulomb_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
+  return hipGetLastError();
+}
+
+hipError_t launch_virial_energy(int D, const DevCtx& x, int nchain, int nwindow, double* energy, hipStream_t st) {
+  if (D == 2) hipLaunchKernelGGL(virial_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, nwindow, energy);
+  else hipLaunchKernelGGL(virial_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, nwindow, energy);
   return hipGetLastError();
 }
 

@@ -56,6 +56,7 @@ hipError_t launch_node_eval(int D, int NC, bool mixed, const DevCtx& x, int chai
                             double* dist, hipStream_t st);
 hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st);
 hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps, double* energy, hipStream_t st);
+hipError_t launch_virial_energy(int D, const DevCtx& x, int nchain, int nwindow, double* energy, hipStream_t st);
 hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const int* qi, const int* qt,
                             double* out, hipStream_t st);
 hipError_t launch_histogram(int D, const DevCtx& x, int nchain, int s1, int s2, double rmin, double rmax,

@@ -262,6 +262,7 @@ def load_library(path=LIB_PATH):
         "pq_get_node_scale": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "pq_measure_energy": (C.c_int, [C.c_void_p, dp]),
         "pq_measure_coulomb_energy": (C.c_int, [C.c_void_p, C.c_double, dp]),
+        "pq_measure_virial_energy": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "pq_pair_histogram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_int, P(C.c_uint64)]),
         "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
@@ -410,4 +411,9 @@ class Batch:
     def coulomb_energy(self, epsilon=1.0):
         e = np.empty(len(self.descs))
         check(self.lib.pq_measure_coulomb_energy(self.h, epsilon, dptr(e)))
+        return e
+
+    def virial_energy(self, nwindow=1):
+        e = np.empty(len(self.descs))
+        check(self.lib.pq_measure_virial_energy(self.h, nwindow, dptr(e)))
         return e

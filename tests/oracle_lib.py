@@ -64,6 +64,8 @@ def load():
         "oq_get_stats": (None, [vp, P(capi.ChainStats)]),
         "oq_thermo_energy": (C.c_double, [vp]),
         "oq_coulomb_energy": (C.c_double, [vp, C.c_double]),
+        "oq_bead_action": (None, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
+        "oq_virial_energy": (C.c_double, [vp, C.c_int]),
         "oq_energy_blocks": (None, [vp, C.c_int, C.c_int, dp]),
         "oq_pair_histogram": (None, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                      P(C.c_uint64)]),
@@ -171,6 +173,15 @@ class Chain:
 
     def coulomb_energy(self, epsilon=1.0):
         return self.lib.oq_coulomb_energy(self.h, epsilon)
+
+    def bead_action(self, i, t):
+        u, ut = C.c_double(), C.c_double()
+        fm, fp = np.zeros(3), np.zeros(3)
+        self.lib.oq_bead_action(self.h, i, t, C.byref(u), C.byref(ut), dptr(fm), dptr(fp))
+        return u.value, ut.value, fm[:self.D], fp[:self.D]
+
+    def virial_energy(self, nwindow=1):
+        return self.lib.oq_virial_energy(self.h, nwindow)
 
     def energy_blocks(self, nblock, nmeasure):
         out = np.zeros(nblock)

@@ -448,19 +448,25 @@ def test_action_difference_mixed(name):
     assert err.max() <= 5e-5
 
 
-def _chain_means(desc, nchain, nequil, nmeasure, seed, beads_fn):
+def _chain_means(desc, nchain, nequil, nmeasure, seed, beads_fn, virial_window=0):
     """Device sampler on nchain independent copies (Philox stream per chain):
     nequil sweeps, then nmeasure sweeps with one energy measurement each;
-    returns the per-chain time averages."""
+    returns the per-chain time averages (and of the virial estimator when a
+    window is given)."""
     b = capi.Batch([desc] * nchain)
     for c in range(nchain):
         b.set_paths(c, beads_fn(c))
     b.seed(seed, 0)
     b.run_sweeps(nequil)
     acc = np.zeros(nchain)
+    vir = np.zeros(nchain)
     for _ in range(nmeasure):
         b.run_sweeps(1)
         acc += b.energy()
+        if virial_window:
+            vir += b.virial_energy(virial_window)
+    if virial_window:
+        return acc / nmeasure, vir / nmeasure
     return acc / nmeasure
 
 
@@ -473,12 +479,16 @@ def test_sampled_sho_energy_matches_analytic():
     standard errors."""
     print("[test] sampled_sho_energy", flush=True)
     d = capi.make_desc(3, 200, 3, 0.05, [(1, 1.0, -1.0), (1, 1.0, -1.0)], sho=[(0, 1.0), (1, 1.0)])
-    means = _chain_means(d, 256, 300, 300, 31, lambda c: capi.initial_beads(d, 7000 + c, replicate=True))
+    means, vmeans = _chain_means(d, 256, 300, 300, 31, lambda c: capi.initial_beads(d, 7000 + c, replicate=True),
+                                 virial_window=10)
     exact = 3.0 / np.tanh(5.0)
-    err = means.std(ddof=1) / np.sqrt(len(means))
-    z = (means.mean() - exact) / err
-    print("E", means.mean(), "+-", err, "exact", exact, "z", z)
-    assert abs(z) < 3
+    for name, m in (("thermo", means), ("virial", vmeans)):
+        err = m.std(ddof=1) / np.sqrt(len(m))
+        z = (m.mean() - exact) / err
+        print(name, "E", m.mean(), "+-", err, "exact", exact, "z", z)
+        assert abs(z) < 3
+    # the windowed virial estimator has the smaller variance
+    assert vmeans.std() < means.std()
 
 
 def test_sampled_fermion_energy_matches_exact_nodes():
@@ -570,3 +580,19 @@ def test_coulomb_energy_estimator():
         for eps in (1.0, 12.5):
             got = b.coulomb_energy(eps)[0]
             assert rel_close(got, oc.coulomb_energy(eps), 1e-12, scale=1e-300), (got, oc.coulomb_energy(eps))
+
+
+def test_virial_energy_estimator():
+    """VirialEnergyEstimator on the device against the oracle for several
+    windows, open and periodic cells, node chains (zero node force) and
+    permuted bosons (1e-11 of the summed scale)."""
+    print("[test] virial energy estimator", flush=True)
+    bos = capi.make_desc(2, 64, 3, 2.0 / 64, [(4, 1.0, 0.0)], sho=[(0, 1.0)], cell=20.0,
+                         samples=[(0, 2, capi.PQ_CHOOSER_WALKING), (0, 1, capi.PQ_CHOOSER_WALKING)])
+    for desc in (_descs()["cfg1"], _descs()["pbc"], small_cfg2(), bos):
+        b, oc = _paired(desc, 4)
+        for nw in (1, 3, 8):
+            got = b.virial_energy(nw)[0]
+            want = oc.virial_energy(nw)
+            assert rel_close(got, want, 1e-11, scale=1.0), (nw, got, want)
+        assert rel_close(b.virial_energy(1)[0], b.energy()[0], 1e-11, scale=1.0)

@@ -79,3 +79,66 @@ def test_coulomb_energy_brute_force_with_images():
                 dv = _pbc(b[t, i] - b[t, j], cell)
                 want += q[i] * q[j] / (2.0 * math.sqrt(np.dot(dv, dv)))
     assert ch.coulomb_energy(2.0) == pytest.approx(want / M, rel=1e-13)
+
+
+def _link_total(ch, t, spring_lam_tau):
+    """U_t: the composite link action into slice t summed over particles
+    (getBeadAction's u: SHO and half of each pair) plus the spring link action
+    |r(t) - r(t-1)|^2 / (4 lambda tau), which SpringAction's getBeadAction
+    leaves out of u."""
+    b = ch.get_beads()
+    M = b.shape[0]
+    tot = 0.0
+    for k in range(b.shape[1]):
+        u, _, _, _ = ch.bead_action(k, t % M)
+        tot += u
+        if spring_lam_tau:
+            prev = b[(t - 1) % M, k]
+            tot += np.dot(b[t % M, k] - prev, b[t % M, k] - prev) / (4 * spring_lam_tau)
+    return tot
+
+
+def test_bead_forces_are_link_action_gradients():
+    """getBeadAction's fm = -dU_t/dr_i(t) (the link into slice t) and
+    fp = -dU_{t+1}/dr_i(t) (the link out of it), for spring + SHO + Coulomb
+    pair action, by central differences (SpringAction.cc:112-123,
+    SHOAction.cc:71-105, PairAction.cc:509-599 with uk0CalcDerivatives)."""
+    d = capi.make_desc(3, 16, 2, 0.05, [(2, 1.0, -1.0), (1, 2.0, 1.0)], sho=[(0, 1.0), (1, 0.6)],
+                       coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024))
+    ch = ol.Chain(d)
+    rng = np.random.default_rng(9)
+    base = rng.standard_normal((1, 3, 3)) * 1.2
+    b = base + 0.2 * rng.standard_normal((16, 3, 3))
+    ch.set_beads(b)
+    h = 1e-6
+    for i, t in ((0, 5), (1, 9), (2, 3)):
+        _, _, fm, fp = ch.bead_action(i, t)
+        for dd in range(3):
+            got = []
+            for sgn in (1, -1):
+                bb = b.copy()
+                bb[t, i, dd] += sgn * h
+                ch.set_beads(bb)
+                got.append((_link_total(ch, t, 0.0), _link_total(ch, t + 1, 0.0)))
+            ch.set_beads(b)
+            # the spring part in closed form of the same link actions
+            lam = 0.5 / (1.0 if i < 2 else 2.0)
+            sm = -(b[t, i, dd] - b[t - 1, i, dd]) / (2 * lam * d.tau)
+            sp = -(b[t, i, dd] - b[t + 1, i, dd]) / (2 * lam * d.tau)
+            want_m = -(got[0][0] - got[1][0]) / (2 * h) + sm
+            want_p = -(got[0][1] - got[1][1]) / (2 * h) + sp
+            assert fm[dd] == pytest.approx(want_m, rel=2e-6, abs=2e-5), (i, t, dd)
+            assert fp[dd] == pytest.approx(want_p, rel=2e-6, abs=2e-5), (i, t, dd)
+
+
+def test_virial_with_unit_window_is_the_thermodynamic_estimator():
+    """With nwindow = 1 every link's window closes on itself: the -F.r/2tau
+    term and the window's closing F.r0/2tau cancel, leaving sum utau / M."""
+    for d in (capi.cfg1_desc(), capi.make_desc(2, 24, 2, 0.05, [(3, 1.0, -1.0)], sho=[(0, 1.0)], cell=3.0,
+                                                coulomb=dict(epsilon=2.0, norder=2, ngpts=300))):
+        ch = ol.Chain(d)
+        ch.set_beads(capi.initial_beads(d, 5, replicate=False))
+        ch.rng_philox(3, 0)
+        ch.run_sweeps(3)
+        assert ch.virial_energy(1) == pytest.approx(ch.energy(), rel=1e-12)
+        assert ch.virial_energy(4) != pytest.approx(ch.energy(), rel=1e-6)
