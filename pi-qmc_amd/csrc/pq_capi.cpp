@@ -481,7 +481,7 @@ static SweepLayoutH layout_for(const pq_batch* b, const Sched& req) {
   lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
   lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
   o = align8(o);
-  lay.off_choose = (int)o; o += 128;   // Eng::ChooseBuf
+  lay.off_choose = (int)o; o += 256;   // Eng::ChooseBuf
   lay.off_perm = (int)o; o += align8(5 * mN * 4);   // perm, inverse, section perm, two scratch
   lay.off_walk = -1;
   if (mWalk) { lay.off_walk = (int)o; o += mWalk * 8; }

@@ -300,6 +300,7 @@ struct Eng {
     double lnTP;              // log transition probability of the permutation
     int ok;
     int pad;
+    double lnT[8];            // the mover's ln(T_old/T_new) of every level of the move
   };
   const DevCtx& x;
   const ChainP& c;
@@ -558,6 +559,13 @@ struct Eng {
   // (RandomNumGenerator.h:75-86), then, for FreeMoverPBC, D image uniforms per
   // moving particle; each lane computes its own from the draw offsets.
   // Returns ln(T_old/T_new).
+  // uniforms the mover draws at a level (FreeMover / FreeMoverPBC)
+  __device__ __forceinline__ int mover_draws(int level) const {
+    const int nmoved = (level < c.L) ? (1 << (c.L - 1 - level)) : 0;
+    const int ng = nmov() * D;
+    return nmoved * (2 * ((ng + 1) / 2) + (c.mover == PQ_MOVER_FREE_PBC ? ng : 0));
+  }
+
   __device__ double mover(int level, uint64_t k0, int* ndraws) {
     const int s = 1 << level;
     const int nmoved = (level < c.L) ? (1 << (c.L - 1 - level)) : 0;
@@ -1271,6 +1279,72 @@ struct Eng {
         const double* od = fn_dist(fa);
         if (top_known) {
           // the last slice keeps its distances
+          for (int k = tid; k < 2 * n; k += NT) nd[(int64_t)(c.S - 1) * 2 * n + k] = od[(int64_t)(c.S - 1) * 2 * n + k];
+          node_distances(fa, 1, c.S - 2, true, nullptr, scale[fa], nd);
+        } else {
+          node_distances(fa, 1, c.S - 1, true, nullptr, scale[fa], nd);
+        }
+        double part = 0;
+        for (int it = tid; it < (c.S - 1) * n; it += NT) {
+          const int t = 1 + it / n, k = it % n;
+#pragma unroll
+          for (int pass = 0; pass < 2; ++pass) {
+            const int64_t a1 = ((int64_t)t * 2 + pass) * n + k;
+            const int64_t a0 = ((int64_t)(t - 1) * 2 + pass) * n + k;
+            part += log((1 - exp(-od[a1] * od[a0])) / (1 - exp(-nd[a1] * nd[a0])));
+          }
+        }
+        dA = bsum(part);
+      }
+      total += dA;
+    }
+    return total;
+  }
+
+  // The sampler proposes every level of a move before trying any of them, so
+  // FixedNodeAction::getActionDifference's determinants of all the slices a
+  // move changes are evaluated here in one pass instead of one pass per
+  // level.  newdm(t) holds what the level that moves slice t evaluates; an
+  // evaluation that errs is stored as 0, which the sign test rejects, as the
+  // reference rejects the error.  The top slice comes from initialize when
+  // no cycle changed it.
+  __device__ void fn_eval_moved() {
+    for (int fa = 0; fa < c.nfn; ++fa) {
+      const FnP& f = c.fn[fa];
+      bool mine = false;
+      for (int m = 0; m < nmov(); ++m) mine |= mv(m) >= f.ifirst && mv(m) < f.ifirst + f.ns;
+      if (tid == 0) notmine[fa] = mine ? 0 : 1;
+      if (!mine) continue;
+      double* ndm = fn_newdm(fa);
+      const int nt = top_known ? c.S - 2 : c.S - 1;
+      node_matrices(fa, 1, 1, nt, true, scale[fa]);
+      for (int m = tid; m < nt; m += NT) ndm[1 + m] = werr[m] ? 0.0 : wdet[m];
+      if (top_known && tid == 0) ndm[c.S - 1] = lasterr[fa] ? 0.0 : fn_dm(fa)[c.S - 1];
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+
+  // FixedNodeAction::getActionDifference (FixedNodeAction.cc:39-131) at one
+  // level from the determinants fn_eval_moved left in newdm; level 0 adds the
+  // nodal action from the distances of every slice.
+  __device__ double fn_diff_moved(int level) {
+    double total = 0;
+    for (int fa = 0; fa < c.nfn; ++fa) {
+      if (notmine[fa]) continue;
+      const FnP& f = c.fn[fa];
+      const int nStride = 1 << (level + 1);
+      const double dm0 = fn_dm(fa)[0];
+      const double* ndm = fn_newdm(fa);
+      bool crossed = false;
+      for (int t = nStride / 2; t < c.S; t += nStride) crossed |= ndm[t] * dm0 <= 1e-200;
+      if (crossed) { total += 2e100; continue; }
+      double dA = 0;
+      if (f.nodal && level == 0) {
+        double* nd = fn_newdist(fa);
+        const int n = f.ns;
+        const double* od = fn_dist(fa);
+        if (top_known) {
           for (int k = tid; k < 2 * n; k += NT) nd[(int64_t)(c.S - 1) * 2 * n + k] = od[(int64_t)(c.S - 1) * 2 * n + k];
           node_distances(fa, 1, c.S - 2, true, nullptr, scale[fa], nd);
         } else {

@@ -93,6 +93,7 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
   if (c.fnc_off >= 0) e.fcache = x.fncache + c.fnc_off;
   e.top_known = true;
   e.lasterr = e.notmine + PQ_MAX_NODES;
+  static_assert(sizeof(typename Eng<D, NC, R, NT, MM>::ChooseBuf) <= 256, "chooser scratch outgrew its LDS slot");
   e.cbuf = reinterpret_cast<typename Eng<D, NC, R, NT, MM>::ChooseBuf*>(base + lay.off_choose);
   if (lay.off_walk >= 0) e.wrows = reinterpret_cast<double*>(base + lay.off_walk);
   // path permutation and inverse, section permutation, scratch
@@ -206,7 +207,22 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
           e.mov[((S - 1) * nmov + m) * D + d] = e.sb(e.sec1, S - 1, e.mv(MM > 1 ? mcyc[m] : m))[d];
         }
         __syncthreads();
-        // MultiLevelSampler::tryMove (MultiLevelSampler.cc:100-177)
+        // MultiLevelSampler::tryMove (MultiLevelSampler.cc:100-177).  Every
+        // level's beads are proposed first: a level's mover only reads the
+        // coarser levels' new beads, and its draws start after the draws and
+        // the accept draw of the levels above, wherever the move stops.  The
+        // node determinants of every changed slice then take one pass.
+        {
+          uint64_t kk = k;
+          for (int l = L; l >= 0; --l) {
+            int nd;
+            const double lnT = e.mover(l, kk, &nd);
+            kk += nd + 1;
+            if (tid == 0) e.cbuf->lnT[l] = lnT;
+            __syncthreads();
+          }
+          if (c.nfn) e.fn_eval_moved();
+        }
         double old = lnTP;
         bool accepted = true;
         for (int l = L; l >= 0; --l) {
@@ -214,14 +230,11 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
             cnt[l]++;
             if (l < L) cnt[16] += (unsigned long long)nmov << (L - 1 - l);
           }
-          int nd;
-          const double lnT = e.mover(l, k, &nd);
-          k += nd;
-          __syncthreads();
+          k += e.mover_draws(l);
           double dA = e.action_diff(l);
-          if (c.nfn) dA += e.fn_diff(l);
+          if (c.nfn) dA += e.fn_diff_moved(l);
           const double piRatio = -dA + old;
-          const double acceptProb = exp(lnT + piRatio);
+          const double acceptProb = exp(e.cbuf->lnT[l] + piRatio);
           if (e.U(k++) > acceptProb) { accepted = false; break; }
           old = dA;
           if (tid == 0) cnt[8 + l]++;
