@@ -77,6 +77,8 @@ typedef struct pq_species_desc {
   int count;
   double mass;
   double charge;
+  int is_static;       /* isStatic="true": SpringAction and the virial estimator skip it
+                          (src/action/SpringAction.cc:69,115, VirialEnergyEstimator.cc:36,51,60,66) */
 } pq_species_desc;
 
 /* <SHOAction omega=".." species=".." x y z ndim> (src/parser/ActionParser.cc:307-324). */
@@ -87,8 +89,11 @@ typedef struct pq_sho_desc {
   int ndim;            /* 0 means the full dimension */
 } pq_sho_desc;
 
-/* <CoulombAction epsilon norder rmin rmax ngridPoints> without Ewald or exchange
-   (src/parser/ActionParser.cc:118-153, src/action/CoulombAction.cc:19-91). */
+/* <CoulombAction epsilon norder rmin rmax ngridPoints useEwald ewaldRcut ewaldKcut>
+   without exchange (src/parser/ActionParser.cc:118-153, src/action/CoulombAction.cc:19-91).
+   useEwald takes the default ewaldType (optEwald) with ewaldNDim = ndim: the pair
+   tables lose tau q1 q2 f(r) from u_0 and level 0 adds the k-space sum's change
+   times tau/epsilon on slices 1..2^L-1 (CoulombAction.cc:28-44,107-119,240-242). */
 typedef struct pq_coulomb_desc {
   int enabled;
   double epsilon;      /* 0 means 1 */
@@ -96,6 +101,9 @@ typedef struct pq_coulomb_desc {
   double rmin;         /* 0 means 0.005/(2 mu |q1 q2|) of the first interacting pair */
   double rmax;         /* 0 means the length of the cell diagonal */
   int ngpts;           /* 0 means 500 */
+  int use_ewald;       /* OptEwaldSum long-range part (src/util/OptEwaldSum.cc:27-173) */
+  double ewald_rcut;   /* 0 means a[0]/2 */
+  double ewald_kcut;   /* k-vectors with 0 < |k| < ewald_kcut */
 } pq_coulomb_desc;
 
 /* <FixedNodeAction species model omega temperature noNodalAction>
@@ -264,13 +272,22 @@ int pq_get_node_scale(pq_batch* batch, int chain, double* scale);
 int pq_run_measured(pq_batch* batch, int nblock, int nsweep, double* energies);
 
 /* Thermodynamic energy estimate of every chain's current paths:
-   E = (1/M) sum_{t,i} utau(i,t) (ThermoEnergyEstimator + CompositeAction::getBeadAction). */
+   E = (1/M) sum_{t,i} utau(i,t) (ThermoEnergyEstimator + CompositeAction::getBeadAction),
+   with CoulombAction's Ewald k-space sum / epsilon of every slice when useEwald is set. */
 int pq_measure_energy(pq_batch* batch, double* energy);
 /* Coulomb potential energy of every chain's current paths
    (<CoulombEnergyEstimator epsilon>, src/estimator/CoulombEnergyEstimator.cc:30-40,
    without Ewald): (1/M) sum_t sum_{j<i} q_i q_j / (epsilon |pbc(r_i - r_j)|);
    epsilon 0 means 1, as the parser's default. */
 int pq_measure_coulomb_energy(pq_batch* batch, double epsilon, double* energy);
+/* Coulomb energy with the optimized Ewald sum (<CoulombEnergyEstimator useEwald="true"
+   rcut kcut epsilon>, ewaldType optEwald: EwaldCoulombEstimator,
+   src/estimator/EwaldCoulombEstimator.cc:63-140, src/parser/EstimatorParser.cc:169-190):
+   per slice, sum_{j<i, r<rcut} q_i q_j (1/(eps r) - f(r)/eps) with f read off a
+   1001-point grid the way the reference reads it, plus the k-space sum / eps;
+   (1/M) over slices.  The fit is built once per (rcut, kcut) and cell. */
+int pq_measure_ewald_coulomb_energy(pq_batch* batch, double epsilon, double rcut, double kcut,
+                                    double* energy);
 /* Virial energy of every chain's current paths (<VirialEnergyEstimator nwindow>,
    src/estimator/VirialEnergyEstimator.cc:27-78): utau plus the centroid-virial
    force terms of getBeadAction over windows of nwindow links (0 means 1, the
@@ -285,8 +302,8 @@ int pq_pair_histogram(pq_batch* batch, int species1, int species2, double rmin,
 
 /* Reference call-level entry points.  A query is one section (S=2^L+1 slices,
    [S][N][D]) with the moving beads ([S][nmov][D]) and their particle indices.
-   dS[q] = CompositeAction::getActionDifference (spring + SHO + Coulomb pair)
-   of query q at level[q] for chain chain[q]. */
+   dS[q] = CompositeAction::getActionDifference (spring + SHO + Coulomb pair, and
+   at level 0 the Ewald k-space term) of query q at level[q] for chain chain[q]. */
 int pq_action_difference(pq_batch* batch, int nquery, const int* chain,
                          const int* level, const double* section,
                          const double* moving, const int* index, int nmov,

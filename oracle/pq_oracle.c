@@ -367,21 +367,359 @@ double oq_coulomb3d(double stau, int which, double reff) {
   return c3d(&c, which, reff);
 }
 
-/* CoulombAction::u (src/action/CoulombAction.cc:209-248), no screening shift */
-static double coulomb_u(double r, int order, double mu, double q1q2, double tau) {
+/* ------------------------------------------------------------ Ewald sum */
+
+/* OptEwaldSum (src/util/OptEwaldSum.cc:27-280) on the k-vectors of
+   EwaldSum (src/util/EwaldSum.cc:16-112): f(r) = 1/r beyond rcut and an even
+   polynomial of npoly = 8 terms inside, fitted over the k-space halo
+   kcut < k < 4 kcut with value and slope pinned at rcut. */
+#define EW_NPOLY 8
+static const double EW_PI = 3.14159265358979;   /* EwaldSum::PI (EwaldSum.cc:187) */
+
+typedef struct {
+  int D, N, nk;
+  double rcut, kcut;
+  double coef[EW_NPOLY];
+  int ikmax[3];
+  double dk[3];          /* 2 PI b_d */
+  int* kv;               /* [nk][3] half-space integer k-vectors */
+  double* vk;            /* [nk] evalFK(|k|) */
+  double oneOver2V, self;
+  double q[MAXN];
+} ewald_t;
+
+/* evalFR (OptEwaldSum.cc:178-190) */
+static double ew_fr(const ewald_t* e, double r) {
+  double v = 0.;
+  if (r > e->rcut) return 1. / r;
+  double r2 = r * r;
+  for (int i = EW_NPOLY - 1; i >= 0; --i) { v *= r2; v += e->coef[i]; }
+  return v;
+}
+
+/* the 2D even-power transforms of OptEwaldSum.cc:92-118 / 206-232: the
+   Bessel series of the n = -1 term (*fm1) and of r^n for even n < 2 npoly */
+static void ew_series2d(double k, double rcut, double* fm1, double* fn_even) {
+  double factor = -0.25 * k * k * rcut * rcut;
+  double term = 2. * EW_PI;
+  double fn = term * rcut;
+  int m = 1;
+  while (fabs(term) > 1e-100) { term *= factor / (m * m); fn += term * rcut / (2 * m + 1); ++m; }
+  *fm1 = fn;
+  double rnplus2 = rcut * rcut;
+  for (int n = 0; n < 2 * EW_NPOLY; n += 2) {
+    double t = 2. * EW_PI;
+    double f = t * rnplus2 / (n + 2);
+    int mm = 1;
+    while (fabs(t) > 1e-100) { t *= factor / (mm * mm); f += t * rnplus2 / (2 * mm + n + 2); ++mm; }
+    fn_even[n / 2] = f;
+    rnplus2 *= rcut * rcut;
+  }
+}
+
+/* the 3D recursion of OptEwaldSum.cc:76-89 / 193-205: temp_n = int_0^rcut
+   r^n e^{ikr} dr by parts; ft[n/2] = 4 PI Im(temp_n)/k for even n, and the
+   n = -1 term 4 PI Im((e^{ik rcut} - 1)/(ik))/k in *fm1 */
+static void ew_series3d(double k, double rcut, double* fm1, double* ft) {
+  double er = cos(k * rcut), ei = sin(k * rcut);
+  /* temp = (eikr - 1)/(ik) = (er - 1 + i ei)(-i/k) */
+  double tr = ei / k, ti = -(er - 1.) / k;
+  *fm1 = 4. * EW_PI * ti / k;
+  double rn = rcut;
+  for (int n = 0; n < 2 * EW_NPOLY - 1; ++n) {
+    /* temp = (rn eikr - (n+1) temp)/(ik) */
+    double nr = rn * er - (n + 1.) * tr, ni = rn * ei - (n + 1.) * ti;
+    tr = ni / k; ti = -nr / k;
+    rn *= rcut;
+    if (n % 2 == 0) ft[n / 2] = 4. * EW_PI * ti / k;
+  }
+}
+
+/* evalFK (OptEwaldSum.cc:192-250), in its own operation order: (4 PI/k) Im */
+static double ew_fk(const ewald_t* e, double k) {
+  double fm1, f[EW_NPOLY];
+  if (e->D == 3) {
+    const double rcut = e->rcut;
+    double v = 4. * EW_PI / (k * k);
+    double er = cos(k * rcut), ei = sin(k * rcut);
+    double tr = ei / k, ti = -(er - 1.) / k;
+    const double fpioverk = 4. * EW_PI / k;
+    v -= fpioverk * ti;
+    double rn = rcut;
+    for (int n = 0; n < 2 * EW_NPOLY - 1; ++n) {
+      double nr = rn * er - (n + 1.) * tr, ni = rn * ei - (n + 1.) * ti;
+      tr = ni / k; ti = -nr / k;
+      rn *= rcut;
+      if (n % 2 == 0) v += e->coef[n / 2] * ti * fpioverk;
+    }
+    return v;
+  }
+  double v = 2. * EW_PI / k;
+  if (k * e->rcut < 30.) {
+    ew_series2d(k, e->rcut, &fm1, f);
+    v -= fm1;
+    for (int i = 0; i < EW_NPOLY; ++i) v += e->coef[i] * f[i];
+  }
+  return v;
+}
+
+/* evalFK0 (OptEwaldSum.cc:252-270) */
+static double ew_fk0(const ewald_t* e) {
+  const double rc = e->rcut;
+  if (e->D == 3) {
+    double fn = 2. * EW_PI * rc * rc, v = -fn;
+    fn *= rc / 3.;
+    for (int n = 0; n < 2 * EW_NPOLY; n += 2) { v += e->coef[n / 2] * fn; fn *= (rc * rc) * (n + 3.) / (n + 5.); }
+    return v;
+  }
+  double fn = 2. * EW_PI * rc, v = -fn;
+  fn *= rc / 2.;
+  for (int n = 0; n < 2 * EW_NPOLY; n += 2) { v += e->coef[n / 2] * fn; fn *= (rc * rc) * (2 + n) / (4 + n); }
+  return v;
+}
+
+/* evalFRcut (OptEwaldSum.cc:272-280) */
+static double ew_frcut(const ewald_t* e, const double* t, int nd) {
+  double v = 0;
+  for (int i = 0; i < EW_NPOLY; ++i) {
+    double x = pow(e->rcut, 2 * i - nd);
+    for (int j = 0; j < nd; ++j) x *= 2 * i - j;
+    v += t[i] * x;
+  }
+  return v;
+}
+
+/* LU with partial pivoting of a row-major n x n matrix in place (dgetrf) and
+   the solve of A x = b with it (dgetrs 'N') */
+static void ew_lu(double* a, int n, int* piv) {
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i) if (fabs(a[i * n + k]) > fabs(a[p * n + k])) p = i;
+    piv[k] = p;
+    if (p != k) for (int j = 0; j < n; ++j) { double t = a[k * n + j]; a[k * n + j] = a[p * n + j]; a[p * n + j] = t; }
+    if (a[k * n + k] == 0) continue;
+    double r = 1.0 / a[k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      double l = a[i * n + k] * r;
+      a[i * n + k] = l;
+      for (int j = k + 1; j < n; ++j) a[i * n + j] -= l * a[k * n + j];
+    }
+  }
+}
+static void ew_lu_solve(const double* a, int n, const int* piv, double* b) {
+  for (int k = 0; k < n; ++k) if (piv[k] != k) { double t = b[k]; b[k] = b[piv[k]]; b[piv[k]] = t; }
+  for (int i = 0; i < n; ++i) for (int j = 0; j < i; ++j) b[i] -= a[i * n + j] * b[j];
+  for (int i = n - 1; i >= 0; --i) {
+    for (int j = i + 1; j < n; ++j) b[i] -= a[i * n + j] * b[j];
+    b[i] /= a[i * n + i];
+  }
+}
+
+/* OptEwaldSum constructor: the constrained least-squares fit
+   (OptEwaldSum.cc:27-173).  The reference hands C-ordered blitz arrays to the
+   column-major LAPACK routines, so they factor and solve the transposes of
+   a and mat; so does this. */
+static void ew_fit(ewald_t* e, const double* a_cell) {
+  const int D = e->D, np = EW_NPOLY;
+  const double kcut = e->kcut, khalo = 4 * e->kcut, rcut = e->rcut;
+  const double kcut2 = kcut * kcut, khalo2 = khalo * khalo;
+  double deltak[3], dk2[3];
+  for (int i = 0; i < D; ++i) { deltak[i] = 2 * EW_PI / a_cell[i]; dk2[i] = deltak[i] * deltak[i]; }
+  double A[EW_NPOLY * EW_NPOLY] = {0}, b[EW_NPOLY] = {0};
+  for (int i = 0; i < khalo / deltak[0]; ++i) {
+    double kx2 = i * i * dk2[0];
+    int idx = (i == 0) ? 1 : 2;
+    for (int j = 0; j < sqrt(khalo2 - kx2) / deltak[1]; ++j) {
+      double kx2ky2 = kx2 + j * j * dk2[1];
+      int idxy = idx * ((j == 0) ? 1 : 2);
+      /* 2D: one pass with k2 = kx2ky2; 3D: the kz column of the octant */
+      const int kzn = (D == 3) ? -1 : 1;
+      for (int k = 0; (D == 3) ? (k < sqrt(khalo2 - kx2ky2) / deltak[2]) : (k < kzn); ++k) {
+        double k2 = (D == 3) ? kx2ky2 + k * k * dk2[2] : kx2ky2;
+        int idegen = (D == 3) ? idxy * ((k == 0) ? 1 : 2) : idxy;
+        if (!(k2 > kcut2)) continue;
+        double kk = sqrt(k2), v, fm1, ft[EW_NPOLY];
+        if (D == 3) {
+          ew_series3d(kk, rcut, &fm1, ft);
+          v = -4. * EW_PI / (kk * kk) + fm1;
+        } else {
+          v = -2. * EW_PI / kk;
+          if (kk * rcut > 30.) {
+            v = 0;
+            for (int p = 0; p < np; ++p) ft[p] = 0.;
+          } else {
+            ew_series2d(kk, rcut, &fm1, ft);
+            v += fm1;
+          }
+        }
+        for (int p = 0; p < np; ++p) {
+          b[p] += idegen * v * ft[p];
+          for (int q = 0; q < np; ++q) A[p * np + q] += idegen * ft[p] * ft[q];
+        }
+      }
+    }
+  }
+  /* the matrix LAPACK sees: A^T */
+  double At[EW_NPOLY * EW_NPOLY];
+  for (int p = 0; p < np; ++p) for (int q = 0; q < np; ++q) At[p * np + q] = A[q * np + p];
+  int piv[EW_NPOLY];
+  ew_lu(At, np, piv);
+  double c[EW_NPOLY], d[EW_NPOLY], t0[EW_NPOLY], tc[EW_NPOLY], td[EW_NPOLY], t[EW_NPOLY];
+  for (int i = 0; i < np; ++i) { c[i] = pow(rcut, 2 * i); d[i] = (2 * i) * pow(rcut, 2 * i - 1); }
+  memcpy(t0, b, sizeof t0);
+  ew_lu_solve(At, np, piv, t0);
+  double f0 = ew_frcut(e, t0, 0), df0 = ew_frcut(e, t0, 1);
+  double lamc = 10 * rcut, lamd = -rcut * rcut;
+  for (int i = 0; i < np; ++i) { tc[i] = b[i] + lamc * c[i]; td[i] = b[i] + lamd * d[i]; }
+  ew_lu_solve(At, np, piv, tc);
+  ew_lu_solve(At, np, piv, td);
+  double mat[4], rhs[2];
+  mat[0] = (ew_frcut(e, tc, 0) - f0) / lamc;   /* mat(0,0) */
+  mat[1] = (ew_frcut(e, td, 0) - f0) / lamd;   /* mat(0,1) */
+  mat[2] = (ew_frcut(e, tc, 1) - df0) / lamc;  /* mat(1,0) */
+  mat[3] = (ew_frcut(e, td, 1) - df0) / lamd;  /* mat(1,1) */
+  rhs[0] = 1. / rcut - f0;
+  rhs[1] = -1. / (rcut * rcut) - df0;
+  double mt[4] = {mat[0], mat[2], mat[1], mat[3]};
+  int piv2[2];
+  ew_lu(mt, 2, piv2);
+  ew_lu_solve(mt, 2, piv2, rhs);
+  lamc = rhs[0]; lamd = rhs[1];
+  for (int i = 0; i < np; ++i) t[i] = b[i] + lamc * c[i] + lamd * d[i];
+  ew_lu_solve(At, np, piv, t);
+  memcpy(e->coef, t, sizeof t);
+}
+
+/* EwaldSum constructor's k-vectors (EwaldSum.cc:16-88), setLongRangeArray and
+   evalSelfEnergy (EwaldSum.cc:90-97,180-185) with the particles' charges, as
+   CoulombAction's constructor leaves them (CoulombAction.cc:28-44) */
+static void ewald_init(ewald_t* e, const cell_t* cell, int N, const double* q,
+                       double rcut, double kcut) {
+  const int D = cell->D;
+  e->D = D; e->N = N;
+  e->rcut = rcut; e->kcut = kcut;
+  for (int i = 0; i < 3; ++i) { e->ikmax[i] = 0; e->dk[i] = 0; }
+  for (int i = 0; i < D; ++i) {
+    e->ikmax[i] = (int)(cell->a[i] * kcut / (2 * EW_PI));
+    e->dk[i] = 2 * EW_PI * cell->b[i];
+  }
+  double vol = 1;
+  for (int i = 0; i < D; ++i) vol *= cell->a[i];
+  e->oneOver2V = 0.5 / vol;
+  ew_fit(e, cell->a);
+  int cap = 64;
+  e->kv = (int*)malloc(sizeof(int) * 3 * cap);
+  e->nk = 0;
+  const int kzmax = (D == 3) ? e->ikmax[2] : 0;
+  for (int kx = 0; kx <= e->ikmax[0]; ++kx) {
+    double kx2 = kx * kx * e->dk[0] * e->dk[0];
+    for (int ky = ((kx == 0) ? 0 : -e->ikmax[1]); ky <= e->ikmax[1]; ++ky) {
+      double ky2 = ky * ky * e->dk[1] * e->dk[1];
+      int kzlo = (D == 3) ? ((kx == 0 && ky == 0) ? 0 : -kzmax) : 0;
+      for (int kz = kzlo; kz <= kzmax; ++kz) {
+        double k2 = (D == 3) ? kx2 + ky2 + kz * kz * e->dk[2] * e->dk[2] : kx2 + ky * ky * e->dk[1] * e->dk[1];
+        if (k2 < kcut * kcut && k2 != 0) {
+          if (e->nk == cap) { cap *= 2; e->kv = (int*)realloc(e->kv, sizeof(int) * 3 * cap); }
+          e->kv[3 * e->nk] = kx; e->kv[3 * e->nk + 1] = ky; e->kv[3 * e->nk + 2] = kz;
+          e->nk++;
+        }
+      }
+    }
+  }
+  e->vk = (double*)malloc(sizeof(double) * (e->nk ? e->nk : 1));
+  for (int k = 0; k < e->nk; ++k) {
+    const int* v = &e->kv[3 * k];
+    /* kvec2 as the constructor accumulates it */
+    double k2 = v[0] * v[0] * e->dk[0] * e->dk[0] + v[1] * v[1] * e->dk[1] * e->dk[1];
+    if (D == 3) k2 += v[2] * v[2] * e->dk[2] * e->dk[2];
+    e->vk[k] = ew_fk(e, sqrt(k2));
+  }
+  double Q = 0, Q2 = 0;
+  for (int i = 0; i < N; ++i) { e->q[i] = q[i]; Q += q[i]; Q2 += q[i] * q[i]; }
+  e->self = -0.5 * Q2 * e->coef[0] + e->oneOver2V * ew_fk0(e) * Q * Q;
+}
+
+static void ewald_free(ewald_t* e) { free(e->kv); free(e->vk); e->kv = 0; e->vk = 0; }
+
+/* EwaldSum::evalLongRange (EwaldSum.cc:114-178): the k-space sum of the
+   configuration r [N][D], with e^{i k r} built by the same recursions */
+static double ewald_long_range(const ewald_t* e, const double* r) {
+  const int D = e->D, N = e->N;
+  const int nx = e->ikmax[0] + 1, ny = 2 * e->ikmax[1] + 1, nz = (D == 3) ? 2 * e->ikmax[2] + 1 : 1;
+  double* ex = (double*)malloc(sizeof(double) * 2 * nx * N);
+  double* ey = (double*)malloc(sizeof(double) * 2 * ny * N);
+  double* ez = (double*)malloc(sizeof(double) * 2 * nz * N);
+#define EX(k, i) (ex + 2 * ((size_t)(k) * N + (i)))
+#define EY(k, i) (ey + 2 * ((size_t)((k) + e->ikmax[1]) * N + (i)))
+#define EZ(k, i) (ez + 2 * ((size_t)((k) + e->ikmax[2]) * N + (i)))
+  for (int i = 0; i < N; ++i) {
+    EX(0, i)[0] = 1; EX(0, i)[1] = 0;
+    EY(0, i)[0] = 1; EY(0, i)[1] = 0;
+    if (D == 3) { EZ(0, i)[0] = 1; EZ(0, i)[1] = 0; }
+  }
+  if (e->ikmax[0] > 0)
+    for (int i = 0; i < N; ++i) { double a = e->dk[0] * r[i * D]; EX(1, i)[0] = cos(a); EX(1, i)[1] = sin(a); }
+  if (e->ikmax[1] > 0)
+    for (int i = 0; i < N; ++i) {
+      double a = e->dk[1] * r[i * D + 1];
+      EY(1, i)[0] = cos(a); EY(1, i)[1] = sin(a);
+      EY(-1, i)[0] = EY(1, i)[0]; EY(-1, i)[1] = -EY(1, i)[1];
+    }
+  if (D == 3 && e->ikmax[2] > 0)
+    for (int i = 0; i < N; ++i) {
+      double a = e->dk[2] * r[i * D + 2];
+      EZ(1, i)[0] = cos(a); EZ(1, i)[1] = sin(a);
+      EZ(-1, i)[0] = EZ(1, i)[0]; EZ(-1, i)[1] = -EZ(1, i)[1];
+    }
+#define CMUL(o, p, q) do { double _r = (p)[0] * (q)[0] - (p)[1] * (q)[1]; \
+                           double _i = (p)[0] * (q)[1] + (p)[1] * (q)[0]; (o)[0] = _r; (o)[1] = _i; } while (0)
+  for (int k = 2; k <= e->ikmax[0]; ++k)
+    for (int i = 0; i < N; ++i) CMUL(EX(k, i), EX(k - 1, i), EX(1, i));
+  for (int k = 2; k <= e->ikmax[1]; ++k)
+    for (int i = 0; i < N; ++i) { CMUL(EY(k, i), EY(k - 1, i), EY(1, i)); CMUL(EY(-k, i), EY(-k + 1, i), EY(-1, i)); }
+  if (D == 3)
+    for (int k = 2; k <= e->ikmax[2]; ++k)
+      for (int i = 0; i < N; ++i) { CMUL(EZ(k, i), EZ(k - 1, i), EZ(1, i)); CMUL(EZ(-k, i), EZ(-k + 1, i), EZ(-1, i)); }
+  double sum = 0;
+  for (int k = 0; k < e->nk; ++k) {
+    const int* v = &e->kv[3 * k];
+    double cs[2] = {0, 0};
+    for (int j = 0; j < N; ++j) {
+      /* q_j * eikx * eiky (* eikz), left to right */
+      double a[2] = {e->q[j] * EX(v[0], j)[0], e->q[j] * EX(v[0], j)[1]}, t[2], u[2];
+      CMUL(t, a, EY(v[1], j));
+      if (D == 3) { CMUL(u, t, EZ(v[2], j)); } else { u[0] = t[0]; u[1] = t[1]; }
+      cs[0] += u[0];
+      cs[1] += u[1];
+    }
+    sum += 2 * e->vk[k] * (cs[0] * cs[0] + cs[1] * cs[1]);
+  }
+#undef CMUL
+#undef EX
+#undef EY
+#undef EZ
+  free(ex); free(ey); free(ez);
+  return sum * e->oneOver2V + e->self;
+}
+
+/* CoulombAction::u (src/action/CoulombAction.cc:209-248), no screening
+   shift; with an Ewald sum u_0 loses tau q1 q2 f(r) (:240-242) */
+static double coulomb_u(double r, int order, double mu, double q1q2, double tau, const ewald_t* ew) {
   r = sqrt(r * r + 0.0);
   double reff = 2.0 * mu * q1q2 * r;
   double stau = q1q2 * sqrt(2.0 * mu * tau);
   c1d_t c;
   c1d_init(&c, stau);
-  return c3d(&c, order, reff);
+  double u = c3d(&c, order, reff);
+  if (order == 0 && ew) u -= tau * q1q2 * ew_fr(ew, r);
+  return u;
 }
 
 /* CoulombAction::utau (src/action/CoulombAction.cc:250-259) */
-static double coulomb_utau(double r, int order, double mu, double q1q2, double tau) {
+static double coulomb_utau(double r, int order, double mu, double q1q2, double tau, const ewald_t* ew) {
   const double eps = 1e-7;
-  double up = coulomb_u(r, order, mu, q1q2, tau * (1.0 + eps));
-  double um = coulomb_u(r, order, mu, q1q2, tau * (1.0 - eps));
+  double up = coulomb_u(r, order, mu, q1q2, tau * (1.0 + eps), ew);
+  double um = coulomb_u(r, order, mu, q1q2, tau * (1.0 - eps), ew);
   return (up - um) / (2.0 * eps * tau);
 }
 
@@ -419,6 +757,10 @@ struct oq_chain {
   double* beads;
   int npair;
   pair_t pair[16];
+  int has_ewald;                   /* CoulombAction useEwald (optEwald) */
+  ewald_t ew;
+  double ceps;                     /* CoulombAction epsilon */
+  double* ewr;                     /* [N][D] configuration scratch of the Ewald sum */
   int nfn;
   fn_t fn[PQ_MAX_NODES];
   double *sec1, *sec2, *mov;
@@ -499,8 +841,19 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
   /* CoulombAction constructor (src/action/CoulombAction.cc:45-90): one
      PairAction per interacting species pair; rmin/rmax/ngpts defaults are
      written back and inherited by later pairs. */
+  ch->ewr = (double*)calloc((size_t)ch->N * ch->D, sizeof(double));
   if (desc->coulomb.enabled) {
     double eps = desc->coulomb.epsilon == 0 ? 1 : desc->coulomb.epsilon;
+    ch->ceps = eps;
+    /* the Ewald sum exists before the pair tables that subtract f(r)
+       (CoulombAction.cc:28-44); its charges are the particles' own */
+    if (desc->coulomb.use_ewald && (ch->D == 2 || ch->D == 3)) {
+      double rc = desc->coulomb.ewald_rcut, q[MAXN];
+      if (rc == 0.) rc = desc->cell[0] / 2.;
+      for (int i = 0; i < ch->N; ++i) q[i] = desc->species[ch->spec[i]].charge;
+      ewald_init(&ch->ew, &ch->cell, ch->N, q, rc, desc->coulomb.ewald_kcut);
+      ch->has_ewald = 1;
+    }
     double rmin = desc->coulomb.rmin, rmax = desc->coulomb.rmax;
     int ngpts = desc->coulomb.ngpts;
     for (int i = 0; i < ch->nsp; ++i)
@@ -529,8 +882,8 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
           for (int ig = 0; ig < ngpts; ++ig) {
             double r = (1. / p->rgridinv) * exp(ig / p->logrratioinv);
             for (int k = 0; k < K1; ++k) {
-              p->u[ig * K1 + k] = coulomb_u(r, k, mu, q1q2, ch->tau);
-              p->ut[ig * K1 + k] = coulomb_utau(r, k, mu, q1q2, ch->tau);
+              p->u[ig * K1 + k] = coulomb_u(r, k, mu, q1q2, ch->tau, ch->has_ewald ? &ch->ew : NULL);
+              p->ut[ig * K1 + k] = coulomb_utau(r, k, mu, q1q2, ch->tau, ch->has_ewald ? &ch->ew : NULL);
             }
           }
         }
@@ -588,6 +941,8 @@ void oq_destroy(oq_chain* ch) {
   free(ch->spg); free(ch->mpg); free(ch->spg_on); free(ch->mpg_on);
   free(ch->beads); free(ch->sec1); free(ch->sec2); free(ch->mov);
   free(ch->wt); free(ch->wcump); free(ch->wrow);
+  if (ch->has_ewald) ewald_free(&ch->ew);
+  free(ch->ewr);
   free(ch);
 }
 
@@ -655,6 +1010,7 @@ static double spring_diff(oq_chain* ch, const double* sec, const double* mov,
   for (int t = s; t < ch->S; t += s) {
     for (int m = 0; m < nmov; ++m) {
       int i = idx[m], sp = ch->spec[i];
+      if (ch->d.species[sp].is_static) continue;
       double inv2Sigma2 = 0.25 / (ch->lambda[i] * ch->tau * s);
       double delta[3];
       for (int d = 0; d < D; ++d) delta[d] = MOV(mov, t, m, d) - MOV(mov, t - s, m, d);
@@ -831,9 +1187,23 @@ double oq_action_difference(oq_chain* ch, const double* sec, const double* mov,
   if ((mask & 1) && ch->d.spring) diff += spring_diff(ch, sec, mov, idx, nmov, level);
   if (mask & 2)
     for (int a = 0; a < ch->d.nsho; ++a) diff += sho_diff(ch, &ch->d.sho[a], sec, mov, idx, nmov, level);
-  if ((mask & 4) && ch->npair) {
+  if ((mask & 12) && (ch->npair || ch->has_ewald)) {
     double u = 0;
-    for (int p = 0; p < ch->npair; ++p) u += pair_diff(ch, &ch->pair[p], sec, mov, idx, nmov, level);
+    if (mask & 4)
+      for (int p = 0; p < ch->npair; ++p) u += pair_diff(ch, &ch->pair[p], sec, mov, idx, nmov, level);
+    /* the long-range Ewald action at the lowest level, slices 1..2^L-1
+       (CoulombAction.cc:107-119); bit 4 of mask takes CoulombAction whole,
+       bit 8 alone its Ewald part */
+    if (ch->has_ewald && level == 0 && (mask & 12)) {
+      const size_t sl = (size_t)ch->N * ch->D;
+      for (int t = 1; t < ch->S - 1; ++t) {
+        memcpy(ch->ewr, &sec[t * sl], sl * sizeof(double));
+        u -= ewald_long_range(&ch->ew, ch->ewr) * ch->tau / ch->ceps;
+        for (int m = 0; m < nmov; ++m)
+          for (int d = 0; d < ch->D; ++d) ch->ewr[idx[m] * ch->D + d] = MOV(mov, t, m, d);
+        u += ewald_long_range(&ch->ew, ch->ewr) * ch->tau / ch->ceps;
+      }
+    }
     diff += u;
   }
   return diff;
@@ -848,13 +1218,18 @@ static const double* bead(const oq_chain* ch, int i, int t) {
   return &ch->beads[((size_t)t * ch->N + i) * ch->D];
 }
 
+/* EwaldSum::evalLongRange of path slice 0 <= t < M (Paths::getSlice) */
+static double slice_long_range(const oq_chain* ch, int t) {
+  return ewald_long_range(&ch->ew, &ch->beads[(size_t)t * ch->N * ch->D]);
+}
+
 /* getBeadAction utau of SpringAction (SpringAction.cc:112-123), SHOAction
    (SHOAction.cc:71-105), PairAction (PairAction.cc:509-576), summed as
    CompositeAction::getBeadAction (CompositeAction.cc:57-67). */
 double oq_bead_utau(const oq_chain* ch, int ipart, int islice, int mask) {
   const int D = ch->D;
   double utau = 0;
-  if ((mask & 1) && ch->d.spring) {
+  if ((mask & 1) && ch->d.spring && !ch->d.species[ch->spec[ipart]].is_static) {
     double delta[3];
     const double *a = bead(ch, ipart, islice), *b = bead(ch, ipart, islice - 1);
     for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
@@ -887,7 +1262,7 @@ double oq_bead_utau(const oq_chain* ch, int ipart, int islice, int mask) {
       utau += ui;
     }
   }
-  if ((mask & 4) && ch->npair) {
+  if ((mask & 4) && (ch->npair || ch->has_ewald)) {
     double uc = 0;
     for (int pp = 0; pp < ch->npair; ++pp) {
       const pair_t* p = &ch->pair[pp];
@@ -918,6 +1293,8 @@ double oq_bead_utau(const oq_chain* ch, int ipart, int islice, int mask) {
       }
       uc += up;
     }
+    /* the long-range part rides on particle 0 (CoulombAction.cc:164-170) */
+    if (ch->has_ewald && ipart == 0) uc += slice_long_range(ch, islice) / ch->ceps;
     utau += uc;
   }
   return utau;
@@ -1654,7 +2031,7 @@ void oq_bead_action(const oq_chain* ch, int ipart, int islice, double* u_out, do
   for (int d = 0; d < D; ++d) { dm[d] = b1[d] - b0[d]; dp[d] = b1[d] - b2[d]; }
   pbc(&ch->cell, dm);
   pbc(&ch->cell, dp);
-  if (ch->d.spring) {
+  if (ch->d.spring && !ch->d.species[ch->spec[ipart]].is_static) {
     const double lam = ch->lambda[ipart], tau = ch->tau;
     for (int d = 0; d < D; ++d) { fm[d] -= dm[d] / (2 * lam * tau); fp[d] -= dp[d] / (2 * lam * tau); }
     utau += D / (2 * tau) - dot3(dm, dm, D) / (4.0 * lam * tau * tau);
@@ -1722,12 +2099,19 @@ void oq_bead_action(const oq_chain* ch, int ipart, int islice, double* u_out, do
         fp[d] -= vq * delta[d] / (2 * r) + vs2 * (2 * svec[d] / (q * q) - s2 * delta[d] / (q * r));
     }
   }
+  /* CoulombAction's long-range part: u and utau of particle 0, no force
+     (CoulombAction.cc:164-170) */
+  if (ch->has_ewald && ipart == 0) {
+    const double lr = slice_long_range(ch, islice) / ch->ceps;
+    u += lr * ch->tau;
+    utau += lr;
+  }
   *u_out = u;
   *utau_out = utau;
 }
 
 /* VirialEnergyEstimator (src/estimator/VirialEnergyEstimator.cc:27-78) over
-   SerialPaths::sumOverLinks with firstSlice 0 and no static particles; the
+   SerialPaths::sumOverLinks with firstSlice 0, static particles skipped; the
    FixedNodeActions' getBeadAction gives zero forces and, without
    useDistDerivative, zero utau (FixedNodeAction.cc:254-329), so they add
    nothing.  Returns the energy of the current paths (energy / nslice). */
@@ -1742,7 +2126,8 @@ double oq_virial_energy(const oq_chain* ch, int nwindow) {
   for (int t = 0; t < M; ++t)
     for (int i = 0; i < N; ++i) {
       double* R0 = &r0[i * 3]; double* RA = &rav[i * 3]; double* FA = &fav[i * 3];
-      if (t % nwindow == 0) {
+      const int stat = ch->d.species[ch->spec[i]].is_static;
+      if (t % nwindow == 0 && !stat) {
         energy += (1 / (2 * tau)) * dot3(FA, R0, D);
         for (int d = 0; d < 3; ++d) R0[d] = RA[d] = FA[d] = 0;
       }
@@ -1759,9 +2144,10 @@ double oq_virial_energy(const oq_chain* ch, int nwindow) {
       oq_bead_action(ch, i, t, &u, &utau, fm, fp);
       for (int d = 0; d < D; ++d) { f[d] = fm[d] + fp[d]; FA[d] += f[d]; }
       energy += utau;
-      energy += -(1. - 1. / W) * (D / (2 * tau)) - (1 / (2 * tau)) * dot3(f, RA, D);
+      if (!stat) energy += -(1. - 1. / W) * (D / (2 * tau)) - (1 / (2 * tau)) * dot3(f, RA, D);
     }
-  for (int i = 0; i < N; ++i) energy += (1 / (2 * tau)) * dot3(&fav[i * 3], &r0[i * 3], D);
+  for (int i = 0; i < N; ++i)
+    if (!ch->d.species[ch->spec[i]].is_static) energy += (1 / (2 * tau)) * dot3(&fav[i * 3], &r0[i * 3], D);
   free(r0); free(rav); free(fav);
   return energy / M;
 }
@@ -1788,6 +2174,73 @@ double oq_coulomb_energy(const oq_chain* ch, double epsilon) {
       value += energy;
     }
   return value / ch->M;
+}
+
+/* EwaldCoulombEstimator with its OptEwaldSum (src/estimator/EwaldCoulombEstimator.cc:63-140,
+   CoulombEnergyEstimator useEwald="true", EstimatorParser.cc:178-190): the
+   short-range part q_i q_j (1/(r eps) - f(r)/eps) inside rcut from a 1001-point
+   grid of -f/eps, whose interpolation weight is r - ig dr (not divided by dr,
+   as the reference has it), and the k-space sum on particle 0 of every slice */
+double oq_ewald_coulomb_energy(const oq_chain* ch, double epsilon, double rcut, double kcut) {
+  const int D = ch->D, N = ch->N;
+  if (epsilon == 0) epsilon = 1;
+  double q[MAXN];
+  for (int i = 0; i < N; ++i) q[i] = ch->d.species[ch->spec[i]].charge;
+  ewald_t e;
+  ewald_init(&e, &ch->cell, N, q, rcut, kcut);
+  const int nradial = 1001;
+  const double dr = rcut / 1000, drinv = 1. / dr;
+  double* vgrid = (double*)malloc(sizeof(double) * nradial);
+  vgrid[0] = -e.coef[0] / epsilon;
+  for (int i = 1; i < nradial; ++i) vgrid[i] = -ew_fr(&e, i * dr) / epsilon;
+  double value = 0;
+  for (int t = 0; t < ch->M; ++t)
+    for (int i = 0; i < N; ++i) {
+      double energy = 0.0;
+      const double* end = bead(ch, i, t);
+      for (int j = 0; j < i; ++j) {
+        double delta[3];
+        const double* rj = bead(ch, j, t);
+        for (int d = 0; d < D; ++d) delta[d] = end[d] - rj[d];
+        pbc(&ch->cell, delta);
+        double r = sqrt(dot3(delta, delta, D));
+        if (r < rcut) {
+          int ig = (int)(r * drinv);
+          double x = r - ig * dr;
+          energy += q[i] * q[j] * (1. / (r * epsilon) + (1 - x) * vgrid[ig] + x * vgrid[ig + 1]);
+        }
+      }
+      if (i == 0) energy += ewald_long_range(&e, &ch->beads[(size_t)t * N * D]) / epsilon;
+      value += energy;
+    }
+  free(vgrid);
+  ewald_free(&e);
+  return value / ch->M;
+}
+
+/* the OptEwaldSum of a chain's CoulombAction: its fitted coefficients
+   coef[8], the number of k-vectors, the self energy and, for 0 <= ik < nk,
+   kv[3] and vk; returns 0 without an Ewald sum */
+int oq_ewald_info(const oq_chain* ch, double* coef, int* nk, double* self, int ik, int* kv, double* vk) {
+  if (!ch->has_ewald) return 0;
+  if (coef) memcpy(coef, ch->ew.coef, sizeof ch->ew.coef);
+  if (nk) *nk = ch->ew.nk;
+  if (self) *self = ch->ew.self;
+  if (ik >= 0 && ik < ch->ew.nk) {
+    if (kv) for (int d = 0; d < 3; ++d) kv[d] = ch->ew.kv[3 * ik + d];
+    if (vk) *vk = ch->ew.vk[ik];
+  }
+  return 1;
+}
+
+/* EwaldSum::evalLongRange of a configuration r [N][D] with the chain's sum */
+double oq_ewald_long_range(const oq_chain* ch, const double* r) {
+  return ch->has_ewald ? ewald_long_range(&ch->ew, r) : 0.0;
+}
+
+/* the f(r) of the chain's OptEwaldSum (OptEwaldSum.cc:178-190) */
+double oq_ewald_fr(const oq_chain* ch, double r) {
+  return ch->has_ewald ? ew_fr(&ch->ew, r) : 0.0;
 }
 
 /* PairCFEstimator<1>::handleLink with PairRadial, two species

@@ -52,7 +52,8 @@ void oq_rng_inject(oq_chain* ch, const double* u, int64_t n);
 double oq_rng_next(oq_chain* ch);
 uint64_t oq_rng_count(const oq_chain* ch);
 
-/* action layer on explicit sections.  mask bit0 spring, bit1 SHO, bit2 pair */
+/* action layer on explicit sections.  mask bit0 spring, bit1 SHO, bit2
+   CoulombAction (pair tables and the Ewald term), bit3 the Ewald term alone */
 double oq_action_difference(oq_chain* ch, const double* sec, const double* mov,
                             const int* idx, int nmov, int level, int mask);
 double oq_bead_utau(const oq_chain* ch, int ipart, int islice, int mask);
@@ -77,6 +78,12 @@ void oq_run_sweeps(oq_chain* ch, int nsweep);
 void oq_get_stats(const oq_chain* ch, pq_chain_stats* st);
 double oq_thermo_energy(const oq_chain* ch);
 double oq_coulomb_energy(const oq_chain* ch, double epsilon);
+/* CoulombEnergyEstimator useEwald="true" (optEwald) with its own rcut, kcut */
+double oq_ewald_coulomb_energy(const oq_chain* ch, double epsilon, double rcut, double kcut);
+/* the chain's CoulombAction Ewald sum: fit, k-vector ik, self energy */
+int oq_ewald_info(const oq_chain* ch, double* coef, int* nk, double* self, int ik, int* kv, double* vk);
+double oq_ewald_long_range(const oq_chain* ch, const double* r);
+double oq_ewald_fr(const oq_chain* ch, double r);
 /* getBeadAction of the composite action: u, utau, forces fm/fp [D] */
 void oq_bead_action(const oq_chain* ch, int ipart, int islice, double* u, double* utau,
                     double* fm, double* fp);

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/piqmc_hip.h"
+#include "pq_ewald.h"
 #include "pq_launch.h"
 #include "pq_params.h"
 
@@ -137,18 +138,21 @@ double coulomb3d(const Fit1D& F, int order, double r) {
   }
 }
 
-// CoulombAction::u and ::utau (src/action/CoulombAction.cc:209-259)
-double coulomb_u(double r, int order, double mu, double q1q2, double tau) {
+// CoulombAction::u and ::utau (src/action/CoulombAction.cc:209-259); with an
+// Ewald sum u_0 gives up tau q1 q2 f(r), which the k-space sum restores
+double coulomb_u(double r, int order, double mu, double q1q2, double tau, const EwaldFit* ew) {
   r = std::sqrt(r * r + 0.0);
   const double reff = 2.0 * mu * q1q2 * r;
   Fit1D F(q1q2 * std::sqrt(2.0 * mu * tau));
-  return coulomb3d(F, order, reff);
+  double u = coulomb3d(F, order, reff);
+  if (order == 0 && ew) u -= tau * q1q2 * ew->fr(r);
+  return u;
 }
 
-double coulomb_utau(double r, int order, double mu, double q1q2, double tau) {
+double coulomb_utau(double r, int order, double mu, double q1q2, double tau, const EwaldFit* ew) {
   const double eps = 1e-7;
-  const double up = coulomb_u(r, order, mu, q1q2, tau * (1.0 + eps));
-  const double um = coulomb_u(r, order, mu, q1q2, tau * (1.0 - eps));
+  const double up = coulomb_u(r, order, mu, q1q2, tau * (1.0 + eps), ew);
+  const double um = coulomb_u(r, order, mu, q1q2, tau * (1.0 - eps), ew);
   return (up - um) / (2.0 * eps * tau);
 }
 
@@ -178,6 +182,16 @@ struct pq_batch {
   std::vector<ChainP> hc;
   std::vector<TableP> ht;
   std::vector<double> hu, hut;
+  // Ewald sums of the chains' CoulombActions: fits and the k pool [nk][4]
+  std::vector<EwaldFit> fits;
+  std::map<std::tuple<int, double, double, double, double, double>, int> fit_of;
+  std::vector<int64_t> fit_off;
+  std::vector<double> hk;
+  double* dewk = nullptr;
+  // the EwaldCoulombEstimator's sums, kept for repeated (rcut, kcut, epsilon)
+  double est_rcut = -1, est_kcut = -1;
+  EwP* dew_est = nullptr;
+  double* dewk_est = nullptr;
   std::vector<int> order;
   int64_t nbeads = 0;
   SweepLayoutH lay{};
@@ -243,7 +257,40 @@ static DevCtx make_ctx(const pq_batch* b) {
   x.first_stream = b->first_stream;
   x.order = b->dorder;
   x.perm = b->dperm;
+  x.ewk = b->dewk;
   return x;
+}
+
+// The OptEwaldSum of a cell, rcut and kcut, built once per batch: its index
+// in b->fits (its k-vectors at b->fit_off[index] of the k pool), -1 when the
+// fit is not finite.
+static int ewald_fit(pq_batch* b, int D, const double* a, double rcut, double kcut) {
+  const auto key = std::make_tuple(D, a[0], a[1], D == 3 ? a[2] : 0.0, rcut, kcut);
+  auto it = b->fit_of.find(key);
+  if (it != b->fit_of.end()) return it->second;
+  EwaldFit e;
+  if (!ewald_build(e, D, a, rcut, kcut)) return -1;
+  const int id = (int)b->fits.size();
+  b->fit_off.push_back((int64_t)b->hk.size());
+  for (int k = 0; k < e.nk(); ++k) {
+    for (int d = 0; d < 3; ++d) b->hk.push_back(e.kvec[3 * k + d]);
+    b->hk.push_back(2 * e.vk[k] * e.oneOver2V);
+  }
+  b->fits.push_back(std::move(e));
+  b->fit_of[key] = id;
+  return id;
+}
+
+static void ewald_params(const pq_batch* b, int id, const pq_chain_desc& d, int N, EwP& p) {
+  const EwaldFit& e = b->fits[id];
+  std::vector<double> q;
+  for (int s = 0; s < d.nspecies; ++s)
+    for (int i = 0; i < d.species[s].count; ++i) q.push_back(d.species[s].charge);
+  p.nk = e.nk();
+  p.off = b->fit_off[id];
+  p.self = e.self_energy(q.data(), N);
+  p.rcut = e.rcut;
+  for (int i = 0; i < kEwaldPoly; ++i) p.coef[i] = e.coef[i];
 }
 
 static int scratch(pq_batch* b, size_t bytes, void** p) {
@@ -259,7 +306,7 @@ static int scratch(pq_batch* b, size_t bytes, void** p) {
 }
 
 static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
-                       std::map<std::tuple<double, double, double, double, double, int, int>, int>& tabs) {
+                       std::map<std::tuple<double, double, double, double, double, int, int, int>, int>& tabs) {
   std::memset(&c, 0, sizeof c);
   if (d.ndim != 2 && d.ndim != 3) return fail(PQ_ERR_ARG, "ndim must be 2 or 3 (got %d)", d.ndim);
   if (d.nspecies < 1 || d.nspecies > PQ_MAX_SPECIES) return fail(PQ_ERR_ARG, "nspecies out of range");
@@ -275,6 +322,7 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     c.count[s] = d.species[s].count;
     c.mass[s] = d.species[s].mass;
     c.charge[s] = d.species[s].charge;
+    c.stat[s] = d.species[s].is_static ? 1 : 0;
     c.lambda[s] = 0.5 / d.species[s].mass;
     n += d.species[s].count;
   }
@@ -318,6 +366,18 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
   if (d.coulomb.enabled) {
     if (d.coulomb.norder < 0 || d.coulomb.norder > PQ_MAX_NORDER) return fail(PQ_ERR_ARG, "norder out of range");
     const double eps = d.coulomb.epsilon == 0 ? 1 : d.coulomb.epsilon;
+    c.ceps = eps;
+    // the Ewald sum comes first: the tables subtract its f(r)
+    // (CoulombAction.cc:28-44, ewaldType optEwald with ewaldNDim = NDIM)
+    int fit = -1;
+    if (d.coulomb.use_ewald) {
+      const double rc = d.coulomb.ewald_rcut == 0. ? d.cell[0] / 2. : d.coulomb.ewald_rcut;
+      if (!(rc > 0) || !(d.coulomb.ewald_kcut > 0)) return fail(PQ_ERR_ARG, "Ewald sum needs rcut > 0 and kcut > 0");
+      fit = ewald_fit(b, c.D, d.cell, rc, d.coulomb.ewald_kcut);
+      if (fit < 0) return fail(PQ_ERR_ARG, "the Ewald fit for rcut %g, kcut %g is not finite", rc, d.coulomb.ewald_kcut);
+      ewald_params(b, fit, d, c.N, c.ew);
+    }
+    const EwaldFit* ewp = fit >= 0 ? &b->fits[fit] : nullptr;
     double rmin = d.coulomb.rmin, rmax = d.coulomb.rmax;
     int ngpts = d.coulomb.ngpts;
     for (int i = 0; i < c.nsp; ++i)
@@ -335,7 +395,7 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
         if (ngpts < 2) return fail(PQ_ERR_ARG, "ngridPoints must be at least 2");
         const int norder = (mu > 500) ? 0 : d.coulomb.norder;
         if (c.npair >= PQ_MAX_PAIRS) return fail(PQ_ERR_ARG, "too many pair actions");
-        auto key = std::make_tuple(mu, q1q2, c.tau, rmin, rmax, ngpts, norder);
+        auto key = std::make_tuple(mu, q1q2, c.tau, rmin, rmax, ngpts, norder, fit);
         auto it = tabs.find(key);
         int tid;
         if (it == tabs.end()) {
@@ -351,8 +411,8 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
           for (int ig = 0; ig < ngpts; ++ig) {
             const double r = (1. / t.rgridinv) * std::exp(ig / t.logrratioinv);
             for (int k = 0; k < t.K1; ++k) {
-              b->hu[t.off + (int64_t)ig * t.K1 + k] = coulomb_u(r, k, mu, q1q2, c.tau);
-              b->hut[t.off + (int64_t)ig * t.K1 + k] = coulomb_utau(r, k, mu, q1q2, c.tau);
+              b->hu[t.off + (int64_t)ig * t.K1 + k] = coulomb_u(r, k, mu, q1q2, c.tau, ewp);
+              b->hut[t.off + (int64_t)ig * t.K1 + k] = coulomb_utau(r, k, mu, q1q2, c.tau, ewp);
             }
           }
           tid = (int)b->ht.size();
@@ -549,7 +609,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   b->descs.assign(chains, chains + nchain);
   b->hc.resize(nchain);
   b->D = chains[0].ndim;
-  std::map<std::tuple<double, double, double, double, double, int, int>, int> tabs;
+  std::map<std::tuple<double, double, double, double, double, int, int, int>, int> tabs;
   int64_t beads = 0, inv = 0, fnst = 0, egr = 0, fnc = 0;
   int ns_max = 1;
   for (int c = 0; c < nchain; ++c) {
@@ -623,7 +683,8 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
             dalloc((void**)&b->dcounter, sizeof(uint64_t) * nchain) &&
             dalloc((void**)&b->dstats, sizeof(pq_chain_stats) * nchain) &&
             dalloc((void**)&b->dorder, sizeof(int) * nchain) &&
-            dalloc((void**)&b->dperm, sizeof(int) * b->nperm);
+            dalloc((void**)&b->dperm, sizeof(int) * b->nperm) &&
+            dalloc((void**)&b->dewk, sizeof(double) * b->hk.size());
   if (!ok) return cleanup_fail(fail(PQ_ERR_NOMEM, "device allocation failed"));
   b->nfncache = fnc;
   std::vector<float> uf(b->hu.begin(), b->hu.end());
@@ -636,6 +697,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
       (b->hu.size() && hipMemcpy(b->du, b->hu.data(), sizeof(double) * b->hu.size(), hipMemcpyHostToDevice) != hipSuccess) ||
       (b->hut.size() && hipMemcpy(b->dut, b->hut.data(), sizeof(double) * b->hut.size(), hipMemcpyHostToDevice) != hipSuccess) ||
       (uf.size() && hipMemcpy(b->duf, uf.data(), sizeof(float) * uf.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      (b->hk.size() && hipMemcpy(b->dewk, b->hk.data(), sizeof(double) * b->hk.size(), hipMemcpyHostToDevice) != hipSuccess) ||
       hipMemcpy(b->dscale, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->dorder, b->order.data(), sizeof(int) * nchain, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->dperm, ident.data(), sizeof(int) * ident.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -655,7 +717,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache, b->dfnsweep,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr,
-                  b->dperm};
+                  b->dperm, b->dewk, b->dew_est, b->dewk_est};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -1215,6 +1277,64 @@ extern "C" int pq_measure_coulomb_energy(pq_batch* b, double epsilon, double* en
   if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
   HIPCHK(launch_coulomb_energy(b->D, x, b->nchain, epsilon, (double*)p, b->stream));
+  HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_measure_ewald_coulomb_energy(pq_batch* b, double epsilon, double rcut, double kcut,
+                                               double* energy) {
+  if (!b || !energy || !(rcut > 0) || !(kcut > 0))
+    return fail(PQ_ERR_ARG, "pq_measure_ewald_coulomb_energy: bad arguments (rcut and kcut must be positive)");
+  if (epsilon == 0) epsilon = 1;
+  HIPCHK(hipSetDevice(b->device));
+  if (rcut != b->est_rcut || kcut != b->est_kcut) {
+    // one OptEwaldSum per distinct cell of the batch, charges per chain
+    std::map<std::tuple<double, double, double>, std::pair<EwaldFit, int64_t>> fits;
+    std::vector<double> pool;
+    std::vector<EwP> ew(b->nchain);
+    for (int c = 0; c < b->nchain; ++c) {
+      const pq_chain_desc& d = b->descs[c];
+      const auto key = std::make_tuple(d.cell[0], d.cell[1], b->D == 3 ? d.cell[2] : 0.0);
+      auto it = fits.find(key);
+      if (it == fits.end()) {
+        EwaldFit e;
+        if (!ewald_build(e, b->D, d.cell, rcut, kcut))
+          return fail(PQ_ERR_ARG, "the Ewald fit for rcut %g, kcut %g is not finite", rcut, kcut);
+        const int64_t off = (int64_t)pool.size();
+        for (int k = 0; k < e.nk(); ++k) {
+          for (int q = 0; q < 3; ++q) pool.push_back(e.kvec[3 * k + q]);
+          pool.push_back(2 * e.vk[k] * e.oneOver2V);
+        }
+        it = fits.emplace(key, std::make_pair(std::move(e), off)).first;
+      }
+      const EwaldFit& e = it->second.first;
+      std::vector<double> q;
+      for (int s2 = 0; s2 < d.nspecies; ++s2)
+        for (int i = 0; i < d.species[s2].count; ++i) q.push_back(d.species[s2].charge);
+      EwP& p = ew[c];
+      p.nk = e.nk();
+      p.off = it->second.second;
+      p.self = e.self_energy(q.data(), (int)q.size());
+      p.rcut = rcut;
+      for (int i = 0; i < kEwaldPoly; ++i) p.coef[i] = e.coef[i];
+    }
+    if (b->dew_est) (void)hipFree(b->dew_est);
+    if (b->dewk_est) (void)hipFree(b->dewk_est);
+    b->dew_est = nullptr;
+    b->dewk_est = nullptr;
+    b->est_rcut = b->est_kcut = -1;
+    HIPCHK(hipMalloc(&b->dew_est, sizeof(EwP) * b->nchain));
+    HIPCHK(hipMalloc(&b->dewk_est, sizeof(double) * std::max<size_t>(1, pool.size())));
+    HIPCHK(hipMemcpy(b->dew_est, ew.data(), sizeof(EwP) * b->nchain, hipMemcpyHostToDevice));
+    if (pool.size()) HIPCHK(hipMemcpy(b->dewk_est, pool.data(), sizeof(double) * pool.size(), hipMemcpyHostToDevice));
+    b->est_rcut = rcut;
+    b->est_kcut = kcut;
+  }
+  void* p;
+  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_ewald_coulomb_energy(b->D, x, b->nchain, b->dew_est, b->dewk_est, epsilon, (double*)p, b->stream));
   HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;

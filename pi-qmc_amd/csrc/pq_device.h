@@ -43,6 +43,7 @@ struct DevCtx {
   uint32_t first_stream;
   const int* order;       // workgroup -> chain
   int* perm;              // per-chain path permutation then its inverse, [2][N]
+  const double* ewk;      // Ewald k-vector pool, [nk][4] per sum (EwP)
 };
 
 // ------------------------------------------------------------ wave ops
@@ -207,6 +208,35 @@ __device__ __forceinline__ int species_of(const ChainP& c, int i) {
   int s = 0;
   for (int k = 1; k < c.nsp; ++k) s = (i >= c.ifirst[k]) ? k : s;
   return s;
+}
+
+// (cos, sin) of the phase k.r of one particle for the k-vector kv
+template <int D>
+__device__ __forceinline__ void ewald_phase(const double* kv, const double* r, double& co, double& si) {
+  double ph = kv[0] * r[0] + kv[1] * r[1];
+  if constexpr (D == 3) ph += kv[2] * r[2];
+  sincos(ph, &si, &co);
+}
+
+// EwaldSum::evalLongRange (src/util/EwaldSum.cc:114-178) of one slice
+// sl [N][D] by one thread: sum over the half-space k-vectors of
+// 2 f(k)/(2V) |sum_j q_j e^{i k.r_j}|^2, plus the self energy
+template <int D>
+__device__ double ewald_slice(const double* kp, const EwP& ew, const ChainP& c, const double* sl) {
+  double sum = 0;
+  for (int k = 0; k < ew.nk; ++k) {
+    const double* kv = kp + 4 * k;
+    double rr = 0, ri = 0;
+    for (int j = 0; j < c.N; ++j) {
+      double co, si;
+      ewald_phase<D>(kv, sl + j * D, co, si);
+      const double q = c.charge[species_of(c, j)];
+      rr += q * co;
+      ri += q * si;
+    }
+    sum += kv[3] * (rr * rr + ri * ri);
+  }
+  return sum + ew.self;
 }
 
 template <class R> __device__ __forceinline__ const R* table_ptr(const DevCtx& x, const TableP& t);

@@ -745,7 +745,7 @@ struct Eng {
       const double* m0 = mov + ((t - s) * nmov() + m) * D;
       const double* o1 = sb(sec1, t, i);
       const double* o0 = sb(sec1, t - s, i);
-      if (c.spring) {
+      if (c.spring && !c.stat[sp]) {
         const R inv2Sigma2 = (R)c.spring_c[level][sp];
         double dn[D], dd[D];
 #pragma unroll
@@ -851,7 +851,44 @@ struct Eng {
       }
       }
     }
-    return bsum(acc.value());
+    const double links = bsum(acc.value());
+    if (!(c.ew.nk && level == 0)) return links;
+    return links + ewald_diff();
+  }
+
+  // CoulombAction's long-range Ewald action at level 0 (CoulombAction.cc:
+  // 107-119): tau/eps sum over slices 1..2^L-1 of the k-space sum's change
+  // when the moving particles take their new beads.  Lanes over (slice,
+  // k-vector) items; per item rho_k of the section slice and the moving
+  // particles' change d rho_k give w_k (|rho + d rho|^2 - |rho|^2) =
+  // w_k (2 Re(rho* d rho) + |d rho|^2), in FP64 in both precision modes.
+  __device__ double ewald_diff() {
+    const double* kp = x.ewk + c.ew.off;
+    const int nk = c.ew.nk, items = (c.S - 2) * nk;
+    double e = 0;
+    for (int it = tid; it < items; it += NT) {
+      const int t = 1 + it / nk;
+      const double* kv = kp + 4 * (it - (t - 1) * nk);
+      double rr = 0, ri = 0, dr = 0, di = 0;
+      for (int j = 0; j < c.N; ++j) {
+        double co, si;
+        ewald_phase<D>(kv, sb(sec1, t, j), co, si);
+        const double q = c.charge[species_of(c, j)];
+        rr += q * co;
+        ri += q * si;
+      }
+      for (int m = 0; m < nmov(); ++m) {
+        const int i = mv(m);
+        const double q = c.charge[species_of(c, i)];
+        double cn, sn, co, so;
+        ewald_phase<D>(kv, mov + (t * nmov() + m) * D, cn, sn);
+        ewald_phase<D>(kv, sb(sec1, t, i), co, so);
+        dr += q * (cn - co);
+        di += q * (sn - so);
+      }
+      e += kv[3] * (2 * (rr * dr + ri * di) + dr * dr + di * di);
+    }
+    return bsum(e) * c.tau / c.ceps;
   }
 
   // -------------------------------------------------------- node model

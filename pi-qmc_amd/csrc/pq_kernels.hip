@@ -125,8 +125,12 @@ __global__ __launch_bounds__(PQ_WAVE) void node_eval_kernel(DevCtx x, int chain,
 // getBeadAction utau of spring, SHO and pair actions for bead (i,t) of a
 // chain, summed as CompositeAction::getBeadAction (CompositeAction.cc:57-67).
 // Slice -1 is slice M-1 of particle inversePermutation[i] (SerialPaths.cc:68-77).
+// With lr, particle 0 also carries CoulombAction's long-range Ewald utau of
+// slice t (CoulombAction.cc:164-170); the estimator kernels add that sum in
+// parallel instead.
 template <int D>
-__device__ double bead_utau(const DevCtx& x, const ChainP& c, const double* beads, int i, int t) {
+__device__ double bead_utau(const DevCtx& x, const ChainP& c, const double* beads, int i, int t,
+                            bool lr = false) {
   const int* perm = x.perm + c.perm_off;
   const int* invperm = perm + c.N;
   auto P = [&](int p, int tt) -> const double* {
@@ -138,7 +142,7 @@ __device__ double bead_utau(const DevCtx& x, const ChainP& c, const double* bead
   double utau = 0;
   const double* b1 = P(i, t);
   const double* b0 = P(i, t - 1);
-  if (c.spring) {
+  if (c.spring && !c.stat[sp]) {
     double delta[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) delta[d] = b1[d] - b0[d];
@@ -217,8 +221,33 @@ __device__ double bead_utau(const DevCtx& x, const ChainP& c, const double* bead
     }
     uc += up;
   }
+  if (lr && c.ew.nk && i == 0)
+    uc += ewald_slice<D>(x.ewk + c.ew.off, c.ew, c, beads + (int64_t)t * c.N * D) / c.ceps;
   utau += uc;
   return utau;
+}
+
+// sum over slices and half-space k-vectors of 2 f(k)/(2V) |rho_k(t)|^2 plus
+// M self energies: the long-range Ewald sums of all M slices, by the threads
+// of a workgroup (each returns its share)
+template <int D>
+__device__ double ewald_all_slices(const double* kp, const EwP& ew, const ChainP& c, const double* beads) {
+  double acc = 0;
+  for (int q = threadIdx.x; q < c.M * ew.nk; q += blockDim.x) {
+    const int t = q / ew.nk;
+    const double* kv = kp + 4 * (q - t * ew.nk);
+    double rr = 0, ri = 0;
+    for (int j = 0; j < c.N; ++j) {
+      double co, si;
+      ewald_phase<D>(kv, beads + ((int64_t)t * c.N + j) * D, co, si);
+      const double qj = c.charge[species_of(c, j)];
+      rr += qj * co;
+      ri += qj * si;
+    }
+    acc += kv[3] * (rr * rr + ri * ri);
+  }
+  if (threadIdx.x == 0) acc += c.M * ew.self;
+  return acc;
 }
 
 // ThermoEnergyEstimator (ThermoEnergyEstimator.cc:32-54): one workgroup per chain
@@ -237,6 +266,7 @@ __global__ __launch_bounds__(256) void energy_kernel(DevCtx x, double* energy) {
   double acc = 0;
   const int nl = c.M * c.N;
   for (int q = threadIdx.x; q < nl; q += blockDim.x) acc += bead_utau<D>(x, c, beads, q % c.N, q / c.N);
+  if (c.ew.nk) acc += ewald_all_slices<D>(x.ewk + c.ew.off, c.ew, c, beads) / c.ceps;
   acc = wave_sum(acc);
   if (lane_id() == 0) part[threadIdx.x / PQ_WAVE] = acc;
   __syncthreads();
@@ -316,7 +346,7 @@ __device__ double bead_force(const DevCtx& x, const ChainP& c, const double* bea
   pbc<D>(c, dm);
   pbc<D>(c, dp);
   double utau = 0;
-  if (c.spring) {
+  if (c.spring && !c.stat[sp]) {
     const double lam = c.lambda[sp], tau = c.tau;
     double dd = 0;
 #pragma unroll
@@ -401,7 +431,8 @@ __device__ double bead_force(const DevCtx& x, const ChainP& c, const double* bea
 // VirialEnergyEstimator (VirialEnergyEstimator.cc:27-78) over sumOverLinks
 // with firstSlice 0.  Its running sums restart at every window of nwindow
 // links of a particle, so lanes take (particle, window) items; a window
-// closes with F_av . r0 / 2tau.  FixedNodeAction adds zero force and utau.
+// closes with F_av . r0 / 2tau.  FixedNodeAction adds zero force and utau;
+// static particles add utau only (VirialEnergyEstimator.cc:36,51,60,66).
 template <int D>
 __global__ __launch_bounds__(256) void virial_energy_kernel(DevCtx x, int nwindow, double* energy) {
   __shared__ ChainP c;
@@ -420,6 +451,7 @@ __global__ __launch_bounds__(256) void virial_energy_kernel(DevCtx x, int nwindo
   double acc = 0;
   for (int it = threadIdx.x; it < c.N * nwin; it += blockDim.x) {
     const int i = it / nwin, w = it - i * nwin;
+    const bool stat = c.stat[species_of(c, i)];
     const int t0 = w * nwindow, t1 = min(M, t0 + nwindow);
     const int W = (t0 / nwindow < (M - 1) / nwindow) ? nwindow : (M - 1) % nwindow + 1;
     double r0[D], rav[D], fav[D];
@@ -441,13 +473,15 @@ __global__ __launch_bounds__(256) void virial_energy_kernel(DevCtx x, int nwindo
 #pragma unroll
       for (int d = 0; d < D; ++d) { fav[d] += f[d]; fr += f[d] * rav[d]; }
       e += utau;
-      e += -(1. - 1. / W) * (D / (2 * tau)) - (1 / (2 * tau)) * fr;
+      if (!stat) e += -(1. - 1. / W) * (D / (2 * tau)) - (1 / (2 * tau)) * fr;
     }
     double fr0 = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) fr0 += fav[d] * r0[d];
-    acc += e + (1 / (2 * tau)) * fr0;
+    acc += e + (stat ? 0.0 : (1 / (2 * tau)) * fr0);
   }
+  // CoulombAction's long-range utau of particle 0 on every slice (no force)
+  if (c.ew.nk) acc += ewald_all_slices<D>(x.ewk + c.ew.off, c.ew, c, beads) / c.ceps;
   acc = wave_sum(acc);
   if (lane_id() == 0) part[threadIdx.x / PQ_WAVE] = acc;
   __syncthreads();
@@ -464,7 +498,71 @@ __global__ __launch_bounds__(256) void bead_utau_kernel(DevCtx x, int nq, const 
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   const ChainP& c = x.chains[qc[q]];
-  out[q] = bead_utau<D>(x, c, x.beads + c.bead_off, qi[q], qt[q]);
+  out[q] = bead_utau<D>(x, c, x.beads + c.bead_off, qi[q], qt[q], true);
+}
+
+// EwaldCoulombEstimator with its own OptEwaldSum (EwaldCoulombEstimator.cc:
+// 63-140): per link q_i q_j (1/(r eps) + (1-x) v(ig) + x v(ig+1)) over pairs
+// j < i inside rcut, v(i) = -f(i dr)/eps (v(0) = -f(0)/eps) on 1001 points,
+// dr = rcut/1000, ig = (int)(r/dr) and x = r - ig dr as the reference weighs
+// it, and the k-space sum on particle 0 of every slice; divided by M.  One
+// workgroup per chain, ew[chain] its sum.
+template <int D>
+__global__ __launch_bounds__(256) void ewald_coulomb_energy_kernel(DevCtx x, const EwP* ew, const double* kpool,
+                                                                    double eps, double* energy) {
+  __shared__ ChainP c;
+  __shared__ double part[4];
+  const int chain = blockIdx.x;
+  {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(&x.chains[chain]);
+    uint32_t* d = reinterpret_cast<uint32_t*>(&c);
+    for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
+    __syncthreads();
+  }
+  const EwP e = ew[chain];
+  const double* beads = x.beads + c.bead_off;
+  const double dr = e.rcut / 1000, drinv = 1. / dr;
+  auto vgrid = [&](int ig) -> double {
+    if (ig == 0) return -e.coef[0] / eps;
+    const double r = ig * dr;
+    if (r > e.rcut) return -(1. / r) / eps;
+    double v = 0., r2 = r * r;
+    for (int k = 7; k >= 0; --k) { v *= r2; v += e.coef[k]; }
+    return -v / eps;
+  };
+  double acc = 0;
+  for (int q = threadIdx.x; q < c.M * c.N; q += blockDim.x) {
+    const int t = q / c.N, i = q - t * c.N;
+    const double* ri = beads + ((int64_t)t * c.N + i) * D;
+    const double qi = c.charge[species_of(c, i)];
+    double en = 0;
+    for (int j = 0; j < i; ++j) {
+      const double* rj = beads + ((int64_t)t * c.N + j) * D;
+      double delta[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = ri[d] - rj[d];
+      pbc<D>(c, delta);
+      double dd = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) dd += delta[d] * delta[d];
+      const double r = sqrt(dd);
+      if (r < e.rcut) {
+        const int ig = (int)(r * drinv);
+        const double xw = r - ig * dr;
+        en += qi * c.charge[species_of(c, j)] * (1. / (r * eps) + (1 - xw) * vgrid(ig) + xw * vgrid(ig + 1));
+      }
+    }
+    acc += en;
+  }
+  acc += ewald_all_slices<D>(kpool + e.off, e, c, beads) / eps;
+  acc = wave_sum(acc);
+  if (lane_id() == 0) part[threadIdx.x / PQ_WAVE] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0;
+    for (int w = 0; w < (int)(blockDim.x / PQ_WAVE); ++w) v += part[w];
+    energy[chain] = v / c.M;
+  }
 }
 
 // PairCFEstimator<1>::handleLink with PairRadial (PairCFEstimator.h:72-93).
@@ -601,6 +699,13 @@ hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hip
 hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps, double* energy, hipStream_t st) {
   if (D == 2) hipLaunchKernelGGL(coulomb_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
   else hipLaunchKernelGGL(coulomb_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
+  return hipGetLastError();
+}
+
+hipError_t launch_ewald_coulomb_energy(int D, const DevCtx& x, int nchain, const EwP* ew, const double* kpool,
+                                       double eps, double* energy, hipStream_t st) {
+  if (D == 2) hipLaunchKernelGGL(ewald_coulomb_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, ew, kpool, eps, energy);
+  else hipLaunchKernelGGL(ewald_coulomb_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, ew, kpool, eps, energy);
   return hipGetLastError();
 }
 

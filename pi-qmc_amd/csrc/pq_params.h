@@ -33,6 +33,14 @@ struct FnP {             // one FixedNodeAction with its node model
   int pg_img[3];         // evaluate as the sum of the three nearest images
 };
 
+struct EwP {             // the optimized Ewald sum of a CoulombAction or estimator
+  int nk;                // half-space k-vectors, 0: no Ewald sum
+  int64_t off;           // element offset of [nk][4] (k_x, k_y, k_z, 2 f(k)/(2V)) in the k pool
+  double self;           // EwaldSum self energy with the particles' charges
+  double rcut;
+  double coef[8];        // f(r) = sum_i coef[i] r^2i up to rcut, 1/r beyond
+};
+
 struct ChainP {
   int N, D, M, L, S, nsp;
   double tau;
@@ -68,6 +76,9 @@ struct ChainP {
   int64_t fnc_off;       // element offset of this chain's slice-pair node records, -1: none
   int64_t perm_off;      // element offset of this chain's path permutation and its inverse ([2][N])
   int ns_max;            // largest fermion species with a node model
+  int stat[4];           // species isStatic: no spring action, no virial kinetic terms
+  double ceps;           // CoulombAction epsilon
+  EwP ew;                // CoulombAction's Ewald sum (useEwald)
 };
 
 // Doubles of one slice-pair node record of a species of ns particles:

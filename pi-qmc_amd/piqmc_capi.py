@@ -36,7 +36,8 @@ PQ_PREC_MIXED = 1
 
 
 class SpeciesDesc(C.Structure):
-    _fields_ = [("count", C.c_int), ("mass", C.c_double), ("charge", C.c_double)]
+    _fields_ = [("count", C.c_int), ("mass", C.c_double), ("charge", C.c_double),
+                ("is_static", C.c_int)]
 
 
 class ShoDesc(C.Structure):
@@ -46,7 +47,8 @@ class ShoDesc(C.Structure):
 
 class CoulombDesc(C.Structure):
     _fields_ = [("enabled", C.c_int), ("epsilon", C.c_double), ("norder", C.c_int),
-                ("rmin", C.c_double), ("rmax", C.c_double), ("ngpts", C.c_int)]
+                ("rmin", C.c_double), ("rmax", C.c_double), ("ngpts", C.c_int),
+                ("use_ewald", C.c_int), ("ewald_rcut", C.c_double), ("ewald_kcut", C.c_double)]
 
 
 class NodeDesc(C.Structure):
@@ -85,8 +87,9 @@ class ChainStats(C.Structure):
 def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
               sho=(), coulomb=None, nodes=(), mover=PQ_MOVER_FREE, samples=None,
               nsection=None):
-    """species: [(count, mass, charge)]; sho: [(species, omega)];
-    coulomb: dict(epsilon, norder, rmin, rmax, ngpts) or None;
+    """species: [(count, mass, charge[, is_static])]; sho: [(species, omega)];
+    coulomb: dict(epsilon, norder, rmin, rmax, ngpts[, ewald, ewald_rcut,
+    ewald_kcut]) or None;
     nodes: [(species, model, omega)]; samples [(species, npart[, chooser])],
     default one npart=1 SpeciesParticleChooser per species;
     nsection default M / 2^L (one sweep touches every slice once on average)."""
@@ -95,8 +98,9 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
     cells = cell if isinstance(cell, (list, tuple)) else [cell] * 3
     for k in range(3):
         d.cell[k] = cells[k]
-    for s, (n, m, q) in enumerate(species):
-        d.species[s].count, d.species[s].mass, d.species[s].charge = n, m, q
+    for s, sp in enumerate(species):
+        d.species[s].count, d.species[s].mass, d.species[s].charge = sp[0], sp[1], sp[2]
+        d.species[s].is_static = 1 if len(sp) > 3 and sp[3] else 0
     d.spring = 1 if spring else 0
     d.nsho = len(sho)
     for k, (s, w) in enumerate(sho):
@@ -108,6 +112,9 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
         d.coulomb.rmin = coulomb.get("rmin", 0.0)
         d.coulomb.rmax = coulomb.get("rmax", 0.0)
         d.coulomb.ngpts = coulomb.get("ngpts", 0)
+        d.coulomb.use_ewald = 1 if coulomb.get("ewald") else 0
+        d.coulomb.ewald_rcut = coulomb.get("ewald_rcut", 0.0)
+        d.coulomb.ewald_kcut = coulomb.get("ewald_kcut", 0.0)
     d.nnodes = len(nodes)
     for k, (s, model, w) in enumerate(nodes):
         d.nodes[k].species, d.nodes[k].model, d.nodes[k].omega = s, model, w
@@ -263,6 +270,7 @@ def load_library(path=LIB_PATH):
         "pq_measure_energy": (C.c_int, [C.c_void_p, dp]),
         "pq_measure_coulomb_energy": (C.c_int, [C.c_void_p, C.c_double, dp]),
         "pq_measure_virial_energy": (C.c_int, [C.c_void_p, C.c_int, dp]),
+        "pq_measure_ewald_coulomb_energy": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, dp]),
         "pq_pair_histogram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_int, P(C.c_uint64)]),
         "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
@@ -411,6 +419,11 @@ class Batch:
     def coulomb_energy(self, epsilon=1.0):
         e = np.empty(len(self.descs))
         check(self.lib.pq_measure_coulomb_energy(self.h, epsilon, dptr(e)))
+        return e
+
+    def ewald_coulomb_energy(self, epsilon, rcut, kcut):
+        e = np.empty(len(self.descs))
+        check(self.lib.pq_measure_ewald_coulomb_energy(self.h, epsilon, rcut, kcut, dptr(e)))
         return e
 
     def virial_energy(self, nwindow=1):

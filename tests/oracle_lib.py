@@ -64,6 +64,10 @@ def load():
         "oq_get_stats": (None, [vp, P(capi.ChainStats)]),
         "oq_thermo_energy": (C.c_double, [vp]),
         "oq_coulomb_energy": (C.c_double, [vp, C.c_double]),
+        "oq_ewald_coulomb_energy": (C.c_double, [vp, C.c_double, C.c_double, C.c_double]),
+        "oq_ewald_info": (C.c_int, [vp, dp, ip, dp, C.c_int, ip, dp]),
+        "oq_ewald_long_range": (C.c_double, [vp, dp]),
+        "oq_ewald_fr": (C.c_double, [vp, C.c_double]),
         "oq_bead_action": (None, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
         "oq_virial_energy": (C.c_double, [vp, C.c_int]),
         "oq_energy_blocks": (None, [vp, C.c_int, C.c_int, dp]),
@@ -173,6 +177,28 @@ class Chain:
 
     def coulomb_energy(self, epsilon=1.0):
         return self.lib.oq_coulomb_energy(self.h, epsilon)
+
+    def ewald_coulomb_energy(self, epsilon, rcut, kcut):
+        return self.lib.oq_ewald_coulomb_energy(self.h, epsilon, rcut, kcut)
+
+    def ewald_info(self):
+        """(coef[8], nk, self energy, kv [nk][3], vk [nk]) of the CoulombAction's sum."""
+        coef, nk, se = np.zeros(8), C.c_int(), C.c_double()
+        if not self.lib.oq_ewald_info(self.h, dptr(coef), C.byref(nk), C.byref(se), -1, None, None):
+            return None
+        kv, vk = np.zeros((nk.value, 3), dtype=np.int32), np.zeros(nk.value)
+        for k in range(nk.value):
+            v, w = np.zeros(3, dtype=np.int32), C.c_double()
+            self.lib.oq_ewald_info(self.h, None, None, None, k, iptr(v), C.byref(w))
+            kv[k], vk[k] = v, w.value
+        return coef, nk.value, se.value, kv, vk
+
+    def ewald_long_range(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        return self.lib.oq_ewald_long_range(self.h, dptr(r))
+
+    def ewald_fr(self, r):
+        return self.lib.oq_ewald_fr(self.h, r)
 
     def bead_action(self, i, t):
         u, ut = C.c_double(), C.c_double()
