@@ -314,7 +314,10 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
   if (d.tau <= 0) return fail(PQ_ERR_ARG, "tau must be positive");
   c.D = d.ndim; c.M = d.nslice; c.L = d.nlevel; c.S = (1 << d.nlevel) + 1; c.nsp = d.nspecies;
   c.tau = d.tau;
-  if (c.M < c.S) return fail(PQ_ERR_ARG, "nslice %d shorter than a section", c.M);
+  // a section may span the whole path (2^L = M, the reference's own SHO
+  // system test): its moved slices 1..2^L-1 stay distinct and both ends are
+  // bead (i, t0); shorter paths would move a slice twice
+  if (c.M < c.S - 1) return fail(PQ_ERR_ARG, "nslice %d shorter than a section's 2^L = %d links", c.M, c.S - 1);
   int n = 0;
   for (int s = 0; s < c.nsp; ++s) {
     if (d.species[s].count < 1 || d.species[s].mass <= 0) return fail(PQ_ERR_ARG, "bad species %d", s);
@@ -482,6 +485,11 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     c.sample_sp[k] = all ? 0 : d.sample[k].species;
     c.sample_ch[k] = ch;
     c.sample_nm[k] = d.sample[k].nmoving;
+    // a permuted section relabels the slices after it, which a section of
+    // the whole path (2^L = M) would overlap
+    const bool permuting = (ch == PQ_CHOOSER_WALKING || ch == PQ_CHOOSER_ALL_RANDOM) && d.sample[k].nmoving > 1;
+    if (permuting && c.M < c.S)
+      return fail(PQ_ERR_ARG, "sample %d permutes particles: nslice %d must exceed 2^L = %d", k, c.M, c.S - 1);
   }
   c.nsection = d.nsection;
   return PQ_OK;

@@ -421,6 +421,13 @@ class Batch:
         check(self.lib.pq_measure_coulomb_energy(self.h, epsilon, dptr(e)))
         return e
 
+    def run_measured(self, nblock, nsweep):
+        """nblock x (nsweep sweeps, thermodynamic energy) in one captured graph:
+        energies [nblock][nchain]."""
+        e = np.empty((nblock, len(self.descs)))
+        check(self.lib.pq_run_measured(self.h, nblock, nsweep, dptr(e)))
+        return e
+
     def ewald_coulomb_energy(self, epsilon, rcut, kcut):
         e = np.empty(len(self.descs))
         check(self.lib.pq_measure_ewald_coulomb_energy(self.h, epsilon, rcut, kcut, dptr(e)))
