@@ -47,6 +47,7 @@ extern "C" {
 #define PQ_MAX_MOVING 8
 #define PQ_MAX_NORDER 4
 #define PQ_MAX_NODE_PART 32     /* largest fermion species handled by a node model */
+#define PQ_MAX_GROUPS 4         /* <ChooseSection> groups per sweep */
 
 #define PQ_MOVER_FREE 0         /* FreeMover      (src/advancer/mover/FreeMover.cc:59-124) */
 #define PQ_MOVER_FREE_PBC 1     /* FreeMoverPBC   (src/advancer/mover/FreeMoverPBC.cc:59-162) */
@@ -123,8 +124,21 @@ typedef struct pq_sample_desc {
   int chooser;         /* PQ_CHOOSER_* */
 } pq_sample_desc;
 
+/* A further <ChooseSection nlevel> of the sweep with its <Sample> elements,
+   repeated nsection times (a <Loop nrepeat> around it), e.g. the hydrogen-atom
+   input's nlevel 7, 3 x nlevel 6 and 5 x nlevel 5 sections between Measures
+   (test/system/hatom/pimc.xml). */
+typedef struct pq_section_group {
+  int nlevel;
+  int nsample;
+  pq_sample_desc sample[PQ_MAX_SAMPLES];
+  int nsection;
+} pq_section_group;
+
 /* One Markov chain (one clone of the reference).  Particle i of species s has
-   global index species_first(s)+i, species in the order given. */
+   global index species_first(s)+i, species in the order given.  A sweep runs
+   nsection sections of (nlevel, sample[0..nsample)) and then, in order, the
+   ngroup further groups. */
 typedef struct pq_chain_desc {
   int ndim;            /* 2 or 3 */
   int nslice;          /* M */
@@ -143,6 +157,8 @@ typedef struct pq_chain_desc {
   int nsample;         /* <Sample> elements per <ChooseSection>, run in order */
   pq_sample_desc sample[PQ_MAX_SAMPLES];
   int nsection;        /* <ChooseSection> repetitions per sweep */
+  int ngroup;          /* further section groups, 0..PQ_MAX_GROUPS-1 */
+  pq_section_group group[PQ_MAX_GROUPS - 1];
 } pq_chain_desc;
 
 /* Per-chain counters kept on the device by the sampler. */

@@ -775,12 +775,31 @@ struct oq_chain {
   double *wt, *wcump;
   unsigned char* wrow;
   pg_t wpg[PQ_MAX_SPECIES][3];
+  /* the section group being run (set_group) */
+  int gnsample, gnsection;
+  const pq_sample_desc* gsample;
   oq_rng rng;
   pq_chain_stats st;
 };
 
 static int pg_index(const oq_chain* ch, int level, int sp, int d) {
   return (level * ch->nsp + sp) * 3 + d;
+}
+
+/* the sweep's g-th <ChooseSection nlevel> group: its level count, section
+   length and Samples, and the WalkingChooser Gaussians of its Samples,
+   pg(d) = PeriodicGaussian(m/(2 tau 2^L), a_d) (WalkingChooser.cc:20-25),
+   which each ChooseSection's Samples build for their own nlevel */
+static void set_group(oq_chain* ch, int g) {
+  const pq_chain_desc* d = &ch->d;
+  ch->L = g ? d->group[g - 1].nlevel : d->nlevel;
+  ch->S = (1 << ch->L) + 1;
+  ch->gnsample = g ? d->group[g - 1].nsample : d->nsample;
+  ch->gsample = g ? d->group[g - 1].sample : d->sample;
+  ch->gnsection = g ? d->group[g - 1].nsection : d->nsection;
+  for (int s = 0; s < ch->nsp; ++s)
+    for (int k = 0; k < ch->D; ++k)
+      pg_init(&ch->wpg[s][k], d->species[s].mass / (2 * ch->tau * (1 << ch->L)), d->cell[k]);
 }
 
 oq_chain* oq_create(const pq_chain_desc* desc) {
@@ -791,14 +810,22 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
     ntot += desc->species[s].count;
     if (desc->species[s].count > maxcount) maxcount = desc->species[s].count;
   }
-  for (int k = 0; k < desc->nsample; ++k) {
-    const pq_sample_desc* sm = &desc->sample[k];
-    if (sm->chooser < PQ_CHOOSER_SPECIES || sm->chooser > PQ_CHOOSER_ALL_RANDOM) return NULL;
-    if (sm->nmoving < 1 || sm->nmoving > PQ_MAX_MOVING) return NULL;
-    const int pool = (sm->chooser == PQ_CHOOSER_ALL || sm->chooser == PQ_CHOOSER_ALL_RANDOM)
-                         ? ntot : desc->species[sm->species].count;
-    if (sm->nmoving > pool) return NULL;
+  if (desc->ngroup < 0 || desc->ngroup > PQ_MAX_GROUPS - 1) return NULL;
+  int lmax = desc->nlevel;
+  for (int g = 0; g <= desc->ngroup; ++g) {
+    const int ns = g ? desc->group[g - 1].nsample : desc->nsample;
+    const pq_sample_desc* smp = g ? desc->group[g - 1].sample : desc->sample;
+    if (g && desc->group[g - 1].nlevel > lmax) lmax = desc->group[g - 1].nlevel;
+    for (int k = 0; k < ns; ++k) {
+      const pq_sample_desc* sm = &smp[k];
+      if (sm->chooser < PQ_CHOOSER_SPECIES || sm->chooser > PQ_CHOOSER_ALL_RANDOM) return NULL;
+      if (sm->nmoving < 1 || sm->nmoving > PQ_MAX_MOVING) return NULL;
+      const int pool = (sm->chooser == PQ_CHOOSER_ALL || sm->chooser == PQ_CHOOSER_ALL_RANDOM)
+                           ? ntot : desc->species[sm->species].count;
+      if (sm->nmoving > pool) return NULL;
+    }
   }
+  const int smax = (1 << lmax) + 1;
   oq_chain* ch = (oq_chain*)calloc(1, sizeof(oq_chain));
   ch->d = *desc;
   ch->D = desc->ndim;
@@ -821,7 +848,7 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
   cell_init(&ch->cell, desc->cell, ch->D);
   /* SpringAction pg(level,species,dim): alpha = m/(2 tau 2^l) (SpringAction.cc:31-49);
      FreeMover / FreeMoverPBC: alpha = m/(tau 2^l), levels 0..L+1 (FreeMoverPBC.cc:37-51) */
-  ch->npg_levels = ch->L + 2;
+  ch->npg_levels = lmax + 2;
   int npg = ch->npg_levels * ch->nsp * 3;
   ch->spg = (pg_t*)calloc(npg, sizeof(pg_t));
   ch->mpg = (pg_t*)calloc(npg, sizeof(pg_t));
@@ -834,7 +861,7 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
         double as = m / (2 * ch->tau * (1 << l));
         double am = m / (ch->tau * (1 << l));
         int k = pg_index(ch, l, s, d);
-        if (l <= ch->L && oq_pg_nterms(as, len) < 16) { ch->spg_on[k] = 1; pg_init(&ch->spg[k], as, len); }
+        if (l <= lmax && oq_pg_nterms(as, len) < 16) { ch->spg_on[k] = 1; pg_init(&ch->spg[k], as, len); }
         if (oq_pg_nterms(am, len) < 16) { ch->mpg_on[k] = 1; pg_init(&ch->mpg[k], am, len); }
       }
   ch->beads = (double*)calloc((size_t)ch->M * ch->N * ch->D, sizeof(double));
@@ -909,24 +936,21 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
     f->c = f->mass * 0.5 * f->omega / f->sinhwt;
     /* FreeParticleNodes pg[d] = PeriodicGaussian(mass*T, a_d) (FreeParticleNodes.cc:54-58) */
     for (int d = 0; d < ch->D; ++d) pg_init(&f->pg[d], f->mass * f->T, desc->cell[d]);
-    f->dm = (double*)calloc(ch->S, sizeof(double));
-    f->newdm = (double*)calloc(ch->S, sizeof(double));
-    f->dist = (double*)calloc((size_t)ch->S * 2 * f->ns, sizeof(double));
-    f->newdist = (double*)calloc((size_t)ch->S * 2 * f->ns, sizeof(double));
-    f->inv = (double*)calloc((size_t)ch->S * f->ns * f->ns, sizeof(double));
+    f->dm = (double*)calloc(smax, sizeof(double));
+    f->newdm = (double*)calloc(smax, sizeof(double));
+    f->dist = (double*)calloc((size_t)smax * 2 * f->ns, sizeof(double));
+    f->newdist = (double*)calloc((size_t)smax * 2 * f->ns, sizeof(double));
+    f->inv = (double*)calloc((size_t)smax * f->ns * f->ns, sizeof(double));
   }
-  size_t secsz = (size_t)ch->S * ch->N * ch->D;
+  size_t secsz = (size_t)smax * ch->N * ch->D;
   ch->sec1 = (double*)calloc(secsz, sizeof(double));
   ch->sec2 = (double*)calloc(secsz, sizeof(double));
-  ch->mov = (double*)calloc((size_t)ch->S * PQ_MAX_MOVING * ch->D, sizeof(double));
+  ch->mov = (double*)calloc((size_t)smax * PQ_MAX_MOVING * ch->D, sizeof(double));
   for (int i = 0; i < ch->N; ++i) ch->perm[i] = ch->invperm[i] = i;
   ch->wt = (double*)calloc((size_t)maxcount * maxcount, sizeof(double));
   ch->wcump = (double*)calloc((size_t)maxcount * maxcount, sizeof(double));
   ch->wrow = (unsigned char*)calloc(maxcount, 1);
-  /* WalkingChooser pg(d) = PeriodicGaussian(m/(2 tau 2^L), a_d) (WalkingChooser.cc:20-25) */
-  for (int s = 0; s < ch->nsp; ++s)
-    for (int d = 0; d < ch->D; ++d)
-      pg_init(&ch->wpg[s][d], desc->species[s].mass / (2 * ch->tau * (1 << ch->L)), desc->cell[d]);
+  set_group(ch, 0);
   oq_rng_glibc(ch, 0);
   return ch;
 }
@@ -1964,14 +1988,18 @@ static void run_section(oq_chain* ch) {
     if (t0 > ihi) t0 = ihi;
     get_beads(ch, t0, ch->sec1);
   }
-  for (int s = 0; s < ch->d.nsample; ++s) sample(ch, &ch->d.sample[s]);
+  for (int s = 0; s < ch->gnsample; ++s) sample(ch, &ch->gsample[s]);
   if (ch->nfn) put_double_beads(ch, t0, t0b);
   else put_beads(ch, t0, ch->sec1, ch->secperm);
 }
 
 void oq_run_sweeps(oq_chain* ch, int nsweep) {
   for (int k = 0; k < nsweep; ++k)
-    for (int s = 0; s < ch->d.nsection; ++s) run_section(ch);
+    for (int g = 0; g <= ch->d.ngroup; ++g) {
+      set_group(ch, g);
+      for (int s = 0; s < ch->gnsection; ++s) run_section(ch);
+    }
+  set_group(ch, 0);
   ch->st.draws = ch->rng.count;
 }
 

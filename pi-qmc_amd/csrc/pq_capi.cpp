@@ -310,14 +310,22 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
   std::memset(&c, 0, sizeof c);
   if (d.ndim != 2 && d.ndim != 3) return fail(PQ_ERR_ARG, "ndim must be 2 or 3 (got %d)", d.ndim);
   if (d.nspecies < 1 || d.nspecies > PQ_MAX_SPECIES) return fail(PQ_ERR_ARG, "nspecies out of range");
-  if (d.nlevel < 0 || d.nlevel > 6) return fail(PQ_ERR_ARG, "nlevel must be in 0..6");
+  if (d.nlevel < 0 || d.nlevel > 7) return fail(PQ_ERR_ARG, "nlevel must be in 0..7");
   if (d.tau <= 0) return fail(PQ_ERR_ARG, "tau must be positive");
   c.D = d.ndim; c.M = d.nslice; c.L = d.nlevel; c.S = (1 << d.nlevel) + 1; c.nsp = d.nspecies;
   c.tau = d.tau;
+  if (d.ngroup < 0 || d.ngroup > PQ_MAX_GROUPS - 1) return fail(PQ_ERR_ARG, "ngroup must be in 0..%d", PQ_MAX_GROUPS - 1);
+  c.ngroup = 1 + d.ngroup;
+  c.Lmax = c.L;
+  for (int g = 0; g < d.ngroup; ++g) {
+    if (d.group[g].nlevel < 0 || d.group[g].nlevel > 7) return fail(PQ_ERR_ARG, "group %d: nlevel must be in 0..7", g + 1);
+    c.Lmax = std::max(c.Lmax, d.group[g].nlevel);
+  }
+  c.Smax = (1 << c.Lmax) + 1;
   // a section may span the whole path (2^L = M, the reference's own SHO
   // system test): its moved slices 1..2^L-1 stay distinct and both ends are
   // bead (i, t0); shorter paths would move a slice twice
-  if (c.M < c.S - 1) return fail(PQ_ERR_ARG, "nslice %d shorter than a section's 2^L = %d links", c.M, c.S - 1);
+  if (c.M < c.Smax - 1) return fail(PQ_ERR_ARG, "nslice %d shorter than a section's 2^L = %d links", c.M, c.Smax - 1);
   int n = 0;
   for (int s = 0; s < c.nsp; ++s) {
     if (d.species[s].count < 1 || d.species[s].mass <= 0) return fail(PQ_ERR_ARG, "bad species %d", s);
@@ -341,8 +349,8 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
   for (int l = 0; l < PQ_PG_LEVELS; ++l)
     for (int s = 0; s < c.nsp; ++s)
       for (int k = 0; k < c.D; ++k) {
-        c.spg_on[l][s][k] = (l <= c.L && pg_nterms(c.mass[s] / (2 * c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
-        c.mpg_on[l][s][k] = (l <= c.L + 1 && pg_nterms(c.mass[s] / (c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
+        c.spg_on[l][s][k] = (l <= c.Lmax && pg_nterms(c.mass[s] / (2 * c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
+        c.mpg_on[l][s][k] = (l <= c.Lmax + 1 && pg_nterms(c.mass[s] / (c.tau * (1 << l)), c.a[k]) < 16) ? 1 : 0;
       }
   // per-level constants of SpringAction (SpringAction.cc:71) and SHOAction (SHOAction.cc:28-33)
   for (int l = 0; l < PQ_PG_LEVELS; ++l) {
@@ -466,30 +474,46 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     if (nd.model != PQ_NODES_SHO && nd.model != PQ_NODES_FREE) return fail(PQ_ERR_ARG, "unknown node model");
     c.ns_max = std::max(c.ns_max, f.ns);
   }
-  if (c.nfn && 2 * ((1 << c.L) + 1) > c.M)
+  if (c.nfn && 2 * c.Smax > c.M)
     return fail(PQ_ERR_ARG, "the two sections beta/2 apart overlap: need 2(2^L+1) <= M");
   if (d.mover != PQ_MOVER_FREE && d.mover != PQ_MOVER_FREE_PBC) return fail(PQ_ERR_ARG, "unknown mover");
   c.mover = d.mover;
-  if (d.nsample < 1 || d.nsample > PQ_MAX_SAMPLES) return fail(PQ_ERR_ARG, "nsample out of range");
-  c.nsample = d.nsample;
-  for (int k = 0; k < d.nsample; ++k) {
-    const int ch = d.sample[k].chooser;
-    if (ch < PQ_CHOOSER_SPECIES || ch > PQ_CHOOSER_ALL_RANDOM) return fail(PQ_ERR_ARG, "unknown chooser");
-    const bool all = ch == PQ_CHOOSER_ALL || ch == PQ_CHOOSER_ALL_RANDOM;
-    if (!all && (d.sample[k].species < 0 || d.sample[k].species >= c.nsp)) return fail(PQ_ERR_ARG, "sample species");
-    // the reference's choosers redraw until they hold nmoving distinct
-    // particles: more than the pool would never end
-    const int pool = all ? c.N : c.count[d.sample[k].species];
-    if (d.sample[k].nmoving < 1 || d.sample[k].nmoving > PQ_MAX_MOVING || d.sample[k].nmoving > pool)
-      return fail(PQ_ERR_ARG, "sample %d: nmoving %d not in 1..min(%d, %d)", k, d.sample[k].nmoving, PQ_MAX_MOVING, pool);
-    c.sample_sp[k] = all ? 0 : d.sample[k].species;
-    c.sample_ch[k] = ch;
-    c.sample_nm[k] = d.sample[k].nmoving;
-    // a permuted section relabels the slices after it, which a section of
-    // the whole path (2^L = M) would overlap
-    const bool permuting = (ch == PQ_CHOOSER_WALKING || ch == PQ_CHOOSER_ALL_RANDOM) && d.sample[k].nmoving > 1;
-    if (permuting && c.M < c.S)
-      return fail(PQ_ERR_ARG, "sample %d permutes particles: nslice %d must exceed 2^L = %d", k, c.M, c.S - 1);
+  // the section groups of a sweep: group 0 from the chain's own fields
+  for (int g = 0; g < c.ngroup; ++g) {
+    const int L = g ? d.group[g - 1].nlevel : d.nlevel;
+    const int nsample = g ? d.group[g - 1].nsample : d.nsample;
+    const pq_sample_desc* smp = g ? d.group[g - 1].sample : d.sample;
+    GroupP& G = c.grp[g];
+    G.L = L;
+    G.nsection = g ? d.group[g - 1].nsection : d.nsection;
+    if (G.nsection < 0) return fail(PQ_ERR_ARG, "group %d: nsection must not be negative", g);
+    if (nsample < 1 || nsample > PQ_MAX_SAMPLES) return fail(PQ_ERR_ARG, "group %d: nsample out of range", g);
+    G.nsample = nsample;
+    for (int k = 0; k < nsample; ++k) {
+      const int ch = smp[k].chooser;
+      if (ch < PQ_CHOOSER_SPECIES || ch > PQ_CHOOSER_ALL_RANDOM) return fail(PQ_ERR_ARG, "unknown chooser");
+      const bool all = ch == PQ_CHOOSER_ALL || ch == PQ_CHOOSER_ALL_RANDOM;
+      if (!all && (smp[k].species < 0 || smp[k].species >= c.nsp)) return fail(PQ_ERR_ARG, "sample species");
+      // the reference's choosers redraw until they hold nmoving distinct
+      // particles: more than the pool would never end
+      const int pool = all ? c.N : c.count[smp[k].species];
+      if (smp[k].nmoving < 1 || smp[k].nmoving > PQ_MAX_MOVING || smp[k].nmoving > pool)
+        return fail(PQ_ERR_ARG, "sample %d: nmoving %d not in 1..min(%d, %d)", k, smp[k].nmoving, PQ_MAX_MOVING, pool);
+      G.sample_sp[k] = all ? 0 : smp[k].species;
+      G.sample_ch[k] = ch;
+      G.sample_nm[k] = smp[k].nmoving;
+      // a permuted section relabels the slices after it, which a section of
+      // the whole path (2^L = M) would overlap
+      const bool permuting = (ch == PQ_CHOOSER_WALKING || ch == PQ_CHOOSER_ALL_RANDOM) && smp[k].nmoving > 1;
+      if (permuting && c.M < (1 << L) + 1)
+        return fail(PQ_ERR_ARG, "sample %d permutes particles: nslice %d must exceed 2^L = %d", k, c.M, 1 << L);
+    }
+  }
+  c.nsample = c.grp[0].nsample;
+  for (int k = 0; k < c.nsample; ++k) {
+    c.sample_sp[k] = c.grp[0].sample_sp[k];
+    c.sample_ch[k] = c.grp[0].sample_ch[k];
+    c.sample_nm[k] = c.grp[0].sample_nm[k];
   }
   c.nsection = d.nsection;
   return PQ_OK;
@@ -509,18 +533,19 @@ static SweepLayoutH layout_for(const pq_batch* b, const Sched& req) {
   bool nodes = false;
   const int64_t w = b->precision == PQ_PREC_MIXED ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
   for (auto& ch : b->hc) {
-    mS = std::max<int64_t>(mS, ch.S);
-    mSND = std::max<int64_t>(mSND, (int64_t)ch.S * ch.N * ch.D);
+    mS = std::max<int64_t>(mS, ch.Smax);
+    mSND = std::max<int64_t>(mSND, (int64_t)ch.Smax * ch.N * ch.D);
     int nm = 1;
-    for (int k = 0; k < ch.nsample; ++k) {
-      nm = std::max(nm, ch.sample_nm[k]);
-      if (ch.sample_ch[k] == PQ_CHOOSER_WALKING)
-        mWalk = std::max<int64_t>(mWalk, 2 * (int64_t)ch.sample_nm[k] * ch.count[ch.sample_sp[k]]);
-    }
-    mSD = std::max<int64_t>(mSD, (int64_t)ch.S * nm * ch.D);
+    for (int g = 0; g < ch.ngroup; ++g)
+      for (int k = 0; k < ch.grp[g].nsample; ++k) {
+        nm = std::max(nm, ch.grp[g].sample_nm[k]);
+        if (ch.grp[g].sample_ch[k] == PQ_CHOOSER_WALKING)
+          mWalk = std::max<int64_t>(mWalk, 2 * (int64_t)ch.grp[g].sample_nm[k] * ch.count[ch.grp[g].sample_sp[k]]);
+      }
+    mSD = std::max<int64_t>(mSD, (int64_t)ch.Smax * nm * ch.D);
     mN = std::max<int64_t>(mN, ch.N);
     mF = std::max<int64_t>(mF, fn_state_doubles(ch));
-    if (ch.nfn) { nodes = true; mInv = std::max<int64_t>(mInv, (int64_t)ch.S * ch.ns_max * ch.ns_max); }
+    if (ch.nfn) { nodes = true; mInv = std::max<int64_t>(mInv, (int64_t)ch.Smax * ch.ns_max * ch.ns_max); }
     int64_t tabs = 0;
     for (int pp = 0; pp < ch.npair; ++pp) {
       int q = 0;
@@ -627,7 +652,8 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     ns_max = std::max(ns_max, b->hc[c].ns_max);
   }
   for (int c = 0; c < nchain; ++c)
-    for (int k = 0; k < b->hc[c].nsample; ++k) b->multi |= b->hc[c].sample_nm[k] > 1;
+    for (int g = 0; g < b->hc[c].ngroup; ++g)
+      for (int k = 0; k < b->hc[c].grp[g].nsample; ++k) b->multi |= b->hc[c].grp[g].sample_nm[k] > 1;
   // node-matrix row width: the instantiated widths, the smallest that fits
   b->NC = ns_max <= 4 ? 4 : ns_max <= 8 ? 8 : ns_max <= 10 ? 10 : ns_max <= 12 ? 12 : ns_max <= 16 ? 16 : 32;
   for (int c = 0; c < nchain; ++c) {
@@ -635,9 +661,9 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     ch.bead_off = beads;
     beads += (int64_t)ch.M * ch.N * ch.D;
     ch.inv_off = inv;
-    if (ch.nfn) inv += (int64_t)ch.S * ch.ns_max * ch.ns_max;
+    if (ch.nfn) inv += (int64_t)ch.Smax * ch.ns_max * ch.ns_max;
     ch.egr_off = egr;
-    if (ch.nfn) egr += (int64_t)ch.S * ch.ns_max * ch.ns_max * ch.D;
+    if (ch.nfn) egr += (int64_t)ch.Smax * ch.ns_max * ch.ns_max * ch.D;
     ch.fn_off = fnst;
     fnst += fn_state_doubles(ch);
     const int64_t nrec = fn_cache_doubles(ch);
@@ -661,7 +687,10 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     const ChainP& ch = b->hc[c];
     double ns2 = 0;
     for (int a = 0; a < ch.nfn; ++a) ns2 += (double)ch.fn[a].ns * ch.fn[a].ns * ch.fn[a].ns;
-    cost[c] = (double)ch.nsection * ch.nsample * (1 << ch.L) * (ch.N + ns2 / std::max(1, ch.nsample));
+    cost[c] = 0;
+    for (int g = 0; g < ch.ngroup; ++g)
+      cost[c] += (double)ch.grp[g].nsection * ch.grp[g].nsample * (1 << ch.grp[g].L) *
+                 (ch.N + ns2 / std::max(1, ch.grp[g].nsample));
   }
   std::stable_sort(b->order.begin(), b->order.end(), [&](int p, int q) { return cost[p] > cost[q]; });
 

@@ -7,7 +7,7 @@
 
 #define PQ_WAVE 64
 #define PQ_MAX_PAIRS 10
-#define PQ_PG_LEVELS 8
+#define PQ_PG_LEVELS 9      // levels 0..8: a section of up to 2^7 links and the mover's level L+1
 #define PQ_GJ32_SCRATCH_DOUBLES 560   // LDS scratch per wave of the 32-wide node-matrix inversion
 
 struct TableP {          // one tabulated pair action u_k(r), utau_k(r)
@@ -39,6 +39,15 @@ struct EwP {             // the optimized Ewald sum of a CoulombAction or estima
   double self;           // EwaldSum self energy with the particles' charges
   double rcut;
   double coef[8];        // f(r) = sum_i coef[i] r^2i up to rcut, 1/r beyond
+};
+
+#define PQ_GROUPS 4      // <ChooseSection> groups per sweep (PQ_MAX_GROUPS)
+
+struct GroupP {          // one <ChooseSection nlevel> of a sweep with its Samples
+  int L, nsection, nsample;
+  int sample_sp[8];
+  int sample_ch[8];
+  int sample_nm[8];
 };
 
 struct ChainP {
@@ -77,6 +86,9 @@ struct ChainP {
   int64_t perm_off;      // element offset of this chain's path permutation and its inverse ([2][N])
   int ns_max;            // largest fermion species with a node model
   int stat[4];           // species isStatic: no spring action, no virial kinetic terms
+  int ngroup;            // section groups of a sweep; group 0 is L, nsection, nsample, sample_*
+  int Lmax, Smax;        // largest group: sizes of the sampler's section buffers and caches
+  GroupP grp[PQ_GROUPS];
   double ceps;           // CoulombAction epsilon
   EwP ew;                // CoulombAction's Ewald sum (useEwald)
 };
@@ -99,6 +111,6 @@ static inline int64_t fn_cache_doubles(const ChainP& c) {
 // FixedNodeAction, [S] and [S][2][ns]).
 static inline int64_t fn_state_doubles(const ChainP& c) {
   int64_t n = 0;
-  for (int a = 0; a < c.nfn; ++a) n += 2 * c.S + 4 * (int64_t)c.S * c.fn[a].ns;
+  for (int a = 0; a < c.nfn; ++a) n += 2 * c.Smax + 4 * (int64_t)c.Smax * c.fn[a].ns;
   return n;
 }

@@ -138,12 +138,32 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
   }
   __syncthreads();
 
-  const int S = c.S, N = c.N, M = c.M, L = c.L;
+  const int N = c.N, M = c.M;
   const int64_t slice = (int64_t)N * D;
   double* beads = x.beads + c.bead_off;
   uint64_t k = x.counter[chain];
 
-  for (int sw = 0; sw < nsweep; ++sw) {
+  for (int sw = 0; sw < nsweep; ++sw)
+  for (int g = 0; g < c.ngroup; ++g) {
+    // the sweep's g-th <ChooseSection nlevel> with its Samples: its level
+    // count, section length and Samples take the chain parameters' place
+    if (c.ngroup > 1) {
+      __syncthreads();
+      if (tid == 0) {
+        const GroupP& G = c.grp[g];
+        c.L = G.L;
+        c.S = (1 << G.L) + 1;
+        c.nsection = G.nsection;
+        c.nsample = G.nsample;
+        for (int q = 0; q < G.nsample; ++q) {
+          c.sample_sp[q] = G.sample_sp[q];
+          c.sample_ch[q] = G.sample_ch[q];
+          c.sample_nm[q] = G.sample_nm[q];
+        }
+      }
+      __syncthreads();
+    }
+    const int S = c.S, L = c.L;
     for (int isec = 0; isec < c.nsection; ++isec) {
       // SectionChooser::run (SectionChooser.cc:30-48) / DoubleSectionChooser::run (:33-65)
       int t0, t0b = 0;

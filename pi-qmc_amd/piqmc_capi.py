@@ -60,6 +60,14 @@ class SampleDesc(C.Structure):
     _fields_ = [("species", C.c_int), ("nmoving", C.c_int), ("chooser", C.c_int)]
 
 
+PQ_MAX_GROUPS = 4
+
+
+class SectionGroup(C.Structure):
+    _fields_ = [("nlevel", C.c_int), ("nsample", C.c_int),
+                ("sample", SampleDesc * PQ_MAX_SAMPLES), ("nsection", C.c_int)]
+
+
 class ChainDesc(C.Structure):
     _fields_ = [("ndim", C.c_int), ("nslice", C.c_int), ("nlevel", C.c_int),
                 ("nspecies", C.c_int), ("tau", C.c_double), ("cell", C.c_double * 3),
@@ -68,7 +76,8 @@ class ChainDesc(C.Structure):
                 ("coulomb", CoulombDesc), ("nnodes", C.c_int),
                 ("nodes", NodeDesc * PQ_MAX_NODES), ("mover", C.c_int),
                 ("nsample", C.c_int), ("sample", SampleDesc * PQ_MAX_SAMPLES),
-                ("nsection", C.c_int)]
+                ("nsection", C.c_int), ("ngroup", C.c_int),
+                ("group", SectionGroup * (PQ_MAX_GROUPS - 1))]
 
     @property
     def npart(self):
@@ -127,6 +136,18 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
         d.sample[k].species, d.sample[k].nmoving = smp[0], smp[1]
         d.sample[k].chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
     d.nsection = nsection if nsection is not None else max(1, nslice >> nlevel)
+    return d
+
+
+def add_group(d, nlevel, samples, nsection):
+    """Append a <ChooseSection nlevel> group (samples [(species, npart[,
+    chooser])], repeated nsection times) to a sweep."""
+    g = d.group[d.ngroup]
+    g.nlevel, g.nsample, g.nsection = nlevel, len(samples), nsection
+    for k, smp in enumerate(samples):
+        g.sample[k].species, g.sample[k].nmoving = smp[0], smp[1]
+        g.sample[k].chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
+    d.ngroup += 1
     return d
 
 

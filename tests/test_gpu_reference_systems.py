@@ -1,4 +1,6 @@
-"""The reference's own system tests run on the device through the C ABI.
+"""The reference's own system tests run on the device through the C ABI:
+test/system/sho here, test/system/hatom below (section groups), and
+test/system/madelung in test_gpu_ewald.py.
 
 test/system/sho (pimc.xml, test_sho.py:27-32): one particle in a 3D
 oscillator, omega 0.5 Ha, T 1 Ha, 32 slices, <ChooseSection nlevel="5">, so
@@ -67,3 +69,66 @@ def test_reference_sho_system_test():
     de = math.sqrt(blocks.var() / (len(blocks) - 1))   # pitools Scalar.getAverage
     assert abs(e - 0.75 / math.tanh(0.25)) < 0.08, (e, de)
     assert abs(de - 0.041) < 0.01, de
+
+
+def test_section_groups_match_the_oracle():
+    """Sweeps of three <ChooseSection> groups with their own nlevel and
+    Samples (the hydrogen-atom input) make the oracle's decisions."""
+    from reference_systems import hatom_beads, hatom_desc
+    print("[test] section groups: hydrogen atom sweeps against the oracle", flush=True)
+    d = hatom_desc(False)
+    b = capi.Batch([d])
+    oc = ol.Chain(d)
+    b.set_paths(0, hatom_beads())
+    oc.set_beads(hatom_beads())
+    b.seed(3, 0)
+    oc.rng_philox(3, 0)
+    for _ in range(4):
+        b.run_sweeps(5)
+        b.sync()
+        oc.run_sweeps(5)
+        assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9
+    sg, so = b.stats(0), oc.stats()
+    assert sg.draws == so.draws and list(sg.tries) == list(so.tries) and list(sg.accepts) == list(so.accepts)
+    assert sg.tries[7] > 0 and sg.tries[5] > sg.tries[7]
+
+
+def test_reference_hydrogen_atom_system_test():
+    """test/system/hatom (test_hatom.py): 10000 thermalising sections, 100
+    blocks of 300 measured sweeps; thermal, virial (nwindow 500) and Coulomb
+    energies in eV against the reference's expectations, and every block
+    against the oracle's run of the same streams."""
+    from reference_systems import EV, HATOM_KT_EV, block_average, hatom_beads, hatom_desc
+    from test_oracle_systems import run_hatom_oracle
+    print("[test] hydrogen atom system test on the device", flush=True)
+    th = capi.Batch([hatom_desc(True)])
+    th.set_paths(0, hatom_beads())
+    th.seed(7, 0)
+    th.run_sweeps(10000)
+    b = capi.Batch([hatom_desc(False)])
+    b.set_paths(0, th.get_paths(0))
+    b.seed(8, 0)
+    E, V, C = [], [], []
+    for blk in range(100):
+        e = v = c = 0.0
+        for _ in range(300):
+            b.run_sweeps(1)
+            e += b.energy()[0]
+            v += b.virial_energy(500)[0]
+            c += b.coulomb_energy(1.0)[0]
+        E.append(e / 300 * EV)
+        V.append(v / 300 * EV)
+        C.append(c / 300 * EV)
+        if blk % 20 == 19:
+            print(f"[test] hydrogen atom block {blk + 1} of 100", flush=True)
+    print("[test] hydrogen atom device run done; oracle run", flush=True)
+    Eo, Vo, Co = run_hatom_oracle()
+    np.testing.assert_allclose(E, Eo, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(V, Vo, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(C, Co, rtol=1e-9, atol=1e-9)
+    e, de = block_average(E)
+    assert abs(e - (-13.6 + 1.5 * HATOM_KT_EV)) < 1.2 and abs(de - 0.60) < 0.2, (e, de)
+    v, dv = block_average(V)
+    assert abs(v - (-13.6 + 1.5 * HATOM_KT_EV)) < 0.3 and abs(dv - 0.16) < 0.05, (v, dv)
+    c, dc = block_average(C)
+    assert abs(c - (-2 * 13.6)) < 1.2 and abs(dc - 0.20) < 0.1, (c, dc)
