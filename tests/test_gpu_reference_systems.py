@@ -132,3 +132,44 @@ def test_reference_hydrogen_atom_system_test():
     assert abs(v - (-13.6 + 1.5 * HATOM_KT_EV)) < 0.3 and abs(dv - 0.16) < 0.05, (v, dv)
     c, dc = block_average(C)
     assert abs(c - (-2 * 13.6)) < 1.2 and abs(dc - 0.20) < 0.1, (c, dc)
+
+
+def _group_descs():
+    from gpu_helpers import small_cfg2
+    W = capi.PQ_CHOOSER_WALKING
+    # fermions with SHO nodes: sections of 2^4 links over both species, then
+    # 2 x 2^3 over one species and 3 x 2^2 over the other
+    ferm = small_cfg2(64)
+    capi.add_group(ferm, 3, [(0, 1)], 2)
+    capi.add_group(ferm, 2, [(1, 1)], 3)
+    # bosons in a trap: one-particle sections, then WalkingChooser cycles of
+    # two and three particles at other depths
+    bos = capi.make_desc(2, 64, 3, 2.0 / 64, [(4, 1.0, 0.0)], sho=[(0, 1.0)], cell=20.0,
+                         samples=[(0, 1, W)], nsection=4)
+    capi.add_group(bos, 4, [(0, 2, W)], 2)
+    capi.add_group(bos, 2, [(0, 3, W), (0, 1, W)], 3)
+    return {"fermions": ferm, "bosons": bos}
+
+
+@pytest.mark.parametrize("name", ["fermions", "bosons"])
+def test_section_groups_with_nodes_and_permutations(name):
+    print(f"[test] section groups {name}", flush=True)
+    d = _group_descs()[name]
+    b = capi.Batch([d])
+    oc = ol.Chain(d)
+    beads = capi.initial_beads(d, 11, replicate=True)
+    b.set_paths(0, beads)
+    oc.set_beads(beads)
+    b.seed(17, 0)
+    oc.rng_philox(17, 0)
+    for _ in range(4):
+        b.run_sweeps(3)
+        b.sync()
+        oc.run_sweeps(3)
+        assert list(b.get_permutation(0)) == list(oc.get_permutation())
+        assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9
+    sg, so = b.stats(0), oc.stats()
+    assert sg.draws == so.draws and list(sg.tries) == list(so.tries) and list(sg.accepts) == list(so.accepts)
+    assert sg.node_errors == so.node_errors
+    e, eo = b.energy()[0], oc.energy()
+    assert abs(e - eo) <= 1e-12 * max(1.0, abs(eo)), (e, eo)
