@@ -102,9 +102,13 @@ typedef struct pq_coulomb_desc {
   double rmin;         /* 0 means 0.005/(2 mu |q1 q2|) of the first interacting pair */
   double rmax;         /* 0 means the length of the cell diagonal */
   int ngpts;           /* 0 means 500 */
-  int use_ewald;       /* OptEwaldSum long-range part (src/util/OptEwaldSum.cc:27-173) */
+  int use_ewald;       /* long-range Ewald part: OptEwaldSum (src/util/OptEwaldSum.cc:27-173)
+                          or, with ewald_type 1, TradEwaldSum (src/util/TradEwaldSum.cc:13-36) */
   double ewald_rcut;   /* 0 means a[0]/2 */
   double ewald_kcut;   /* k-vectors with 0 < |k| < ewald_kcut */
+  int ewald_type;      /* 0 optEwald (the default), 1 tradEwald with ewaldImages = 1 */
+  double ewald_kappa;  /* tradEwald kappa; 0 means the parser's default
+                          sqrt((3.6 N)^(1/6) sqrt(pi) / min a) (ActionParser.cc:137-146) */
 } pq_coulomb_desc;
 
 /* <FixedNodeAction species model omega temperature noNodalAction>

@@ -378,6 +378,8 @@ static const double EW_PI = 3.14159265358979;   /* EwaldSum::PI (EwaldSum.cc:187
 
 typedef struct {
   int D, N, nk;
+  int trad;              /* TradEwaldSum (TradEwaldSum.cc:13-36) */
+  double kappa;
   double rcut, kcut;
   double coef[EW_NPOLY];
   int ikmax[3];
@@ -388,9 +390,46 @@ typedef struct {
   double q[MAXN];
 } ewald_t;
 
-/* evalFR (OptEwaldSum.cc:178-190) */
+/* the erf the reference links (src/util/erf.cpp): the incomplete gamma
+   function P(1/2, x^2) by its series below x^2 = 3/2 and by the Laguerre
+   continued fraction of Q = 1 - P above (H. Okumura, 1991, p. 227) */
+static double gamma_q(double a, double x, double lga);
+static double gamma_p(double a, double x, double lga) {
+  if (x >= 1 + a) return 1 - gamma_q(a, x, lga);
+  if (x == 0) return 0;
+  double term = exp(a * log(x) - x - lga) / a, sum = term;
+  for (int k = 1; k < 1000; k++) {
+    term *= x / (a + k);
+    double prev = sum;
+    sum += term;
+    if (sum == prev) break;
+  }
+  return sum;
+}
+static double gamma_q(double a, double x, double lga) {
+  if (x < 1 + a) return 1 - gamma_p(a, x, lga);
+  double la = 1, lb = 1 + x - a, w = exp(a * log(x) - x - lga), sum = w / lb;
+  for (int k = 2; k < 1000; k++) {
+    double t = ((k - 1 - a) * (lb - la) + (k + x) * lb) / k;
+    la = lb; lb = t;
+    w *= (k - 1 - a) / k;
+    double prev = sum;
+    sum += w / (la * lb);
+    if (sum == prev) break;
+  }
+  return sum;
+}
+static double ref_erf(double x) {
+  const double lgh = 0.572364942924700087071713675675;
+  if (!isfinite(x)) return x > 0 ? 1.0 : -1.0;
+  return x >= 0 ? gamma_p(0.5, x * x, lgh) : -gamma_p(0.5, x * x, lgh);
+}
+double oq_ref_erf(double x) { return ref_erf(x); }
+
+/* evalFR (OptEwaldSum.cc:178-190; TradEwaldSum.cc:24-26) */
 static double ew_fr(const ewald_t* e, double r) {
   double v = 0.;
+  if (e->trad) return ref_erf(e->kappa * r) / r;
   if (r > e->rcut) return 1. / r;
   double r2 = r * r;
   for (int i = EW_NPOLY - 1; i >= 0; --i) { v *= r2; v += e->coef[i]; }
@@ -438,6 +477,8 @@ static void ew_series3d(double k, double rcut, double* fm1, double* ft) {
 /* evalFK (OptEwaldSum.cc:192-250), in its own operation order: (4 PI/k) Im */
 static double ew_fk(const ewald_t* e, double k) {
   double fm1, f[EW_NPOLY];
+  /* TradEwaldSum::evalFK, the 3D transform in 2D too (TradEwaldSum.cc:32-34) */
+  if (e->trad) return 4 * EW_PI * exp(-(k * k) / (4 * e->kappa * e->kappa)) / (k * k);
   if (e->D == 3) {
     const double rcut = e->rcut;
     double v = 4. * EW_PI / (k * k);
@@ -466,6 +507,7 @@ static double ew_fk(const ewald_t* e, double k) {
 /* evalFK0 (OptEwaldSum.cc:252-270) */
 static double ew_fk0(const ewald_t* e) {
   const double rc = e->rcut;
+  if (e->trad) return -EW_PI / (e->kappa * e->kappa);
   if (e->D == 3) {
     double fn = 2. * EW_PI * rc * rc, v = -fn;
     fn *= rc / 3.;
@@ -594,9 +636,10 @@ static void ew_fit(ewald_t* e, const double* a_cell) {
    evalSelfEnergy (EwaldSum.cc:90-97,180-185) with the particles' charges, as
    CoulombAction's constructor leaves them (CoulombAction.cc:28-44) */
 static void ewald_init(ewald_t* e, const cell_t* cell, int N, const double* q,
-                       double rcut, double kcut) {
+                       double rcut, double kcut, int trad, double kappa) {
   const int D = cell->D;
   e->D = D; e->N = N;
+  e->trad = trad; e->kappa = kappa;
   e->rcut = rcut; e->kcut = kcut;
   for (int i = 0; i < 3; ++i) { e->ikmax[i] = 0; e->dk[i] = 0; }
   for (int i = 0; i < D; ++i) {
@@ -606,7 +649,7 @@ static void ewald_init(ewald_t* e, const cell_t* cell, int N, const double* q,
   double vol = 1;
   for (int i = 0; i < D; ++i) vol *= cell->a[i];
   e->oneOver2V = 0.5 / vol;
-  ew_fit(e, cell->a);
+  if (!trad) ew_fit(e, cell->a);
   int cap = 64;
   e->kv = (int*)malloc(sizeof(int) * 3 * cap);
   e->nk = 0;
@@ -636,7 +679,8 @@ static void ewald_init(ewald_t* e, const cell_t* cell, int N, const double* q,
   }
   double Q = 0, Q2 = 0;
   for (int i = 0; i < N; ++i) { e->q[i] = q[i]; Q += q[i]; Q2 += q[i] * q[i]; }
-  e->self = -0.5 * Q2 * e->coef[0] + e->oneOver2V * ew_fk0(e) * Q * Q;
+  const double fr0 = trad ? 2 * kappa / sqrt(EW_PI) : e->coef[0];   /* evalFR0 */
+  e->self = -0.5 * Q2 * fr0 + e->oneOver2V * ew_fk0(e) * Q * Q;
 }
 
 static void ewald_free(ewald_t* e) { free(e->kv); free(e->vk); e->kv = 0; e->vk = 0; }
@@ -878,7 +922,14 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
       double rc = desc->coulomb.ewald_rcut, q[MAXN];
       if (rc == 0.) rc = desc->cell[0] / 2.;
       for (int i = 0; i < ch->N; ++i) q[i] = desc->species[ch->spec[i]].charge;
-      ewald_init(&ch->ew, &ch->cell, ch->N, q, rc, desc->coulomb.ewald_kcut);
+      double kappa = desc->coulomb.ewald_kappa;
+      if (desc->coulomb.ewald_type == 1 && kappa == 0) {
+        /* the parser's default (ActionParser.cc:137-146) */
+        double lside = 1000000000000000.0;
+        for (int d = 0; d < ch->D; ++d) if (lside > desc->cell[d]) lside = desc->cell[d];
+        kappa = sqrt(pow(3.6 * ch->N, 1.0 / 6.0) * sqrt(3.1415926535897931) / lside);
+      }
+      ewald_init(&ch->ew, &ch->cell, ch->N, q, rc, desc->coulomb.ewald_kcut, desc->coulomb.ewald_type == 1, kappa);
       ch->has_ewald = 1;
     }
     double rmin = desc->coulomb.rmin, rmax = desc->coulomb.rmax;
@@ -2215,7 +2266,7 @@ double oq_ewald_coulomb_energy(const oq_chain* ch, double epsilon, double rcut, 
   double q[MAXN];
   for (int i = 0; i < N; ++i) q[i] = ch->d.species[ch->spec[i]].charge;
   ewald_t e;
-  ewald_init(&e, &ch->cell, N, q, rcut, kcut);
+  ewald_init(&e, &ch->cell, N, q, rcut, kcut, 0, 0.0);
   const int nradial = 1001;
   const double dr = rcut / 1000, drinv = 1. / dr;
   double* vgrid = (double*)malloc(sizeof(double) * nradial);

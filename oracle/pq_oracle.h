@@ -84,6 +84,8 @@ double oq_ewald_coulomb_energy(const oq_chain* ch, double epsilon, double rcut, 
 int oq_ewald_info(const oq_chain* ch, double* coef, int* nk, double* self, int ik, int* kv, double* vk);
 double oq_ewald_long_range(const oq_chain* ch, const double* r);
 double oq_ewald_fr(const oq_chain* ch, double r);
+/* the reference's erf (src/util/erf.cpp) */
+double oq_ref_erf(double x);
 /* getBeadAction of the composite action: u, utau, forces fm/fp [D] */
 void oq_bead_action(const oq_chain* ch, int ipart, int islice, double* u, double* utau,
                     double* fm, double* fp);

@@ -184,7 +184,7 @@ struct pq_batch {
   std::vector<double> hu, hut;
   // Ewald sums of the chains' CoulombActions: fits and the k pool [nk][4]
   std::vector<EwaldFit> fits;
-  std::map<std::tuple<int, double, double, double, double, double>, int> fit_of;
+  std::map<std::tuple<int, double, double, double, double, double, int, double>, int> fit_of;
   std::vector<int64_t> fit_off;
   std::vector<double> hk;
   double* dewk = nullptr;
@@ -264,12 +264,12 @@ static DevCtx make_ctx(const pq_batch* b) {
 // The OptEwaldSum of a cell, rcut and kcut, built once per batch: its index
 // in b->fits (its k-vectors at b->fit_off[index] of the k pool), -1 when the
 // fit is not finite.
-static int ewald_fit(pq_batch* b, int D, const double* a, double rcut, double kcut) {
-  const auto key = std::make_tuple(D, a[0], a[1], D == 3 ? a[2] : 0.0, rcut, kcut);
+static int ewald_fit(pq_batch* b, int D, const double* a, double rcut, double kcut, int type, double kappa) {
+  const auto key = std::make_tuple(D, a[0], a[1], D == 3 ? a[2] : 0.0, rcut, kcut, type, kappa);
   auto it = b->fit_of.find(key);
   if (it != b->fit_of.end()) return it->second;
   EwaldFit e;
-  if (!ewald_build(e, D, a, rcut, kcut)) return -1;
+  if (type ? !ewald_build_trad(e, D, a, rcut, kcut, kappa) : !ewald_build(e, D, a, rcut, kcut)) return -1;
   const int id = (int)b->fits.size();
   b->fit_off.push_back((int64_t)b->hk.size());
   for (int k = 0; k < e.nk(); ++k) {
@@ -384,8 +384,16 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     if (d.coulomb.use_ewald) {
       const double rc = d.coulomb.ewald_rcut == 0. ? d.cell[0] / 2. : d.coulomb.ewald_rcut;
       if (!(rc > 0) || !(d.coulomb.ewald_kcut > 0)) return fail(PQ_ERR_ARG, "Ewald sum needs rcut > 0 and kcut > 0");
-      fit = ewald_fit(b, c.D, d.cell, rc, d.coulomb.ewald_kcut);
-      if (fit < 0) return fail(PQ_ERR_ARG, "the Ewald fit for rcut %g, kcut %g is not finite", rc, d.coulomb.ewald_kcut);
+      if (d.coulomb.ewald_type != 0 && d.coulomb.ewald_type != 1) return fail(PQ_ERR_ARG, "unknown ewald_type");
+      double kappa = d.coulomb.ewald_kappa;
+      if (d.coulomb.ewald_type == 1 && kappa == 0) {
+        // the parser's default (ActionParser.cc:137-146)
+        double lside = 1000000000000000.0;
+        for (int k = 0; k < c.D; ++k) lside = std::min(lside, d.cell[k]);
+        kappa = std::sqrt(std::pow(3.6 * c.N, 1.0 / 6.0) * std::sqrt(3.1415926535897931) / lside);
+      }
+      fit = ewald_fit(b, c.D, d.cell, rc, d.coulomb.ewald_kcut, d.coulomb.ewald_type, kappa);
+      if (fit < 0) return fail(PQ_ERR_ARG, "the Ewald sum for rcut %g, kcut %g is not finite", rc, d.coulomb.ewald_kcut);
       ewald_params(b, fit, d, c.N, c.ew);
     }
     const EwaldFit* ewp = fit >= 0 ? &b->fits[fit] : nullptr;

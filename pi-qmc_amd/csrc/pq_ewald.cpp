@@ -1,5 +1,6 @@
-// pq_ewald.cpp -- the optimized Ewald sum's fit and k-vectors on the host
-// (src/util/OptEwaldSum.cc:27-280, src/util/EwaldSum.cc:16-112).  The pair
+// pq_ewald.cpp -- the Ewald sums' f(r), f(k) and k-vectors on the host
+// (src/util/OptEwaldSum.cc:27-280, src/util/TradEwaldSum.cc:13-36,
+// src/util/EwaldSum.cc:16-112).  The pair
 // tables subtract tau q1 q2 f(r) from u_0 with this f, and the device sums
 // w_k |rho_k|^2 over these k-vectors, so both follow the reference's
 // arithmetic term by term; the file is built without floating-point
@@ -96,9 +97,54 @@ void lu_solve(const double* a, int n, const int* piv, double* b) {
   }
 }
 
+// the incomplete gamma functions P(a, x) and Q(a, x) = 1 - P(a, x): the
+// power series below x = 1 + a and the Laguerre continued fraction above,
+// each summed until the sum stops changing (at most 1000 terms)
+double q_gamma(double a, double x, double loggamma_a);
+
+double p_gamma(double a, double x, double loggamma_a) {
+  if (x >= 1 + a) return 1 - q_gamma(a, x, loggamma_a);
+  if (x == 0) return 0;
+  double term = std::exp(a * std::log(x) - x - loggamma_a) / a;
+  double sum = term;
+  for (int k = 1; k < 1000; ++k) {
+    term *= x / (a + k);
+    const double prev = sum;
+    sum += term;
+    if (sum == prev) break;
+  }
+  return sum;
+}
+
+double q_gamma(double a, double x, double loggamma_a) {
+  if (x < 1 + a) return 1 - p_gamma(a, x, loggamma_a);
+  double la = 1, lb = 1 + x - a;
+  double w = std::exp(a * std::log(x) - x - loggamma_a);
+  double sum = w / lb;
+  for (int k = 2; k < 1000; ++k) {
+    const double t = ((k - 1 - a) * (lb - la) + (k + x) * lb) / k;
+    la = lb;
+    lb = t;
+    w *= (k - 1 - a) / k;
+    const double prev = sum;
+    sum += w / (la * lb);
+    if (sum == prev) break;
+  }
+  return sum;
+}
+
 }  // namespace
 
+double okumura_erf(double x) {
+  const double log_gamma_half = 0.572364942924700087071713675675;   // ln(pi)/2
+  if (!std::isfinite(x)) return x > 0 ? 1.0 : -1.0;
+  return x >= 0 ? p_gamma(0.5, x * x, log_gamma_half) : -p_gamma(0.5, x * x, log_gamma_half);
+}
+
+double EwaldFit::fr0() const { return trad ? 2 * kappa / std::sqrt(kPi) : coef[0]; }
+
 double EwaldFit::fr(double r) const {
+  if (trad) return okumura_erf(kappa * r) / r;
   if (r > rcut) return 1. / r;
   double v = 0., r2 = r * r;
   for (int i = NP - 1; i >= 0; --i) {
@@ -109,6 +155,7 @@ double EwaldFit::fr(double r) const {
 }
 
 double EwaldFit::fk0() const {
+  if (trad) return -kPi / (kappa * kappa);
   if (D == 3) {
     double fn = 2. * kPi * rcut * rcut, v = -fn;
     fn *= rcut / 3.;
@@ -130,11 +177,61 @@ double EwaldFit::fk0() const {
 double EwaldFit::self_energy(const double* q, int n) const {
   double Q = 0, Q2 = 0;
   for (int i = 0; i < n; ++i) { Q += q[i]; Q2 += q[i] * q[i]; }
-  return -0.5 * Q2 * coef[0] + oneOver2V * fk0() * Q * Q;
+  return -0.5 * Q2 * fr0() + oneOver2V * fk0() * Q * Q;
+}
+
+// EwaldSum's half-space k-vectors with 0 < |k| < kcut (EwaldSum.cc:16-88);
+// vk holds |k|^2 until the caller replaces it by f(|k|)
+static void ewald_kvectors(EwaldFit& e, int D, const double* a, double kcut) {
+  int ikmax[3] = {0, 0, 0};
+  double dk[3] = {0, 0, 0}, vol = 1;
+  for (int i = 0; i < D; ++i) {
+    ikmax[i] = (int)(a[i] * kcut / (2 * kPi));
+    dk[i] = 2 * kPi * (1. / a[i]);
+    vol *= a[i];
+  }
+  e.oneOver2V = 0.5 / vol;
+  e.kvec.clear();
+  e.vk.clear();
+  for (int kx = 0; kx <= ikmax[0]; ++kx) {
+    const double kx2 = kx * kx * dk[0] * dk[0];
+    for (int ky = (kx == 0) ? 0 : -ikmax[1]; ky <= ikmax[1]; ++ky) {
+      const double ky2 = ky * ky * dk[1] * dk[1];
+      const int kzlo = (D == 3) ? ((kx == 0 && ky == 0) ? 0 : -ikmax[2]) : 0;
+      const int kzhi = (D == 3) ? ikmax[2] : 0;
+      for (int kz = kzlo; kz <= kzhi; ++kz) {
+        double k2 = kx2 + ky2;
+        if (D == 3) k2 += kz * kz * dk[2] * dk[2];
+        if (k2 < kcut * kcut && k2 != 0) {
+          e.kvec.push_back(kx * dk[0]);
+          e.kvec.push_back(ky * dk[1]);
+          e.kvec.push_back(kz * dk[2]);
+          e.vk.push_back(k2);
+        }
+      }
+    }
+  }
+}
+
+bool ewald_build_trad(EwaldFit& e, int D, const double* a, double rcut, double kcut, double kappa) {
+  e.D = D;
+  e.trad = true;
+  e.kappa = kappa;
+  e.rcut = rcut;
+  e.kcut = kcut;
+  ewald_kvectors(e, D, a, kcut);
+  // TradEwaldSum::evalFK, the 3D transform in 2D too, as the reference has it
+  for (double& w : e.vk) {
+    const double k = std::sqrt(w);
+    w = 4 * kPi * std::exp(-(k * k) / (4 * kappa * kappa)) / (k * k);
+    if (!std::isfinite(w)) return false;
+  }
+  return std::isfinite(kappa) && kappa > 0;
 }
 
 bool ewald_build(EwaldFit& e, int D, const double* a, double rcut, double kcut) {
   e.D = D;
+  e.trad = false;
   e.rcut = rcut;
   e.kcut = kcut;
   // --- the constrained fit over the halo kcut < k < 4 kcut (OptEwaldSum.cc:33-173)
@@ -219,35 +316,7 @@ bool ewald_build(EwaldFit& e, int D, const double* a, double rcut, double kcut) 
     if (!std::isfinite(t[i])) return false;
     e.coef[i] = t[i];
   }
-  // --- k-vectors of EwaldSum: half space, 0 < |k| < kcut (EwaldSum.cc:16-88)
-  int ikmax[3] = {0, 0, 0};
-  double dk[3] = {0, 0, 0}, vol = 1;
-  for (int i = 0; i < D; ++i) {
-    ikmax[i] = (int)(a[i] * kcut / (2 * kPi));
-    dk[i] = 2 * kPi * (1. / a[i]);
-    vol *= a[i];
-  }
-  e.oneOver2V = 0.5 / vol;
-  e.kvec.clear();
-  e.vk.clear();
-  for (int kx = 0; kx <= ikmax[0]; ++kx) {
-    const double kx2 = kx * kx * dk[0] * dk[0];
-    for (int ky = (kx == 0) ? 0 : -ikmax[1]; ky <= ikmax[1]; ++ky) {
-      const double ky2 = ky * ky * dk[1] * dk[1];
-      const int kzlo = (D == 3) ? ((kx == 0 && ky == 0) ? 0 : -ikmax[2]) : 0;
-      const int kzhi = (D == 3) ? ikmax[2] : 0;
-      for (int kz = kzlo; kz <= kzhi; ++kz) {
-        double k2 = kx2 + ky2;
-        if (D == 3) k2 += kz * kz * dk[2] * dk[2];
-        if (k2 < kcut * kcut && k2 != 0) {
-          e.kvec.push_back(kx * dk[0]);
-          e.kvec.push_back(ky * dk[1]);
-          e.kvec.push_back(kz * dk[2]);
-          e.vk.push_back(k2);
-        }
-      }
-    }
-  }
+  ewald_kvectors(e, D, a, kcut);
   // setLongRangeArray: vk = evalFK(|k|) (OptEwaldSum.cc:192-250)
   for (double& w : e.vk) {
     const double k = std::sqrt(w);

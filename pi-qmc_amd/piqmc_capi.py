@@ -48,7 +48,8 @@ class ShoDesc(C.Structure):
 class CoulombDesc(C.Structure):
     _fields_ = [("enabled", C.c_int), ("epsilon", C.c_double), ("norder", C.c_int),
                 ("rmin", C.c_double), ("rmax", C.c_double), ("ngpts", C.c_int),
-                ("use_ewald", C.c_int), ("ewald_rcut", C.c_double), ("ewald_kcut", C.c_double)]
+                ("use_ewald", C.c_int), ("ewald_rcut", C.c_double), ("ewald_kcut", C.c_double),
+                ("ewald_type", C.c_int), ("ewald_kappa", C.c_double)]
 
 
 class NodeDesc(C.Structure):
@@ -124,6 +125,8 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
         d.coulomb.use_ewald = 1 if coulomb.get("ewald") else 0
         d.coulomb.ewald_rcut = coulomb.get("ewald_rcut", 0.0)
         d.coulomb.ewald_kcut = coulomb.get("ewald_kcut", 0.0)
+        d.coulomb.ewald_type = 1 if coulomb.get("ewald_type", "opt") == "trad" else 0
+        d.coulomb.ewald_kappa = coulomb.get("ewald_kappa", 0.0)
     d.nnodes = len(nodes)
     for k, (s, model, w) in enumerate(nodes):
         d.nodes[k].species, d.nodes[k].model, d.nodes[k].omega = s, model, w

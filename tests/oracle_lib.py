@@ -68,6 +68,7 @@ def load():
         "oq_ewald_info": (C.c_int, [vp, dp, ip, dp, C.c_int, ip, dp]),
         "oq_ewald_long_range": (C.c_double, [vp, dp]),
         "oq_ewald_fr": (C.c_double, [vp, C.c_double]),
+        "oq_ref_erf": (C.c_double, [C.c_double]),
         "oq_bead_action": (None, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
         "oq_virial_energy": (C.c_double, [vp, C.c_int]),
         "oq_energy_blocks": (None, [vp, C.c_int, C.c_int, dp]),

@@ -191,3 +191,41 @@ def test_static_species_skip_the_spring():
     mov = sec[:, [0], :] + 0.1
     assert ch.action_difference(sec, mov, [0], 0, mask=1) == 0.0
     assert ch.action_difference(sec, sec[:, [1], :] + 0.1, [1], 0, mask=1) != 0.0
+
+
+def test_reference_erf():
+    from scipy.special import erf
+    lib = ol.load()
+    xs = np.concatenate([np.linspace(-6, 6, 241), [1e-8, 0.5, 1.2247, 1.2248, 3.0]])
+    got = np.array([lib.oq_ref_erf(float(x)) for x in xs])
+    np.testing.assert_allclose(got, erf(xs), rtol=2e-15, atol=2e-16)
+
+
+def test_traditional_ewald_matches_erfc_ewald():
+    """ewaldType="tradEwald": the tables lose tau q1q2 erf(kappa r)/r at
+    every r, and the k-space sum with 4 PI exp(-k^2/4 kappa^2)/k^2 restores
+    it; the total equals the error-function Ewald sum with the same kappa."""
+    D, L = 3, 3.0
+    kappa = 5.0 / (L / 2)
+    d = capi.make_desc(D, 8, 1, 0.1, [(3, 1.0, 1.0), (3, 1.0, -1.0)], cell=L,
+                       coulomb=dict(norder=0, ngpts=200, ewald=True, ewald_type="trad",
+                                    ewald_kappa=kappa, ewald_kcut=40.0))
+    ch = ol.Chain(d)
+    q = np.array([1.0] * 3 + [-1.0] * 3)
+    rs = np.random.default_rng(17)
+    for _ in range(3):
+        pos = rs.uniform(0, L, size=(6, D))
+        exact = erfc_ewald(pos, q, L, D, kappa)
+        sr = 0.0
+        for i in range(6):
+            for j in range(i):
+                dv = pos[i] - pos[j]
+                dv -= L * np.round(dv / L)
+                r = float(np.linalg.norm(dv))
+                sr += q[i] * q[j] * (1 / r - ch.ewald_fr(r))
+        assert abs(sr + ch.ewald_long_range(pos) - exact) < 1e-10 * max(1.0, abs(exact))
+    # the parser's default kappa, sqrt((3.6 N)^(1/6) sqrt(pi) / min a)
+    d.coulomb.ewald_kappa = 0.0
+    ch0 = ol.Chain(d)
+    k0 = math.sqrt((3.6 * 6) ** (1 / 6) * math.sqrt(math.pi) / L)
+    assert ch0.ewald_fr(0.7) == pytest.approx(ol.load().oq_ref_erf(k0 * 0.7) / 0.7, rel=1e-15)

@@ -13,13 +13,14 @@ from test_ewald import MADELUNG, nacl_beads, nacl_desc
 pytestmark = pytest.mark.gpu
 
 
-def box_desc(D=3, L=3.0, nslice=32, nlevel=3, eps=1.0, kcut=None, nsection=None):
+def box_desc(D=3, L=3.0, nslice=32, nlevel=3, eps=1.0, kcut=None, nsection=None, ewald_type="opt"):
     """Two +1 and two -1 charges (masses 1 and 2) in a periodic box with the
-    Ewald sum, FreeMoverPBC, one Sample per species."""
+    Ewald sum (optEwald, or tradEwald with the parser's default kappa),
+    FreeMoverPBC, one Sample per species."""
     kc = kcut if kcut is not None else 2 * np.pi * 3 / L
     return capi.make_desc(D, nslice, nlevel, 0.05, [(2, 1.0, 1.0), (2, 2.0, -1.0)], cell=L,
                           coulomb=dict(epsilon=eps, norder=2, ngpts=400, rmin=0.005, rmax=L * np.sqrt(D),
-                                       ewald=True, ewald_rcut=L / 2, ewald_kcut=kc),
+                                       ewald=True, ewald_rcut=L / 2, ewald_kcut=kc, ewald_type=ewald_type),
                           mover=capi.PQ_MOVER_FREE_PBC, nsection=nsection)
 
 
@@ -48,11 +49,11 @@ def test_madelung_on_device():
     assert rel_close(ev, oc.virial_energy(1), 1e-11, scale=1.0), (ev, oc.virial_energy(1))
 
 
-@pytest.mark.parametrize("D,eps,prec", [(3, 1.0, capi.PQ_PREC_FP64), (2, 2.0, capi.PQ_PREC_FP64),
-                                        (3, 1.0, capi.PQ_PREC_MIXED)])
-def test_ewald_action_difference(D, eps, prec):
-    print(f"[test] ewald action difference D={D} eps={eps} prec={prec}", flush=True)
-    d = box_desc(D=D, eps=eps, L=3.0 if D == 3 else 4.0, kcut=None if D == 3 else 10.0)
+@pytest.mark.parametrize("D,eps,prec,kind", [(3, 1.0, capi.PQ_PREC_FP64, "opt"), (2, 2.0, capi.PQ_PREC_FP64, "opt"),
+                                             (3, 1.0, capi.PQ_PREC_MIXED, "opt"), (3, 1.0, capi.PQ_PREC_FP64, "trad")])
+def test_ewald_action_difference(D, eps, prec, kind):
+    print(f"[test] ewald action difference D={D} eps={eps} prec={prec} {kind}", flush=True)
+    d = box_desc(D=D, eps=eps, L=3.0 if D == 3 else 4.0, kcut=None if D == 3 else 10.0, ewald_type=kind)
     b = capi.Batch([d], precision=prec)
     oc = ol.Chain(d)
     rng = np.random.default_rng(21 + D)
@@ -82,10 +83,10 @@ def test_ewald_action_difference(D, eps, prec):
     assert min(abs(x) for x in ew) > 1e-8
 
 
-@pytest.mark.parametrize("team", [-1, 128])
-def test_ewald_sampler_decisions(team):
-    print(f"[test] ewald sampler team={team}", flush=True)
-    d = box_desc(nslice=32, nlevel=3)
+@pytest.mark.parametrize("team,kind", [(-1, "opt"), (128, "opt"), (-1, "trad")])
+def test_ewald_sampler_decisions(team, kind):
+    print(f"[test] ewald sampler team={team} {kind}", flush=True)
+    d = box_desc(nslice=32, nlevel=3, ewald_type=kind)
     b = capi.Batch([d])
     b.set_schedule(team, -1)
     oc = ol.Chain(d)
