@@ -121,11 +121,13 @@ typedef struct pq_node_desc {
   int nodal_action;    /* 1 unless noNodalAction="true" */
 } pq_node_desc;
 
-/* <Sample npart species noPermutation> inside <ChooseSection>. */
+/* <Sample npart species noPermutation nrepeat> inside <ChooseSection>. */
 typedef struct pq_sample_desc {
   int species;         /* ignored by the PQ_CHOOSER_ALL* choosers */
   int nmoving;         /* npart: 1..PQ_MAX_MOVING, at most the particles it chooses from */
   int chooser;         /* PQ_CHOOSER_* */
+  int nrepeat;         /* <Sample nrepeat>: tries on the same section, 0 means 1
+                          (MultiLevelSampler.cc:68-98, DoubleMLSampler.cc:66-120) */
 } pq_sample_desc;
 
 /* A further <ChooseSection nlevel> of the sweep with its <Sample> elements,

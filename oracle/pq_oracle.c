@@ -1979,7 +1979,7 @@ static int choose(oq_chain* ch, const pq_sample_desc* s, int* idx, int* cyc, dou
    from the chosen particles' first beads and end on the permuted particles'
    last beads; an accepted move writes them back (last slice included) and
    appends the permutation to the section permutation. */
-static int sample(oq_chain* ch, const pq_sample_desc* smp) {
+static int sample_once(oq_chain* ch, const pq_sample_desc* smp) {
   const int D = ch->D, S = ch->S, L = ch->L;
   const int nmov = smp->nmoving;
   int* idx = ch->idx;
@@ -2014,6 +2014,15 @@ static int sample(oq_chain* ch, const pq_sample_desc* smp) {
   for (int m = 0; m < nmov; ++m) ch->secperm[idx[m]] = temp[cyc[m]];
   ch->st.accepted_bead_updates += (uint64_t)nmov * ((1u << L) - 1);
   return 1;
+}
+
+/* MultiLevelSampler::run / DoubleMLSampler::run (MultiLevelSampler.cc:68-98,
+   DoubleMLSampler.cc:66-120): nrepeat tries on one section; the chooser's
+   tables are rebuilt after an accepted try (init()), which the per-try
+   tables of walk_choose equal, the section being unchanged otherwise */
+static void sample(oq_chain* ch, const pq_sample_desc* smp) {
+  const int nrep = smp->nrepeat > 0 ? smp->nrepeat : 1;
+  for (int r = 0; r < nrep; ++r) sample_once(ch, smp);
 }
 
 /* SectionChooser::run (src/advancer/SectionChooser.cc:30-48) and

@@ -510,6 +510,8 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
       G.sample_sp[k] = all ? 0 : smp[k].species;
       G.sample_ch[k] = ch;
       G.sample_nm[k] = smp[k].nmoving;
+      if (smp[k].nrepeat < 0) return fail(PQ_ERR_ARG, "sample %d: nrepeat must not be negative", k);
+      G.sample_rep[k] = smp[k].nrepeat == 0 ? 1 : smp[k].nrepeat;
       // a permuted section relabels the slices after it, which a section of
       // the whole path (2^L = M) would overlap
       const bool permuting = (ch == PQ_CHOOSER_WALKING || ch == PQ_CHOOSER_ALL_RANDOM) && smp[k].nmoving > 1;
@@ -522,6 +524,7 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     c.sample_sp[k] = c.grp[0].sample_sp[k];
     c.sample_ch[k] = c.grp[0].sample_ch[k];
     c.sample_nm[k] = c.grp[0].sample_nm[k];
+    c.sample_rep[k] = c.grp[0].sample_rep[k];
   }
   c.nsection = d.nsection;
   return PQ_OK;

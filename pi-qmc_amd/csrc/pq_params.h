@@ -48,6 +48,7 @@ struct GroupP {          // one <ChooseSection nlevel> of a sweep with its Sampl
   int sample_sp[8];
   int sample_ch[8];
   int sample_nm[8];
+  int sample_rep[8];
 };
 
 struct ChainP {
@@ -71,6 +72,7 @@ struct ChainP {
   int sample_sp[8];
   int sample_ch[8];      // PQ_CHOOSER_* of each Sample
   int sample_nm[8];      // moving particles of each Sample
+  int sample_rep[8];     // tries of each Sample on one section (nrepeat)
   int nsection;
   double spring_c[PQ_PG_LEVELS][4];          // 1/(4 lambda tau 2^l) per level, species
   double sho_cosh[4][PQ_PG_LEVELS];          // cosh(omega tau 2^l) per SHO action

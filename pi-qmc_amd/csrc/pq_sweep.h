@@ -159,6 +159,7 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
           c.sample_sp[q] = G.sample_sp[q];
           c.sample_ch[q] = G.sample_ch[q];
           c.sample_nm[q] = G.sample_nm[q];
+          c.sample_rep[q] = G.sample_rep[q];
         }
       }
       __syncthreads();
@@ -209,7 +210,10 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
         if (tid == 0) cnt[18] += ne;
       }
 
-      for (int smp = 0; smp < c.nsample; ++smp) {
+      // every Sample tries nrepeat moves on the section (MultiLevelSampler::run,
+      // MultiLevelSampler.cc:68-98); the choosers read the current section
+      for (int smp = 0; smp < c.nsample; ++smp)
+      for (int rep = 0; rep < c.sample_rep[smp]; ++rep) {
         double lnTP;
         if (!e.choose(smp, k, lnTP)) continue;
         const int nmov = e.nmov();

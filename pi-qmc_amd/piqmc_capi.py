@@ -58,7 +58,8 @@ class NodeDesc(C.Structure):
 
 
 class SampleDesc(C.Structure):
-    _fields_ = [("species", C.c_int), ("nmoving", C.c_int), ("chooser", C.c_int)]
+    _fields_ = [("species", C.c_int), ("nmoving", C.c_int), ("chooser", C.c_int),
+                ("nrepeat", C.c_int)]
 
 
 PQ_MAX_GROUPS = 4
@@ -100,7 +101,7 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
     """species: [(count, mass, charge[, is_static])]; sho: [(species, omega)];
     coulomb: dict(epsilon, norder, rmin, rmax, ngpts[, ewald, ewald_rcut,
     ewald_kcut]) or None;
-    nodes: [(species, model, omega)]; samples [(species, npart[, chooser])],
+    nodes: [(species, model, omega)]; samples [(species, npart[, chooser[, nrepeat]])],
     default one npart=1 SpeciesParticleChooser per species;
     nsection default M / 2^L (one sweep touches every slice once on average)."""
     d = ChainDesc()
@@ -138,6 +139,7 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
     for k, smp in enumerate(samples):
         d.sample[k].species, d.sample[k].nmoving = smp[0], smp[1]
         d.sample[k].chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
+        d.sample[k].nrepeat = smp[3] if len(smp) > 3 else 1
     d.nsection = nsection if nsection is not None else max(1, nslice >> nlevel)
     return d
 
@@ -150,6 +152,7 @@ def add_group(d, nlevel, samples, nsection):
     for k, smp in enumerate(samples):
         g.sample[k].species, g.sample[k].nmoving = smp[0], smp[1]
         g.sample[k].chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
+        g.sample[k].nrepeat = smp[3] if len(smp) > 3 else 1
     d.ngroup += 1
     return d
 

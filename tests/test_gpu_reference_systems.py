@@ -148,10 +148,16 @@ def _group_descs():
                          samples=[(0, 1, W)], nsection=4)
     capi.add_group(bos, 4, [(0, 2, W)], 2)
     capi.add_group(bos, 2, [(0, 3, W), (0, 1, W)], 3)
-    return {"fermions": ferm, "bosons": bos}
+    # <Sample nrepeat>: three tries per section for the fermions' first
+    # Sample, two for a WalkingChooser cycle of the bosons
+    rep_f = small_cfg2(64)
+    rep_f.sample[0].nrepeat = 3
+    rep_b = capi.make_desc(2, 64, 3, 2.0 / 64, [(4, 1.0, 0.0)], sho=[(0, 1.0)], cell=20.0,
+                           samples=[(0, 2, W, 2), (0, 1, W, 3)], nsection=4)
+    return {"fermions": ferm, "bosons": bos, "repeat-fermions": rep_f, "repeat-bosons": rep_b}
 
 
-@pytest.mark.parametrize("name", ["fermions", "bosons"])
+@pytest.mark.parametrize("name", ["fermions", "bosons", "repeat-fermions", "repeat-bosons"])
 def test_section_groups_with_nodes_and_permutations(name):
     print(f"[test] section groups {name}", flush=True)
     d = _group_descs()[name]

@@ -453,3 +453,19 @@ def test_oracle_rejects_more_moving_particles_than_the_pool():
     with pytest.raises(ValueError):
         ol.Chain(d)
     ol.Chain(capi.make_desc(2, 8, 2, 0.1, [(3, 1.0, 0.0)], samples=[(0, 3)]))
+
+
+def test_sample_nrepeat_tries_and_balance():
+    """<Sample nrepeat="4">: four tries on every section (the top level is
+    tried 4 x nsection times per sweep) and the same SHO energy,
+    E = 0.75 coth(0.25) for omega 0.5, T 1 in 3D."""
+    d = capi.make_desc(3, 32, 4, 1 / 32, [(1, 1.0, -1.0)], cell=20.0, sho=[(0, 0.5)],
+                       samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 4)], nsection=2)
+    ch = ol.Chain(d)
+    ch.set_beads(np.zeros((32, 1, 3)))
+    ch.rng_philox(3, 0)
+    ch.run_sweeps(50)
+    assert ch.stats().tries[4] == 50 * 2 * 4
+    blocks = ch.energy_blocks(400, 100)
+    err = blocks.std() / math.sqrt(len(blocks))
+    assert abs(blocks.mean() - 0.75 / math.tanh(0.25)) < max(4 * err, 0.08), (blocks.mean(), err)
