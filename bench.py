@@ -163,19 +163,24 @@ def all_paths(b):
     return arr
 
 
-def paired_energy(bm, br, em, er):
-    """Mixed vs FP64 on the same seeds and sweeps.  Both precisions move beads
-    in FP64, so a chain whose accept/reject decisions never differed ends on
-    bitwise the same path; its energy difference is the estimator's own.  The
-    paired difference over all chains (diverged chains included) gives the
-    energy error with its standard error."""
+def identical_chains(bm, br):
+    """Chains whose final paths are bitwise the same in both batches."""
     pm, pr = all_paths(bm), all_paths(br)
     same, off = [], 0
     for d in bm.descs:
         n = d.nbeads()
         same.append(bool(np.array_equal(pm[off:off + n], pr[off:off + n])))
         off += n
-    same = np.array(same)
+    return np.array(same)
+
+
+def paired_energy(bm, br, em, er):
+    """Mixed vs FP64 on the same seeds and sweeps.  Both precisions move beads
+    in FP64, so a chain whose accept/reject decisions never differed ends on
+    bitwise the same path; its energy difference is the estimator's own.  The
+    paired difference over all chains (diverged chains included) gives the
+    energy error with its standard error."""
+    same = identical_chains(bm, br)
     de = np.asarray(em) - np.asarray(er)
     n = len(de)
     return {"chains": n, "identical_decision_chains": int(same.sum()),
@@ -183,6 +188,27 @@ def paired_energy(bm, br, em, er):
             "mean_dE": float(de.mean()), "stderr_dE": float(de.std(ddof=1) / np.sqrt(n)) if n > 1 else None,
             "mean_E_fp64": float(np.mean(er)),
             "note": "per-chain thermodynamic energy of the final paths, same Philox streams and sweeps"}
+
+
+def paired_observables(bm, br):
+    """The other estimators of the mixed run against the FP64 run on the same
+    streams: virial (nwindow 8) and Coulomb energies per chain (largest
+    difference on the chains with identical decisions, mean difference with
+    its standard error over all chains) and the integer g(r) counts of the
+    two species (bit-identical on identical chains; total counts otherwise)."""
+    same = identical_chains(bm, br)
+    out = {}
+    for name, fm, fr in (("virial_energy", bm.virial_energy(8), br.virial_energy(8)),
+                         ("coulomb_energy", bm.coulomb_energy(1.0), br.coulomb_energy(1.0))):
+        de = fm - fr
+        out[name] = {"max_abs_d_identical": float(np.max(np.abs(de[same]))) if same.any() else None,
+                     "mean_d": float(de.mean()), "stderr_d": float(de.std(ddof=1) / np.sqrt(len(de))),
+                     "mean_fp64": float(fr.mean())}
+    hm, hr = bm.pair_histogram(0, 1, 0.0, 8.0, 256), br.pair_histogram(0, 1, 0.0, 8.0, 256)
+    out["pair_histogram_0_1"] = {"bins": 256, "rmax": 8.0,
+                                 "identical_chains_bit_exact": bool(np.array_equal(hm[same], hr[same])),
+                                 "total_counts_mixed": int(hm.sum()), "total_counts_fp64": int(hr.sum())}
+    return out
 
 
 def cpu_baseline(descs, budget_s=12.0):
@@ -281,6 +307,7 @@ def main():
         out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
                                "kernel_ms": float(np.mean(ref["ms"]))}
         out["energy_vs_fp64"] = paired_energy(b_main, b64, main_run["energy"], ref["energy"])
+        out["observables_vs_fp64"] = paired_observables(b_main, b64)
         single = [capi.cfg2_desc()]
         _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed, autotune=False)
         r1 = float(one["bu"].sum()) / one["wall"]

@@ -455,6 +455,13 @@ class Batch:
         check(self.lib.pq_run_measured(self.h, nblock, nsweep, dptr(e)))
         return e
 
+    def pair_histogram(self, s1, s2, rmin, rmax, nbin):
+        """Integer g(r) counts [nchain][nbin] of the current paths."""
+        h = np.zeros((len(self.descs), nbin), dtype=np.uint64)
+        check(self.lib.pq_pair_histogram(self.h, s1, s2, rmin, rmax, nbin,
+                                         h.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return h
+
     def ewald_coulomb_energy(self, epsilon, rcut, kcut):
         e = np.empty(len(self.descs))
         check(self.lib.pq_measure_ewald_coulomb_energy(self.h, epsilon, rcut, kcut, dptr(e)))
