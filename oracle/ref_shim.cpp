@@ -1,9 +1,12 @@
 // Test infrastructure only: a C entry point over the reference's own
-// Coulomb1DLinkAction, Coulomb3DLinkAction and PeriodicGaussian classes,
-// compiled from the pi-qmc sources into oracle/_ref to cross-check the oracle.
+// Coulomb1DLinkAction, Coulomb3DLinkAction and PeriodicGaussian classes and
+// its erf, compiled from the pi-qmc sources into oracle/_ref to cross-check
+// the oracle.
+#include <cmath>
 #include "action/coulomb/Coulomb1DLinkAction.h"
 #include "action/coulomb/Coulomb3DLinkAction.h"
 #include "util/PeriodicGaussian.h"
+#include "util/erf.h"
 
 extern "C" double ref_coulomb1d(double stau, int which, double reff) {
   Coulomb1DLinkAction c(stau);
@@ -39,3 +42,7 @@ extern "C" void ref_pg_eval(double alpha, double length, double x, double* out) 
 extern "C" int ref_pg_nterms(double alpha, double length) {
   return PeriodicGaussian::numberOfTerms(alpha, length);
 }
+
+// the reference's incomplete-gamma erf (src/util/erf.cpp), bound inside this
+// library
+extern "C" double ref_erf(double x) { return erf(x); }

@@ -86,7 +86,7 @@ def load():
 
 
 def load_ref():
-    """The reference's own Coulomb/PeriodicGaussian code built into oracle/_ref."""
+    """The reference's own Coulomb/PeriodicGaussian/erf code built into oracle/_ref."""
     if not os.path.exists(REF_PATH):
         return None
     lib = C.CDLL(REF_PATH)
@@ -98,6 +98,8 @@ def load_ref():
     lib.ref_pg_eval.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
     lib.ref_pg_nterms.restype = C.c_int
     lib.ref_pg_nterms.argtypes = [C.c_double, C.c_double]
+    lib.ref_erf.restype = C.c_double
+    lib.ref_erf.argtypes = [C.c_double]
     return lib
 
 

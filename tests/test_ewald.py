@@ -199,6 +199,13 @@ def test_reference_erf():
     xs = np.concatenate([np.linspace(-6, 6, 241), [1e-8, 0.5, 1.2247, 1.2248, 3.0]])
     got = np.array([lib.oq_ref_erf(float(x)) for x in xs])
     np.testing.assert_allclose(got, erf(xs), rtol=2e-15, atol=2e-16)
+    # the reference's own erf.cpp, compiled into oracle/_ref: bit for bit
+    ref = ol.load_ref()
+    if ref is None:
+        pytest.skip("reference sources not mounted: oracle/_ref not built")
+    rs = np.random.default_rng(4)
+    for x in np.concatenate([xs, rs.uniform(-8, 8, 2000), rs.uniform(-1.3, 1.3, 500)]):
+        assert lib.oq_ref_erf(float(x)) == ref.ref_erf(float(x)), x
 
 
 def test_traditional_ewald_matches_erfc_ewald():
