@@ -22,6 +22,11 @@
  *  - The rescale loop of evaluate(...,scaleMagnitude=true) gives up after 200
  *    passes and reports an error instead of spinning (SHONodes.cc:55-126).
  *  - The CompositeAction sums spring, then SHO actions, then Coulomb.
+ *  - OptEwaldSum's dgetrf/dgetrs/dgesv solves of its 8 x 8 and 2 x 2 fit
+ *    systems use the same textbook LU (on the transposed matrices LAPACK
+ *    reads); the fit meets its two constraints to a few parts in 1e9 and
+ *    reproduces the reference's Madelung test, but LAPACK's own operation
+ *    order, and so the last digits of the coefficients, is not pinned.
  */
 #include "pq_oracle.h"
 #include <math.h>
