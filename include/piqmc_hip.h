@@ -119,6 +119,9 @@ typedef struct pq_node_desc {
   double omega;        /* SHONodes only */
   double temperature;  /* 0 means the simulation temperature */
   int nodal_action;    /* 1 unless noNodalAction="true" */
+  int many_body_distance;  /* useManyBodyDistance="true": one distance per slice and
+                              section, 1/sqrt(sum_i 1/d_i^2) over all particles
+                              (FixedNodeAction.cc:99-116) */
 } pq_node_desc;
 
 /* <Sample npart species noPermutation nrepeat> inside <ChooseSection>. */

@@ -1576,8 +1576,27 @@ double oq_fn_action_difference(oq_chain* ch, const double* mov, const int* idx,
           for (int d = 0; d < D; ++d) r1[idx[m] * D + d] = mov[((size_t)t * nmov + m) * D + d];
         node_distance(ch, f, r1, &ch->sec2[t * slice], t, &f->newdist[(t * 2 + 0) * ns],
                       &f->newdist[(t * 2 + 1) * ns]);
+        const double* od = f->dist; const double* nd = f->newdist;
+        if (ch->d.nodes[a].many_body_distance) {
+          /* useManyBodyDistance (FixedNodeAction.cc:99-116): the sums run
+             over all N particles, the other species' at distance 200 */
+          double s[2][4] = {{0}};
+          for (int i = 0; i < N; ++i)
+            for (int p = 0; p < 2; ++p) {
+              const int own = i >= f->ifirst && i < f->ifirst + ns;
+              const int k = i - f->ifirst;
+              const double o1 = own ? od[(t * 2 + p) * ns + k] : 200;
+              const double o0 = own ? od[((t - 1) * 2 + p) * ns + k] : 200;
+              const double n1 = own ? nd[(t * 2 + p) * ns + k] : 200;
+              const double n0 = own ? nd[((t - 1) * 2 + p) * ns + k] : 200;
+              s[p][0] += 1. / (o1 * o1); s[p][1] += 1. / (o0 * o0);
+              s[p][2] += 1. / (n1 * n1); s[p][3] += 1. / (n0 * n0);
+            }
+          for (int p = 0; p < 2; ++p)
+            dA += log((1 - exp(-1. / sqrt(s[p][0] * s[p][1]))) / (1 - exp(-1. / sqrt(s[p][2] * s[p][3]))));
+          continue;
+        }
         for (int i = 0; i < ns; ++i) {
-          const double* od = f->dist; const double* nd = f->newdist;
           dA += log((1 - exp(-od[(t * 2 + 0) * ns + i] * od[((t - 1) * 2 + 0) * ns + i]))
                     / (1 - exp(-nd[(t * 2 + 0) * ns + i] * nd[((t - 1) * 2 + 0) * ns + i])));
           dA += log((1 - exp(-od[(t * 2 + 1) * ns + i] * od[((t - 1) * 2 + 1) * ns + i]))

@@ -459,6 +459,7 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     f.ns = c.count[nd.species];
     if (f.ns > PQ_MAX_NODE_PART) return fail(PQ_ERR_ARG, "node species of %d particles exceeds %d", f.ns, PQ_MAX_NODE_PART);
     f.nodal = nd.nodal_action;
+    f.manybody = nd.many_body_distance ? 1 : 0;
     f.mass = c.mass[nd.species];
     const double Tn = nd.temperature == 0 ? T : nd.temperature;
     // SHONodes constructor (src/fixednode/SHONodes.cc:22-35)

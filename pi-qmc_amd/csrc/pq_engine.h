@@ -1321,17 +1321,7 @@ struct Eng {
         } else {
           node_distances(fa, 1, c.S - 1, true, nullptr, scale[fa], nd);
         }
-        double part = 0;
-        for (int it = tid; it < (c.S - 1) * n; it += NT) {
-          const int t = 1 + it / n, k = it % n;
-#pragma unroll
-          for (int pass = 0; pass < 2; ++pass) {
-            const int64_t a1 = ((int64_t)t * 2 + pass) * n + k;
-            const int64_t a0 = ((int64_t)(t - 1) * 2 + pass) * n + k;
-            part += log((1 - exp(-od[a1] * od[a0])) / (1 - exp(-nd[a1] * nd[a0])));
-          }
-        }
-        dA = bsum(part);
+        dA = nodal_action(f, od, nd);
       }
       total += dA;
     }
@@ -1365,6 +1355,48 @@ struct Eng {
   // FixedNodeAction::getActionDifference (FixedNodeAction.cc:39-131) at one
   // level from the determinants fn_eval_moved left in newdm; level 0 adds the
   // nodal action from the distances of every slice.
+  // the nodal action of FixedNodeAction::getActionDifference at level 0
+  // (FixedNodeAction.cc:75-126) over links t-1..t, t = 1..S-1, of both
+  // sections, old distances od against new nd ([S][2][ns]): per particle
+  // log[(1 - e^{-d(t) d(t-1)}) / (1 - e^{-d'(t) d'(t-1)})], or with
+  // useManyBodyDistance one term per link and section with the distance
+  // 1/sqrt(sum_i 1/d_i^2), where the other species' particles enter with the
+  // node models' 200 (SHONodes.cc:132, FreeParticleNodes.cc:187)
+  __device__ double nodal_action(const FnP& f, const double* od, const double* nd) {
+    const int n = f.ns;
+    double part = 0;
+    if (f.manybody) {
+      const double others = (double)(c.N - n) / (200.0 * 200.0);
+      for (int it = tid; it < (c.S - 1) * 2; it += NT) {
+        const int t = 1 + (it >> 1), pass = it & 1;
+        const double* o1 = od + ((int64_t)t * 2 + pass) * n;
+        const double* o0 = od + ((int64_t)(t - 1) * 2 + pass) * n;
+        const double* n1 = nd + ((int64_t)t * 2 + pass) * n;
+        const double* n0 = nd + ((int64_t)(t - 1) * 2 + pass) * n;
+        double so1 = 0, so0 = 0, sn1 = 0, sn0 = 0;
+        for (int k = 0; k < n; ++k) {
+          so1 += 1. / (o1[k] * o1[k]);
+          so0 += 1. / (o0[k] * o0[k]);
+          sn1 += 1. / (n1[k] * n1[k]);
+          sn0 += 1. / (n0[k] * n0[k]);
+        }
+        so1 += others; so0 += others; sn1 += others; sn0 += others;
+        part += log((1 - exp(-1. / sqrt(so1 * so0))) / (1 - exp(-1. / sqrt(sn1 * sn0))));
+      }
+    } else {
+      for (int it = tid; it < (c.S - 1) * n; it += NT) {
+        const int t = 1 + it / n, k = it % n;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+          const int64_t a1 = ((int64_t)t * 2 + pass) * n + k;
+          const int64_t a0 = ((int64_t)(t - 1) * 2 + pass) * n + k;
+          part += log((1 - exp(-od[a1] * od[a0])) / (1 - exp(-nd[a1] * nd[a0])));
+        }
+      }
+    }
+    return bsum(part);
+  }
+
   __device__ double fn_diff_moved(int level) {
     double total = 0;
     for (int fa = 0; fa < c.nfn; ++fa) {
@@ -1387,17 +1419,7 @@ struct Eng {
         } else {
           node_distances(fa, 1, c.S - 1, true, nullptr, scale[fa], nd);
         }
-        double part = 0;
-        for (int it = tid; it < (c.S - 1) * n; it += NT) {
-          const int t = 1 + it / n, k = it % n;
-#pragma unroll
-          for (int pass = 0; pass < 2; ++pass) {
-            const int64_t a1 = ((int64_t)t * 2 + pass) * n + k;
-            const int64_t a0 = ((int64_t)(t - 1) * 2 + pass) * n + k;
-            part += log((1 - exp(-od[a1] * od[a0])) / (1 - exp(-nd[a1] * nd[a0])));
-          }
-        }
-        dA = bsum(part);
+        dA = nodal_action(f, od, nd);
       }
       total += dA;
     }

@@ -25,6 +25,7 @@ struct PairP {
 
 struct FnP {             // one FixedNodeAction with its node model
   int sp, model, ifirst, ns, nodal;
+  int manybody;          // useManyBodyDistance
   double mass;
   double coshwt, c;      // SHONodes
   double pg_nome[3], pg_pref[3], pg_k[3];

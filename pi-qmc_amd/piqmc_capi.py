@@ -54,7 +54,8 @@ class CoulombDesc(C.Structure):
 
 class NodeDesc(C.Structure):
     _fields_ = [("species", C.c_int), ("model", C.c_int), ("omega", C.c_double),
-                ("temperature", C.c_double), ("nodal_action", C.c_int)]
+                ("temperature", C.c_double), ("nodal_action", C.c_int),
+                ("many_body_distance", C.c_int)]
 
 
 class SampleDesc(C.Structure):

@@ -153,10 +153,15 @@ def test_large_species_nodes(ns, model, L):
     assert np.max(np.abs(bs.get_paths(0) - os_.get_beads())) < 1e-9
 
 
-def test_fixed_node_protocol():
-    """initialize, per-level action difference (with crossings) and accept on one chain."""
-    print("[test] fixed_node_protocol", flush=True)
+@pytest.mark.parametrize("many_body", [0, 1])
+def test_fixed_node_protocol(many_body):
+    """initialize, per-level action difference (with crossings) and accept on
+    one chain; the nodal action per particle or, with useManyBodyDistance,
+    per slice from the many-body distance."""
+    print(f"[test] fixed_node_protocol many_body={many_body}", flush=True)
     desc = small_cfg2(64)
+    for a in range(desc.nnodes):
+        desc.nodes[a].many_body_distance = many_body
     b = capi.Batch([desc])
     oc = ol.Chain(desc)
     lib = capi.load_library()
@@ -596,3 +601,16 @@ def test_virial_energy_estimator():
             want = oc.virial_energy(nw)
             assert rel_close(got, want, 1e-11, scale=1.0), (nw, got, want)
         assert rel_close(b.virial_energy(1)[0], b.energy()[0], 1e-11, scale=1.0)
+
+
+def test_many_body_distance_sampler_decisions():
+    """useManyBodyDistance in the device sampler: the oracle's decisions and
+    paths for the 3+3 fermion dot."""
+    print("[test] many-body distance sampler", flush=True)
+    desc = small_cfg2(64)
+    for a in range(desc.nnodes):
+        desc.nodes[a].many_body_distance = 1
+    b, oc = _paired(desc, 6)
+    sg, so = b.stats(0), oc.stats()
+    assert sg.draws == so.draws and list(sg.accepts) == list(so.accepts)
+    assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9

@@ -469,3 +469,26 @@ def test_sample_nrepeat_tries_and_balance():
     blocks = ch.energy_blocks(400, 100)
     err = blocks.std() / math.sqrt(len(blocks))
     assert abs(blocks.mean() - 0.75 / math.tanh(0.25)) < max(4 * err, 0.08), (blocks.mean(), err)
+
+
+def test_many_body_nodal_distance_reduces_to_one_body():
+    """useManyBodyDistance with one particle in all, of the node species: the
+    distance 1/sqrt(sum 1/d^2) is d itself, so the nodal action equals the
+    one-body form; with a second species it differs."""
+    def run(nodes_many, species):
+        d = capi.make_desc(2, 32, 2, 0.1, species, sho=[(0, 1.0)],
+                           nodes=[(0, capi.PQ_NODES_SHO, 1.0)], nsection=8)
+        d.nodes[0].many_body_distance = nodes_many
+        ch = ol.Chain(d)
+        rs = np.random.default_rng(9)
+        S = 5
+        sec1 = rs.normal(size=(S, d.npart, 2))
+        sec2 = rs.normal(size=(S, d.npart, 2))
+        mov = sec1[:, [0], :] + 0.05 * rs.normal(size=(S, 1, 2))
+        mov[0], mov[-1] = sec1[0, [0]], sec1[-1, [0]]
+        ch.fn_initialize(sec1, sec2)
+        return ch.fn_action_difference(mov, [0], 0)
+    one = [(1, 1.0, 0.0)]
+    assert run(1, one) == pytest.approx(run(0, one), rel=1e-13, abs=1e-15)
+    two = [(1, 1.0, 0.0), (1, 1.0, 0.0)]
+    assert run(1, two) != pytest.approx(run(0, two), rel=1e-6)
