@@ -266,7 +266,12 @@ int pq_get_staging(const pq_batch* batch, int* tables_lds, int* state_lds);
    order of estimated cost) run in a launch of their own with heavy_team
    threads per chain (128, 256 or 512), concurrently with the rest; 0 turns
    the split off.  The sweep's length is the heaviest chain's, so giving it
-   more waves shortens every launch of an uneven batch. */
+   more waves shortens every launch of an uneven batch.  heavy_chains -1
+   splits at the node-matrix width instead, when chains with 17-32-particle
+   node species sit beside narrower ones (the batch then launches the wide
+   ones first): those run with heavy_team threads on the 32-wide instance,
+   the rest on the narrower Gauss-Jordan instance with an LDS layout sized to
+   them; pq_get_split reports -1 for it. */
 int pq_set_split(pq_batch* batch, int heavy_chains, int heavy_team);
 int pq_get_split(const pq_batch* batch, int* heavy_chains, int* heavy_team);
 /* Times every distinct schedule (team 64/128/256/512 x tables, node state and

@@ -166,6 +166,9 @@ struct Sched {
   int team = -1, inv = -1, tab = -1, fst = -1;
   int heavy = 0;          // the heaviest chains launched apart with a larger team
   int heavy_team = 256;
+  int ncsplit = 0;        // split at the node-width boundary instead: the chains with
+                          // 17-32-particle node species apart, the rest on a narrower
+                          // Gauss-Jordan instance with a layout of their own
 };
 
 struct pq_batch {
@@ -174,6 +177,8 @@ struct pq_batch {
   int nchain = 0;
   int D = 3;
   int NC = 4;     // node-matrix width class of the batch: 4, 8, 10, 12, 16 or 32
+  int NC_bulk = 0; // with 32-wide chains among narrower ones: the narrower chains' class
+  int wide = 0;    // ... and the number of 32-wide chains, first in the launch order
   bool multi = false;   // some Sample moves several particles (cycles, relabelling)
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // the heavy chains' launch, forked from and joined to stream
@@ -539,12 +544,16 @@ static int64_t align8(int64_t v) { return (v + 7) & ~(int64_t)7; }
 // inv = node inverse cache in LDS.  Automatic placement fills a 40 KB budget
 // (four chains per CU) in the order tables, fixed-node state, inverses;
 // requested placements may use the whole 160 KB.
-static SweepLayoutH layout_for(const pq_batch* b, const Sched& req) {
+static SweepLayoutH layout_for(const pq_batch* b, const Sched& req, int lo = 0, int hi = -1, int NC = 0) {
   SweepLayoutH lay{};
   int64_t mS = 0, mSND = 0, mSD = 0, mF = 0, mInv = 0, mTab = 0, mN = 0, mWalk = 0;
   bool nodes = false;
   const int64_t w = b->precision == PQ_PREC_MIXED ? (int64_t)sizeof(float) : (int64_t)sizeof(double);
-  for (auto& ch : b->hc) {
+  if (hi < 0) hi = b->nchain;
+  if (NC == 0) NC = b->NC;
+  // the chains of this launch: order[lo..hi)
+  for (int q = lo; q < hi; ++q) {
+    const ChainP& ch = b->hc[b->order.empty() ? q : b->order[q]];
     mS = std::max<int64_t>(mS, ch.Smax);
     mSND = std::max<int64_t>(mSND, (int64_t)ch.Smax * ch.N * ch.D);
     int nm = 1;
@@ -593,7 +602,7 @@ static SweepLayoutH layout_for(const pq_batch* b, const Sched& req) {
   // Gauss-Jordan scratch: pivot rows (2 doubles per thread), or for 32-wide
   // node matrices the MFMA scratch of every wave of the team
   lay.off_gj = (int)o;
-  o += b->NC == 32 ? (int64_t)(team / PQ_WAVE) * PQ_GJ32_SCRATCH_DOUBLES * 8 : (int64_t)2 * 8 * team;
+  o += NC == 32 ? (int64_t)(team / PQ_WAVE) * PQ_GJ32_SCRATCH_DOUBLES * 8 : (int64_t)2 * 8 * team;
   const int64_t budget = 40 * 1024, whole = 160 * 1024;
   auto place = [&](int r, int64_t bytes, bool wanted) -> int {
     if (!wanted || bytes <= 0 || r == 0) return -1;
@@ -610,6 +619,13 @@ static SweepLayoutH layout_for(const pq_batch* b, const Sched& req) {
 }
 
 static void make_layout(pq_batch* b, const Sched& req) {
+  if (req.ncsplit && b->wide > 0 && b->wide < b->nchain) {
+    Sched h = req;
+    h.team = req.heavy_team;
+    b->lay_heavy = layout_for(b, h, 0, b->wide, b->NC);
+    b->lay = layout_for(b, req, b->wide, b->nchain, b->NC_bulk);
+    return;
+  }
   b->lay = layout_for(b, req);
   b->lay_heavy = b->lay;
   if (req.heavy > 0) {
@@ -626,14 +642,16 @@ static int layout_bytes(const pq_batch* b) { return std::max(b->lay.bytes, b->la
 // batch stream; the batch stream waits for both
 static hipError_t run_launch(pq_batch* b, const DevCtx& x, int nsweep) {
   const bool mixed = b->precision == PQ_PREC_MIXED;
-  const int h = b->sched.heavy;
+  const bool ncs = b->sched.ncsplit && b->wide > 0 && b->wide < b->nchain;
+  const int h = ncs ? b->wide : b->sched.heavy;
+  const int nc_rest = ncs ? b->NC_bulk : b->NC;
   if (h <= 0 || h >= b->nchain) return launch_sweep(b->D, b->NC, mixed, b->multi, x, b->nchain, nsweep, b->lay, b->stream);
   hipError_t e = hipEventRecord(b->evf, b->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(b->stream2, b->evf, 0);
   if (e == hipSuccess) e = launch_sweep(b->D, b->NC, mixed, b->multi, x, h, nsweep, b->lay_heavy, b->stream2);
   DevCtx xl = x;
   xl.order = x.order + h;
-  if (e == hipSuccess) e = launch_sweep(b->D, b->NC, mixed, b->multi, xl, b->nchain - h, nsweep, b->lay, b->stream);
+  if (e == hipSuccess) e = launch_sweep(b->D, nc_rest, mixed, b->multi, xl, b->nchain - h, nsweep, b->lay, b->stream);
   if (e == hipSuccess) e = hipEventRecord(b->evj, b->stream2);
   if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->evj, 0);
   return e;
@@ -705,6 +723,23 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
                  (ch.N + ns2 / std::max(1, ch.grp[g].nsample));
   }
   std::stable_sort(b->order.begin(), b->order.end(), [&](int p, int q) { return cost[p] > cost[q]; });
+  // with 32-wide node matrices beside narrower ones, the 32-wide chains go
+  // first (each class in cost order): a node-width split launches them apart
+  // and the rest with the narrower Gauss-Jordan instance
+  if (b->NC == 32) {
+    int ns_rest = 0;
+    for (int c = 0; c < nchain; ++c) {
+      if (b->hc[c].ns_max > 16) ++b->wide;
+      else ns_rest = std::max(ns_rest, b->hc[c].ns_max);
+    }
+    if (b->wide > 0 && b->wide < nchain) {
+      std::stable_sort(b->order.begin(), b->order.end(),
+                       [&](int p, int q) { return (b->hc[p].ns_max > 16) > (b->hc[q].ns_max > 16); });
+      b->NC_bulk = ns_rest <= 4 ? 4 : ns_rest <= 8 ? 8 : ns_rest <= 10 ? 10 : ns_rest <= 12 ? 12 : 16;
+    } else {
+      b->wide = 0;
+    }
+  }
 
   auto cleanup_fail = [&](int rc) { pq_batch_destroy(b); return rc; };
   if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->stream2) != hipSuccess)
@@ -1123,19 +1158,22 @@ extern "C" int pq_get_staging(const pq_batch* b, int* tables_lds, int* state_lds
 
 extern "C" int pq_set_split(pq_batch* b, int heavy_chains, int heavy_team) {
   if (!b) return fail(PQ_ERR_ARG, "null batch");
-  if (heavy_chains < 0 || heavy_chains >= b->nchain) return fail(PQ_ERR_ARG, "heavy_chains must be in [0, nchain)");
+  if (heavy_chains < -1 || heavy_chains >= b->nchain) return fail(PQ_ERR_ARG, "heavy_chains must be in [-1, nchain)");
+  if (heavy_chains == -1 && b->wide == 0)
+    return fail(PQ_ERR_ARG, "the node-width split needs 32-wide node species beside narrower ones");
   if (heavy_team != 128 && heavy_team != 256 && heavy_team != 512) return fail(PQ_ERR_ARG, "heavy_team must be 128, 256 or 512");
-  if (b->multi && heavy_chains > 0 && heavy_team > 128)
+  if (b->multi && heavy_chains != 0 && heavy_team > 128)
     return fail(PQ_ERR_ARG, "batches with several moving particles per Sample run teams of at most 128");
   Sched s = b->sched;
-  s.heavy = heavy_chains;
+  s.ncsplit = heavy_chains == -1;
+  s.heavy = s.ncsplit ? b->wide : heavy_chains;
   s.heavy_team = heavy_team;
   return apply_schedule(b, s);
 }
 
 extern "C" int pq_get_split(const pq_batch* b, int* heavy_chains, int* heavy_team) {
   if (!b) return fail(PQ_ERR_ARG, "null batch");
-  if (heavy_chains) *heavy_chains = b->sched.heavy;
+  if (heavy_chains) *heavy_chains = b->sched.ncsplit ? -1 : b->sched.heavy;
   if (heavy_team) *heavy_team = b->sched.heavy_team;
   return PQ_OK;
 }
@@ -1175,10 +1213,11 @@ extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed)
     b->sched = s;
     make_layout(b, s);
     if (layout_bytes(b) > 160 * 1024) return PQ_OK;
-    std::vector<char> key(2 * sizeof(SweepLayoutH) + sizeof(int));
+    std::vector<char> key(2 * sizeof(SweepLayoutH) + 2 * sizeof(int));
     std::memcpy(key.data(), &b->lay, sizeof(SweepLayoutH));
     std::memcpy(key.data() + sizeof(SweepLayoutH), &b->lay_heavy, sizeof(SweepLayoutH));
     std::memcpy(key.data() + 2 * sizeof(SweepLayoutH), &s.heavy, sizeof(int));
+    std::memcpy(key.data() + 2 * sizeof(SweepLayoutH) + sizeof(int), &s.ncsplit, sizeof(int));
     for (const auto& k : seen)
       if (k == key) return PQ_OK;
     seen.push_back(key);
@@ -1207,11 +1246,25 @@ extern "C" int pq_autotune(pq_batch* b, int nsweep, float* best_ms, int* ntimed)
     for (int frac : {16, 8, 4})
       for (int ht : {256, 512}) {
         Sched s = bases[q].second;
+        s.ncsplit = 0;
         s.heavy = b->nchain / frac;
         s.heavy_team = ht;
         if (s.heavy < 1 || s.heavy >= b->nchain) continue;
         if (int rc = trial(s)) return rc;
       }
+  // and the node-width split: 32-wide chains apart, the rest on the narrower
+  // instance with a layout sized to them, at every team size of the rest
+  if (b->wide > 0 && !b->multi)
+    for (int team : {64, 128, 256})
+      for (int ht : {256, 512})
+        for (size_t q = 0; q < bases.size() && q < 2; ++q) {
+          Sched s = bases[q].second;
+          s.team = team;
+          s.ncsplit = 1;
+          s.heavy = b->wide;
+          s.heavy_team = ht;
+          if (int rc = trial(s)) return rc;
+        }
   // a chain's sweep time drifts with its state by about as much as the
   // schedules differ, so the two fastest single launches and the three
   // fastest splits are timed again, three rounds in turn, and the lowest
@@ -1434,7 +1487,7 @@ extern "C" int pq_run_measured(pq_batch* b, int nblock, int nsweep, double* ener
       (uint64_t)(uintptr_t)x.beads, (uint64_t)(uintptr_t)x.inj, (uint64_t)x.inj_per_chain,
       (uint64_t)x.rng_mode, x.seed, (uint64_t)x.first_stream, (uint64_t)(uintptr_t)x.inv,
       (uint64_t)(uintptr_t)x.fncache, (uint64_t)(uintptr_t)x.fnsweep, (uint64_t)nblock, (uint64_t)nsweep,
-      (uint64_t)b->sched.heavy};
+      (uint64_t)b->sched.heavy, (uint64_t)b->sched.ncsplit};
   std::vector<char> key(sizeof(fields) + 2 * sizeof(SweepLayoutH));
   std::memcpy(key.data(), fields, sizeof(fields));
   std::memcpy(key.data() + sizeof(fields), &b->lay, sizeof(SweepLayoutH));

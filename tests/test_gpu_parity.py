@@ -3,6 +3,7 @@ on the same seeded inputs.  FP64 action and node values agree to 1e-12
 relative (relative to the larger of the value and the summed term scale);
 Metropolis decisions are identical given the same uniforms; integer
 histograms are bit-exact."""
+import ctypes as C
 import os
 
 import numpy as np
@@ -614,3 +615,37 @@ def test_many_body_distance_sampler_decisions():
     sg, so = b.stats(0), oc.stats()
     assert sg.draws == so.draws and list(sg.accepts) == list(so.accepts)
     assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9
+
+
+@pytest.mark.parametrize("team,heavy_team", [(128, 256), (64, 512)])
+def test_node_width_split_decisions(team, heavy_team):
+    """pq_set_split(-1): chains with 17-32-particle node species on the
+    32-wide instance, the narrower ones on their own instance and layout;
+    every chain makes the oracle's decisions."""
+    print(f"[test] node-width split team={team} heavy_team={heavy_team}", flush=True)
+    def chain(nup, ndn, M=64):
+        return capi.make_desc(2, M, 3, 0.05, [(nup, 1.0, -1.0), (ndn, 1.0, -1.0)], sho=[(0, 1.0), (1, 1.0)],
+                              nodes=[(0, capi.PQ_NODES_SHO, 1.0), (1, capi.PQ_NODES_SHO, 1.0)], nsection=8)
+    descs = [chain(3, 2), chain(18, 3), chain(2, 2), chain(4, 20), chain(5, 5)]
+    b = capi.Batch(descs)
+    b.set_schedule(team, -1)
+    capi.check(b.lib.pq_set_split(b.h, -1, heavy_team))
+    h, ht = C.c_int(), C.c_int()
+    capi.check(b.lib.pq_get_split(b.h, C.byref(h), C.byref(ht)))
+    assert h.value == -1 and ht.value == heavy_team
+    ocs = []
+    for c, d in enumerate(descs):
+        beads = capi.initial_beads(d, 40 + c, replicate=True)
+        b.set_paths(c, beads)
+        oc = ol.Chain(d)
+        oc.set_beads(beads)
+        oc.rng_philox(77, c)
+        ocs.append(oc)
+    b.seed(77, 0)
+    b.run_sweeps(4)
+    b.sync()
+    for c, oc in enumerate(ocs):
+        oc.run_sweeps(4)
+        sg, so = b.stats(c), oc.stats()
+        assert sg.draws == so.draws and list(sg.accepts) == list(so.accepts), c
+        assert np.max(np.abs(b.get_paths(c) - oc.get_beads())) < 1e-9, c
