@@ -33,3 +33,34 @@ def test_struct_sizes_match_header_layout():
     assert capi.ChainDesc.nsection.offset > capi.ChainDesc.sample.offset
     lib = capi.load_library()
     assert lib.pq_last_error() is not None
+
+
+def test_ctypes_mirror_matches_the_c_layout(tmp_path):
+    """sizeof and every field offset of the descriptor structs, as the C
+    compiler lays out include/piqmc_hip.h, against the ctypes mirror."""
+    import shutil
+    import subprocess
+    import pytest
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {"pq_species_desc": capi.SpeciesDesc, "pq_sho_desc": capi.ShoDesc,
+               "pq_coulomb_desc": capi.CoulombDesc, "pq_node_desc": capi.NodeDesc,
+               "pq_sample_desc": capi.SampleDesc, "pq_section_group": capi.SectionGroup,
+               "pq_chain_desc": capi.ChainDesc, "pq_chain_stats": capi.ChainStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "piqmc_hip.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        cname, field, value = line.split()
+        py = structs[cname]
+        got = C.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert got == int(value), (cname, field, got, value)
