@@ -124,13 +124,21 @@ typedef struct pq_node_desc {
                               (FixedNodeAction.cc:99-116) */
 } pq_node_desc;
 
-/* <Sample npart species noPermutation nrepeat> inside <ChooseSection>. */
+/* <Sample npart species noPermutation nrepeat delayedRejection
+   defaultSigmaFactor newSigmaFactor> inside <ChooseSection>
+   (PIMCParser.cc:312-322). */
 typedef struct pq_sample_desc {
   int species;         /* ignored by the PQ_CHOOSER_ALL* choosers */
   int nmoving;         /* npart: 1..PQ_MAX_MOVING, at most the particles it chooses from */
   int chooser;         /* PQ_CHOOSER_* */
   int nrepeat;         /* <Sample nrepeat>: tries on the same section, 0 means 1
                           (MultiLevelSampler.cc:68-98, DoubleMLSampler.cc:66-120) */
+  int delayed_rejection;   /* a second proposal at level nlevel-1 after a rejection
+                              (MultiLevelSampler.cc:114-146, DoubleMLSampler.cc:150-209);
+                              needs PQ_MOVER_FREE (FreeMoverPBC leaves its forward
+                              probability unset, FreeMoverPBC.h:18) */
+  double sigma_factor;     /* defaultSigmaFactor: the movers' sigma scale, 0 means 1 */
+  double new_sigma_factor; /* newSigmaFactor of the second proposal, 0 means 0.75 */
 } pq_sample_desc;
 
 /* A further <ChooseSection nlevel> of the sweep with its <Sample> elements,

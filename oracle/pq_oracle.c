@@ -827,6 +827,10 @@ struct oq_chain {
   /* the section group being run (set_group) */
   int gnsample, gnsection;
   const pq_sample_desc* gsample;
+  /* MultiLevelSampler::factor, the movers' sigma scale, and the mover's
+     forwardProb of its last call (FreeMover.h:33-35, 45) */
+  double fac, fwd;
+  double* rej;                     /* delayed rejection: the rejected moving beads */
   oq_rng rng;
   pq_chain_stats st;
 };
@@ -876,6 +880,7 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
   }
   const int smax = (1 << lmax) + 1;
   oq_chain* ch = (oq_chain*)calloc(1, sizeof(oq_chain));
+  if (ch) ch->fac = 1.0;
   ch->d = *desc;
   ch->D = desc->ndim;
   ch->M = desc->nslice;
@@ -1002,6 +1007,7 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
   ch->sec1 = (double*)calloc(secsz, sizeof(double));
   ch->sec2 = (double*)calloc(secsz, sizeof(double));
   ch->mov = (double*)calloc((size_t)smax * PQ_MAX_MOVING * ch->D, sizeof(double));
+  ch->rej = (double*)calloc((size_t)smax * PQ_MAX_MOVING * ch->D, sizeof(double));
   for (int i = 0; i < ch->N; ++i) ch->perm[i] = ch->invperm[i] = i;
   ch->wt = (double*)calloc((size_t)maxcount * maxcount, sizeof(double));
   ch->wcump = (double*)calloc((size_t)maxcount * maxcount, sizeof(double));
@@ -1019,7 +1025,7 @@ void oq_destroy(oq_chain* ch) {
     free(ch->fn[a].newdist); free(ch->fn[a].inv);
   }
   free(ch->spg); free(ch->mpg); free(ch->spg_on); free(ch->mpg_on);
-  free(ch->beads); free(ch->sec1); free(ch->sec2); free(ch->mov);
+  free(ch->beads); free(ch->sec1); free(ch->sec2); free(ch->mov); free(ch->rej);
   free(ch->wt); free(ch->wcump); free(ch->wrow);
   if (ch->has_ewald) ewald_free(&ch->ew);
   free(ch->ewr);
@@ -1627,11 +1633,12 @@ static double free_mover(oq_chain* ch, const double* sec, double* mov, const int
   const int s = 1 << level, D = ch->D;
   double g[PQ_MAX_MOVING * 3];
   double lnT = 0;
+  ch->fwd = 0;
   for (int t = s; t < ch->S - s; t += 2 * s) {
     gauss_rand(&ch->rng, g, nmov * D);
     for (int m = 0; m < nmov; ++m) {
       int i = idx[m], sp = ch->spec[i];
-      double sigma = 1.0 * sqrt(ch->lambda[i] * ch->tau * s);
+      double sigma = ch->fac * sqrt(ch->lambda[i] * ch->tau * s);
       double inv2Sigma2 = 0.5 / (sigma * sigma);
       double mid[3], delta[3];
       for (int d = 0; d < D; ++d) mid[d] = MOV(mov, t + s, m, d) - MOV(mov, t - s, m, d);
@@ -1648,9 +1655,14 @@ static double free_mover(oq_chain* ch, const double* sec, double* mov, const int
       for (int d = 0; d < D; ++d) {
         int k = pg_index(ch, level, sp, d);
         if (ch->mpg_on[k]) temp *= pg_eval(&ch->mpg[k], delta[d]);
-        else lnT += delta[d] * delta[d] * inv2Sigma2;
+        else {
+          double tmp = delta[d] * delta[d] * inv2Sigma2;
+          ch->fwd += tmp;
+          lnT += tmp;
+        }
       }
       temp = 1.0 / temp;
+      ch->fwd += log(temp);
       for (int d = 0; d < D; ++d) mid[d] = SEC(sec, t + s, i, d) - SEC(sec, t - s, i, d);
       pbc(&ch->cell, mid);
       for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += SEC(sec, t - s, i, d); }
@@ -1678,7 +1690,7 @@ static double free_mover_pbc(oq_chain* ch, const double* sec, double* mov, const
     gauss_rand(&ch->rng, g, nmov * D);
     for (int m = 0; m < nmov; ++m) {
       int i = idx[m], sp = ch->spec[i];
-      double sigma = 1.0 * sqrt(ch->lambda[i] * ch->tau * s);
+      double sigma = ch->fac * sqrt(ch->lambda[i] * ch->tau * s);
       double inv2Sigma2 = 0.5 / (sigma * sigma);
       double mid[3], mid2[3];
       for (int d = 0; d < D; ++d) mid[d] = MOV(mov, t + s, m, d) - MOV(mov, t - s, m, d);
@@ -1740,6 +1752,58 @@ double oq_mover(oq_chain* ch, const double* sec, double* mov, const int* idx,
                 int nmov, int level) {
   if (ch->d.mover == PQ_MOVER_FREE) return free_mover(ch, sec, mov, idx, nmov, level);
   return free_mover_pbc(ch, sec, mov, idx, nmov, level);
+}
+
+/* FreeMover::makeDelayedMove (src/advancer/mover/FreeMover.cc:126-184): for
+   every slice of the level, ln(T_old/T_new) of proposing the rejected beads
+   rej ([S][nmov][D]) from the neighbours in mov, against mov's beads from
+   rej's neighbours; forwardProb takes minus the rejected beads' Gaussian
+   exponents plus the log of their periodic-Gaussian weights.  No draws. */
+static double free_delayed_mover(oq_chain* ch, const double* rej, const double* mov,
+                                 const int* idx, int nmov, int level) {
+  const int s = 1 << level, D = ch->D;
+  double lnT = 0;
+  ch->fwd = 0;
+  for (int t = s; t < ch->S - s; t += 2 * s) {
+    for (int m = 0; m < nmov; ++m) {
+      int i = idx[m], sp = ch->spec[i];
+      double sigma = ch->fac * sqrt(ch->lambda[i] * ch->tau * s);
+      double inv2Sigma2 = 0.5 / (sigma * sigma);
+      double temp = 1.0;
+      double mid[3], delta[3];
+      for (int d = 0; d < D; ++d) mid[d] = MOV(mov, t + s, m, d) - MOV(mov, t - s, m, d);
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += MOV(mov, t - s, m, d); }
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) delta[d] = MOV(rej, t, m, d) - mid[d];
+      pbc(&ch->cell, delta);
+      for (int d = 0; d < D; ++d) {
+        int k = pg_index(ch, level, sp, d);
+        if (ch->mpg_on[k]) temp *= pg_eval(&ch->mpg[k], delta[d]);
+        else {
+          double tmp = delta[d] * delta[d] * inv2Sigma2;
+          ch->fwd -= tmp;
+          lnT += tmp;
+        }
+      }
+      ch->fwd += log(temp);
+      temp = 1.0 / temp;
+      /* reverse: mov's bead about the rejected beads' midpoint */
+      for (int d = 0; d < D; ++d) mid[d] = MOV(rej, t + s, m, d) - MOV(rej, t - s, m, d);
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += MOV(rej, t - s, m, d); }
+      pbc(&ch->cell, mid);
+      for (int d = 0; d < D; ++d) delta[d] = MOV(mov, t, m, d) - mid[d];
+      pbc(&ch->cell, delta);
+      for (int d = 0; d < D; ++d) {
+        int k = pg_index(ch, level, sp, d);
+        if (ch->mpg_on[k]) temp *= pg_eval(&ch->mpg[k], delta[d]);
+        else lnT -= delta[d] * delta[d] * inv2Sigma2;
+      }
+      lnT += log(temp);
+    }
+  }
+  return lnT;
 }
 
 /* ------------------------------------------------------------ sampler */
@@ -1998,8 +2062,9 @@ static int choose(oq_chain* ch, const pq_sample_desc* s, int* idx, int* cyc, dou
 }
 
 /* MultiLevelSampler::run/tryMove (src/advancer/MultiLevelSampler.cc:68-177) and
-   DoubleMLSampler::run/tryMove (src/advancer/DoubleMLSampler.cc:66-245), nrepeat
-   1, no delayed rejection, section 2 not sampled.  The moving beads start
+   DoubleMLSampler::run/tryMove (src/advancer/DoubleMLSampler.cc:66-245), with
+   delayed rejection at level L-1 when the Sample asks for it (Free mover);
+   section 2 is not sampled.  The moving beads start
    from the chosen particles' first beads and end on the permuted particles'
    last beads; an accepted move writes them back (last slice included) and
    appends the permutation to the section permutation. */
@@ -2016,16 +2081,43 @@ static int sample_once(oq_chain* ch, const pq_sample_desc* smp) {
       MOV(mov, 0, m, d) = SEC(ch->sec1, 0, idx[m], d);
       MOV(mov, S - 1, m, d) = SEC(ch->sec1, S - 1, idx[cyc[m]], d);
     }
+  const int dr = smp->delayed_rejection != 0;
+  const double fac = smp->sigma_factor == 0 ? 1.0 : smp->sigma_factor;
+  const double nfac = smp->new_sigma_factor == 0 ? 0.75 : smp->new_sigma_factor;
   double old = lnTP;
   for (int l = L; l >= 0; --l) {
     ch->st.tries[l]++;
     if (l < L) ch->st.bead_updates += (uint64_t)nmov << (L - 1 - l);
+    ch->fac = fac;
     double lnT = oq_mover(ch, ch->sec1, mov, idx, nmov, l);
     double dA = oq_action_difference(ch, ch->sec1, mov, idx, nmov, l, 7);
     if (ch->nfn) dA += oq_fn_action_difference(ch, mov, idx, nmov, l);
     double piRatio = -dA + old;
     double acceptProb = exp(lnT + piRatio);
-    if (rng_next(&ch->rng) > acceptProb) return 0;
+    if (dr && l == L - 1) {
+      if (rng_next(&ch->rng) > acceptProb) {
+        /* MultiLevelSampler.cc:116-143 / DoubleMLSampler.cc:150-206 */
+        double accRatioA2B = acceptProb;
+        double transA2B = ch->fwd;
+        memcpy(ch->rej, mov, sizeof(double) * S * nmov * D);
+        ch->fac = nfac;
+        double lnTC = oq_mover(ch, ch->sec1, mov, idx, nmov, l);
+        ch->st.bead_updates += (uint64_t)nmov << (L - 1 - l);
+        dA = oq_action_difference(ch, ch->sec1, mov, idx, nmov, l, 7);
+        if (ch->nfn) dA += oq_fn_action_difference(ch, mov, idx, nmov, l);
+        double piRatioCoA = -dA + old;
+        double accRatioA2C = exp(lnTC + piRatioCoA);
+        ch->fac = fac;
+        double lnTC2B = free_delayed_mover(ch, ch->rej, mov, idx, nmov, l);
+        double transC2B = ch->fwd;
+        double accRatioC2B = exp(lnTC2B - piRatioCoA + piRatio);
+        accRatioC2B = (accRatioC2B >= 1) ? 1 : accRatioC2B;
+        double accRatioA2B2C = accRatioA2C * exp(transC2B + transA2B)
+                               * (1 - accRatioC2B) / (1 - accRatioA2B);
+        dA = dA - transC2B - transA2B - log((1 - accRatioC2B) / (1 - accRatioA2B));
+        if (rng_next(&ch->rng) > accRatioA2B2C) return 0;
+      }
+    } else if (rng_next(&ch->rng) > acceptProb) return 0;
     old = dA;
     ch->st.accepts[l]++;
   }

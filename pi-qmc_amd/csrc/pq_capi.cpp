@@ -518,6 +518,16 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
       G.sample_nm[k] = smp[k].nmoving;
       if (smp[k].nrepeat < 0) return fail(PQ_ERR_ARG, "sample %d: nrepeat must not be negative", k);
       G.sample_rep[k] = smp[k].nrepeat == 0 ? 1 : smp[k].nrepeat;
+      // PIMCParser.cc:312-322: defaultSigmaFactor 0 means 1, newSigmaFactor 0
+      // means 0.75 under delayedRejection
+      if (!(smp[k].sigma_factor >= 0) || !(smp[k].new_sigma_factor >= 0) ||
+          !std::isfinite(smp[k].sigma_factor) || !std::isfinite(smp[k].new_sigma_factor))
+        return fail(PQ_ERR_ARG, "sample %d: sigma factors must be finite and not negative", k);
+      G.sample_dr[k] = smp[k].delayed_rejection ? 1 : 0;
+      G.sample_fac[k] = smp[k].sigma_factor == 0 ? 1.0 : smp[k].sigma_factor;
+      G.sample_nfac[k] = smp[k].new_sigma_factor == 0 ? (G.sample_dr[k] ? 0.75 : 0.0) : smp[k].new_sigma_factor;
+      if (G.sample_dr[k] && d.mover != PQ_MOVER_FREE)
+        return fail(PQ_ERR_ARG, "sample %d: delayed rejection needs the Free mover", k);
       // a permuted section relabels the slices after it, which a section of
       // the whole path (2^L = M) would overlap
       const bool permuting = (ch == PQ_CHOOSER_WALKING || ch == PQ_CHOOSER_ALL_RANDOM) && smp[k].nmoving > 1;
@@ -531,6 +541,9 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     c.sample_ch[k] = c.grp[0].sample_ch[k];
     c.sample_nm[k] = c.grp[0].sample_nm[k];
     c.sample_rep[k] = c.grp[0].sample_rep[k];
+    c.sample_dr[k] = c.grp[0].sample_dr[k];
+    c.sample_fac[k] = c.grp[0].sample_fac[k];
+    c.sample_nfac[k] = c.grp[0].sample_nfac[k];
   }
   c.nsection = d.nsection;
   return PQ_OK;
@@ -595,7 +608,7 @@ static SweepLayoutH layout_for(const pq_batch* b, const Sched& req, int lo = 0, 
   lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
   lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
   o = align8(o);
-  lay.off_choose = (int)o; o += 256;   // Eng::ChooseBuf
+  lay.off_choose = (int)o; o += 512;   // Eng::ChooseBuf
   lay.off_perm = (int)o; o += align8(5 * mN * 4);   // perm, inverse, section perm, two scratch
   lay.off_walk = -1;
   if (mWalk) { lay.off_walk = (int)o; o += mWalk * 8; }
@@ -683,7 +696,11 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   }
   for (int c = 0; c < nchain; ++c)
     for (int g = 0; g < b->hc[c].ngroup; ++g)
-      for (int k = 0; k < b->hc[c].grp[g].nsample; ++k) b->multi |= b->hc[c].grp[g].sample_nm[k] > 1;
+      for (int k = 0; k < b->hc[c].grp[g].nsample; ++k) {
+        // the general instances also carry sigma factors and delayed rejection
+        const GroupP& G = b->hc[c].grp[g];
+        b->multi |= G.sample_nm[k] > 1 || G.sample_dr[k] || G.sample_fac[k] != 1.0;
+      }
   // node-matrix row width: the instantiated widths, the smallest that fits
   b->NC = ns_max <= 4 ? 4 : ns_max <= 8 ? 8 : ns_max <= 10 ? 10 : ns_max <= 12 ? 12 : ns_max <= 16 ? 16 : 32;
   for (int c = 0; c < nchain; ++c) {

@@ -301,6 +301,9 @@ struct Eng {
     int ok;
     int pad;
     double lnT[8];            // the mover's ln(T_old/T_new) of every level of the move
+    double rej[PQ_MAX_MOVING * 3];   // delayed rejection: the rejected level L-1 beads,
+    double transA2B, accA2B, piBoA;  // their forward probability, acceptance and pi ratio,
+    unsigned long long kC;           // and the first draw of the second proposal
   };
   const DevCtx& x;
   const ChainP& c;
@@ -314,6 +317,7 @@ struct Eng {
   double* sec2;     // [S][N][D]
   double* mov;      // [S][nmov][D]
   int nmov_ = 1;    // moving particles of the current move, nmov()
+  double fac = 1.0; // the movers' sigma factor (MultiLevelSampler::factor), MM > 1 only
   const int* midx = nullptr;   // [nmov] their indices
   int mi0 = 0;      // the moving particle when MM = 1 (team-uniform register copy)
   double* fst;      // fixed-node state block
@@ -558,7 +562,9 @@ struct Eng {
   // nmov D Gaussians by Box-Muller pairs over the flattened array
   // (RandomNumGenerator.h:75-86), then, for FreeMoverPBC, D image uniforms per
   // moving particle; each lane computes its own from the draw offsets.
-  // Returns ln(T_old/T_new).
+  // Returns ln(T_old/T_new); with fwd, FreeMover's forward probability
+  // (getForwardProb: the new beads' Gaussian exponents and the log of their
+  // inverse periodic-Gaussian weights) goes there.
   // uniforms the mover draws at a level (FreeMover / FreeMoverPBC)
   __device__ __forceinline__ int mover_draws(int level) const {
     const int nmoved = (level < c.L) ? (1 << (c.L - 1 - level)) : 0;
@@ -566,7 +572,7 @@ struct Eng {
     return nmoved * (2 * ((ng + 1) / 2) + (c.mover == PQ_MOVER_FREE_PBC ? ng : 0));
   }
 
-  __device__ double mover(int level, uint64_t k0, int* ndraws) {
+  __device__ double mover(int level, uint64_t k0, int* ndraws, double* fwd = nullptr) {
     const int s = 1 << level;
     const int nmoved = (level < c.L) ? (1 << (c.L - 1 - level)) : 0;
     const bool pbcmv = c.mover == PQ_MOVER_FREE_PBC;
@@ -574,7 +580,7 @@ struct Eng {
     const int gd = 2 * ((ng + 1) / 2);
     const int pd = gd + (pbcmv ? ng : 0);
     *ndraws = nmoved * pd;
-    double lnT = 0;
+    double lnT = 0, fw = 0;
     for (int it = tid; it < nmoved * nmov(); it += NT) {
       int kk = it, m = 0;
       if (nmov() > 1) { kk = it / nmov(); m = it - kk * nmov(); }
@@ -610,7 +616,7 @@ struct Eng {
         }
       }
       const double lam = c.lambda[sp];
-      const double sigma = 1.0 * sqrt(lam * c.tau * s);
+      const double sigma = (MM > 1 ? fac : 1.0) * sqrt(lam * c.tau * s);
       const double inv2Sigma2 = 0.5 / (sigma * sigma);
       const double* mp = mov + ((t - s) * nmov() + m) * D;
       const double* mn = mov + ((t + s) * nmov() + m) * D;
@@ -639,9 +645,14 @@ struct Eng {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           if (c.mpg_on[level][sp][d]) temp *= pg_eval(pg_make(c.mass[sp] / (c.tau * s), c.a[d]), delta[d]);
-          else lnT += delta[d] * delta[d] * inv2Sigma2;
+          else {
+            const double tmp = delta[d] * delta[d] * inv2Sigma2;
+            if (fwd) fw += tmp;
+            lnT += tmp;
+          }
         }
         temp = 1.0 / temp;
+        if (fwd) fw += log(temp);
 #pragma unroll
         for (int d = 0; d < D; ++d) mid[d] = on[d] - op[d];
         pbc<D>(c, mid);
@@ -722,7 +733,69 @@ struct Eng {
         lnT += log(temp);
       }
     }
-    return bsum(lnT);
+    const double r = bsum(lnT);
+    if (fwd) {
+      const double f = bsum(fw);
+      if (tid == 0) *fwd = f;
+    }
+    return r;
+  }
+
+  // FreeMover::makeDelayedMove (FreeMover.cc:126-184) at level L-1, whose one
+  // moved slice t = 2^(L-1) lies between the section's end slices: ln(T_old/
+  // T_new) of proposing the rejected beads B (rej, [nmov][D]) from the second
+  // proposal C (in mov) against C from B, both about the same midpoint, and
+  // its forward probability (minus B's Gaussian exponents plus the log of
+  // B's periodic-Gaussian weights) in fwd.
+  __device__ double delayed_mover(int level, const double* rej, double* fwd) {
+    const int s = 1 << level;
+    const int t = s;
+    double lnT = 0, fw = 0;
+    for (int m = tid; m < nmov(); m += NT) {
+      const int i = mv(m);
+      const int sp = species_of(c, i);
+      const double sigma = fac * sqrt(c.lambda[sp] * c.tau * s);
+      const double inv2Sigma2 = 0.5 / (sigma * sigma);
+      const double* mp = mov + ((t - s) * nmov() + m) * D;
+      const double* mn = mov + ((t + s) * nmov() + m) * D;
+      const double* mc = mov + (t * nmov() + m) * D;
+      const double* rb = rej + m * D;
+      double mid[D], delta[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) mid[d] = mn[d] - mp[d];
+      pbc<D>(c, mid);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += mp[d]; }
+      pbc<D>(c, mid);
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = rb[d] - mid[d];
+      pbc<D>(c, delta);
+      double temp = 1.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (c.mpg_on[level][sp][d]) temp *= pg_eval(pg_make(c.mass[sp] / (c.tau * s), c.a[d]), delta[d]);
+        else {
+          const double tmp = delta[d] * delta[d] * inv2Sigma2;
+          fw -= tmp;
+          lnT += tmp;
+        }
+      }
+      fw += log(temp);
+      temp = 1.0 / temp;
+      // the end slices are the same in B and C, and so is the midpoint
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = mc[d] - mid[d];
+      pbc<D>(c, delta);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (c.mpg_on[level][sp][d]) temp *= pg_eval(pg_make(c.mass[sp] / (c.tau * s), c.a[d]), delta[d]);
+        else lnT -= delta[d] * delta[d] * inv2Sigma2;
+      }
+      lnT += log(temp);
+    }
+    const double r = bsum(lnT);
+    *fwd = bsum(fw);
+    return r;
   }
 
   // ---------------------------------------------------- action layer

@@ -50,6 +50,9 @@ struct GroupP {          // one <ChooseSection nlevel> of a sweep with its Sampl
   int sample_ch[8];
   int sample_nm[8];
   int sample_rep[8];
+  int sample_dr[8];
+  double sample_fac[8];
+  double sample_nfac[8];
 };
 
 struct ChainP {
@@ -94,6 +97,9 @@ struct ChainP {
   GroupP grp[PQ_GROUPS];
   double ceps;           // CoulombAction epsilon
   EwP ew;                // CoulombAction's Ewald sum (useEwald)
+  int sample_dr[8];      // delayed rejection at level L-1
+  double sample_fac[8];  // the movers' sigma factor (defaultSigmaFactor)
+  double sample_nfac[8]; // sigma factor of the delayed second proposal
 };
 
 // Doubles of one slice-pair node record of a species of ns particles:

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
   if (c.fnc_off >= 0) e.fcache = x.fncache + c.fnc_off;
   e.top_known = true;
   e.lasterr = e.notmine + PQ_MAX_NODES;
-  static_assert(sizeof(typename Eng<D, NC, R, NT, MM>::ChooseBuf) <= 256, "chooser scratch outgrew its LDS slot");
+  static_assert(sizeof(typename Eng<D, NC, R, NT, MM>::ChooseBuf) <= 512, "chooser scratch outgrew its LDS slot");
   e.cbuf = reinterpret_cast<typename Eng<D, NC, R, NT, MM>::ChooseBuf*>(base + lay.off_choose);
   if (lay.off_walk >= 0) e.wrows = reinterpret_cast<double*>(base + lay.off_walk);
   // path permutation and inverse, section permutation, scratch
@@ -160,6 +160,11 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
           c.sample_ch[q] = G.sample_ch[q];
           c.sample_nm[q] = G.sample_nm[q];
           c.sample_rep[q] = G.sample_rep[q];
+          if constexpr (MM > 1) {
+            c.sample_dr[q] = G.sample_dr[q];
+            c.sample_fac[q] = G.sample_fac[q];
+            c.sample_nfac[q] = G.sample_nfac[q];
+          }
         }
       }
       __syncthreads();
@@ -236,32 +241,103 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
         // coarser levels' new beads, and its draws start after the draws and
         // the accept draw of the levels above, wherever the move stops.  The
         // node determinants of every changed slice then take one pass.
-        {
-          uint64_t kk = k;
-          for (int l = L; l >= 0; --l) {
-            int nd;
-            const double lnT = e.mover(l, kk, &nd);
-            kk += nd + 1;
-            if (tid == 0) e.cbuf->lnT[l] = lnT;
-            __syncthreads();
-          }
-          if (c.nfn) e.fn_eval_moved();
-        }
+        // The general instances (MM > 1) also run the movers' sigma factors
+        // and delayed rejection (MultiLevelSampler.cc:114-143,
+        // DoubleMLSampler.cc:150-206): the beads B rejected at level L-1 are
+        // kept and the move goes round once more, C at level L-1 with the
+        // second sigma factor, its draws after B's accept draw, and the
+        // levels below proposed again after C's accept draw.
         double old = lnTP;
         bool accepted = true;
-        for (int l = L; l >= 0; --l) {
-          if (tid == 0) {
-            cnt[l]++;
-            if (l < L) cnt[16] += (unsigned long long)nmov << (L - 1 - l);
+        if constexpr (MM == 1) {
+          {
+            uint64_t kk = k;
+            for (int l = L; l >= 0; --l) {
+              int nd;
+              const double lnT = e.mover(l, kk, &nd);
+              kk += nd + 1;
+              if (tid == 0) e.cbuf->lnT[l] = lnT;
+              __syncthreads();
+            }
+            if (c.nfn) e.fn_eval_moved();
           }
-          k += e.mover_draws(l);
-          double dA = e.action_diff(l);
-          if (c.nfn) dA += e.fn_diff_moved(l);
-          const double piRatio = -dA + old;
-          const double acceptProb = exp(e.cbuf->lnT[l] + piRatio);
-          if (e.U(k++) > acceptProb) { accepted = false; break; }
-          old = dA;
-          if (tid == 0) cnt[8 + l]++;
+          for (int l = L; l >= 0; --l) {
+            if (tid == 0) {
+              cnt[l]++;
+              if (l < L) cnt[16] += (unsigned long long)nmov << (L - 1 - l);
+            }
+            k += e.mover_draws(l);
+            double dA = e.action_diff(l);
+            if (c.nfn) dA += e.fn_diff_moved(l);
+            const double piRatio = -dA + old;
+            const double acceptProb = exp(e.cbuf->lnT[l] + piRatio);
+            if (e.U(k++) > acceptProb) { accepted = false; break; }
+            old = dA;
+            if (tid == 0) cnt[8 + l]++;
+          }
+        } else {
+          const bool dr = c.sample_dr[smp] && L >= 1;
+          int lfrom = L;     // below L: the delayed second proposal from level lfrom
+          for (;;) {
+            {
+              uint64_t kk = lfrom < L ? e.cbuf->kC : k;
+              for (int l = lfrom; l >= 0; --l) {
+                int nd;
+                e.fac = (lfrom < L && l == lfrom) ? c.sample_nfac[smp] : c.sample_fac[smp];
+                const double lnT = e.mover(l, kk, &nd, dr && lfrom == L && l == L - 1 ? &e.cbuf->transA2B : nullptr);
+                kk += nd + 1;
+                if (tid == 0) e.cbuf->lnT[l] = lnT;
+                __syncthreads();
+              }
+              e.fac = c.sample_fac[smp];
+              if (c.nfn) e.fn_eval_moved();
+            }
+            const int lstart = lfrom;
+            for (int l = lstart; l >= 0; --l) {
+              const bool second = lstart < L && l == lstart;
+              if (tid == 0) {
+                if (!second) cnt[l]++;
+                if (l < L) cnt[16] += (unsigned long long)nmov << (L - 1 - l);
+              }
+              k += e.mover_draws(l);
+              double dA = e.action_diff(l);
+              if (c.nfn) dA += e.fn_diff_moved(l);
+              if (second) {
+                const double transA2B = e.cbuf->transA2B, accA2B = e.cbuf->accA2B;
+                const double piCoA = -dA + old;
+                const double accA2C = exp(e.cbuf->lnT[l] + piCoA);
+                double transC2B;
+                const double lnTC2B = e.delayed_mover(l, e.cbuf->rej, &transC2B);
+                double accC2B = exp(lnTC2B - piCoA + e.cbuf->piBoA);
+                accC2B = (accC2B >= 1) ? 1 : accC2B;
+                const double accA2B2C = accA2C * exp(transC2B + transA2B) * (1 - accC2B) / (1 - accA2B);
+                dA = dA - transC2B - transA2B - log((1 - accC2B) / (1 - accA2B));
+                if (e.U(k++) > accA2B2C) { accepted = false; break; }
+              } else {
+                const double piRatio = -dA + old;
+                const double acceptProb = exp(e.cbuf->lnT[l] + piRatio);
+                if (e.U(k++) > acceptProb) {
+                  if (dr && l == L - 1) {
+                    const int tm = 1 << l;
+                    for (int q = tid; q < nmov * D; q += NT) e.cbuf->rej[q] = e.mov[tm * nmov * D + q];
+                    if (tid == 0) {
+                      e.cbuf->accA2B = acceptProb;
+                      e.cbuf->piBoA = piRatio;
+                      e.cbuf->kC = k;
+                    }
+                    __syncthreads();
+                    lfrom = l;
+                  } else {
+                    accepted = false;
+                  }
+                  break;
+                }
+              }
+              old = dA;
+              if (tid == 0) cnt[8 + l]++;
+            }
+            if (!accepted || lfrom == lstart) break;
+          }
         }
         if (accepted) {
           if (c.nfn) e.fn_accept(permuted);

@@ -60,7 +60,20 @@ class NodeDesc(C.Structure):
 
 class SampleDesc(C.Structure):
     _fields_ = [("species", C.c_int), ("nmoving", C.c_int), ("chooser", C.c_int),
-                ("nrepeat", C.c_int)]
+                ("nrepeat", C.c_int), ("delayed_rejection", C.c_int),
+                ("sigma_factor", C.c_double), ("new_sigma_factor", C.c_double)]
+
+
+def _fill_sample(dst, smp):
+    """smp: (species, npart[, chooser[, nrepeat[, opts]]]) with opts a dict of
+    delayed_rejection, sigma_factor (defaultSigmaFactor) and new_sigma_factor."""
+    dst.species, dst.nmoving = smp[0], smp[1]
+    dst.chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
+    dst.nrepeat = smp[3] if len(smp) > 3 else 1
+    opts = smp[4] if len(smp) > 4 else {}
+    dst.delayed_rejection = 1 if opts.get("delayed_rejection") else 0
+    dst.sigma_factor = opts.get("sigma_factor", 0.0)
+    dst.new_sigma_factor = opts.get("new_sigma_factor", 0.0)
 
 
 PQ_MAX_GROUPS = 4
@@ -102,7 +115,8 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
     """species: [(count, mass, charge[, is_static])]; sho: [(species, omega)];
     coulomb: dict(epsilon, norder, rmin, rmax, ngpts[, ewald, ewald_rcut,
     ewald_kcut]) or None;
-    nodes: [(species, model, omega)]; samples [(species, npart[, chooser[, nrepeat]])],
+    nodes: [(species, model, omega)]; samples [(species, npart[, chooser[, nrepeat[,
+    opts]]])] with opts {delayed_rejection, sigma_factor, new_sigma_factor},
     default one npart=1 SpeciesParticleChooser per species;
     nsection default M / 2^L (one sweep touches every slice once on average)."""
     d = ChainDesc()
@@ -138,22 +152,18 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
         samples = [(s, 1) for s in range(len(species))]
     d.nsample = len(samples)
     for k, smp in enumerate(samples):
-        d.sample[k].species, d.sample[k].nmoving = smp[0], smp[1]
-        d.sample[k].chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
-        d.sample[k].nrepeat = smp[3] if len(smp) > 3 else 1
+        _fill_sample(d.sample[k], smp)
     d.nsection = nsection if nsection is not None else max(1, nslice >> nlevel)
     return d
 
 
 def add_group(d, nlevel, samples, nsection):
-    """Append a <ChooseSection nlevel> group (samples [(species, npart[,
-    chooser])], repeated nsection times) to a sweep."""
+    """Append a <ChooseSection nlevel> group (samples as for make_desc,
+    repeated nsection times) to a sweep."""
     g = d.group[d.ngroup]
     g.nlevel, g.nsample, g.nsection = nlevel, len(samples), nsection
     for k, smp in enumerate(samples):
-        g.sample[k].species, g.sample[k].nmoving = smp[0], smp[1]
-        g.sample[k].chooser = smp[2] if len(smp) > 2 else PQ_CHOOSER_SPECIES
-        g.sample[k].nrepeat = smp[3] if len(smp) > 3 else 1
+        _fill_sample(g.sample[k], smp)
     d.ngroup += 1
     return d
 

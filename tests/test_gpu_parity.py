@@ -649,3 +649,76 @@ def test_node_width_split_decisions(team, heavy_team):
         sg, so = b.stats(c), oc.stats()
         assert sg.draws == so.draws and list(sg.accepts) == list(so.accepts), c
         assert np.max(np.abs(b.get_paths(c) - oc.get_beads())) < 1e-9, c
+
+
+def _dr_descs():
+    """<Sample delayedRejection defaultSigmaFactor newSigmaFactor> cases."""
+    wide = {"delayed_rejection": 1, "sigma_factor": 1.5, "new_sigma_factor": 3.0}
+    sho1 = capi.make_desc(3, 64, 3, 1 / 64, [(1, 1.0, -1.0)], cell=50.0, sho=[(0, 1.0)],
+                          samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, wide)])
+    cfg1 = capi.cfg1_desc()
+    for k in range(cfg1.nsample):
+        cfg1.sample[k].delayed_rejection = 1
+        cfg1.sample[k].sigma_factor = 1.2
+    cfg2 = small_cfg2()
+    for k in range(cfg2.nsample):
+        cfg2.sample[k].delayed_rejection = 1
+        cfg2.sample[k].sigma_factor = 1.3
+        cfg2.sample[k].new_sigma_factor = 0.6
+    # a small box: the movers' periodic Gaussians at the delayed level
+    box = capi.make_desc(3, 64, 3, 0.05, [(2, 1.0, -1.0), (2, 2.0, 1.0)], cell=2.0,
+                         coulomb=dict(epsilon=1.0, norder=3, rmin=0.0, rmax=0.0, ngpts=512),
+                         samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, wide), (1, 1, capi.PQ_CHOOSER_SPECIES, 1, wide)])
+    # two moving particles per move (the multi-particle sampler instance)
+    multi = small_cfg2()
+    multi.nsample = 1
+    multi.sample[0].nmoving = 2
+    multi.sample[0].delayed_rejection = 1
+    multi.sample[0].sigma_factor = 1.3
+    # a scaled mover without delayed rejection
+    fac = capi.cfg1_desc()
+    for k in range(fac.nsample):
+        fac.sample[k].sigma_factor = 0.8
+    return {"sho1": sho1, "cfg1": cfg1, "cfg2": cfg2, "box": box, "multi": multi, "factor": fac}
+
+
+@pytest.mark.parametrize("name,inject", [("sho1", True), ("sho1", False), ("cfg1", True), ("cfg2", False),
+                                         ("box", False), ("multi", True), ("factor", False)])
+def test_delayed_rejection_decisions(name, inject):
+    """Delayed rejection at level L-1 and the movers' sigma factors
+    (MultiLevelSampler.cc:100-177, DoubleMLSampler.cc:122-245): the second
+    proposal's draws, the lower levels proposed again after it, and the
+    oracle's decisions, counters and paths."""
+    print(f"[test] delayed rejection {name} inject={inject}", flush=True)
+    desc = _dr_descs()[name]
+    b, oc = _paired(desc, 6, inject=inject)
+    sg, so = b.stats(0), oc.stats()
+    assert sg.draws == so.draws
+    assert list(sg.tries) == list(so.tries)
+    assert list(sg.accepts) == list(so.accepts)
+    assert sg.bead_updates == so.bead_updates
+    assert sg.node_errors == so.node_errors
+    L = desc.nlevel
+    nm = max(desc.sample[k].nmoving for k in range(desc.nsample))
+    plain = sum(sg.tries[l] * (nm << (L - 1 - l)) for l in range(L))
+    if desc.sample[0].delayed_rejection:
+        assert sg.bead_updates > plain       # second proposals happened
+    assert np.max(np.abs(b.get_paths(0) - oc.get_beads())) < 1e-9
+
+
+def test_delayed_rejection_inputs_are_checked():
+    """Delayed rejection needs the Free mover (FreeMoverPBC never sets the
+    forward probability the second stage reads); sigma factors must be
+    finite and not negative."""
+    d = _descs()["pbc"]
+    d.sample[0].delayed_rejection = 1
+    with pytest.raises(capi.PQError):
+        capi.Batch([d])
+    d = capi.cfg1_desc()
+    d.sample[0].sigma_factor = -1.0
+    with pytest.raises(capi.PQError):
+        capi.Batch([d])
+    d = capi.cfg1_desc()
+    d.sample[1].new_sigma_factor = float("nan")
+    with pytest.raises(capi.PQError):
+        capi.Batch([d])

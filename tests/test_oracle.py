@@ -471,6 +471,92 @@ def test_sample_nrepeat_tries_and_balance():
     assert abs(blocks.mean() - 0.75 / math.tanh(0.25)) < max(4 * err, 0.08), (blocks.mean(), err)
 
 
+def _sho_seed_energies(opts, L=3, nseed=8, nblock=100, blocksize=250):
+    """One SHO particle (3D, omega 0.5, T 1, M 32) per seed; the mean thermal
+    energy of each seed's run and the sampler counters of the last one."""
+    means, st = [], None
+    for seed in range(nseed):
+        d = capi.make_desc(3, 32, L, 1 / 32, [(1, 1.0, -1.0)], cell=20.0, sho=[(0, 0.5)],
+                           samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, opts)], nsection=4)
+        ch = ol.Chain(d)
+        ch.set_beads(np.zeros((32, 1, 3)))
+        ch.rng_philox(100 + seed, 0)
+        ch.run_sweeps(200)
+        means.append(ch.energy_blocks(nblock, blocksize).mean())
+        st = ch.stats()
+    return np.array(means), st
+
+
+@pytest.mark.parametrize("opts", [{"sigma_factor": 0.9}, {"delayed_rejection": 1}])
+def test_sigma_factor_and_delayed_rejection_balance(opts):
+    """<Sample defaultSigmaFactor newSigmaFactor delayedRejection>
+    (MultiLevelSampler.cc:100-177): a scaled mover and the delayed second
+    proposal at level L-1 keep detailed balance, so independent seeds give
+    E = 0.75 coth(0.25) for the 3D SHO, omega 0.5, T 1.  Delayed rejection
+    shows in the bead updates: each second proposal adds one bead beyond
+    what the tried levels propose."""
+    means, st = _sho_seed_energies(opts)
+    err = means.std(ddof=1) / math.sqrt(len(means))
+    exact = 0.75 / math.tanh(0.25)
+    assert abs(means.mean() - exact) < max(4 * err, 0.05), (means.mean(), err)
+    L = 3
+    plain = sum(st.tries[l] << (L - 1 - l) for l in range(L))
+    if opts.get("delayed_rejection"):
+        assert st.bead_updates > plain
+    else:
+        assert st.bead_updates == plain
+
+
+@pytest.mark.parametrize("L,M", [(1, 4), (2, 8)])
+def test_delayed_rejection_conditional_bead_variance(L, M):
+    """Delayed rejection with a wide first (defaultSigmaFactor 1.5) and a
+    wider second proposal (newSigmaFactor 3): with L = 1 the move is
+    Tierney-Mira two-stage delayed rejection of one bead, and the exact SHO
+    link action (spring + SHOAction, m = omega = 1, tau = 1/M) fixes each
+    bead's law given its neighbours: Gaussian about (x_{t-1} + x_{t+1}) /
+    (2 cosh tau) with variance tanh(tau)/2 per coordinate.  The sample mean
+    of the squared residual must match it; using the second proposal's
+    factor in the return move (C to B) instead of the first's moves it by
+    more than 20 standard errors at L = 1."""
+    tau = 1.0 / M
+    exact = math.tanh(tau) / 2
+    opts = {"delayed_rejection": 1, "sigma_factor": 1.5, "new_sigma_factor": 3.0}
+    means = []
+    for seed in range(4):
+        d = capi.make_desc(2, M, L, tau, [(1, 1.0, -1.0)], cell=50.0, sho=[(0, 1.0)],
+                           samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, opts)], nsection=M)
+        ch = ol.Chain(d)
+        ch.set_beads(np.zeros((M, 1, 2)))
+        ch.rng_philox(seed, 0)
+        ch.run_sweeps(100)
+        acc = []
+        for _ in range(15000):
+            ch.run_sweeps(1)
+            x = ch.get_beads()[:, 0, :]
+            r = x - (np.roll(x, 1, 0) + np.roll(x, -1, 0)) / (2 * math.cosh(tau))
+            acc.append(np.mean(r * r))
+        means.append(np.mean(acc))
+    means = np.array(means)
+    err = means.std(ddof=1) / math.sqrt(len(means))
+    assert abs(means.mean() - exact) < max(4 * err, 0.01 * exact), (means.mean(), err, exact)
+
+
+def test_sigma_factor_changes_the_proposals_only():
+    """defaultSigmaFactor 1 is the reference's default: the same draws give
+    the same path as no factor; another factor changes the path."""
+    def path(opts):
+        d = capi.make_desc(3, 32, 3, 1 / 32, [(1, 1.0, -1.0)], cell=20.0, sho=[(0, 0.5)],
+                           samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, opts)], nsection=4)
+        ch = ol.Chain(d)
+        ch.set_beads(np.zeros((32, 1, 3)))
+        ch.rng_philox(5, 0)
+        ch.run_sweeps(20)
+        return ch.get_beads()
+    base = path({})
+    assert np.array_equal(path({"sigma_factor": 1.0}), base)
+    assert not np.array_equal(path({"sigma_factor": 0.9}), base)
+
+
 def test_many_body_nodal_distance_reduces_to_one_body():
     """useManyBodyDistance with one particle in all, of the node species: the
     distance 1/sqrt(sum 1/d^2) is d itself, so the nodal action equals the
