@@ -722,3 +722,37 @@ def test_delayed_rejection_inputs_are_checked():
     d.sample[1].new_sigma_factor = float("nan")
     with pytest.raises(capi.PQError):
         capi.Batch([d])
+
+
+def test_delayed_rejection_samples_the_exact_bead_law():
+    """The device's delayed rejection on its own: 256 chains of one SHO
+    particle (2D, m = omega = 1, M = 4, tau = 1/4, L = 1, defaultSigmaFactor
+    1.5, newSigmaFactor 3), each on its own Philox stream.  With the exact
+    link action each bead's law given its neighbours is Gaussian about
+    (x_{t-1} + x_{t+1}) / (2 cosh tau) with variance tanh(tau)/2 per
+    coordinate; the chains' mean squared residual matches it."""
+    print("[test] delayed rejection bead law on the device", flush=True)
+    M, tau, nch = 4, 0.25, 256
+    opts = {"delayed_rejection": 1, "sigma_factor": 1.5, "new_sigma_factor": 3.0}
+    d = capi.make_desc(2, M, 1, tau, [(1, 1.0, -1.0)], cell=50.0, sho=[(0, 1.0)],
+                       samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, opts)], nsection=M)
+    b = capi.Batch([d] * nch)
+    for c in range(nch):
+        b.set_paths(c, np.zeros((M, 1, 2)))
+    b.seed(2024, 0)
+    b.run_sweeps(100)
+    acc = np.zeros(nch)
+    nsnap = 150
+    for _ in range(nsnap):
+        b.run_sweeps(2)
+        b.sync()
+        for c in range(nch):
+            x = b.get_paths(c)[:, 0, :]
+            r = x - (np.roll(x, 1, 0) + np.roll(x, -1, 0)) / (2 * np.cosh(tau))
+            acc[c] += np.mean(r * r)
+    per_chain = acc / nsnap
+    exact = np.tanh(tau) / 2
+    err = per_chain.std(ddof=1) / np.sqrt(nch)
+    assert abs(per_chain.mean() - exact) < max(4 * err, 0.01 * exact), (per_chain.mean(), err, exact)
+    st = b.stats(0)
+    assert st.bead_updates > st.tries[0]          # second proposals happened
