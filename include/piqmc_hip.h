@@ -326,6 +326,13 @@ int pq_measure_coulomb_energy(pq_batch* batch, double epsilon, double* energy);
    (1/M) over slices.  The fit is built once per (rcut, kcut) and cell. */
 int pq_measure_ewald_coulomb_energy(pq_batch* batch, double epsilon, double rcut, double kcut,
                                     double* energy);
+/* The same estimator with ewaldType="tradEwald" and ewaldImages 1 (the
+   default): a TradEwaldSum, f(r) = erf(kappa r)/r with the reference's erf,
+   kappa 0 meaning the parser's default sqrt((3.6 N)^(1/6) sqrt(pi) / Lside),
+   Lside the shortest cell side (EwaldCoulombEstimator.cc:17-61,
+   EstimatorParser.cc:191-209).  ewaldImages > 1 is not modelled. */
+int pq_measure_trad_ewald_coulomb_energy(pq_batch* batch, double epsilon, double rcut, double kcut,
+                                         double kappa, double* energy);
 /* Virial energy of every chain's current paths (<VirialEnergyEstimator nwindow>,
    src/estimator/VirialEnergyEstimator.cc:27-78): utau plus the centroid-virial
    force terms of getBeadAction over windows of nwindow links (0 means 1, the

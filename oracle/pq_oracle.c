@@ -2386,16 +2386,34 @@ double oq_coulomb_energy(const oq_chain* ch, double epsilon) {
    grid of -f/eps, whose interpolation weight is r - ig dr (not divided by dr,
    as the reference has it), and the k-space sum on particle 0 of every slice */
 double oq_ewald_coulomb_energy(const oq_chain* ch, double epsilon, double rcut, double kcut) {
+  return oq_ewald_coulomb_energy_type(ch, epsilon, rcut, kcut, 0, 0.0);
+}
+
+/* the estimator's default kappa of its TradEwaldSum (EstimatorParser.cc:194-203):
+   sqrt((3.6 N)^(1/6) sqrt(pi) / Lside), Lside the shortest cell side */
+double oq_trad_estimator_kappa(const oq_chain* ch) {
+  double lside = 1000000000000000.0;
+  for (int d = 0; d < ch->D; ++d)
+    if (lside > ch->cell.a[d]) lside = ch->cell.a[d];
+  return sqrt(pow(3.6 * ch->N, 1.0 / 6.0) * sqrt(3.1415926535897931) / lside);
+}
+
+/* EwaldCoulombEstimator with an OptEwaldSum (trad 0) or, for ewaldType
+   "tradEwald" with ewaldImages 1, a TradEwaldSum of kappa (0: the parser's
+   default) (EwaldCoulombEstimator.cc:17-61,105-140) */
+double oq_ewald_coulomb_energy_type(const oq_chain* ch, double epsilon, double rcut, double kcut,
+                                    int trad, double kappa) {
   const int D = ch->D, N = ch->N;
   if (epsilon == 0) epsilon = 1;
+  if (trad && kappa == 0) kappa = oq_trad_estimator_kappa(ch);
   double q[MAXN];
   for (int i = 0; i < N; ++i) q[i] = ch->d.species[ch->spec[i]].charge;
   ewald_t e;
-  ewald_init(&e, &ch->cell, N, q, rcut, kcut, 0, 0.0);
+  ewald_init(&e, &ch->cell, N, q, rcut, kcut, trad, kappa);
   const int nradial = 1001;
   const double dr = rcut / 1000, drinv = 1. / dr;
   double* vgrid = (double*)malloc(sizeof(double) * nradial);
-  vgrid[0] = -e.coef[0] / epsilon;
+  vgrid[0] = -(trad ? 2 * kappa / sqrt(EW_PI) : e.coef[0]) / epsilon;   /* evalFR0 */
   for (int i = 1; i < nradial; ++i) vgrid[i] = -ew_fr(&e, i * dr) / epsilon;
   double value = 0;
   for (int t = 0; t < ch->M; ++t)

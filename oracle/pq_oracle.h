@@ -80,6 +80,9 @@ double oq_thermo_energy(const oq_chain* ch);
 double oq_coulomb_energy(const oq_chain* ch, double epsilon);
 /* CoulombEnergyEstimator useEwald="true" (optEwald) with its own rcut, kcut */
 double oq_ewald_coulomb_energy(const oq_chain* ch, double epsilon, double rcut, double kcut);
+double oq_ewald_coulomb_energy_type(const oq_chain* ch, double epsilon, double rcut, double kcut,
+                                    int trad, double kappa);
+double oq_trad_estimator_kappa(const oq_chain* ch);
 /* the chain's CoulombAction Ewald sum: fit, k-vector ik, self energy */
 int oq_ewald_info(const oq_chain* ch, double* coef, int* nk, double* self, int ik, int* kv, double* vk);
 double oq_ewald_long_range(const oq_chain* ch, const double* r);

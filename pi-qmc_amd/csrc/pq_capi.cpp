@@ -194,9 +194,12 @@ struct pq_batch {
   std::vector<double> hk;
   double* dewk = nullptr;
   // the EwaldCoulombEstimator's sums, kept for repeated (rcut, kcut, epsilon)
-  double est_rcut = -1, est_kcut = -1;
+  double est_rcut = -1, est_kcut = -1, est_kappa = -1;
+  int est_type = -1;
   EwP* dew_est = nullptr;
   double* dewk_est = nullptr;
+  int64_t* dvoff_est = nullptr;   // tradEwald: offset of each chain's -f(i dr) grid, -1 for optEwald
+  double* dvgrid_est = nullptr;
   std::vector<int> order;
   int64_t nbeads = 0;
   SweepLayoutH lay{};
@@ -818,7 +821,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache, b->dfnsweep,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr,
-                  b->dperm, b->dewk, b->dew_est, b->dewk_est};
+                  b->dperm, b->dewk, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -1401,62 +1404,108 @@ extern "C" int pq_measure_coulomb_energy(pq_batch* b, double epsilon, double* en
   return PQ_OK;
 }
 
-extern "C" int pq_measure_ewald_coulomb_energy(pq_batch* b, double epsilon, double rcut, double kcut,
-                                               double* energy) {
-  if (!b || !energy || !(rcut > 0) || !(kcut > 0))
-    return fail(PQ_ERR_ARG, "pq_measure_ewald_coulomb_energy: bad arguments (rcut and kcut must be positive)");
+// EwaldCoulombEstimator (EwaldCoulombEstimator.cc:17-140) with an
+// OptEwaldSum (trad false) or a TradEwaldSum of kappa, 0 meaning the parser's
+// default sqrt((3.6 N)^(1/6) sqrt(pi) / Lside) (EstimatorParser.cc:192-203).
+// One fit per distinct cell (and, for tradEwald, kappa) of the batch, charges
+// per chain; tradEwald's 1001-point grid of -f(i rcut/1000) is tabulated on
+// the host with the reference's erf.
+static int measure_ewald_coulomb(pq_batch* b, double epsilon, double rcut, double kcut, bool trad,
+                                 double kappa, double* energy) {
+  if (!b || !energy || !(rcut > 0) || !(kcut > 0) || !(kappa >= 0) || !std::isfinite(kappa))
+    return fail(PQ_ERR_ARG, "Ewald Coulomb estimator: bad arguments (rcut and kcut must be positive)");
   if (epsilon == 0) epsilon = 1;
   HIPCHK(hipSetDevice(b->device));
-  if (rcut != b->est_rcut || kcut != b->est_kcut) {
-    // one OptEwaldSum per distinct cell of the batch, charges per chain
-    std::map<std::tuple<double, double, double>, std::pair<EwaldFit, int64_t>> fits;
-    std::vector<double> pool;
+  if (rcut != b->est_rcut || kcut != b->est_kcut || (int)trad != b->est_type || kappa != b->est_kappa) {
+    std::map<std::tuple<double, double, double, double>, std::tuple<EwaldFit, int64_t, int64_t>> fits;
+    std::vector<double> pool, vpool;
     std::vector<EwP> ew(b->nchain);
+    std::vector<int64_t> voff(b->nchain, -1);
     for (int c = 0; c < b->nchain; ++c) {
       const pq_chain_desc& d = b->descs[c];
-      const auto key = std::make_tuple(d.cell[0], d.cell[1], b->D == 3 ? d.cell[2] : 0.0);
+      double kap = 0;
+      if (trad) {
+        kap = kappa;
+        if (kap == 0) {
+          int n = 0;
+          for (int s2 = 0; s2 < d.nspecies; ++s2) n += d.species[s2].count;
+          double lside = 1000000000000000.0;
+          for (int i = 0; i < b->D; ++i) lside = std::min(lside, d.cell[i]);
+          kap = std::sqrt(std::pow(3.6 * n, 1.0 / 6.0) * std::sqrt(3.1415926535897931) / lside);
+        }
+      }
+      const auto key = std::make_tuple(d.cell[0], d.cell[1], b->D == 3 ? d.cell[2] : 0.0, kap);
       auto it = fits.find(key);
       if (it == fits.end()) {
         EwaldFit e;
-        if (!ewald_build(e, b->D, d.cell, rcut, kcut))
-          return fail(PQ_ERR_ARG, "the Ewald fit for rcut %g, kcut %g is not finite", rcut, kcut);
+        const bool ok = trad ? ewald_build_trad(e, b->D, d.cell, rcut, kcut, kap) : ewald_build(e, b->D, d.cell, rcut, kcut);
+        if (!ok) return fail(PQ_ERR_ARG, "the Ewald fit for rcut %g, kcut %g is not finite", rcut, kcut);
         const int64_t off = (int64_t)pool.size();
         for (int k = 0; k < e.nk(); ++k) {
           for (int q = 0; q < 3; ++q) pool.push_back(e.kvec[3 * k + q]);
           pool.push_back(2 * e.vk[k] * e.oneOver2V);
         }
-        it = fits.emplace(key, std::make_pair(std::move(e), off)).first;
+        int64_t vo = -1;
+        if (trad) {
+          vo = (int64_t)vpool.size();
+          const double dr = rcut / 1000;
+          vpool.push_back(-e.fr0());
+          for (int i = 1; i < 1001; ++i) vpool.push_back(-e.fr(i * dr));
+        }
+        it = fits.emplace(key, std::make_tuple(std::move(e), off, vo)).first;
       }
-      const EwaldFit& e = it->second.first;
+      const EwaldFit& e = std::get<0>(it->second);
       std::vector<double> q;
       for (int s2 = 0; s2 < d.nspecies; ++s2)
         for (int i = 0; i < d.species[s2].count; ++i) q.push_back(d.species[s2].charge);
       EwP& p = ew[c];
       p.nk = e.nk();
-      p.off = it->second.second;
+      p.off = std::get<1>(it->second);
       p.self = e.self_energy(q.data(), (int)q.size());
       p.rcut = rcut;
       for (int i = 0; i < kEwaldPoly; ++i) p.coef[i] = e.coef[i];
+      voff[c] = std::get<2>(it->second);
     }
-    if (b->dew_est) (void)hipFree(b->dew_est);
-    if (b->dewk_est) (void)hipFree(b->dewk_est);
+    void* olds[] = {b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est};
+    for (void* o : olds)
+      if (o) (void)hipFree(o);
     b->dew_est = nullptr;
     b->dewk_est = nullptr;
-    b->est_rcut = b->est_kcut = -1;
+    b->dvoff_est = nullptr;
+    b->dvgrid_est = nullptr;
+    b->est_rcut = b->est_kcut = b->est_kappa = -1;
+    b->est_type = -1;
     HIPCHK(hipMalloc(&b->dew_est, sizeof(EwP) * b->nchain));
     HIPCHK(hipMalloc(&b->dewk_est, sizeof(double) * std::max<size_t>(1, pool.size())));
+    HIPCHK(hipMalloc(&b->dvoff_est, sizeof(int64_t) * b->nchain));
+    HIPCHK(hipMalloc(&b->dvgrid_est, sizeof(double) * std::max<size_t>(1, vpool.size())));
     HIPCHK(hipMemcpy(b->dew_est, ew.data(), sizeof(EwP) * b->nchain, hipMemcpyHostToDevice));
     if (pool.size()) HIPCHK(hipMemcpy(b->dewk_est, pool.data(), sizeof(double) * pool.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->dvoff_est, voff.data(), sizeof(int64_t) * b->nchain, hipMemcpyHostToDevice));
+    if (vpool.size()) HIPCHK(hipMemcpy(b->dvgrid_est, vpool.data(), sizeof(double) * vpool.size(), hipMemcpyHostToDevice));
     b->est_rcut = rcut;
     b->est_kcut = kcut;
+    b->est_kappa = kappa;
+    b->est_type = trad ? 1 : 0;
   }
   void* p;
   if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_ewald_coulomb_energy(b->D, x, b->nchain, b->dew_est, b->dewk_est, epsilon, (double*)p, b->stream));
+  HIPCHK(launch_ewald_coulomb_energy(b->D, x, b->nchain, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est,
+                                     epsilon, (double*)p, b->stream));
   HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
+}
+
+extern "C" int pq_measure_ewald_coulomb_energy(pq_batch* b, double epsilon, double rcut, double kcut,
+                                               double* energy) {
+  return measure_ewald_coulomb(b, epsilon, rcut, kcut, false, 0.0, energy);
+}
+
+extern "C" int pq_measure_trad_ewald_coulomb_energy(pq_batch* b, double epsilon, double rcut, double kcut,
+                                                    double kappa, double* energy) {
+  return measure_ewald_coulomb(b, epsilon, rcut, kcut, true, kappa, energy);
 }
 
 extern "C" int pq_measure_virial_energy(pq_batch* b, int nwindow, double* energy) {

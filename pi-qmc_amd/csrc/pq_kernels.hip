@@ -509,6 +509,7 @@ __global__ __launch_bounds__(256) void bead_utau_kernel(DevCtx x, int nq, const 
 // workgroup per chain, ew[chain] its sum.
 template <int D>
 __global__ __launch_bounds__(256) void ewald_coulomb_energy_kernel(DevCtx x, const EwP* ew, const double* kpool,
+                                                                    const int64_t* voff, const double* vtab,
                                                                     double eps, double* energy) {
   __shared__ ChainP c;
   __shared__ double part[4];
@@ -522,7 +523,10 @@ __global__ __launch_bounds__(256) void ewald_coulomb_energy_kernel(DevCtx x, con
   const EwP e = ew[chain];
   const double* beads = x.beads + c.bead_off;
   const double dr = e.rcut / 1000, drinv = 1. / dr;
+  // tradEwald: -f(i dr) tabulated on the host; optEwald: the fitted polynomial
+  const int64_t vo = voff[chain];
   auto vgrid = [&](int ig) -> double {
+    if (vo >= 0) return vtab[vo + ig] / eps;
     if (ig == 0) return -e.coef[0] / eps;
     const double r = ig * dr;
     if (r > e.rcut) return -(1. / r) / eps;
@@ -703,9 +707,12 @@ hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps,
 }
 
 hipError_t launch_ewald_coulomb_energy(int D, const DevCtx& x, int nchain, const EwP* ew, const double* kpool,
-                                       double eps, double* energy, hipStream_t st) {
-  if (D == 2) hipLaunchKernelGGL(ewald_coulomb_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, ew, kpool, eps, energy);
-  else hipLaunchKernelGGL(ewald_coulomb_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, ew, kpool, eps, energy);
+                                       const int64_t* voff, const double* vtab, double eps, double* energy,
+                                       hipStream_t st) {
+  if (D == 2)
+    hipLaunchKernelGGL(ewald_coulomb_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, ew, kpool, voff, vtab, eps, energy);
+  else
+    hipLaunchKernelGGL(ewald_coulomb_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, ew, kpool, voff, vtab, eps, energy);
   return hipGetLastError();
 }
 

@@ -58,7 +58,8 @@ hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hip
 hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps, double* energy, hipStream_t st);
 hipError_t launch_virial_energy(int D, const DevCtx& x, int nchain, int nwindow, double* energy, hipStream_t st);
 hipError_t launch_ewald_coulomb_energy(int D, const DevCtx& x, int nchain, const EwP* ew, const double* kpool,
-                                       double eps, double* energy, hipStream_t st);
+                                       const int64_t* voff, const double* vgrid, double eps, double* energy,
+                                       hipStream_t st);
 hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const int* qi, const int* qt,
                             double* out, hipStream_t st);
 hipError_t launch_histogram(int D, const DevCtx& x, int nchain, int s1, int s2, double rmin, double rmax,

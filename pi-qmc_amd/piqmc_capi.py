@@ -309,6 +309,8 @@ def load_library(path=LIB_PATH):
         "pq_measure_coulomb_energy": (C.c_int, [C.c_void_p, C.c_double, dp]),
         "pq_measure_virial_energy": (C.c_int, [C.c_void_p, C.c_int, dp]),
         "pq_measure_ewald_coulomb_energy": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, dp]),
+        "pq_measure_trad_ewald_coulomb_energy": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                                           C.c_double, dp]),
         "pq_pair_histogram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_int, P(C.c_uint64)]),
         "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
@@ -473,9 +475,12 @@ class Batch:
                                          h.ctypes.data_as(C.POINTER(C.c_uint64))))
         return h
 
-    def ewald_coulomb_energy(self, epsilon, rcut, kcut):
+    def ewald_coulomb_energy(self, epsilon, rcut, kcut, trad=False, kappa=0.0):
         e = np.empty(len(self.descs))
-        check(self.lib.pq_measure_ewald_coulomb_energy(self.h, epsilon, rcut, kcut, dptr(e)))
+        if trad:
+            check(self.lib.pq_measure_trad_ewald_coulomb_energy(self.h, epsilon, rcut, kcut, kappa, dptr(e)))
+        else:
+            check(self.lib.pq_measure_ewald_coulomb_energy(self.h, epsilon, rcut, kcut, dptr(e)))
         return e
 
     def virial_energy(self, nwindow=1):

@@ -65,6 +65,8 @@ def load():
         "oq_thermo_energy": (C.c_double, [vp]),
         "oq_coulomb_energy": (C.c_double, [vp, C.c_double]),
         "oq_ewald_coulomb_energy": (C.c_double, [vp, C.c_double, C.c_double, C.c_double]),
+        "oq_ewald_coulomb_energy_type": (C.c_double, [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double]),
+        "oq_trad_estimator_kappa": (C.c_double, [vp]),
         "oq_ewald_info": (C.c_int, [vp, dp, ip, dp, C.c_int, ip, dp]),
         "oq_ewald_long_range": (C.c_double, [vp, dp]),
         "oq_ewald_fr": (C.c_double, [vp, C.c_double]),
@@ -183,6 +185,12 @@ class Chain:
 
     def ewald_coulomb_energy(self, epsilon, rcut, kcut):
         return self.lib.oq_ewald_coulomb_energy(self.h, epsilon, rcut, kcut)
+
+    def trad_ewald_coulomb_energy(self, epsilon, rcut, kcut, kappa=0.0):
+        return self.lib.oq_ewald_coulomb_energy_type(self.h, epsilon, rcut, kcut, 1, kappa)
+
+    def trad_estimator_kappa(self):
+        return self.lib.oq_trad_estimator_kappa(self.h)
 
     def ewald_info(self):
         """(coef[8], nk, self energy, kv [nk][3], vk [nk]) of the CoulombAction's sum."""

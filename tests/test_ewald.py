@@ -54,6 +54,22 @@ def test_madelung_energies():
     assert abs(e_thermo - MADELUNG) < 1e-4, e_thermo
 
 
+def test_trad_ewald_estimator_madelung():
+    """<CoulombEnergyEstimator useEwald="true" ewaldType="tradEwald">: the
+    TradEwaldSum estimator gives the NaCl Madelung energy when erfc(kappa r)
+    has died out by the nearest neighbour (r = 1, kappa = 5) and the k-space
+    Gaussian by kcut (exp(-kcut^2 / 4 kappa^2) = e^-25); the parser's default
+    kappa is sqrt((3.6 N)^(1/6) sqrt(pi) / Lside)."""
+    ch = ol.Chain(nacl_desc())
+    ch.set_beads(nacl_beads())
+    e = ch.trad_ewald_coulomb_energy(1.0, 0.5, 50.0, 5.0) / 4
+    assert abs(e - MADELUNG) < 1e-8, e
+    assert ch.trad_estimator_kappa() == pytest.approx(math.sqrt((3.6 * 8) ** (1 / 6) * math.sqrt(math.pi) / 2.0),
+                                                      rel=1e-15)
+    # with the default kappa (1.4) the real-space tail beyond rcut is lost
+    assert abs(ch.trad_ewald_coulomb_energy(1.0, 0.5, 50.0) / 4 - MADELUNG) > 1e-3
+
+
 def test_fit_meets_its_constraints():
     for D, L, rc, kc in [(3, 2.0, 0.5, 50.0), (3, 5.0, 2.5, 8.0), (2, 4.0, 2.0, 10.0)]:
         d = capi.make_desc(D, 8, 1, 0.1, [(2, 1.0, 1.0), (2, 1.0, -1.0)], cell=L,

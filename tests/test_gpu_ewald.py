@@ -138,3 +138,40 @@ def test_bad_ewald_inputs_are_refused():
     b = capi.Batch([box_desc()])
     with pytest.raises(capi.PQError):
         b.ewald_coulomb_energy(1.0, 0.0, 5.0)
+
+
+def test_trad_ewald_estimator_on_device():
+    """pq_measure_trad_ewald_coulomb_energy: NaCl's Madelung energy with
+    kappa 5, and the oracle's values on random boxes (3D and 2D, the parser's
+    default kappa and an explicit one, two cells in one batch); switching
+    between the optEwald and tradEwald estimators rebuilds the cached fits."""
+    print("[test] trad ewald estimator", flush=True)
+    d = nacl_desc()
+    b = capi.Batch([d])
+    b.set_paths(0, nacl_beads())
+    oc = ol.Chain(d)
+    oc.set_beads(nacl_beads())
+    et = b.ewald_coulomb_energy(1.0, 0.5, 50.0, trad=True, kappa=5.0)[0]
+    assert rel_close(et, oc.trad_ewald_coulomb_energy(1.0, 0.5, 50.0, 5.0), 1e-12, scale=1.0), et
+    assert abs(et / 4 - MADELUNG) < 1e-8
+    eo = b.ewald_coulomb_energy(1.0, 0.5, 50.0)[0]
+    assert rel_close(eo, oc.ewald_coulomb_energy(1.0, 0.5, 50.0), 1e-12, scale=1.0), eo
+    for D in (3, 2):
+        descs = [box_desc(D=D, L=3.0), box_desc(D=D, L=4.0)]
+        b = capi.Batch(descs)
+        ocs = []
+        for c, dd in enumerate(descs):
+            beads = capi.initial_beads(dd, 60 + c)
+            beads = np.mod(beads, dd.cell[0])
+            b.set_paths(c, beads)
+            o = ol.Chain(dd)
+            o.set_beads(beads)
+            ocs.append(o)
+        for kappa in (0.0, 2.5):
+            for eps in (1.0, 2.0):
+                got = b.ewald_coulomb_energy(eps, 1.4, 6.0, trad=True, kappa=kappa)
+                for c, o in enumerate(ocs):
+                    want = o.trad_ewald_coulomb_energy(eps, 1.4, 6.0, kappa)
+                    assert rel_close(got[c], want, 1e-12, scale=1.0), (D, kappa, eps, c, got[c], want)
+    with pytest.raises(capi.PQError):
+        b.ewald_coulomb_energy(1.0, 1.4, 6.0, trad=True, kappa=-1.0)
