@@ -244,7 +244,7 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
         // The general instances (MM > 1) also run the movers' sigma factors
         // and delayed rejection (MultiLevelSampler.cc:114-143,
         // DoubleMLSampler.cc:150-206): the beads B rejected at level L-1 are
-        // kept and the move goes round once more, C at level L-1 with the
+        // kept and the move is tried once more, C at level L-1 with the
         // second sigma factor, its draws after B's accept draw, and the
         // levels below proposed again after C's accept draw.
         double old = lnTP;
