@@ -246,7 +246,9 @@ int pq_read_all_paths(pq_batch* batch, const char* filename, int bfactor);
 
 /* Uniform stream of every chain: Philox4x32-10 keyed by seed, stream id =
    first_stream + chain, or caller-supplied uniforms (per_chain values per
-   chain, chain-major).  Replaces RandomNumGenerator (src/util/RandomNumGenerator.h:26-86). */
+   chain, chain-major).  Replaces RandomNumGenerator (src/util/RandomNumGenerator.h:26-86).
+   A chain that draws past the end of its supplied stream makes pq_sync,
+   pq_get_stats and pq_get_all_stats return PQ_ERR_STATE. */
 int pq_seed(pq_batch* batch, uint64_t seed, uint32_t first_stream);
 int pq_set_uniforms(pq_batch* batch, const double* u, int64_t per_chain);
 

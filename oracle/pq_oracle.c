@@ -128,6 +128,8 @@ static double rng_next(oq_rng* g) {
   } else if (g->mode == 1) {
     u = oq_philox_uniform(g->seed, g->stream, g->count);
   } else {
+    /* past the end of an injected stream: a placeholder draw, and the
+       stream reports itself exhausted (oq_rng_exhausted) */
     u = (g->count < (uint64_t)g->ninj) ? g->inj[g->count] : 0.5;
   }
   g->count++;
@@ -1082,6 +1084,10 @@ void oq_rng_inject(oq_chain* ch, const double* u, int64_t n) {
 }
 double oq_rng_next(oq_chain* ch) { return rng_next(&ch->rng); }
 uint64_t oq_rng_count(const oq_chain* ch) { return ch->rng.count; }
+
+int oq_rng_exhausted(const oq_chain* ch) {
+  return ch->rng.mode == 2 && ch->rng.count > (uint64_t)ch->rng.ninj;
+}
 
 /* ---------------------------------------------------------- actions */
 
@@ -2412,9 +2418,12 @@ double oq_ewald_coulomb_energy_type(const oq_chain* ch, double epsilon, double r
   ewald_init(&e, &ch->cell, N, q, rcut, kcut, trad, kappa);
   const int nradial = 1001;
   const double dr = rcut / 1000, drinv = 1. / dr;
-  double* vgrid = (double*)malloc(sizeof(double) * nradial);
+  /* one entry past the reference's 1001: r just below rcut can round to
+     igrid = 1000, whose weight on vgrid(1001) is ~0 (the reference reads one
+     past its array there); -f(1001 dr)/eps keeps that read in bounds */
+  double* vgrid = (double*)malloc(sizeof(double) * (nradial + 1));
   vgrid[0] = -(trad ? 2 * kappa / sqrt(EW_PI) : e.coef[0]) / epsilon;   /* evalFR0 */
-  for (int i = 1; i < nradial; ++i) vgrid[i] = -ew_fr(&e, i * dr) / epsilon;
+  for (int i = 1; i <= nradial; ++i) vgrid[i] = -ew_fr(&e, i * dr) / epsilon;
   double value = 0;
   for (int t = 0; t < ch->M; ++t)
     for (int i = 0; i < N; ++i) {
@@ -2465,29 +2474,42 @@ double oq_ewald_fr(const oq_chain* ch, double r) {
   return ch->has_ewald ? ew_fr(&ch->ew, r) : 0.0;
 }
 
-/* PairCFEstimator<1>::handleLink with PairRadial, two species
-   (src/estimator/PairCFEstimator.h:72-93, src/util/PairDistance.h:22-39) */
-void oq_pair_histogram(const oq_chain* ch, int s1, int s2, double rmin,
-                       double rmax, int nbin, uint64_t* counts) {
+/* PairCFEstimator<1>::handleLink with PairRadial (src/estimator/PairCFEstimator.h:72-93,
+   src/util/PairDistance.h:22-39): for i of species[0] and each partner species
+   species[1..nspec-1] in the order listed, every j != i of that species on the
+   same slice: ibin = int((|pbc(r_i - r_j)| - rmin) * nbin/(rmax - rmin)),
+   dropped when d < rmin or ibin > nbin - 1; integer counts summed over slices
+   (the reference's float temp, exact below 2^24, before its /nslice) */
+void oq_pair_cf_histogram(const oq_chain* ch, int nspec, const int* species, double rmin,
+                          double rmax, int nbin, uint64_t* counts) {
   const int D = ch->D;
   double deltaInv = nbin / (rmax - rmin);
-  int if1 = ch->ifirst[s1], n1 = ch->d.species[s1].count;
-  int if2 = ch->ifirst[s2], n2 = ch->d.species[s2].count;
+  if (nspec < 1) return;
+  int if1 = ch->ifirst[species[0]], n1 = ch->d.species[species[0]].count;
   for (int t = 0; t < ch->M; ++t)
     for (int i = if1; i < if1 + n1; ++i)
-      for (int j = if2; j < if2 + n2; ++j) {
-        if (i == j) continue;
-        const double *a = bead(ch, i, t), *b = bead(ch, j, t);
-        double delta[3];
-        for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
-        pbc(&ch->cell, delta);
-        double r2 = 0;
-        for (int d = 0; d < D; ++d) r2 += delta[d] * delta[d] * 1.0;
-        double dd = sqrt(r2);
-        int ibin = (int)((dd - rmin) * deltaInv);
-        if (dd < rmin || ibin > nbin - 1) continue;
-        counts[ibin]++;
+      for (int ks = 1; ks < nspec; ++ks) {
+        const int if2 = ch->ifirst[species[ks]], n2 = ch->d.species[species[ks]].count;
+        for (int j = if2; j < if2 + n2; ++j) {
+          if (i == j) continue;
+          const double *a = bead(ch, i, t), *b = bead(ch, j, t);
+          double delta[3];
+          for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
+          pbc(&ch->cell, delta);
+          double r2 = 0;
+          for (int d = 0; d < D; ++d) r2 += delta[d] * delta[d] * 1.0;
+          double dd = sqrt(r2);
+          int ibin = (int)((dd - rmin) * deltaInv);
+          if (dd < rmin || ibin > nbin - 1) continue;
+          counts[ibin]++;
+        }
       }
+}
+
+void oq_pair_histogram(const oq_chain* ch, int s1, int s2, double rmin,
+                       double rmax, int nbin, uint64_t* counts) {
+  const int sp[2] = {s1, s2};
+  oq_pair_cf_histogram(ch, 2, sp, rmin, rmax, nbin, counts);
 }
 
 uint64_t oq_run_batch(const pq_chain_desc* descs, int nchain, double* beads,

@@ -51,6 +51,8 @@ void oq_rng_philox(oq_chain* ch, uint64_t seed, uint32_t stream);
 void oq_rng_inject(oq_chain* ch, const double* u, int64_t n);
 double oq_rng_next(oq_chain* ch);
 uint64_t oq_rng_count(const oq_chain* ch);
+/* 1 when an injected stream was read past its end (those draws were 0.5) */
+int oq_rng_exhausted(const oq_chain* ch);
 
 /* action layer on explicit sections.  mask bit0 spring, bit1 SHO, bit2
    CoulombAction (pair tables and the Ewald term), bit3 the Ewald term alone */
@@ -96,6 +98,9 @@ double oq_virial_energy(const oq_chain* ch, int nwindow);
 void oq_energy_blocks(oq_chain* ch, int nblock, int nmeasure, double* block);
 void oq_pair_histogram(const oq_chain* ch, int s1, int s2, double rmin,
                        double rmax, int nbin, uint64_t* counts);
+/* species[0] against the partner species species[1..nspec-1] */
+void oq_pair_cf_histogram(const oq_chain* ch, int nspec, const int* species, double rmin,
+                          double rmax, int nbin, uint64_t* counts);
 
 /* many chains on nthreads host threads (OpenMP), Philox streams
    first_stream+c; beads concatenated per chain; returns bead updates */

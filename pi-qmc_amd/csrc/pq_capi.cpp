@@ -319,7 +319,13 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
   if (d.ndim != 2 && d.ndim != 3) return fail(PQ_ERR_ARG, "ndim must be 2 or 3 (got %d)", d.ndim);
   if (d.nspecies < 1 || d.nspecies > PQ_MAX_SPECIES) return fail(PQ_ERR_ARG, "nspecies out of range");
   if (d.nlevel < 0 || d.nlevel > 7) return fail(PQ_ERR_ARG, "nlevel must be in 0..7");
-  if (d.tau <= 0) return fail(PQ_ERR_ARG, "tau must be positive");
+  if (!(d.tau > 0) || !std::isfinite(d.tau)) return fail(PQ_ERR_ARG, "tau must be positive and finite");
+  // a chain needs at least one slice (a zero-length path has no bead to
+  // start a section from) and a finite, positive SuperCell side per dimension
+  if (d.nslice < 1) return fail(PQ_ERR_ARG, "nslice must be at least 1 (got %d)", d.nslice);
+  for (int k = 0; k < d.ndim; ++k)
+    if (!(d.cell[k] > 0) || !std::isfinite(d.cell[k]))
+      return fail(PQ_ERR_ARG, "cell side %d must be positive and finite (got %g)", k, d.cell[k]);
   c.D = d.ndim; c.M = d.nslice; c.L = d.nlevel; c.S = (1 << d.nlevel) + 1; c.nsp = d.nspecies;
   c.tau = d.tau;
   if (d.ngroup < 0 || d.ngroup > PQ_MAX_GROUPS - 1) return fail(PQ_ERR_ARG, "ngroup must be in 0..%d", PQ_MAX_GROUPS - 1);
@@ -677,10 +683,6 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
                                pq_batch** out) {
   if (!chains || nchain < 1 || !out) return fail(PQ_ERR_ARG, "pq_batch_create: bad arguments");
   if (precision != PQ_PREC_FP64 && precision != PQ_PREC_MIXED) return fail(PQ_ERR_ARG, "unknown precision");
-  int ndev = 0;
-  HIPCHK(hipGetDeviceCount(&ndev));
-  if (device < 0 || device >= ndev) return fail(PQ_ERR_ARG, "device %d not present (%d devices)", device, ndev);
-  HIPCHK(hipSetDevice(device));
   pq_batch* b = new pq_batch;
   b->device = device;
   b->precision = precision;
@@ -696,6 +698,15 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     int rc = build_chain(b, chains[c], b->hc[c], tabs);
     if (rc) { delete b; return rc; }
     ns_max = std::max(ns_max, b->hc[c].ns_max);
+  }
+  // the descriptors are checked before any device call, so a bad argument
+  // is PQ_ERR_ARG on any host
+  {
+    int ndev = 0;
+    const hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess) { delete b; return fail(PQ_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e)); }
+    if (device < 0 || device >= ndev) { delete b; return fail(PQ_ERR_ARG, "device %d not present (%d devices)", device, ndev); }
+    if (hipSetDevice(device) != hipSuccess) { delete b; return fail(PQ_ERR_HIP, "hipSetDevice(%d) failed", device); }
   }
   for (int c = 0; c < nchain; ++c)
     for (int g = 0; g < b->hc[c].ngroup; ++g)
@@ -1334,11 +1345,25 @@ extern "C" int pq_run_sweeps_timed(pq_batch* b, int nsweep, float* ms) {
   return PQ_OK;
 }
 
+// After the batch stream has drained: with caller-supplied uniforms, a chain
+// that drew past the end of its stream read placeholder values (0.5), so its
+// decisions no longer follow the stream; that is an error, not a result.
+static int check_stream(pq_batch* b) {
+  if (b->rng_mode != PQ_RNG_INJECTED) return PQ_OK;
+  std::vector<uint64_t> cnt(b->nchain);
+  HIPCHK(hipMemcpy(cnt.data(), b->dcounter, sizeof(uint64_t) * b->nchain, hipMemcpyDeviceToHost));
+  for (int c = 0; c < b->nchain; ++c)
+    if (cnt[c] > (uint64_t)b->inj_per_chain)
+      return fail(PQ_ERR_STATE, "chain %d drew %llu uniforms from an injected stream of %lld", c,
+                  (unsigned long long)cnt[c], (long long)b->inj_per_chain);
+  return PQ_OK;
+}
+
 extern "C" int pq_sync(pq_batch* b) {
   if (!b) return fail(PQ_ERR_ARG, "null batch");
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
-  return PQ_OK;
+  return check_stream(b);
 }
 
 extern "C" int pq_get_stats(pq_batch* b, int c, pq_chain_stats* out) {
@@ -1347,7 +1372,7 @@ extern "C" int pq_get_stats(pq_batch* b, int c, pq_chain_stats* out) {
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   HIPCHK(hipMemcpy(out, b->dstats + c, sizeof(pq_chain_stats), hipMemcpyDeviceToHost));
-  return PQ_OK;
+  return check_stream(b);
 }
 
 extern "C" int pq_get_all_stats(pq_batch* b, pq_chain_stats* out) {
@@ -1355,7 +1380,7 @@ extern "C" int pq_get_all_stats(pq_batch* b, pq_chain_stats* out) {
   HIPCHK(hipSetDevice(b->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   HIPCHK(hipMemcpy(out, b->dstats, sizeof(pq_chain_stats) * b->nchain, hipMemcpyDeviceToHost));
-  return PQ_OK;
+  return check_stream(b);
 }
 
 extern "C" int pq_reset_stats(pq_batch* b) {
@@ -1450,7 +1475,9 @@ static int measure_ewald_coulomb(pq_batch* b, double epsilon, double rcut, doubl
           vo = (int64_t)vpool.size();
           const double dr = rcut / 1000;
           vpool.push_back(-e.fr0());
-          for (int i = 1; i < 1001; ++i) vpool.push_back(-e.fr(i * dr));
+          // 1001 grid points and one more: r just below rcut can round to
+          // ig = 1000, which reads point 1001 with a weight of ~0
+          for (int i = 1; i <= 1001; ++i) vpool.push_back(-e.fr(i * dr));
         }
         it = fits.emplace(key, std::make_tuple(std::move(e), off, vo)).first;
       }

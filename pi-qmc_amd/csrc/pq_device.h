@@ -105,6 +105,8 @@ __device__ __forceinline__ uint32_t philox_word(uint64_t seed, uint32_t stream, 
 __device__ __forceinline__ double uniform(const DevCtx& x, int chain, uint64_t k) {
   if (x.rng_mode == PQ_RNG_PHILOX)
     return (double)philox_word(x.seed, x.first_stream + (uint32_t)chain, k) * 2.3283064365386963e-10;
+  // past the end of an injected stream: a placeholder; the host reports the
+  // overrun as an error once the stream has drained (pq_sync, pq_get_stats)
   return (k < (uint64_t)x.inj_per_chain) ? x.inj[(int64_t)chain * x.inj_per_chain + (int64_t)k] : 0.5;
 }
 

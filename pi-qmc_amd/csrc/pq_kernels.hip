@@ -503,7 +503,8 @@ __global__ __launch_bounds__(256) void bead_utau_kernel(DevCtx x, int nq, const 
 
 // EwaldCoulombEstimator with its own OptEwaldSum (EwaldCoulombEstimator.cc:
 // 63-140): per link q_i q_j (1/(r eps) + (1-x) v(ig) + x v(ig+1)) over pairs
-// j < i inside rcut, v(i) = -f(i dr)/eps (v(0) = -f(0)/eps) on 1001 points,
+// j < i inside rcut, v(i) = -f(i dr)/eps (v(0) = -f(0)/eps) on 1001 points
+// (one more stored: r just below rcut can round to ig = 1000),
 // dr = rcut/1000, ig = (int)(r/dr) and x = r - ig dr as the reference weighs
 // it, and the k-space sum on particle 0 of every slice; divided by M.  One
 // workgroup per chain, ew[chain] its sum.

@@ -51,6 +51,7 @@ def load():
         "oq_rng_inject": (None, [vp, dp, C.c_int64]),
         "oq_rng_next": (C.c_double, [vp]),
         "oq_rng_count": (C.c_uint64, [vp]),
+        "oq_rng_exhausted": (C.c_int, [vp]),
         "oq_action_difference": (C.c_double, [vp, dp, dp, ip, C.c_int, C.c_int, C.c_int]),
         "oq_bead_utau": (C.c_double, [vp, C.c_int, C.c_int, C.c_int]),
         "oq_node_evaluate": (C.c_int, [vp, C.c_int, dp, dp, C.c_int, C.c_int, dp]),
@@ -76,6 +77,8 @@ def load():
         "oq_energy_blocks": (None, [vp, C.c_int, C.c_int, dp]),
         "oq_pair_histogram": (None, [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                      P(C.c_uint64)]),
+        "oq_pair_cf_histogram": (None, [vp, C.c_int, ip, C.c_double, C.c_double, C.c_int,
+                                        P(C.c_uint64)]),
         "oq_run_batch": (C.c_uint64, [P(capi.ChainDesc), C.c_int, dp, C.c_int, C.c_uint64,
                                       C.c_uint32, C.c_int, dp]),
     }
@@ -169,8 +172,13 @@ class Chain:
     def bead_utau(self, i, t, mask=7):
         return self.lib.oq_bead_utau(self.h, i, t, mask)
 
+    def _check_stream(self):
+        if self.lib.oq_rng_exhausted(self.h):
+            raise RuntimeError(f"injected uniform stream exhausted after {self.lib.oq_rng_count(self.h)} draws")
+
     def run_sweeps(self, n):
         self.lib.oq_run_sweeps(self.h, n)
+        self._check_stream()
 
     def stats(self):
         st = capi.ChainStats()
@@ -223,6 +231,7 @@ class Chain:
     def energy_blocks(self, nblock, nmeasure):
         out = np.zeros(nblock)
         self.lib.oq_energy_blocks(self.h, nblock, nmeasure, dptr(out))
+        self._check_stream()
         return out
 
     def pair_table(self, ip):
@@ -266,10 +275,20 @@ class Chain:
         sec = np.ascontiguousarray(sec, dtype=np.float64)
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         assert mov.flags.c_contiguous and mov.dtype == np.float64
-        return self.lib.oq_mover(self.h, dptr(sec), dptr(mov), iptr(idx), len(idx), level)
+        v = self.lib.oq_mover(self.h, dptr(sec), dptr(mov), iptr(idx), len(idx), level)
+        self._check_stream()
+        return v
 
     def pair_histogram(self, s1, s2, rmin, rmax, nbin):
         h = np.zeros(nbin, dtype=np.uint64)
         self.lib.oq_pair_histogram(self.h, s1, s2, rmin, rmax, nbin,
                                    h.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return h
+
+    def pair_cf_histogram(self, species, rmin, rmax, nbin):
+        """species[0] against the partner list species[1:]."""
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        h = np.zeros(nbin, dtype=np.uint64)
+        self.lib.oq_pair_cf_histogram(self.h, len(sp), iptr(sp), rmin, rmax, nbin,
+                                      h.ctypes.data_as(C.POINTER(C.c_uint64)))
         return h

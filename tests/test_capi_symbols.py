@@ -64,3 +64,31 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
         py = structs[cname]
         got = C.sizeof(py) if field == "size" else getattr(py, field).offset
         assert got == int(value), (cname, field, got, value)
+
+
+def _create_rc(desc):
+    lib = capi.load_library()
+    h = C.c_void_p()
+    arr = capi.descs_array([desc])
+    return lib.pq_batch_create(arr, 1, capi.PQ_PREC_FP64, 0, C.byref(h)), h
+
+
+def test_batch_create_rejects_empty_paths_and_bad_cells():
+    """Descriptors are checked before any device call: a chain of zero
+    slices, or a cell side that is zero, negative or not finite, is
+    PQ_ERR_ARG on any host (no kernel runs)."""
+    bad = []
+    d = capi.make_desc(2, 8, 0, 0.1, [(1, 1.0, 0.0)], nsection=1)
+    d.nslice = 0
+    bad.append(d)
+    for side in (0.0, -1.0, float("nan"), float("inf")):
+        d = capi.make_desc(3, 16, 2, 0.1, [(2, 1.0, -1.0)], cell=4.0)
+        d.cell[1] = side
+        bad.append(d)
+    d = capi.make_desc(2, 16, 2, 0.1, [(2, 1.0, -1.0)])
+    d.tau = float("nan")
+    bad.append(d)
+    for d in bad:
+        rc, h = _create_rc(d)
+        assert rc == capi.PQ_ERR_ARG, (rc, capi.load_library().pq_last_error())
+        assert not h.value

@@ -261,9 +261,10 @@ def test_action_difference_antisymmetric():
 
 @pytest.mark.parametrize("model", [capi.PQ_NODES_SHO, capi.PQ_NODES_FREE])
 def test_node_det_and_distance_pinned(model):
-    """det against numpy.linalg of the same matrix (row/column swap antisymmetry),
-    and the nodal distance against a finite-difference gradient of log|det|:
-    d_j = sqrt(2m/((|grad_j log|det||^2 + 1e-15) tau))."""
+    """Row-swap antisymmetry of det, and the nodal distance against a
+    finite-difference gradient of log|det|: d_j = sqrt(2m/((|grad_j
+    log|det||^2 + 1e-15) tau)).  The determinant values themselves are checked
+    against numpy.linalg.det of the closed-form matrix in test_oracle_pins.py."""
     d = capi.make_desc(2, 64, 4, 0.02, [(3, 1.0, -1.0)], cell=6.0, sho=[(0, 1.0)],
                        nodes=[(0, model, 1.0)])
     ch = ol.Chain(d)
