@@ -253,3 +253,35 @@ def test_free_mover_pbc_torus_windings_and_energy():
     e = ch.energy_blocks(20, 4000)
     eerr = e.std(ddof=1) / math.sqrt(len(e))
     assert abs(e.mean() - e_exact) < max(4 * eerr, 0.35), (e.mean(), eerr, e_exact)
+
+
+def test_sho_node_matrix_rank_one_at_low_temperature():
+    """Why the device parity tests of the 3D and 2D batch chains set a node
+    temperature: at the simulation temperature of a chain with M = 2048, tau
+    = 0.05 (beta/2 = 51.2), 2c r.r' ~ 1e-22 is below FP64 resolution next to
+    c cosh ~ 0.5, so the closed-form SHO matrix is rank one to working
+    precision and its determinant is rounding noise -- in numpy as in the
+    oracle (and the reference).  At temperature 0.5 the same positions give a
+    resolved determinant that numpy and the oracle agree on."""
+    d = capi.cfg5_descs(4096)[114]                 # N_s 20 + 20, M 2048
+    assert d.nslice == 2048 and d.species[0].count == 20
+    ns = 20
+    rng = np.random.default_rng(1)
+    r1 = rng.standard_normal((d.npart, 3))
+    r2 = r1 + 0.3 * rng.standard_normal((d.npart, 3))
+    for T, resolved in ((1.0 / (d.tau * d.nslice), False), (0.5, True)):
+        c = 0.5 / math.sinh(0.5 / T)
+        mat, _ = _sho_matrix(r1[:ns], r2[:ns], c, math.cosh(0.5 / T))
+        sv = np.linalg.svd(mat, compute_uv=False)
+        e = capi.ChainDesc()
+        capi.C.pointer(e)[0] = d
+        e.nodes[0].temperature = T
+        det, err = ol.Chain(e).node_evaluate(0, r1, r2, 0)
+        assert err == 0
+        if resolved:
+            assert sv[-1] / sv[0] > 1e-8
+            assert det == pytest.approx(np.linalg.det(mat), rel=1e-9)
+        else:
+            assert sv[1] / sv[0] < 1e-14
+            # |det| below eps^(N_s - 1) of the diagonal product: no digit is significant
+            assert abs(det) < np.prod(np.abs(np.diag(mat))) * 1e-15 ** (ns - 1) * 1e30
