@@ -1,20 +1,27 @@
 #!/usr/bin/env python3
 """Bead-updates/s of the bisection-move hot path on one MI355X.
 
-A step is one sweep of every chain of the batch: one fused sweep_kernel
-launch (choose section, node caches, every Sample's levels L..0, write back).
-Default workload: 1024 independent 2D quantum-dot chains, N ~ U{2..20} split
-between two spin species, M in {128,256,512,1024}, omega ~ U[0.5,2], SHO nodes
-on even chains and free-particle nodes on odd chains, 4-level bisection, run
-in mixed precision.  Beside it, on the same uniforms: the FP64 run of the same
-batch (energy error and FP64 batched rate) and the FP64 single-chain rate of
-the 3+3-electron SHONodes dot (M=400).  The CPU baseline is the oracle, the
-FP64 C restatement, on a bounded sample of the same chains on all host cores.
+A step is one sweep of every chain of the batch followed by one Measure: the
+sweep is a captured HIP graph of the fused sweep_kernel launch(es) (choose
+section, node caches, every Sample's levels L..0, write back), timed by HIP
+events around its launch on the batch stream; the Measure is the thermal
+energy of every chain and the 256-bin pair-correlation histogram of the
+first species against both species, accumulated on the device.
+
+Default workload: 4096 independent 3D quantum-dot chains, N ~ U{2..64} in two
+species, M = 16 U{16..128}, SHONodes on both species (node matrices up to
+32 x 32), 4-level bisection, mixed precision.  Beside it: the same batch in
+FP64 on the same uniforms (FP64 batched rate, paired energies); the
+1024-chain uneven 2D batch as a second line; a paired mixed/FP64 energy study
+over 1000 measured sweeps of the 2D batch; the FP64 single-chain rate of the
+3+3-electron SHONodes dot; and the oracle, the FP64 C restatement, on a
+bounded sample of the headline chains on all host cores and on one core.
 Prints one JSON line.
 """
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -29,11 +36,19 @@ import piqmc_capi as capi  # noqa: E402
 
 METRIC = "bead-updates/sec on 1 MI355X, mixed-batched vs FP64 at fixed energy error"
 HBM_PEAK_GBS = 8000.0
-FP64_VECTOR_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6
+TUNE_AFTER = 20          # sweeps from the replicated start before the autotuner times anything
+STEADY = {"cfg5": 120, "cfg3": 220}   # sweeps before the timed steps (tuning sweeps included)
+PAIR_CF = ([0, 0, 1], 0.0, 8.0, 256)  # species 0 against species 0 and 1, r in [0, 8), 256 bins
+
+
+def log(msg):
+    """Progress on stderr (stdout carries the one JSON line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def measured_traffic(workload_name, precision):
-    """HBM bytes per step (every sweep_kernel launch of one sweep) from the
+    """HBM bytes per sweep (every sweep_kernel launch of one step) from the
     committed rocprofv3 FETCH_SIZE and WRITE_SIZE summary of this workload
     (copy-calibrated), or None."""
     path = os.path.join(ROOT, "profiles", f"sweep_kernel_{workload_name}_{precision}.md")
@@ -51,7 +66,7 @@ def workload(name, nchain):
     if name == "cfg3":
         return capi.cfg3_descs(nchain), "1024 uneven 2D dots (N 2-20, M 128-1024, SHO/free nodes, L=4)"
     if name == "cfg5":
-        return capi.cfg5_descs(nchain), "4096 uneven 3D dots (N 2-64, M 256-2048, SHO nodes, L=4)"
+        return capi.cfg5_descs(nchain), "4096 uneven 3D dots (N 2-64 in two species, M 256-2048, SHO nodes, L=4)"
     if name == "cfg2":
         return [capi.cfg2_desc()] * nchain, "3+3 e- 2D dot, SHONodes, M=400, L=4"
     if name == "cfg1":
@@ -93,72 +108,12 @@ def make_batch(descs, precision, seed):
 
 def all_stats(b):
     st = (capi.ChainStats * len(b.descs))()
-    b.lib.pq_get_all_stats.restype = C.c_int
-    b.lib.pq_get_all_stats.argtypes = [C.c_void_p, C.POINTER(capi.ChainStats)]
     capi.check(b.lib.pq_get_all_stats(b.h, st))
     return st
 
 
-def timed_sweeps(b, steps):
-    """steps launches bracketed by HIP events on the batch stream; returns the
-    per-launch kernel times (ms) and the wall time of the whole timed region."""
-    lib = b.lib
-    lib.pq_run_sweeps_timed.restype = C.c_int
-    lib.pq_run_sweeps_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
-    ms = []
-    b.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        v = C.c_float()
-        capi.check(lib.pq_run_sweeps_timed(b.h, 1, C.byref(v)))
-        ms.append(v.value)
-    b.sync()
-    return ms, time.perf_counter() - t0
-
-
-TUNE_AFTER = 20   # sweeps from the replicated start before the autotuner times anything
-
-
-def run_gpu(descs, precision, steps, warmup, seed, autotune=True, total_warmup=None, schedule=None):
-    """warmup sweeps (the autotuner's timing sweeps count among them, and
-    total_warmup, when given, pads the warmup to that many sweeps), then steps
-    timed sweeps.  The autotuner runs after TUNE_AFTER sweeps, when the chains
-    have left the replicated start and the node records are warm.
-    schedule = (team, inverse, tables, node state[, heavy chains, heavy
-    team]) fixes the schedule instead of autotuning."""
-    b = make_batch(descs, precision, seed)
-    tuned, done = None, 0
-    if schedule is not None:
-        b.set_schedule(schedule[0], schedule[1])
-        b.set_staging(schedule[2], schedule[3])
-        if len(schedule) > 4:
-            b.set_split(schedule[4], schedule[5])
-    elif autotune:
-        b.run_sweeps(TUNE_AFTER)
-        tuned, done = b.autotune(2)
-        done += TUNE_AFTER
-    extra = warmup if total_warmup is None else max(0, total_warmup - done)
-    b.run_sweeps(extra)
-    b.sync()
-    done += extra
-    capi.check(b.lib.pq_reset_stats(b.h))
-    ms, wall = timed_sweeps(b, steps)
-    st = all_stats(b)
-    bu = np.array([s.bead_updates for s in st], dtype=np.float64)
-    abu = np.array([s.accepted_bead_updates for s in st], dtype=np.float64)
-    energy = b.energy()
-    team, inv, lds = b.schedule()
-    tab, fst = b.staging()
-    heavy, heavy_team = b.split()
-    sched = {"team": team, "inverse_lds": inv, "tables_lds": tab, "node_state_lds": fst,
-             "lds_bytes": lds, "heavy_chains": heavy, "heavy_team": heavy_team, "autotune_ms": tuned}
-    return b, dict(ms=ms, wall=wall, bu=bu, abu=abu, energy=energy, schedule=sched, warmup_done=done)
-
-
 def all_paths(b):
     arr = np.empty(int(sum(d.nbeads() for d in b.descs)), dtype=np.float64)
-    b.lib.pq_get_all_paths.restype = C.c_int
-    b.lib.pq_get_all_paths.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int64]
     capi.check(b.lib.pq_get_all_paths(b.h, capi.dptr(arr), arr.size))
     return arr
 
@@ -174,67 +129,161 @@ def identical_chains(bm, br):
     return np.array(same)
 
 
-def paired_energy(bm, br, em, er):
-    """Mixed vs FP64 on the same seeds and sweeps.  Both precisions move beads
-    in FP64, so a chain whose accept/reject decisions never differed ends on
-    bitwise the same path; its energy difference is the estimator's own.  The
-    paired difference over all chains (diverged chains included) gives the
-    energy error with its standard error."""
-    same = identical_chains(bm, br)
-    de = np.asarray(em) - np.asarray(er)
-    n = len(de)
-    return {"chains": n, "identical_decision_chains": int(same.sum()),
-            "max_abs_dE_identical": float(np.max(np.abs(de[same]))) if same.any() else None,
-            "mean_dE": float(de.mean()), "stderr_dE": float(de.std(ddof=1) / np.sqrt(n)) if n > 1 else None,
-            "mean_E_fp64": float(np.mean(er)),
-            "note": "per-chain thermodynamic energy of the final paths, same Philox streams and sweeps"}
+def run_gpu(descs, precision, steps, warmup, seed, steady, autotune=True, schedule=None, pair_cf=None):
+    """Sweeps from the replicated start up to `steady` (the autotuner's timing
+    sweeps among them; it runs after TUNE_AFTER sweeps, when the chains have
+    left the replicated start and the node records are warm), then `warmup`
+    untimed steps and `steps` timed steps, each step one graph-launched sweep
+    plus the Measure.  schedule = (team, inverse, tables, node state[, heavy
+    chains, heavy team]) fixes the schedule instead of autotuning."""
+    b = make_batch(descs, precision, seed)
+    if pair_cf:
+        b.set_measure_pair_cf(*pair_cf)
+    tuned, done = None, 0
+    log(f"{len(descs)} chains, precision {precision}: batch ready")
+    if schedule is not None:
+        b.set_schedule(schedule[0], schedule[1])
+        b.set_staging(schedule[2], schedule[3])
+        if len(schedule) > 4:
+            b.set_split(schedule[4], schedule[5])
+    elif autotune:
+        b.run_sweeps(TUNE_AFTER)
+        tuned, done = b.autotune(1)
+        done += TUNE_AFTER
+    extra = max(0, steady - done)
+    b.run_sweeps(extra)
+    b.sync()
+    done += extra
+    log(f"{done} sweeps done (autotuned: {tuned})")
+    if warmup > 0:
+        b.run_measured_timed(warmup, 1)
+        done += warmup
+    if pair_cf:
+        b.collect_pair_cf()
+    b.sync()
+    capi.check(b.lib.pq_reset_stats(b.h))
+    t0 = time.perf_counter()
+    energies, sweep_ms = b.run_measured_timed(steps, 1)
+    wall = time.perf_counter() - t0
+    counts = b.collect_pair_cf() if pair_cf else None
+    log(f"{steps} timed steps in {wall:.2f} s")
+    st = all_stats(b)
+    bu = np.array([s.bead_updates for s in st], dtype=np.float64)
+    abu = np.array([s.accepted_bead_updates for s in st], dtype=np.float64)
+    team, inv, lds = b.schedule()
+    tab, fst = b.staging()
+    heavy, heavy_team = b.split()
+    sched = {"team": team, "inverse_lds": inv, "tables_lds": tab, "node_state_lds": fst,
+             "lds_bytes": lds, "heavy_chains": heavy, "heavy_team": heavy_team, "autotune_ms": tuned}
+    return b, dict(wall=wall, sweep_ms=sweep_ms, bu=bu, abu=abu, energies=energies, counts=counts,
+                   schedule=sched, warmup_done=done)
 
 
-def paired_observables(bm, br):
-    """The other estimators of the mixed run against the FP64 run on the same
-    streams: virial (nwindow 8) and Coulomb energies per chain (largest
-    difference on the chains with identical decisions, mean difference with
-    its standard error over all chains) and the integer g(r) counts of the
-    two species (bit-identical on identical chains; total counts otherwise)."""
+def roofline(descs, run, steps):
+    kernel_ms = float(np.mean(run["sweep_ms"]))
+    bytes_launch = float(sum(bu * bytes_per_update(d) for bu, d in zip(run["bu"], descs))) / steps
+    flops_launch = float(sum(bu * flops_per_update(d) for bu, d in zip(run["bu"], descs))) / steps
+    achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    return kernel_ms, {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": bytes_launch,
+                       "kernel": "sweep_kernel", "kernel_ms": kernel_ms,
+                       "fp64_equiv_tflops": flops_launch / (kernel_ms * 1e-3) / 1e12,
+                       "fp64_frac": flops_launch / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+
+
+def paired_energy(bm, br, em, er, label):
+    """Mixed vs FP64 on the same seeds and sweeps: per-chain energies averaged
+    over the measured sweeps, their paired difference over all chains (mean
+    and standard error), and the chains whose paths stayed bitwise identical
+    (both precisions move beads in FP64 and estimate in FP64, so a chain whose
+    decisions never differed has dE = 0 exactly)."""
     same = identical_chains(bm, br)
-    out = {}
-    for name, fm, fr in (("virial_energy", bm.virial_energy(8), br.virial_energy(8)),
-                         ("coulomb_energy", bm.coulomb_energy(1.0), br.coulomb_energy(1.0))):
-        de = fm - fr
-        out[name] = {"max_abs_d_identical": float(np.max(np.abs(de[same]))) if same.any() else None,
-                     "mean_d": float(de.mean()), "stderr_d": float(de.std(ddof=1) / np.sqrt(len(de))),
-                     "mean_fp64": float(fr.mean())}
-    hm, hr = bm.pair_histogram(0, 1, 0.0, 8.0, 256), br.pair_histogram(0, 1, 0.0, 8.0, 256)
-    out["pair_histogram_0_1"] = {"bins": 256, "rmax": 8.0,
-                                 "identical_chains_bit_exact": bool(np.array_equal(hm[same], hr[same])),
-                                 "total_counts_mixed": int(hm.sum()), "total_counts_fp64": int(hr.sum())}
+    em, er = np.asarray(em), np.asarray(er)
+    de = em.mean(axis=0) - er.mean(axis=0)
+    n = de.size
+    err = float(de.std(ddof=1) / math.sqrt(n)) if n > 1 else None
+    out = {"measured_sweeps": int(em.shape[0]), "chains": n, "identical_decision_chains": int(same.sum()),
+           "max_abs_dE_identical": float(np.max(np.abs(de[same]))) if same.any() else None,
+           "mean_dE": float(de.mean()), "stderr_dE": err,
+           "mean_E_fp64": float(er.mean()), "note": label}
+    if err is not None:
+        out["resolves_1e-5"] = bool(err < 1e-5)
     return out
 
 
-def cpu_baseline(descs, budget_s=12.0):
+def energy_study(nsweep, seed, steady):
+    """The 1024-chain 2D batch in both precisions on the same Philox streams:
+    `steady` sweeps, then nsweep sweeps each followed by a Measure.  Reports
+    the paired difference of the per-chain time-averaged energies, and the
+    two batch-mean energies with blocked errors (blocks of 50 sweeps)."""
+    descs, _ = workload("cfg3", 1024)
+    runs = {}
+    for name, prec in (("mixed", capi.PQ_PREC_MIXED), ("fp64", capi.PQ_PREC_FP64)):
+        b = make_batch(descs, prec, seed)
+        b.run_sweeps(steady)
+        e = b.run_measured(nsweep, 1)
+        runs[name] = (b, e)
+        log(f"energy study: {name} done")
+    (bm, em), (br, er) = runs["mixed"], runs["fp64"]
+    out = paired_energy(bm, br, em, er, f"1024-chain 2D batch, {nsweep} measured sweeps after {steady}, "
+                                        "per-chain time averages")
+    for name, e in (("mixed", em), ("fp64", er)):
+        blocks = e.mean(axis=1)[: (nsweep // 50) * 50].reshape(-1, 50).mean(axis=1)
+        out[f"E_{name}"] = float(blocks.mean())
+        out[f"E_{name}_err"] = float(blocks.std(ddof=1) / math.sqrt(len(blocks)))
+    d = out["E_mixed"] - out["E_fp64"]
+    out["batch_means_agree_2sigma"] = bool(abs(d) <= 2 * math.hypot(out["E_mixed_err"], out["E_fp64_err"]))
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(descs, budget_s=12.0, single_s=8.0):
     """The oracle (FP64 C restatement) on host cores: a bounded sample of the
-    workload's chains, one chain per thread (OpenMP)."""
+    workload's chains, one chain per thread (OpenMP), on all cores and then on
+    one core.  The sampler only: no Measure, which favours the CPU."""
     import oracle_lib as ol
     lib = ol.load()
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    nch = min(len(descs), max(cores, 2 * cores))
+    nch = min(len(descs), 2 * cores)
     sample = descs[:nch]
     arr = capi.descs_array(sample)
     beads = np.concatenate([capi.initial_beads(d, 1000 + c).ravel() for c, d in enumerate(sample)])
-    # calibrate one sweep, then scale the sweep count to the time budget
     t0 = time.perf_counter()
-    bu = lib.oq_run_batch(arr, nch, capi.dptr(beads), 1, 7, 0, cores, None)
+    lib.oq_run_batch(arr, nch, capi.dptr(beads), 1, 7, 0, cores, None)
     t1 = time.perf_counter() - t0
     nsweep = max(1, min(1000, int(budget_s / max(t1, 1e-3))))
     t0 = time.perf_counter()
     bu = lib.oq_run_batch(arr, nch, capi.dptr(beads), nsweep, 7, 0, cores, None)
     dt = time.perf_counter() - t0
+    # one core: the same chains, one at a time, for about single_s seconds
+    one = capi.descs_array(sample[:1])
+    b1 = capi.initial_beads(sample[0], 1000).ravel().copy()
+    t0 = time.perf_counter()
+    lib.oq_run_batch(one, 1, capi.dptr(b1), 1, 7, 0, 1, None)
+    t1 = time.perf_counter() - t0
+    ns1 = max(1, min(1000, int(single_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    bu1 = lib.oq_run_batch(one, 1, capi.dptr(b1), ns1, 7, 0, 1, None)
+    dt1 = time.perf_counter() - t0
     return dict(value=bu / dt, unit="bead-updates/s", cores=cores, kind="port",
-                sample=f"{nch} chains of the workload x {nsweep} sweeps, oracle (FP64 C, "
-                       f"-O3 -march=x86-64-v3, OpenMP {cores} threads), {dt:.1f} s")
+                sample=f"{nch} chains of the workload x {nsweep} sweeps, oracle (FP64 C, -O3 -march=x86-64-v3, "
+                       f"OpenMP {cores} threads), {dt:.1f} s; sampler only, no Measure",
+                cpu_model=cpu_model(), nproc=os.cpu_count(),
+                single_core={"value": bu1 / dt1, "sample": f"chain 0 of the workload x {ns1} sweeps on one "
+                                                            f"thread, {dt1:.1f} s"})
 
 
 def main():
@@ -242,11 +291,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cfg3")
+    ap.add_argument("--workload", default="cfg5")
     ap.add_argument("--nchain", type=int, default=None)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--steady", type=int, default=None,
+                    help="sweeps from the replicated start before the warmup steps (tuning included)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-pair", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only: no second workload, energy study or single-chain rate")
+    ap.add_argument("--energy-sweeps", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--schedule", default=None,
                     help="team,inverse_lds,tables_lds,node_state_lds[,heavy_chains,heavy_team] "
@@ -261,6 +315,7 @@ def main():
     nchain = args.nchain or {"cfg3": 1024, "cfg5": 4096}.get(args.workload, 1)
     descs, wdesc = workload(args.workload, nchain)
     prec = capi.PQ_PREC_MIXED if args.precision == "mixed" else capi.PQ_PREC_FP64
+    steady = args.steady if args.steady is not None else STEADY.get(args.workload, 40)
     if args.calib_copy:
         # a plain copy of known size: its FETCH_SIZE/WRITE_SIZE scale the counters
         import torch
@@ -270,50 +325,63 @@ def main():
         del src, dst
 
     sched = tuple(int(v) for v in args.schedule.split(",")) if args.schedule else None
-    b_main, main_run = run_gpu(descs, prec, args.steps, args.warmup, args.seed, schedule=sched)
-    bu_total = float(main_run["bu"].sum())
-    value = bu_total / main_run["wall"]
-    kernel_ms = float(np.mean(main_run["ms"]))
-    w = 8
-    bytes_launch = float(sum(bu_c * bytes_per_update(d, w) for bu_c, d in zip(main_run["bu"], descs))) / args.steps
-    flops_launch = float(sum(bu_c * flops_per_update(d) for bu_c, d in zip(main_run["bu"], descs))) / args.steps
-    achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    b_main, run = run_gpu(descs, prec, args.steps, args.warmup, args.seed, steady, schedule=sched, pair_cf=PAIR_CF)
+    bu_total = float(run["bu"].sum())
+    value = bu_total / run["wall"]
+    kernel_ms, roof = roofline(descs, run, args.steps)
     traffic, traffic_src = measured_traffic(args.workload, args.precision)
+    roof["traffic"] = traffic
+    roof["traffic_source"] = traffic_src
+    ms_step = 1e3 * run["wall"] / args.steps
+    beads_per_measure = float(sum(d.nbeads() for d in descs))
     out = {
         "metric": METRIC, "value": value, "unit": "bead-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "warmup_sweeps_done": main_run["warmup_done"],
-        "ms_per_step": 1e3 * main_run["wall"] / args.steps,
+        "steps": args.steps, "warmup": args.warmup, "warmup_sweeps_done": run["warmup_done"],
+        "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32+fp64" if prec == capi.PQ_PREC_MIXED else "fp64",
         "data": "synthetic (seeded Gaussian beads, replicated across slices for node chains)",
         "config": {"workload": f"{args.workload}: {wdesc}", "chains": len(descs),
-                   "precision": args.precision, "step": "one sweep of every chain (one kernel launch)",
-                   "schedule": main_run["schedule"]},
-        "accepted_bead_updates_per_s": float(main_run["abu"].sum()) / main_run["wall"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": traffic_src, "algorithmic_bytes": bytes_launch,
-                     "kernel": "sweep_kernel", "kernel_ms": kernel_ms,
-                     "fp64_equiv_tflops": flops_launch / (kernel_ms * 1e-3) / 1e12,
-                     "valu_frac_of_fp64_peak": flops_launch / (kernel_ms * 1e-3) / 1e12 / FP64_VECTOR_TFLOPS},
+                   "precision": args.precision,
+                   "step": "one sweep of every chain (captured graph of the sweep_kernel launches) + one Measure "
+                           "(thermal energy and 256-bin g(r) of every chain, on the device)",
+                   "schedule": run["schedule"]},
+        "accepted_bead_updates_per_s": float(run["abu"].sum()) / run["wall"],
+        "measure_ms": ms_step - kernel_ms,
+        "measure_bytes": 8 * beads_per_measure * 2,
+        "roofline": roof,
+        "pair_cf_counts": int(run["counts"].sum()) if run["counts"] is not None else None,
     }
     if not args.no_fp64_pair and prec == capi.PQ_PREC_MIXED:
-        # the same number of sweeps (the mixed run's autotuning sweeps are
-        # matched by plain warmup sweeps) on the library's default schedule:
-        # schedules change where data lives, not the decisions, and the mixed
-        # run's LDS placement does not suit FP64 tables and node state
+        # the same sweeps (the mixed run's tuning sweeps are matched by plain
+        # sweeps) on the library's default schedule: schedules change where data
+        # lives, not the decisions
         b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed,
-                           autotune=False, total_warmup=main_run["warmup_done"])
+                           run["warmup_done"] - args.warmup, autotune=False, pair_cf=PAIR_CF)
         out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
-                               "kernel_ms": float(np.mean(ref["ms"]))}
-        out["energy_vs_fp64"] = paired_energy(b_main, b64, main_run["energy"], ref["energy"])
-        out["observables_vs_fp64"] = paired_observables(b_main, b64)
+                               "kernel_ms": float(np.mean(ref["sweep_ms"]))}
+        out["energy_vs_fp64"] = paired_energy(b_main, b64, run["energies"], ref["energies"],
+                                              f"headline batch, the {args.steps} timed steps")
+        del b64
+    if not args.no_extras:
+        if args.workload != "cfg3":
+            d3, w3 = workload("cfg3", 1024)
+            b3, r3 = run_gpu(d3, prec, args.steps, args.warmup, args.seed, STEADY["cfg3"], pair_cf=PAIR_CF)
+            k3, roof3 = roofline(d3, r3, args.steps)
+            t3, t3src = measured_traffic("cfg3", args.precision)
+            roof3["traffic"], roof3["traffic_source"] = t3, t3src
+            out["second_workload"] = {"workload": f"cfg3: {w3}", "value": float(r3["bu"].sum()) / r3["wall"],
+                                      "ms_per_step": 1e3 * r3["wall"] / args.steps, "schedule": r3["schedule"],
+                                      "roofline": roof3}
+            del b3
+        out["energy_study"] = energy_study(args.energy_sweeps, args.seed, STEADY["cfg3"])
         single = [capi.cfg2_desc()]
-        _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed, autotune=False)
+        _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed, 20, autotune=False)
         r1 = float(one["bu"].sum()) / one["wall"]
         out["fp64_single_chain"] = {"workload": "cfg2: 3+3 e- 2D dot, SHONodes, M=400", "value": r1}
         out["ratio_vs_fp64_single_chain"] = value / r1
     if not args.no_cpu:
+        log("cpu baseline")
         cb = cpu_baseline(descs)
         out["cpu_baseline"] = cb
         out["ratio_vs_cpu"] = value / cb["value"]

@@ -24,6 +24,8 @@
  *   DoubleSectionChooser::run              src/advancer/DoubleSectionChooser.cc:33-65
  *   ThermoEnergyEstimator::handleLink      src/estimator/ThermoEnergyEstimator.cc:32-54
  *   PairCFEstimator<1>::handleLink         src/estimator/PairCFEstimator.h:72-93
+ *   VirialEnergyEstimator, CoulombEnergyEstimator, EwaldCoulombEstimator
+ *                                          src/estimator/{Virial,Coulomb,EwaldCoulomb}*.cc
  */
 #ifndef PIQMC_HIP_H
 #define PIQMC_HIP_H
@@ -310,6 +312,22 @@ int pq_get_node_scale(pq_batch* batch, int chain, double* scale);
    same results as nblock x (pq_run_sweeps(nsweep), pq_measure_energy), without a
    host round trip per block (the <Loop>/<Measure> pattern). */
 int pq_run_measured(pq_batch* batch, int nblock, int nsweep, double* energies);
+/* pq_run_measured with sweep_ms[block] = the duration of that block's
+   sweeps: each block launches a captured graph of its sampler launch between
+   two HIP events on the batch stream, then its Measure kernels. */
+int pq_run_measured_timed(pq_batch* batch, int nblock, int nsweep, double* energies, float* sweep_ms);
+/* Adds a pair-correlation estimator (<PairCFEstimator nspecies species1..n>
+   with <Radial nbin min max>, src/parser/EstimatorParser.cc:477-488,643-725)
+   to every Measure of pq_run_measured: its integer counts, as
+   pq_pair_cf_histogram gives them, accumulate on the device over the
+   Measures; nspecies 0 removes it.  pq_collect_pair_cf adds the accumulated
+   counts[chain][nbin] to the caller's array and restarts the accumulation
+   (a <Collect>: the reference divides each Measure's counts by nslice and
+   averages a block's Measures, which the caller does with nslice and the
+   number of Measures). */
+int pq_set_measure_pair_cf(pq_batch* batch, int nspecies, const int* species, double rmin,
+                           double rmax, int nbin);
+int pq_collect_pair_cf(pq_batch* batch, uint64_t* counts);
 
 /* Thermodynamic energy estimate of every chain's current paths:
    E = (1/M) sum_{t,i} utau(i,t) (ThermoEnergyEstimator + CompositeAction::getBeadAction),
@@ -341,9 +359,16 @@ int pq_measure_trad_ewald_coulomb_energy(pq_batch* batch, double epsilon, double
    parser's default; nwindow = 1 is the thermodynamic estimator).  The
    FixedNodeActions contribute zero force and utau, as in the reference. */
 int pq_measure_virial_energy(pq_batch* batch, int nwindow, double* energy);
-/* Radial pair histogram of every chain (PairCFEstimator<1> with PairRadial):
-   integer counts over all slices, species1 particle i against every particle
-   j != i of species2; counts[chain][nbin], added to what the array holds. */
+/* Radial pair histogram of every chain (PairCFEstimator<1>::handleLink with
+   PairRadial, src/estimator/PairCFEstimator.h:72-93): integer counts over all
+   slices, particle i of species[0] against every particle j != i of each
+   partner species species[1..nspecies-1] in list order (a partner listed
+   twice counts twice), bin int((r - rmin) nbin/(rmax - rmin)), dropped when r
+   < rmin or the bin exceeds nbin - 1; counts[chain][nbin], added to what the
+   array holds.  A chain that lacks one of the species counts nothing. */
+int pq_pair_cf_histogram(pq_batch* batch, int nspecies, const int* species, double rmin,
+                         double rmax, int nbin, uint64_t* counts);
+/* The same with one partner species: species {species1, species2}. */
 int pq_pair_histogram(pq_batch* batch, int species1, int species2, double rmin,
                       double rmax, int nbin, uint64_t* counts);
 

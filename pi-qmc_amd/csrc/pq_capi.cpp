@@ -240,6 +240,23 @@ struct pq_batch {
   std::vector<char> gkey;        // what the graph was captured with
   double* dmeas = nullptr;       // [nblock][nchain] energies written by the graph
   size_t nmeas = 0;
+  std::vector<hipEvent_t> gev;   // per block: before and after the sweep graph's launch (timed runs)
+  // the sweeps of one block alone, captured for timed runs
+  hipGraph_t sgraph = nullptr;
+  hipGraphExec_t sexec = nullptr;
+  std::vector<char> skey;
+  // Measure work items (slices of one chain): energy items with their
+  // per-chain ranges and partial sums, and the histogram's coarser items
+  std::vector<MeasItem> items_e, items_g;
+  std::vector<int> first_e;
+  MeasItem* ditems_e = nullptr;
+  MeasItem* ditems_g = nullptr;
+  int* dfirst_e = nullptr;
+  double* dpart = nullptr;
+  int slice_e = 0, slice_g = 0;  // largest staged slice block of an item, doubles
+  // the pair-correlation estimator of every pq_run_measured Measure
+  PairCfArgs cf{};
+  uint64_t* dcf = nullptr;       // [nchain][nbin] counts since the last collect
 };
 
 static DevCtx make_ctx(const pq_batch* b) {
@@ -679,6 +696,34 @@ static hipError_t run_launch(pq_batch* b, const DevCtx& x, int nsweep) {
   return e;
 }
 
+// Measure work items.  An energy item costs about N spring/SHO links plus
+// N(N-1)/2 pair links per slice; items hold ~4096 of those (16 per thread of
+// a 256-thread workgroup) and at most 2048 staged doubles.  Histogram items
+// are coarser (~64k distances) so that each adds its bins to the chain's
+// global counts once.
+static void build_measure_items(pq_batch* b) {
+  b->items_e.clear();
+  b->items_g.clear();
+  b->first_e.assign(1, 0);
+  b->slice_e = b->slice_g = 0;
+  for (int c = 0; c < b->nchain; ++c) {
+    const ChainP& ch = b->hc[c];
+    const int64_t nd = (int64_t)ch.N * ch.D;
+    const int64_t per = ch.N + (ch.npair ? (int64_t)ch.N * (ch.N - 1) / 2 : 0) + (ch.ew.nk ? (int64_t)ch.ew.nk * ch.N / 4 : 0);
+    int64_t te = std::max<int64_t>(1, 4096 / std::max<int64_t>(1, per));
+    te = std::min<int64_t>(te, std::max<int64_t>(1, 2048 / nd - 1));
+    te = std::min<int64_t>(te, ch.M);
+    for (int t = 0; t < ch.M; t += (int)te) b->items_e.push_back(MeasItem{c, t, (int)std::min<int64_t>(ch.M, t + te)});
+    b->first_e.push_back((int)b->items_e.size());
+    b->slice_e = std::max<int>(b->slice_e, (int)((te + 1) * nd));
+    int64_t tg = std::max<int64_t>(1, 65536 / std::max<int64_t>(1, (int64_t)ch.N * ch.N));
+    tg = std::min<int64_t>(tg, std::max<int64_t>(1, 4096 / nd - 1));
+    tg = std::min<int64_t>(tg, ch.M);
+    for (int t = 0; t < ch.M; t += (int)tg) b->items_g.push_back(MeasItem{c, t, (int)std::min<int64_t>(ch.M, t + tg)});
+    b->slice_g = std::max<int>(b->slice_g, (int)((tg + 1) * nd));
+  }
+}
+
 extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int precision, int device,
                                pq_batch** out) {
   if (!chains || nchain < 1 || !out) return fail(PQ_ERR_ARG, "pq_batch_create: bad arguments");
@@ -733,6 +778,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     ch.perm_off = b->nperm;
     b->nperm += 2 * (int64_t)ch.N;
   }
+  build_measure_items(b);
   make_layout(b, b->sched);
   if (layout_bytes(b) > 160 * 1024) {
     int n = layout_bytes(b);
@@ -799,7 +845,11 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
             dalloc((void**)&b->dstats, sizeof(pq_chain_stats) * nchain) &&
             dalloc((void**)&b->dorder, sizeof(int) * nchain) &&
             dalloc((void**)&b->dperm, sizeof(int) * b->nperm) &&
-            dalloc((void**)&b->dewk, sizeof(double) * b->hk.size());
+            dalloc((void**)&b->dewk, sizeof(double) * b->hk.size()) &&
+            dalloc((void**)&b->ditems_e, sizeof(MeasItem) * b->items_e.size()) &&
+            dalloc((void**)&b->ditems_g, sizeof(MeasItem) * b->items_g.size()) &&
+            dalloc((void**)&b->dfirst_e, sizeof(int) * b->first_e.size()) &&
+            dalloc((void**)&b->dpart, sizeof(double) * b->items_e.size());
   if (!ok) return cleanup_fail(fail(PQ_ERR_NOMEM, "device allocation failed"));
   b->nfncache = fnc;
   std::vector<float> uf(b->hu.begin(), b->hu.end());
@@ -813,6 +863,9 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
       (b->hut.size() && hipMemcpy(b->dut, b->hut.data(), sizeof(double) * b->hut.size(), hipMemcpyHostToDevice) != hipSuccess) ||
       (uf.size() && hipMemcpy(b->duf, uf.data(), sizeof(float) * uf.size(), hipMemcpyHostToDevice) != hipSuccess) ||
       (b->hk.size() && hipMemcpy(b->dewk, b->hk.data(), sizeof(double) * b->hk.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(b->ditems_e, b->items_e.data(), sizeof(MeasItem) * b->items_e.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->ditems_g, b->items_g.data(), sizeof(MeasItem) * b->items_g.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->dfirst_e, b->first_e.data(), sizeof(int) * b->first_e.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->dscale, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->dorder, b->order.data(), sizeof(int) * nchain, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->dperm, ident.data(), sizeof(int) * ident.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -826,18 +879,21 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   return PQ_OK;
 }
 
+static void drop_graph(pq_batch* b);
+
 extern "C" void pq_batch_destroy(pq_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache, b->dfnsweep,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr,
-                  b->dperm, b->dewk, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est};
+                  b->dperm, b->dewk, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est,
+                  b->ditems_e, b->ditems_g, b->dfirst_e, b->dpart, b->dcf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  drop_graph(b);
   if (b->dmeas) (void)hipFree(b->dmeas);
+  for (hipEvent_t e : b->gev) (void)hipEventDestroy(e);
   if (b->stream2) (void)hipStreamSynchronize(b->stream2);
   if (b->evf) (void)hipEventDestroy(b->evf);
   if (b->evj) (void)hipEventDestroy(b->evj);
@@ -1410,7 +1466,8 @@ extern "C" int pq_measure_energy(pq_batch* b, double* energy) {
   void* p;
   if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_energy(b->D, x, b->nchain, (double*)p, b->stream));
+  HIPCHK(launch_energy_items(b->D, x, b->ditems_e, (int)b->items_e.size(), b->dfirst_e, b->nchain, b->slice_e,
+                             b->dpart, (double*)p, b->stream));
   HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
@@ -1551,80 +1608,220 @@ extern "C" int pq_measure_virial_energy(pq_batch* b, int nwindow, double* energy
 static void drop_graph(pq_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->sexec) (void)hipGraphExecDestroy(b->sexec);
+  if (b->sgraph) (void)hipGraphDestroy(b->sgraph);
   b->gexec = nullptr;
   b->graph = nullptr;
+  b->sexec = nullptr;
+  b->sgraph = nullptr;
   b->gkey.clear();
+  b->skey.clear();
 }
 
-// nblock x (nsweep sweeps, then ThermoEnergyEstimator on every chain) as one
-// HIP graph on the batch stream: the Loop/Measure pattern of a run without a
-// host round trip per block.  The graph is captured on the first call and
-// replayed while the batch's kernel arguments (schedule, seed, uniform
-// source) and (nblock, nsweep) stay the same.
-extern "C" int pq_run_measured(pq_batch* b, int nblock, int nsweep, double* energies) {
+// The Measure of block blk: thermodynamic energies into dmeas[blk] and, when
+// one is set, the pair-correlation counts added to dcf.
+static hipError_t launch_measure(pq_batch* b, const DevCtx& x, int blk) {
+  hipError_t e = launch_energy_items(b->D, x, b->ditems_e, (int)b->items_e.size(), b->dfirst_e, b->nchain, b->slice_e,
+                                     b->dpart, b->dmeas + (size_t)blk * b->nchain, b->stream);
+  if (e == hipSuccess && b->cf.nsp > 0)
+    e = launch_pair_cf_items(b->D, x, b->ditems_g, (int)b->items_g.size(), b->slice_g, b->cf, b->dcf, b->stream);
+  return e;
+}
+
+// Captures the kernel launches that fn enqueues on the batch stream as a graph.
+template <class F>
+static int capture(pq_batch* b, hipGraph_t* graph, hipGraphExec_t* exec, F&& fn) {
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t e = fn();
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(PQ_ERR_HIP, "capturing the measured sweeps failed: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+  }
+  *graph = g;
+  HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return PQ_OK;
+}
+
+// nblock x (nsweep sweeps, then the Measure: ThermoEnergyEstimator on every
+// chain and, when one is set, the pair-correlation estimator) on the batch
+// stream: the Loop/Measure pattern of a run without a host round trip per
+// block.  Untimed, the whole run is one HIP graph, captured on the first call
+// and replayed while the batch's kernel arguments (schedule, seed, uniform
+// source, the estimator) and (nblock, nsweep) stay the same.  Timed
+// (sweep_ms given), each block launches a captured graph of its sweeps between
+// two HIP events, whose elapsed time is that block's sampler time, and then
+// its Measure kernels (HIP does not time events recorded inside a graph).
+static int run_measured(pq_batch* b, int nblock, int nsweep, double* energies, float* sweep_ms) {
   if (!b || nblock < 1 || nsweep < 1 || !energies) return fail(PQ_ERR_ARG, "pq_run_measured: bad arguments");
   HIPCHK(hipSetDevice(b->device));
   const size_t need = (size_t)nblock * b->nchain;
+  const bool timed = sweep_ms != nullptr;
   if (need > b->nmeas) {
+    // the whole-run graph writes into dmeas; the sweep graph does not
     HIPCHK(hipStreamSynchronize(b->stream));
-    drop_graph(b);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->gkey.clear();
     if (b->dmeas) (void)hipFree(b->dmeas);
     b->dmeas = nullptr;
     b->nmeas = 0;
     HIPCHK(hipMalloc(&b->dmeas, sizeof(double) * need));
     b->nmeas = need;
   }
+  while (timed && b->gev.size() < 2 * (size_t)nblock) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    b->gev.push_back(e);
+  }
   DevCtx x = make_ctx(b);
-  // the kernel arguments the graph bakes in (field by field: no padding bytes)
+  // the kernel arguments a graph bakes in (field by field: no padding bytes)
   const uint64_t fields[] = {
       (uint64_t)(uintptr_t)x.beads, (uint64_t)(uintptr_t)x.inj, (uint64_t)x.inj_per_chain,
       (uint64_t)x.rng_mode, x.seed, (uint64_t)x.first_stream, (uint64_t)(uintptr_t)x.inv,
       (uint64_t)(uintptr_t)x.fncache, (uint64_t)(uintptr_t)x.fnsweep, (uint64_t)nblock, (uint64_t)nsweep,
-      (uint64_t)b->sched.heavy, (uint64_t)b->sched.ncsplit};
-  std::vector<char> key(sizeof(fields) + 2 * sizeof(SweepLayoutH));
-  std::memcpy(key.data(), fields, sizeof(fields));
-  std::memcpy(key.data() + sizeof(fields), &b->lay, sizeof(SweepLayoutH));
-  std::memcpy(key.data() + sizeof(fields) + sizeof(SweepLayoutH), &b->lay_heavy, sizeof(SweepLayoutH));
-  if (!b->gexec || key != b->gkey) {
-    drop_graph(b);
-    HIPCHK(hipStreamSynchronize(b->stream));
-    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = hipSuccess;
-    for (int blk = 0; blk < nblock && e == hipSuccess; ++blk) {
-      e = run_launch(b, x, nsweep);
-      if (e == hipSuccess) e = launch_energy(b->D, x, b->nchain, b->dmeas + (size_t)blk * b->nchain, b->stream);
+      (uint64_t)b->sched.heavy, (uint64_t)b->sched.ncsplit, (uint64_t)(uintptr_t)b->dcf,
+      (uint64_t)b->cf.nsp, (uint64_t)b->cf.nbin, (uint64_t)(uintptr_t)b->dmeas};
+  std::vector<char> key(sizeof(fields) + 2 * sizeof(SweepLayoutH) + sizeof(b->cf.sp) + 2 * sizeof(double));
+  char* kp = key.data();
+  std::memcpy(kp, fields, sizeof(fields)); kp += sizeof(fields);
+  std::memcpy(kp, &b->lay, sizeof(SweepLayoutH)); kp += sizeof(SweepLayoutH);
+  std::memcpy(kp, &b->lay_heavy, sizeof(SweepLayoutH)); kp += sizeof(SweepLayoutH);
+  std::memcpy(kp, b->cf.sp, sizeof(b->cf.sp)); kp += sizeof(b->cf.sp);
+  std::memcpy(kp, &b->cf.rmin, sizeof(double)); kp += sizeof(double);
+  std::memcpy(kp, &b->cf.rmax, sizeof(double));
+  if (timed) {
+    // the sweep graph depends on neither nblock nor the Measure's buffers
+    std::vector<char> skey(key);
+    uint64_t* sf = reinterpret_cast<uint64_t*>(skey.data());
+    sf[9] = 0;                                    // nblock
+    for (int k = 13; k < 17; ++k) sf[k] = 0;      // dcf, cf.nsp, cf.nbin, dmeas
+    std::memset(skey.data() + sizeof(fields) + 2 * sizeof(SweepLayoutH), 0, sizeof(b->cf.sp) + 2 * sizeof(double));
+    if (!b->sexec || skey != b->skey) {
+      if (b->sexec) (void)hipGraphExecDestroy(b->sexec);
+      if (b->sgraph) (void)hipGraphDestroy(b->sgraph);
+      b->sexec = nullptr;
+      b->sgraph = nullptr;
+      b->skey.clear();
+      if (int rc = capture(b, &b->sgraph, &b->sexec, [&] { return run_launch(b, x, nsweep); })) return rc;
+      b->skey = skey;
     }
-    hipGraph_t g = nullptr;
-    const hipError_t ee = hipStreamEndCapture(b->stream, &g);
-    if (e != hipSuccess || ee != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      return fail(PQ_ERR_HIP, "capturing the measured sweeps failed: %s",
-                  hipGetErrorString(e != hipSuccess ? e : ee));
+    for (int blk = 0; blk < nblock; ++blk) {
+      HIPCHK(hipEventRecord(b->gev[2 * blk], b->stream));
+      HIPCHK(hipGraphLaunch(b->sexec, b->stream));
+      HIPCHK(hipEventRecord(b->gev[2 * blk + 1], b->stream));
+      HIPCHK(launch_measure(b, x, blk));
     }
-    b->graph = g;
-    HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-    b->gkey = key;
+  } else {
+    if (!b->gexec || key != b->gkey) {
+      if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->gexec = nullptr;
+      b->graph = nullptr;
+      b->gkey.clear();
+      if (int rc = capture(b, &b->graph, &b->gexec, [&] {
+            hipError_t e = hipSuccess;
+            for (int blk = 0; blk < nblock && e == hipSuccess; ++blk) {
+              e = run_launch(b, x, nsweep);
+              if (e == hipSuccess) e = launch_measure(b, x, blk);
+            }
+            return e;
+          }))
+        return rc;
+      b->gkey = key;
+    }
+    HIPCHK(hipGraphLaunch(b->gexec, b->stream));
   }
-  HIPCHK(hipGraphLaunch(b->gexec, b->stream));
   HIPCHK(hipMemcpyAsync(energies, b->dmeas, sizeof(double) * need, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (timed)
+    for (int blk = 0; blk < nblock; ++blk) HIPCHK(hipEventElapsedTime(&sweep_ms[blk], b->gev[2 * blk], b->gev[2 * blk + 1]));
+  return check_stream(b);
+}
+
+extern "C" int pq_run_measured(pq_batch* b, int nblock, int nsweep, double* energies) {
+  return run_measured(b, nblock, nsweep, energies, nullptr);
+}
+
+extern "C" int pq_run_measured_timed(pq_batch* b, int nblock, int nsweep, double* energies, float* sweep_ms) {
+  if (!sweep_ms) return fail(PQ_ERR_ARG, "pq_run_measured_timed: null sweep_ms");
+  return run_measured(b, nblock, nsweep, energies, sweep_ms);
+}
+
+static int cf_args(int nspecies, const int* species, double rmin, double rmax, int nbin, PairCfArgs* g) {
+  if (nspecies < 1 || nspecies > PQ_MAX_CF_SPECIES || !species)
+    return fail(PQ_ERR_ARG, "pair correlation: nspecies must be in 1..%d", PQ_MAX_CF_SPECIES);
+  if (nbin < 1 || nbin > 16384 || !(rmax > rmin) || !std::isfinite(rmin) || !std::isfinite(rmax))
+    return fail(PQ_ERR_ARG, "pair correlation: need 1 <= nbin <= 16384 and finite rmin < rmax");
+  std::memset(g, 0, sizeof *g);
+  g->nsp = nspecies;
+  for (int k = 0; k < nspecies; ++k) {
+    if (species[k] < 0 || species[k] >= PQ_MAX_SPECIES) return fail(PQ_ERR_ARG, "pair correlation: species out of range");
+    g->sp[k] = species[k];
+  }
+  g->nbin = nbin;
+  g->rmin = rmin;
+  g->rmax = rmax;
+  g->deltaInv = nbin / (rmax - rmin);    // PairCFEstimator's deltaInv = nbin / (max - min)
   return PQ_OK;
 }
 
-extern "C" int pq_pair_histogram(pq_batch* b, int s1, int s2, double rmin, double rmax, int nbin,
-                                 uint64_t* counts) {
-  if (!b || !counts || nbin < 1 || nbin > 16384 || !(rmax > rmin))
-    return fail(PQ_ERR_ARG, "pq_pair_histogram: bad arguments");
-  if (s1 < 0 || s2 < 0 || s1 >= PQ_MAX_SPECIES || s2 >= PQ_MAX_SPECIES) return fail(PQ_ERR_ARG, "species out of range");
+extern "C" int pq_pair_cf_histogram(pq_batch* b, int nspecies, const int* species, double rmin, double rmax,
+                                    int nbin, uint64_t* counts) {
+  if (!b || !counts) return fail(PQ_ERR_ARG, "pq_pair_cf_histogram: bad arguments");
+  PairCfArgs g;
+  if (int rc = cf_args(nspecies, species, rmin, rmax, nbin, &g)) return rc;
   HIPCHK(hipSetDevice(b->device));
   void* p;
   const size_t bytes = sizeof(uint64_t) * (size_t)nbin * b->nchain;
   if (int rc = scratch(b, bytes, &p)) return rc;
   HIPCHK(hipMemcpyAsync(p, counts, bytes, hipMemcpyHostToDevice, b->stream));
   DevCtx x = make_ctx(b);
-  HIPCHK(launch_histogram(b->D, x, b->nchain, s1, s2, rmin, rmax, nbin, (uint64_t*)p, b->stream));
+  HIPCHK(launch_pair_cf_items(b->D, x, b->ditems_g, (int)b->items_g.size(), b->slice_g, g, (uint64_t*)p, b->stream));
   HIPCHK(hipMemcpyAsync(counts, p, bytes, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_pair_histogram(pq_batch* b, int s1, int s2, double rmin, double rmax, int nbin,
+                                 uint64_t* counts) {
+  const int sp[2] = {s1, s2};
+  return pq_pair_cf_histogram(b, 2, sp, rmin, rmax, nbin, counts);
+}
+
+extern "C" int pq_set_measure_pair_cf(pq_batch* b, int nspecies, const int* species, double rmin, double rmax,
+                                      int nbin) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  drop_graph(b);
+  if (b->dcf) (void)hipFree(b->dcf);
+  b->dcf = nullptr;
+  std::memset(&b->cf, 0, sizeof b->cf);
+  if (nspecies == 0) return PQ_OK;
+  PairCfArgs g;
+  if (int rc = cf_args(nspecies, species, rmin, rmax, nbin, &g)) return rc;
+  const size_t bytes = sizeof(uint64_t) * (size_t)nbin * b->nchain;
+  HIPCHK(hipMalloc(&b->dcf, bytes));
+  HIPCHK(hipMemset(b->dcf, 0, bytes));
+  b->cf = g;
+  return PQ_OK;
+}
+
+extern "C" int pq_collect_pair_cf(pq_batch* b, uint64_t* counts) {
+  if (!b || !counts) return fail(PQ_ERR_ARG, "pq_collect_pair_cf: bad arguments");
+  if (b->cf.nsp == 0) return fail(PQ_ERR_STATE, "no pair-correlation estimator is set");
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const size_t n = (size_t)b->cf.nbin * b->nchain;
+  std::vector<uint64_t> h(n);
+  HIPCHK(hipMemcpy(h.data(), b->dcf, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(b->dcf, 0, sizeof(uint64_t) * n));
+  for (size_t k = 0; k < n; ++k) counts[k] += h[k];
   return PQ_OK;
 }
 

@@ -8,8 +8,8 @@
 //
 // The call-level kernels (action_diff_kernel, fn_*_kernel, node_eval_kernel)
 // run the same Eng routines on caller-provided sections in global memory; the
-// estimator kernels compute the thermodynamic energy and g(r) counts of the
-// resident paths.
+// estimator kernels here compute the Coulomb and virial energies of the
+// resident paths (the thermodynamic energy and g(r) are in pq_measure.hip).
 #include "pq_engine.h"
 #include "pq_launch.h"
 
@@ -248,33 +248,6 @@ __device__ double ewald_all_slices(const double* kp, const EwP& ew, const ChainP
   }
   if (threadIdx.x == 0) acc += c.M * ew.self;
   return acc;
-}
-
-// ThermoEnergyEstimator (ThermoEnergyEstimator.cc:32-54): one workgroup per chain
-template <int D>
-__global__ __launch_bounds__(256) void energy_kernel(DevCtx x, double* energy) {
-  __shared__ ChainP c;
-  __shared__ double part[4];
-  const int chain = blockIdx.x;
-  {
-    const uint32_t* s = reinterpret_cast<const uint32_t*>(&x.chains[chain]);
-    uint32_t* d = reinterpret_cast<uint32_t*>(&c);
-    for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
-    __syncthreads();
-  }
-  const double* beads = x.beads + c.bead_off;
-  double acc = 0;
-  const int nl = c.M * c.N;
-  for (int q = threadIdx.x; q < nl; q += blockDim.x) acc += bead_utau<D>(x, c, beads, q % c.N, q / c.N);
-  if (c.ew.nk) acc += ewald_all_slices<D>(x.ewk + c.ew.off, c.ew, c, beads) / c.ceps;
-  acc = wave_sum(acc);
-  if (lane_id() == 0) part[threadIdx.x / PQ_WAVE] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double v = 0;
-    for (int w = 0; w < (int)(blockDim.x / PQ_WAVE); ++w) v += part[w];
-    energy[chain] = v / c.M;
-  }
 }
 
 // CoulombEnergyEstimator::handleLink (CoulombEnergyEstimator.cc:30-40) summed
@@ -570,52 +543,6 @@ __global__ __launch_bounds__(256) void ewald_coulomb_energy_kernel(DevCtx x, con
   }
 }
 
-// PairCFEstimator<1>::handleLink with PairRadial (PairCFEstimator.h:72-93).
-// Integer counts per chain; contraction is off so the bin of every distance
-// is the one the FP64 expression gives on the host.
-template <int D>
-__global__ __launch_bounds__(256) void histogram_kernel(DevCtx x, int s1, int s2, double rmin,
-                                                         double rmax, int nbin, uint64_t* counts) {
-#pragma clang fp contract(off)
-  extern __shared__ unsigned int hist[];
-  __shared__ ChainP c;
-  const int chain = blockIdx.x;
-  {
-    const uint32_t* s = reinterpret_cast<const uint32_t*>(&x.chains[chain]);
-    uint32_t* d = reinterpret_cast<uint32_t*>(&c);
-    for (int k = threadIdx.x; k < (int)(sizeof(ChainP) / 4); k += blockDim.x) d[k] = s[k];
-  }
-  for (int b = threadIdx.x; b < nbin; b += blockDim.x) hist[b] = 0;
-  __syncthreads();
-  if (s1 < c.nsp && s2 < c.nsp) {
-    const double* beads = x.beads + c.bead_off;
-    const double deltaInv = nbin / (rmax - rmin);
-    const int n1 = c.count[s1], n2 = c.count[s2];
-    const int64_t items = (int64_t)c.M * n1 * n2;
-    for (int64_t q = threadIdx.x; q < items; q += blockDim.x) {
-      const int t = (int)(q / ((int64_t)n1 * n2));
-      const int rem = (int)(q - (int64_t)t * n1 * n2);
-      const int i = c.ifirst[s1] + rem / n2, j = c.ifirst[s2] + rem % n2;
-      if (i == j) continue;
-      const double* a = beads + ((int64_t)t * c.N + i) * D;
-      const double* b = beads + ((int64_t)t * c.N + j) * D;
-      double delta[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
-      pbc<D>(c, delta);
-      double r2 = 0;
-#pragma unroll
-      for (int d = 0; d < D; ++d) r2 += delta[d] * delta[d] * 1.0;
-      const double dd = sqrt(r2);
-      const int ibin = (int)((dd - rmin) * deltaInv);
-      if (dd < rmin || ibin > nbin - 1) continue;
-      atomicAdd(&hist[ibin], 1u);
-    }
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < nbin; b += blockDim.x) counts[(int64_t)chain * nbin + b] += hist[b];
-}
-
 // ------------------------------------------------------------ launchers
 
 #define PQ_DISPATCH_R(FN, D_, R_, ...)                                              \
@@ -695,12 +622,6 @@ hipError_t launch_node_eval(int D, int NC, bool mixed, const DevCtx& x, int chai
   PQ_DISPATCH(launch_node_eval_t, x, chain, fa, nmat, r1, r2, inv, egr, det, err, dist, st);
 }
 
-hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st) {
-  if (D == 2) hipLaunchKernelGGL(energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, energy);
-  else hipLaunchKernelGGL(energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, energy);
-  return hipGetLastError();
-}
-
 hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps, double* energy, hipStream_t st) {
   if (D == 2) hipLaunchKernelGGL(coulomb_energy_kernel<2>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
   else hipLaunchKernelGGL(coulomb_energy_kernel<3>, dim3(nchain), dim3(256), 0, st, x, eps, energy);
@@ -728,14 +649,6 @@ hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const
   const int nb = (nq + 255) / 256;
   if (D == 2) hipLaunchKernelGGL(bead_utau_kernel<2>, dim3(nb), dim3(256), 0, st, x, nq, qc, qi, qt, out);
   else hipLaunchKernelGGL(bead_utau_kernel<3>, dim3(nb), dim3(256), 0, st, x, nq, qc, qi, qt, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_histogram(int D, const DevCtx& x, int nchain, int s1, int s2, double rmin, double rmax,
-                            int nbin, uint64_t* counts, hipStream_t st) {
-  const size_t lds = sizeof(unsigned int) * nbin;
-  if (D == 2) hipLaunchKernelGGL(histogram_kernel<2>, dim3(nchain), dim3(256), lds, st, x, s1, s2, rmin, rmax, nbin, counts);
-  else hipLaunchKernelGGL(histogram_kernel<3>, dim3(nchain), dim3(256), lds, st, x, s1, s2, rmin, rmax, nbin, counts);
   return hipGetLastError();
 }
 

@@ -45,6 +45,27 @@ struct FnBufsH {
   double* out;
 };
 
+// A Measure work item: slices [t0, t1) of one chain.
+struct MeasItem {
+  int chain, t0, t1;
+};
+
+#define PQ_MAX_CF_SPECIES 8   // species1 and up to seven partner species of a pair-correlation histogram
+
+// PairCFEstimator<1> with PairRadial: sp[0] against partners sp[1..nsp-1]
+struct PairCfArgs {
+  int nsp;
+  int sp[PQ_MAX_CF_SPECIES];
+  int nbin;
+  double rmin, rmax, deltaInv;
+};
+
+hipError_t launch_energy_items(int D, const DevCtx& x, const MeasItem* items, int nitems, const int* first_item,
+                               int nchain, int max_slice_doubles, double* part, double* energy, hipStream_t st);
+hipError_t launch_pair_cf_items(int D, const DevCtx& x, const MeasItem* items, int nitems, int max_slice_doubles,
+                                const PairCfArgs& g, uint64_t* counts, hipStream_t st);
+hipError_t launch_fill_u64(uint64_t* p, int64_t n, hipStream_t st);
+
 hipError_t launch_sweep(int D, int NC, bool mixed, bool multi, const DevCtx& x, int nchain, int nsweep,
                         const SweepLayoutH& L, hipStream_t st);
 hipError_t launch_action_diff(int D, int NC, bool mixed, const DevCtx& x, int nq, const CallBufsH& b,
@@ -54,7 +75,6 @@ hipError_t launch_fn(int D, int NC, bool mixed, const DevCtx& x, int chain, cons
 hipError_t launch_node_eval(int D, int NC, bool mixed, const DevCtx& x, int chain, int fa, int nmat,
                             double* r1, double* r2, double* inv, double* egr, double* det, int* err,
                             double* dist, hipStream_t st);
-hipError_t launch_energy(int D, const DevCtx& x, int nchain, double* energy, hipStream_t st);
 hipError_t launch_coulomb_energy(int D, const DevCtx& x, int nchain, double eps, double* energy, hipStream_t st);
 hipError_t launch_virial_energy(int D, const DevCtx& x, int nchain, int nwindow, double* energy, hipStream_t st);
 hipError_t launch_ewald_coulomb_energy(int D, const DevCtx& x, int nchain, const EwP* ew, const double* kpool,
@@ -62,7 +82,5 @@ hipError_t launch_ewald_coulomb_energy(int D, const DevCtx& x, int nchain, const
                                        hipStream_t st);
 hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const int* qi, const int* qt,
                             double* out, hipStream_t st);
-hipError_t launch_histogram(int D, const DevCtx& x, int nchain, int s1, int s2, double rmin, double rmax,
-                            int nbin, uint64_t* counts, hipStream_t st);
 
 }  // namespace pq

@@ -1,0 +1,405 @@
+// pq_measure.hip -- the Measure step on the resident paths: thermodynamic
+// energy (ThermoEnergyEstimator over CompositeAction::getBeadAction) and the
+// pair-correlation histogram (PairCFEstimator<1> with PairRadial).
+//
+// Work is split into items of consecutive slices of one chain, sized so that
+// every item costs about the same whatever the chain's N and M: a chain of
+// 64 particles gives a few slices per item, a chain of 2 particles hundreds.
+// A persistent grid takes contiguous runs of the item list; an item stages its slices (and the
+// slice before the first, through the path permutation at slice 0) and the
+// chain's utau tables in LDS, so every bead and table entry is read from HBM
+// or L2 once per item.  Energy items write one partial sum each and a second
+// kernel adds a chain's partials in item order: the result does not depend
+// on scheduling.  Histogram items count in LDS and add their bins to the
+// chain's global counts with integer atomics, also order-independent.
+#include "pq_device.h"
+#include "pq_launch.h"
+
+namespace pq {
+
+constexpr int kMeasThreads = 256;
+
+// the CoulombAction pair table of species (si, sj), -1 when they do not interact
+struct PairMap {
+  int slot[PQ_MAX_SPECIES][PQ_MAX_SPECIES];
+};
+
+// Stage slices t0-1 .. t1-1 of chain c into sl[(t1-t0+1)][N][D]; slice -1 is
+// slice M-1 of inversePermutation[i] (SerialPaths.cc:68-77).
+template <int D>
+__device__ void stage_slices(const DevCtx& x, const ChainP& c, int t0, int t1, double* sl) {
+  const double* beads = x.beads + c.bead_off;
+  const int* invperm = x.perm + c.perm_off + c.N;
+  const int nd = c.N * D;
+  const int n = (t1 - t0 + 1) * nd;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int k = q / nd, r = q - k * nd;
+    const int t = t0 - 1 + k;
+    double v;
+    if (t < 0) {
+      const int i = r / D, d = r - i * D;
+      v = beads[((int64_t)(c.M - 1) * c.N + invperm[i]) * D + d];
+    } else {
+      v = beads[(int64_t)t * nd + r];
+    }
+    sl[q] = v;
+  }
+}
+
+// unordered pair p of n particles, i < j: p < n ((n-1)/2) walks the circulant
+// differences 1..(n-1)/2, the rest (even n only) the difference n/2
+__device__ __forceinline__ void pair_of(int n, int p, int& i, int& j) {
+  const int full = n * ((n - 1) / 2);
+  int a, b;
+  if (p < full) {
+    const int dd = p / n + 1;
+    a = p - (dd - 1) * n;
+    b = a + dd;
+    if (b >= n) b -= n;
+  } else {
+    a = p - full;
+    b = a + n / 2;
+  }
+  i = min(a, b);
+  j = max(a, b);
+}
+
+// utau of the links (t-1, t) of slices t0..t1-1 of one chain, this thread's
+// share: SpringAction (SpringAction.cc:112-123), SHOAction (SHOAction.cc:
+// 71-105), PairAction (PairAction.cc:509-599, uk0CalcDerivatives :640-657)
+// counted once per unordered pair -- each bead of the pair takes half of the
+// same vtau -- and CoulombAction's long-range utau of the slices
+// (CoulombAction.cc:164-170).
+template <int D>
+__device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, const PairMap& pm, const double* sl,
+                                            const double* tab, const int* utoff, const TableP* tbs, int t0,
+                                            int t1) {
+  const int N = c.N, T = t1 - t0;
+  const int nd = N * D;
+  double acc = 0;
+  // spring and SHO: lanes over (slice, particle)
+  if (c.spring || c.nsho) {
+    for (int q = threadIdx.x; q < T * N; q += blockDim.x) {
+      const int k = q / N, i = q - k * N;
+      const double* b1 = sl + (k + 1) * nd + i * D;
+      const double* b0 = sl + k * nd + i * D;
+      const int sp = species_of(c, i);
+      double delta[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = b1[d] - b0[d];
+      pbc<D>(c, delta);
+      if (c.spring && !c.stat[sp]) {
+        double dd = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) dd += delta[d] * delta[d];
+        acc += D / (2 * c.tau) - dd / (4.0 * c.lambda[sp] * c.tau * c.tau);
+      }
+      for (int a = 0; a < c.nsho; ++a) {
+        const int ssp = c.sho_sp[a];
+        if (i < c.ifirst[ssp] || i >= c.ifirst[ssp] + c.count[ssp]) continue;
+        const int ndim = c.sho_ndim[a] ? c.sho_ndim[a] : D;
+        const double omega = c.sho_omega[a], m = c.mass[ssp];
+        const double wt = omega * c.tau, sinhwt = sinh(wt), coshwt = cosh(wt);
+        const double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / c.tau;
+        double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
+        for (int d = D - ndim; d < D; ++d) {
+          const double r1 = b1[d] - c.sho_center[a][d], r0 = b0[d] - c.sho_center[a][d];
+          r0r0 += r0 * r0; r0r1 += r0 * r1; r1r1 += r1 * r1; delta2 += delta[d] * delta[d];
+        }
+        acc += 0.5 * ndim * omega * cothwt
+               + 0.5 * m * omega * omega * ((r1r1 + r0r0) * (1 - cothwt * cothwt) + 2.0 * r0r1 * coshwt * cschwt * cschwt)
+               + 0.5 * m * delta2 * tin * tin - 0.5 * ndim * tin;
+      }
+    }
+  }
+  // pair actions: lanes over (slice, unordered pair)
+  if (c.npair) {
+    const int P = N * (N - 1) / 2;
+    for (int q = threadIdx.x; q < T * P; q += blockDim.x) {
+      const int k = q / P;
+      int i, j;
+      pair_of(N, q - k * P, i, j);
+      const int si = species_of(c, i), sj = species_of(c, j);
+      const int slot = pm.slot[si][sj];
+      if (slot < 0) continue;
+      const TableP& tb = tbs[slot];
+      const double* u = tab + utoff[slot];
+      const double* a1 = sl + (k + 1) * nd + i * D;
+      const double* b1 = sl + (k + 1) * nd + j * D;
+      const double* a0 = sl + k * nd + i * D;
+      const double* b0 = sl + k * nd + j * D;
+      double dl[D], pd[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) { dl[d] = a1[d] - b1[d]; pd[d] = a0[d] - b0[d]; }
+      pbc<D>(c, dl);
+      pbc<D>(c, pd);
+      double r = 0, pr = 0, s2 = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) { r += dl[d] * dl[d]; pr += pd[d] * pd[d]; }
+      r = sqrt(r);
+      pr = sqrt(pr);
+      const double qm = 0.5 * (r + pr);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { const double sv = dl[d] - pd[d]; s2 += sv * sv; }
+      s2 /= qm * qm;
+      const double qq = qm * tb.rgridinv;
+      double xx = log(qq) * tb.logrratioinv;
+      int ig = (int)xx;
+      if (qq < 1) { ig = 0; xx = 0; }
+      else if (ig > tb.ngpts - 2) { ig = tb.ngpts - 2; xx = 1; }
+      else xx -= ig;
+      double vt = 0;
+      const double* u0 = u + ig * tb.K1;
+      for (int kk = tb.K1 - 1; kk >= 0; --kk) {
+        vt *= s2;
+        vt += (1 - xx) * u0[kk] + xx * u0[tb.K1 + kk];
+      }
+      acc += vt;
+    }
+  }
+  // Ewald k-space utau of every slice (particle 0 carries it): lanes over (slice, k)
+  if (c.ew.nk) {
+    const double* kp = x.ewk + c.ew.off;
+    double lr = 0;
+    for (int q = threadIdx.x; q < T * c.ew.nk; q += blockDim.x) {
+      const int k = q / c.ew.nk;
+      const double* kv = kp + 4 * (q - k * c.ew.nk);
+      const double* s = sl + (k + 1) * nd;
+      double rr = 0, ri = 0;
+      for (int jj = 0; jj < N; ++jj) {
+        double co, si;
+        ewald_phase<D>(kv, s + jj * D, co, si);
+        const double qj = c.charge[species_of(c, jj)];
+        rr += qj * co;
+        ri += qj * si;
+      }
+      lr += kv[3] * (rr * rr + ri * ri);
+    }
+    if (threadIdx.x == 0) lr += T * c.ew.self;
+    acc += lr / c.ceps;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if (lane_id() == 0) red[threadIdx.x / PQ_WAVE] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x / PQ_WAVE); ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+// LDS budget of the utau tables staged per item; larger table sets are read
+// from global memory (L2)
+constexpr int kTabLdsDoubles = 4096;
+
+template <int D>
+__global__ __launch_bounds__(kMeasThreads) void energy_items_kernel(DevCtx x, const MeasItem* items, int nitems,
+                                                                     double* part) {
+  extern __shared__ double lds[];
+  __shared__ ChainP c;
+  __shared__ PairMap pm;
+  __shared__ int utoff[PQ_MAX_PAIRS];
+  __shared__ TableP tbs[PQ_MAX_PAIRS];
+  __shared__ int in_lds;
+  __shared__ double red[kMeasThreads / PQ_WAVE];
+  int cur = -1;
+  double* tab = lds;
+  double* sl = lds + kTabLdsDoubles;
+  // contiguous runs of items per workgroup: a workgroup meets few chains and
+  // stages each chain's parameters and tables once
+  const int run = (nitems + gridDim.x - 1) / gridDim.x;
+  const int it_end = min(nitems, (int)(blockIdx.x + 1) * run);
+  for (int it = blockIdx.x * run; it < it_end; ++it) {
+    const MeasItem m = items[it];
+    if (m.chain != cur) {
+      __syncthreads();
+      copy_chain(&c, &x.chains[m.chain]);
+      if (threadIdx.x == 0) {
+        for (int a = 0; a < PQ_MAX_SPECIES; ++a)
+          for (int b = 0; b < PQ_MAX_SPECIES; ++b) pm.slot[a][b] = -1;
+        // the chain's distinct utau tables, packed; staged in LDS when they fit
+        int used = 0;
+        for (int pp = 0; pp < c.npair; ++pp) {
+          pm.slot[c.pair[pp].s1][c.pair[pp].s2] = pp;
+          pm.slot[c.pair[pp].s2][c.pair[pp].s1] = pp;
+          tbs[pp] = x.tables[c.pair[pp].table];
+          int q = 0;
+          while (q < pp && c.pair[q].table != c.pair[pp].table) ++q;
+          if (q < pp) { utoff[pp] = utoff[q]; continue; }
+          const TableP& tb = x.tables[c.pair[pp].table];
+          utoff[pp] = used;
+          used += tb.ngpts * tb.K1;
+        }
+        in_lds = used <= kTabLdsDoubles;
+        if (!in_lds)
+          for (int pp = 0; pp < c.npair; ++pp) utoff[pp] = (int)x.tables[c.pair[pp].table].off;
+      }
+      __syncthreads();
+      if (in_lds)
+        for (int pp = 0; pp < c.npair; ++pp) {
+          int q = 0;
+          while (q < pp && c.pair[q].table != c.pair[pp].table) ++q;
+          if (q < pp) continue;
+          const TableP& tb = x.tables[c.pair[pp].table];
+          const double* src = x.tab_ut + tb.off;
+          for (int e = threadIdx.x; e < tb.ngpts * tb.K1; e += blockDim.x) tab[utoff[pp] + e] = src[e];
+        }
+      cur = m.chain;
+    }
+    __syncthreads();
+    stage_slices<D>(x, c, m.t0, m.t1, sl);
+    __syncthreads();
+    const double v = in_lds ? item_utau<D>(x, c, pm, sl, tab, utoff, tbs, m.t0, m.t1)
+                            : item_utau<D>(x, c, pm, sl, x.tab_ut, utoff, tbs, m.t0, m.t1);
+    const double s = block_sum(v, red);
+    if (threadIdx.x == 0) part[it] = s;
+  }
+}
+
+// E = (1/M) sum over a chain's items in order (SimpleScalarAccumulator's
+// storeValue divides by the slice count)
+__global__ void energy_reduce_kernel(const DevCtx x, const int* first_item, int nchain, const double* part,
+                                     double* energy) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchain) return;
+  double s = 0;
+  for (int it = first_item[c]; it < first_item[c + 1]; ++it) s += part[it];
+  energy[c] = s / x.chains[c].M;
+}
+
+// SuperCell::pbc without contraction: the bins of the histogram are those of
+// the FP64 expression evaluated operation by operation
+template <int D>
+__device__ __forceinline__ void pbc_exact(const ChainP& c, double* v) {
+#pragma clang fp contract(off)
+  double r2 = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) r2 += v[d] * v[d];
+  if (r2 > c.rcut2) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double i = floor(c.b[d] * v[d] + 1000.5) - 1000;
+      v[d] -= c.a[d] * i;
+    }
+  }
+}
+
+// PairCFEstimator<1>::handleLink with PairRadial (PairCFEstimator.h:72-93,
+// PairDistance.h:22-39) on slices t0..t1-1: particle i of species sp[0]
+// against every j != i of each partner species sp[1..nsp-1] in list order.
+template <int D>
+__global__ __launch_bounds__(kMeasThreads) void pair_cf_items_kernel(DevCtx x, const MeasItem* items, int nitems,
+                                                                      PairCfArgs g, uint64_t* counts) {
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];
+  __shared__ ChainP c;
+  __shared__ int jfirst[PQ_MAX_CF_SPECIES], jcount[PQ_MAX_CF_SPECIES];
+  __shared__ int njtot, if1, n1, valid;
+  unsigned int* hist = reinterpret_cast<unsigned int*>(lds);
+  double* sl = lds + (g.nbin + 1) / 2;
+  int cur = -1;
+  // contiguous runs of items per workgroup: a workgroup meets few chains and
+  // stages each chain's parameters and tables once
+  const int run = (nitems + gridDim.x - 1) / gridDim.x;
+  const int it_end = min(nitems, (int)(blockIdx.x + 1) * run);
+  for (int it = blockIdx.x * run; it < it_end; ++it) {
+    const MeasItem m = items[it];
+    if (m.chain != cur) {
+      __syncthreads();
+      copy_chain(&c, &x.chains[m.chain]);
+      if (threadIdx.x == 0) {
+        valid = g.sp[0] < c.nsp;
+        int n = 0;
+        for (int k = 1; k < g.nsp; ++k) {
+          const bool ok = g.sp[k] < c.nsp;
+          valid &= ok;
+          jfirst[k] = ok ? c.ifirst[g.sp[k]] : 0;
+          jcount[k] = ok ? c.count[g.sp[k]] : 0;
+          n += jcount[k];
+        }
+        njtot = n;
+        if1 = valid ? c.ifirst[g.sp[0]] : 0;
+        n1 = valid ? c.count[g.sp[0]] : 0;
+      }
+      __syncthreads();
+      cur = m.chain;
+    }
+    if (!valid || njtot == 0) continue;
+    for (int b = threadIdx.x; b < g.nbin; b += blockDim.x) hist[b] = 0;
+    stage_slices<D>(x, c, m.t0, m.t1, sl);
+    __syncthreads();
+    const int nd = c.N * D, per = n1 * njtot, T = m.t1 - m.t0;
+    for (int q = threadIdx.x; q < T * per; q += blockDim.x) {
+      const int k = q / per, r = q - k * per;
+      const int i = if1 + r / njtot;
+      int rr = r - (r / njtot) * njtot, kk = 1;
+      while (rr >= jcount[kk]) { rr -= jcount[kk]; ++kk; }
+      const int j = jfirst[kk] + rr;
+      if (i == j) continue;
+      const double* a = sl + (k + 1) * nd + i * D;
+      const double* b = sl + (k + 1) * nd + j * D;
+      double delta[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
+      pbc_exact<D>(c, delta);
+      double r2 = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) r2 += delta[d] * delta[d] * 1.0;
+      const double dd = sqrt(r2);
+      const int ibin = (int)((dd - g.rmin) * g.deltaInv);
+      if (dd < g.rmin || ibin > g.nbin - 1) continue;
+      atomicAdd(&hist[ibin], 1u);
+    }
+    __syncthreads();
+    uint64_t* out = counts + (int64_t)m.chain * g.nbin;
+    for (int b = threadIdx.x; b < g.nbin; b += blockDim.x)
+      if (hist[b]) atomicAdd(reinterpret_cast<unsigned long long*>(out + b), (unsigned long long)hist[b]);
+    __syncthreads();
+  }
+}
+
+__global__ void fill_u64_kernel(uint64_t* p, int64_t n) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    p[q] = 0;
+}
+
+// three 48 KB workgroups fit a CU's LDS: four rounds of them over 256 CUs
+static int meas_grid(int nitems) { return nitems < 3072 ? (nitems > 0 ? nitems : 1) : 3072; }
+
+hipError_t launch_energy_items(int D, const DevCtx& x, const MeasItem* items, int nitems, const int* first_item,
+                               int nchain, int max_slice_doubles, double* part, double* energy, hipStream_t st) {
+  const size_t lds = sizeof(double) * ((size_t)kTabLdsDoubles + max_slice_doubles);
+  const int grid = meas_grid(nitems);
+  if (D == 2)
+    hipLaunchKernelGGL(energy_items_kernel<2>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, part);
+  else
+    hipLaunchKernelGGL(energy_items_kernel<3>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(energy_reduce_kernel, dim3((nchain + 127) / 128), dim3(128), 0, st, x, first_item, nchain,
+                     part, energy);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_cf_items(int D, const DevCtx& x, const MeasItem* items, int nitems, int max_slice_doubles,
+                                const PairCfArgs& g, uint64_t* counts, hipStream_t st) {
+  const size_t lds = sizeof(double) * ((size_t)(g.nbin + 1) / 2 + max_slice_doubles);
+  const int grid = meas_grid(nitems);
+  if (D == 2)
+    hipLaunchKernelGGL(pair_cf_items_kernel<2>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, g, counts);
+  else
+    hipLaunchKernelGGL(pair_cf_items_kernel<3>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, g, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* p, int64_t n, hipStream_t st) {
+  const int64_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)(nb < 4096 ? (nb > 0 ? nb : 1) : 4096)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace pq

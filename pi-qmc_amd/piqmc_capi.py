@@ -16,7 +16,7 @@ PQ_MAX_SPECIES = 4
 PQ_MAX_SHO = 4
 PQ_MAX_NODES = 4
 PQ_MAX_SAMPLES = 8
-PQ_MAX_MOVING = 4
+PQ_MAX_MOVING = 8
 PQ_MAX_NODE_PART = 32
 PQ_MOVER_FREE = 0
 PQ_MOVER_FREE_PBC = 1
@@ -313,6 +313,11 @@ def load_library(path=LIB_PATH):
                                                            C.c_double, dp]),
         "pq_pair_histogram": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_int, P(C.c_uint64)]),
+        "pq_pair_cf_histogram": (C.c_int, [C.c_void_p, C.c_int, ip, C.c_double, C.c_double,
+                                           C.c_int, P(C.c_uint64)]),
+        "pq_set_measure_pair_cf": (C.c_int, [C.c_void_p, C.c_int, ip, C.c_double, C.c_double, C.c_int]),
+        "pq_collect_pair_cf": (C.c_int, [C.c_void_p, P(C.c_uint64)]),
+        "pq_run_measured_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, P(C.c_float)]),
         "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
         "pq_bead_utau": (C.c_int, [C.c_void_p, C.c_int, ip, ip, ip, dp]),
         "pq_section_begin": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
@@ -474,6 +479,32 @@ class Batch:
         check(self.lib.pq_pair_histogram(self.h, s1, s2, rmin, rmax, nbin,
                                          h.ctypes.data_as(C.POINTER(C.c_uint64))))
         return h
+
+    def pair_cf_histogram(self, species, rmin, rmax, nbin):
+        """species[0] against the partner species species[1:]: counts [nchain][nbin]."""
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        h = np.zeros((len(self.descs), nbin), dtype=np.uint64)
+        check(self.lib.pq_pair_cf_histogram(self.h, len(sp), iptr(sp), rmin, rmax, nbin,
+                                            h.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return h
+
+    def set_measure_pair_cf(self, species, rmin=0.0, rmax=1.0, nbin=1):
+        """Pair-correlation estimator of every run_measured Measure (species [] removes it)."""
+        sp = np.ascontiguousarray(species if len(species) else [0], dtype=np.int32)
+        check(self.lib.pq_set_measure_pair_cf(self.h, len(species), iptr(sp), rmin, rmax, nbin))
+        self._cf_nbin = nbin if len(species) else 0
+
+    def collect_pair_cf(self):
+        h = np.zeros((len(self.descs), self._cf_nbin), dtype=np.uint64)
+        check(self.lib.pq_collect_pair_cf(self.h, h.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return h
+
+    def run_measured_timed(self, nblock, nsweep):
+        """run_measured with each block's sampler-launch time (ms) from events in the graph."""
+        e = np.empty((nblock, len(self.descs)))
+        ms = (C.c_float * nblock)()
+        check(self.lib.pq_run_measured_timed(self.h, nblock, nsweep, dptr(e), ms))
+        return e, np.array(ms[:], dtype=np.float64)
 
     def ewald_coulomb_energy(self, epsilon, rcut, kcut, trad=False, kappa=0.0):
         e = np.empty(len(self.descs))
