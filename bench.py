@@ -118,13 +118,19 @@ def all_paths(b):
     return arr
 
 
-def identical_chains(bm, br):
-    """Chains whose final paths are bitwise the same in both batches."""
+def identical_chains(bm, br, em, er):
+    """Chains whose energies were bitwise equal at every measured sweep and
+    whose final paths are bitwise equal in both batches: their decision
+    streams never differed over the measurement.  (Equal final paths alone do
+    not show it: a decision flipped at level 0 draws as many uniforms as the
+    other outcome, so the streams stay aligned and a later accepted move over
+    the same slices can make the paths equal again.)"""
     pm, pr = all_paths(bm), all_paths(br)
     same, off = [], 0
-    for d in bm.descs:
+    for c, d in enumerate(bm.descs):
         n = d.nbeads()
-        same.append(bool(np.array_equal(pm[off:off + n], pr[off:off + n])))
+        same.append(bool(np.array_equal(pm[off:off + n], pr[off:off + n])) and
+                    bool(np.array_equal(em[:, c], er[:, c])))
         off += n
     return np.array(same)
 
@@ -144,7 +150,7 @@ def run_gpu(descs, precision, steps, warmup, seed, steady, autotune=True, schedu
     if schedule is not None:
         b.set_schedule(schedule[0], schedule[1])
         b.set_staging(schedule[2], schedule[3])
-        if len(schedule) > 4:
+        if len(schedule) > 4 and schedule[4] != 0:
             b.set_split(schedule[4], schedule[5])
     elif autotune:
         b.run_sweeps(TUNE_AFTER)
@@ -197,8 +203,8 @@ def paired_energy(bm, br, em, er, label):
     and standard error), and the chains whose paths stayed bitwise identical
     (both precisions move beads in FP64 and estimate in FP64, so a chain whose
     decisions never differed has dE = 0 exactly)."""
-    same = identical_chains(bm, br)
     em, er = np.asarray(em), np.asarray(er)
+    same = identical_chains(bm, br, em, er)
     de = em.mean(axis=0) - er.mean(axis=0)
     n = de.size
     err = float(de.std(ddof=1) / math.sqrt(n)) if n > 1 else None
@@ -249,8 +255,10 @@ def cpu_model():
 
 def cpu_baseline(descs, budget_s=12.0, single_s=8.0):
     """The oracle (FP64 C restatement) on host cores: a bounded sample of the
-    workload's chains, one chain per thread (OpenMP), on all cores and then on
-    one core.  The sampler only: no Measure, which favours the CPU."""
+    workload's chains spread evenly over the batch, one chain per thread
+    (OpenMP), every sweep followed by the same Measure as the device's
+    (thermal energy and the pair-correlation counts), on all cores and on
+    one core."""
     import oracle_lib as ol
     lib = ol.load()
     try:
@@ -258,32 +266,35 @@ def cpu_baseline(descs, budget_s=12.0, single_s=8.0):
     except AttributeError:
         cores = os.cpu_count() or 1
     nch = min(len(descs), 2 * cores)
-    sample = descs[:nch]
-    arr = capi.descs_array(sample)
-    beads = np.concatenate([capi.initial_beads(d, 1000 + c).ravel() for c, d in enumerate(sample)])
-    t0 = time.perf_counter()
-    lib.oq_run_batch(arr, nch, capi.dptr(beads), 1, 7, 0, cores, None)
-    t1 = time.perf_counter() - t0
+    stride = max(1, len(descs) // nch)
+    sample = [descs[k * stride] for k in range(nch)]
+    sp = np.ascontiguousarray(PAIR_CF[0], dtype=np.int32)
+
+    def run(chains, nsweep, threads):
+        arr = capi.descs_array(chains)
+        beads = np.concatenate([capi.initial_beads(d, 1000 + c).ravel() for c, d in enumerate(chains)])
+        es = np.zeros(len(chains))
+        counts = np.zeros(len(chains) * PAIR_CF[3], dtype=np.uint64)
+        t0 = time.perf_counter()
+        bu = lib.oq_run_batch_measured(arr, len(chains), capi.dptr(beads), nsweep, 7, 0, threads, len(sp),
+                                       capi.iptr(sp), PAIR_CF[1], PAIR_CF[2], PAIR_CF[3], capi.dptr(es),
+                                       counts.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return bu, time.perf_counter() - t0
+
+    _, t1 = run(sample, 1, cores)
     nsweep = max(1, min(1000, int(budget_s / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    bu = lib.oq_run_batch(arr, nch, capi.dptr(beads), nsweep, 7, 0, cores, None)
-    dt = time.perf_counter() - t0
-    # one core: the same chains, one at a time, for about single_s seconds
-    one = capi.descs_array(sample[:1])
-    b1 = capi.initial_beads(sample[0], 1000).ravel().copy()
-    t0 = time.perf_counter()
-    lib.oq_run_batch(one, 1, capi.dptr(b1), 1, 7, 0, 1, None)
-    t1 = time.perf_counter() - t0
+    bu, dt = run(sample, nsweep, cores)
+    one = sample[::4]
+    _, t1 = run(one, 1, 1)
     ns1 = max(1, min(1000, int(single_s / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    bu1 = lib.oq_run_batch(one, 1, capi.dptr(b1), ns1, 7, 0, 1, None)
-    dt1 = time.perf_counter() - t0
+    bu1, dt1 = run(one, ns1, 1)
     return dict(value=bu / dt, unit="bead-updates/s", cores=cores, kind="port",
-                sample=f"{nch} chains of the workload x {nsweep} sweeps, oracle (FP64 C, -O3 -march=x86-64-v3, "
-                       f"OpenMP {cores} threads), {dt:.1f} s; sampler only, no Measure",
+                sample=f"{nch} chains spread over the workload (every {stride}th) x {nsweep} sweeps, each sweep "
+                       f"followed by the Measure (thermal energy, 256-bin g(r)); oracle (FP64 C, -O3 "
+                       f"-march=x86-64-v3, OpenMP {cores} threads), {dt:.1f} s",
                 cpu_model=cpu_model(), nproc=os.cpu_count(),
-                single_core={"value": bu1 / dt1, "sample": f"chain 0 of the workload x {ns1} sweeps on one "
-                                                            f"thread, {dt1:.1f} s"})
+                single_core={"value": bu1 / dt1, "sample": f"{len(one)} of those chains x {ns1} sweeps with the "
+                                                            f"Measure on one thread, {dt1:.1f} s"})
 
 
 def main():
@@ -354,10 +365,13 @@ def main():
     }
     if not args.no_fp64_pair and prec == capi.PQ_PREC_MIXED:
         # the same sweeps (the mixed run's tuning sweeps are matched by plain
-        # sweeps) on the library's default schedule: schedules change where data
-        # lives, not the decisions
+        # sweeps) with the mixed run's team size and launch split, so that both
+        # runs evaluate node matrices with the same Gauss-Jordan instance (LDS
+        # placement is left automatic: it moves data, not arithmetic)
+        sc = run["schedule"]
         b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed,
-                           run["warmup_done"] - args.warmup, autotune=False, pair_cf=PAIR_CF)
+                           run["warmup_done"] - args.warmup, pair_cf=PAIR_CF,
+                           schedule=(sc["team"], -1, -1, -1, sc["heavy_chains"], sc["heavy_team"]))
         out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
                                "kernel_ms": float(np.mean(ref["sweep_ms"]))}
         out["energy_vs_fp64"] = paired_energy(b_main, b64, run["energies"], ref["energies"],

@@ -188,6 +188,9 @@ typedef struct pq_chain_stats {
   uint64_t tries[8];             /* per level: moves that reached this level */
   uint64_t accepts[8];           /* per level: moves accepted at this level */
   uint64_t node_errors;          /* crossings or singular matrices met in initialize */
+  uint64_t sweeps;               /* sweeps run */
+  uint64_t t_start, t_end;       /* device wall clock (pq_clock_hz ticks per second) when the
+                                    chain's workgroup of the last sampler launch started and ended */
 } pq_chain_stats;
 
 typedef struct pq_batch pq_batch;
@@ -299,6 +302,17 @@ int pq_get_split(const pq_batch* batch, int* heavy_chains, int* heavy_team);
    to rounding.  Replaces nothing in the reference (it has no device
    schedule). */
 int pq_autotune(pq_batch* batch, int nsweep, float* best_ms, int* ntimed);
+/* Sweeps per launch of each chain: chain c runs nsweep * sweeps[c] sweeps
+   in every sampler launch (pq_run_sweeps, pq_run_measured and the rest),
+   1 <= sweeps[c] <= 4096; a null array gives every chain one.  Chains are
+   independent Markov chains, so a chain's decisions and paths after n sweeps
+   do not depend on the plan; a plan that gives light chains more sweeps
+   keeps the device busy while the heaviest chains run.  No counterpart in
+   the reference (its clones run at their own pace in separate processes). */
+int pq_set_sweep_plan(pq_batch* batch, const int* sweeps);
+int pq_get_sweep_plan(const pq_batch* batch, int* sweeps);
+/* Ticks per second of the device clock of pq_chain_stats.t_start/t_end. */
+int pq_clock_hz(const pq_batch* batch, double* hz);
 int pq_get_stats(pq_batch* batch, int chain, pq_chain_stats* out);
 int pq_get_all_stats(pq_batch* batch, pq_chain_stats* out);
 int pq_reset_stats(pq_batch* batch);
@@ -380,6 +394,16 @@ int pq_action_difference(pq_batch* batch, int nquery, const int* chain,
                          const int* level, const double* section,
                          const double* moving, const int* index, int nmov,
                          double* dS);
+/* Action::getBeadAction (src/action/Action.h:53-54) of the chain's
+   CompositeAction (CompositeAction.cc:57-67: SpringAction, SHOAction,
+   CoulombAction's pair actions and its long-range Ewald part) for nquery
+   beads (ipart[q], islice[q]), 0 <= islice < M, of caller-supplied paths:
+   beads [M][N][D] with perm their path permutation (null: the identity), or,
+   with beads and perm both null, the chain's device-resident paths.
+   out[q][3 + 2D] = u, utau, ulambda (0 for these actions), fm[D] (the force
+   from the link into slice islice), fp[D] (from the link out of it). */
+int pq_bead_action(pq_batch* batch, int chain, const double* beads, const int* perm,
+                   int nquery, const int* ipart, const int* islice, double* out);
 /* Per-link utau of CompositeAction::getBeadAction for a set of (chain, ipart,
    islice) on the current device paths. */
 int pq_bead_utau(pq_batch* batch, int nquery, const int* chain,

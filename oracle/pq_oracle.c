@@ -2183,6 +2183,7 @@ void oq_run_sweeps(oq_chain* ch, int nsweep) {
     }
   set_group(ch, 0);
   ch->st.draws = ch->rng.count;
+  ch->st.sweeps += (uint64_t)nsweep;
 }
 
 void oq_get_stats(const oq_chain* ch, pq_chain_stats* st) {
@@ -2534,6 +2535,44 @@ uint64_t oq_run_batch(const pq_chain_desc* descs, int nchain, double* beads,
     oq_run_sweeps(ch, nsweep);
     total += ch->st.bead_updates;
     if (energy) energy[c] = oq_thermo_energy(ch);
+    oq_get_beads(ch, beads + off[c]);
+    oq_destroy(ch);
+  }
+  free(off);
+  return total;
+}
+
+/* oq_run_batch with a Measure after every sweep: the thermal energy (summed
+   into energy_sum[c]) and the pair-correlation counts of species[0] against
+   species[1..nspec-1] (added to counts[c][nbin]) */
+uint64_t oq_run_batch_measured(const pq_chain_desc* descs, int nchain, double* beads, int nsweep,
+                               uint64_t seed, uint32_t first_stream, int nthreads, int nspec,
+                               const int* species, double rmin, double rmax, int nbin,
+                               double* energy_sum, uint64_t* counts) {
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (nchain + 1));
+  off[0] = 0;
+  for (int c = 0; c < nchain; ++c) {
+    int n = 0;
+    for (int s = 0; s < descs[c].nspecies; ++s) n += descs[c].species[s].count;
+    off[c + 1] = off[c] + (int64_t)descs[c].nslice * n * descs[c].ndim;
+  }
+  uint64_t total = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+#endif
+  for (int c = 0; c < nchain; ++c) {
+    oq_chain* ch = oq_create(&descs[c]);
+    oq_set_beads(ch, beads + off[c]);
+    oq_rng_philox(ch, seed, first_stream + (uint32_t)c);
+    double e = 0;
+    for (int k = 0; k < nsweep; ++k) {
+      oq_run_sweeps(ch, 1);
+      e += oq_thermo_energy(ch);
+      if (nspec > 0 && counts) oq_pair_cf_histogram(ch, nspec, species, rmin, rmax, nbin, counts + (size_t)c * nbin);
+    }
+    total += ch->st.bead_updates;
+    if (energy_sum) energy_sum[c] = e;
     oq_get_beads(ch, beads + off[c]);
     oq_destroy(ch);
   }

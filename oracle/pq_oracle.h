@@ -108,6 +108,14 @@ uint64_t oq_run_batch(const pq_chain_desc* descs, int nchain, double* beads,
                       int nsweep, uint64_t seed, uint32_t first_stream,
                       int nthreads, double* energy);
 
+/* the same with a Measure after every sweep: thermal energy summed per chain
+   and pair-correlation counts [nchain][nbin] of species[0] against
+   species[1..nspec-1] */
+uint64_t oq_run_batch_measured(const pq_chain_desc* descs, int nchain, double* beads, int nsweep,
+                               uint64_t seed, uint32_t first_stream, int nthreads, int nspec,
+                               const int* species, double rmin, double rmax, int nbin,
+                               double* energy_sum, uint64_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -223,6 +223,7 @@ struct pq_batch {
   double* dinj = nullptr;
   int64_t inj_per_chain = 0;
   int* dorder = nullptr;
+  int* dplan = nullptr;         // sweeps per launch of each chain, null: one
   int* dperm = nullptr;         // per-chain path permutation and its inverse
   int64_t nperm = 0;
   int rng_mode = PQ_RNG_PHILOX;
@@ -233,6 +234,8 @@ struct pq_batch {
   size_t scr_bytes = 0;
   double* dfnscr = nullptr;   // fixed-node protocol buffers, kept between calls
   size_t fnscr_bytes = 0;
+  double* dbead_stage = nullptr; // caller paths of pq_bead_action, then their permutation
+  size_t bead_stage_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // measured-sweep graph: nblock x (sweeps, energy) captured once, replayed
   hipGraph_t graph = nullptr;
@@ -283,6 +286,7 @@ static DevCtx make_ctx(const pq_batch* b) {
   x.order = b->dorder;
   x.perm = b->dperm;
   x.ewk = b->dewk;
+  x.plan = b->dplan;
   return x;
 }
 
@@ -697,10 +701,11 @@ static hipError_t run_launch(pq_batch* b, const DevCtx& x, int nsweep) {
 }
 
 // Measure work items.  An energy item costs about N spring/SHO links plus
-// N(N-1)/2 pair links per slice; items hold ~4096 of those (16 per thread of
-// a 256-thread workgroup) and at most 2048 staged doubles.  Histogram items
-// are coarser (~64k distances) so that each adds its bins to the chain's
-// global counts once.
+// N(N-1)/2 pair links per slice; items hold up to ~16k of those (64 per
+// thread of a 256-thread workgroup, and always fewer than 2^16 (slice,
+// pair) entries) and at most 2048 staged doubles.  Histogram items are
+// coarser (~64k distances) so that each adds its bins to the chain's global
+// counts once.
 static void build_measure_items(pq_batch* b) {
   b->items_e.clear();
   b->items_g.clear();
@@ -710,7 +715,7 @@ static void build_measure_items(pq_batch* b) {
     const ChainP& ch = b->hc[c];
     const int64_t nd = (int64_t)ch.N * ch.D;
     const int64_t per = ch.N + (ch.npair ? (int64_t)ch.N * (ch.N - 1) / 2 : 0) + (ch.ew.nk ? (int64_t)ch.ew.nk * ch.N / 4 : 0);
-    int64_t te = std::max<int64_t>(1, 4096 / std::max<int64_t>(1, per));
+    int64_t te = std::max<int64_t>(1, 16384 / std::max<int64_t>(1, per));
     te = std::min<int64_t>(te, std::max<int64_t>(1, 2048 / nd - 1));
     te = std::min<int64_t>(te, ch.M);
     for (int t = 0; t < ch.M; t += (int)te) b->items_e.push_back(MeasItem{c, t, (int)std::min<int64_t>(ch.M, t + te)});
@@ -888,7 +893,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache, b->dfnsweep,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr,
                   b->dperm, b->dewk, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est,
-                  b->ditems_e, b->ditems_g, b->dfirst_e, b->dpart, b->dcf};
+                  b->ditems_e, b->ditems_g, b->dfirst_e, b->dpart, b->dcf, b->dplan, b->dbead_stage};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   drop_graph(b);
@@ -1422,6 +1427,48 @@ extern "C" int pq_sync(pq_batch* b) {
   return check_stream(b);
 }
 
+extern "C" int pq_set_sweep_plan(pq_batch* b, const int* sweeps) {
+  if (!b) return fail(PQ_ERR_ARG, "null batch");
+  if (sweeps)
+    for (int c = 0; c < b->nchain; ++c)
+      if (sweeps[c] < 1 || sweeps[c] > 4096) return fail(PQ_ERR_ARG, "sweep plan of chain %d must be in 1..4096", c);
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (!sweeps) {
+    // the captured graphs hold the plan's address
+    if (b->dplan) { drop_graph(b); (void)hipFree(b->dplan); }
+    b->dplan = nullptr;
+    return PQ_OK;
+  }
+  if (!b->dplan) {
+    drop_graph(b);
+    HIPCHK(hipMalloc(&b->dplan, sizeof(int) * b->nchain));
+  }
+  // the kernels read the plan at run time: captured graphs stay valid
+  HIPCHK(hipMemcpy(b->dplan, sweeps, sizeof(int) * b->nchain, hipMemcpyHostToDevice));
+  return PQ_OK;
+}
+
+extern "C" int pq_get_sweep_plan(const pq_batch* b, int* sweeps) {
+  if (!b || !sweeps) return fail(PQ_ERR_ARG, "pq_get_sweep_plan: bad arguments");
+  if (!b->dplan) {
+    for (int c = 0; c < b->nchain; ++c) sweeps[c] = 1;
+    return PQ_OK;
+  }
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(sweeps, b->dplan, sizeof(int) * b->nchain, hipMemcpyDeviceToHost));
+  return PQ_OK;
+}
+
+extern "C" int pq_clock_hz(const pq_batch* b, double* hz) {
+  if (!b || !hz) return fail(PQ_ERR_ARG, "pq_clock_hz: bad arguments");
+  int khz = 0;
+  HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+  *hz = 1e3 * khz;
+  return PQ_OK;
+}
+
 extern "C" int pq_get_stats(pq_batch* b, int c, pq_chain_stats* out) {
   if (int rc = check_chain(b, c)) return rc;
   if (!out) return fail(PQ_ERR_ARG, "null stats");
@@ -1685,7 +1732,7 @@ static int run_measured(pq_batch* b, int nblock, int nsweep, double* energies, f
       (uint64_t)x.rng_mode, x.seed, (uint64_t)x.first_stream, (uint64_t)(uintptr_t)x.inv,
       (uint64_t)(uintptr_t)x.fncache, (uint64_t)(uintptr_t)x.fnsweep, (uint64_t)nblock, (uint64_t)nsweep,
       (uint64_t)b->sched.heavy, (uint64_t)b->sched.ncsplit, (uint64_t)(uintptr_t)b->dcf,
-      (uint64_t)b->cf.nsp, (uint64_t)b->cf.nbin, (uint64_t)(uintptr_t)b->dmeas};
+      (uint64_t)b->cf.nsp, (uint64_t)b->cf.nbin, (uint64_t)(uintptr_t)b->dmeas, (uint64_t)(uintptr_t)b->dplan};
   std::vector<char> key(sizeof(fields) + 2 * sizeof(SweepLayoutH) + sizeof(b->cf.sp) + 2 * sizeof(double));
   char* kp = key.data();
   std::memcpy(kp, fields, sizeof(fields)); kp += sizeof(fields);
@@ -1877,26 +1924,83 @@ extern "C" int pq_action_difference(pq_batch* b, int nq, const int* chain, const
   return PQ_OK;
 }
 
+// getBeadAction of nq beads of one chain on the paths at dbeads/dperm
+// (device pointers): out[nq][3 + 2D] on the host
+static int bead_action_dev(pq_batch* b, int c, const double* dbeads, const int* dperm, int nq, const int* ipart,
+                           const int* islice, double* out) {
+  const ChainP& ch = b->hc[c];
+  for (int q = 0; q < nq; ++q) {
+    if (ipart[q] < 0 || ipart[q] >= ch.N) return fail(PQ_ERR_ARG, "query %d: particle %d out of range", q, ipart[q]);
+    if (islice[q] < 0 || islice[q] >= ch.M) return fail(PQ_ERR_ARG, "query %d: slice %d out of range", q, islice[q]);
+  }
+  const int w = 3 + 2 * ch.D;
+  const size_t need = sizeof(double) * (size_t)nq * w + sizeof(int) * 2 * (size_t)nq + 64;
+  void* p;
+  if (int rc = scratch(b, need, &p)) return rc;
+  double* dout = (double*)p;
+  int* dq = (int*)(dout + (size_t)nq * w);
+  HIPCHK(hipMemcpyAsync(dq, ipart, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq + nq, islice, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_bead_action(b->D, x, c, dbeads, dperm, nq, dq, dq + nq, dout, b->stream));
+  HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)nq * w, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
+}
+
+extern "C" int pq_bead_action(pq_batch* b, int c, const double* beads, const int* perm, int nq, const int* ipart,
+                              const int* islice, double* out) {
+  if (int rc = check_chain(b, c)) return rc;
+  if (nq < 1 || !ipart || !islice || !out) return fail(PQ_ERR_ARG, "pq_bead_action: bad arguments");
+  const ChainP& ch = b->hc[c];
+  HIPCHK(hipSetDevice(b->device));
+  if (!beads) {
+    if (perm) return fail(PQ_ERR_ARG, "pq_bead_action: a permutation goes with caller beads (the device paths keep theirs)");
+    return bead_action_dev(b, c, b->dbeads + ch.bead_off, b->dperm + ch.perm_off, nq, ipart, islice, out);
+  }
+  // the caller's paths: staged in a buffer of their own, with the permutation and its inverse
+  const int64_t nb = (int64_t)ch.M * ch.N * ch.D;
+  std::vector<int> pi(2 * (size_t)ch.N);
+  for (int i = 0; i < ch.N; ++i) pi[i] = perm ? perm[i] : i;
+  for (int i = 0; i < ch.N; ++i) {
+    if (pi[i] < 0 || pi[i] >= ch.N) return fail(PQ_ERR_ARG, "pq_bead_action: perm is not a permutation");
+    pi[ch.N + pi[i]] = i + 1;
+  }
+  for (int i = 0; i < ch.N; ++i) {
+    if (pi[ch.N + i] == 0) return fail(PQ_ERR_ARG, "pq_bead_action: perm is not a permutation");
+    pi[ch.N + i] -= 1;
+  }
+  const size_t need = sizeof(double) * nb + sizeof(int) * pi.size() + 64;
+  if (b->bead_stage_bytes < need) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->dbead_stage) (void)hipFree(b->dbead_stage);
+    b->dbead_stage = nullptr;
+    b->bead_stage_bytes = 0;
+    HIPCHK(hipMalloc(&b->dbead_stage, need));
+    b->bead_stage_bytes = need;
+  }
+  int* dpi = reinterpret_cast<int*>(b->dbead_stage + nb);
+  HIPCHK(hipMemcpyAsync(b->dbead_stage, beads, sizeof(double) * nb, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dpi, pi.data(), sizeof(int) * pi.size(), hipMemcpyHostToDevice, b->stream));
+  return bead_action_dev(b, c, b->dbead_stage, dpi, nq, ipart, islice, out);
+}
+
 extern "C" int pq_bead_utau(pq_batch* b, int nq, const int* chain, const int* ipart, const int* islice,
                             double* utau) {
   if (!b || nq < 1 || !chain || !ipart || !islice || !utau) return fail(PQ_ERR_ARG, "pq_bead_utau: bad arguments");
-  for (int q = 0; q < nq; ++q) {
+  for (int q = 0; q < nq; ++q)
     if (int rc = check_chain(b, chain[q])) return rc;
-    const ChainP& ch = b->hc[chain[q]];
-    if (ipart[q] < 0 || ipart[q] >= ch.N) return fail(PQ_ERR_ARG, "query %d: particle out of range", q);
+  // one launch per run of queries on the same chain
+  std::vector<double> out;
+  for (int q0 = 0; q0 < nq;) {
+    int q1 = q0 + 1;
+    while (q1 < nq && chain[q1] == chain[q0]) ++q1;
+    const int c = chain[q0], w = 3 + 2 * b->hc[c].D;
+    out.assign((size_t)(q1 - q0) * w, 0.0);
+    if (int rc = pq_bead_action(b, c, nullptr, nullptr, q1 - q0, ipart + q0, islice + q0, out.data())) return rc;
+    for (int q = q0; q < q1; ++q) utau[q] = out[(size_t)(q - q0) * w + 1];
+    q0 = q1;
   }
-  HIPCHK(hipSetDevice(b->device));
-  void* p;
-  if (int rc = scratch(b, sizeof(double) * nq + sizeof(int) * 3 * nq + 64, &p)) return rc;
-  double* dout = (double*)p;
-  int* dq = (int*)(dout + nq);
-  HIPCHK(hipMemcpyAsync(dq, chain, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
-  HIPCHK(hipMemcpyAsync(dq + nq, ipart, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
-  HIPCHK(hipMemcpyAsync(dq + 2 * nq, islice, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
-  DevCtx x = make_ctx(b);
-  HIPCHK(launch_bead_utau(b->D, x, nq, dq, dq + nq, dq + 2 * nq, dout, b->stream));
-  HIPCHK(hipMemcpyAsync(utau, dout, sizeof(double) * nq, hipMemcpyDeviceToHost, b->stream));
-  HIPCHK(hipStreamSynchronize(b->stream));
   return PQ_OK;
 }
 

@@ -44,6 +44,7 @@ struct DevCtx {
   const int* order;       // workgroup -> chain
   int* perm;              // per-chain path permutation then its inverse, [2][N]
   const double* ewk;      // Ewald k-vector pool, [nk][4] per sum (EwP)
+  const int* plan;        // sweeps per launch of each chain times nsweep, null: 1
 };
 
 // ------------------------------------------------------------ wave ops

@@ -122,111 +122,6 @@ __global__ __launch_bounds__(PQ_WAVE) void node_eval_kernel(DevCtx x, int chain,
 
 // ------------------------------------------------------------ estimators
 
-// getBeadAction utau of spring, SHO and pair actions for bead (i,t) of a
-// chain, summed as CompositeAction::getBeadAction (CompositeAction.cc:57-67).
-// Slice -1 is slice M-1 of particle inversePermutation[i] (SerialPaths.cc:68-77).
-// With lr, particle 0 also carries CoulombAction's long-range Ewald utau of
-// slice t (CoulombAction.cc:164-170); the estimator kernels add that sum in
-// parallel instead.
-template <int D>
-__device__ double bead_utau(const DevCtx& x, const ChainP& c, const double* beads, int i, int t,
-                            bool lr = false) {
-  const int* perm = x.perm + c.perm_off;
-  const int* invperm = perm + c.N;
-  auto P = [&](int p, int tt) -> const double* {
-    if (tt < 0) { tt += c.M; p = invperm[p]; }
-    else if (tt >= c.M) { tt -= c.M; p = perm[p]; }
-    return beads + ((int64_t)tt * c.N + p) * D;
-  };
-  const int sp = species_of(c, i);
-  double utau = 0;
-  const double* b1 = P(i, t);
-  const double* b0 = P(i, t - 1);
-  if (c.spring && !c.stat[sp]) {
-    double delta[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) delta[d] = b1[d] - b0[d];
-    pbc<D>(c, delta);
-    double dd = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) dd += delta[d] * delta[d];
-    utau += D / (2 * c.tau) - dd / (4.0 * c.lambda[sp] * c.tau * c.tau);
-  }
-  for (int a = 0; a < c.nsho; ++a) {
-    const int ssp = c.sho_sp[a];
-    double ui = 0;
-    if (!(i < c.ifirst[ssp] || i >= c.ifirst[ssp] + c.count[ssp])) {
-      const int ndim = c.sho_ndim[a] ? c.sho_ndim[a] : D;
-      const double omega = c.sho_omega[a], m = c.mass[ssp];
-      const double wt = omega * c.tau, sinhwt = sinh(wt), coshwt = cosh(wt);
-      const double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / c.tau;
-      double delta[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) delta[d] = b1[d] - b0[d];
-      pbc<D>(c, delta);
-      double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
-      for (int d = D - ndim; d < D; ++d) {
-        double r1 = b1[d] - c.sho_center[a][d], r0 = b0[d] - c.sho_center[a][d];
-        r0r0 += r0 * r0; r0r1 += r0 * r1; r1r1 += r1 * r1; delta2 += delta[d] * delta[d];
-      }
-      ui = 0.5 * ndim * omega * cothwt
-           + 0.5 * m * omega * omega * ((r1r1 + r0r0) * (1 - cothwt * cothwt) + 2.0 * r0r1 * coshwt * cschwt * cschwt)
-           + 0.5 * m * delta2 * tin * tin - 0.5 * ndim * tin;
-    }
-    utau += ui;
-  }
-  double uc = 0;
-  for (int pp = 0; pp < c.npair; ++pp) {
-    const PairP& PP = c.pair[pp];
-    int js;
-    if (sp == PP.s1) js = PP.s2;
-    else if (sp == PP.s2) js = PP.s1;
-    else continue;
-    const TableP T = x.tables[PP.table];
-    const double* u = x.tab_u + T.off;
-    const double* ut = x.tab_ut + T.off;
-    double up = 0;
-    for (int j = c.ifirst[js]; j < c.ifirst[js] + c.count[js]; ++j) {
-      if (j == i) continue;
-      const double* a1 = P(j, t);
-      const double* a0 = P(j, t - 1);
-      double dl[D], pd[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) { dl[d] = b1[d] - a1[d]; pd[d] = b0[d] - a0[d]; }
-      pbc<D>(c, dl);
-      pbc<D>(c, pd);
-      double r = 0, pr = 0, s2 = 0;
-#pragma unroll
-      for (int d = 0; d < D; ++d) { r += dl[d] * dl[d]; pr += pd[d] * pd[d]; }
-      r = sqrt(r);
-      pr = sqrt(pr);
-      const double q = 0.5 * (r + pr);
-#pragma unroll
-      for (int d = 0; d < D; ++d) { double sv = dl[d] - pd[d]; s2 += sv * sv; }
-      s2 /= q * q;
-      // uk0CalcDerivatives, utau part (PairAction.cc:640-657)
-      double qq = q * T.rgridinv;
-      double xx = log(qq) * T.logrratioinv;
-      int ig = (int)xx;
-      if (qq < 1) { ig = 0; xx = 0; }
-      else if (ig > T.ngpts - 2) { ig = T.ngpts - 2; xx = 1; }
-      else xx -= ig;
-      double vt = 0;
-      for (int kk = T.K1 - 1; kk >= 0; --kk) {
-        vt *= s2;
-        vt += (1 - xx) * ut[ig * T.K1 + kk] + xx * ut[(ig + 1) * T.K1 + kk];
-      }
-      (void)u;
-      up += 0.5 * vt;
-    }
-    uc += up;
-  }
-  if (lr && c.ew.nk && i == 0)
-    uc += ewald_slice<D>(x.ewk + c.ew.off, c.ew, c, beads + (int64_t)t * c.N * D) / c.ceps;
-  utau += uc;
-  return utau;
-}
-
 // sum over slices and half-space k-vectors of 2 f(k)/(2V) |rho_k(t)|^2 plus
 // M self energies: the long-range Ewald sums of all M slices, by the threads
 // of a workgroup (each returns its share)
@@ -296,53 +191,63 @@ __global__ __launch_bounds__(256) void coulomb_energy_kernel(DevCtx x, double ep
   }
 }
 
-// getBeadAction's utau and force fm + fp on bead (i,t) of the composite
-// action: SpringAction (SpringAction.cc:112-123), SHOAction (SHOAction.cc:
-// 71-105), PairAction (PairAction.cc:509-599) with uk0CalcDerivatives
-// (:640-657).  Slices -1 and M wrap through the permutation.
+// CompositeAction::getBeadAction (CompositeAction.cc:57-67) on bead (i, t):
+// SpringAction (SpringAction.cc:112-123: utau and forces, u = 0), SHOAction
+// (SHOAction.cc:71-105), PairAction (PairAction.cc:509-599, no exchange) with
+// uk0CalcDerivatives (:640-657), and with lr CoulombAction's long-range
+// Ewald part on particle 0 (CoulombAction.cc:164-170: u += tau LR, utau +=
+// LR, no force).  beads [M][N][D] with perm = the path permutation and its
+// inverse ([2][N]): slices -1 and M wrap through them (SerialPaths.cc:68-117).
+// out = u, utau, fm[D] (from the link into slice t), fp[D] (the link out);
+// ulambda is 0 for every action here.
 template <int D>
-__device__ double bead_force(const DevCtx& x, const ChainP& c, const double* beads, int i, int t, double* f) {
-  const int* perm = x.perm + c.perm_off;
+__device__ void bead_action(const DevCtx& x, const ChainP& c, const double* beads, const int* perm, int i, int t,
+                            bool lr, double* out) {
   const int* invperm = perm + c.N;
   auto P = [&](int p, int tt) -> const double* {
     if (tt < 0) { tt += c.M; p = invperm[p]; }
     else if (tt >= c.M) { tt -= c.M; p = perm[p]; }
     return beads + ((int64_t)tt * c.N + p) * D;
   };
+  double* fm = out + 2;
+  double* fp = out + 2 + D;
   const int sp = species_of(c, i);
   const double* b1 = P(i, t);
   const double* b0 = P(i, t - 1);
   const double* b2 = P(i, t + 1);
   double dm[D], dp[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) { dm[d] = b1[d] - b0[d]; dp[d] = b1[d] - b2[d]; f[d] = 0; }
+  for (int d = 0; d < D; ++d) { dm[d] = b1[d] - b0[d]; dp[d] = b1[d] - b2[d]; fm[d] = 0; fp[d] = 0; }
   pbc<D>(c, dm);
   pbc<D>(c, dp);
-  double utau = 0;
+  double u = 0, utau = 0;
   if (c.spring && !c.stat[sp]) {
     const double lam = c.lambda[sp], tau = c.tau;
     double dd = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) { f[d] -= dm[d] / (2 * lam * tau) + dp[d] / (2 * lam * tau); dd += dm[d] * dm[d]; }
+    for (int d = 0; d < D; ++d) { fm[d] -= dm[d] / (2 * lam * tau); fp[d] -= dp[d] / (2 * lam * tau); dd += dm[d] * dm[d]; }
     utau += D / (2 * tau) - dd / (4.0 * lam * tau * tau);
   }
   for (int a = 0; a < c.nsho; ++a) {
     const int ssp = c.sho_sp[a];
     if (i < c.ifirst[ssp] || i >= c.ifirst[ssp] + c.count[ssp]) continue;
     const int ndim = c.sho_ndim[a] ? c.sho_ndim[a] : D;
-    const double omega = c.sho_omega[a], m = c.mass[ssp];
-    const double wt = omega * c.tau, sinhwt = sinh(wt), coshwt = cosh(wt);
-    const double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / c.tau;
+    const double omega = c.sho_omega[a], m = c.mass[ssp], tau = c.tau;
+    const double wt = omega * tau, sinhwt = sinh(wt), coshwt = cosh(wt);
+    const double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / tau;
     double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
     for (int d = D - ndim; d < D; ++d) {
       const double r1 = b1[d] - c.sho_center[a][d], r0 = b0[d] - c.sho_center[a][d], r2 = b2[d] - c.sho_center[a][d];
       r0r0 += r0 * r0; r0r1 += r0 * r1; r1r1 += r1 * r1; delta2 += dm[d] * dm[d];
-      f[d] -= m * omega * (r1 * coshwt - r0) * cschwt - m * dm[d] * tin;
-      f[d] -= m * omega * (r1 * coshwt - r2) * cschwt - m * dp[d] * tin;
+      fm[d] -= m * omega * (r1 * coshwt - r0) * cschwt - m * dm[d] * tin;
+      fp[d] -= m * omega * (r1 * coshwt - r2) * cschwt - m * dp[d] * tin;
     }
     utau += 0.5 * ndim * omega * cothwt
             + 0.5 * m * omega * omega * ((r1r1 + r0r0) * (1 - cothwt * cothwt) + 2.0 * r0r1 * coshwt * cschwt * cschwt)
             + 0.5 * m * delta2 * tin * tin - 0.5 * ndim * tin;
+    u += 0.5 * ndim * log(2 * 3.14159265358979 * sinhwt / (m * omega)) +
+         m * omega * ((r1r1 + r0r0) * coshwt - 2.0 * r0r1) * cschwt * 0.5 -
+         0.5 * log(2.0 * 3.14159265358979 * tau / m) - 0.5 * m * delta2 / tau;
   }
   for (int pp = 0; pp < c.npair; ++pp) {
     const PairP& PP = c.pair[pp];
@@ -351,7 +256,7 @@ __device__ double bead_force(const DevCtx& x, const ChainP& c, const double* bea
     else if (sp == PP.s2) js = PP.s1;
     else continue;
     const TableP T = x.tables[PP.table];
-    const double* u = x.tab_u + T.off;
+    const double* tu = x.tab_u + T.off;
     const double* ut = x.tab_ut + T.off;
     for (int j = c.ifirst[js]; j < c.ifirst[js] + c.count[js]; ++j) {
       if (j == i) continue;
@@ -383,22 +288,41 @@ __device__ double bead_force(const DevCtx& x, const ChainP& c, const double* bea
         if (qq < 1) { ig = 0; xx = 0; }
         else if (ig > T.ngpts - 2) { ig = T.ngpts - 2; xx = 1; }
         else xx -= ig;
-        double vt = 0, vq = 0, vs2 = 0;
+        double v = 0, vt = 0, vq = 0, vs2 = 0;
         for (int k = T.K1 - 1; k >= 0; --k) {
+          v *= s2; v += (1 - xx) * tu[ig * T.K1 + k] + xx * tu[(ig + 1) * T.K1 + k];
           vt *= s2; vt += (1 - xx) * ut[ig * T.K1 + k] + xx * ut[(ig + 1) * T.K1 + k];
-          vq *= s2; vq += u[(ig + 1) * T.K1 + k] - u[ig * T.K1 + k];
+          vq *= s2; vq += tu[(ig + 1) * T.K1 + k] - tu[ig * T.K1 + k];
         }
         vq *= T.logrratioinv * T.rgridinv / qq;
         for (int k = T.K1 - 1; k > 0; --k) {
-          vs2 *= s2; vs2 += k * ((1 - xx) * u[ig * T.K1 + k] + xx * u[(ig + 1) * T.K1 + k]);
+          vs2 *= s2; vs2 += k * ((1 - xx) * tu[ig * T.K1 + k] + xx * tu[(ig + 1) * T.K1 + k]);
         }
-        if (side == 0) utau += 0.5 * vt;
+        if (side == 0) { u += 0.5 * v; utau += 0.5 * vt; }
+        double* f = side ? fp : fm;
 #pragma unroll
         for (int d = 0; d < D; ++d) f[d] -= vq * dl[d] / (2 * r) + vs2 * (2 * sv[d] / (q * q) - s2 * dl[d] / (q * r));
       }
     }
   }
-  return utau;
+  if (lr && c.ew.nk && i == 0) {
+    const double l = ewald_slice<D>(x.ewk + c.ew.off, c.ew, c, beads + (int64_t)t * c.N * D) / c.ceps;
+    u += l * c.tau;
+    utau += l;
+  }
+  out[0] = u;
+  out[1] = utau;
+}
+
+// utau and the force fm + fp of bead (i, t) of the resident paths (the
+// virial estimator adds the long-range Ewald utau in parallel itself)
+template <int D>
+__device__ double bead_force(const DevCtx& x, const ChainP& c, const double* beads, int i, int t, double* f) {
+  double o[2 + 2 * D];
+  bead_action<D>(x, c, beads, x.perm + c.perm_off, i, t, false, o);
+#pragma unroll
+  for (int d = 0; d < D; ++d) f[d] = o[2 + d] + o[2 + D + d];
+  return o[1];
 }
 
 // VirialEnergyEstimator (VirialEnergyEstimator.cc:27-78) over sumOverLinks
@@ -465,13 +389,22 @@ __global__ __launch_bounds__(256) void virial_energy_kernel(DevCtx x, int nwindo
   }
 }
 
+// getBeadAction of a set of beads of one chain: out[q] = u, utau, ulambda
+// (0), fm[D], fp[D] of bead (qi[q], qt[q]) on the paths beads/perm
 template <int D>
-__global__ __launch_bounds__(256) void bead_utau_kernel(DevCtx x, int nq, const int* qc, const int* qi,
-                                                         const int* qt, double* out) {
+__global__ __launch_bounds__(256) void bead_action_kernel(DevCtx x, int chain, const double* beads, const int* perm,
+                                                           int nq, const int* qi, const int* qt, double* out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
-  const ChainP& c = x.chains[qc[q]];
-  out[q] = bead_utau<D>(x, c, x.beads + c.bead_off, qi[q], qt[q], true);
+  const ChainP& c = x.chains[chain];
+  double o[2 + 2 * D];
+  bead_action<D>(x, c, beads, perm, qi[q], qt[q], true, o);
+  double* w = out + (int64_t)q * (3 + 2 * D);
+  w[0] = o[0];
+  w[1] = o[1];
+  w[2] = 0.0;
+#pragma unroll
+  for (int d = 0; d < 2 * D; ++d) w[3 + d] = o[2 + d];
 }
 
 // EwaldCoulombEstimator with its own OptEwaldSum (EwaldCoulombEstimator.cc:
@@ -644,11 +577,11 @@ hipError_t launch_virial_energy(int D, const DevCtx& x, int nchain, int nwindow,
   return hipGetLastError();
 }
 
-hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const int* qi, const int* qt,
-                            double* out, hipStream_t st) {
+hipError_t launch_bead_action(int D, const DevCtx& x, int chain, const double* beads, const int* perm, int nq,
+                              const int* qi, const int* qt, double* out, hipStream_t st) {
   const int nb = (nq + 255) / 256;
-  if (D == 2) hipLaunchKernelGGL(bead_utau_kernel<2>, dim3(nb), dim3(256), 0, st, x, nq, qc, qi, qt, out);
-  else hipLaunchKernelGGL(bead_utau_kernel<3>, dim3(nb), dim3(256), 0, st, x, nq, qc, qi, qt, out);
+  if (D == 2) hipLaunchKernelGGL(bead_action_kernel<2>, dim3(nb), dim3(256), 0, st, x, chain, beads, perm, nq, qi, qt, out);
+  else hipLaunchKernelGGL(bead_action_kernel<3>, dim3(nb), dim3(256), 0, st, x, chain, beads, perm, nq, qi, qt, out);
   return hipGetLastError();
 }
 

@@ -80,7 +80,7 @@ hipError_t launch_virial_energy(int D, const DevCtx& x, int nchain, int nwindow,
 hipError_t launch_ewald_coulomb_energy(int D, const DevCtx& x, int nchain, const EwP* ew, const double* kpool,
                                        const int64_t* voff, const double* vgrid, double eps, double* energy,
                                        hipStream_t st);
-hipError_t launch_bead_utau(int D, const DevCtx& x, int nq, const int* qc, const int* qi, const int* qt,
-                            double* out, hipStream_t st);
+hipError_t launch_bead_action(int D, const DevCtx& x, int chain, const double* beads, const int* perm, int nq,
+                              const int* qi, const int* qt, double* out, hipStream_t st);
 
 }  // namespace pq

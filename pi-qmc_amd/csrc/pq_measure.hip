@@ -46,15 +46,20 @@ __device__ void stage_slices(const DevCtx& x, const ChainP& c, int t0, int t1, d
   }
 }
 
+// q / d and q % d for q < 2^16 and 0 < d < 2^16 by a float reciprocal:
+// (q + 1/2)/d sits at least 1/(2d) from an integer, and the float product's
+// error stays below q 2^-23 < 1/(2d) there, so the truncation is exact
+__device__ __forceinline__ int div_small(int q, int d, float invd) { return (int)(((float)q + 0.5f) * invd); }
+
 // unordered pair p of n particles, i < j: p < n ((n-1)/2) walks the circulant
 // differences 1..(n-1)/2, the rest (even n only) the difference n/2
-__device__ __forceinline__ void pair_of(int n, int p, int& i, int& j) {
+__device__ __forceinline__ void pair_of(int n, float invn, int p, int& i, int& j) {
   const int full = n * ((n - 1) / 2);
   int a, b;
   if (p < full) {
-    const int dd = p / n + 1;
-    a = p - (dd - 1) * n;
-    b = a + dd;
+    const int dd = div_small(p, n, invn);
+    a = p - dd * n;
+    b = a + dd + 1;
     if (b >= n) b -= n;
   } else {
     a = p - full;
@@ -64,23 +69,65 @@ __device__ __forceinline__ void pair_of(int n, int p, int& i, int& j) {
   j = max(a, b);
 }
 
+// the separation of the pair (i, j) on staged slice k: pbc(r_i - r_j) and its length
+template <int D>
+__device__ __forceinline__ double pair_sep(const ChainP& c, const double* sl, int nd, int k, int i, int j,
+                                           double* dl) {
+  const double* a = sl + k * nd + i * D;
+  const double* b = sl + k * nd + j * D;
+#pragma unroll
+  for (int d = 0; d < D; ++d) dl[d] = a[d] - b[d];
+  pbc<D>(c, dl);
+  double r = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) r += dl[d] * dl[d];
+  return sqrt(r);
+}
+
+// vtau of a pair on one link from its separations dl (slice t, length r) and
+// pd (slice t-1, length pr) (uk0CalcDerivatives' vtau, PairAction.cc:640-657)
+template <int D>
+__device__ __forceinline__ double link_vt(const TableP& tb, const double* u, const double* dl, double r,
+                                          const double* pd, double pr) {
+  const double qm = 0.5 * (r + pr);
+  double s2 = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) { const double sv = dl[d] - pd[d]; s2 += sv * sv; }
+  s2 /= qm * qm;
+  const double qq = qm * tb.rgridinv;
+  double xx = log(qq) * tb.logrratioinv;
+  int ig = (int)xx;
+  if (qq < 1) { ig = 0; xx = 0; }
+  else if (ig > tb.ngpts - 2) { ig = tb.ngpts - 2; xx = 1; }
+  else xx -= ig;
+  double vt = 0;
+  const double* u0 = u + ig * tb.K1;
+  for (int kk = tb.K1 - 1; kk >= 0; --kk) {
+    vt *= s2;
+    vt += (1 - xx) * u0[kk] + xx * u0[tb.K1 + kk];
+  }
+  return vt;
+}
+
 // utau of the links (t-1, t) of slices t0..t1-1 of one chain, this thread's
 // share: SpringAction (SpringAction.cc:112-123), SHOAction (SHOAction.cc:
 // 71-105), PairAction (PairAction.cc:509-599, uk0CalcDerivatives :640-657)
 // counted once per unordered pair -- each bead of the pair takes half of the
 // same vtau -- and CoulombAction's long-range utau of the slices
-// (CoulombAction.cc:164-170).
+// (CoulombAction.cc:164-170).  Items hold fewer than 2^16 (slice, pair)
+// entries, so indices split by div_small.
 template <int D>
 __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, const PairMap& pm, const double* sl,
                                             const double* tab, const int* utoff, const TableP* tbs, int t0,
                                             int t1) {
   const int N = c.N, T = t1 - t0;
   const int nd = N * D;
+  const float invN = 1.0f / N;
   double acc = 0;
   // spring and SHO: lanes over (slice, particle)
   if (c.spring || c.nsho) {
     for (int q = threadIdx.x; q < T * N; q += blockDim.x) {
-      const int k = q / N, i = q - k * N;
+      const int k = div_small(q, N, invN), i = q - k * N;
       const double* b1 = sl + (k + 1) * nd + i * D;
       const double* b0 = sl + k * nd + i * D;
       const int sp = species_of(c, i);
@@ -112,49 +159,32 @@ __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, co
       }
     }
   }
-  // pair actions: lanes over (slice, unordered pair)
+  // pair actions: lanes over (slice group, pair), H groups of consecutive
+  // slices chosen so that every lane has work; walking its slices, a lane
+  // takes a link's earlier separation from the previous link (one pbc and one
+  // sqrt per link)
   if (c.npair) {
     const int P = N * (N - 1) / 2;
-    for (int q = threadIdx.x; q < T * P; q += blockDim.x) {
-      const int k = q / P;
+    const int H = P > 0 ? min(T, max(1, ((int)blockDim.x + P - 1) / P)) : 0;
+    const float invP = 1.0f / max(P, 1);
+    for (int q = threadIdx.x; q < H * P; q += blockDim.x) {
+      const int h = div_small(q, P, invP), p = q - h * P;
       int i, j;
-      pair_of(N, q - k * P, i, j);
-      const int si = species_of(c, i), sj = species_of(c, j);
-      const int slot = pm.slot[si][sj];
+      pair_of(N, invN, p, i, j);
+      const int slot = pm.slot[species_of(c, i)][species_of(c, j)];
       if (slot < 0) continue;
-      const TableP& tb = tbs[slot];
+      const TableP tb = tbs[slot];
       const double* u = tab + utoff[slot];
-      const double* a1 = sl + (k + 1) * nd + i * D;
-      const double* b1 = sl + (k + 1) * nd + j * D;
-      const double* a0 = sl + k * nd + i * D;
-      const double* b0 = sl + k * nd + j * D;
-      double dl[D], pd[D];
+      const int k0 = (h * T) / H, k1 = ((h + 1) * T) / H;
+      double pd[D], dl[D];
+      double pr = pair_sep<D>(c, sl, nd, k0, i, j, pd);
+      for (int k = k0; k < k1; ++k) {
+        const double r = pair_sep<D>(c, sl, nd, k + 1, i, j, dl);
+        acc += link_vt<D>(tb, u, dl, r, pd, pr);
 #pragma unroll
-      for (int d = 0; d < D; ++d) { dl[d] = a1[d] - b1[d]; pd[d] = a0[d] - b0[d]; }
-      pbc<D>(c, dl);
-      pbc<D>(c, pd);
-      double r = 0, pr = 0, s2 = 0;
-#pragma unroll
-      for (int d = 0; d < D; ++d) { r += dl[d] * dl[d]; pr += pd[d] * pd[d]; }
-      r = sqrt(r);
-      pr = sqrt(pr);
-      const double qm = 0.5 * (r + pr);
-#pragma unroll
-      for (int d = 0; d < D; ++d) { const double sv = dl[d] - pd[d]; s2 += sv * sv; }
-      s2 /= qm * qm;
-      const double qq = qm * tb.rgridinv;
-      double xx = log(qq) * tb.logrratioinv;
-      int ig = (int)xx;
-      if (qq < 1) { ig = 0; xx = 0; }
-      else if (ig > tb.ngpts - 2) { ig = tb.ngpts - 2; xx = 1; }
-      else xx -= ig;
-      double vt = 0;
-      const double* u0 = u + ig * tb.K1;
-      for (int kk = tb.K1 - 1; kk >= 0; --kk) {
-        vt *= s2;
-        vt += (1 - xx) * u0[kk] + xx * u0[tb.K1 + kk];
+        for (int d = 0; d < D; ++d) pd[d] = dl[d];
+        pr = r;
       }
-      acc += vt;
     }
   }
   // Ewald k-space utau of every slice (particle 0 carries it): lanes over (slice, k)
@@ -260,15 +290,17 @@ __global__ __launch_bounds__(kMeasThreads) void energy_items_kernel(DevCtx x, co
   }
 }
 
-// E = (1/M) sum over a chain's items in order (SimpleScalarAccumulator's
-// storeValue divides by the slice count)
+// E = (1/M) sum over a chain's items (SimpleScalarAccumulator's storeValue
+// divides by the slice count): one wave per chain, lanes over the items in a
+// fixed order and a fixed butterfly, so the sum does not depend on scheduling
 __global__ void energy_reduce_kernel(const DevCtx x, const int* first_item, int nchain, const double* part,
                                      double* energy) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * (blockDim.x / PQ_WAVE) + threadIdx.x / PQ_WAVE;
   if (c >= nchain) return;
   double s = 0;
-  for (int it = first_item[c]; it < first_item[c + 1]; ++it) s += part[it];
-  energy[c] = s / x.chains[c].M;
+  for (int it = first_item[c] + lane_id(); it < first_item[c + 1]; it += PQ_WAVE) s += part[it];
+  s = wave_sum(s);
+  if (lane_id() == 0) energy[c] = s / x.chains[c].M;
 }
 
 // SuperCell::pbc without contraction: the bins of the histogram are those of
@@ -333,26 +365,39 @@ __global__ __launch_bounds__(kMeasThreads) void pair_cf_items_kernel(DevCtx x, c
     stage_slices<D>(x, c, m.t0, m.t1, sl);
     __syncthreads();
     const int nd = c.N * D, per = n1 * njtot, T = m.t1 - m.t0;
-    for (int q = threadIdx.x; q < T * per; q += blockDim.x) {
-      const int k = q / per, r = q - k * per;
-      const int i = if1 + r / njtot;
-      int rr = r - (r / njtot) * njtot, kk = 1;
+    const float invnj = 1.0f / njtot;
+    // a lane owns an (i, j) pair over all the item's slices when there are
+    // enough pairs for every lane, else lanes over (slice, pair)
+    const bool pair_major = per >= (int)blockDim.x / 2;
+    const int count = pair_major ? per : T * per;
+    for (int q = threadIdx.x; q < count; q += blockDim.x) {
+      int k0 = 0, r = q;
+      if (!pair_major) {
+        k0 = T * per < 65536 ? div_small(q, per, 1.0f / per) : q / per;
+        r = q - k0 * per;
+      }
+      const int ii = per < 65536 ? div_small(r, njtot, invnj) : r / njtot;
+      const int i = if1 + ii;
+      int rr = r - ii * njtot, kk = 1;
       while (rr >= jcount[kk]) { rr -= jcount[kk]; ++kk; }
       const int j = jfirst[kk] + rr;
       if (i == j) continue;
-      const double* a = sl + (k + 1) * nd + i * D;
-      const double* b = sl + (k + 1) * nd + j * D;
-      double delta[D];
+      const int k1 = pair_major ? T : k0 + 1;
+      for (int k = k0; k < k1; ++k) {
+        const double* a = sl + (k + 1) * nd + i * D;
+        const double* b = sl + (k + 1) * nd + j * D;
+        double delta[D];
 #pragma unroll
-      for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
-      pbc_exact<D>(c, delta);
-      double r2 = 0;
+        for (int d = 0; d < D; ++d) delta[d] = a[d] - b[d];
+        pbc_exact<D>(c, delta);
+        double r2 = 0;
 #pragma unroll
-      for (int d = 0; d < D; ++d) r2 += delta[d] * delta[d] * 1.0;
-      const double dd = sqrt(r2);
-      const int ibin = (int)((dd - g.rmin) * g.deltaInv);
-      if (dd < g.rmin || ibin > g.nbin - 1) continue;
-      atomicAdd(&hist[ibin], 1u);
+        for (int d = 0; d < D; ++d) r2 += delta[d] * delta[d] * 1.0;
+        const double dd = sqrt(r2);
+        const int ibin = (int)((dd - g.rmin) * g.deltaInv);
+        if (dd < g.rmin || ibin > g.nbin - 1) continue;
+        atomicAdd(&hist[ibin], 1u);
+      }
     }
     __syncthreads();
     uint64_t* out = counts + (int64_t)m.chain * g.nbin;
@@ -380,7 +425,7 @@ hipError_t launch_energy_items(int D, const DevCtx& x, const MeasItem* items, in
     hipLaunchKernelGGL(energy_items_kernel<3>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(energy_reduce_kernel, dim3((nchain + 127) / 128), dim3(128), 0, st, x, first_item, nchain,
+  hipLaunchKernelGGL(energy_reduce_kernel, dim3((nchain + 3) / 4), dim3(4 * PQ_WAVE), 0, st, x, first_item, nchain,
                      part, energy);
   return hipGetLastError();
 }

@@ -73,6 +73,7 @@ template <int D, int NC, class R, int NT, int MM>
 __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, int nsweep, SweepLayoutH lay) {
   extern __shared__ double smem[];
   const int chain = x.order[blockIdx.x];
+  const uint64_t t_begin = wall_clock64();
   ChainP& c = *reinterpret_cast<ChainP*>(smem);
   copy_chain(&c, &x.chains[chain]);
   char* base = reinterpret_cast<char*>(smem);
@@ -143,7 +144,8 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
   double* beads = x.beads + c.bead_off;
   uint64_t k = x.counter[chain];
 
-  for (int sw = 0; sw < nsweep; ++sw)
+  const int nsw = x.plan ? nsweep * x.plan[chain] : nsweep;
+  for (int sw = 0; sw < nsw; ++sw)
   for (int g = 0; g < c.ngroup; ++g) {
     // the sweep's g-th <ChooseSection nlevel> with its Samples: its level
     // count, section length and Samples take the chain parameters' place
@@ -407,6 +409,9 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
     st.accepted_bead_updates += cnt[17];
     st.node_errors += cnt[18];
     for (int l = 0; l < 8; ++l) { st.tries[l] += cnt[l]; st.accepts[l] += cnt[8 + l]; }
+    st.sweeps += nsw;
+    st.t_start = t_begin;
+    st.t_end = wall_clock64();
   }
   if (tid < c.nfn) x.scale[(int64_t)chain * PQ_MAX_NODES + tid] = e.scale[tid];
 }

@@ -106,7 +106,8 @@ class ChainDesc(C.Structure):
 class ChainStats(C.Structure):
     _fields_ = [("draws", C.c_uint64), ("bead_updates", C.c_uint64),
                 ("accepted_bead_updates", C.c_uint64), ("tries", C.c_uint64 * 8),
-                ("accepts", C.c_uint64 * 8), ("node_errors", C.c_uint64)]
+                ("accepts", C.c_uint64 * 8), ("node_errors", C.c_uint64),
+                ("sweeps", C.c_uint64), ("t_start", C.c_uint64), ("t_end", C.c_uint64)]
 
 
 def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
@@ -318,8 +319,12 @@ def load_library(path=LIB_PATH):
         "pq_set_measure_pair_cf": (C.c_int, [C.c_void_p, C.c_int, ip, C.c_double, C.c_double, C.c_int]),
         "pq_collect_pair_cf": (C.c_int, [C.c_void_p, P(C.c_uint64)]),
         "pq_run_measured_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, P(C.c_float)]),
+        "pq_set_sweep_plan": (C.c_int, [C.c_void_p, ip]),
+        "pq_get_sweep_plan": (C.c_int, [C.c_void_p, ip]),
+        "pq_clock_hz": (C.c_int, [C.c_void_p, dp]),
         "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),
         "pq_bead_utau": (C.c_int, [C.c_void_p, C.c_int, ip, ip, ip, dp]),
+        "pq_bead_action": (C.c_int, [C.c_void_p, C.c_int, dp, ip, C.c_int, ip, ip, dp]),
         "pq_section_begin": (C.c_int, [C.c_void_p, C.c_int, dp, dp]),
         "pq_double_action_difference": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, ip, C.c_int, dp]),
         "pq_accept_last_move": (C.c_int, [C.c_void_p, C.c_int]),
@@ -450,6 +455,48 @@ class Batch:
         ms, n = C.c_float(), C.c_int()
         check(self.lib.pq_autotune(self.h, nsweep, C.byref(ms), C.byref(n)))
         return ms.value, n.value * nsweep
+
+    def bead_action(self, c, ipart, islice, beads=None, perm=None):
+        """getBeadAction of beads (ipart[q], islice[q]) of chain c: [nq][3 + 2D]
+        = u, utau, ulambda, fm, fp; on the caller's paths (beads [M][N][D],
+        perm) or, with beads None, on the device-resident paths."""
+        ii = np.ascontiguousarray(ipart, dtype=np.int32)
+        tt = np.ascontiguousarray(islice, dtype=np.int32)
+        D = self.descs[c].ndim
+        out = np.zeros((ii.size, 3 + 2 * D))
+        bp = dptr(np.ascontiguousarray(beads, dtype=np.float64)) if beads is not None else None
+        keep = None
+        if perm is not None:
+            keep = np.ascontiguousarray(perm, dtype=np.int32)
+        if beads is not None:
+            self._bead_keep = np.ascontiguousarray(beads, dtype=np.float64)
+            bp = dptr(self._bead_keep)
+        check(self.lib.pq_bead_action(self.h, c, bp, iptr(keep) if keep is not None else None, ii.size,
+                                      iptr(ii), iptr(tt), dptr(out)))
+        return out
+
+    def set_sweep_plan(self, sweeps=None):
+        """Sweeps per launch of each chain (None: one each)."""
+        if sweeps is None:
+            check(self.lib.pq_set_sweep_plan(self.h, None))
+            return
+        p = np.ascontiguousarray(sweeps, dtype=np.int32)
+        check(self.lib.pq_set_sweep_plan(self.h, iptr(p)))
+
+    def sweep_plan(self):
+        p = np.zeros(len(self.descs), dtype=np.int32)
+        check(self.lib.pq_get_sweep_plan(self.h, iptr(p)))
+        return p
+
+    def clock_hz(self):
+        v = C.c_double()
+        check(self.lib.pq_clock_hz(self.h, C.byref(v)))
+        return v.value
+
+    def all_stats(self):
+        st = (ChainStats * len(self.descs))()
+        check(self.lib.pq_get_all_stats(self.h, st))
+        return st
 
     def stats(self, c):
         st = ChainStats()

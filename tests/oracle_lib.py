@@ -81,6 +81,9 @@ def load():
                                         P(C.c_uint64)]),
         "oq_run_batch": (C.c_uint64, [P(capi.ChainDesc), C.c_int, dp, C.c_int, C.c_uint64,
                                       C.c_uint32, C.c_int, dp]),
+        "oq_run_batch_measured": (C.c_uint64, [P(capi.ChainDesc), C.c_int, dp, C.c_int, C.c_uint64,
+                                               C.c_uint32, C.c_int, C.c_int, ip, C.c_double, C.c_double,
+                                               C.c_int, dp, P(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_sizes_match_header_layout():
     # offsets of the ctypes mirror follow the C layout rules of the header
-    assert C.sizeof(capi.ChainStats) == 8 * 20
+    assert C.sizeof(capi.ChainStats) == 8 * 23
     assert capi.ChainDesc.nsection.offset > capi.ChainDesc.sample.offset
     lib = capi.load_library()
     assert lib.pq_last_error() is not None
