@@ -404,6 +404,19 @@ int pq_action_difference(pq_batch* batch, int nquery, const int* chain,
    from the link into slice islice), fp[D] (from the link out of it). */
 int pq_bead_action(pq_batch* batch, int chain, const double* beads, const int* perm,
                    int nquery, const int* ipart, const int* islice, double* out);
+/* pq_action_difference for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc); it has no counterpart in the reference.  The same queries and
+   results; its staging buffers live in a 4 MiB device array per GPU,
+   reserved when the library is loaded and looked up at pq_batch_create, so
+   the call makes no hipMalloc or hipFree.  Queries needing more than 4 MiB
+   ((S N D + S nmov D + 1) doubles + (2 + nmov) ints each, S N D the largest
+   among the queried chains) are refused with PQ_ERR_ARG.  Calls on one GPU share that
+   array: calls that overlap on one GPU (other host threads) race. */
+int pq_action_difference_static(pq_batch* batch, int nquery, const int* chain,
+                                const int* level, const double* section,
+                                const double* moving, const int* index, int nmov,
+                                double* dS);
 /* Per-link utau of CompositeAction::getBeadAction for a set of (chain, ipart,
    islice) on the current device paths. */
 int pq_bead_utau(pq_batch* batch, int nquery, const int* chain,

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -322,7 +323,7 @@ static void ewald_params(const pq_batch* b, int id, const pq_chain_desc& d, int 
   for (int i = 0; i < kEwaldPoly; ++i) p.coef[i] = e.coef[i];
 }
 
-static int scratch(pq_batch* b, size_t bytes, void** p) {
+static int scratch_buffer(pq_batch* b, size_t bytes, void** p) {
   if (bytes > b->scr_bytes) {
     if (b->dscr) (void)hipFree(b->dscr);
     b->dscr = nullptr;
@@ -729,6 +730,8 @@ static void build_measure_items(pq_batch* b) {
   }
 }
 
+static int static_scratch_setup(int device);
+
 extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int precision, int device,
                                pq_batch** out) {
   if (!chains || nchain < 1 || !out) return fail(PQ_ERR_ARG, "pq_batch_create: bad arguments");
@@ -757,6 +760,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     if (e != hipSuccess) { delete b; return fail(PQ_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e)); }
     if (device < 0 || device >= ndev) { delete b; return fail(PQ_ERR_ARG, "device %d not present (%d devices)", device, ndev); }
     if (hipSetDevice(device) != hipSuccess) { delete b; return fail(PQ_ERR_HIP, "hipSetDevice(%d) failed", device); }
+    if (int rc = static_scratch_setup(device)) { delete b; return rc; }
   }
   for (int c = 0; c < nchain; ++c)
     for (int g = 0; g < b->hc[c].ngroup; ++g)
@@ -1511,7 +1515,7 @@ extern "C" int pq_measure_energy(pq_batch* b, double* energy) {
   if (!b || !energy) return fail(PQ_ERR_ARG, "pq_measure_energy: bad arguments");
   HIPCHK(hipSetDevice(b->device));
   void* p;
-  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  if (int rc = scratch_buffer(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
   HIPCHK(launch_energy_items(b->D, x, b->ditems_e, (int)b->items_e.size(), b->dfirst_e, b->nchain, b->slice_e,
                              b->dpart, (double*)p, b->stream));
@@ -1525,7 +1529,7 @@ extern "C" int pq_measure_coulomb_energy(pq_batch* b, double epsilon, double* en
   if (epsilon == 0) epsilon = 1;
   HIPCHK(hipSetDevice(b->device));
   void* p;
-  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  if (int rc = scratch_buffer(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
   HIPCHK(launch_coulomb_energy(b->D, x, b->nchain, epsilon, (double*)p, b->stream));
   HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
@@ -1620,7 +1624,7 @@ static int measure_ewald_coulomb(pq_batch* b, double epsilon, double rcut, doubl
     b->est_type = trad ? 1 : 0;
   }
   void* p;
-  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  if (int rc = scratch_buffer(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
   HIPCHK(launch_ewald_coulomb_energy(b->D, x, b->nchain, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est,
                                      epsilon, (double*)p, b->stream));
@@ -1644,7 +1648,7 @@ extern "C" int pq_measure_virial_energy(pq_batch* b, int nwindow, double* energy
   if (nwindow <= 0) nwindow = 1;
   HIPCHK(hipSetDevice(b->device));
   void* p;
-  if (int rc = scratch(b, sizeof(double) * b->nchain, &p)) return rc;
+  if (int rc = scratch_buffer(b, sizeof(double) * b->nchain, &p)) return rc;
   DevCtx x = make_ctx(b);
   HIPCHK(launch_virial_energy(b->D, x, b->nchain, nwindow, (double*)p, b->stream));
   HIPCHK(hipMemcpyAsync(energy, p, sizeof(double) * b->nchain, hipMemcpyDeviceToHost, b->stream));
@@ -1825,7 +1829,7 @@ extern "C" int pq_pair_cf_histogram(pq_batch* b, int nspecies, const int* specie
   HIPCHK(hipSetDevice(b->device));
   void* p;
   const size_t bytes = sizeof(uint64_t) * (size_t)nbin * b->nchain;
-  if (int rc = scratch(b, bytes, &p)) return rc;
+  if (int rc = scratch_buffer(b, bytes, &p)) return rc;
   HIPCHK(hipMemcpyAsync(p, counts, bytes, hipMemcpyHostToDevice, b->stream));
   DevCtx x = make_ctx(b);
   HIPCHK(launch_pair_cf_items(b->D, x, b->ditems_g, (int)b->items_g.size(), b->slice_g, g, (uint64_t*)p, b->stream));
@@ -1872,13 +1876,15 @@ extern "C" int pq_collect_pair_cf(pq_batch* b, uint64_t* counts) {
   return PQ_OK;
 }
 
-extern "C" int pq_action_difference(pq_batch* b, int nq, const int* chain, const int* level,
-                                    const double* section, const double* moving, const int* index,
-                                    int nmov, double* dS) {
+// the queries of pq_action_difference: chains, levels and moving particles
+// in range, no particle twice; the largest section and moving-bead blocks
+static int check_queries(pq_batch* b, int nq, const int* chain, const int* level, const double* section,
+                         const double* moving, const int* index, int nmov, double* dS, int64_t* sec_stride,
+                         int64_t* mov_stride) {
   if (!b || nq < 1 || !chain || !level || !section || !moving || !index || !dS)
     return fail(PQ_ERR_ARG, "pq_action_difference: bad arguments");
   if (nmov < 1 || nmov > PQ_MAX_MOVING) return fail(PQ_ERR_ARG, "pq_action_difference: nmov must be in 1..%d", PQ_MAX_MOVING);
-  int64_t sec_stride = 0, mov_stride = 0;
+  *sec_stride = *mov_stride = 0;
   for (int q = 0; q < nq; ++q) {
     if (int rc = check_chain(b, chain[q])) return rc;
     const ChainP& ch = b->hc[chain[q]];
@@ -1889,9 +1895,18 @@ extern "C" int pq_action_difference(pq_batch* b, int nq, const int* chain, const
       for (int m2 = 0; m2 < m; ++m2)
         if (index[(size_t)q * nmov + m2] == i) return fail(PQ_ERR_ARG, "query %d: particle %d moves twice", q, i);
     }
-    sec_stride = std::max<int64_t>(sec_stride, (int64_t)ch.S * ch.N * ch.D);
-    mov_stride = std::max<int64_t>(mov_stride, (int64_t)ch.S * nmov * ch.D);
+    *sec_stride = std::max<int64_t>(*sec_stride, (int64_t)ch.S * ch.N * ch.D);
+    *mov_stride = std::max<int64_t>(*mov_stride, (int64_t)ch.S * nmov * ch.D);
   }
+  return PQ_OK;
+}
+
+extern "C" int pq_action_difference(pq_batch* b, int nq, const int* chain, const int* level,
+                                    const double* section, const double* moving, const int* index,
+                                    int nmov, double* dS) {
+  int64_t sec_stride = 0, mov_stride = 0;
+  if (int rc = check_queries(b, nq, chain, level, section, moving, index, nmov, dS, &sec_stride, &mov_stride))
+    return rc;
   // queries arrive packed by their own chain's sizes; restage with a fixed stride
   std::vector<double> hs((size_t)nq * sec_stride, 0.0), hm((size_t)nq * mov_stride, 0.0);
   int64_t os = 0, om = 0;
@@ -1906,7 +1921,7 @@ extern "C" int pq_action_difference(pq_batch* b, int nq, const int* chain, const
   HIPCHK(hipSetDevice(b->device));
   const size_t bytes = sizeof(double) * (hs.size() + hm.size() + nq) + sizeof(int) * (2 + nmov) * nq + 64;
   void* p;
-  if (int rc = scratch(b, bytes, &p)) return rc;
+  if (int rc = scratch_buffer(b, bytes, &p)) return rc;
   double* dsec = (double*)p;
   double* dmov = dsec + hs.size();
   double* dout = dmov + hm.size();
@@ -1936,7 +1951,7 @@ static int bead_action_dev(pq_batch* b, int c, const double* dbeads, const int* 
   const int w = 3 + 2 * ch.D;
   const size_t need = sizeof(double) * (size_t)nq * w + sizeof(int) * 2 * (size_t)nq + 64;
   void* p;
-  if (int rc = scratch(b, need, &p)) return rc;
+  if (int rc = scratch_buffer(b, need, &p)) return rc;
   double* dout = (double*)p;
   int* dq = (int*)(dout + (size_t)nq * w);
   HIPCHK(hipMemcpyAsync(dq, ipart, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
@@ -1983,6 +1998,73 @@ extern "C" int pq_bead_action(pq_batch* b, int c, const double* beads, const int
   HIPCHK(hipMemcpyAsync(b->dbead_stage, beads, sizeof(double) * nb, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipMemcpyAsync(dpi, pi.data(), sizeof(int) * pi.size(), hipMemcpyHostToDevice, b->stream));
   return bead_action_dev(b, c, b->dbead_stage, dpi, nq, ipart, islice, out);
+}
+
+// ----------------------------------------- the entry point that allocates nothing
+// The device memory of pq_action_difference_static: a file-scope array on
+// every GPU the library runs on, held until the process ends.  In host code
+// the name `scratch` is a stand-in, not a device address: host code uses the
+// address hipGetSymbolAddress gives for each device, looked up once per
+// device (with that device current) when a batch is created there.
+#define PQ_STATIC_SCRATCH_BYTES (4 << 20)
+#define PQ_MAX_DEVICES 64
+static __device__ __align__(256) char scratch[PQ_STATIC_SCRATCH_BYTES];
+static void* g_scratch_ptr[PQ_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int static_scratch_setup(int device) {
+  if (device < 0 || device >= PQ_MAX_DEVICES) return fail(PQ_ERR_ARG, "device %d beyond %d", device, PQ_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PQ_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(PQ_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return PQ_OK;
+}
+
+extern "C" int pq_action_difference_static(pq_batch* b, int nq, const int* chain, const int* level,
+                                           const double* section, const double* moving, const int* index,
+                                           int nmov, double* dS) {
+  int64_t sec_stride = 0, mov_stride = 0;
+  if (int rc = check_queries(b, nq, chain, level, section, moving, index, nmov, dS, &sec_stride, &mov_stride))
+    return rc;
+  const size_t nsec = (size_t)nq * sec_stride, nmv = (size_t)nq * mov_stride;
+  const size_t bytes = sizeof(double) * (nsec + nmv + nq) + sizeof(int) * (2 + (size_t)nmov) * nq;
+  if (bytes > PQ_STATIC_SCRATCH_BYTES)
+    return fail(PQ_ERR_ARG, "pq_action_difference_static: %d queries need %zu bytes of device scratch, more than its "
+                "%d; split them over several calls", nq, bytes, PQ_STATIC_SCRATCH_BYTES);
+  void* base;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch_ptr[b->device];
+  }
+  if (!base) return fail(PQ_ERR_STATE, "the static scratch of device %d was not set up", b->device);
+  HIPCHK(hipSetDevice(b->device));
+  // every byte the kernel reads is written here first: nothing needs clearing
+  double* dsec = (double*)base;
+  double* dmov = dsec + nsec;
+  double* dout = dmov + nmv;
+  int* dq = (int*)(dout + nq);
+  int64_t os = 0, om = 0;
+  for (int q = 0; q < nq; ++q) {
+    const ChainP& ch = b->hc[chain[q]];
+    const int64_t ns = (int64_t)ch.S * ch.N * ch.D, nm = (int64_t)ch.S * nmov * ch.D;
+    HIPCHK(hipMemcpyAsync(dsec + q * sec_stride, section + os, sizeof(double) * ns, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(dmov + q * mov_stride, moving + om, sizeof(double) * nm, hipMemcpyHostToDevice, b->stream));
+    os += ns;
+    om += nm;
+  }
+  HIPCHK(hipMemcpyAsync(dq, chain, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq + nq, level, sizeof(int) * nq, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync(dq + 2 * nq, index, sizeof(int) * nq * nmov, hipMemcpyHostToDevice, b->stream));
+  CallBufsH cb{dq, dq + nq, dsec, dmov, dq + 2 * nq, nmov, sec_stride, mov_stride, dout};
+  DevCtx x = make_ctx(b);
+  HIPCHK(launch_action_diff(b->D, b->NC, b->precision == PQ_PREC_MIXED, x, nq, cb, b->stream));
+  HIPCHK(hipMemcpyAsync(dS, dout, sizeof(double) * nq, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PQ_OK;
 }
 
 extern "C" int pq_bead_utau(pq_batch* b, int nq, const int* chain, const int* ipart, const int* islice,
@@ -2115,7 +2197,7 @@ extern "C" int pq_node_evaluate(pq_batch* b, int c, int inode, int nmat, const d
   const size_t bytes = sizeof(double) * (2 * nd + (size_t)nmat * nsm * nsm * (1 + ch.D) + nmat + (size_t)nmat * 2 * ns) +
                        sizeof(int) * nmat + 256;
   void* p;
-  if (int rc = scratch(b, bytes, &p)) return rc;
+  if (int rc = scratch_buffer(b, bytes, &p)) return rc;
   double* dr1 = (double*)p;
   double* dr2 = dr1 + nd;
   double* dinv = dr2 + nd;
