@@ -193,9 +193,59 @@ typedef struct pq_chain_stats {
                                     chain's workgroup of the last sampler launch started and ended */
 } pq_chain_stats;
 
+/* A pi-qmc input file read for the device path (pq_read_xml).  The sweep is
+   the body of the first <Loop> that holds a <Measure>: its <ChooseSection>
+   elements, and <Loop>s of one <ChooseSection> each, become the section
+   groups of one sweep, so a batch of that descriptor runs one Measure per
+   sweep; nmeasure is that loop's nrepeat and nblock the nrepeat of an
+   enclosing <Loop> with a <Collect>.  The first <Loop> before it without a
+   Measure is the thermalisation, a descriptor of its own repeated
+   nthermalize times. */
+#define PQ_XML_MAX_CF_SPECIES 8
+typedef struct pq_xml_input {
+  pq_chain_desc sweep;
+  int nmeasure;
+  int nblock;
+  int has_thermalize;
+  pq_chain_desc thermalize;
+  int nthermalize;
+  int has_seed;            /* <RandomGenerator>; iseed 0 when the attribute is absent (srand(0)) */
+  int iseed;
+  char read_paths[256];    /* <ReadPaths file>, empty when absent */
+  int read_bfactor;        /* 1 when absent */
+  int thermal_energy;      /* <ThermalEnergyEstimator> */
+  int virial_nwindow;      /* <VirialEnergyEstimator nwindow> (absent: 1), 0: no estimator */
+  int coulomb_energy;      /* <CoulombEnergyEstimator epsilon useEwald rcut kcut ewaldType kappa> */
+  double coulomb_epsilon;
+  int coulomb_ewald;
+  double coulomb_rcut, coulomb_kcut;
+  int coulomb_ewald_type;  /* 0 optEwald, 1 tradEwald */
+  double coulomb_kappa;
+  int pair_cf_nspecies;    /* <PairCFEstimator nspecies species1..> with <Radial nbin min max>; 0: none */
+  int pair_cf_species[PQ_XML_MAX_CF_SPECIES];
+  double pair_cf_rmin, pair_cf_rmax;
+  int pair_cf_nbin;
+  int nignored;            /* top-level PIMC elements with no device meaning (path
+                              initialisers, writers), left to the caller */
+} pq_xml_input;
+
 typedef struct pq_batch pq_batch;
 
 const char* pq_last_error(void);
+
+/* Reads a pi-qmc input file into descriptors for a reference binary built
+   with NDIM = ndim, following SimInfoParser (SimInfoParser.cc:30-105),
+   ActionParser::parseActions / parseNodeModel (ActionParser.cc:102-456,
+   759-815), EstimatorParser (EstimatorParser.cc:150-215,477-488,643-725) and
+   PIMCParser::parseAlgorithm (PIMCParser.cc:125-490,560-640): attribute
+   values as the reference reads them, units scaled to Hartree atomic units
+   with the reference's table.  Elements and attributes that the device path
+   does not model (other actions, node models or movers, exchange, image
+   Ewald sums, ...) are refused with PQ_ERR_ARG and a message naming them,
+   never skipped.  pq_parse_xml reads the same from a string.  Touches no
+   device. */
+int pq_read_xml(const char* filename, int ndim, pq_xml_input* out);
+int pq_parse_xml(const char* text, int ndim, pq_xml_input* out);
 
 /* Builds pair tables, uploads parameters and allocates per-chain device state
    (beads slice-major [M][N][D] in FP64).  precision is PQ_PREC_*. */

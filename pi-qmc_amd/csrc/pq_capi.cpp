@@ -42,6 +42,9 @@ static int fail(int code, const char* fmt, ...) {
 
 extern "C" const char* pq_last_error(void) { return g_err.c_str(); }
 
+// failures of the host-only parts in other files (the XML reader)
+void pq_set_last_error(const char* msg) { g_err = msg; }
+
 // ------------------------------------------------------- Coulomb tables
 // Coulomb1DLinkAction (src/action/coulomb/Coulomb1DLinkAction.cpp:4-133) and
 // Coulomb3DLinkAction (src/action/coulomb/Coulomb3DLinkAction.cpp:13-90):

@@ -103,6 +103,32 @@ class ChainDesc(C.Structure):
         return self.nslice * self.npart * self.ndim
 
 
+PQ_XML_MAX_CF_SPECIES = 8
+
+
+class XmlInput(C.Structure):
+    """pq_xml_input: a pi-qmc input file read for the device path."""
+    _fields_ = [("sweep", ChainDesc), ("nmeasure", C.c_int), ("nblock", C.c_int),
+                ("has_thermalize", C.c_int), ("thermalize", ChainDesc), ("nthermalize", C.c_int),
+                ("has_seed", C.c_int), ("iseed", C.c_int), ("read_paths", C.c_char * 256),
+                ("read_bfactor", C.c_int), ("thermal_energy", C.c_int), ("virial_nwindow", C.c_int),
+                ("coulomb_energy", C.c_int), ("coulomb_epsilon", C.c_double), ("coulomb_ewald", C.c_int),
+                ("coulomb_rcut", C.c_double), ("coulomb_kcut", C.c_double), ("coulomb_ewald_type", C.c_int),
+                ("coulomb_kappa", C.c_double), ("pair_cf_nspecies", C.c_int),
+                ("pair_cf_species", C.c_int * PQ_XML_MAX_CF_SPECIES), ("pair_cf_rmin", C.c_double),
+                ("pair_cf_rmax", C.c_double), ("pair_cf_nbin", C.c_int), ("nignored", C.c_int)]
+
+
+def read_xml(path_or_text, ndim=3, text=False):
+    """pq_read_xml (a file) or pq_parse_xml (text=True): an XmlInput; raises
+    PQError with the reader's message on inputs the device path does not model."""
+    lib = load_library()
+    out = XmlInput()
+    fn = lib.pq_parse_xml if text else lib.pq_read_xml
+    check(fn(path_or_text.encode() if isinstance(path_or_text, str) else path_or_text, ndim, C.byref(out)))
+    return out
+
+
 class ChainStats(C.Structure):
     _fields_ = [("draws", C.c_uint64), ("bead_updates", C.c_uint64),
                 ("accepted_bead_updates", C.c_uint64), ("tries", C.c_uint64 * 8),
@@ -320,6 +346,8 @@ def load_library(path=LIB_PATH):
         "pq_collect_pair_cf": (C.c_int, [C.c_void_p, P(C.c_uint64)]),
         "pq_run_measured_timed": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, P(C.c_float)]),
         "pq_set_sweep_plan": (C.c_int, [C.c_void_p, ip]),
+        "pq_read_xml": (C.c_int, [C.c_char_p, C.c_int, P(XmlInput)]),
+        "pq_parse_xml": (C.c_int, [C.c_char_p, C.c_int, P(XmlInput)]),
         "pq_get_sweep_plan": (C.c_int, [C.c_void_p, ip]),
         "pq_clock_hz": (C.c_int, [C.c_void_p, dp]),
         "pq_action_difference": (C.c_int, [C.c_void_p, C.c_int, ip, ip, dp, dp, ip, C.c_int, dp]),

@@ -47,7 +47,8 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
     structs = {"pq_species_desc": capi.SpeciesDesc, "pq_sho_desc": capi.ShoDesc,
                "pq_coulomb_desc": capi.CoulombDesc, "pq_node_desc": capi.NodeDesc,
                "pq_sample_desc": capi.SampleDesc, "pq_section_group": capi.SectionGroup,
-               "pq_chain_desc": capi.ChainDesc, "pq_chain_stats": capi.ChainStats}
+               "pq_chain_desc": capi.ChainDesc, "pq_chain_stats": capi.ChainStats,
+               "pq_xml_input": capi.XmlInput}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "piqmc_hip.h"', 'int main(void) {']
     for cname, py in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
