@@ -633,12 +633,15 @@ static SweepLayoutH layout_for(const pq_batch* b, const Sched& req, int lo = 0, 
   lay.off_sec1 = (int)o; o += mSND * 8;
   lay.off_sec2 = (int)o; o += mSND * 8;
   lay.off_mov = (int)o; o += align8(mSD * 8);
-  lay.off_wdet = (int)o; o += PQ_WAVE * 8;
+  // per-matrix determinants and error flags: one pass of node matrices holds
+  // up to team / 4 of them (four-lane groups), a section up to Smax
+  const int64_t nwd = std::max<int64_t>(PQ_WAVE, std::max<int64_t>(mS, team / 4));
+  lay.off_wdet = (int)o; o += nwd * 8;
   lay.off_wscale = (int)o; o += align8(mS * 8);
   lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
   lay.off_red = (int)o; o += 8 * 8;
   lay.off_cnt = (int)o; o += 24 * 8;
-  lay.off_werr = (int)o; o += PQ_WAVE * 4;
+  lay.off_werr = (int)o; o += nwd * 4;
   lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags
   lay.off_tabofs = (int)o; o += PQ_MAX_PAIRS * 4;
   o = align8(o);

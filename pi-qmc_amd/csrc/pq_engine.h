@@ -310,6 +310,7 @@ struct Eng {
   int chain;
   int tid;
   int lane;
+  uint32_t moved_done = 0;   // per node action (8 bits each): moved slices evaluated this move
   int wave;
   int ins;          // row stride of the inverse cache (largest node species)
   double* red;      // [NT/64] block-reduction scratch
@@ -986,9 +987,32 @@ struct Eng {
     return sb(sec1, t, p);
   }
 
+  // Slice of position p in the coarse-first order of a move's changed slices:
+  // the last slice first when a cycle moved it (the top level tests it), then
+  // level L-1's slice, level L-2's two, ..., level 0's 2^(L-1), each level in
+  // increasing slice order.  Level l holds slices 2^l (2i + 1).
+  __device__ __forceinline__ int moved_slice(int p) const {
+    if (!top_known) {
+      if (p == 0) return c.S - 1;
+      --p;
+    }
+    const int q = p + 1;
+    const int lg = 31 - __clz(q);
+    return (1 << (c.L - 1 - lg)) * (2 * (q - (1 << lg)) + 1);
+  }
+  // positions of that order a test at the given level needs evaluated
+  __device__ __forceinline__ int moved_needed(int level) const {
+    const int top = top_known ? 0 : 1;
+    return level >= c.L ? top : top + (1 << (c.L - level)) - 1;
+  }
+
   // NodeModel::evaluate (SHONodes.cc:51-128, FreeParticleNodes.cc:72-157) on
-  // slices t0, t0+step, ... (nt of them) with the given scale; inverses go to
-  // the cache, determinants and error flags to wdet/werr[0..nt).
+  // slices t0, t0+step, ... (nt of them), or with step 0 on the moved slices
+  // at positions t0..t0+nt-1 of moved_slice's order, with the given scale;
+  // inverses go to the cache, determinants and error flags to wdet/werr[0..nt).
+  __device__ __forceinline__ int matrix_slice(int t0, int step, int m) const {
+    return step ? t0 + m * step : moved_slice(t0 + m);
+  }
   __device__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, double scl) {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
@@ -998,7 +1022,7 @@ struct Eng {
     for (int q = tid; q < nt * per; q += NT) {
       const int m = q / per, rem = q - m * per;
       const int ii = rem / n, jj = rem - ii * n;
-      const int t = t0 + m * step;
+      const int t = matrix_slice(t0, step, m);
       const double* ri = sb(sec2, t, f.ifirst + ii);
       const double* rj = r1pos(t, f.ifirst + jj, use_mov);
       double gf[D];
@@ -1015,7 +1039,7 @@ struct Eng {
       for (int base = 0; base < nt; base += NW) {
         const int m = base + wave;
         if (m < nt) {
-          const int t = t0 + m * step;
+          const int t = matrix_slice(t0, step, m);
           double* slot = inv + (int64_t)t * ins * ins;
           pq_d4 acc[2][2];
 #pragma unroll
@@ -1056,7 +1080,7 @@ struct Eng {
       if (base + wave * GPW < nt) {
         const int m = base + g;
         const bool live = m < nt;
-        const int t = t0 + m * step;
+        const int t = live ? matrix_slice(t0, step, m) : t0;
         double a[NC];
         const double* src = inv + ((int64_t)t * ins + r) * ins;
 #pragma unroll
@@ -1401,28 +1425,43 @@ struct Eng {
     return total;
   }
 
+  // FixedNodeAction::getActionDifference evaluates, at each level, the
+  // determinants of the slices that level moves (FixedNodeAction.cc:59-74).
   // The sampler proposes every level of a move before trying any of them, so
-  // FixedNodeAction::getActionDifference's determinants of all the slices a
-  // move changes are evaluated here in one pass instead of one pass per
-  // level.  newdm(t) holds what the level that moves slice t evaluates; an
-  // evaluation that errs is stored as 0, which the sign test rejects, as the
-  // reference rejects the error.  The top slice comes from initialize when
-  // no cycle changed it.
+  // the slices can be evaluated ahead of their level: fn_diff_moved evaluates
+  // them coarse level first, a pass of the team's matrices at a time (the
+  // pass a level needs is filled with the next levels' slices), and a move
+  // rejected at a coarse level never evaluates the finer ones.  newdm(t)
+  // holds what the level that moves slice t evaluates; an evaluation that
+  // errs is stored as 0, which the sign test rejects, as the reference
+  // rejects the error.  The top slice comes from initialize when no cycle
+  // changed it.  fn_eval_moved starts a move (or the delayed second
+  // proposal) with nothing evaluated.
   __device__ void fn_eval_moved() {
+    moved_done = 0;
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
       bool mine = false;
       for (int m = 0; m < nmov(); ++m) mine |= mv(m) >= f.ifirst && mv(m) < f.ifirst + f.ns;
       if (tid == 0) notmine[fa] = mine ? 0 : 1;
-      if (!mine) continue;
-      double* ndm = fn_newdm(fa);
-      const int nt = top_known ? c.S - 2 : c.S - 1;
-      node_matrices(fa, 1, 1, nt, true, scale[fa]);
-      for (int m = tid; m < nt; m += NT) ndm[1 + m] = werr[m] ? 0.0 : wdet[m];
-      if (top_known && tid == 0) ndm[c.S - 1] = lasterr[fa] ? 0.0 : fn_dm(fa)[c.S - 1];
-      __syncthreads();
+      if (mine && top_known && tid == 0) fn_newdm(fa)[c.S - 1] = lasterr[fa] ? 0.0 : fn_dm(fa)[c.S - 1];
     }
     __syncthreads();
+  }
+
+  // evaluate node action fa's moved slices up to those the given level tests
+  __device__ void fn_eval_level(int fa, int level) {
+    const int done = (moved_done >> (8 * fa)) & 0xFF;
+    const int need = moved_needed(level);
+    if (done >= need) return;
+    constexpr int PASS = NC == 32 ? NT / PQ_WAVE : NT / G;
+    const int total = moved_needed(0);
+    const int upto = min(total, done + (need - done + PASS - 1) / PASS * PASS);
+    node_matrices(fa, done, 0, upto - done, true, scale[fa]);
+    double* ndm = fn_newdm(fa);
+    for (int m = tid; m < upto - done; m += NT) ndm[moved_slice(done + m)] = werr[m] ? 0.0 : wdet[m];
+    __syncthreads();
+    moved_done = (moved_done & ~(0xFFu << (8 * fa))) | ((uint32_t)upto << (8 * fa));
   }
 
   // FixedNodeAction::getActionDifference (FixedNodeAction.cc:39-131) at one
@@ -1475,6 +1514,7 @@ struct Eng {
     for (int fa = 0; fa < c.nfn; ++fa) {
       if (notmine[fa]) continue;
       const FnP& f = c.fn[fa];
+      fn_eval_level(fa, level);
       const int nStride = 1 << (level + 1);
       const double dm0 = fn_dm(fa)[0];
       const double* ndm = fn_newdm(fa);
