@@ -146,19 +146,129 @@ static_assert(sizeof(GjMfmaScratch) <= PQ_GJ32_SCRATCH_DOUBLES * sizeof(double),
 
 typedef double pq_d4 __attribute__((ext_vector_type(4)));
 
-// element (16 rt + 4 r + lane/16, column tile ct) of the accumulator tiles for a
-// wave-uniform rt, r
-__device__ __forceinline__ double acc_row(const pq_d4 (&acc)[2][2], int rt, int ct, int r) {
-  switch (rt * 4 + r) {
-    case 0: return acc[0][ct][0];
-    case 1: return acc[0][ct][1];
-    case 2: return acc[0][ct][2];
-    case 3: return acc[0][ct][3];
-    case 4: return acc[1][ct][0];
-    case 5: return acc[1][ct][1];
-    case 6: return acc[1][ct][2];
-    default: return acc[1][ct][3];
+// element (16 rt + 4 r + lane/16, column tile CT) of the accumulator tiles for
+// a wave-uniform k = 4 rt + r, as a chain of value selects (an index into the
+// tiles at run time would move them to scratch memory)
+template <int CT>
+__device__ __forceinline__ double acc_row(const pq_d4 (&acc)[2][2], int k) {
+  double v = acc[0][CT][0];
+  v = k == 1 ? acc[0][CT][1] : v;
+  v = k == 2 ? acc[0][CT][2] : v;
+  v = k == 3 ? acc[0][CT][3] : v;
+  v = k == 4 ? acc[1][CT][0] : v;
+  v = k == 5 ? acc[1][CT][1] : v;
+  v = k == 6 ? acc[1][CT][2] : v;
+  v = k == 7 ? acc[1][CT][3] : v;
+  return v;
+}
+
+// One panel (columns 4 kb .. 4 kb + 3, in column tile CT = kb / 4) of the
+// 32-wide inversion below.
+template <int CT>
+__device__ __forceinline__ void gj32_panel(pq_d4 (&acc)[2][2], int kb, GjMfmaScratch* sc, int lane,
+                                           int& mypos, double& det, int& err) {
+  const int row = lane & 31;
+  const bool act = lane < 32;
+  const int c0 = 4 * kb;
+  const int cc0 = c0 & 15;
+  const int jj0 = (lane & 15) - cc0;
+  const bool holds = jj0 >= 0 && jj0 < 4;
+  // (1) panel columns to LDS, [row][4]
+  if (holds) {
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sc->P[(16 * rt + 4 * r + (lane >> 4)) * 4 + jj0] = acc[rt][CT][r];
   }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  // (2) the panel's four steps, row per lane
+  double p[4], w[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) p[j] = act ? sc->P[row * 4 + j] : 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = c0 + j;
+    const uint64_t key = (act && mypos >= k)
+        ? ((__double_as_longlong(fabs(p[j])) & ~0x3Full) | (uint64_t)(63 - mypos)) : 0ull;
+    const double myinv = 1.0 / p[j];
+    const uint64_t kmax = group_max_u64<32>(key);
+    const int ppos = 63 - (int)(kmax & 0x3F);
+    const bool isp = act && key == kmax;
+    if (ppos != k) det = -det;
+    if (isp) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sc->prow[q] = p[q];
+      sc->prow[4] = myinv;
+      sc->pv[j] = row;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const double piv = sc->prow[j];
+    det *= piv;
+    const bool zero = piv == 0.0;
+    if (zero) err = 1;
+    const double ipiv = zero ? 0.0 : sc->prow[4];
+    const double f = (act && !isp) ? p[j] * ipiv : 0.0;
+    const double sp = (isp && !zero) ? ipiv : 1.0;
+    w[j] = (zero || !act) ? 0.0 : (isp ? 1.0 - ipiv : f);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q != j) p[q] = fma(-f, sc->prow[q], p[q]) * sp;
+    if (!zero && act) p[j] = isp ? ipiv : -f;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (isp) mypos = k;
+    else if (mypos == k) mypos = ppos;
+  }
+  // (3) final panel and multipliers to LDS
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { sc->P[row * 4 + q] = p[q]; sc->W[row * 4 + q] = w[q]; }
+  }
+  // (4) the pivot rows as they were before the panel, [4][column]
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int pr = sc->pv[j];
+    if ((lane >> 4) == (pr & 3)) {
+      const int k8 = (pr >> 4) * 4 + ((pr & 15) >> 2);
+      sc->RP[j * 32 + (lane & 15)] = acc_row<0>(acc, k8);
+      sc->RP[j * 32 + 16 + (lane & 15)] = acc_row<1>(acc, k8);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  // (5) s for every column outside the panel (lanes 0..31 are columns)
+  if (act) {
+    const int col = lane;
+    const bool inpanel = col >= c0 && col < c0 + 4;
+    const int p1 = sc->pv[1], p2 = sc->pv[2], p3 = sc->pv[3];
+    const double s0 = sc->RP[col];
+    const double s1 = sc->RP[32 + col] - sc->W[p1 * 4 + 0] * s0;
+    const double s2 = sc->RP[64 + col] - sc->W[p2 * 4 + 0] * s0 - sc->W[p2 * 4 + 1] * s1;
+    const double s3 = sc->RP[96 + col] - sc->W[p3 * 4 + 0] * s0 - sc->W[p3 * 4 + 1] * s1 -
+                      sc->W[p3 * 4 + 2] * s2;
+    sc->S[col] = inpanel ? 0.0 : s0;
+    sc->S[32 + col] = inpanel ? 0.0 : s1;
+    sc->S[64 + col] = inpanel ? 0.0 : s2;
+    sc->S[96 + col] = inpanel ? 0.0 : s3;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  // (6) rank-4 update of every tile: acc -= W s
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt) {
+    const double a = -sc->W[(16 * rt + (lane & 15)) * 4 + (lane >> 4)];
+#pragma unroll
+    for (int t2 = 0; t2 < 2; ++t2) {
+      const double bb = sc->S[(lane >> 4) * 32 + 16 * t2 + (lane & 15)];
+      acc[rt][t2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[rt][t2], 0, 0, 0);
+    }
+  }
+  // (7) the panel's columns take the row-per-lane result
+  if (holds) {
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[rt][CT][r] = sc->P[(16 * rt + 4 * r + (lane >> 4)) * 4 + jj0];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
 // In-place inversion of the n x n matrix held in acc (rows and columns n..31
@@ -173,111 +283,8 @@ __device__ __forceinline__ void gj_invert_mfma32(pq_d4 (&acc)[2][2], int n, GjMf
   int mypos = row;
   det = 1.0;
   err = 0;
-#pragma unroll
-  for (int kb = 0; kb < 8; ++kb) {
-    if (4 * kb >= n) break;
-    const int c0 = 4 * kb;
-    const int ct = kb >> 2;          // column tile of the panel
-    const int cc0 = c0 & 15;
-    const int jj0 = (lane & 15) - cc0;
-    const bool holds = jj0 >= 0 && jj0 < 4;
-    // (1) panel columns to LDS, [row][4]
-    if (holds) {
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) sc->P[(16 * rt + 4 * r + (lane >> 4)) * 4 + jj0] = acc[rt][ct][r];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // (2) the panel's four steps, row per lane
-    double p[4], w[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) p[j] = act ? sc->P[row * 4 + j] : 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = c0 + j;
-      const uint64_t key = (act && mypos >= k)
-          ? ((__double_as_longlong(fabs(p[j])) & ~0x3Full) | (uint64_t)(63 - mypos)) : 0ull;
-      const double myinv = 1.0 / p[j];
-      const uint64_t kmax = group_max_u64<32>(key);
-      const int ppos = 63 - (int)(kmax & 0x3F);
-      const bool isp = act && key == kmax;
-      if (ppos != k) det = -det;
-      if (isp) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sc->prow[q] = p[q];
-        sc->prow[4] = myinv;
-        sc->pv[j] = row;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      const double piv = sc->prow[j];
-      det *= piv;
-      const bool zero = piv == 0.0;
-      if (zero) err = 1;
-      const double ipiv = zero ? 0.0 : sc->prow[4];
-      const double f = (act && !isp) ? p[j] * ipiv : 0.0;
-      const double sp = (isp && !zero) ? ipiv : 1.0;
-      w[j] = (zero || !act) ? 0.0 : (isp ? 1.0 - ipiv : f);
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (q != j) p[q] = fma(-f, sc->prow[q], p[q]) * sp;
-      if (!zero && act) p[j] = isp ? ipiv : -f;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (isp) mypos = k;
-      else if (mypos == k) mypos = ppos;
-    }
-    // (3) final panel and multipliers to LDS
-    if (act) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) { sc->P[row * 4 + q] = p[q]; sc->W[row * 4 + q] = w[q]; }
-    }
-    // (4) the pivot rows as they were before the panel, [4][column]
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int pr = sc->pv[j];
-      if ((lane >> 4) == (pr & 3)) {
-#pragma unroll
-        for (int t2 = 0; t2 < 2; ++t2)
-          sc->RP[j * 32 + 16 * t2 + (lane & 15)] = acc_row(acc, pr >> 4, t2, (pr & 15) >> 2);
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // (5) s for every column outside the panel (lanes 0..31 are columns)
-    if (act) {
-      const int col = lane;
-      const bool inpanel = col >= c0 && col < c0 + 4;
-      const int p1 = sc->pv[1], p2 = sc->pv[2], p3 = sc->pv[3];
-      const double s0 = sc->RP[col];
-      const double s1 = sc->RP[32 + col] - sc->W[p1 * 4 + 0] * s0;
-      const double s2 = sc->RP[64 + col] - sc->W[p2 * 4 + 0] * s0 - sc->W[p2 * 4 + 1] * s1;
-      const double s3 = sc->RP[96 + col] - sc->W[p3 * 4 + 0] * s0 - sc->W[p3 * 4 + 1] * s1 -
-                        sc->W[p3 * 4 + 2] * s2;
-      sc->S[col] = inpanel ? 0.0 : s0;
-      sc->S[32 + col] = inpanel ? 0.0 : s1;
-      sc->S[64 + col] = inpanel ? 0.0 : s2;
-      sc->S[96 + col] = inpanel ? 0.0 : s3;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // (6) rank-4 update of every tile: acc -= W s
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt) {
-      const double a = -sc->W[(16 * rt + (lane & 15)) * 4 + (lane >> 4)];
-#pragma unroll
-      for (int t2 = 0; t2 < 2; ++t2) {
-        const double bb = sc->S[(lane >> 4) * 32 + 16 * t2 + (lane & 15)];
-        acc[rt][t2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[rt][t2], 0, 0, 0);
-      }
-    }
-    // (7) the panel's columns take the row-per-lane result
-    if (holds) {
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[rt][ct][r] = sc->P[(16 * rt + 4 * r + (lane >> 4)) * 4 + jj0];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  }
+  for (int kb = 0; kb < 4 && 4 * kb < n; ++kb) gj32_panel<0>(acc, kb, sc, lane, mypos, det, err);
+  for (int kb = 4; kb < 8 && 4 * kb < n; ++kb) gj32_panel<1>(acc, kb, sc, lane, mypos, det, err);
   if (act) {
     sc->pos[row] = mypos;
     sc->perm[mypos] = row;
@@ -1013,7 +1020,7 @@ struct Eng {
   __device__ __forceinline__ int matrix_slice(int t0, int step, int m) const {
     return step ? t0 + m * step : moved_slice(t0 + m);
   }
-  __device__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, double scl) {
+  __device__ __forceinline__ void node_matrices(int fa, int t0, int step, int nt, bool use_mov, double scl) {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
     // fill: every thread takes matrix elements; they are staged in the inverse
