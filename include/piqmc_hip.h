@@ -124,6 +124,10 @@ typedef struct pq_node_desc {
   int many_body_distance;  /* useManyBodyDistance="true": one distance per slice and
                               section, 1/sqrt(sum_i 1/d_i^2) over all particles
                               (FixedNodeAction.cc:99-116) */
+  int use_updates;     /* FreeParticleNodes useUpdates="true": determinant ratios and
+                          inverses of the moved slices by FreeParticleNodes::MatrixUpdate
+                          (FreeParticleNodes.cc:894-1058, FixedNodeAction.cc:61-92,366)
+                          instead of a new evaluation; PQ_NODES_FREE only */
 } pq_node_desc;
 
 /* <Sample npart species noPermutation nrepeat delayedRejection

@@ -789,6 +789,15 @@ typedef struct {
   pg_t pg[3];
   double *dm, *newdm, *dist, *newdist;  /* dist [S][2][ns] */
   double* inv;                           /* [S][ns*ns] latest inverse per slice */
+  /* FreeParticleNodes::MatrixUpdate (useUpdates): the moving particles' new
+     matrix columns phi [S][ns][PQ_MAX_MOVING], the updated inverses newinv
+     [S][ns*ns], the moving particles of this species (index1 - ifirst and
+     their positions mindex1 among the moving particles) and
+     isNewMatrixUpdated */
+  int upd;
+  double *phi, *newinv;
+  int nm, index1[PQ_MAX_MOVING], mindex1[PQ_MAX_MOVING];
+  int new_updated;
   int not_my_species;
   long nerror;
 } fn_t;
@@ -1004,6 +1013,11 @@ oq_chain* oq_create(const pq_chain_desc* desc) {
     f->dist = (double*)calloc((size_t)smax * 2 * f->ns, sizeof(double));
     f->newdist = (double*)calloc((size_t)smax * 2 * f->ns, sizeof(double));
     f->inv = (double*)calloc((size_t)smax * f->ns * f->ns, sizeof(double));
+    f->upd = nd->use_updates != 0;
+    if (f->upd) {
+      f->phi = (double*)calloc((size_t)smax * f->ns * PQ_MAX_MOVING, sizeof(double));
+      f->newinv = (double*)calloc((size_t)smax * f->ns * f->ns, sizeof(double));
+    }
   }
   size_t secsz = (size_t)smax * ch->N * ch->D;
   ch->sec1 = (double*)calloc(secsz, sizeof(double));
@@ -1025,6 +1039,7 @@ void oq_destroy(oq_chain* ch) {
   for (int a = 0; a < ch->nfn; ++a) {
     free(ch->fn[a].dm); free(ch->fn[a].newdm); free(ch->fn[a].dist);
     free(ch->fn[a].newdist); free(ch->fn[a].inv);
+    free(ch->fn[a].phi); free(ch->fn[a].newinv);
   }
   free(ch->spg); free(ch->mpg); free(ch->spg_on); free(ch->mpg_on);
   free(ch->beads); free(ch->sec1); free(ch->sec2); free(ch->mov); free(ch->rej);
@@ -1529,6 +1544,148 @@ void oq_node_distance(oq_chain* ch, int inode, const double* r1, const double* r
 }
 double oq_node_scale(const oq_chain* ch, int inode) { return ch->fn[inode].scale; }
 
+/* ------------------------------------ FreeParticleNodes::MatrixUpdate */
+
+/* determinant of the k x k row-major matrix a by LU with partial pivoting
+   (dgetf2 order, as lu_det_inverse); a is overwritten */
+static double small_det(double* a, int k) {
+  double det = 1;
+  for (int c = 0; c < k; ++c) {
+    int p = c;
+    double amax = fabs(a[c * k + c]);
+    for (int r = c + 1; r < k; ++r)
+      if (fabs(a[r * k + c]) > amax) { amax = fabs(a[r * k + c]); p = r; }
+    if (a[p * k + c] != 0.0) {
+      if (p != c)
+        for (int q = 0; q < k; ++q) { double t = a[c * k + q]; a[c * k + q] = a[p * k + q]; a[p * k + q] = t; }
+      for (int r = c + 1; r < k; ++r) a[r * k + c] /= a[c * k + c];
+    }
+    for (int r = c + 1; r < k; ++r)
+      for (int q = c + 1; q < k; ++q) a[r * k + q] -= a[r * k + c] * a[c * k + q];
+    det *= a[c * k + c];
+    det *= (p == c) ? 1 : -1;
+  }
+  return det;
+}
+
+/* MatrixUpdate::evaluateChange (FreeParticleNodes.cc:894-948): the moving
+   particles' new columns phi(k, jm) = scale prod_d PG(x_moving - x2_k) at
+   slice t, b(i, jm) = sum_k inv(i, k) phi(k, jm) on the moving rows, and the
+   determinant of that nm x nm block, the ratio det(new)/det(old). */
+static double upd_change(oq_chain* ch, fn_t* f, const double* mov, const int* idx, int nmov, int t) {
+  const int n = f->ns, D = ch->D;
+  const size_t slice = (size_t)ch->N * D;
+  f->new_updated = 0;
+  f->nm = 0;
+  for (int m = 0; m < nmov; ++m)
+    if (idx[m] >= f->ifirst && idx[m] < f->ifirst + n) {
+      f->index1[f->nm] = idx[m] - f->ifirst;
+      f->mindex1[f->nm] = m;
+      f->nm++;
+    }
+  const int nm = f->nm;
+  double* phi = &f->phi[(size_t)t * n * PQ_MAX_MOVING];
+  const double* inv = &f->inv[(size_t)t * n * n];
+  double b[PQ_MAX_MOVING * PQ_MAX_MOVING];
+  for (int jm = 0; jm < nm; ++jm) {
+    const double* rm = &mov[((size_t)t * nmov + f->mindex1[jm]) * D];
+    for (int k = 0; k < n; ++k) {
+      const double* r2 = &ch->sec2[t * slice + (size_t)(f->ifirst + k) * D];
+      double delta[3];
+      for (int d = 0; d < D; ++d) delta[d] = rm[d] - r2[d];
+      pbc(&ch->cell, delta);
+      double ear2 = f->scale;
+      for (int d = 0; d < D; ++d) ear2 *= pg_eval(&f->pg[d], delta[d]);
+      phi[k * PQ_MAX_MOVING + jm] = ear2;
+    }
+    for (int im = 0; im < nm; ++im) {
+      const int ip = f->index1[im];
+      double v = 0;
+      for (int k = 0; k < n; ++k) v += phi[k * PQ_MAX_MOVING + jm] * inv[ip * n + k];
+      b[im * nm + jm] = v;
+    }
+  }
+  return small_det(b, nm);
+}
+
+/* the Sherman-Morrison row updates of evaluateNewInverse and of
+   acceptLastMove (FreeParticleNodes.cc:950-974, 1026-1057) on m (n x n),
+   one moving particle after the other */
+static void upd_rows(const fn_t* f, double* m, const double* phi) {
+  const int n = f->ns;
+  for (int jm = 0; jm < f->nm; ++jm) {
+    const int jp = f->index1[jm];
+    double bjjinv = 0;
+    for (int k = 0; k < n; ++k) bjjinv += phi[k * PQ_MAX_MOVING + jm] * m[jp * n + k];
+    bjjinv = 1. / bjjinv;
+    for (int k = 0; k < n; ++k) m[jp * n + k] *= bjjinv;
+    for (int ip = 0; ip < n; ++ip) {
+      if (ip == jp) continue;
+      double bij = 0;
+      for (int k = 0; k < n; ++k) bij += phi[k * PQ_MAX_MOVING + jm] * m[ip * n + k];
+      for (int k = 0; k < n; ++k) m[ip * n + k] -= m[jp * n + k] * bij;
+    }
+  }
+}
+
+/* MatrixUpdate::evaluateNewInverse (FreeParticleNodes.cc:950-974) */
+static void upd_new_inverse(oq_chain* ch, fn_t* f, int t) {
+  const int n = f->ns;
+  double* m = &f->newinv[(size_t)t * n * n];
+  memcpy(m, &f->inv[(size_t)t * n * n], sizeof(double) * n * n);
+  upd_rows(f, m, &f->phi[(size_t)t * n * PQ_MAX_MOVING]);
+  f->new_updated = 1;
+}
+
+/* MatrixUpdate::evaluateNewDistance (FreeParticleNodes.cc:976-1024) with the
+   updated inverse.  Unlike evaluateDistance it leaves the scale out of the
+   log gradient, and in its first loop (d1) each gradient component is
+   multiplied by its own dimension's value only (grad[i] *= value[i]); the
+   second loop (d2) multiplies the whole vector by every value. */
+static void upd_distance(oq_chain* ch, fn_t* f, const double* r1, const double* r2, int t,
+                         double* d1, double* d2) {
+  const int n = f->ns, D = ch->D;
+  const double* m = &f->newinv[(size_t)t * n * n];
+  for (int pass = 0; pass < 2; ++pass)
+    for (int a = 0; a < n; ++a) {
+      double lg[3] = {0, 0, 0};
+      for (int b = 0; b < n; ++b) {
+        const int jp = pass == 0 ? a : b, ip = pass == 0 ? b : a;
+        const double* rj = &r1[(size_t)(f->ifirst + jp) * D];
+        const double* ri = &r2[(size_t)(f->ifirst + ip) * D];
+        double delta[3], value[3], grad[3];
+        for (int d = 0; d < D; ++d) delta[d] = pass == 0 ? rj[d] - ri[d] : ri[d] - rj[d];
+        pbc(&ch->cell, delta);
+        for (int d = 0; d < D; ++d) {
+          double gr;
+          value[d] = pg_eval3(&f->pg[d], delta[d], &gr, 0);
+          grad[d] = gr / (value[d] + 1e-300);
+        }
+        if (pass == 0) {
+          for (int d = 0; d < D; ++d) grad[d] *= value[d];
+        } else {
+          for (int k = 0; k < D; ++k) for (int d = 0; d < D; ++d) grad[d] *= value[k];
+        }
+        const double w = m[jp * n + ip];
+        for (int d = 0; d < D; ++d) lg[d] += w * grad[d];
+      }
+      const double dist = sqrt(2 * f->mass / ((dot3(lg, lg, D) + 1e-15) * ch->tau));
+      if (pass == 0) d1[a] = dist; else d2[a] = dist;
+    }
+}
+
+/* MatrixUpdate::acceptLastMove (FreeParticleNodes.cc:1026-1057): slices
+   1..S-2 take the updated inverses, or are updated in place when no level-0
+   nodal action made them; the last slice keeps its inverse */
+static void upd_accept(oq_chain* ch, fn_t* f) {
+  const int n = f->ns;
+  for (int t = 1; t < ch->S - 1; ++t) {
+    double* m = &f->inv[(size_t)t * n * n];
+    if (f->new_updated) memcpy(m, &f->newinv[(size_t)t * n * n], sizeof(double) * n * n);
+    else upd_rows(f, m, &f->phi[(size_t)t * n * PQ_MAX_MOVING]);
+  }
+}
+
 /* FixedNodeAction::initialize (src/fixednode/FixedNodeAction.cc:331-357) */
 void oq_fn_initialize(oq_chain* ch, const double* sec1, const double* sec2) {
   const size_t slice = (size_t)ch->N * ch->D;
@@ -1571,13 +1728,18 @@ double oq_fn_action_difference(oq_chain* ch, const double* mov, const int* idx,
     const int nStride = 1 << (level + 1);
     int crossed = 0;
     for (int t = nStride / 2; t < ch->S; t += nStride) {
-      memcpy(r1, &ch->sec1[t * slice], sizeof(double) * slice);
-      for (int m = 0; m < nmov; ++m)
-        for (int d = 0; d < D; ++d) r1[idx[m] * D + d] = mov[((size_t)t * nmov + m) * D + d];
-      double det;
-      int err = node_evaluate(ch, f, r1, &ch->sec2[t * slice], t, 0, &det);
-      if (err) { crossed = 1; break; }
-      f->newdm[t] = det;
+      if (f->upd) {
+        /* newDMValue = evaluateChange * dmValue (FixedNodeAction.cc:61-63) */
+        f->newdm[t] = upd_change(ch, f, mov, idx, nmov, t) * f->dm[t];
+      } else {
+        memcpy(r1, &ch->sec1[t * slice], sizeof(double) * slice);
+        for (int m = 0; m < nmov; ++m)
+          for (int d = 0; d < D; ++d) r1[idx[m] * D + d] = mov[((size_t)t * nmov + m) * D + d];
+        double det;
+        int err = node_evaluate(ch, f, r1, &ch->sec2[t * slice], t, 0, &det);
+        if (err) { crossed = 1; break; }
+        f->newdm[t] = det;
+      }
       if (f->newdm[t] * f->dm[0] <= 1e-200) { crossed = 1; break; }
     }
     if (crossed) { dA = 2e100; total += dA; continue; }
@@ -1586,8 +1748,15 @@ double oq_fn_action_difference(oq_chain* ch, const double* mov, const int* idx,
         memcpy(r1, &ch->sec1[t * slice], sizeof(double) * slice);
         for (int m = 0; m < nmov; ++m)
           for (int d = 0; d < D; ++d) r1[idx[m] * D + d] = mov[((size_t)t * nmov + m) * D + d];
-        node_distance(ch, f, r1, &ch->sec2[t * slice], t, &f->newdist[(t * 2 + 0) * ns],
-                      &f->newdist[(t * 2 + 1) * ns]);
+        if (f->upd) {
+          /* evaluateNewInverse + evaluateNewDistance (FixedNodeAction.cc:79-91) */
+          upd_new_inverse(ch, f, t);
+          upd_distance(ch, f, r1, &ch->sec2[t * slice], t, &f->newdist[(t * 2 + 0) * ns],
+                       &f->newdist[(t * 2 + 1) * ns]);
+        } else {
+          node_distance(ch, f, r1, &ch->sec2[t * slice], t, &f->newdist[(t * 2 + 0) * ns],
+                        &f->newdist[(t * 2 + 1) * ns]);
+        }
         const double* od = f->dist; const double* nd = f->newdist;
         if (ch->d.nodes[a].many_body_distance) {
           /* useManyBodyDistance (FixedNodeAction.cc:99-116): the sums run
@@ -1621,6 +1790,12 @@ double oq_fn_action_difference(oq_chain* ch, const double* mov, const int* idx,
   return total;
 }
 
+void oq_fn_state(const oq_chain* ch, int inode, int which, double* det, double* dist) {
+  const fn_t* f = &ch->fn[inode];
+  if (det) memcpy(det, which ? f->newdm : f->dm, sizeof(double) * ch->S);
+  if (dist) memcpy(dist, which ? f->newdist : f->dist, sizeof(double) * ch->S * 2 * f->ns);
+}
+
 /* FixedNodeAction::acceptLastMove (src/fixednode/FixedNodeAction.cc:359-367) */
 void oq_fn_accept(oq_chain* ch) {
   for (int a = 0; a < ch->nfn; ++a) {
@@ -1628,6 +1803,7 @@ void oq_fn_accept(oq_chain* ch) {
     if (f->not_my_species) continue;
     memcpy(f->dm, f->newdm, sizeof(double) * ch->S);
     memcpy(f->dist, f->newdist, sizeof(double) * ch->S * 2 * f->ns);
+    if (f->upd) upd_accept(ch, f);
   }
 }
 

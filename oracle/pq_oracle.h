@@ -70,6 +70,9 @@ void oq_fn_initialize(oq_chain* ch, const double* sec1, const double* sec2);
 double oq_fn_action_difference(oq_chain* ch, const double* mov, const int* idx,
                                int nmov, int level);
 void oq_fn_accept(oq_chain* ch);
+/* node action inode's determinants: dmValue (which 0) or newDMValue (1) of
+   the S section slices, and its distances dist / newDist [S][2][ns] */
+void oq_fn_state(const oq_chain* ch, int inode, int which, double* det, double* dist);
 
 /* mover on explicit section/moving buffers (mov modified in place) */
 double oq_mover(oq_chain* ch, const double* sec, double* mov, const int* idx,

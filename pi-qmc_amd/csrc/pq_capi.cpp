@@ -217,6 +217,7 @@ struct pq_batch {
   double* dbeads = nullptr;
   double* dinv = nullptr;
   double* degr = nullptr;
+  double* dupd = nullptr;   // MatrixUpdate blocks
   double* dfn = nullptr;
   double* dfncache = nullptr;   // slice-pair node records of the sampler
   double* dfnsweep = nullptr;   // sampler fixed-node state when it does not fit LDS
@@ -276,6 +277,7 @@ static DevCtx make_ctx(const pq_batch* b) {
   x.beads = b->dbeads;
   x.inv = b->dinv;
   x.egr = b->degr;
+  x.upd = b->dupd;
   x.fnstate = b->dfn;
   x.fncache = b->dfncache;
   x.fnsweep = b->dfnsweep;
@@ -499,6 +501,9 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
     if (f.ns > PQ_MAX_NODE_PART) return fail(PQ_ERR_ARG, "node species of %d particles exceeds %d", f.ns, PQ_MAX_NODE_PART);
     f.nodal = nd.nodal_action;
     f.manybody = nd.many_body_distance ? 1 : 0;
+    f.updates = nd.use_updates ? 1 : 0;
+    if (f.updates && nd.model != PQ_NODES_FREE)
+      return fail(PQ_ERR_ARG, "use_updates is a FreeParticleNodes option (FreeParticleNodes.cc:60)");
     f.mass = c.mass[nd.species];
     const double Tn = nd.temperature == 0 ? T : nd.temperature;
     // SHONodes constructor (src/fixednode/SHONodes.cc:22-35)
@@ -567,6 +572,12 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
       const bool permuting = (ch == PQ_CHOOSER_WALKING || ch == PQ_CHOOSER_ALL_RANDOM) && smp[k].nmoving > 1;
       if (permuting && c.M < (1 << L) + 1)
         return fail(PQ_ERR_ARG, "sample %d permutes particles: nslice %d must exceed 2^L = %d", k, c.M, 1 << L);
+      // the reference sizes MatrixUpdate for 3 moving particles of the
+      // species (maxMovers, ActionParser.cc:809-811)
+      for (int a = 0; a < c.nfn; ++a)
+        if (c.fn[a].updates && (all || smp[k].species == c.fn[a].sp) && smp[k].nmoving > 3)
+          return fail(PQ_ERR_ARG, "sample %d: nmoving %d exceeds the 3 moving particles of use_updates", k,
+                      smp[k].nmoving);
     }
   }
   c.nsample = c.grp[0].nsample;
@@ -750,7 +761,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   b->hc.resize(nchain);
   b->D = chains[0].ndim;
   std::map<std::tuple<double, double, double, double, double, int, int, int>, int> tabs;
-  int64_t beads = 0, inv = 0, fnst = 0, egr = 0, fnc = 0;
+  int64_t beads = 0, inv = 0, fnst = 0, egr = 0, fnc = 0, upd = 0;
   int ns_max = 1;
   for (int c = 0; c < nchain; ++c) {
     if (chains[c].ndim != b->D) { delete b; return fail(PQ_ERR_ARG, "all chains of a batch share ndim"); }
@@ -785,6 +796,11 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     if (ch.nfn) inv += (int64_t)ch.Smax * ch.ns_max * ch.ns_max;
     ch.egr_off = egr;
     if (ch.nfn) egr += (int64_t)ch.Smax * ch.ns_max * ch.ns_max * ch.D;
+    ch.upd_off = -1;
+    if (chain_has_updates(ch)) {
+      ch.upd_off = upd;
+      upd += ch.nfn * upd_block_doubles(ch);
+    }
     ch.fn_off = fnst;
     fnst += fn_state_doubles(ch);
     const int64_t nrec = fn_cache_doubles(ch);
@@ -852,6 +868,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
             dalloc((void**)&b->dbeads, sizeof(double) * beads) &&
             dalloc((void**)&b->dinv, sizeof(double) * inv) &&
             dalloc((void**)&b->degr, sizeof(double) * egr) &&
+            dalloc((void**)&b->dupd, sizeof(double) * upd) &&
             dalloc((void**)&b->dfn, sizeof(double) * fnst) &&
             dalloc((void**)&b->dfncache, sizeof(double) * fnc) &&
             dalloc((void**)&b->dfnsweep, sizeof(double) * fnst) &&
@@ -888,7 +905,8 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
       hipMemset(b->dstats, 0, sizeof(pq_chain_stats) * nchain) != hipSuccess ||
       hipMemset(b->dbeads, 0, sizeof(double) * std::max<int64_t>(1, beads)) != hipSuccess ||
       hipMemset(b->dfn, 0, sizeof(double) * std::max<int64_t>(1, fnst)) != hipSuccess ||
-      hipMemset(b->dfncache, 0, sizeof(double) * std::max<int64_t>(1, fnc)) != hipSuccess)
+      hipMemset(b->dfncache, 0, sizeof(double) * std::max<int64_t>(1, fnc)) != hipSuccess ||
+      hipMemset(b->dupd, 0, sizeof(double) * std::max<int64_t>(1, upd)) != hipSuccess)
     return cleanup_fail(fail(PQ_ERR_HIP, "device upload failed"));
   *out = b;
   return PQ_OK;
@@ -901,7 +919,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache, b->dfnsweep,
-                  b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr,
+                  b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr, b->dupd,
                   b->dperm, b->dewk, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est,
                   b->ditems_e, b->ditems_g, b->dfirst_e, b->dpart, b->dcf, b->dplan, b->dbead_stage};
   for (void* p : ptrs)

@@ -30,6 +30,7 @@ struct DevCtx {
   double* beads;
   double* inv;            // per-chain [S][ns_max][ns_max] inverse cache
   double* egr;            // per-chain [S][ns_max][ns_max][D] density-matrix gradient factors
+  double* upd;            // per-chain MatrixUpdate blocks of the node actions with useUpdates
   double* fnstate;        // per-chain fixed-node state (call-level API)
   double* fncache;        // per-chain node records of every slice pair (sampler)
   double* fnsweep;        // per-chain fixed-node state of the sampler when not in LDS

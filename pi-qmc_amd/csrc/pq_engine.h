@@ -1208,6 +1208,234 @@ struct Eng {
     __syncthreads();
   }
 
+  // ------------------------------------- FreeParticleNodes::MatrixUpdate
+  // Node actions with useUpdates keep, through a section, the inverse of
+  // every slice's matrix from initialize (FixedNodeAction::initialize
+  // evaluates them all) in their own block, and a move evaluates the
+  // determinant ratio of the moving particles' new columns instead of the
+  // new matrix (FreeParticleNodes.cc:894-1058).
+  __device__ __forceinline__ double* upd_block(int fa) const {
+    return x.upd + c.upd_off + (int64_t)fa * upd_block_doubles(c);
+  }
+  __device__ __forceinline__ double* upd_inv(int fa) const { return upd_block(fa) + 8; }
+  __device__ __forceinline__ double* upd_new(int fa) const {
+    return upd_inv(fa) + (int64_t)c.Smax * ins * ins;
+  }
+  __device__ __forceinline__ double* upd_phi(int fa) const {
+    return upd_new(fa) + (int64_t)c.Smax * ins * ins;
+  }
+  __device__ __forceinline__ double* upd_scr(int fa) const {
+    return upd_phi(fa) + (int64_t)c.Smax * ins * PQ_MAX_MOVING;
+  }
+  // the moving particles of the node action's species, in the order of the
+  // moving particles: how many, and the position of the jm-th among them
+  __device__ __forceinline__ int upd_count(const FnP& f) const {
+    int k = 0;
+    for (int m = 0; m < nmov(); ++m) k += mv(m) >= f.ifirst && mv(m) < f.ifirst + f.ns;
+    return k;
+  }
+  __device__ __forceinline__ int upd_moving(const FnP& f, int jm) const {
+    for (int m = 0; m < nmov(); ++m)
+      if (mv(m) >= f.ifirst && mv(m) < f.ifirst + f.ns) {
+        if (jm == 0) return m;
+        --jm;
+      }
+    return 0;
+  }
+
+  // MatrixUpdate::evaluateChange (FreeParticleNodes.cc:894-948) on slices t0,
+  // t0+step, ... (nt of them): the moving particles' new columns
+  // phi(k, jm) = scale prod_d PG(x_jm - x2_k), the block
+  // b(im, jm) = sum_k phi(k, jm) inv(index1(im), k) and its determinant by
+  // LU with partial pivoting (dgetf2 order), the ratio det(new)/det(old);
+  // newDMValue = ratio * dmValue (FixedNodeAction.cc:61-63).
+  __device__ void upd_change(int fa, int t0, int step, int nt) {
+    const FnP& f = c.fn[fa];
+    const int n = f.ns, nm = upd_count(f);
+    const double scl = scale[fa];
+    double* phi = upd_phi(fa);
+    const double* bi = upd_inv(fa);
+    double* scr = upd_scr(fa);
+    if (tid == 0) upd_block(fa)[0] = 0.0;   // isNewMatrixUpdated = false
+    for (int q = tid; q < nt * n * nm; q += NT) {
+      const int sl = q / (n * nm), rem = q - sl * n * nm;
+      const int k = rem / nm, jm = rem - k * nm;
+      const int t = t0 + sl * step;
+      double gf[D];
+      phi[((int64_t)t * ins + k) * PQ_MAX_MOVING + jm] =
+          elem(f, mov + (t * nmov() + upd_moving(f, jm)) * D, sb(sec2, t, f.ifirst + k), scl, gf);
+    }
+    __syncthreads();
+    for (int q = tid; q < nt * nm * nm; q += NT) {
+      const int sl = q / (nm * nm), rem = q - sl * nm * nm;
+      const int im = rem / nm, jm = rem - im * nm;
+      const int t = t0 + sl * step;
+      const double* row = bi + ((int64_t)t * ins + (mv(upd_moving(f, im)) - f.ifirst)) * ins;
+      const double* ph = phi + (int64_t)t * ins * PQ_MAX_MOVING + jm;
+      double v = 0;
+      for (int k = 0; k < n; ++k) v += ph[k * PQ_MAX_MOVING] * row[k];
+      scr[(int64_t)t * 64 + im * nm + jm] = v;
+    }
+    __syncthreads();
+    double* ndm = fn_newdm(fa);
+    const double* dm = fn_dm(fa);
+    for (int sl = tid; sl < nt; sl += NT) {
+      const int t = t0 + sl * step;
+      double* a = scr + (int64_t)t * 64;
+      double det = 1;
+      for (int cc = 0; cc < nm; ++cc) {
+        int p = cc;
+        double amax = fabs(a[cc * nm + cc]);
+        for (int r = cc + 1; r < nm; ++r)
+          if (fabs(a[r * nm + cc]) > amax) { amax = fabs(a[r * nm + cc]); p = r; }
+        if (a[p * nm + cc] != 0.0) {
+          if (p != cc)
+            for (int q = 0; q < nm; ++q) {
+              const double tmp = a[cc * nm + q];
+              a[cc * nm + q] = a[p * nm + q];
+              a[p * nm + q] = tmp;
+            }
+          for (int r = cc + 1; r < nm; ++r) a[r * nm + cc] /= a[cc * nm + cc];
+        }
+        for (int r = cc + 1; r < nm; ++r)
+          for (int q = cc + 1; q < nm; ++q) a[r * nm + q] -= a[r * nm + cc] * a[cc * nm + q];
+        det *= a[cc * nm + cc];
+        det *= (p == cc) ? 1 : -1;
+      }
+      ndm[t] = det * dm[t];
+    }
+    __syncthreads();
+  }
+
+  // The Sherman-Morrison row updates of evaluateNewInverse and acceptLastMove
+  // (FreeParticleNodes.cc:950-974, 1026-1057) on the matrices m of slices
+  // t0..t0+nt-1, one moving particle jm after the other: with
+  // b_i = sum_k phi(k, jm) m(i, k), row jp (the particle) is scaled by 1/b_jp
+  // and every other row takes m(i, k) -= m(jp, k) b_i.
+  __device__ void upd_rows(int fa, double* m, int t0, int nt) {
+    const FnP& f = c.fn[fa];
+    const int n = f.ns, nm = upd_count(f);
+    const double* phi = upd_phi(fa);
+    double* scr = upd_scr(fa);
+    for (int jm = 0; jm < nm; ++jm) {
+      const int jp = mv(upd_moving(f, jm)) - f.ifirst;
+      for (int q = tid; q < nt * n; q += NT) {
+        const int t = t0 + q / n, i = q % n;
+        const double* row = m + ((int64_t)t * ins + i) * ins;
+        const double* ph = phi + (int64_t)t * ins * PQ_MAX_MOVING + jm;
+        double v = 0;
+        for (int k = 0; k < n; ++k) v += ph[k * PQ_MAX_MOVING] * row[k];
+        scr[(int64_t)t * 64 + i] = v;
+      }
+      __syncthreads();
+      for (int q = tid; q < nt * n * n; q += NT) {
+        const int t = t0 + q / (n * n), rem = q % (n * n);
+        const int i = rem / n, k = rem % n;
+        if (i == jp) continue;
+        double* mt = m + (int64_t)t * ins * ins;
+        const double sj = mt[jp * ins + k] * (1. / scr[(int64_t)t * 64 + jp]);
+        mt[i * ins + k] -= sj * scr[(int64_t)t * 64 + i];
+      }
+      __syncthreads();
+      for (int q = tid; q < nt * n; q += NT) {
+        const int t = t0 + q / n, k = q % n;
+        m[((int64_t)t * ins + jp) * ins + k] *= 1. / scr[(int64_t)t * 64 + jp];
+      }
+      __syncthreads();
+    }
+  }
+
+  // MatrixUpdate::evaluateNewInverse (FreeParticleNodes.cc:950-974) of slices
+  // 1..S-1 from the section's inverses and the move's new columns
+  __device__ void upd_new_inverse(int fa) {
+    const int n = c.fn[fa].ns, nt = c.S - 1;
+    const double* bi = upd_inv(fa);
+    double* nw = upd_new(fa);
+    for (int q = tid; q < nt * n * n; q += NT) {
+      const int t = 1 + q / (n * n), rem = q % (n * n);
+      const int64_t e = ((int64_t)t * ins + rem / n) * ins + rem % n;
+      nw[e] = bi[e];
+    }
+    __syncthreads();
+    upd_rows(fa, nw, 1, nt);
+    if (tid == 0) upd_block(fa)[0] = 1.0;   // isNewMatrixUpdated
+    __syncthreads();
+  }
+
+  // MatrixUpdate::evaluateNewDistance (FreeParticleNodes.cc:976-1024) for
+  // slices 1..S-1 into dist[t][2][ns] from the updated inverses.  Unlike
+  // evaluateDistance it leaves the scale out of the log gradient, and in its
+  // first loop (d1) each gradient component takes its own dimension's value
+  // only (grad[i] *= value[i]); the second loop (d2) multiplies the whole
+  // vector by every value.
+  __device__ void upd_distances(int fa, double* dist) {
+    const FnP& f = c.fn[fa];
+    const int n = f.ns, nt = c.S - 1;
+    const double* mat = upd_new(fa);
+    for (int it = tid; it < nt * 2 * n; it += NT) {
+      const int t = 1 + it / (2 * n);
+      const int pass = (it / n) & 1;
+      const int aa = it % n;
+      const double* mt = mat + (int64_t)t * ins * ins;
+      double lg[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) lg[d] = 0;
+      for (int bb = 0; bb < n; ++bb) {
+        const int jp = pass == 0 ? aa : bb, ip = pass == 0 ? bb : aa;
+        const double* rj = r1pos(t, f.ifirst + jp, true);
+        const double* ri = sb(sec2, t, f.ifirst + ip);
+        double delta[D], value[D], grad[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) delta[d] = pass == 0 ? rj[d] - ri[d] : ri[d] - rj[d];
+        pbc<D>(c, delta);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          double gr;
+          if (f.pg_img[d]) value[d] = pg_images<double>(f.pg_alpha[d], f.pg_len[d], delta[d], &gr);
+          else value[d] = pg_eval(f.pg_nome[d], f.pg_pref[d], f.pg_k[d], f.pg_nmax[d], delta[d], &gr);
+          grad[d] = ratio<double>(gr, value[d]);
+        }
+        if (pass == 0) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) grad[d] *= value[d];
+        } else {
+#pragma unroll
+          for (int k = 0; k < D; ++k)
+#pragma unroll
+            for (int d = 0; d < D; ++d) grad[d] *= value[k];
+        }
+        const double w = mt[jp * ins + ip];
+#pragma unroll
+        for (int d = 0; d < D; ++d) lg[d] += w * grad[d];
+      }
+      double g2 = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) g2 += lg[d] * lg[d];
+      dist[((int64_t)t * 2 + pass) * n + aa] = sqrt(2 * f.mass / ((g2 + 1e-15) * c.tau));
+    }
+    __syncthreads();
+  }
+
+  // MatrixUpdate::acceptLastMove (FreeParticleNodes.cc:1026-1057): slices
+  // 1..S-2 take the updated inverses, or are updated in place when no
+  // level-0 nodal action made them; the last slice keeps its inverse
+  __device__ void upd_accept(int fa) {
+    const int n = c.fn[fa].ns, nt = c.S - 2;
+    if (nt <= 0) return;
+    double* bi = upd_inv(fa);
+    if (upd_block(fa)[0] != 0.0) {
+      const double* nw = upd_new(fa);
+      for (int q = tid; q < nt * n * n; q += NT) {
+        const int t = 1 + q / (n * n), rem = q % (n * n);
+        const int64_t e = ((int64_t)t * ins + rem / n) * ins + rem % n;
+        bi[e] = nw[e];
+      }
+      __syncthreads();
+    } else {
+      upd_rows(fa, bi, 1, nt);
+    }
+  }
+
   // Node record of section slice j for node action fa.  Section 1 slice j is
   // absolute slice a and section 2 slice j is a + M/2 (mod M), so slices
   // t and t + M/2 share one record, kept with the lower slice as r1.  Both
@@ -1316,11 +1544,16 @@ struct Eng {
   __device__ int fn_initialize() {
     int nerr = 0;
     constexpr int NG = NT / G;
+    double* const inv_shared = inv;
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
       double* dm = fn_dm(fa);
       double* eflag = fn_newdm(fa);   // per-slice error flags until the end
-      if (!(fcache && rec_on && fn_init_cached(fa, nerr))) {
+      // with useUpdates the inverses stay in the node action's own block for
+      // the section's moves, and every slice is evaluated
+      inv = f.updates ? upd_inv(fa) : inv_shared;
+      const bool recs = fcache && rec_on && !f.updates;
+      if (!(recs && fn_init_cached(fa, nerr))) {
         double scl = scale[fa];
         int t = 0, passes = 0;
         int errs = 0;
@@ -1368,9 +1601,10 @@ struct Eng {
         }
         __syncthreads();
         if (f.nodal) node_distances(fa, 0, c.S, false, wscale, 0.0, fn_dist(fa));
-        if (fcache && rec_on) fn_store_records(fa, 0, c.S, dm, wscale, 0.0, eflag, fn_dist(fa));
+        if (recs) fn_store_records(fa, 0, c.S, dm, wscale, 0.0, eflag, fn_dist(fa));
         __syncthreads();
       }
+      inv = inv_shared;
       // newDMValue(0)=dmValue(0); newDist(0)=dist(0)
       const int n = f.ns;
       double* nd = fn_newdist(fa);
@@ -1398,7 +1632,10 @@ struct Eng {
       const double dm0 = fn_dm(fa)[0];
       int crossed = 0;
       double* ndm = fn_newdm(fa);
-      if (top_known && level == c.L) {
+      if (f.updates) {
+        upd_change(fa, t0, nStride, nt);
+        for (int m = 0; m < nt; ++m) crossed |= ndm[t0 + m * nStride] * dm0 <= 1e-200;
+      } else if (top_known && level == c.L) {
         // the top level evaluates only the last slice, which no level moves:
         // its determinant is the one initialize committed at the current scale
         const double dl = fn_dm(fa)[c.S - 1];
@@ -1414,7 +1651,12 @@ struct Eng {
       __syncthreads();
       if (crossed) { total += 2e100; continue; }
       double dA = 0;
-      if (f.nodal && level == 0) {
+      if (f.nodal && level == 0 && f.updates) {
+        double* nd = fn_newdist(fa);
+        upd_new_inverse(fa);
+        upd_distances(fa, nd);
+        dA = nodal_action(f, fn_dist(fa), nd);
+      } else if (f.nodal && level == 0) {
         double* nd = fn_newdist(fa);
         const int n = f.ns;
         const double* od = fn_dist(fa);
@@ -1451,13 +1693,21 @@ struct Eng {
       bool mine = false;
       for (int m = 0; m < nmov(); ++m) mine |= mv(m) >= f.ifirst && mv(m) < f.ifirst + f.ns;
       if (tid == 0) notmine[fa] = mine ? 0 : 1;
-      if (mine && top_known && tid == 0) fn_newdm(fa)[c.S - 1] = lasterr[fa] ? 0.0 : fn_dm(fa)[c.S - 1];
+      if (mine && top_known && !f.updates && tid == 0) fn_newdm(fa)[c.S - 1] = lasterr[fa] ? 0.0 : fn_dm(fa)[c.S - 1];
     }
     __syncthreads();
   }
 
   // evaluate node action fa's moved slices up to those the given level tests
   __device__ void fn_eval_level(int fa, int level) {
+    if (c.fn[fa].updates) {
+      // MatrixUpdate::evaluateChange on the level's slices, the last slice
+      // at the top level included (FixedNodeAction.cc:59-63)
+      const int nStride = 2 << level;
+      const int t0 = nStride / 2;
+      upd_change(fa, t0, nStride, (c.S - 1 - t0) / nStride + 1);
+      return;
+    }
     const int done = (moved_done >> (8 * fa)) & 0xFF;
     const int need = moved_needed(level);
     if (done >= need) return;
@@ -1533,7 +1783,11 @@ struct Eng {
         double* nd = fn_newdist(fa);
         const int n = f.ns;
         const double* od = fn_dist(fa);
-        if (top_known) {
+        if (f.updates) {
+          // evaluateNewInverse + evaluateNewDistance (FixedNodeAction.cc:79-91)
+          upd_new_inverse(fa);
+          upd_distances(fa, nd);
+        } else if (top_known) {
           for (int k = tid; k < 2 * n; k += NT) nd[(int64_t)(c.S - 1) * 2 * n + k] = od[(int64_t)(c.S - 1) * 2 * n + k];
           node_distances(fa, 1, c.S - 2, true, nullptr, scale[fa], nd);
         } else {
@@ -1561,7 +1815,8 @@ struct Eng {
       const double* nd = fn_newdist(fa);
       for (int k = tid; k < c.S; k += NT) dm[k] = ndm[k];
       for (int k = tid; k < 2 * c.S * n; k += NT) dd[k] = nd[k];
-      if (fcache && rec_on) fn_store_records(fa, 1, c.S - 1, ndm, nullptr, scale[fa], nullptr, nd);
+      if (c.fn[fa].updates) upd_accept(fa);
+      else if (fcache && rec_on) fn_store_records(fa, 1, c.S - 1, ndm, nullptr, scale[fa], nullptr, nd);
     }
     __syncthreads();
   }

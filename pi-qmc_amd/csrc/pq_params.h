@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/piqmc_hip.h"
+
 #define PQ_WAVE 64
 #define PQ_MAX_PAIRS 10
 #define PQ_PG_LEVELS 9      // levels 0..8: a section of up to 2^7 links and the mover's level L+1
@@ -32,6 +34,7 @@ struct FnP {             // one FixedNodeAction with its node model
   int pg_nmax[3];        // FreeParticleNodes periodic Gaussians, alpha = m T
   double pg_alpha[3], pg_len[3];
   int pg_img[3];         // evaluate as the sum of the three nearest images
+  int updates;           // FreeParticleNodes::MatrixUpdate (useUpdates)
 };
 
 struct EwP {             // the optimized Ewald sum of a CoulombAction or estimator
@@ -88,6 +91,7 @@ struct ChainP {
   int64_t inv_off;       // element offset of this chain's [S][ns_max][ns_max] inverse cache
   int64_t fn_off;        // element offset of this chain's fixed-node state (global copy)
   int64_t egr_off;       // element offset of this chain's [S][ns_max][ns_max][D] element-gradient cache
+  int64_t upd_off;       // element offset of this chain's MatrixUpdate blocks (one per node action), -1 none
   int64_t fnc_off;       // element offset of this chain's slice-pair node records, -1: none
   int64_t perm_off;      // element offset of this chain's path permutation and its inverse ([2][N])
   int ns_max;            // largest fermion species with a node model
@@ -114,6 +118,20 @@ static inline int64_t fn_cache_doubles(const ChainP& c) {
   int64_t n = 0;
   for (int a = 0; a < c.nfn; ++a) n += (int64_t)(c.M / 2) * fn_record_doubles(c.fn[a].ns);
   return n;
+}
+
+// FreeParticleNodes::MatrixUpdate block of one node action: a header of 8
+// doubles (isNewMatrixUpdated first), the section slices' inverses and updated
+// inverses [Smax][ns_max][ns_max], the moving particles' new matrix columns
+// [Smax][ns_max][PQ_MAX_MOVING] and per-slice scratch [Smax][64].
+__host__ __device__ inline int64_t upd_block_doubles(const ChainP& c) {
+  const int64_t ins = c.ns_max > 0 ? c.ns_max : 1;
+  return 8 + (int64_t)c.Smax * (2 * ins * ins + PQ_MAX_MOVING * ins + 64);
+}
+static inline bool chain_has_updates(const ChainP& c) {
+  for (int a = 0; a < c.nfn; ++a)
+    if (c.fn[a].updates) return true;
+  return false;
 }
 
 // Sizes of the per-chain fixed-node state block (dm, newdm, dist, newdist per

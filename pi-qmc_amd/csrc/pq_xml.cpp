@@ -339,9 +339,12 @@ int actions(Ctx& x, const Node& root, pq_chain_desc& d) {
       f.nodal_action = k.b("noNodalAction") ? 0 : 1;
       f.many_body_distance = k.b("useManyBodyDistance") ? 1 : 0;
       if (k.b("useDistDerivative")) return xml_fail("line %d: useDistDerivative is not on the device path", k.line);
-      if (f.model == PQ_NODES_FREE && (k.b("useHungarian") || k.i("useIterations") > 0 || k.b("useUpdates")))
-        return xml_fail("line %d: FreeParticleNodes with useHungarian, useIterations or useUpdates is not on the "
-                        "device path", k.line);
+      if (f.model == PQ_NODES_FREE && (k.b("useHungarian") || k.i("useIterations") > 0))
+        return xml_fail("line %d: FreeParticleNodes with useHungarian or useIterations is not on the device path",
+                        k.line);
+      // FreeParticleNodes::MatrixUpdate (ActionParser.cc:808-811); SHONodes
+      // has no useUpdates attribute
+      f.use_updates = (f.model == PQ_NODES_FREE && k.b("useUpdates")) ? 1 : 0;
     } else {
       return xml_fail("line %d: <%s> is not on the device path", k.line, k.name.c_str());
     }

@@ -55,7 +55,7 @@ class CoulombDesc(C.Structure):
 class NodeDesc(C.Structure):
     _fields_ = [("species", C.c_int), ("model", C.c_int), ("omega", C.c_double),
                 ("temperature", C.c_double), ("nodal_action", C.c_int),
-                ("many_body_distance", C.c_int)]
+                ("many_body_distance", C.c_int), ("use_updates", C.c_int)]
 
 
 class SampleDesc(C.Structure):
@@ -142,7 +142,7 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
     """species: [(count, mass, charge[, is_static])]; sho: [(species, omega)];
     coulomb: dict(epsilon, norder, rmin, rmax, ngpts[, ewald, ewald_rcut,
     ewald_kcut]) or None;
-    nodes: [(species, model, omega)]; samples [(species, npart[, chooser[, nrepeat[,
+    nodes: [(species, model, omega[, use_updates])]; samples [(species, npart[, chooser[, nrepeat[,
     opts]]])] with opts {delayed_rejection, sigma_factor, new_sigma_factor},
     default one npart=1 SpeciesParticleChooser per species;
     nsection default M / 2^L (one sweep touches every slice once on average)."""
@@ -171,9 +171,11 @@ def make_desc(ndim, nslice, nlevel, tau, species, cell=1000.0, spring=True,
         d.coulomb.ewald_type = 1 if coulomb.get("ewald_type", "opt") == "trad" else 0
         d.coulomb.ewald_kappa = coulomb.get("ewald_kappa", 0.0)
     d.nnodes = len(nodes)
-    for k, (s, model, w) in enumerate(nodes):
+    for k, nd in enumerate(nodes):
+        s, model, w = nd[:3]
         d.nodes[k].species, d.nodes[k].model, d.nodes[k].omega = s, model, w
         d.nodes[k].temperature, d.nodes[k].nodal_action = 0.0, 1
+        d.nodes[k].use_updates = 1 if len(nd) > 3 and nd[3] else 0
     d.mover = mover
     if samples is None:
         samples = [(s, 1) for s in range(len(species))]

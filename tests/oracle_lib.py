@@ -60,6 +60,7 @@ def load():
         "oq_fn_initialize": (None, [vp, dp, dp]),
         "oq_fn_action_difference": (C.c_double, [vp, dp, ip, C.c_int, C.c_int]),
         "oq_fn_accept": (None, [vp]),
+        "oq_fn_state": (None, [vp, C.c_int, C.c_int, dp, dp]),
         "oq_mover": (C.c_double, [vp, dp, dp, ip, C.c_int, C.c_int]),
         "oq_run_sweeps": (None, [vp, C.c_int]),
         "oq_get_stats": (None, [vp, P(capi.ChainStats)]),
@@ -258,6 +259,18 @@ class Chain:
 
     def fn_accept(self):
         self.lib.oq_fn_accept(self.h)
+
+    def node_scale(self, inode):
+        return self.lib.oq_node_scale(self.h, inode)
+
+    def fn_state(self, inode, new=False):
+        """(determinants [S], distances [S][2][ns]) of node action inode:
+        dmValue/dist, or newDMValue/newDist with new=True"""
+        S = (1 << self.desc.nlevel) + 1
+        ns = self.desc.species[self.desc.nodes[inode].species].count
+        det, dist = np.zeros(S), np.zeros(S * 2 * ns)
+        self.lib.oq_fn_state(self.h, inode, 1 if new else 0, dptr(det), dptr(dist))
+        return det, dist.reshape(S, 2, ns)
 
     def node_evaluate(self, inode, r1, r2, islice, scale_magnitude=0):
         r1 = np.ascontiguousarray(r1, dtype=np.float64)

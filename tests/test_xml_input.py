@@ -219,10 +219,25 @@ def test_inputs_outside_the_device_path_are_refused(edit, word):
     assert word in _refused(text)
 
 
-@pytest.mark.parametrize("attr", ['useUpdates="true"', 'useHungarian="true"', 'useIterations="2"'])
+@pytest.mark.parametrize("attr", ['useHungarian="true"', 'useIterations="2"'])
 def test_free_particle_node_variants_are_refused(attr):
     text = FERMIONS.replace('noNodalAction="true"', f'noNodalAction="true" {attr}')
     assert "FreeParticleNodes" in _refused(text, 2)
+
+
+def test_use_updates_reads_into_the_node_descriptor():
+    """useUpdates="true" on FreeParticleNodes selects FreeParticleNodes::
+    MatrixUpdate (ActionParser.cc:808-811); absent it stays off"""
+    x = capi.read_xml(FERMIONS, 2, text=True)
+    assert [x.sweep.nodes[a].use_updates for a in range(x.sweep.nnodes)] == [0, 0]
+    text = FERMIONS.replace('noNodalAction="true"', 'noNodalAction="true" useUpdates="true"')
+    x = capi.read_xml(text, 2, text=True)
+    assert [x.sweep.nodes[a].model for a in range(2)] == [capi.PQ_NODES_SHO, capi.PQ_NODES_FREE]
+    assert [x.sweep.nodes[a].use_updates for a in range(2)] == [0, 1]
+    # SHONodes takes no useUpdates
+    text = text.replace('temperature="0.2"/>', 'temperature="0.2" useUpdates="true"/>')
+    x = capi.read_xml(text, 2, text=True)
+    assert [x.sweep.nodes[a].use_updates for a in range(2)] == [0, 1]
 
 
 def test_coulomb_variants_are_refused():
