@@ -786,6 +786,8 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
         const GroupP& G = b->hc[c].grp[g];
         b->multi |= G.sample_nm[k] > 1 || G.sample_dr[k] || G.sample_fac[k] != 1.0;
       }
+  // and FreeParticleNodes::MatrixUpdate
+  for (int c = 0; c < nchain; ++c) b->multi |= chain_has_updates(b->hc[c]);
   // node-matrix row width: the instantiated widths, the smallest that fits
   b->NC = ns_max <= 4 ? 4 : ns_max <= 8 ? 8 : ns_max <= 10 ? 10 : ns_max <= 12 ? 12 : ns_max <= 16 ? 16 : 32;
   for (int c = 0; c < nchain; ++c) {

@@ -299,6 +299,11 @@ constexpr int gj_group(int nc) { return nc <= 4 ? 4 : (nc <= 8 ? 8 : (nc <= 16 ?
 template <int D, int NC, class R, int NT, int MM = PQ_MAX_MOVING>
 struct Eng {
   static constexpr int G = gj_group(NC);
+  // FreeParticleNodes::MatrixUpdate is compiled into the general instances
+  // only (the host selects them for any chain with use_updates), so the
+  // one-particle instances keep their registers for the full-recompute path
+  static constexpr bool kUpd = MM > 1;
+  __device__ __forceinline__ bool upd(const FnP& f) const { return kUpd && f.updates; }
   // chooser results shared by the team through LDS
   struct ChooseBuf {
     int idx[PQ_MAX_MOVING];   // moving particles
@@ -1551,8 +1556,8 @@ struct Eng {
       double* eflag = fn_newdm(fa);   // per-slice error flags until the end
       // with useUpdates the inverses stay in the node action's own block for
       // the section's moves, and every slice is evaluated
-      inv = f.updates ? upd_inv(fa) : inv_shared;
-      const bool recs = fcache && rec_on && !f.updates;
+      inv = upd(f) ? upd_inv(fa) : inv_shared;
+      const bool recs = fcache && rec_on && !upd(f);
       if (!(recs && fn_init_cached(fa, nerr))) {
         double scl = scale[fa];
         int t = 0, passes = 0;
@@ -1632,7 +1637,7 @@ struct Eng {
       const double dm0 = fn_dm(fa)[0];
       int crossed = 0;
       double* ndm = fn_newdm(fa);
-      if (f.updates) {
+      if (upd(f)) {
         upd_change(fa, t0, nStride, nt);
         for (int m = 0; m < nt; ++m) crossed |= ndm[t0 + m * nStride] * dm0 <= 1e-200;
       } else if (top_known && level == c.L) {
@@ -1651,7 +1656,7 @@ struct Eng {
       __syncthreads();
       if (crossed) { total += 2e100; continue; }
       double dA = 0;
-      if (f.nodal && level == 0 && f.updates) {
+      if (f.nodal && level == 0 && upd(f)) {
         double* nd = fn_newdist(fa);
         upd_new_inverse(fa);
         upd_distances(fa, nd);
@@ -1693,14 +1698,14 @@ struct Eng {
       bool mine = false;
       for (int m = 0; m < nmov(); ++m) mine |= mv(m) >= f.ifirst && mv(m) < f.ifirst + f.ns;
       if (tid == 0) notmine[fa] = mine ? 0 : 1;
-      if (mine && top_known && !f.updates && tid == 0) fn_newdm(fa)[c.S - 1] = lasterr[fa] ? 0.0 : fn_dm(fa)[c.S - 1];
+      if (mine && top_known && !upd(f) && tid == 0) fn_newdm(fa)[c.S - 1] = lasterr[fa] ? 0.0 : fn_dm(fa)[c.S - 1];
     }
     __syncthreads();
   }
 
   // evaluate node action fa's moved slices up to those the given level tests
   __device__ void fn_eval_level(int fa, int level) {
-    if (c.fn[fa].updates) {
+    if (upd(c.fn[fa])) {
       // MatrixUpdate::evaluateChange on the level's slices, the last slice
       // at the top level included (FixedNodeAction.cc:59-63)
       const int nStride = 2 << level;
@@ -1783,7 +1788,7 @@ struct Eng {
         double* nd = fn_newdist(fa);
         const int n = f.ns;
         const double* od = fn_dist(fa);
-        if (f.updates) {
+        if (upd(f)) {
           // evaluateNewInverse + evaluateNewDistance (FixedNodeAction.cc:79-91)
           upd_new_inverse(fa);
           upd_distances(fa, nd);
@@ -1815,7 +1820,7 @@ struct Eng {
       const double* nd = fn_newdist(fa);
       for (int k = tid; k < c.S; k += NT) dm[k] = ndm[k];
       for (int k = tid; k < 2 * c.S * n; k += NT) dd[k] = nd[k];
-      if (c.fn[fa].updates) upd_accept(fa);
+      if (upd(c.fn[fa])) upd_accept(fa);
       else if (fcache && rec_on) fn_store_records(fa, 1, c.S - 1, ndm, nullptr, scale[fa], nullptr, nd);
     }
     __syncthreads();
