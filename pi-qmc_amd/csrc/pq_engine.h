@@ -13,7 +13,10 @@
 // inversion keeps one matrix row per lane in groups of G = gj_group(NC) lanes,
 // NT/G matrices per pass.
 // All routines are called by every thread of the team with team-uniform
-// arguments and return team-uniform results.
+// arguments and return team-uniform results.  They are all force-inlined
+// into the sampler kernel: left to itself the inliner outlines the largest
+// of them, and each call then saves the callee-saved registers it uses (SGPRs
+// through VGPR lanes, VGPRs through scratch) around every level of a move.
 #pragma once
 #include "pq_device.h"
 
@@ -397,7 +400,7 @@ struct Eng {
   //               (RandomPermutationChooser.cc:12-24) + SimpleParticleChooser
   // The draws are sequential and cheap: one thread makes them and the team
   // reads the result back from LDS.
-  __device__ bool choose(int smp, uint64_t& k, double& lnTP) {
+  __device__ __forceinline__ bool choose(int smp, uint64_t& k, double& lnTP) {
     const int kind = c.sample_ch[smp];
     const int sp = c.sample_sp[smp];
     const int nm = MM == 1 ? 1 : c.sample_nm[smp];
@@ -463,7 +466,7 @@ struct Eng {
   // free-particle propagator of the WalkingChooser between particle i's first
   // and particle j's last section bead, prod_d PG_{m/(2 tau 2^L)}(pbc(delta)_d)
   // (WalkingChooser.cc:89-103)
-  __device__ double walk_t(int sp, int i, int j) const {
+  __device__ __forceinline__ double walk_t(int sp, int i, int j) const {
     double delta[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) delta[d] = sb(sec1, 0, i)[d] - sb(sec1, c.S - 1, j)[d];
@@ -486,7 +489,7 @@ struct Eng {
   // a repeated particle ends the move; the cycle m -> m+1 (mod nsize) is
   // accepted with probability revTranProb/tranProb and lnTP = log prod_m
   // t(i_m,i_m)/t(i_m,i_m+1).
-  __device__ bool walk_choose(int sp, int nsize, uint64_t& k, double& lnTP) {
+  __device__ __forceinline__ bool walk_choose(int sp, int nsize, uint64_t& k, double& lnTP) {
     ChooseBuf& cb = *cbuf;
     const int npart = c.count[sp], first = c.ifirst[sp];
     double tran = 0, rev = 0;   // thread 0's running sums
@@ -585,7 +588,7 @@ struct Eng {
     return nmoved * (2 * ((ng + 1) / 2) + (c.mover == PQ_MOVER_FREE_PBC ? ng : 0));
   }
 
-  __device__ double mover(int level, uint64_t k0, int* ndraws, double* fwd = nullptr) {
+  __device__ __forceinline__ double mover(int level, uint64_t k0, int* ndraws, double* fwd = nullptr) {
     const int s = 1 << level;
     const int nmoved = (level < c.L) ? (1 << (c.L - 1 - level)) : 0;
     const bool pbcmv = c.mover == PQ_MOVER_FREE_PBC;
@@ -760,7 +763,7 @@ struct Eng {
   // proposal C (in mov) against C from B, both about the same midpoint, and
   // its forward probability (minus B's Gaussian exponents plus the log of
   // B's periodic-Gaussian weights) in fwd.
-  __device__ double delayed_mover(int level, const double* rej, double* fwd) {
+  __device__ __forceinline__ double delayed_mover(int level, const double* rej, double* fwd) {
     const int s = 1 << level;
     const int t = s;
     double lnT = 0, fw = 0;
@@ -817,7 +820,7 @@ struct Eng {
   // the moving particles at a level.  Lanes split the (link, moving particle)
   // and (moving particle, partner, link) items; a pair of two moving particles
   // counts once, from the higher index, with both new paths.
-  __device__ double action_diff(int level) {
+  __device__ __forceinline__ double action_diff(int level) {
     const int s = 1 << level;
     const int nl = (c.S - 1) >> level;
     Acc<R> acc;
@@ -948,7 +951,7 @@ struct Eng {
   // k-vector) items; per item rho_k of the section slice and the moving
   // particles' change d rho_k give w_k (|rho + d rho|^2 - |rho|^2) =
   // w_k (2 Re(rho* d rho) + |d rho|^2), in FP64 in both precision modes.
-  __device__ double ewald_diff() {
+  __device__ __forceinline__ double ewald_diff() {
     const double* kp = x.ewk + c.ew.off;
     const int nk = c.ew.nk, items = (c.S - 2) * nk;
     double e = 0;
@@ -1171,7 +1174,7 @@ struct Eng {
   // evaluateDistance (SHONodes.cc:130-170, FreeParticleNodes.cc:184-234) for
   // slices t0..t0+nt-1 into dist[t][2][ns]; slice t uses scale scl[t] when
   // scl is given, else scl0.
-  __device__ void node_distances(int fa, int t0, int nt, bool use_mov, const double* scl,
+  __device__ __forceinline__ void node_distances(int fa, int t0, int nt, bool use_mov, const double* scl,
                                  double scl0, double* dist) {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
@@ -1254,7 +1257,7 @@ struct Eng {
   // b(im, jm) = sum_k phi(k, jm) inv(index1(im), k) and its determinant by
   // LU with partial pivoting (dgetf2 order), the ratio det(new)/det(old);
   // newDMValue = ratio * dmValue (FixedNodeAction.cc:61-63).
-  __device__ void upd_change(int fa, int t0, int step, int nt) {
+  __device__ __forceinline__ void upd_change(int fa, int t0, int step, int nt) {
     const FnP& f = c.fn[fa];
     const int n = f.ns, nm = upd_count(f);
     const double scl = scale[fa];
@@ -1317,7 +1320,7 @@ struct Eng {
   // t0..t0+nt-1, one moving particle jm after the other: with
   // b_i = sum_k phi(k, jm) m(i, k), row jp (the particle) is scaled by 1/b_jp
   // and every other row takes m(i, k) -= m(jp, k) b_i.
-  __device__ void upd_rows(int fa, double* m, int t0, int nt) {
+  __device__ __forceinline__ void upd_rows(int fa, double* m, int t0, int nt) {
     const FnP& f = c.fn[fa];
     const int n = f.ns, nm = upd_count(f);
     const double* phi = upd_phi(fa);
@@ -1352,7 +1355,7 @@ struct Eng {
 
   // MatrixUpdate::evaluateNewInverse (FreeParticleNodes.cc:950-974) of slices
   // 1..S-1 from the section's inverses and the move's new columns
-  __device__ void upd_new_inverse(int fa) {
+  __device__ __forceinline__ void upd_new_inverse(int fa) {
     const int n = c.fn[fa].ns, nt = c.S - 1;
     const double* bi = upd_inv(fa);
     double* nw = upd_new(fa);
@@ -1373,7 +1376,7 @@ struct Eng {
   // first loop (d1) each gradient component takes its own dimension's value
   // only (grad[i] *= value[i]); the second loop (d2) multiplies the whole
   // vector by every value.
-  __device__ void upd_distances(int fa, double* dist) {
+  __device__ __forceinline__ void upd_distances(int fa, double* dist) {
     const FnP& f = c.fn[fa];
     const int n = f.ns, nt = c.S - 1;
     const double* mat = upd_new(fa);
@@ -1424,7 +1427,7 @@ struct Eng {
   // MatrixUpdate::acceptLastMove (FreeParticleNodes.cc:1026-1057): slices
   // 1..S-2 take the updated inverses, or are updated in place when no
   // level-0 nodal action made them; the last slice keeps its inverse
-  __device__ void upd_accept(int fa) {
+  __device__ __forceinline__ void upd_accept(int fa) {
     const int n = c.fn[fa].ns, nt = c.S - 2;
     if (nt <= 0) return;
     double* bi = upd_inv(fa);
@@ -1462,7 +1465,7 @@ struct Eng {
   // the rescale loop runs on the recorded determinants, det(s) = det(s_r)
   // (s/s_r)^n, in slice order as the evaluating loop does.  Returns false
   // (nothing changed) when a record is missing.
-  __device__ bool fn_init_cached(int fa, int& nerr) {
+  __device__ __forceinline__ bool fn_init_cached(int fa, int& nerr) {
     const FnP& f = c.fn[fa];
     const int n = f.ns, S = c.S;
     double* dm = fn_dm(fa);
@@ -1520,7 +1523,7 @@ struct Eng {
 
   // write the records of section slices j0..j1-1 from dm/scale/dist; a slice
   // whose evaluation erred (flag) gets no record
-  __device__ void fn_store_records(int fa, int j0, int j1, const double* det, const double* scl,
+  __device__ __forceinline__ void fn_store_records(int fa, int j0, int j1, const double* det, const double* scl,
                                    double scl0, const double* errflag, const double* dist) {
     const int n = c.fn[fa].ns, rl = fn_record_doubles(n);
     for (int q = tid; q < (j1 - j0) * rl; q += NT) {
@@ -1546,7 +1549,7 @@ struct Eng {
   // resumes from that slice, exactly as the sequential loop would.  In the
   // sampler the slice-pair records replace the evaluation when all are
   // present.  Returns the number of crossed or singular slices.
-  __device__ int fn_initialize() {
+  __device__ __forceinline__ int fn_initialize() {
     int nerr = 0;
     constexpr int NG = NT / G;
     double* const inv_shared = inv;
@@ -1623,7 +1626,7 @@ struct Eng {
 
   // CompositeDoubleAction::getActionDifference over FixedNodeAction::
   // getActionDifference (FixedNodeAction.cc:39-131) for the moving particles.
-  __device__ double fn_diff(int level) {
+  __device__ __forceinline__ double fn_diff(int level) {
     double total = 0;
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
@@ -1691,7 +1694,7 @@ struct Eng {
   // rejects the error.  The top slice comes from initialize when no cycle
   // changed it.  fn_eval_moved starts a move (or the delayed second
   // proposal) with nothing evaluated.
-  __device__ void fn_eval_moved() {
+  __device__ __forceinline__ void fn_eval_moved() {
     moved_done = 0;
     for (int fa = 0; fa < c.nfn; ++fa) {
       const FnP& f = c.fn[fa];
@@ -1704,7 +1707,7 @@ struct Eng {
   }
 
   // evaluate node action fa's moved slices up to those the given level tests
-  __device__ void fn_eval_level(int fa, int level) {
+  __device__ __forceinline__ void fn_eval_level(int fa, int level) {
     if (upd(c.fn[fa])) {
       // MatrixUpdate::evaluateChange on the level's slices, the last slice
       // at the top level included (FixedNodeAction.cc:59-63)
@@ -1736,7 +1739,7 @@ struct Eng {
   // useManyBodyDistance one term per link and section with the distance
   // 1/sqrt(sum_i 1/d_i^2), where the other species' particles enter with the
   // node models' 200 (SHONodes.cc:132, FreeParticleNodes.cc:187)
-  __device__ double nodal_action(const FnP& f, const double* od, const double* nd) {
+  __device__ __forceinline__ double nodal_action(const FnP& f, const double* od, const double* nd) {
     const int n = f.ns;
     double part = 0;
     if (f.manybody) {
@@ -1771,7 +1774,7 @@ struct Eng {
     return bsum(part);
   }
 
-  __device__ double fn_diff_moved(int level) {
+  __device__ __forceinline__ double fn_diff_moved(int level) {
     double total = 0;
     for (int fa = 0; fa < c.nfn; ++fa) {
       if (notmine[fa]) continue;
@@ -1809,7 +1812,7 @@ struct Eng {
   // sampler the moved slices' records take the accepted values.  A move that
   // permuted the last slice evaluated it without error, so the error flag
   // initialize gave that slice no longer applies.
-  __device__ void fn_accept(bool permuted = false) {
+  __device__ __forceinline__ void fn_accept(bool permuted = false) {
     for (int fa = 0; fa < c.nfn; ++fa) {
       if (notmine[fa]) continue;
       if (permuted && lasterr && tid == 0) lasterr[fa] = 0;

@@ -13,7 +13,7 @@ namespace pq {
 // SerialPaths::putBeads (SerialPaths.cc:144-185) of section slices k0..k1-1:
 // slices before 0 are skipped, slices from M on go to particle perm[i].
 template <int D, int NT>
-__device__ void put_slices(double* beads, const double* sec, const ChainP& c, int t0, int k0, int k1,
+__device__ __forceinline__ void put_slices(double* beads, const double* sec, const ChainP& c, int t0, int k0, int k1,
                            const int* perm) {
   const int slice = c.N * D;
   for (int q = k0 * slice + (int)threadIdx.x; q < k1 * slice; q += NT) {
@@ -36,7 +36,7 @@ __device__ void put_slices(double* beads, const double* sec, const ChainP& c, in
 // append, Permutation.cc:23-43), staged through buf (S slices at a time); then
 // perm <- perm o temp and its inverse.  tmp and tmp2 are N-int scratch.
 template <int D, int NT>
-__device__ void relabel_following(double* beads, double* buf, const ChainP& c, int t0, const int* sp,
+__device__ __forceinline__ void relabel_following(double* beads, double* buf, const ChainP& c, int t0, const int* sp,
                                   int* perm, int* invperm, int* tmp, int* tmp2) {
   const int N = c.N, M = c.M, S = c.S;
   const int slice = N * D;
