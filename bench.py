@@ -38,7 +38,7 @@ METRIC = "bead-updates/sec on 1 MI355X, mixed-batched vs FP64 at fixed energy er
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 TUNE_AFTER = 20          # sweeps from the replicated start before the autotuner times anything
-STEADY = {"cfg5": 120, "cfg3": 220}   # sweeps before the timed steps (tuning sweeps included)
+STEADY = {"batch3d": 120, "batch2d": 220}   # sweeps before the timed steps (tuning sweeps included)
 PAIR_CF = ([0, 0, 1], 0.0, 8.0, 256)  # species 0 against species 0 and 1, r in [0, 8), 256 bins
 
 
@@ -63,16 +63,16 @@ def measured_traffic(workload_name, precision):
 
 
 def workload(name, nchain):
-    if name == "cfg3":
-        return capi.cfg3_descs(nchain), "1024 uneven 2D dots (N 2-20, M 128-1024, SHO/free nodes, L=4)"
-    if name == "cfg5":
-        return capi.cfg5_descs(nchain), "4096 uneven 3D dots (N 2-64 in two species, M 256-2048, SHO nodes, L=4)"
-    if name == "cfg2":
-        return [capi.cfg2_desc()] * nchain, "3+3 e- 2D dot, SHONodes, M=400, L=4"
-    if name == "cfg1":
-        return [capi.cfg1_desc()] * nchain, "2 e- 3D dot, Coulomb, M=200, L=3"
-    if name == "cfg4":
-        return [capi.cfg4_desc()] * nchain, "32 e + 32 h, box L=10, FreeParticleNodes 32x32, M=1024, L=5"
+    if name == "batch2d":
+        return capi.batch2d_descs(nchain), "1024 uneven 2D dots (N 2-20, M 128-1024, SHO/free nodes, L=4)"
+    if name == "batch3d":
+        return capi.batch3d_descs(nchain), "4096 uneven 3D dots (N 2-64 in two species, M 256-2048, SHO nodes, L=4)"
+    if name == "dot2d":
+        return [capi.dot2d_3p3_desc()] * nchain, "3+3 e- 2D dot, SHONodes, M=400, L=4"
+    if name == "dot3d":
+        return [capi.dot3d_2e_desc()] * nchain, "2 e- 3D dot, Coulomb, M=200, L=3"
+    if name == "ehbox":
+        return [capi.ehbox_desc()] * nchain, "32 e + 32 h, box L=10, FreeParticleNodes 32x32, M=1024, L=5"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -222,7 +222,7 @@ def energy_study(nsweep, seed, steady):
     `steady` sweeps, then nsweep sweeps each followed by a Measure.  Reports
     the paired difference of the per-chain time-averaged energies, and the
     two batch-mean energies with blocked errors (blocks of 50 sweeps)."""
-    descs, _ = workload("cfg3", 1024)
+    descs, _ = workload("batch2d", 1024)
     runs = {}
     for name, prec in (("mixed", capi.PQ_PREC_MIXED), ("fp64", capi.PQ_PREC_FP64)):
         b = make_batch(descs, prec, seed)
@@ -302,7 +302,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cfg5")
+    ap.add_argument("--workload", default="batch3d")
     ap.add_argument("--nchain", type=int, default=None)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--steady", type=int, default=None,
@@ -323,7 +323,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described "
                                    "in DESIGN.md and not part of the benchmark", "n_gpus": args.gpus}))
         sys.exit(2)
-    nchain = args.nchain or {"cfg3": 1024, "cfg5": 4096}.get(args.workload, 1)
+    nchain = args.nchain or {"batch2d": 1024, "batch3d": 4096}.get(args.workload, 1)
     descs, wdesc = workload(args.workload, nchain)
     prec = capi.PQ_PREC_MIXED if args.precision == "mixed" else capi.PQ_PREC_FP64
     steady = args.steady if args.steady is not None else STEADY.get(args.workload, 40)
@@ -378,21 +378,21 @@ def main():
                                               f"headline batch, the {args.steps} timed steps")
         del b64
     if not args.no_extras:
-        if args.workload != "cfg3":
-            d3, w3 = workload("cfg3", 1024)
-            b3, r3 = run_gpu(d3, prec, args.steps, args.warmup, args.seed, STEADY["cfg3"], pair_cf=PAIR_CF)
+        if args.workload != "batch2d":
+            d3, w3 = workload("batch2d", 1024)
+            b3, r3 = run_gpu(d3, prec, args.steps, args.warmup, args.seed, STEADY["batch2d"], pair_cf=PAIR_CF)
             k3, roof3 = roofline(d3, r3, args.steps)
-            t3, t3src = measured_traffic("cfg3", args.precision)
+            t3, t3src = measured_traffic("batch2d", args.precision)
             roof3["traffic"], roof3["traffic_source"] = t3, t3src
-            out["second_workload"] = {"workload": f"cfg3: {w3}", "value": float(r3["bu"].sum()) / r3["wall"],
+            out["second_workload"] = {"workload": f"batch2d: {w3}", "value": float(r3["bu"].sum()) / r3["wall"],
                                       "ms_per_step": 1e3 * r3["wall"] / args.steps, "schedule": r3["schedule"],
                                       "roofline": roof3}
             del b3
-        out["energy_study"] = energy_study(args.energy_sweeps, args.seed, STEADY["cfg3"])
-        single = [capi.cfg2_desc()]
+        out["energy_study"] = energy_study(args.energy_sweeps, args.seed, STEADY["batch2d"])
+        single = [capi.dot2d_3p3_desc()]
         _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed, 20, autotune=False)
         r1 = float(one["bu"].sum()) / one["wall"]
-        out["fp64_single_chain"] = {"workload": "cfg2: 3+3 e- 2D dot, SHONodes, M=400", "value": r1}
+        out["fp64_single_chain"] = {"workload": "dot2d: 3+3 e- 2D dot, SHONodes, M=400", "value": r1}
         out["ratio_vs_fp64_single_chain"] = value / r1
     if not args.no_cpu:
         log("cpu baseline")

@@ -199,14 +199,14 @@ def add_group(d, nlevel, samples, nsection):
 
 # ----------------------------------------------------------- workloads
 
-def cfg1_desc():
+def dot3d_2e_desc():
     """2 e- (up, dn) in a 3D harmonic dot, omega=1, Coulomb 1/r, beta=10, M=200, L=3."""
     return make_desc(3, 200, 3, 0.05, [(1, 1.0, -1.0), (1, 1.0, -1.0)],
                      sho=[(0, 1.0), (1, 1.0)],
                      coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024))
 
 
-def cfg2_desc():
+def dot2d_3p3_desc():
     """3+3 e- in a 2D dot, omega=1, Coulomb eps=1, SHONodes, beta=8, M=400, L=4."""
     return make_desc(2, 400, 4, 0.02, [(3, 1.0, -1.0), (3, 1.0, -1.0)],
                      sho=[(0, 1.0), (1, 1.0)],
@@ -214,7 +214,7 @@ def cfg2_desc():
                      nodes=[(0, PQ_NODES_SHO, 1.0), (1, PQ_NODES_SHO, 1.0)])
 
 
-def cfg3_descs(nchain=1024, seed=0x5EED):
+def batch2d_descs(nchain=1024, seed=0x5EED):
     """Uneven batch of 2D dots: N~U{2..20}, M in {128..1024}, omega~U[0.5,2],
     SHONodes on even chains, FreeParticleNodes (box 20) on odd chains."""
     out = []
@@ -234,7 +234,7 @@ def cfg3_descs(nchain=1024, seed=0x5EED):
     return out
 
 
-def cfg4_desc():
+def ehbox_desc():
     """32 e (m=1) + 32 h (m=2), spin-polarised, cubic box L=10, min-image Coulomb,
     FreeParticleNodes 32x32 per species, beta=20, M=1024, L=5, FreePBC mover."""
     return make_desc(3, 1024, 5, 20.0 / 1024, [(32, 1.0, -1.0), (32, 2.0, 1.0)], cell=10.0,
@@ -244,7 +244,7 @@ def cfg4_desc():
                      mover=PQ_MOVER_FREE_PBC)
 
 
-def cfg5_descs(nchain=4096, seed=0xC0FFEE):
+def batch3d_descs(nchain=4096, seed=0xC0FFEE):
     """Large uneven batch of 3D dots: N~U{2..64} in two species, M=16*U{16..128}, SHONodes."""
     out = []
     for c in range(nchain):

@@ -34,8 +34,8 @@ def rel_close(a, b, rtol, scale=1.0):
     return np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), scale))
 
 
-def small_cfg2(nslice=64):
-    d = capi.cfg2_desc()
+def small_dot2d_3p3(nslice=64):
+    d = capi.dot2d_3p3_desc()
     d.nslice = nslice
     d.nsection = nslice >> d.nlevel
     return d

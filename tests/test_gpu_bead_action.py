@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _cases():
-    c1 = capi.cfg1_desc()
+    c1 = capi.dot3d_2e_desc()
     box = capi.make_desc(3, 32, 3, 0.05, [(3, 1.0, -1.0), (2, 1e9, 1.0, True)], cell=3.0,
                          coulomb=dict(epsilon=1.0, norder=3, rmin=0.0, rmax=0.0, ngpts=400),
                          mover=capi.PQ_MOVER_FREE_PBC)
@@ -24,10 +24,10 @@ def _cases():
                         mover=capi.PQ_MOVER_FREE_PBC)
     dot2 = capi.make_desc(2, 40, 3, 0.02, [(3, 1.0, -1.0), (3, 1.0, -1.0)], sho=[(0, 1.0), (1, 1.3)],
                           coulomb=dict(epsilon=2.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024))
-    return {"cfg1": c1, "box": box, "ewald": ew, "dot2": dot2}
+    return {"dot3d": c1, "box": box, "ewald": ew, "dot2": dot2}
 
 
-@pytest.mark.parametrize("name", ["cfg1", "box", "ewald", "dot2"])
+@pytest.mark.parametrize("name", ["dot3d", "box", "ewald", "dot2"])
 def test_bead_action_on_caller_paths(name):
     print(f"[test] bead action {name}", flush=True)
     d = _cases()[name]

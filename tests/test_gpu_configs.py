@@ -99,7 +99,7 @@ def test_electron_hole_box(inject):
     """32 e (m 1) + 32 h (m 2), box L = 10, FreeMoverPBC, FreeParticleNodes
     32 x 32 on the 32-wide MFMA instance: two sweeps."""
     print(f"[test] electron-hole box inject={inject}", flush=True)
-    d = capi.cfg4_desc()
+    d = capi.ehbox_desc()
     b, ocs = _run_pair([d], 2, inject, team=256)
     _assert_same(b, ocs)
     e_gpu, e_cpu = float(b.energy()[0]), ocs[0].energy()
@@ -114,7 +114,7 @@ def test_large_3d_batch_chains(split):
     256-bin g(r) to r = 8): two sweeps, in one launch or split at the node
     width (32-wide chains on their own launch)."""
     print(f"[test] 3D batch chains split={split}", flush=True)
-    all_descs = capi.cfg5_descs(4096)
+    all_descs = capi.batch3d_descs(4096)
     descs = resolved_nodes([all_descs[c] for c in CFG5_PICK])
     assert max(d.nslice for d in descs) == 2048
     assert max(d.species[0].count for d in descs) == 32
@@ -134,7 +134,7 @@ def test_large_3d_batch_chains(split):
 
 def test_large_3d_batch_chains_injected():
     print("[test] 3D batch chains, injected glibc streams", flush=True)
-    all_descs = capi.cfg5_descs(4096)
+    all_descs = capi.batch3d_descs(4096)
     descs = resolved_nodes([all_descs[c] for c in CFG5_PICK[:4]])
     b, ocs = _run_pair(descs, 1, True)
     _assert_same(b, ocs)
@@ -144,7 +144,7 @@ def test_large_3d_batch_chains_injected():
 def test_three_plus_three_dot_full_size(inject):
     """The 3+3-electron SHONodes dot at M = 400, four sweeps."""
     print(f"[test] 3+3 dot M=400 inject={inject}", flush=True)
-    d = capi.cfg2_desc()
+    d = capi.dot2d_3p3_desc()
     assert d.nslice == 400
     b, ocs = _run_pair([d], 4, inject)
     _assert_same(b, ocs)
@@ -154,7 +154,7 @@ def test_heaviest_2d_batch_chains():
     """Eight of the 1024-chain 2D batch's heaviest chains (M = 1024, N up to
     20, SHO and free-particle nodes), with the heavy chains split off."""
     print("[test] heaviest 2D batch chains", flush=True)
-    all_descs = capi.cfg3_descs(1024)
+    all_descs = capi.batch2d_descs(1024)
     heavy = sorted(range(1024), key=lambda c: -all_descs[c].nslice * all_descs[c].npart ** 2)
     descs = resolved_nodes([all_descs[c] for c in heavy[:8]])
     assert all(d.nslice == 1024 for d in descs)

@@ -116,7 +116,7 @@ def test_matrix_update_electron_hole_box():
     """the 32 + 32 electron-hole box with useUpdates: two sweeps on the
     32-wide instance (initialize by MFMA elimination, moves by updates)"""
     print("[test] matrix update electron-hole box", flush=True)
-    d = capi.cfg4_desc()
+    d = capi.ehbox_desc()
     for a in range(d.nnodes):
         d.nodes[a].use_updates = 1
     b, ocs = _run_pair([d], 2, False, team=256)

@@ -86,7 +86,7 @@ def test_measured_graph_with_pair_cf_matches_eager():
     the same steps launched one by one on a twin batch: identical energies and
     counts."""
     print("[test] measured graph with pair cf", flush=True)
-    descs = [d for d in capi.cfg3_descs(16) if d.nslice <= 256][:4]
+    descs = [d for d in capi.batch2d_descs(16) if d.nslice <= 256][:4]
     twins = []
     for _ in range(2):
         b = capi.Batch(descs)

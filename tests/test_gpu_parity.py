@@ -11,7 +11,7 @@ import pytest
 
 import oracle_lib as ol
 import piqmc_capi as capi
-from gpu_helpers import glibc_uniforms, random_section, rel_close, small_cfg2
+from gpu_helpers import glibc_uniforms, random_section, rel_close, small_dot2d_3p3
 
 pytestmark = pytest.mark.gpu
 
@@ -19,25 +19,25 @@ RTOL = 1e-12
 
 
 def _descs():
-    c1 = capi.cfg1_desc()
-    c2 = small_cfg2()
+    c1 = capi.dot3d_2e_desc()
+    c2 = small_dot2d_3p3()
     pbc = capi.make_desc(3, 64, 3, 0.05, [(2, 1.0, -1.0), (2, 2.0, 1.0)], cell=4.0,
                          coulomb=dict(epsilon=1.0, norder=3, rmin=0.0, rmax=0.0, ngpts=512),
                          mover=capi.PQ_MOVER_FREE_PBC)
     # the reference's other choosers of one moving particle
-    walk1 = capi.cfg1_desc()
+    walk1 = capi.dot3d_2e_desc()
     for k in range(walk1.nsample):
         walk1.sample[k].chooser = capi.PQ_CHOOSER_WALKING
-    all1 = capi.cfg1_desc()
+    all1 = capi.dot3d_2e_desc()
     all1.sample[0].chooser = capi.PQ_CHOOSER_ALL
     all1.sample[1].chooser = capi.PQ_CHOOSER_ALL_RANDOM
-    walk2 = small_cfg2()
+    walk2 = small_dot2d_3p3()
     for k in range(walk2.nsample):
         walk2.sample[k].chooser = capi.PQ_CHOOSER_WALKING
-    return {"cfg1": c1, "cfg2": c2, "pbc": pbc, "cfg1_walk": walk1, "cfg1_all": all1, "cfg2_walk": walk2}
+    return {"dot3d": c1, "dot2d": c2, "pbc": pbc, "dot3d_walk": walk1, "dot3d_all": all1, "dot2d_walk": walk2}
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2", "pbc"])
+@pytest.mark.parametrize("name", ["dot3d", "dot2d", "pbc"])
 def test_action_difference(name):
     print(f"[test] action_difference {name}", flush=True)
     desc = _descs()[name]
@@ -160,7 +160,7 @@ def test_fixed_node_protocol(many_body):
     one chain; the nodal action per particle or, with useManyBodyDistance,
     per slice from the many-body distance."""
     print(f"[test] fixed_node_protocol many_body={many_body}", flush=True)
-    desc = small_cfg2(64)
+    desc = small_dot2d_3p3(64)
     for a in range(desc.nnodes):
         desc.nodes[a].many_body_distance = many_body
     b = capi.Batch([desc])
@@ -223,12 +223,12 @@ def _paired(desc, nsweep, seed=7, precision=capi.PQ_PREC_FP64, inject=False, tea
     return b, oc
 
 
-@pytest.mark.parametrize("name,inject,team,inv", [("cfg1", True, -1, -1), ("cfg1", False, 128, -1),
-                                                   ("cfg2", True, 64, 0), ("cfg2", True, 128, 1),
-                                                   ("cfg2", False, 64, 1), ("pbc", False, -1, -1),
-                                                   ("cfg1_walk", True, -1, -1), ("cfg1_all", True, -1, -1),
-                                                   ("cfg2_walk", False, -1, -1), ("cfg2", True, 256, 0),
-                                                   ("pbc", True, 256, -1), ("cfg2", False, 512, 1)])
+@pytest.mark.parametrize("name,inject,team,inv", [("dot3d", True, -1, -1), ("dot3d", False, 128, -1),
+                                                   ("dot2d", True, 64, 0), ("dot2d", True, 128, 1),
+                                                   ("dot2d", False, 64, 1), ("pbc", False, -1, -1),
+                                                   ("dot3d_walk", True, -1, -1), ("dot3d_all", True, -1, -1),
+                                                   ("dot2d_walk", False, -1, -1), ("dot2d", True, 256, 0),
+                                                   ("pbc", True, 256, -1), ("dot2d", False, 512, 1)])
 def test_sampler_decisions(name, inject, team, inv):
     """Same uniforms in, same accept/reject decisions and paths out, for every schedule."""
     print(f"[test] sampler_decisions {name} inject={inject} team={team} inv={inv}", flush=True)
@@ -244,7 +244,7 @@ def test_sampler_decisions(name, inject, team, inv):
     assert np.max(np.abs(pg - po)) < 1e-9
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2"])
+@pytest.mark.parametrize("name", ["dot3d", "dot2d"])
 @pytest.mark.parametrize("staging", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_staging_keeps_decisions(name, staging):
     """Pair tables and node state in LDS or global memory: same decisions."""
@@ -263,7 +263,7 @@ def test_autotune_keeps_decisions():
     """The autotuner's timing sweeps are ordinary sweeps: after it, the chain
     equals the oracle's after the same number of sweeps."""
     print("[test] autotune", flush=True)
-    desc = small_cfg2(64)
+    desc = small_dot2d_3p3(64)
     b = capi.Batch([desc])
     oc = ol.Chain(desc)
     beads = capi.initial_beads(desc, 8)
@@ -288,7 +288,7 @@ def test_autotune_keeps_decisions():
 
 def test_energy_estimator():
     print("[test] energy_estimator", flush=True)
-    descs = [capi.cfg1_desc(), capi.cfg1_desc()]
+    descs = [capi.dot3d_2e_desc(), capi.dot3d_2e_desc()]
     b = capi.Batch(descs)
     for c, d in enumerate(descs):
         beads = capi.initial_beads(d, 100 + c) * 0.7
@@ -302,7 +302,7 @@ def test_energy_estimator():
 
 def test_histogram_bit_exact():
     print("[test] histogram", flush=True)
-    desc = small_cfg2(128)
+    desc = small_dot2d_3p3(128)
     b = capi.Batch([desc, desc])
     lib = capi.load_library()
     import ctypes as C
@@ -325,7 +325,7 @@ def test_batched_uneven_decisions(split):
     """A slice of the uneven batch (SHO and free-particle nodes, N 2..20, M up
     to 256), whole or with its heaviest chains in a launch of their own."""
     print(f"[test] batched_uneven split={split}", flush=True)
-    descs = [d for d in capi.cfg3_descs(24) if d.nslice <= 256][:8]
+    descs = [d for d in capi.batch2d_descs(24) if d.nslice <= 256][:8]
     b = capi.Batch(descs)
     assert b.schedule()[0] == 128
     b.set_split(*split)
@@ -378,7 +378,7 @@ def test_node_records_follow_new_paths():
     across sections; replacing the paths must drop them.  Sweeps, new paths,
     more sweeps: decisions and paths stay the oracle's."""
     print("[test] node_records_follow_new_paths", flush=True)
-    desc = small_cfg2(64)
+    desc = small_dot2d_3p3(64)
     b = capi.Batch([desc])
     oc = ol.Chain(desc)
     for k, seed in enumerate((21, 22)):
@@ -404,7 +404,7 @@ def test_mixed_precision_energy():
     need a uniform within ~1e-6 of the acceptance probability, so nearly
     every chain keeps the FP64 decision stream over a few sweeps."""
     print("[test] mixed_precision_energy", flush=True)
-    descs = capi.cfg3_descs(64)
+    descs = capi.batch2d_descs(64)
     runs = []
     for prec in (capi.PQ_PREC_FP64, capi.PQ_PREC_MIXED):
         b = capi.Batch(descs, precision=prec)
@@ -423,7 +423,7 @@ def test_mixed_precision_energy():
     assert np.max(np.abs(e32[same] - e64[same])) <= 1e-5
 
 
-@pytest.mark.parametrize("name", ["cfg1", "cfg2", "pbc"])
+@pytest.mark.parametrize("name", ["dot3d", "dot2d", "pbc"])
 def test_action_difference_mixed(name):
     """FP32 link arithmetic with compensated sums against the FP64 oracle.
     Each of the ~10^2 summed terms carries a 2^-24 relative rounding, and
@@ -525,7 +525,7 @@ def test_batch_path_files_round_trip(tmp_path):
     """pq_write_all_paths / pq_read_all_paths: one file per clone (name + clone
     index when the batch has several), sampled paths back to 15 digits."""
     print("[test] batch_path_files", flush=True)
-    d = capi.cfg1_desc()
+    d = capi.dot3d_2e_desc()
     b = capi.Batch([d, d])
     for c in range(2):
         b.set_paths(c, capi.initial_beads(d, 60 + c))
@@ -549,7 +549,7 @@ def test_measured_graph_matches_eager():
     gives the energies (and paths) of the same sweeps run one launch at a
     time on a twin batch."""
     print("[test] measured_graph", flush=True)
-    descs = [d for d in capi.cfg3_descs(16) if d.nslice <= 256][:6]
+    descs = [d for d in capi.batch2d_descs(16) if d.nslice <= 256][:6]
     twins = []
     for _ in range(2):
         b = capi.Batch(descs, precision=capi.PQ_PREC_MIXED)
@@ -580,7 +580,7 @@ def test_coulomb_energy_estimator():
     """CoulombEnergyEstimator on the device against the oracle (1e-12), open
     and periodic cells, after sweeps."""
     print("[test] coulomb energy estimator", flush=True)
-    descs = [_descs()["cfg1"], _descs()["pbc"], small_cfg2()]
+    descs = [_descs()["dot3d"], _descs()["pbc"], small_dot2d_3p3()]
     for desc in descs:
         b, oc = _paired(desc, 3)
         for eps in (1.0, 12.5):
@@ -595,7 +595,7 @@ def test_virial_energy_estimator():
     print("[test] virial energy estimator", flush=True)
     bos = capi.make_desc(2, 64, 3, 2.0 / 64, [(4, 1.0, 0.0)], sho=[(0, 1.0)], cell=20.0,
                          samples=[(0, 2, capi.PQ_CHOOSER_WALKING), (0, 1, capi.PQ_CHOOSER_WALKING)])
-    for desc in (_descs()["cfg1"], _descs()["pbc"], small_cfg2(), bos):
+    for desc in (_descs()["dot3d"], _descs()["pbc"], small_dot2d_3p3(), bos):
         b, oc = _paired(desc, 4)
         for nw in (1, 3, 8):
             got = b.virial_energy(nw)[0]
@@ -608,7 +608,7 @@ def test_many_body_distance_sampler_decisions():
     """useManyBodyDistance in the device sampler: the oracle's decisions and
     paths for the 3+3 fermion dot."""
     print("[test] many-body distance sampler", flush=True)
-    desc = small_cfg2(64)
+    desc = small_dot2d_3p3(64)
     for a in range(desc.nnodes):
         desc.nodes[a].many_body_distance = 1
     b, oc = _paired(desc, 6)
@@ -656,33 +656,33 @@ def _dr_descs():
     wide = {"delayed_rejection": 1, "sigma_factor": 1.5, "new_sigma_factor": 3.0}
     sho1 = capi.make_desc(3, 64, 3, 1 / 64, [(1, 1.0, -1.0)], cell=50.0, sho=[(0, 1.0)],
                           samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, wide)])
-    cfg1 = capi.cfg1_desc()
-    for k in range(cfg1.nsample):
-        cfg1.sample[k].delayed_rejection = 1
-        cfg1.sample[k].sigma_factor = 1.2
-    cfg2 = small_cfg2()
-    for k in range(cfg2.nsample):
-        cfg2.sample[k].delayed_rejection = 1
-        cfg2.sample[k].sigma_factor = 1.3
-        cfg2.sample[k].new_sigma_factor = 0.6
+    dot3d = capi.dot3d_2e_desc()
+    for k in range(dot3d.nsample):
+        dot3d.sample[k].delayed_rejection = 1
+        dot3d.sample[k].sigma_factor = 1.2
+    dot2d = small_dot2d_3p3()
+    for k in range(dot2d.nsample):
+        dot2d.sample[k].delayed_rejection = 1
+        dot2d.sample[k].sigma_factor = 1.3
+        dot2d.sample[k].new_sigma_factor = 0.6
     # a small box: the movers' periodic Gaussians at the delayed level
     box = capi.make_desc(3, 64, 3, 0.05, [(2, 1.0, -1.0), (2, 2.0, 1.0)], cell=2.0,
                          coulomb=dict(epsilon=1.0, norder=3, rmin=0.0, rmax=0.0, ngpts=512),
                          samples=[(0, 1, capi.PQ_CHOOSER_SPECIES, 1, wide), (1, 1, capi.PQ_CHOOSER_SPECIES, 1, wide)])
     # two moving particles per move (the multi-particle sampler instance)
-    multi = small_cfg2()
+    multi = small_dot2d_3p3()
     multi.nsample = 1
     multi.sample[0].nmoving = 2
     multi.sample[0].delayed_rejection = 1
     multi.sample[0].sigma_factor = 1.3
     # a scaled mover without delayed rejection
-    fac = capi.cfg1_desc()
+    fac = capi.dot3d_2e_desc()
     for k in range(fac.nsample):
         fac.sample[k].sigma_factor = 0.8
-    return {"sho1": sho1, "cfg1": cfg1, "cfg2": cfg2, "box": box, "multi": multi, "factor": fac}
+    return {"sho1": sho1, "dot3d": dot3d, "dot2d": dot2d, "box": box, "multi": multi, "factor": fac}
 
 
-@pytest.mark.parametrize("name,inject", [("sho1", True), ("sho1", False), ("cfg1", True), ("cfg2", False),
+@pytest.mark.parametrize("name,inject", [("sho1", True), ("sho1", False), ("dot3d", True), ("dot2d", False),
                                          ("box", False), ("multi", True), ("factor", False)])
 def test_delayed_rejection_decisions(name, inject):
     """Delayed rejection at level L-1 and the movers' sigma factors
@@ -714,11 +714,11 @@ def test_delayed_rejection_inputs_are_checked():
     d.sample[0].delayed_rejection = 1
     with pytest.raises(capi.PQError):
         capi.Batch([d])
-    d = capi.cfg1_desc()
+    d = capi.dot3d_2e_desc()
     d.sample[0].sigma_factor = -1.0
     with pytest.raises(capi.PQError):
         capi.Batch([d])
-    d = capi.cfg1_desc()
+    d = capi.dot3d_2e_desc()
     d.sample[1].new_sigma_factor = float("nan")
     with pytest.raises(capi.PQError):
         capi.Batch([d])

@@ -10,7 +10,7 @@ import pytest
 
 import oracle_lib as ol
 import piqmc_capi as capi
-from gpu_helpers import glibc_uniforms, rel_close, small_cfg2
+from gpu_helpers import glibc_uniforms, rel_close, small_dot2d_3p3
 
 pytestmark = pytest.mark.gpu
 
@@ -31,7 +31,7 @@ def _perm_descs():
                            coulomb=coul, samples=[(0, 2, ALLR), (0, 3, ALL), (1, 2, SP)])
     # fermions: cycles of 2 cross the nodes at the top level, cycles of 3 may not;
     # DoubleSectionChooser's putDoubleBeads relabels section 2
-    ferm = small_cfg2(64)
+    ferm = small_dot2d_3p3(64)
     ferm.nsample = 2
     ferm.sample[0].species, ferm.sample[0].nmoving, ferm.sample[0].chooser = 0, 3, W
     ferm.sample[1].species, ferm.sample[1].nmoving, ferm.sample[1].chooser = 1, 2, W
@@ -98,7 +98,7 @@ def test_action_difference_two_moving():
     """pq_action_difference with nmov = 2: the moving pair counted once with
     both new paths (PairAction.cc:241-247), same-species and cross-species."""
     print("[test] action_difference two moving", flush=True)
-    desc = capi.cfg1_desc()
+    desc = capi.dot3d_2e_desc()
     desc2 = capi.make_desc(3, 64, 3, 0.05, [(2, 1.0, -1.0), (2, 2.0, 1.0)], sho=[(0, 1.0), (1, 0.7)],
                            coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024))
     for d in (desc, desc2):
@@ -135,7 +135,7 @@ def test_double_action_difference_two_moving():
     included: the top level sees the exchanged last slice (a sign change, so
     2e100), and the rest agrees with the oracle to 1e-11."""
     print("[test] double action two moving", flush=True)
-    desc = small_cfg2(64)
+    desc = small_dot2d_3p3(64)
     b = capi.Batch([desc])
     oc = ol.Chain(desc)
     rng = np.random.default_rng(21)
@@ -172,7 +172,7 @@ def test_double_action_difference_two_moving():
 
 
 def test_energy_with_a_set_permutation():
-    desc = capi.cfg1_desc()
+    desc = capi.dot3d_2e_desc()
     desc3 = capi.make_desc(2, 64, 3, 0.05, [(3, 1.0, -1.0), (2, 1.0, -1.0)], sho=[(0, 1.0), (1, 1.0)],
                            coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024))
     rng = np.random.default_rng(2)

@@ -135,11 +135,11 @@ def test_reference_hydrogen_atom_system_test():
 
 
 def _group_descs():
-    from gpu_helpers import small_cfg2
+    from gpu_helpers import small_dot2d_3p3
     W = capi.PQ_CHOOSER_WALKING
     # fermions with SHO nodes: sections of 2^4 links over both species, then
     # 2 x 2^3 over one species and 3 x 2^2 over the other
-    ferm = small_cfg2(64)
+    ferm = small_dot2d_3p3(64)
     capi.add_group(ferm, 3, [(0, 1)], 2)
     capi.add_group(ferm, 2, [(1, 1)], 3)
     # bosons in a trap: one-particle sections, then WalkingChooser cycles of
@@ -150,7 +150,7 @@ def _group_descs():
     capi.add_group(bos, 2, [(0, 3, W), (0, 1, W)], 3)
     # <Sample nrepeat>: three tries per section for the fermions' first
     # Sample, two for a WalkingChooser cycle of the bosons
-    rep_f = small_cfg2(64)
+    rep_f = small_dot2d_3p3(64)
     rep_f.sample[0].nrepeat = 3
     rep_b = capi.make_desc(2, 64, 3, 2.0 / 64, [(4, 1.0, 0.0)], sho=[(0, 1.0)], cell=20.0,
                            samples=[(0, 2, W, 2), (0, 1, W, 3)], nsection=4)

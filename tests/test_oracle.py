@@ -242,7 +242,7 @@ def test_pair_action_on_grid_points():
 
 def test_action_difference_antisymmetric():
     """Swapping the old and new paths of the moving particle negates dS."""
-    d = capi.cfg1_desc()
+    d = capi.dot3d_2e_desc()
     ch = ol.Chain(d)
     rng = np.random.default_rng(4)
     S = 9
@@ -380,7 +380,7 @@ def test_two_fermion_sho_fixed_node_energy():
 
 
 def test_decisions_repeatable_and_counted():
-    d = capi.cfg1_desc()
+    d = capi.dot3d_2e_desc()
     beads = capi.initial_beads(d, 1)
     a, b = ol.Chain(d), ol.Chain(d)
     for ch in (a, b):

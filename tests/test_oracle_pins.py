@@ -263,7 +263,7 @@ def test_sho_node_matrix_rank_one_at_low_temperature():
     precision and its determinant is rounding noise -- in numpy as in the
     oracle (and the reference).  At temperature 0.5 the same positions give a
     resolved determinant that numpy and the oracle agree on."""
-    d = capi.cfg5_descs(4096)[114]                 # N_s 20 + 20, M 2048
+    d = capi.batch3d_descs(4096)[114]                 # N_s 20 + 20, M 2048
     assert d.nslice == 2048 and d.species[0].count == 20
     ns = 20
     rng = np.random.default_rng(1)

@@ -134,7 +134,7 @@ def test_bead_forces_are_link_action_gradients():
 def test_virial_with_unit_window_is_the_thermodynamic_estimator():
     """With nwindow = 1 every link's window closes on itself: the -F.r/2tau
     term and the window's closing F.r0/2tau cancel, leaving sum utau / M."""
-    for d in (capi.cfg1_desc(), capi.make_desc(2, 24, 2, 0.05, [(3, 1.0, -1.0)], sho=[(0, 1.0)], cell=3.0,
+    for d in (capi.dot3d_2e_desc(), capi.make_desc(2, 24, 2, 0.05, [(3, 1.0, -1.0)], sho=[(0, 1.0)], cell=3.0,
                                                 coulomb=dict(epsilon=2.0, norder=2, ngpts=300))):
         ch = ol.Chain(d)
         ch.set_beads(capi.initial_beads(d, 5, replicate=False))

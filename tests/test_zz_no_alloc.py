@@ -46,7 +46,7 @@ def test_static_entry_point_matches_the_allocating_one():
                          mover=capi.PQ_MOVER_FREE_PBC)
     dot = capi.make_desc(3, 64, 4, 0.05, [(3, 1.0, -1.0), (2, 1.0, -1.0)], sho=[(0, 1.0), (1, 0.7)],
                          coulomb=dict(epsilon=1.0, norder=3, rmin=0.005, rmax=50.0, ngpts=1024))
-    descs = [capi.cfg1_desc(), pbc, dot]
+    descs = [capi.dot3d_2e_desc(), pbc, dot]
     b = capi.Batch(descs)
     rng = np.random.default_rng(17)
     for nq in (64, 9):
