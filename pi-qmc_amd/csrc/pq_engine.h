@@ -338,7 +338,7 @@ struct Eng {
   int mi0 = 0;      // the moving particle when MM = 1 (team-uniform register copy)
   double* fst;      // fixed-node state block
   double* inv;      // [S][ins][ins] latest inverse per section slice
-  double* egr;      // [S][ins][ins][D] gradient factors of the latest matrix elements
+  double* egr;      // [S][ins][ins][D] gradient factors of the latest FreeParticleNodes elements
   double* scale;    // [nfn]
   int* notmine;     // [nfn]
   double* wdet;     // [NT/G] scratch: determinants of a pass
@@ -1032,8 +1032,10 @@ struct Eng {
     const FnP& f = c.fn[fa];
     const int n = f.ns;
     // fill: every thread takes matrix elements; they are staged in the inverse
-    // slot of their slice, the gradient factors go to their own cache
+    // slot of their slice, FreeParticleNodes' gradient factors go to their own
+    // cache (SHONodes' distances recompute theirs from the positions)
     const int per = n * n;
+    const bool sho = f.model == PQ_NODES_SHO;
     for (int q = tid; q < nt * per; q += NT) {
       const int m = q / per, rem = q - m * per;
       const int ii = rem / n, jj = rem - ii * n;
@@ -1043,8 +1045,10 @@ struct Eng {
       double gf[D];
       const int64_t e = ((int64_t)t * ins + ii) * ins + jj;
       inv[e] = elem(f, rj, ri, scl, gf);
+      if (!sho) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) egr[e * D + d] = gf[d];
+        for (int d = 0; d < D; ++d) egr[e * D + d] = gf[d];
+      }
     }
     __syncthreads();
     if constexpr (NC == 32) {
@@ -1130,16 +1134,20 @@ struct Eng {
   // in the order of FreeParticleNodes.cc:196-207.
   // Always FP64: at low temperature the SHO matrix is close to rank one and
   // its determinant cancels, so element rounding decides the sign test.
+  // SHONodes' unscaled element exp(-c[(r_i^2 + r_j^2) cosh - 2 r_i.r_j])
+  __device__ __forceinline__ double sho_e(const FnP& f, const double* rj, const double* ri) const {
+    double rj2 = 0, ri2 = 0, rirj = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double aj = rj[d], ai = ri[d];
+      rj2 += aj * aj; ri2 += ai * ai; rirj += ai * aj;
+    }
+    return exp(-f.c * ((ri2 + rj2) * f.coshwt - 2.0 * rirj));
+  }
   __device__ __forceinline__ double elem(const FnP& f, const double* rj, const double* ri, double scl,
                                          double* gf) const {
     if (f.model == PQ_NODES_SHO) {
-      double rj2 = 0, ri2 = 0, rirj = 0;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const double aj = rj[d], ai = ri[d];
-        rj2 += aj * aj; ri2 += ai * ai; rirj += ai * aj;
-      }
-      const double e = exp(-f.c * ((ri2 + rj2) * f.coshwt - 2.0 * rirj));
+      const double e = sho_e(f, rj, ri);
       gf[0] = e;
 #pragma unroll
       for (int d = 1; d < D; ++d) gf[d] = 0.0;
@@ -1191,11 +1199,10 @@ struct Eng {
       for (int bb = 0; bb < n; ++bb) {
         const int jp = pass == 0 ? aa : bb, ip = pass == 0 ? bb : aa;
         const double w = iv[jp * ins + ip];
-        const double* gf = egr + (((int64_t)t * ins + ip) * ins + jp) * D;
         if (f.model == PQ_NODES_SHO) {
           const double* rj = r1pos(t, f.ifirst + jp, use_mov);
           const double* ri = sb(sec2, t, f.ifirst + ip);
-          const double e = gf[0];
+          const double e = sho_e(f, rj, ri);
 #pragma unroll
           for (int d = 0; d < D; ++d) {
             double gg = pass == 0 ? 2 * f.c * (ri[d] - f.coshwt * rj[d]) * e
@@ -1203,6 +1210,7 @@ struct Eng {
             lg[d] += w * sc * gg;
           }
         } else {
+          const double* gf = egr + (((int64_t)t * ins + ip) * ins + jp) * D;
           // the reversed separation negates the odd gradient factor exactly
 #pragma unroll
           for (int d = 0; d < D; ++d) lg[d] += w * (pass == 0 ? gf[d] : -gf[d]) * sc;
