@@ -650,7 +650,7 @@ static SweepLayoutH layout_for(const pq_batch* b, const Sched& req, int lo = 0, 
   lay.off_wdet = (int)o; o += nwd * 8;
   lay.off_wscale = (int)o; o += align8(mS * 8);
   lay.off_scale = (int)o; o += PQ_MAX_NODES * 8;
-  lay.off_red = (int)o; o += 8 * 8;
+  lay.off_red = (int)o; o += 16 * 8;   // two halves of 8 waves
   lay.off_cnt = (int)o; o += 24 * 8;
   lay.off_werr = (int)o; o += nwd * 4;
   lay.off_notmine = (int)o; o += 2 * PQ_MAX_NODES * 4;   // notmine, then last-slice error flags

@@ -328,7 +328,8 @@ struct Eng {
   uint32_t moved_done = 0;   // per node action (8 bits each): moved slices evaluated this move
   int wave;
   int ins;          // row stride of the inverse cache (largest node species)
-  double* red;      // [NT/64] block-reduction scratch
+  double* red;      // [2][8] block-reduction scratch, halves used in turn
+  int redsel = 0;   // the half the next bsum writes
   double* sec1;     // [S][N][D]
   double* sec2;     // [S][N][D]
   double* mov;      // [S][nmov][D]
@@ -365,16 +366,20 @@ struct Eng {
       : x(x_), c(c_), chain(chain_), tid(threadIdx.x), lane(threadIdx.x & 63), wave(threadIdx.x >> 6),
         ins(c_.ns_max > 0 ? c_.ns_max : 1), red(nullptr) {}
 
-  // sum over the NT threads of the team; every thread gets the result
+  // sum over the NT threads of the team; every thread gets the result.
+  // Consecutive sums write alternate halves of red: a wave writes a half
+  // again only after the next sum's barrier, which every wave reaches after
+  // reading that half, so one barrier per sum suffices.
   __device__ __forceinline__ double bsum(double v) {
     v = wave_sum(v);
     if constexpr (NT == PQ_WAVE) return v;
-    if (lane == 0) red[wave] = v;
+    double* r = red + redsel;
+    redsel ^= 8;
+    if (lane == 0) r[wave] = v;
     __syncthreads();
     double s = 0;
 #pragma unroll
-    for (int w = 0; w < NT / PQ_WAVE; ++w) s += red[w];
-    __syncthreads();
+    for (int w = 0; w < NT / PQ_WAVE; ++w) s += r[w];
     return s;
   }
 
@@ -581,6 +586,186 @@ struct Eng {
   // Returns ln(T_old/T_new); with fwd, FreeMover's forward probability
   // (getForwardProb: the new beads' Gaussian exponents and the log of their
   // inverse periodic-Gaussian weights) goes there.
+  // one new bead of the mover: moving particle m (particle i of species sp)
+  // at slice t = s(2kk+1) of level l, from its Gaussian g and the level's
+  // draw offset kb; adds its ln(T_old/T_new) terms to lnT and, with fwd, its
+  // forward-probability terms to fw
+  __device__ __forceinline__ void place_bead(int level, int m, int i, int sp, int t, uint64_t kb, int gd,
+                                             const double* g, double* fwd, double& lnT, double& fw) {
+    const int s = 1 << level;
+    const bool pbcmv = c.mover == PQ_MOVER_FREE_PBC;
+    const double lam = c.lambda[sp];
+    const double sigma = (MM > 1 ? fac : 1.0) * sqrt(lam * c.tau * s);
+    const double inv2Sigma2 = 0.5 / (sigma * sigma);
+    const double* mp = mov + ((t - s) * nmov() + m) * D;
+    const double* mn = mov + ((t + s) * nmov() + m) * D;
+    const double* op = sb(sec1, t - s, i);
+    const double* on = sb(sec1, t + s, i);
+    const double* oc = sb(sec1, t, i);
+    double* mc = mov + (t * nmov() + m) * D;
+    if (!pbcmv) {
+      double mid[D], delta[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) mid[d] = mn[d] - mp[d];
+      pbc<D>(c, mid);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += mp[d]; }
+      pbc<D>(c, mid);
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = g[d] * sigma;
+      pbc<D>(c, delta);
+      double nb[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) nb[d] = mid[d] + delta[d];
+      pbc<D>(c, nb);
+#pragma unroll
+      for (int d = 0; d < D; ++d) mc[d] = nb[d];
+      double temp = 1.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (c.mpg_on[level][sp][d]) temp *= pg_eval(pg_make(c.mass[sp] / (c.tau * s), c.a[d]), delta[d]);
+        else {
+          const double tmp = delta[d] * delta[d] * inv2Sigma2;
+          if (fwd) fw += tmp;
+          lnT += tmp;
+        }
+      }
+      temp = 1.0 / temp;
+      if (fwd && temp != 1.0) fw += log(temp);
+#pragma unroll
+      for (int d = 0; d < D; ++d) mid[d] = on[d] - op[d];
+      pbc<D>(c, mid);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += op[d]; }
+      pbc<D>(c, mid);
+#pragma unroll
+      for (int d = 0; d < D; ++d) delta[d] = oc[d] - mid[d];
+      pbc<D>(c, delta);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (c.mpg_on[level][sp][d]) temp *= pg_eval(pg_make(c.mass[sp] / (c.tau * s), c.a[d]), delta[d]);
+        else lnT -= delta[d] * delta[d] * inv2Sigma2;
+      }
+      if (temp != 1.0) lnT += log(temp);   // log 1 = 0 exactly
+    } else {
+      double mid[D], mid2[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) mid[d] = mn[d] - mp[d];
+      pbc<D>(c, mid);
+#pragma unroll
+      for (int d = 0; d < D; ++d) mid[d] *= 0.5;
+#pragma unroll
+      for (int d = 0; d < D; ++d) mid2[d] = mid[d] + 0.5 * c.a[d];
+      pbc<D>(c, mid2);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double w1, w2;
+        if (c.mpg_on[level][sp][d]) {
+          PG p = pg_make(c.mass[sp] / (c.tau * s), c.a[d]);
+          w1 = pg_eval(p, mid[d]);
+          w2 = pg_eval(p, mid2[d]);
+        } else {
+          w1 = exp(-inv2Sigma2 * mid[d] * mid[d]);
+          w2 = exp(-inv2Sigma2 * mid2[d] * mid2[d]);
+        }
+        if (U(kb + gd + m * D + d) > w1 / (w1 + w2)) mid[d] = mid2[d];
+      }
+      double nb[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) { mid[d] += mp[d]; nb[d] = mid[d] + g[d] * sigma; }
+      pbc<D>(c, nb);
+#pragma unroll
+      for (int d = 0; d < D; ++d) mc[d] = nb[d];
+      double dpn[D], dnn[D], dnew[D], dpo[D], dno[D], dold[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        dpn[d] = nb[d] - mp[d];
+        dnn[d] = nb[d] - mn[d];
+        dnew[d] = mn[d] - mp[d];
+        dpo[d] = oc[d] - op[d];
+        dno[d] = oc[d] - on[d];
+        dold[d] = on[d] - op[d];
+      }
+      pbc<D>(c, dpn); pbc<D>(c, dnn); pbc<D>(c, dnew);
+      pbc<D>(c, dpo); pbc<D>(c, dno); pbc<D>(c, dold);
+      double temp = 1.;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (c.mpg_on[level + 1][sp][d]) {
+          PG p = pg_make(c.mass[sp] / (c.tau * (s << 1)), c.a[d]);
+          temp *= pg_eval(p, dpo[d]); temp *= pg_eval(p, dno[d]);
+          temp /= pg_eval(p, dpn[d]); temp /= pg_eval(p, dnn[d]);
+        } else {
+          lnT -= dpo[d] * dpo[d] * inv2Sigma2 * 0.5;
+          lnT -= dno[d] * dno[d] * inv2Sigma2 * 0.5;
+          lnT += dpn[d] * dpn[d] * inv2Sigma2 * 0.5;
+          lnT += dnn[d] * dnn[d] * inv2Sigma2 * 0.5;
+        }
+        if (level + 2 < PQ_PG_LEVELS && c.mpg_on[level + 2][sp][d]) {
+          PG p = pg_make(c.mass[sp] / (c.tau * (s << 2)), c.a[d]);
+          temp /= pg_eval(p, dold[d]); temp *= pg_eval(p, dnew[d]);
+        } else {
+          lnT += dold[d] * dold[d] * inv2Sigma2 * 0.25;
+          lnT -= dnew[d] * dnew[d] * inv2Sigma2 * 0.25;
+        }
+      }
+      if (temp != 1.0) lnT += log(temp);   // log 1 = 0 exactly
+    }
+  }
+
+  // Every level of a one-particle move, proposed by the calling wave: the
+  // Box-Muller Gaussians do not depend on the beads, so the lanes first draw
+  // those of every level's beads at once (each bead's Gaussian is parked in
+  // its own slot of mov), then the levels place their beads coarse to fine,
+  // a level reading only the coarser levels' beads.  Level l's draws start
+  // at k + (L - l) + pd (2^(L-1-l) - 1): the draws of the levels above and
+  // one accept draw per level, as the level-by-level mover consumes them.
+  // lnT[l] receives each level's ln(T_old/T_new) (0 at the top level).
+  __device__ __forceinline__ void mover_all(uint64_t k, double* lnT_out) {
+    const int L = c.L, nb = (1 << L) - 1;
+    const bool pbcmv = c.mover == PQ_MOVER_FREE_PBC;
+    const int gd = 2 * ((D + 1) / 2);
+    const int pd = gd + (pbcmv ? D : 0);
+    const int i = mv(0);
+    const int sp = species_of(c, i);
+    for (int q = lane; q < nb; q += PQ_WAVE) {
+      const int lg = 31 - __clz(q + 1);
+      const int l = L - 1 - lg, kk = q + 1 - (1 << lg);
+      const int t = (1 << l) * (2 * kk + 1);
+      const uint64_t kb = k + (L - l) + (uint64_t)pd * ((1u << lg) - 1) + (uint64_t)kk * pd;
+      double gg[4];
+#pragma unroll
+      for (int p = 0; p < (D + 1) / 2; ++p) {
+        double t1 = 1 - 0.9999999999 * U(kb + 2 * p);
+        double t2 = U(kb + 2 * p + 1);
+        double rr = sqrt(-2.0 * log(t1));
+        double sn, cs;
+        sincos(6.283185306 * t2, &sn, &cs);
+        gg[2 * p] = rr * cs;
+        gg[2 * p + 1] = rr * sn;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) mov[t * D + d] = gg[d];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane == 0) lnT_out[L] = 0.0;
+    for (int l = L - 1; l >= 0; --l) {
+      const int lg = L - 1 - l;
+      double lnT = 0, fw = 0;
+      for (int kk = lane; kk < (1 << lg); kk += PQ_WAVE) {
+        const int t = (1 << l) * (2 * kk + 1);
+        const uint64_t kb = k + (L - l) + (uint64_t)pd * ((1u << lg) - 1) + (uint64_t)kk * pd;
+        double g[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) g[d] = mov[t * D + d];
+        place_bead(l, 0, i, sp, t, kb, gd, g, nullptr, lnT, fw);
+      }
+      lnT = wave_sum(lnT);
+      if (lane == 0) lnT_out[l] = lnT;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+  }
+
   // uniforms the mover draws at a level (FreeMover / FreeMoverPBC)
   __device__ __forceinline__ int mover_draws(int level) const {
     const int nmoved = (level < c.L) ? (1 << (c.L - 1 - level)) : 0;
@@ -631,123 +816,7 @@ struct Eng {
           g[d] = (e & 1) ? rr * sn : rr * cs;
         }
       }
-      const double lam = c.lambda[sp];
-      const double sigma = (MM > 1 ? fac : 1.0) * sqrt(lam * c.tau * s);
-      const double inv2Sigma2 = 0.5 / (sigma * sigma);
-      const double* mp = mov + ((t - s) * nmov() + m) * D;
-      const double* mn = mov + ((t + s) * nmov() + m) * D;
-      const double* op = sb(sec1, t - s, i);
-      const double* on = sb(sec1, t + s, i);
-      const double* oc = sb(sec1, t, i);
-      double* mc = mov + (t * nmov() + m) * D;
-      if (!pbcmv) {
-        double mid[D], delta[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) mid[d] = mn[d] - mp[d];
-        pbc<D>(c, mid);
-#pragma unroll
-        for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += mp[d]; }
-        pbc<D>(c, mid);
-#pragma unroll
-        for (int d = 0; d < D; ++d) delta[d] = g[d] * sigma;
-        pbc<D>(c, delta);
-        double nb[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) nb[d] = mid[d] + delta[d];
-        pbc<D>(c, nb);
-#pragma unroll
-        for (int d = 0; d < D; ++d) mc[d] = nb[d];
-        double temp = 1.0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          if (c.mpg_on[level][sp][d]) temp *= pg_eval(pg_make(c.mass[sp] / (c.tau * s), c.a[d]), delta[d]);
-          else {
-            const double tmp = delta[d] * delta[d] * inv2Sigma2;
-            if (fwd) fw += tmp;
-            lnT += tmp;
-          }
-        }
-        temp = 1.0 / temp;
-        if (fwd) fw += log(temp);
-#pragma unroll
-        for (int d = 0; d < D; ++d) mid[d] = on[d] - op[d];
-        pbc<D>(c, mid);
-#pragma unroll
-        for (int d = 0; d < D; ++d) { mid[d] *= 0.5; mid[d] += op[d]; }
-        pbc<D>(c, mid);
-#pragma unroll
-        for (int d = 0; d < D; ++d) delta[d] = oc[d] - mid[d];
-        pbc<D>(c, delta);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          if (c.mpg_on[level][sp][d]) temp *= pg_eval(pg_make(c.mass[sp] / (c.tau * s), c.a[d]), delta[d]);
-          else lnT -= delta[d] * delta[d] * inv2Sigma2;
-        }
-        lnT += log(temp);
-      } else {
-        double mid[D], mid2[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) mid[d] = mn[d] - mp[d];
-        pbc<D>(c, mid);
-#pragma unroll
-        for (int d = 0; d < D; ++d) mid[d] *= 0.5;
-#pragma unroll
-        for (int d = 0; d < D; ++d) mid2[d] = mid[d] + 0.5 * c.a[d];
-        pbc<D>(c, mid2);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          double w1, w2;
-          if (c.mpg_on[level][sp][d]) {
-            PG p = pg_make(c.mass[sp] / (c.tau * s), c.a[d]);
-            w1 = pg_eval(p, mid[d]);
-            w2 = pg_eval(p, mid2[d]);
-          } else {
-            w1 = exp(-inv2Sigma2 * mid[d] * mid[d]);
-            w2 = exp(-inv2Sigma2 * mid2[d] * mid2[d]);
-          }
-          if (U(kb + gd + m * D + d) > w1 / (w1 + w2)) mid[d] = mid2[d];
-        }
-        double nb[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) { mid[d] += mp[d]; nb[d] = mid[d] + g[d] * sigma; }
-        pbc<D>(c, nb);
-#pragma unroll
-        for (int d = 0; d < D; ++d) mc[d] = nb[d];
-        double dpn[D], dnn[D], dnew[D], dpo[D], dno[D], dold[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          dpn[d] = nb[d] - mp[d];
-          dnn[d] = nb[d] - mn[d];
-          dnew[d] = mn[d] - mp[d];
-          dpo[d] = oc[d] - op[d];
-          dno[d] = oc[d] - on[d];
-          dold[d] = on[d] - op[d];
-        }
-        pbc<D>(c, dpn); pbc<D>(c, dnn); pbc<D>(c, dnew);
-        pbc<D>(c, dpo); pbc<D>(c, dno); pbc<D>(c, dold);
-        double temp = 1.;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          if (c.mpg_on[level + 1][sp][d]) {
-            PG p = pg_make(c.mass[sp] / (c.tau * (s << 1)), c.a[d]);
-            temp *= pg_eval(p, dpo[d]); temp *= pg_eval(p, dno[d]);
-            temp /= pg_eval(p, dpn[d]); temp /= pg_eval(p, dnn[d]);
-          } else {
-            lnT -= dpo[d] * dpo[d] * inv2Sigma2 * 0.5;
-            lnT -= dno[d] * dno[d] * inv2Sigma2 * 0.5;
-            lnT += dpn[d] * dpn[d] * inv2Sigma2 * 0.5;
-            lnT += dnn[d] * dnn[d] * inv2Sigma2 * 0.5;
-          }
-          if (level + 2 < PQ_PG_LEVELS && c.mpg_on[level + 2][sp][d]) {
-            PG p = pg_make(c.mass[sp] / (c.tau * (s << 2)), c.a[d]);
-            temp /= pg_eval(p, dold[d]); temp *= pg_eval(p, dnew[d]);
-          } else {
-            lnT += dold[d] * dold[d] * inv2Sigma2 * 0.25;
-            lnT -= dnew[d] * dnew[d] * inv2Sigma2 * 0.25;
-          }
-        }
-        lnT += log(temp);
-      }
+      place_bead(level, m, i, sp, t, kb, gd, g, fwd, lnT, fw);
     }
     const double r = bsum(lnT);
     if (fwd) {

@@ -253,14 +253,9 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
         bool accepted = true;
         if constexpr (MM == 1) {
           {
-            uint64_t kk = k;
-            for (int l = L; l >= 0; --l) {
-              int nd;
-              const double lnT = e.mover(l, kk, &nd);
-              kk += nd + 1;
-              if (tid == 0) e.cbuf->lnT[l] = lnT;
-              __syncthreads();
-            }
+            // the first wave proposes every level; the team synchronises once
+            if (e.wave == 0) e.mover_all(k, e.cbuf->lnT);
+            __syncthreads();
             if (c.nfn) e.fn_eval_moved();
           }
           for (int l = L; l >= 0; --l) {
