@@ -48,12 +48,24 @@ __device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
     v = o > v ? o : v;
   }
   if constexpr (G >= 32) {
-    const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)v, 0x401F);   // lane ^ 16
-    const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(v >> 32), 0x401F);
-    o = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    // v_permlane16_swap leaves each lane the values of itself and of lane ^ 16
+    // (in an order that depends on the row), without a trip through LDS
+    const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(v >> 32), (uint32_t)(v >> 32), false, false);
+    const uint64_t a = ((uint64_t)(uint32_t)hi[0] << 32) | (uint32_t)lo[0];
+    const uint64_t b = ((uint64_t)(uint32_t)hi[1] << 32) | (uint32_t)lo[1];
+    o = a > b ? a : b;
     v = o > v ? o : v;
   }
   return v;
+}
+
+// the value v of lane l (wave-uniform l), through scalar registers
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // Row-per-lane in-place Gauss-Jordan inversion with partial pivoting of one
@@ -140,8 +152,8 @@ struct GjMfmaScratch {
   double W[32 * 4];    // multipliers [row][4]
   double RP[4 * 32];   // pivot rows before the panel [4][column]
   double S[4 * 32];    // s [4][column]
-  double prow[8];      // pivot row of the current step and its reciprocal
-  int pv[4];           // physical pivot rows of the panel's steps
+  double spare[8];
+  int spare_i[4];
   int pos[32];         // position of each row on exit
   int perm[32];        // P(j): the row that was pivot at step j
 };
@@ -184,8 +196,11 @@ __device__ __forceinline__ void gj32_panel(pq_d4 (&acc)[2][2], int kb, GjMfmaScr
       for (int r = 0; r < 4; ++r) sc->P[(16 * rt + 4 * r + (lane >> 4)) * 4 + jj0] = acc[rt][CT][r];
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  // (2) the panel's four steps, row per lane
+  // (2) the panel's four steps, row per lane.  The pivot lane is the one
+  // lane whose key is the group maximum; its row values and reciprocal
+  // reach every lane through scalar registers (readlane).
   double p[4], w[4];
+  int pv[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) p[j] = act ? sc->P[row * 4 + j] : 0.0;
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -198,27 +213,25 @@ __device__ __forceinline__ void gj32_panel(pq_d4 (&acc)[2][2], int kb, GjMfmaScr
     const uint64_t kmax = group_max_u64<32>(key);
     const int ppos = 63 - (int)(kmax & 0x3F);
     const bool isp = act && key == kmax;
+    const int pl = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(isp));
+    pv[j] = pl;
     if (ppos != k) det = -det;
-    if (isp) {
+    double prow[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) sc->prow[q] = p[q];
-      sc->prow[4] = myinv;
-      sc->pv[j] = row;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const double piv = sc->prow[j];
+    for (int q = 0; q < 4; ++q) prow[q] = lane_value(p[q], pl);
+    const double pinv = lane_value(myinv, pl);
+    const double piv = prow[j];
     det *= piv;
     const bool zero = piv == 0.0;
     if (zero) err = 1;
-    const double ipiv = zero ? 0.0 : sc->prow[4];
+    const double ipiv = zero ? 0.0 : pinv;
     const double f = (act && !isp) ? p[j] * ipiv : 0.0;
     const double sp = (isp && !zero) ? ipiv : 1.0;
     w[j] = (zero || !act) ? 0.0 : (isp ? 1.0 - ipiv : f);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      if (q != j) p[q] = fma(-f, sc->prow[q], p[q]) * sp;
+      if (q != j) p[q] = fma(-f, prow[q], p[q]) * sp;
     if (!zero && act) p[j] = isp ? ipiv : -f;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (isp) mypos = k;
     else if (mypos == k) mypos = ppos;
   }
@@ -230,7 +243,7 @@ __device__ __forceinline__ void gj32_panel(pq_d4 (&acc)[2][2], int kb, GjMfmaScr
   // (4) the pivot rows as they were before the panel, [4][column]
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int pr = sc->pv[j];
+    const int pr = pv[j];
     if ((lane >> 4) == (pr & 3)) {
       const int k8 = (pr >> 4) * 4 + ((pr & 15) >> 2);
       sc->RP[j * 32 + (lane & 15)] = acc_row<0>(acc, k8);
@@ -242,7 +255,7 @@ __device__ __forceinline__ void gj32_panel(pq_d4 (&acc)[2][2], int kb, GjMfmaScr
   if (act) {
     const int col = lane;
     const bool inpanel = col >= c0 && col < c0 + 4;
-    const int p1 = sc->pv[1], p2 = sc->pv[2], p3 = sc->pv[3];
+    const int p1 = pv[1], p2 = pv[2], p3 = pv[3];
     const double s0 = sc->RP[col];
     const double s1 = sc->RP[32 + col] - sc->W[p1 * 4 + 0] * s0;
     const double s2 = sc->RP[64 + col] - sc->W[p2 * 4 + 0] * s0 - sc->W[p2 * 4 + 1] * s1;
@@ -1547,6 +1560,9 @@ struct Eng {
     const int n = f.ns, S = c.S;
     double* dm = fn_dm(fa);
     double* flag = fn_newdm(fa);
+    double* dd = fn_dist(fa);
+    // every record of the section in one pass: determinant, scale, flag and
+    // the distances (the latter are overwritten when a record is missing)
     for (int j = tid; j < S; j += NT) {
       bool lower;
       const double* r = slot_rec(fa, j, lower);
@@ -1554,39 +1570,7 @@ struct Eng {
       wscale[j] = r[1];
       flag[j] = r[2];
     }
-    __syncthreads();
-    bool ok = true;
-    for (int j = 0; j < S; ++j) ok = ok && flag[j] == 1.0;
-    if (!ok) return false;
-    if (tid == 0) {
-      double scl = scale[fa], dm0 = 0;
-      int errs = 0, passes = 0;
-      for (int j = 0; j < S;) {
-        const double lr = (double)n * log(scl / wscale[j]);
-        const double dt = dm[j] * exp(lr);
-        const double ad = fabs(dt);
-        if (ad < 1e-50 || ad > 1e50) {
-          ++passes;
-          const double ld = log(fabs(dm[j])) + lr;   // log|det| at the current scale
-          if (ld < -575.6463) scl *= 2;               // |det| < 1e-250
-          else scl *= exp(-ld / n);
-          if (passes < 200) continue;
-        }
-        dm[j] = dt;
-        wscale[j] = scl;
-        if (j == 0) dm0 = dt;
-        if (dt * dm0 <= 1e-200) errs++;
-        passes = 0;
-        ++j;
-      }
-      scale[fa] = scl;
-      werr[0] = errs;
-      lasterr[fa] = 0;
-    }
-    __syncthreads();
-    nerr += werr[0];
     if (f.nodal) {
-      double* dd = fn_dist(fa);
       for (int q = tid; q < S * 2 * n; q += NT) {
         const int j = q / (2 * n), pass = (q / n) & 1, k = q % n;
         bool lower;
@@ -1595,9 +1579,60 @@ struct Eng {
       }
     }
     __syncthreads();
+    bool ok = true;
+    for (int j = 0; j < S; ++j) ok = ok && flag[j] == 1.0;
+    if (!ok) return false;
+    // the rescale loop of initialize on the recorded determinants,
+    // det(s) = det(s_r) (s/s_r)^n.  The first wave evaluates every slice at
+    // the current scale at once; when none leaves [1e-50, 1e50] the loop
+    // makes no pass and those are its values.  Otherwise one thread runs it.
+    if (wave == 0) {
+      const double scl0 = scale[fa];
+      bool in = true;
+      double dt = 0;
+      if (S <= PQ_WAVE && lane < S) {
+        const double lr = (double)n * log(scl0 / wscale[lane]);
+        dt = dm[lane] * exp(lr);
+        const double ad = fabs(dt);
+        in = !(ad < 1e-50 || ad > 1e50);
+      }
+      const bool fast = S <= PQ_WAVE && __builtin_amdgcn_ballot_w64(!in) == 0;
+      if (fast) {
+        const double dm0 = lane_value(dt, 0);
+        const bool e = lane < S && dt * dm0 <= 1e-200;
+        const int errs = __builtin_popcountll(__builtin_amdgcn_ballot_w64(e));
+        if (lane < S) { dm[lane] = dt; wscale[lane] = scl0; }
+        if (lane == 0) { werr[0] = errs; lasterr[fa] = 0; }
+      } else if (lane == 0) {
+        double scl = scl0, dm0 = 0;
+        int errs = 0, passes = 0;
+        for (int j = 0; j < S;) {
+          const double lr = (double)n * log(scl / wscale[j]);
+          const double dt = dm[j] * exp(lr);
+          const double ad = fabs(dt);
+          if (ad < 1e-50 || ad > 1e50) {
+            ++passes;
+            const double ld = log(fabs(dm[j])) + lr;   // log|det| at the current scale
+            if (ld < -575.6463) scl *= 2;               // |det| < 1e-250
+            else scl *= exp(-ld / n);
+            if (passes < 200) continue;
+          }
+          dm[j] = dt;
+          wscale[j] = scl;
+          if (j == 0) dm0 = dt;
+          if (dt * dm0 <= 1e-200) errs++;
+          passes = 0;
+          ++j;
+        }
+        scale[fa] = scl;
+        werr[0] = errs;
+        lasterr[fa] = 0;
+      }
+    }
+    __syncthreads();
+    nerr += werr[0];
     return true;
   }
-
   // write the records of section slices j0..j1-1 from dm/scale/dist; a slice
   // whose evaluation erred (flag) gets no record
   __device__ __forceinline__ void fn_store_records(int fa, int j0, int j1, const double* det, const double* scl,
