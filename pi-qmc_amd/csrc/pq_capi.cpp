@@ -484,6 +484,7 @@ static int build_chain(pq_batch* b, const pq_chain_desc& d, ChainP& c,
         c.pair[c.npair].s1 = i;
         c.pair[c.npair].s2 = j;
         c.pair[c.npair].table = tid;
+        c.pair[c.npair].tp = b->ht[tid];
         c.npair++;
       }
   }

@@ -902,11 +902,17 @@ struct Eng {
   // the moving particles at a level.  Lanes split the (link, moving particle)
   // and (moving particle, partner, link) items; a pair of two moving particles
   // counts once, from the higher index, with both new paths.
+  // the partner species of a pair action for a moving particle of species
+  // sp, or -1 when the pair action does not involve sp
+  __device__ __forceinline__ static int pair_partner(const PairP& P, int sp) {
+    return sp == P.s1 ? P.s2 : (sp == P.s2 ? P.s1 : -1);
+  }
   __device__ __forceinline__ double action_diff(int level) {
     const int s = 1 << level;
     const int nl = (c.S - 1) >> level;
+    const int nA = nl * nmov();   // spring and SHO links, lanes 0..nA-1
     Acc<R> acc;
-    for (int it = tid; it < nl * nmov(); it += NT) {
+    for (int it = tid; it < nA; it += NT) {
       int k = it, m = 0;
       if (nmov() > 1) { k = it / nmov(); m = it - k * nmov(); }
       const int i = mv(m);
@@ -963,29 +969,41 @@ struct Eng {
         acc.add(-v[1]);
       }
     }
-    for (int pp = 0; pp < c.npair; ++pp) {
-      const PairP& P = c.pair[pp];
-      const TableP T = x.tables[P.table];
+    // pair actions: one index space over the (pair action, moving particle)
+    // blocks of partner x link items, spread over the whole team after the
+    // lanes of the links above
+    int npi = 0;
+    for (int pp = 0; pp < c.npair; ++pp)
+      for (int m = 0; m < nmov(); ++m) {
+        const int js = pair_partner(c.pair[pp], species_of(c, mv(m)));
+        if (js >= 0) npi += c.count[js] * nl;
+      }
+    for (int it = tid; it < nA + npi; it += NT) {
+      if (it < nA) continue;
+      // the block holding item it - nA
+      int r = it - nA, pp = 0, m = 0, js = -1;
+      for (int q = 0; q < c.npair * nmov(); ++q) {
+        pp = q / nmov();
+        m = q - pp * nmov();
+        js = pair_partner(c.pair[pp], species_of(c, mv(m)));
+        if (js < 0) continue;
+        const int cnt = c.count[js] * nl;
+        if (r < cnt) break;
+        r -= cnt;
+      }
+      const int i = mv(m);
+      const TableP& T = c.pair[pp].tp;
       const R* u = tab ? tab + tabofs[pp] : table_ptr<R>(x, T);
       const bool offdiag = level < 3 && T.K1 > 1;
-      for (int m = 0; m < nmov(); ++m) {
-      const int i = mv(m);
-      const int sp = species_of(c, i);
-      int js;
-      if (sp == P.s1) js = P.s2;
-      else if (sp == P.s2) js = P.s1;
-      else continue;
-      const int jb = c.ifirst[js], nj = c.count[js];
-      const int items = nj * nl;
-      for (int it = tid; it < items; it += NT) {
-        const int j = jb + it / nl;
+      {
+        const int j = c.ifirst[js] + r / nl;
         if (j == i) continue;
         int jm = -1;
         if constexpr (MM > 1)
           for (int q = 0; q < nmov(); ++q)
             if (midx[q] == j) jm = q;
         if (jm >= 0 && i <= j) continue;
-        const int t = s * (it % nl + 1);
+        const int t = s * (r % nl + 1);
         const double* pj1 = sb(sec1, t, j);
         const double* pj0 = sb(sec1, t - s, j);
         R vals[2];
@@ -1019,7 +1037,6 @@ struct Eng {
         }
         acc.add(vals[0]);
         acc.add(-vals[1]);
-      }
       }
     }
     const double links = bsum(acc.value());

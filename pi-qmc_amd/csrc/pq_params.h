@@ -23,6 +23,7 @@ struct TableP {          // one tabulated pair action u_k(r), utau_k(r)
 struct PairP {
   int s1, s2;
   int table;
+  TableP tp;             // a copy of tables[table], read from the chain's LDS block
 };
 
 struct FnP {             // one FixedNodeAction with its node model
