@@ -236,6 +236,7 @@ struct pq_batch {
   uint32_t first_stream = 0;
   // call-level scratch
   double* dscr = nullptr;
+  uint64_t* dqueue = nullptr;   // queue counter of the heavy chains' launch
   size_t scr_bytes = 0;
   double* dfnscr = nullptr;   // fixed-node protocol buffers, kept between calls
   size_t fnscr_bytes = 0;
@@ -710,7 +711,14 @@ static hipError_t run_launch(pq_batch* b, const DevCtx& x, int nsweep) {
   if (h <= 0 || h >= b->nchain) return launch_sweep(b->D, b->NC, mixed, b->multi, x, b->nchain, nsweep, b->lay, b->stream);
   hipError_t e = hipEventRecord(b->evf, b->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(b->stream2, b->evf, 0);
-  if (e == hipSuccess) e = launch_sweep(b->D, b->NC, mixed, b->multi, x, h, nsweep, b->lay_heavy, b->stream2);
+  // the heavy chains' launch takes its chains from a queue (zeroed by a fill
+  // kernel, which graph capture records as a kernel node): its workgroups
+  // keep their slots until every heavy chain is taken, and the other
+  // launch fills the slots they leave after that
+  SweepLayoutH lh = b->lay_heavy;
+  if (!b->multi && lh.team >= 256) lh.queue = reinterpret_cast<int*>(b->dqueue);
+  if (e == hipSuccess && lh.queue) e = launch_fill_u64(b->dqueue, 1, b->stream2);
+  if (e == hipSuccess) e = launch_sweep(b->D, b->NC, mixed, b->multi, x, h, nsweep, lh, b->stream2);
   DevCtx xl = x;
   xl.order = x.order + h;
   if (e == hipSuccess) e = launch_sweep(b->D, nc_rest, mixed, b->multi, xl, b->nchain - h, nsweep, b->lay, b->stream);
@@ -823,15 +831,16 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
   // heaviest chains first so the longest workgroups start earliest
   b->order.resize(nchain);
   for (int c = 0; c < nchain; ++c) b->order[c] = c;
+  // estimated sweep time: moves x levels x (a fixed cost + one per particle).
+  // Measured per-chain sweep times of the uneven batches follow this to a
+  // correlation of 0.97 (the node matrices' n^3 work hides under the
+  // latency of their n pivot steps and of the level chain).
   std::vector<double> cost(nchain);
   for (int c = 0; c < nchain; ++c) {
     const ChainP& ch = b->hc[c];
-    double ns2 = 0;
-    for (int a = 0; a < ch.nfn; ++a) ns2 += (double)ch.fn[a].ns * ch.fn[a].ns * ch.fn[a].ns;
     cost[c] = 0;
     for (int g = 0; g < ch.ngroup; ++g)
-      cost[c] += (double)ch.grp[g].nsection * ch.grp[g].nsample * (1 << ch.grp[g].L) *
-                 (ch.N + ns2 / std::max(1, ch.grp[g].nsample));
+      cost[c] += (double)ch.grp[g].nsection * ch.grp[g].nsample * (ch.grp[g].L + 1) * (40.0 + ch.N);
   }
   std::stable_sort(b->order.begin(), b->order.end(), [&](int p, int q) { return cost[p] > cost[q]; });
   // with 32-wide node matrices beside narrower ones, the 32-wide chains go
@@ -864,6 +873,7 @@ extern "C" int pq_batch_create(const pq_chain_desc* chains, int nchain, int prec
     return hipMalloc(p, bytes) == hipSuccess;
   };
   bool ok = dalloc((void**)&b->dchains, sizeof(ChainP) * nchain) &&
+            dalloc((void**)&b->dqueue, sizeof(uint64_t)) &&
             dalloc((void**)&b->dtables, sizeof(TableP) * std::max<size_t>(1, b->ht.size())) &&
             dalloc((void**)&b->du, sizeof(double) * b->hu.size()) &&
             dalloc((void**)&b->dut, sizeof(double) * b->hut.size()) &&
@@ -924,7 +934,7 @@ extern "C" void pq_batch_destroy(pq_batch* b) {
   void* ptrs[] = {b->dchains, b->dtables, b->du, b->dut, b->duf, b->dbeads, b->dinv, b->dfn, b->dfncache, b->dfnsweep,
                   b->dscale, b->dcounter, b->dstats, b->dinj, b->dorder, b->dscr, b->dfnscr, b->degr, b->dupd,
                   b->dperm, b->dewk, b->dew_est, b->dewk_est, b->dvoff_est, b->dvgrid_est,
-                  b->ditems_e, b->ditems_g, b->dfirst_e, b->dpart, b->dcf, b->dplan, b->dbead_stage};
+                  b->ditems_e, b->ditems_g, b->dfirst_e, b->dpart, b->dcf, b->dplan, b->dbead_stage, b->dqueue};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   drop_graph(b);

@@ -16,6 +16,8 @@ struct SweepLayoutH {
   int off_walk;                    // WalkingChooser rows, < 0: no walking Sample
   int bytes;
   int team;                        // threads per chain: 64 to 512
+  int* queue;                      // queue counter (zeroed before the launch), or null: one chain per workgroup
+  int nwork;                       // chains of the launch (set by the launcher)
 };
 
 struct CallBufsH {

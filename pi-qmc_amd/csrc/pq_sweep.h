@@ -69,10 +69,10 @@ __device__ __forceinline__ void relabel_following(double* beads, double* buf, co
 // two waves per SIMD whatever the team: up to 256 threads two workgroups share
 // a CU, a 512-thread team has one to itself.  MM bounds the moving particles
 // per Sample: MM = 1 never permutes and carries no relabelling code.
+// All sweeps of one chain by the calling workgroup.
 template <int D, int NC, class R, int NT, int MM>
-__global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, int nsweep, SweepLayoutH lay) {
+__device__ __forceinline__ void sweep_chain(const DevCtx& x, int chain, int nsweep, const SweepLayoutH& lay) {
   extern __shared__ double smem[];
-  const int chain = x.order[blockIdx.x];
   const uint64_t t_begin = wall_clock64();
   ChainP& c = *reinterpret_cast<ChainP*>(smem);
   copy_chain(&c, &x.chains[chain]);
@@ -411,13 +411,49 @@ __global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, 
   if (tid < c.nfn) x.scale[(int64_t)chain * PQ_MAX_NODES + tid] = e.scale[tid];
 }
 
-template <int D, int NC, class R, int NT, int MM>
+// One chain per workgroup; or (Q) a grid of at most the resident
+// workgroups in which a workgroup that finishes a chain takes the next one of
+// the launch's order (heaviest first) from the queue counter lay.queue
+// (zeroed before the launch) until the lay.nwork chains are taken, so the
+// launch keeps its slots until its queue is empty.  Q has its own instances:
+// the loop around the chain's sweeps costs registers.
+template <int D, int NC, class R, int NT, int MM, bool Q = false>
+__global__ __launch_bounds__(NT, NT <= 256 ? 2 : 1) void sweep_kernel(DevCtx x, int nsweep, SweepLayoutH lay) {
+  if constexpr (!Q) {
+    sweep_chain<D, NC, R, NT, MM>(x, x.order[blockIdx.x], nsweep, lay);
+    return;
+  }
+  __shared__ int next;
+  int wi = blockIdx.x;
+  for (;;) {
+    sweep_chain<D, NC, R, NT, MM>(x, x.order[wi], nsweep, lay);
+    __syncthreads();
+    if (threadIdx.x == 0) next = (int)gridDim.x + atomicAdd(lay.queue, 1);
+    __syncthreads();
+    wi = next;
+    if (wi >= lay.nwork) return;
+  }
+}
+
+template <int D, int NC, class R, int NT, int MM, bool Q = false>
 static hipError_t launch_sweep_nt(const DevCtx& x, int nchain, int nsweep, const SweepLayoutH& L,
                                   hipStream_t st) {
-  auto k = sweep_kernel<D, NC, R, NT, MM>;
+  auto k = sweep_kernel<D, NC, R, NT, MM, Q>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, L.bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(nchain), dim3(NT), L.bytes, st, x, nsweep, L);
+  int grid = nchain;
+  if (Q) {
+    // a queued launch: as many workgroups as can be resident at once
+    int per_cu = 0, dev = 0, ncu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, NT, L.bytes);
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if (per_cu > 0 && ncu > 0 && per_cu * ncu < nchain) grid = per_cu * ncu;
+  }
+  SweepLayoutH Lq = L;
+  Lq.nwork = nchain;
+  hipLaunchKernelGGL(k, dim3(grid), dim3(NT), L.bytes, st, x, nsweep, Lq);
   return hipGetLastError();
 }
 
@@ -432,6 +468,9 @@ static hipError_t launch_sweep_t(bool multi, const DevCtx& x, int nchain, int ns
     if (L.team == PQ_WAVE) return launch_sweep_nt<D, NC, R, PQ_WAVE, PQ_MAX_MOVING>(x, nchain, nsweep, L, st);
     return hipErrorInvalidValue;
   }
+  // queued launches: the heavy chains' teams of 256 or 512 threads
+  if (L.queue && L.team == 512) return launch_sweep_nt<D, NC, R, 512, 1, true>(x, nchain, nsweep, L, st);
+  if (L.queue && L.team == 256) return launch_sweep_nt<D, NC, R, 256, 1, true>(x, nchain, nsweep, L, st);
   if (L.team == 512) return launch_sweep_nt<D, NC, R, 512, 1>(x, nchain, nsweep, L, st);
   if (L.team == 256) return launch_sweep_nt<D, NC, R, 256, 1>(x, nchain, nsweep, L, st);
   if (L.team == 128) return launch_sweep_nt<D, NC, R, 128, 1>(x, nchain, nsweep, L, st);

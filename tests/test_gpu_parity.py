@@ -349,6 +349,47 @@ def test_batched_uneven_decisions(split):
         assert np.max(np.abs(b.get_paths(c) - oc.get_beads())) < 1e-9
 
 
+def test_queued_heavy_launch_takes_every_chain():
+    """The heavy chains' launch with 512-thread teams takes its chains from a
+    queue with at most the resident workgroups (one per CU): 300 heavy chains
+    are more than there are CUs, so workgroups run several chains in turn.
+    Every chain, queued or not, makes the oracle's decisions."""
+    print("[test] queued heavy launch", flush=True)
+    rng = np.random.default_rng(5)
+    descs = []
+    for c in range(340):
+        nup, ndn = int(rng.integers(1, 4)), int(rng.integers(1, 4))
+        descs.append(capi.make_desc(2, 16 * int(rng.integers(1, 3)), 2, 0.05, [(nup, 1.0, -1.0), (ndn, 1.0, -1.0)],
+                                    sho=[(0, 1.0), (1, 1.0)],
+                                    nodes=[(0, capi.PQ_NODES_SHO, 1.0), (1, capi.PQ_NODES_SHO, 1.0)]))
+    b = capi.Batch(descs)
+    b.set_split(300, 512)
+    assert b.split() == (300, 512)
+    ocs = []
+    for c, d in enumerate(descs):
+        beads = capi.initial_beads(d, 900 + c, replicate=True)
+        b.set_paths(c, beads)
+        oc = ol.Chain(d)
+        oc.set_beads(beads)
+        oc.rng_philox(31, c)
+        ocs.append(oc)
+    b.seed(31, 0)
+    b.run_sweeps(2)
+    b.sync()
+    paths = np.empty(int(sum(d.nbeads() for d in descs)))
+    capi.check(b.lib.pq_get_all_paths(b.h, capi.dptr(paths), paths.size))
+    off = 0
+    for c, oc in enumerate(ocs):
+        oc.run_sweeps(2)
+        sg, so = b.stats(c), oc.stats()
+        assert sg.sweeps == 2, c
+        assert sg.draws == so.draws, c
+        assert list(sg.accepts) == list(so.accepts), c
+        n = descs[c].nbeads()
+        assert np.max(np.abs(paths[off:off + n] - oc.get_beads().ravel())) < 1e-9, c
+        off += n
+
+
 def test_walking_chooser_veto_on_device():
     """An injected uniform of exactly 1.0 in WalkingChooser's veto slot turns
     every move down on the device as in the oracle; just below 1 the move runs."""
