@@ -167,8 +167,61 @@ __device__ __forceinline__ double pg_eval(const PG& p, double x) {
   return pg_eval(p.nome, p.pref, p.k, p.nmax, x, nullptr);
 }
 
+// Natural logarithm of a positive normal double to a few ulp: x = m 2^e with
+// m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with s = (m - 1)/(m + 1),
+// |s| < 0.1716, by the odd series 2 s (1 + s^2/3 + ... + s^20/21), whose
+// truncation is below 1e-18 relative.  About a third of the instructions of
+// the library's log, which carries a double-double result.
+__device__ __forceinline__ double fast_log(double x) {
+  double m = __builtin_amdgcn_frexp_mant(x);
+  int e = __builtin_amdgcn_frexp_exp(x);
+  if (m < 0.70710678118654752) { m += m; --e; }
+  const double num = m - 1.0, den = m + 1.0;
+  double r = __builtin_amdgcn_rcp(den);
+  r = fma(fma(-den, r, 1.0), r, r);
+  r = fma(fma(-den, r, 1.0), r, r);
+  double s = num * r;
+  s = fma(fma(-den, s, num), r, s);
+  const double s2 = s * s;
+  double p = 1.0 / 21;
+  p = fma(p, s2, 1.0 / 19);
+  p = fma(p, s2, 1.0 / 17);
+  p = fma(p, s2, 1.0 / 15);
+  p = fma(p, s2, 1.0 / 13);
+  p = fma(p, s2, 1.0 / 11);
+  p = fma(p, s2, 1.0 / 9);
+  p = fma(p, s2, 1.0 / 7);
+  p = fma(p, s2, 1.0 / 5);
+  p = fma(p, s2, 1.0 / 3);
+  const double t = (s + s) * (s2 * p);   // 2 atanh(s) - 2 s
+  const double de = (double)e;
+  // ln 2 split so that e ln2_hi is exact for |e| < 2^11
+  return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, t) + (s + s));
+}
+
+// 1/d for a positive normal double to about an ulp: the hardware estimate and
+// two Newton steps
+__device__ __forceinline__ double fast_rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return fma(fma(-d, r, 1.0), r, r);
+}
+
+// sqrt of a non-negative double to about an ulp (0 gives 0) for arguments
+// in the normal range: the hardware reciprocal square root refined by one
+// Goldschmidt step and one Newton correction
+__device__ __forceinline__ double fast_sqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  g = fma(fma(-g, g, x), h, g);
+  return x > 0.0 ? g : 0.0;
+}
+
 // fast transcendental overloads for the arithmetic type
-__device__ __forceinline__ double r_log(double v) { return log(v); }
+__device__ __forceinline__ double r_log(double v) { return fast_log(v); }
 __device__ __forceinline__ float r_log(float v) { return __logf(v); }
 __device__ __forceinline__ double r_sqrt(double v) { return sqrt(v); }
 __device__ __forceinline__ float r_sqrt(float v) { return __fsqrt_rn(v); }

@@ -81,7 +81,7 @@ __device__ __forceinline__ double pair_sep(const ChainP& c, const double* sl, in
   double r = 0;
 #pragma unroll
   for (int d = 0; d < D; ++d) r += dl[d] * dl[d];
-  return sqrt(r);
+  return fast_sqrt(r);
 }
 
 // vtau of a pair on one link from its separations dl (slice t, length r) and
@@ -93,9 +93,9 @@ __device__ __forceinline__ double link_vt(const TableP& tb, const double* u, con
   double s2 = 0;
 #pragma unroll
   for (int d = 0; d < D; ++d) { const double sv = dl[d] - pd[d]; s2 += sv * sv; }
-  s2 /= qm * qm;
+  s2 *= fast_rcp(qm * qm);
   const double qq = qm * tb.rgridinv;
-  double xx = log(qq) * tb.logrratioinv;
+  double xx = fast_log(qq) * tb.logrratioinv;
   int ig = (int)xx;
   if (qq < 1) { ig = 0; xx = 0; }
   else if (ig > tb.ngpts - 2) { ig = tb.ngpts - 2; xx = 1; }
