@@ -38,7 +38,7 @@ METRIC = "bead-updates/sec on 1 MI355X, mixed-batched vs FP64 at fixed energy er
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 TUNE_AFTER = 20          # sweeps from the replicated start before the autotuner times anything
-STEADY = {"batch3d": 120, "batch2d": 220}   # sweeps before the timed steps (tuning sweeps included)
+STEADY = {"batch3d": 120, "batch2d": 220}   # sweeps from the replicated start before the warm-up steps
 PAIR_CF = ([0, 0, 1], 0.0, 8.0, 256)  # species 0 against species 0 and 1, r in [0, 8), 256 bins
 
 
@@ -135,32 +135,41 @@ def identical_chains(bm, br, em, er):
     return np.array(same)
 
 
-def run_gpu(descs, precision, steps, warmup, seed, steady, autotune=True, schedule=None, pair_cf=None):
-    """Sweeps from the replicated start up to `steady` (the autotuner's timing
-    sweeps among them; it runs after TUNE_AFTER sweeps, when the chains have
-    left the replicated start and the node records are warm), then `warmup`
-    untimed steps and `steps` timed steps, each step one graph-launched sweep
-    plus the Measure.  schedule = (team, inverse, tables, node state[, heavy
-    chains, heavy team]) fixes the schedule instead of autotuning."""
+def tune(descs, precision, seed):
+    """The autotuner's pick on a batch of its own (TUNE_AFTER sweeps from the
+    replicated start, then the timed schedules), so that the measured runs
+    keep one schedule from their first sweep: the node-matrix width a chain
+    is evaluated with changes the rounding, and a paired mixed/FP64
+    comparison needs the same arithmetic apart from the precision."""
+    b = make_batch(descs, precision, seed)
+    b.run_sweeps(TUNE_AFTER)
+    ms, _ = b.autotune(1)
+    team, inv, _ = b.schedule()
+    tab, fst = b.staging()
+    heavy, heavy_team = b.split()
+    del b
+    log(f"autotuned: team {team} inverse {inv} tables {tab} state {fst} split {heavy}x{heavy_team} ({ms:.2f} ms)")
+    return (team, inv, tab, fst, heavy, heavy_team), ms
+
+
+def run_gpu(descs, precision, steps, warmup, seed, steady, schedule=None, pair_cf=None, tuned=None):
+    """`steady` sweeps from the replicated start, then `warmup` untimed steps
+    and `steps` timed steps, each step one graph-launched sweep plus the
+    Measure, all under one schedule = (team, inverse, tables, node state[,
+    heavy chains, heavy team]), or the library's default."""
     b = make_batch(descs, precision, seed)
     if pair_cf:
         b.set_measure_pair_cf(*pair_cf)
-    tuned, done = None, 0
     log(f"{len(descs)} chains, precision {precision}: batch ready")
     if schedule is not None:
         b.set_schedule(schedule[0], schedule[1])
         b.set_staging(schedule[2], schedule[3])
         if len(schedule) > 4 and schedule[4] != 0:
             b.set_split(schedule[4], schedule[5])
-    elif autotune:
-        b.run_sweeps(TUNE_AFTER)
-        tuned, done = b.autotune(1)
-        done += TUNE_AFTER
-    extra = max(0, steady - done)
-    b.run_sweeps(extra)
+    b.run_sweeps(steady)
     b.sync()
-    done += extra
-    log(f"{done} sweeps done (autotuned: {tuned})")
+    done = steady
+    log(f"{done} sweeps done")
     if warmup > 0:
         b.run_measured_timed(warmup, 1)
         done += warmup
@@ -335,8 +344,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    sched = tuple(int(v) for v in args.schedule.split(",")) if args.schedule else None
-    b_main, run = run_gpu(descs, prec, args.steps, args.warmup, args.seed, steady, schedule=sched, pair_cf=PAIR_CF)
+    tuned = None
+    if args.schedule:
+        sched = tuple(int(v) for v in args.schedule.split(","))
+    else:
+        sched, tuned = tune(descs, prec, args.seed)
+    b_main, run = run_gpu(descs, prec, args.steps, args.warmup, args.seed, steady, schedule=sched, pair_cf=PAIR_CF,
+                          tuned=tuned)
     bu_total = float(run["bu"].sum())
     value = bu_total / run["wall"]
     kernel_ms, roof = roofline(descs, run, args.steps)
@@ -364,13 +378,12 @@ def main():
         "pair_cf_counts": int(run["counts"].sum()) if run["counts"] is not None else None,
     }
     if not args.no_fp64_pair and prec == capi.PQ_PREC_MIXED:
-        # the same sweeps (the mixed run's tuning sweeps are matched by plain
-        # sweeps) with the mixed run's team size and launch split, so that both
-        # runs evaluate node matrices with the same Gauss-Jordan instance (LDS
-        # placement is left automatic: it moves data, not arithmetic)
+        # the same sweeps from the same start with the mixed run's team size and
+        # launch split from the first sweep, so that both runs evaluate node
+        # matrices with the same Gauss-Jordan instance (LDS placement is left
+        # automatic: it moves data, not arithmetic)
         sc = run["schedule"]
-        b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed,
-                           run["warmup_done"] - args.warmup, pair_cf=PAIR_CF,
+        b64, ref = run_gpu(descs, capi.PQ_PREC_FP64, args.steps, args.warmup, args.seed, steady, pair_cf=PAIR_CF,
                            schedule=(sc["team"], -1, -1, -1, sc["heavy_chains"], sc["heavy_team"]))
         out["fp64_batched"] = {"value": float(ref["bu"].sum()) / ref["wall"],
                                "kernel_ms": float(np.mean(ref["sweep_ms"]))}
@@ -380,7 +393,9 @@ def main():
     if not args.no_extras:
         if args.workload != "batch2d":
             d3, w3 = workload("batch2d", 1024)
-            b3, r3 = run_gpu(d3, prec, args.steps, args.warmup, args.seed, STEADY["batch2d"], pair_cf=PAIR_CF)
+            s3, t3ms = tune(d3, prec, args.seed)
+            b3, r3 = run_gpu(d3, prec, args.steps, args.warmup, args.seed, STEADY["batch2d"], schedule=s3,
+                             pair_cf=PAIR_CF, tuned=t3ms)
             k3, roof3 = roofline(d3, r3, args.steps)
             t3, t3src = measured_traffic("batch2d", args.precision)
             roof3["traffic"], roof3["traffic_source"] = t3, t3src
@@ -390,7 +405,7 @@ def main():
             del b3
         out["energy_study"] = energy_study(args.energy_sweeps, args.seed, STEADY["batch2d"])
         single = [capi.dot2d_3p3_desc()]
-        _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed, 20, autotune=False)
+        _, one = run_gpu(single, capi.PQ_PREC_FP64, max(2, args.steps // 5), 1, args.seed, 20)
         r1 = float(one["bu"].sum()) / one["wall"]
         out["fp64_single_chain"] = {"workload": "dot2d: 3+3 e- 2D dot, SHONodes, M=400", "value": r1}
         out["ratio_vs_fp64_single_chain"] = value / r1
