@@ -52,6 +52,13 @@ struct MeasItem {
   int chain, t0, t1;
 };
 
+// Stride in doubles of the energy items' staged slices: odd, so that lanes
+// reading one particle on consecutive slices fall on different LDS banks.
+inline __host__ __device__ int meas_slice_stride(int nd) { return nd | 1; }
+
+// Links per lane and pair in the energy items (one separation more than links)
+constexpr int kMeasPairLinks = 4;
+
 #define PQ_MAX_CF_SPECIES 8   // species1 and up to seven partner species of a pair-correlation histogram
 
 // PairCFEstimator<1> with PairRadial: sp[0] against partners sp[1..nsp-1]

@@ -27,7 +27,7 @@ struct PairMap {
 // Stage slices t0-1 .. t1-1 of chain c into sl[(t1-t0+1)][N][D]; slice -1 is
 // slice M-1 of inversePermutation[i] (SerialPaths.cc:68-77).
 template <int D>
-__device__ void stage_slices(const DevCtx& x, const ChainP& c, int t0, int t1, double* sl) {
+__device__ void stage_slices(const DevCtx& x, const ChainP& c, int t0, int t1, double* sl, int ss) {
   const double* beads = x.beads + c.bead_off;
   const int* invperm = x.perm + c.perm_off + c.N;
   const int nd = c.N * D;
@@ -42,7 +42,7 @@ __device__ void stage_slices(const DevCtx& x, const ChainP& c, int t0, int t1, d
     } else {
       v = beads[(int64_t)t * nd + r];
     }
-    sl[q] = v;
+    sl[k * ss + r] = v;
   }
 }
 
@@ -71,10 +71,10 @@ __device__ __forceinline__ void pair_of(int n, float invn, int p, int& i, int& j
 
 // the separation of the pair (i, j) on staged slice k: pbc(r_i - r_j) and its length
 template <int D>
-__device__ __forceinline__ double pair_sep(const ChainP& c, const double* sl, int nd, int k, int i, int j,
+__device__ __forceinline__ double pair_sep(const ChainP& c, const double* sl, int ss, int k, int i, int j,
                                            double* dl) {
-  const double* a = sl + k * nd + i * D;
-  const double* b = sl + k * nd + j * D;
+  const double* a = sl + k * ss + i * D;
+  const double* b = sl + k * ss + j * D;
 #pragma unroll
   for (int d = 0; d < D; ++d) dl[d] = a[d] - b[d];
   pbc<D>(c, dl);
@@ -85,28 +85,59 @@ __device__ __forceinline__ double pair_sep(const ChainP& c, const double* sl, in
 }
 
 // vtau of a pair on one link from its separations dl (slice t, length r) and
-// pd (slice t-1, length pr) (uk0CalcDerivatives' vtau, PairAction.cc:640-657)
-template <int D>
-__device__ __forceinline__ double link_vt(const TableP& tb, const double* u, const double* dl, double r,
-                                          const double* pd, double pr) {
+// pd (slice t-1, length pr) (uk0CalcDerivatives' vtau, PairAction.cc:640-657):
+// the Horner sums in s^2/q^2 of the two bracketing grid points' coefficients,
+// A and B, interpolated as A + x (B - A).  K1 > 0 fixes the number of
+// coefficients at compile time (the table row of both grid points is read at
+// once); K1 = 0 reads tb.K1.  Coefficient k of grid point g is u[g sg + k sk]:
+// rows [g][k] in global memory (sg = K1, sk = 1), planes [k][g] in LDS (sg =
+// 1, sk = ngpts), where a lane's two grid points are adjacent doubles and
+// random rows of different lanes spread over the banks.
+template <int D, int K1>
+__device__ __forceinline__ double link_vt(const TableP& tb, const double* u, int sg, int sk, const double* dl,
+                                          double r, const double* pd, double pr) {
   const double qm = 0.5 * (r + pr);
   double s2 = 0;
 #pragma unroll
   for (int d = 0; d < D; ++d) { const double sv = dl[d] - pd[d]; s2 += sv * sv; }
   s2 *= fast_rcp(qm * qm);
   const double qq = qm * tb.rgridinv;
-  double xx = fast_log(qq) * tb.logrratioinv;
-  int ig = (int)xx;
-  if (qq < 1) { ig = 0; xx = 0; }
-  else if (ig > tb.ngpts - 2) { ig = tb.ngpts - 2; xx = 1; }
-  else xx -= ig;
-  double vt = 0;
-  const double* u0 = u + ig * tb.K1;
-  for (int kk = tb.K1 - 1; kk >= 0; --kk) {
-    vt *= s2;
-    vt += (1 - xx) * u0[kk] + xx * u0[tb.K1 + kk];
+  double xx = 0;
+  int ig = 0;
+  if (qq >= 1) {
+    xx = fast_log(qq) * tb.logrratioinv;
+    ig = (int)xx;
+    if (ig > tb.ngpts - 2) { ig = tb.ngpts - 2; xx = 1; }
+    else xx -= ig;
   }
-  return vt;
+  const int k1 = K1 > 0 ? K1 : tb.K1;
+  const double* u0 = u + ig * sg;
+  double a = u0[(k1 - 1) * sk], b = u0[(k1 - 1) * sk + sg];
+  if constexpr (K1 > 0) {
+#pragma unroll
+    for (int kk = K1 - 2; kk >= 0; --kk) { a = fma(a, s2, u0[kk * sk]); b = fma(b, s2, u0[kk * sk + sg]); }
+  } else {
+    for (int kk = k1 - 2; kk >= 0; --kk) { a = fma(a, s2, u0[kk * sk]); b = fma(b, s2, u0[kk * sk + sg]); }
+  }
+  return fma(xx, b - a, a);
+}
+
+// The links k0..k1-1 of the pair (i, j): a lane takes each link's earlier
+// separation from the previous link (one pbc and one sqrt per link).
+template <int D, int K1>
+__device__ __forceinline__ double pair_links(const ChainP& c, const TableP& tb, const double* u, int sg, int sk,
+                                             const double* sl, int ss, int k0, int k1, int i, int j) {
+  double acc = 0;
+  double pd[D], dl[D];
+  double pr = pair_sep<D>(c, sl, ss, k0, i, j, pd);
+  for (int k = k0; k < k1; ++k) {
+    const double r = pair_sep<D>(c, sl, ss, k + 1, i, j, dl);
+    acc += link_vt<D, K1>(tb, u, sg, sk, dl, r, pd, pr);
+#pragma unroll
+    for (int d = 0; d < D; ++d) pd[d] = dl[d];
+    pr = r;
+  }
+  return acc;
 }
 
 // utau of the links (t-1, t) of slices t0..t1-1 of one chain, this thread's
@@ -118,18 +149,18 @@ __device__ __forceinline__ double link_vt(const TableP& tb, const double* u, con
 // entries, so indices split by div_small.
 template <int D>
 __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, const PairMap& pm, const double* sl,
-                                            const double* tab, const int* utoff, const TableP* tbs, int t0,
-                                            int t1) {
+                                            const double* tab, bool planes, const int* utoff,
+                                            const TableP* tbs, int t0, int t1) {
   const int N = c.N, T = t1 - t0;
-  const int nd = N * D;
+  const int ss = meas_slice_stride(N * D);
   const float invN = 1.0f / N;
   double acc = 0;
   // spring and SHO: lanes over (slice, particle)
   if (c.spring || c.nsho) {
     for (int q = threadIdx.x; q < T * N; q += blockDim.x) {
       const int k = div_small(q, N, invN), i = q - k * N;
-      const double* b1 = sl + (k + 1) * nd + i * D;
-      const double* b0 = sl + k * nd + i * D;
+      const double* b1 = sl + (k + 1) * ss + i * D;
+      const double* b0 = sl + k * ss + i * D;
       const int sp = species_of(c, i);
       double delta[D];
 #pragma unroll
@@ -146,8 +177,9 @@ __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, co
         if (i < c.ifirst[ssp] || i >= c.ifirst[ssp] + c.count[ssp]) continue;
         const int ndim = c.sho_ndim[a] ? c.sho_ndim[a] : D;
         const double omega = c.sho_omega[a], m = c.mass[ssp];
-        const double wt = omega * c.tau, sinhwt = sinh(wt), coshwt = cosh(wt);
-        const double cschwt = 1.0 / sinhwt, cothwt = coshwt * cschwt, tin = 1.0 / c.tau;
+        // cosh(omega tau) and 1/(2 sinh(omega tau)) of the chain block
+        const double coshwt = c.sho_cosh[a][0], cschwt = 2.0 * c.sho_div1[a][0];
+        const double cothwt = coshwt * cschwt, tin = 1.0 / c.tau;
         double r0r0 = 0, r0r1 = 0, r1r1 = 0, delta2 = 0;
         for (int d = D - ndim; d < D; ++d) {
           const double r1 = b1[d] - c.sho_center[a][d], r0 = b0[d] - c.sho_center[a][d];
@@ -159,32 +191,28 @@ __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, co
       }
     }
   }
-  // pair actions: lanes over (slice group, pair), H groups of consecutive
-  // slices chosen so that every lane has work; walking its slices, a lane
-  // takes a link's earlier separation from the previous link (one pbc and one
-  // sqrt per link)
+  // pair actions: lanes over (pair, group of kMeasPairLinks links),
+  // pair-major, so that the lanes of one pair walk consecutive slices and
+  // look up the same or neighbouring table rows (random rows across a wave
+  // make the gathers conflict in the LDS banks); a lane takes each link's
+  // earlier separation from the previous link
   if (c.npair) {
     const int P = N * (N - 1) / 2;
-    const int H = P > 0 ? min(T, max(1, ((int)blockDim.x + P - 1) / P)) : 0;
-    const float invP = 1.0f / max(P, 1);
+    const int H = (T + kMeasPairLinks - 1) / kMeasPairLinks;
+    const float invH = 1.0f / H;
     for (int q = threadIdx.x; q < H * P; q += blockDim.x) {
-      const int h = div_small(q, P, invP), p = q - h * P;
+      const int p = div_small(q, H, invH), h = q - p * H;
       int i, j;
       pair_of(N, invN, p, i, j);
       const int slot = pm.slot[species_of(c, i)][species_of(c, j)];
       if (slot < 0) continue;
       const TableP tb = tbs[slot];
       const double* u = tab + utoff[slot];
-      const int k0 = (h * T) / H, k1 = ((h + 1) * T) / H;
-      double pd[D], dl[D];
-      double pr = pair_sep<D>(c, sl, nd, k0, i, j, pd);
-      for (int k = k0; k < k1; ++k) {
-        const double r = pair_sep<D>(c, sl, nd, k + 1, i, j, dl);
-        acc += link_vt<D>(tb, u, dl, r, pd, pr);
-#pragma unroll
-        for (int d = 0; d < D; ++d) pd[d] = dl[d];
-        pr = r;
-      }
+      const int k0 = h * kMeasPairLinks, k1 = min(T, k0 + kMeasPairLinks);
+      const int sg = planes ? 1 : tb.K1, sk = planes ? tb.ngpts : 1;
+      if (tb.K1 == 4) acc += pair_links<D, 4>(c, tb, u, sg, sk, sl, ss, k0, k1, i, j);
+      else if (tb.K1 == 1) acc += pair_links<D, 1>(c, tb, u, sg, sk, sl, ss, k0, k1, i, j);
+      else acc += pair_links<D, 0>(c, tb, u, sg, sk, sl, ss, k0, k1, i, j);
     }
   }
   // Ewald k-space utau of every slice (particle 0 carries it): lanes over (slice, k)
@@ -194,7 +222,7 @@ __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, co
     for (int q = threadIdx.x; q < T * c.ew.nk; q += blockDim.x) {
       const int k = q / c.ew.nk;
       const double* kv = kp + 4 * (q - k * c.ew.nk);
-      const double* s = sl + (k + 1) * nd;
+      const double* s = sl + (k + 1) * ss;
       double rr = 0, ri = 0;
       for (int jj = 0; jj < N; ++jj) {
         double co, si;
@@ -276,15 +304,19 @@ __global__ __launch_bounds__(kMeasThreads) void energy_items_kernel(DevCtx x, co
           if (q < pp) continue;
           const TableP& tb = x.tables[c.pair[pp].table];
           const double* src = x.tab_ut + tb.off;
-          for (int e = threadIdx.x; e < tb.ngpts * tb.K1; e += blockDim.x) tab[utoff[pp] + e] = src[e];
+          // rows [g][k] to planes [k][g]
+          for (int e = threadIdx.x; e < tb.ngpts * tb.K1; e += blockDim.x) {
+            const int g = e / tb.K1, k = e - g * tb.K1;
+            tab[utoff[pp] + k * tb.ngpts + g] = src[e];
+          }
         }
       cur = m.chain;
     }
     __syncthreads();
-    stage_slices<D>(x, c, m.t0, m.t1, sl);
+    stage_slices<D>(x, c, m.t0, m.t1, sl, meas_slice_stride(c.N * D));
     __syncthreads();
-    const double v = in_lds ? item_utau<D>(x, c, pm, sl, tab, utoff, tbs, m.t0, m.t1)
-                            : item_utau<D>(x, c, pm, sl, x.tab_ut, utoff, tbs, m.t0, m.t1);
+    const double v = in_lds ? item_utau<D>(x, c, pm, sl, tab, true, utoff, tbs, m.t0, m.t1)
+                            : item_utau<D>(x, c, pm, sl, x.tab_ut, false, utoff, tbs, m.t0, m.t1);
     const double s = block_sum(v, red);
     if (threadIdx.x == 0) part[it] = s;
   }
@@ -362,7 +394,7 @@ __global__ __launch_bounds__(kMeasThreads) void pair_cf_items_kernel(DevCtx x, c
     }
     if (!valid || njtot == 0) continue;
     for (int b = threadIdx.x; b < g.nbin; b += blockDim.x) hist[b] = 0;
-    stage_slices<D>(x, c, m.t0, m.t1, sl);
+    stage_slices<D>(x, c, m.t0, m.t1, sl, c.N * D);
     __syncthreads();
     const int nd = c.N * D, per = n1 * njtot, T = m.t1 - m.t0;
     const float invnj = 1.0f / njtot;
@@ -412,7 +444,7 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n) {
     p[q] = 0;
 }
 
-// three 48 KB workgroups fit a CU's LDS: four rounds of them over 256 CUs
+// three energy workgroups (under 54 KB of LDS each) fit a CU: four rounds of them over 256 CUs
 static int meas_grid(int nitems) { return nitems < 3072 ? (nitems > 0 ? nitems : 1) : 3072; }
 
 hipError_t launch_energy_items(int D, const DevCtx& x, const MeasItem* items, int nitems, const int* first_item,
