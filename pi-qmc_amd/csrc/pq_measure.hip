@@ -191,7 +191,7 @@ __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, co
       }
     }
   }
-  // pair actions: lanes over (pair, group of kMeasPairLinks links),
+  // pair actions: lanes over (pair, group of up to kMeasPairLinks links),
   // pair-major, so that the lanes of one pair walk consecutive slices and
   // look up the same or neighbouring table rows (random rows across a wave
   // make the gathers conflict in the LDS banks); a lane takes each link's
@@ -208,7 +208,7 @@ __device__ __forceinline__ double item_utau(const DevCtx& x, const ChainP& c, co
       if (slot < 0) continue;
       const TableP tb = tbs[slot];
       const double* u = tab + utoff[slot];
-      const int k0 = h * kMeasPairLinks, k1 = min(T, k0 + kMeasPairLinks);
+      const int k0 = (h * T) / H, k1 = ((h + 1) * T) / H;   // groups of equal length
       const int sg = planes ? 1 : tb.K1, sk = planes ? tb.ngpts : 1;
       if (tb.K1 == 4) acc += pair_links<D, 4>(c, tb, u, sg, sk, sl, ss, k0, k1, i, j);
       else if (tb.K1 == 1) acc += pair_links<D, 1>(c, tb, u, sg, sk, sl, ss, k0, k1, i, j);
