@@ -728,11 +728,11 @@ static hipError_t run_launch(pq_batch* b, const DevCtx& x, int nsweep) {
 }
 
 // Measure work items.  An energy item costs about N spring/SHO links plus
-// N(N-1)/2 pair links per slice; items hold up to ~16k of those (64 per
-// thread of a 256-thread workgroup, and always fewer than 2^16 (slice,
-// pair) entries) and at most 1920 staged doubles: with the 32 KB of utau
-// tables and ~6 KB of fixed LDS an energy workgroup stays under a third of
-// the CU's 160 KB, so three run per CU.  Histogram items are
+// N(N-1)/2 pair links per slice; items hold up to ~32k of those (64 per
+// thread of a 512-thread workgroup, and always fewer than 2^16 (slice,
+// pair) entries) and at most 3840 staged doubles: with the 32 KB of utau
+// tables and ~6 KB of fixed LDS an energy workgroup stays under half of
+// the CU's 160 KB, so two run per CU.  Histogram items are
 // coarser (~64k distances) so that each adds its bins to the chain's global
 // counts once.
 static void build_measure_items(pq_batch* b) {
@@ -744,9 +744,9 @@ static void build_measure_items(pq_batch* b) {
     const ChainP& ch = b->hc[c];
     const int64_t nd = (int64_t)ch.N * ch.D;
     const int64_t per = ch.N + (ch.npair ? (int64_t)ch.N * (ch.N - 1) / 2 : 0) + (ch.ew.nk ? (int64_t)ch.ew.nk * ch.N / 4 : 0);
-    int64_t te = std::max<int64_t>(1, 16384 / std::max<int64_t>(1, per));
+    int64_t te = std::max<int64_t>(1, 32768 / std::max<int64_t>(1, per));
     const int64_t ss = meas_slice_stride((int)nd);
-    te = std::min<int64_t>(te, std::max<int64_t>(1, 1920 / ss - 1));
+    te = std::min<int64_t>(te, std::max<int64_t>(1, 3840 / ss - 1));
     te = std::min<int64_t>(te, ch.M);
     for (int t = 0; t < ch.M; t += (int)te) b->items_e.push_back(MeasItem{c, t, (int)std::min<int64_t>(ch.M, t + te)});
     b->first_e.push_back((int)b->items_e.size());

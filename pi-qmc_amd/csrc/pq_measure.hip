@@ -17,7 +17,8 @@
 
 namespace pq {
 
-constexpr int kMeasThreads = 256;
+constexpr int kMeasThreads = 256;     // histogram workgroups
+constexpr int kEnergyThreads = 512;   // energy workgroups: two per CU share their tables across eight waves
 
 // the CoulombAction pair table of species (si, sj), -1 when they do not interact
 struct PairMap {
@@ -255,7 +256,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 constexpr int kTabLdsDoubles = 4096;
 
 template <int D>
-__global__ __launch_bounds__(kMeasThreads) void energy_items_kernel(DevCtx x, const MeasItem* items, int nitems,
+__global__ __launch_bounds__(kEnergyThreads, 4) void energy_items_kernel(DevCtx x, const MeasItem* items, int nitems,
                                                                      double* part) {
   extern __shared__ double lds[];
   __shared__ ChainP c;
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(kMeasThreads) void energy_items_kernel(DevCtx x, co
   __shared__ int utoff[PQ_MAX_PAIRS];
   __shared__ TableP tbs[PQ_MAX_PAIRS];
   __shared__ int in_lds;
-  __shared__ double red[kMeasThreads / PQ_WAVE];
+  __shared__ double red[kEnergyThreads / PQ_WAVE];
   int cur = -1;
   double* tab = lds;
   double* sl = lds + kTabLdsDoubles;
@@ -444,17 +445,21 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n) {
     p[q] = 0;
 }
 
-// three energy workgroups (under 54 KB of LDS each) fit a CU: four rounds of them over 256 CUs
-static int meas_grid(int nitems) { return nitems < 3072 ? (nitems > 0 ? nitems : 1) : 3072; }
+// persistent grids: three histogram workgroups per CU, or two energy
+// workgroups (under 80 KB of LDS each), four rounds of them over 256 CUs
+static int meas_grid(int nitems, int per_cu) {
+  const int g = 4 * 256 * per_cu;
+  return nitems < g ? (nitems > 0 ? nitems : 1) : g;
+}
 
 hipError_t launch_energy_items(int D, const DevCtx& x, const MeasItem* items, int nitems, const int* first_item,
                                int nchain, int max_slice_doubles, double* part, double* energy, hipStream_t st) {
   const size_t lds = sizeof(double) * ((size_t)kTabLdsDoubles + max_slice_doubles);
-  const int grid = meas_grid(nitems);
+  const int grid = meas_grid(nitems, 2);
   if (D == 2)
-    hipLaunchKernelGGL(energy_items_kernel<2>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, part);
+    hipLaunchKernelGGL(energy_items_kernel<2>, dim3(grid), dim3(kEnergyThreads), lds, st, x, items, nitems, part);
   else
-    hipLaunchKernelGGL(energy_items_kernel<3>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, part);
+    hipLaunchKernelGGL(energy_items_kernel<3>, dim3(grid), dim3(kEnergyThreads), lds, st, x, items, nitems, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(energy_reduce_kernel, dim3((nchain + 3) / 4), dim3(4 * PQ_WAVE), 0, st, x, first_item, nchain,
@@ -465,7 +470,7 @@ hipError_t launch_energy_items(int D, const DevCtx& x, const MeasItem* items, in
 hipError_t launch_pair_cf_items(int D, const DevCtx& x, const MeasItem* items, int nitems, int max_slice_doubles,
                                 const PairCfArgs& g, uint64_t* counts, hipStream_t st) {
   const size_t lds = sizeof(double) * ((size_t)(g.nbin + 1) / 2 + max_slice_doubles);
-  const int grid = meas_grid(nitems);
+  const int grid = meas_grid(nitems, 3);
   if (D == 2)
     hipLaunchKernelGGL(pair_cf_items_kernel<2>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, g, counts);
   else
