@@ -440,10 +440,13 @@ def test_node_records_follow_new_paths():
 
 def test_mixed_precision_energy():
     """Mixed precision on the same Philox streams as FP64.  Both precisions
-    move beads in FP64, so a chain whose decisions never differed ends on the
-    same path and its energy matches FP64 to the estimator's rounding; flips
+    move beads in FP64 and estimate in FP64, so a chain whose decisions never
+    differed ends on the same path with the same energy, bit for bit; flips
     need a uniform within ~1e-6 of the acceptance probability, so nearly
-    every chain keeps the FP64 decision stream over a few sweeps."""
+    every chain keeps the FP64 decision stream over a few sweeps.  (Whether
+    the flips bias the sampled energy is the statistical tests' question:
+    test_mixed_precision_sampled_energy and the mixed cases of the SHO and
+    fermion energies against their exact values.)"""
     print("[test] mixed_precision_energy", flush=True)
     descs = capi.batch2d_descs(64)
     runs = []
@@ -460,8 +463,26 @@ def test_mixed_precision_energy():
     print("identical chains", int(same.sum()), "of", len(descs))
     assert np.all(np.isfinite(e32))
     assert same.sum() >= 0.9 * len(descs)
-    # |E_mixed - E_FP64| <= 1e-5 Ha on identical decision streams
-    assert np.max(np.abs(e32[same] - e64[same])) <= 1e-5
+    # identical decision streams give identical energies
+    assert np.array_equal(e32[same], e64[same])
+
+
+def test_mixed_precision_sampled_energy():
+    """The interacting 2-electron 3D dot (Coulomb pair action, M 200, L 3)
+    sampled on 512 independent chains in each precision, on different
+    Philox streams: the two ensembles' mean thermal energies agree within 3
+    combined standard errors -- no bias of the mixed mode's FP32 spring, SHO
+    and pair-spline arithmetic at the ~0.5 % (0.019 Ha*) level of these runs."""
+    print("[test] mixed_precision_sampled_energy", flush=True)
+    d = capi.dot3d_2e_desc()
+    runs = {}
+    for prec, seed in ((capi.PQ_PREC_FP64, 61), (capi.PQ_PREC_MIXED, 62)):
+        m = _chain_means(d, 512, 200, 200, seed, lambda c: capi.initial_beads(d, 9000 + c), precision=prec)
+        runs[prec] = (m.mean(), m.std(ddof=1) / np.sqrt(len(m)))
+    (e64, s64), (e32, s32) = runs[capi.PQ_PREC_FP64], runs[capi.PQ_PREC_MIXED]
+    z = (e32 - e64) / np.hypot(s64, s32)
+    print("E fp64", e64, "+-", s64, "mixed", e32, "+-", s32, "z", z)
+    assert abs(z) < 3
 
 
 @pytest.mark.parametrize("name", ["dot3d", "dot2d", "pbc"])
@@ -495,12 +516,12 @@ def test_action_difference_mixed(name):
     assert err.max() <= 5e-5
 
 
-def _chain_means(desc, nchain, nequil, nmeasure, seed, beads_fn, virial_window=0):
+def _chain_means(desc, nchain, nequil, nmeasure, seed, beads_fn, virial_window=0, precision=capi.PQ_PREC_FP64):
     """Device sampler on nchain independent copies (Philox stream per chain):
     nequil sweeps, then nmeasure sweeps with one energy measurement each;
     returns the per-chain time averages (and of the virial estimator when a
     window is given)."""
-    b = capi.Batch([desc] * nchain)
+    b = capi.Batch([desc] * nchain, precision=precision)
     for c in range(nchain):
         b.set_paths(c, beads_fn(c))
     b.seed(seed, 0)
@@ -517,17 +538,19 @@ def _chain_means(desc, nchain, nequil, nmeasure, seed, beads_fn, virial_window=0
     return acc / nmeasure
 
 
-def test_sampled_sho_energy_matches_analytic():
+@pytest.mark.parametrize("precision", [capi.PQ_PREC_FP64, capi.PQ_PREC_MIXED])
+def test_sampled_sho_energy_matches_analytic(precision):
     """Two distinguishable particles in a 3D dot (omega 1, beta 10, M 200,
     L 3, no Coulomb): E = 2 (3/2) coth(beta/2) = 3 coth(5).  Chains start on
     straight paths (one Gaussian position per particle) and equilibrate for
     300 sweeps, long against the relaxation of the path's slow modes; chain
     means of 256 independent chains agree with the exact value within 3
-    standard errors."""
-    print("[test] sampled_sho_energy", flush=True)
+    standard errors, in FP64 and with the mixed mode's FP32 spring and SHO
+    arithmetic in the sampler."""
+    print(f"[test] sampled_sho_energy precision={precision}", flush=True)
     d = capi.make_desc(3, 200, 3, 0.05, [(1, 1.0, -1.0), (1, 1.0, -1.0)], sho=[(0, 1.0), (1, 1.0)])
     means, vmeans = _chain_means(d, 256, 300, 300, 31, lambda c: capi.initial_beads(d, 7000 + c, replicate=True),
-                                 virial_window=10)
+                                 virial_window=10, precision=precision)
     exact = 3.0 / np.tanh(5.0)
     for name, m in (("thermo", means), ("virial", vmeans)):
         err = m.std(ddof=1) / np.sqrt(len(m))
@@ -538,18 +561,20 @@ def test_sampled_sho_energy_matches_analytic():
     assert vmeans.std() < means.std()
 
 
-def test_sampled_fermion_energy_matches_exact_nodes():
+@pytest.mark.parametrize("precision", [capi.PQ_PREC_FP64, capi.PQ_PREC_MIXED])
+def test_sampled_fermion_energy_matches_exact_nodes(precision):
     """Two spin-polarised free fermions in a 2D dot with exact SHO nodes
     (beta 2, M 64, L 3): E = -d ln Z_F / d beta, Z_F = (Z1(b)^2 - Z1(2b))/2,
     Z1 = (2 sinh(b/2))^-2.  300 sweeps of equilibration from straight paths,
-    then chain means of 256 chains within 3 standard errors."""
+    then chain means of 256 chains within 3 standard errors, in both
+    precision modes."""
     import math
-    print("[test] sampled_fermion_energy", flush=True)
+    print(f"[test] sampled_fermion_energy precision={precision}", flush=True)
     beta, M = 2.0, 64
     d = capi.make_desc(2, M, 3, beta / M, [(2, 1.0, -1.0)], sho=[(0, 1.0)],
                        nodes=[(0, capi.PQ_NODES_SHO, 1.0)], nsection=M // 8)
     start = np.broadcast_to(np.array([[0.7, 0.0], [-0.7, 0.1]]), (M, 2, 2)).copy()
-    means = _chain_means(d, 256, 300, 300, 47, lambda c: start)
+    means = _chain_means(d, 256, 300, 300, 47, lambda c: start, precision=precision)
 
     def lnz(bb):
         z1 = lambda x: (2 * math.sinh(x / 2)) ** -2
