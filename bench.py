@@ -47,19 +47,24 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(workload_name, precision):
-    """HBM bytes per sweep (every sweep_kernel launch of one step) from the
-    committed rocprofv3 FETCH_SIZE and WRITE_SIZE summary of this workload
-    (copy-calibrated), or None."""
+def profile_figure(workload_name, precision, key):
+    """A per-step figure (`key: value` line) of the committed rocprofv3
+    summary of this workload, or None: traffic_bytes_per_step (every
+    sweep_kernel launch of one step, copy-calibrated FETCH_SIZE +
+    WRITE_SIZE), mfma_fp64_flops_per_step (SQ_INSTS_VALU_MFMA_MOPS_F64)."""
     path = os.path.join(ROOT, "profiles", f"sweep_kernel_{workload_name}_{precision}.md")
     try:
         with open(path) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_step:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":", 1)[1]), os.path.relpath(path, ROOT)
     except OSError:
         pass
     return None, None
+
+
+def measured_traffic(workload_name, precision):
+    return profile_figure(workload_name, precision, "traffic_bytes_per_step")
 
 
 def workload(name, nchain):
@@ -357,6 +362,13 @@ def main():
     traffic, traffic_src = measured_traffic(args.workload, args.precision)
     roof["traffic"] = traffic
     roof["traffic_source"] = traffic_src
+    mf, mf_src = profile_figure(args.workload, args.precision, "mfma_fp64_flops_per_step")
+    if mf:
+        # the node matrices' FP64 MFMA flops of one step (rocprofv3 count)
+        # over the sweep span timed here
+        roof["mfma_fp64"] = {"flops_per_step": mf, "achieved_tflops": mf / (kernel_ms * 1e-3) / 1e12,
+                             "peak_tflops": FP64_PEAK_TFLOPS,
+                             "frac": mf / (kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, "source": mf_src}
     ms_step = 1e3 * run["wall"] / args.steps
     beads_per_measure = float(sum(d.nbeads() for d in descs))
     out = {
