@@ -1588,11 +1588,26 @@ struct Eng {
       flag[j] = r[2];
     }
     if (f.nodal) {
-      for (int q = tid; q < S * 2 * n; q += NT) {
-        const int j = q / (2 * n), pass = (q / n) & 1, k = q % n;
-        bool lower;
-        const double* r = slot_rec(fa, j, lower);
-        dd[q] = r[3 + (lower ? pass : 1 - pass) * n + k];
+      // four loads in flight per thread before their stores
+      const int tot = S * 2 * n;
+      for (int q0 = tid; q0 < tot; q0 += 4 * NT) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = q0 + u * NT;
+          v[u] = 0.0;
+          if (q < tot) {
+            const int j = q / (2 * n), pass = (q / n) & 1, k = q % n;
+            bool lower;
+            const double* r = slot_rec(fa, j, lower);
+            v[u] = r[3 + (lower ? pass : 1 - pass) * n + k];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = q0 + u * NT;
+          if (q < tot) dd[q] = v[u];
+        }
       }
     }
     __syncthreads();

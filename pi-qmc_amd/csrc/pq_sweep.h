@@ -189,8 +189,33 @@ __device__ __forceinline__ void sweep_chain(const DevCtx& x, int chain, int nswe
         if (t0 > ihi) t0 = ihi;
       }
       // SerialPaths::getBeads (SerialPaths.cc:119-132): slices before 0 through
-      // the inverse permutation, slices from M on through the permutation
-      for (int q = tid; q < S * (int)slice; q += NT) {
+      // the inverse permutation, slices from M on through the permutation.
+      // Sections inside [0, M) are contiguous runs of S slices: copied with
+      // four loads of each section in flight per thread before their stores
+      // (a load-store loop waits out one memory latency per element).
+      const int nsec = S * (int)slice;
+      if (t0 >= 0 && t0 + S <= M && (!c.nfn || (t0b >= 0 && t0b + S <= M))) {
+        const double* s1 = beads + (int64_t)t0 * slice;
+        const double* s2 = beads + (int64_t)t0b * slice;
+        for (int q0 = tid; q0 < nsec; q0 += 4 * NT) {
+          double v1[4], v2[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * NT;
+            v1[u] = q < nsec ? s1[q] : 0.0;
+            v2[u] = (c.nfn && q < nsec) ? s2[q] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * NT;
+            if (q < nsec) {
+              e.sec1[q] = v1[u];
+              if (c.nfn) e.sec2[q] = v2[u];
+            }
+          }
+        }
+      } else
+      for (int q = tid; q < nsec; q += NT) {
         const int kk = q / (int)slice;
         const int rem = q - kk * (int)slice;
         const int i = rem / D, d = rem - i * D;
