@@ -1579,41 +1579,45 @@ struct Eng {
     double* flag = fn_newdm(fa);
     double* dd = fn_dist(fa);
     // every record of the section in one pass: determinant, scale, flag and
-    // the distances (the latter are overwritten when a record is missing)
-    for (int j = tid; j < S; j += NT) {
-      bool lower;
-      const double* r = slot_rec(fa, j, lower);
-      dm[j] = r[0];
-      wscale[j] = r[1];
-      flag[j] = r[2];
-    }
-    if (f.nodal) {
-      // four loads in flight per thread before their stores
-      const int tot = S * 2 * n;
-      for (int q0 = tid; q0 < tot; q0 += 4 * NT) {
-        double v[4];
+    // the distances (the latter are overwritten when a record is missing),
+    // as one index space of single loads, four in flight per thread before
+    // their stores
+    const int nrec = 3 * S;
+    const int tot = nrec + (f.nodal ? S * 2 * n : 0);
+    for (int q0 = tid; q0 < tot; q0 += 4 * NT) {
+      double v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int q = q0 + u * NT;
-          v[u] = 0.0;
-          if (q < tot) {
-            const int j = q / (2 * n), pass = (q / n) & 1, k = q % n;
-            bool lower;
-            const double* r = slot_rec(fa, j, lower);
-            v[u] = r[3 + (lower ? pass : 1 - pass) * n + k];
-          }
+      for (int u = 0; u < 4; ++u) {
+        const int q = q0 + u * NT;
+        v[u] = 0.0;
+        if (q < nrec) {
+          const int j = q / 3;
+          bool lower;
+          v[u] = slot_rec(fa, j, lower)[q - 3 * j];
+        } else if (q < tot) {
+          const int qd = q - nrec;
+          const int j = qd / (2 * n), pass = (qd / n) & 1, k = qd % n;
+          bool lower;
+          const double* r = slot_rec(fa, j, lower);
+          v[u] = r[3 + (lower ? pass : 1 - pass) * n + k];
         }
+      }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int q = q0 + u * NT;
-          if (q < tot) dd[q] = v[u];
+      for (int u = 0; u < 4; ++u) {
+        const int q = q0 + u * NT;
+        if (q < nrec) {
+          const int j = q / 3, fld = q - 3 * j;
+          (fld == 0 ? dm : fld == 1 ? wscale : flag)[j] = v[u];
+        } else if (q < tot) {
+          dd[q - nrec] = v[u];
         }
       }
     }
     __syncthreads();
-    bool ok = true;
-    for (int j = 0; j < S; ++j) ok = ok && flag[j] == 1.0;
-    if (!ok) return false;
+    // every slice has a record: the lanes of each wave read the flags at once
+    int missing = 0;
+    for (int j = lane; j < S; j += PQ_WAVE) missing |= flag[j] != 1.0;
+    if (wave_or(missing)) return false;
     // the rescale loop of initialize on the recorded determinants,
     // det(s) = det(s_r) (s/s_r)^n.  The first wave evaluates every slice at
     // the current scale at once; when none leaves [1e-50, 1e50] the loop
