@@ -1295,24 +1295,37 @@ struct Eng {
       double lg[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) lg[d] = 0;
-      for (int bb = 0; bb < n; ++bb) {
-        const int jp = pass == 0 ? aa : bb, ip = pass == 0 ? bb : aa;
-        const double w = iv[jp * ins + ip];
-        if (f.model == PQ_NODES_SHO) {
-          const double* rj = r1pos(t, f.ifirst + jp, use_mov);
-          const double* ri = sb(sec2, t, f.ifirst + ip);
-          const double e = sho_e(f, rj, ri);
+      // the inverse's row or column eight elements at a time, their loads
+      // in flight together before the elements are used
+      for (int b0 = 0; b0 < n; b0 += 8) {
+        double wv[8];
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            double gg = pass == 0 ? 2 * f.c * (ri[d] - f.coshwt * rj[d]) * e
-                                  : 2 * f.c * (rj[d] - f.coshwt * ri[d]) * e;
-            lg[d] += w * sc * gg;
+        for (int u = 0; u < 8; ++u) {
+          const int bb = b0 + u;
+          wv[u] = bb < n ? iv[pass == 0 ? aa * ins + bb : bb * ins + aa] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int bb = b0 + u;
+          if (bb >= n) break;
+          const int jp = pass == 0 ? aa : bb, ip = pass == 0 ? bb : aa;
+          const double w = wv[u];
+          if (f.model == PQ_NODES_SHO) {
+            const double* rj = r1pos(t, f.ifirst + jp, use_mov);
+            const double* ri = sb(sec2, t, f.ifirst + ip);
+            const double e = sho_e(f, rj, ri);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              double gg = pass == 0 ? 2 * f.c * (ri[d] - f.coshwt * rj[d]) * e
+                                    : 2 * f.c * (rj[d] - f.coshwt * ri[d]) * e;
+              lg[d] += w * sc * gg;
+            }
+          } else {
+            const double* gf = egr + (((int64_t)t * ins + ip) * ins + jp) * D;
+            // the reversed separation negates the odd gradient factor exactly
+#pragma unroll
+            for (int d = 0; d < D; ++d) lg[d] += w * (pass == 0 ? gf[d] : -gf[d]) * sc;
           }
-        } else {
-          const double* gf = egr + (((int64_t)t * ins + ip) * ins + jp) * D;
-          // the reversed separation negates the odd gradient factor exactly
-#pragma unroll
-          for (int d = 0; d < D; ++d) lg[d] += w * (pass == 0 ? gf[d] : -gf[d]) * sc;
         }
       }
       double g2 = 0;
