@@ -317,11 +317,22 @@ __device__ __forceinline__ R pair_u(const TableP& t, const R* u, R q, R s2, bool
   const R* u0 = u + (int64_t)i * t.K1;
   const R* u1 = u0 + t.K1;
   if (!offdiag) return ((R)1 - xx) * u0[0] + xx * u1[0];
-  R a = 0;
-  for (int k = t.K1 - 1; k >= 0; --k) {
-    a *= s2;
-    a += ((R)1 - xx) * u0[k] + xx * u1[k];
+  // both grid points' coefficients loaded at once (the tables may be in
+  // global memory), then the Horner sum in the same order
+  constexpr int KM = PQ_MAX_NORDER + 1;
+  R c0[KM], c1[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) {
+    c0[k] = k < t.K1 ? u0[k] : (R)0;
+    c1[k] = k < t.K1 ? u1[k] : (R)0;
   }
+  R a = 0;
+#pragma unroll
+  for (int k = KM - 1; k >= 0; --k)
+    if (k < t.K1) {
+      a *= s2;
+      a += ((R)1 - xx) * c0[k] + xx * c1[k];
+    }
   return a;
 }
 
