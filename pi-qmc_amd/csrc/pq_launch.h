@@ -56,8 +56,6 @@ struct MeasItem {
 // reading one particle on consecutive slices fall on different LDS banks.
 inline __host__ __device__ int meas_slice_stride(int nd) { return nd | 1; }
 
-// Links per lane and pair in the energy items (one separation more than links)
-constexpr int kMeasPairLinks = 4;
 
 #define PQ_MAX_CF_SPECIES 8   // species1 and up to seven partner species of a pair-correlation histogram
 

@@ -19,6 +19,7 @@ namespace pq {
 
 constexpr int kMeasThreads = 256;     // histogram workgroups
 constexpr int kEnergyThreads = 512;   // energy workgroups: two per CU share their tables across eight waves
+constexpr int kMeasPairLinks = 16;     // links per lane and pair in the energy items
 
 // the CoulombAction pair table of species (si, sj), -1 when they do not interact
 struct PairMap {
