@@ -733,8 +733,8 @@ static hipError_t run_launch(pq_batch* b, const DevCtx& x, int nsweep) {
 // pair) entries) and at most 3840 staged doubles: with the 32 KB of utau
 // tables and ~6 KB of fixed LDS an energy workgroup stays under half of
 // the CU's 160 KB, so two run per CU.  Histogram items are
-// coarser (~64k distances) so that each adds its bins to the chain's global
-// counts once.
+// coarser (~64k distances, at most 2048 staged doubles, six workgroups per
+// CU) so that each adds its bins to the chain's global counts once.
 static void build_measure_items(pq_batch* b) {
   b->items_e.clear();
   b->items_g.clear();
@@ -752,7 +752,7 @@ static void build_measure_items(pq_batch* b) {
     b->first_e.push_back((int)b->items_e.size());
     b->slice_e = std::max<int>(b->slice_e, (int)((te + 1) * ss));
     int64_t tg = std::max<int64_t>(1, 65536 / std::max<int64_t>(1, (int64_t)ch.N * ch.N));
-    tg = std::min<int64_t>(tg, std::max<int64_t>(1, 4096 / nd - 1));
+    tg = std::min<int64_t>(tg, std::max<int64_t>(1, 2048 / nd - 1));
     tg = std::min<int64_t>(tg, ch.M);
     for (int t = 0; t < ch.M; t += (int)tg) b->items_g.push_back(MeasItem{c, t, (int)std::min<int64_t>(ch.M, t + tg)});
     b->slice_g = std::max<int>(b->slice_g, (int)((tg + 1) * nd));

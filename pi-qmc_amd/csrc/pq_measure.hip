@@ -446,8 +446,9 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n) {
     p[q] = 0;
 }
 
-// persistent grids: three histogram workgroups per CU, or two energy
-// workgroups (under 80 KB of LDS each), four rounds of them over 256 CUs
+// persistent grids: six histogram workgroups (~23 KB of LDS, 62 VGPRs
+// each) or two energy workgroups (under 80 KB of LDS each) per CU, four
+// rounds of them over 256 CUs
 static int meas_grid(int nitems, int per_cu) {
   const int g = 4 * 256 * per_cu;
   return nitems < g ? (nitems > 0 ? nitems : 1) : g;
@@ -471,7 +472,7 @@ hipError_t launch_energy_items(int D, const DevCtx& x, const MeasItem* items, in
 hipError_t launch_pair_cf_items(int D, const DevCtx& x, const MeasItem* items, int nitems, int max_slice_doubles,
                                 const PairCfArgs& g, uint64_t* counts, hipStream_t st) {
   const size_t lds = sizeof(double) * ((size_t)(g.nbin + 1) / 2 + max_slice_doubles);
-  const int grid = meas_grid(nitems, 3);
+  const int grid = meas_grid(nitems, 6);
   if (D == 2)
     hipLaunchKernelGGL(pair_cf_items_kernel<2>, dim3(grid), dim3(kMeasThreads), lds, st, x, items, nitems, g, counts);
   else
