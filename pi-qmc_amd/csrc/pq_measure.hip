@@ -5,7 +5,7 @@
 // Work is split into items of consecutive slices of one chain, sized so that
 // every item costs about the same whatever the chain's N and M: a chain of
 // 64 particles gives a few slices per item, a chain of 2 particles hundreds.
-// A persistent grid takes contiguous runs of the item list; an item stages its slices (and the
+// Each workgroup takes a contiguous run of the item list; an item stages its slices (and the
 // slice before the first, through the path permutation at slice 0) and the
 // chain's utau tables in LDS, so every bead and table entry is read from HBM
 // or L2 once per item.  Energy items write one partial sum each and a second
@@ -446,11 +446,14 @@ __global__ void fill_u64_kernel(uint64_t* p, int64_t n) {
     p[q] = 0;
 }
 
-// persistent grids: six histogram workgroups (~23 KB of LDS, 62 VGPRs
-// each) or two energy workgroups (under 80 KB of LDS each) per CU, four
-// rounds of them over 256 CUs
+// grids of 32 rounds over 256 CUs of six histogram workgroups (~23 KB of
+// LDS, 62 VGPRs each) or two energy workgroups (under 80 KB of LDS each)
+// per CU, each workgroup taking a contiguous run of items: short runs keep
+// the end of the launch even (4 rounds: 19.0 + 5.8 ms on the 4096-chain 3D
+// batch, 32 rounds: 17.2 + 4.9 ms), long enough that a workgroup stages a
+// chain's tables for several items
 static int meas_grid(int nitems, int per_cu) {
-  const int g = 4 * 256 * per_cu;
+  const int g = 32 * 256 * per_cu;
   return nitems < g ? (nitems > 0 ? nitems : 1) : g;
 }
 
