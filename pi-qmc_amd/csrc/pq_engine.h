@@ -741,24 +741,21 @@ struct Eng {
     const int pd = gd + (pbcmv ? D : 0);
     const int i = mv(0);
     const int sp = species_of(c, i);
-    for (int q = lane; q < nb; q += PQ_WAVE) {
+    // one Box-Muller pair per lane: item (bead q, pair p)
+    constexpr int NP = (D + 1) / 2;
+    for (int it = lane; it < nb * NP; it += PQ_WAVE) {
+      const int q = NP == 1 ? it : it >> 1, p = NP == 1 ? 0 : it & 1;
       const int lg = 31 - __clz(q + 1);
       const int l = L - 1 - lg, kk = q + 1 - (1 << lg);
       const int t = (1 << l) * (2 * kk + 1);
       const uint64_t kb = k + (L - l) + (uint64_t)pd * ((1u << lg) - 1) + (uint64_t)kk * pd;
-      double gg[4];
-#pragma unroll
-      for (int p = 0; p < (D + 1) / 2; ++p) {
-        double t1 = 1 - 0.9999999999 * U(kb + 2 * p);
-        double t2 = U(kb + 2 * p + 1);
-        double rr = sqrt(-2.0 * log(t1));
-        double sn, cs;
-        sincos(6.283185306 * t2, &sn, &cs);
-        gg[2 * p] = rr * cs;
-        gg[2 * p + 1] = rr * sn;
-      }
-#pragma unroll
-      for (int d = 0; d < D; ++d) mov[t * D + d] = gg[d];
+      double t1 = 1 - 0.9999999999 * U(kb + 2 * p);
+      double t2 = U(kb + 2 * p + 1);
+      double rr = sqrt(-2.0 * log(t1));
+      double sn, cs;
+      sincos(6.283185306 * t2, &sn, &cs);
+      mov[t * D + 2 * p] = rr * cs;
+      if (2 * p + 1 < D) mov[t * D + 2 * p + 1] = rr * sn;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (lane == 0) lnT_out[L] = 0.0;
