@@ -37,7 +37,7 @@ import piqmc_capi as capi  # noqa: E402
 METRIC = "bead-updates/sec on 1 MI355X, mixed-batched vs FP64 at fixed energy error"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-TUNE_AFTER = 20          # sweeps from the replicated start before the autotuner times anything
+TUNE_SWEEPS = 2         # sweeps per timed schedule in the autotuner
 STEADY = {"batch3d": 120, "batch2d": 220}   # sweeps from the replicated start before the warm-up steps
 PAIR_CF = ([0, 0, 1], 0.0, 8.0, 256)  # species 0 against species 0 and 1, r in [0, 8), 256 bins
 
@@ -140,15 +140,15 @@ def identical_chains(bm, br, em, er):
     return np.array(same)
 
 
-def tune(descs, precision, seed):
-    """The autotuner's pick on a batch of its own (TUNE_AFTER sweeps from the
+def tune(descs, precision, seed, steady):
+    """The autotuner's pick on a batch of its own (`steady` sweeps from the
     replicated start, then the timed schedules), so that the measured runs
     keep one schedule from their first sweep: the node-matrix width a chain
     is evaluated with changes the rounding, and a paired mixed/FP64
     comparison needs the same arithmetic apart from the precision."""
     b = make_batch(descs, precision, seed)
-    b.run_sweeps(TUNE_AFTER)
-    ms, _ = b.autotune(1)
+    b.run_sweeps(steady)
+    ms, _ = b.autotune(TUNE_SWEEPS)
     team, inv, _ = b.schedule()
     tab, fst = b.staging()
     heavy, heavy_team = b.split()
@@ -353,7 +353,7 @@ def main():
     if args.schedule:
         sched = tuple(int(v) for v in args.schedule.split(","))
     else:
-        sched, tuned = tune(descs, prec, args.seed)
+        sched, tuned = tune(descs, prec, args.seed, steady)
     b_main, run = run_gpu(descs, prec, args.steps, args.warmup, args.seed, steady, schedule=sched, pair_cf=PAIR_CF,
                           tuned=tuned)
     bu_total = float(run["bu"].sum())
@@ -405,7 +405,7 @@ def main():
     if not args.no_extras:
         if args.workload != "batch2d":
             d3, w3 = workload("batch2d", 1024)
-            s3, t3ms = tune(d3, prec, args.seed)
+            s3, t3ms = tune(d3, prec, args.seed, STEADY["batch2d"])
             b3, r3 = run_gpu(d3, prec, args.steps, args.warmup, args.seed, STEADY["batch2d"], schedule=s3,
                              pair_cf=PAIR_CF, tuned=t3ms)
             k3, roof3 = roofline(d3, r3, args.steps)
