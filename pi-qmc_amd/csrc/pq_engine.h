@@ -1588,10 +1588,13 @@ struct Eng {
     double* dm = fn_dm(fa);
     double* flag = fn_newdm(fa);
     double* dd = fn_dist(fa);
+    double* nd0 = fn_newdist(fa);
     // every record of the section in one pass: determinant, scale, flag and
     // the distances (the latter are overwritten when a record is missing),
     // as one index space of single loads, four in flight per thread before
-    // their stores
+    // their stores.  The threads storing the flags note a missing record,
+    // and slice 0's distances also go to newDist (newDist(0) = dist(0)).
+    int missing = 0;
     const int nrec = 3 * S;
     const int tot = nrec + (f.nodal ? S * 2 * n : 0);
     for (int q0 = tid; q0 < tot; q0 += 4 * NT) {
@@ -1618,16 +1621,16 @@ struct Eng {
         if (q < nrec) {
           const int j = q / 3, fld = q - 3 * j;
           (fld == 0 ? dm : fld == 1 ? wscale : flag)[j] = v[u];
+          if (fld == 2) missing |= v[u] != 1.0;
         } else if (q < tot) {
-          dd[q - nrec] = v[u];
+          const int qd = q - nrec;
+          dd[qd] = v[u];
+          if (qd < 2 * n) nd0[qd] = v[u];
         }
       }
     }
-    __syncthreads();
-    // every slice has a record: the lanes of each wave read the flags at once
-    int missing = 0;
-    for (int j = lane; j < S; j += PQ_WAVE) missing |= flag[j] != 1.0;
-    if (wave_or(missing)) return false;
+    // every slice has a record
+    if (__syncthreads_or(missing)) return false;
     // the rescale loop of initialize on the recorded determinants,
     // det(s) = det(s_r) (s/s_r)^n.  The first wave evaluates every slice at
     // the current scale at once; when none leaves [1e-50, 1e50] the loop
@@ -1648,7 +1651,7 @@ struct Eng {
         const bool e = lane < S && dt * dm0 <= 1e-200;
         const int errs = __builtin_popcountll(__builtin_amdgcn_ballot_w64(e));
         if (lane < S) { dm[lane] = dt; wscale[lane] = scl0; }
-        if (lane == 0) { werr[0] = errs; lasterr[fa] = 0; }
+        if (lane == 0) { werr[0] = errs; lasterr[fa] = 0; flag[0] = dm0; }
       } else if (lane == 0) {
         double scl = scl0, dm0 = 0;
         int errs = 0, passes = 0;
@@ -1673,6 +1676,7 @@ struct Eng {
         scale[fa] = scl;
         werr[0] = errs;
         lasterr[fa] = 0;
+        flag[0] = dm0;   // newDMValue(0) = dmValue(0)
       }
     }
     __syncthreads();
@@ -1719,7 +1723,8 @@ struct Eng {
       // the section's moves, and every slice is evaluated
       inv = upd(f) ? upd_inv(fa) : inv_shared;
       const bool recs = fcache && rec_on && !upd(f);
-      if (!(recs && fn_init_cached(fa, nerr))) {
+      const bool cached = recs && fn_init_cached(fa, nerr);
+      if (!cached) {
         double scl = scale[fa];
         int t = 0, passes = 0;
         int errs = 0;
@@ -1771,13 +1776,15 @@ struct Eng {
         __syncthreads();
       }
       inv = inv_shared;
-      // newDMValue(0)=dmValue(0); newDist(0)=dist(0)
-      const int n = f.ns;
-      double* nd = fn_newdist(fa);
-      const double* od = fn_dist(fa);
-      for (int k = tid; k < 2 * n; k += NT) nd[k] = od[k];
-      if (tid == 0) fn_newdm(fa)[0] = dm[0];
-      __syncthreads();
+      if (!cached) {
+        // newDMValue(0)=dmValue(0); newDist(0)=dist(0) (the cached path set them)
+        const int n = f.ns;
+        double* nd = fn_newdist(fa);
+        const double* od = fn_dist(fa);
+        for (int k = tid; k < 2 * n; k += NT) nd[k] = od[k];
+        if (tid == 0) fn_newdm(fa)[0] = dm[0];
+        __syncthreads();
+      }
     }
     return nerr;
   }
