@@ -42,6 +42,37 @@ def test_pair_cf_partner_list_bit_exact():
     assert np.array_equal(b.pair_histogram(2, 0, 0.1, 2.0, 64), b.pair_cf_histogram([2, 0], 0.1, 2.0, 64))
 
 
+def test_pair_cf_bin_edges_bit_exact():
+    """Pair distances exactly on bin edges, one ulp either side of them, at
+    rmin and at rmax: the device's bins equal the oracle's FP64 expression
+    (correctly rounded square root, no contraction), for a bin width that is
+    a power of two and for ones that are not."""
+    print("[test] pair cf bin edges", flush=True)
+    n = 24
+    d = capi.make_desc(3, 8, 1, 0.05, [(n, 1.0, -1.0)], coulomb=None)
+    b = capi.Batch([d])
+    oc = ol.Chain(d)
+    rng = np.random.default_rng(3)
+    for rmin, rmax, nbin in ((0.0, 8.0, 256), (0.1, 2.0, 64), (0.25, 3.0, 37)):
+        w = (rmax - rmin) / nbin
+        beads = np.zeros((d.nslice, n, 3))
+        for t in range(d.nslice):
+            ks = rng.integers(0, nbin + 2, size=n)
+            x = rmin + ks * w
+            x = np.where(rng.random(n) < 0.3, np.nextafter(x, np.inf), x)
+            x = np.where(rng.random(n) < 0.3, np.nextafter(x, -np.inf), x)
+            # particle 0 at the origin: its distances are the x's; the others
+            # meet each other at differences of edges
+            x[0] = 0.0
+            beads[t, :, 0] = x
+        b.set_paths(0, beads)
+        oc.set_beads(beads)
+        got = b.pair_cf_histogram([0, 0], rmin, rmax, nbin)
+        want = oc.pair_cf_histogram([0, 0], rmin, rmax, nbin)
+        assert np.array_equal(got[0], want), (rmin, rmax, nbin)
+        assert want.sum() > 0
+
+
 def _shapes():
     out = []
     for n, m in ((2, 16), (7, 200), (20, 1024), (33, 96), (64, 2048), (1, 40)):
